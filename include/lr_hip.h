@@ -1,0 +1,114 @@
+/* lr_hip.h - the C ABI of little_renderer's path-tracing hot path on AMD MI355X (gfx950).
+ *
+ * Plain pointers and sizes only: no HIP, torch or WebGPU types. Pointers named d_* are device
+ * (HBM) addresses; everything else is host memory. Every call returns 0 on success or a
+ * negative LRH_E* code, with a message in lrh_last_error() (per host thread).
+ *
+ * Layouts are the reference's GPU layouts (engine/gpu/trace_gpu.c:825-911):
+ *   box   8 floats  lo.x lo.y lo.z 0 | hi.x hi.y hi.z 0             (build_queue, :587-641)
+ *   node  8 words   lo.xyz (f32) a (u32) | hi.xyz (f32) b (u32)     (trace.wgsl:2)
+ *                   inner: a, b = absolute child indices; leaf: a = first aux slot,
+ *                   b = 0x80000000 | count
+ *   tri   12 words  q.xyz n0 | u.xyz n1 | v.xyz (material << 26 | n2) (put_tris, :898-911)
+ *   ray   8 floats  o.xyz tmin | d.xyz tmax
+ *   hit   4 words   t (f32), shape id (u32, 0xffffffff = miss), b1, b2 (f32)
+ */
+#ifndef LR_HIP_H
+#define LR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LRH_OK 0
+#define LRH_EINVAL (-1)     /* a bad argument */
+#define LRH_ENOMEM (-2)     /* device memory could not be had */
+#define LRH_EDEVICE (-3)    /* the device failed (a lost device: the caller falls back to the CPU) */
+#define LRH_EREFUSED (-4)   /* the request does not fit (as gpu_trace_sync's 0, trace_gpu.c:1571-1581) */
+
+#define LRH_LBVH 0
+#define LRH_PLOC 1
+
+typedef struct LrhContext LrhContext;
+
+typedef struct {
+    int depth;       /* the tree's depth in levels (check_tree's, trace_gpu.c:768-805) */
+    int rounds;      /* PLOC rounds (0 for an LBVH) */
+    float ms;        /* device time of the build, hipEvent-timed on the context's stream */
+} LrhBuildStats;
+
+/* The message of the last failed call on this host thread. */
+const char *lrh_last_error(void);
+
+/* A context on one device with its own stream (in place of gpu_trace_create,
+ * engine/gpu/trace_gpu.c:437 and gpu_create, gpu.h). stream: a hipStream_t to use, or 0 for
+ * a new one of default priority. */
+int lrh_create(LrhContext **out, int device, void *stream);
+void lrh_destroy(LrhContext *c);
+/* waits for everything queued on the context's stream (gpu_wait, engine/gpu/gpu.c:420) */
+int lrh_sync(LrhContext *c);
+/* the context's stream, as a hipStream_t */
+void *lrh_stream(LrhContext *c);
+
+/* triangle_bounds (engine/scene/scene.c:65-74) of n packed triangles on the device: the box
+ * of q, q + u, q + v padded by 1e-3 (the boxes upload_instances makes, trace_gpu.c:1497-1502) */
+int lrh_tri_boxes(LrhContext *c, const float *d_tris, int n, float *d_boxes);
+
+/* The union of n boxes (6 floats: lo.xyz, hi.xyz) into d_bounds (device). */
+int lrh_bounds(LrhContext *c, const float *d_boxes, int n, float *d_bounds);
+
+/* One BVH over n boxes, into d_nodes[node0 .. node0 + 2n - 2] and d_aux[ref0 .. ref0 + n - 1]
+ * (a leaf's shape id = its item + ref_add): build_queue + gpu_trace_build
+ * (engine/gpu/trace_gpu.c:587-764) with builder LRH_LBVH (trace_build.wgsl) or LRH_PLOC
+ * (trace_ploc.wgsl; n <= 2 builds an LBVH, as the reference does). h_bounds: the bounds the
+ * Morton codes are taken in (6 floats, host), or NULL for the boxes' union computed on the
+ * device. d_sorted (optional, 2n words): the sorted Morton codes, then the sorted items.
+ * stats (optional): makes the call wait for the build and fills it. */
+int lrh_build(LrhContext *c, const float *d_boxes, int n, const float *h_bounds, int builder,
+              uint32_t node0, uint32_t ref0, uint32_t ref_add, uint32_t *d_nodes, uint32_t *d_aux,
+              uint32_t *d_sorted, LrhBuildStats *stats);
+
+/* Closest hits (any = 0) or first hits (any = 1, shadow rays) of n rays against one binary
+ * tree of triangles in the reference GPU layout: the walk of trace.wgsl traverse
+ * (:270-392) for a scene of triangles. The shape id of a hit is the aux entry of its leaf.
+ * Unlike the reference, the walk never drops a push: max_depth (the tree's depth, as
+ * lrh_build reports it) sizes its stack, which spills from LDS to device memory. */
+int lrh_trace(LrhContext *c, const uint32_t *d_nodes, uint32_t root, const uint32_t *d_aux,
+              const float *d_tris, const float *d_rays, int n, int any, float *d_hits, int max_depth);
+
+/* lrh_trace over a queue whose length is on the device (*d_count, at most max_n): the
+ * extension and shadow kernels of the wavefront tracer (wf_ext, wf_shadow,
+ * trace_wave.wgsl:72-84, 335-355), which the reference sizes by indirect dispatch
+ * (wf_args, :320-333). */
+int lrh_trace_queue(LrhContext *c, const uint32_t *d_nodes, uint32_t root, const uint32_t *d_aux, const float *d_tris,
+                    const float *d_rays, int max_n, const uint32_t *d_count, int any, float *d_hits, int max_depth);
+
+/* lrh_trace (d_count NULL) or lrh_trace_queue, also adding to d_counts[0] the node records
+ * the walk fetched (each popped node, and an inner node's two children: 32 bytes each) and to
+ * d_counts[1] the triangles it tested (48 bytes + a 4-byte aux entry each): the algorithmic
+ * traffic the roofline is computed from. Slower (atomics): for measurement, not timing. */
+int lrh_trace_count(LrhContext *c, const uint32_t *d_nodes, uint32_t root, const uint32_t *d_aux, const float *d_tris,
+                    const float *d_rays, int max_n, const uint32_t *d_count, int any, float *d_hits, int max_depth,
+                    uint64_t *d_counts);
+
+/* Pinhole primaries through each pixel's centre of a w x h image (camera_ray,
+ * engine/scene/camera.h:38-42; cam as lrs_camera_make gives it, host), row-major,
+ * unit directions, t in (tmin, tmax); wf_gen's role (trace_wave.wgsl:51-70). */
+int lrh_rays_primary(LrhContext *c, const float *cam, int w, int h, float tmin, float tmax, float *d_rays);
+
+/* For each of n rays with a hit: one cosine-weighted bounce about the geometric normal
+ * facing the ray (sample_cosine_hemisphere_u, engine/base/rng.h:53-70, its two numbers
+ * hashed from (seed, ray index)) and one shadow ray toward the point light (host, 3 floats),
+ * both from the hit point with tmin = eps, the shadow ray ending eps before the light.
+ * Appended at the same slot of d_bounce and d_shadow; d_pix gets the source ray's index;
+ * *d_count the number appended (queue order varies between runs; each entry does not). */
+int lrh_rays_secondary(LrhContext *c, const float *d_rays, const float *d_hits, int n, const float *d_tris,
+                       const float *light, float eps, uint32_t seed, float *d_bounce, float *d_shadow, uint32_t *d_pix,
+                       uint32_t *d_count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,826 @@
+// lr_build.hip - the BVH builders on gfx950: LBVH (Karras 2012 numbering, built bottom-up in
+// one pass after Apetrei 2014) and PLOC (Meister & Bittner 2018, radius 16), over Morton
+// codes sorted by an LSD radix sort with decoupled look-back (one kernel per 8-bit digit).
+//
+// The rules are the reference's (engine/gpu/shaders/trace/trace_build.wgsl,
+// trace_ploc.wgsl); the kernels are not: the reference's 210 bitonic steps become 4 radix
+// passes, its 54 refit passes one bottom-up pass with per-node arrival counters, and each
+// PLOC round (4 dispatches there) one kernel whose cluster windows sit in LDS.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "lr_device.h"
+#include "lr_internal.h"
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = BLOCK * SORT_ITEMS;   // 4096 keys per tile
+constexpr int SORT_PASSES = 4;                  // 30-bit codes: bits 0-7, 8-15, 16-23, 24-29
+constexpr int HIST_COPIES = 8;                  // one global histogram per XCD (less contention)
+constexpr int PLOC_R = 16;
+constexpr int PLOC_TILE = 256;
+constexpr int PLOC_WIN = PLOC_TILE + 4 * PLOC_R;   // boxes of positions base-2R .. base+256+2R
+constexpr int PLOC_NN = PLOC_TILE + 2 * PLOC_R;    // nearest of positions base-R .. base+256+R
+constexpr int PLOC_MAX_ROUNDS = 128;
+constexpr int BOUNDS_BLOCKS = 512;
+
+constexpr uint32_t FLAG_AGG = 1u << 30;
+constexpr uint32_t FLAG_PREFIX = 2u << 30;
+constexpr uint32_t VALUE_MASK = (1u << 30) - 1u;
+
+// ---- block-wide scans (256 threads, wave64) ------------------------------------------------
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// exclusive scan of one value per thread over a block of NW waves; *total gets the sum
+template <int NW>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total, uint32_t *s_wsum)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) s_wsum[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+        uint32_t s = s_wsum[k];
+        off += k < w ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+// ---- triangle boxes (scene.c:65-74) --------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_tri_boxes(const float4 *__restrict__ tris, int n, float4 *__restrict__ boxes)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float4 q = tris[3 * (size_t)i], u = tris[3 * (size_t)i + 1], v = tris[3 * (size_t)i + 2];
+    const float qq[3] = {q.x, q.y, q.z}, uu[3] = {u.x, u.y, u.z}, vv[3] = {v.x, v.y, v.z};
+    float lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float a = qq[k] + uu[k], b = qq[k] + vv[k];
+        lo[k] = rfmin(rfmin(qq[k], a), b);
+        hi[k] = rfmax(rfmax(qq[k], a), b);
+    }
+    boxes[2 * (size_t)i] = make_float4(lo[0] - 1e-3f, lo[1] - 1e-3f, lo[2] - 1e-3f, 0.0f);
+    boxes[2 * (size_t)i + 1] = make_float4(hi[0] + 1e-3f, hi[1] + 1e-3f, hi[2] + 1e-3f, 0.0f);
+}
+
+// ---- the bounds of the boxes, and the Morton parameters ---------------------------------------
+__device__ __forceinline__ void min6(float *a, const float *b)
+{
+#pragma unroll
+    for (int k = 0; k < 3; k++) { a[k] = rfmin(a[k], b[k]); a[3 + k] = rfmax(a[3 + k], b[3 + k]); }
+}
+
+__device__ void block_min6(float *v, float (*s)[6])
+{
+    // within the wave, then across the 4 waves
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        float t[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) t[k] = __shfl_xor(v[k], o, 64);
+        min6(v, t);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < 6; k++) s[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 1; q < BLOCK / 64; q++) min6(v, s[q]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bounds(const float4 *__restrict__ boxes, int n, float *partial,
+                                                    uint32_t *ticket, LrBuildParamsDev *out)
+{
+    __shared__ float s[BLOCK / 64][6];
+    __shared__ bool last;
+    float v[6] = {1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f};
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        const float4 a = boxes[2 * (size_t)i], b = boxes[2 * (size_t)i + 1];
+        const float t[6] = {a.x, a.y, a.z, b.x, b.y, b.z};
+        min6(v, t);
+    }
+    block_min6(v, s);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; k++) st_dev((uint32_t *)&partial[6 * blockIdx.x + k], __float_as_uint(v[k]));
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    float w[6] = {1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f};
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLOCK) {
+        float t[6];
+        for (int k = 0; k < 6; k++) t[k] = __uint_as_float(ld_dev((const uint32_t *)&partial[6 * b + k]));
+        min6(w, t);
+    }
+    __syncthreads();
+    block_min6(w, s);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 3; k++) {
+            const float e = w[3 + k] - w[k];
+            out->lo[k] = w[k];
+            out->hi[k] = w[3 + k];
+            out->ext[k] = e > 0.0f ? (float)(1.0 / (double)e) : 0.0f;   // trace_gpu.c:609-611
+        }
+        out->lo[3] = out->ext[3] = out->hi[3] = 0.0f;
+        *ticket = 0;
+    }
+}
+
+// bounds given by the caller: only the parameters
+__global__ void k_params(LrBuildParamsDev *out, float lx, float ly, float lz, float hx, float hy, float hz)
+{
+    const float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
+    for (int k = 0; k < 3; k++) {
+        const float e = hi[k] - lo[k];
+        out->lo[k] = lo[k];
+        out->hi[k] = hi[k];
+        out->ext[k] = e > 0.0f ? (float)(1.0 / (double)e) : 0.0f;
+    }
+    out->lo[3] = out->ext[3] = out->hi[3] = 0.0f;
+}
+
+// ---- Morton codes (trace_build.wgsl:16-33) and the sort's global histograms ----------------------
+__device__ __forceinline__ uint32_t spread(uint32_t v)
+{
+    uint32_t x = v & 1023u;
+    x = (x | (x << 16u)) & 0x030000ffu;
+    x = (x | (x << 8u)) & 0x0300f00fu;
+    x = (x | (x << 4u)) & 0x030c30c3u;
+    x = (x | (x << 2u)) & 0x09249249u;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t quant(float c, float lo, float ext)
+{
+    return (uint32_t)(wmin(wmax((c - lo) * ext, 0.0f), 0.999f) * 1024.0f);
+}
+
+// one tile of 4096 items per block: codes (and the items 0..n-1 as the sort's values), the
+// four digits' counts into the histograms; the last block to finish turns them into each
+// digit's start (exclusive scans) and clears them for the next build. Also clears the memory
+// the later kernels expect zeroed (look-back states, tile counters, arrival counters).
+__global__ __launch_bounds__(BLOCK) void k_morton(const float4 *__restrict__ boxes, int n, const LrBuildParamsDev *prm,
+                                                    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                    uint32_t *hist, uint32_t *ticket, uint32_t *bucket_base,
+                                                    uint32_t *zero, size_t zero_words)
+{
+    __shared__ uint32_t h[SORT_PASSES][256];
+    __shared__ uint32_t s_wsum[BLOCK / 64];
+    __shared__ bool last;
+    for (int i = threadIdx.x; i < SORT_PASSES * 256; i += BLOCK) (&h[0][0])[i] = 0;
+    for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < zero_words; i += (size_t)gridDim.x * BLOCK) zero[i] = 0;
+    __syncthreads();
+    const float lx = prm->lo[0], ly = prm->lo[1], lz = prm->lo[2];
+    const float ex = prm->ext[0], ey = prm->ext[1], ez = prm->ext[2];
+    const int base = blockIdx.x * SORT_TILE;
+#pragma unroll 4
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        const int i = base + r * BLOCK + threadIdx.x;
+        if (i >= n) break;
+        const float4 a = boxes[2 * (size_t)i], b = boxes[2 * (size_t)i + 1];
+        const float cx = (a.x + b.x) * 0.5f, cy = (a.y + b.y) * 0.5f, cz = (a.z + b.z) * 0.5f;
+        const uint32_t code = (spread(quant(cx, lx, ex)) << 2u) | (spread(quant(cy, ly, ey)) << 1u) | spread(quant(cz, lz, ez));
+        keys[i] = code;
+        vals[i] = (uint32_t)i;
+#pragma unroll
+        for (int p = 0; p < SORT_PASSES; p++) atomicAdd(&h[p][(code >> (8 * p)) & 255u], 1u);
+    }
+    __syncthreads();
+    uint32_t *mine = hist + (blockIdx.x % HIST_COPIES) * SORT_PASSES * 256;
+    for (int i = threadIdx.x; i < SORT_PASSES * 256; i += BLOCK) {
+        const uint32_t v = (&h[0][0])[i];
+        if (v) atomicAdd(&mine[i], v);
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int p = 0; p < SORT_PASSES; p++) {
+        uint32_t v = 0;
+        for (int c = 0; c < HIST_COPIES; c++) {
+            uint32_t *x = &hist[(c * SORT_PASSES + p) * 256 + threadIdx.x];
+            v += ld_dev(x);
+            st_dev(x, 0u);
+        }
+        uint32_t tot;
+        const uint32_t ex_ = block_excl_scan<BLOCK / 64>(v, &tot, s_wsum);
+        bucket_base[p * 256 + threadIdx.x] = ex_;
+    }
+    if (threadIdx.x == 0) *ticket = 0;
+}
+
+// ---- one radix-sort pass: 8 bits, stable, decoupled look-back (onesweep) -------------------------
+// Each wave ranks its 1024 keys (16 rounds of 64, wave-contiguous, so the rank order is the
+// input order) with a 64-lane match over the digit's 8 bits; per-wave digit counts give each
+// key its place in the tile; the tile's place in each digit's run comes from the tiles before
+// it (look-back over per-tile, per-digit states). The tile is reordered in LDS first, so that
+// the stores to each digit's run are contiguous.
+__global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                       uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, int n,
+                                                       int shift, const uint32_t *__restrict__ bucket_base,
+                                                       uint32_t *status, uint32_t *tile_ctr)
+{
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t wc[BLOCK / 64][256];
+    __shared__ uint32_t s_start[256];
+    __shared__ uint32_t s_glob[256];
+    __shared__ uint32_t s_wsum[BLOCK / 64];
+    __shared__ uint32_t s_k[SORT_TILE];
+    __shared__ uint32_t s_v[SORT_TILE];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    for (int i = tid; i < (BLOCK / 64) * 256; i += BLOCK) (&wc[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const int base = (int)tile * SORT_TILE;
+    uint32_t k[SORT_ITEMS], v[SORT_ITEMS], rank[SORT_ITEMS];
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        const int i = base + w * (64 * SORT_ITEMS) + r * 64 + lane;
+        const bool ok = i < n;
+        k[r] = ok ? kin[i] : 0u;
+        v[r] = ok ? vin[i] : 0u;
+    }
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        const int i = base + w * (64 * SORT_ITEMS) + r * 64 + lane;
+        const bool ok = i < n;
+        const uint32_t d = (k[r] >> shift) & 255u;
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        const uint32_t before = ok ? wc[w][d] : 0u;
+        rank[r] = before + (uint32_t)__popcll(peers & lt);
+        __builtin_amdgcn_wave_barrier();
+        if (ok && (peers & lt) == 0) wc[w][d] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // thread t: digit t
+    uint32_t c[BLOCK / 64], tot = 0;
+#pragma unroll
+    for (int q = 0; q < BLOCK / 64; q++) { c[q] = wc[q][tid]; wc[q][tid] = tot; tot += c[q]; }
+    if (tile == 0) st_dev(&status[tid], FLAG_PREFIX | tot);
+    else st_dev(&status[tile * 256 + tid], FLAG_AGG | tot);
+    uint32_t all;
+    const uint32_t start = block_excl_scan<BLOCK / 64>(tot, &all, s_wsum);
+    uint32_t excl = 0;
+    if (tile > 0) {
+        int t = (int)tile - 1;
+        for (;;) {
+            const uint32_t s = ld_dev(&status[t * 256 + tid]);
+            if ((s & ~VALUE_MASK) == 0) { __builtin_amdgcn_s_sleep(1); continue; }
+            excl += s & VALUE_MASK;
+            if ((s & ~VALUE_MASK) == FLAG_PREFIX) break;
+            t--;
+        }
+        st_dev(&status[tile * 256 + tid], FLAG_PREFIX | (excl + tot));
+    }
+    s_start[tid] = start;
+    s_glob[tid] = bucket_base[tid] + excl - start;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        const int i = base + w * (64 * SORT_ITEMS) + r * 64 + lane;
+        if (i < n) {
+            const uint32_t d = (k[r] >> shift) & 255u;
+            const uint32_t pos = s_start[d] + wc[w][d] + rank[r];
+            s_k[pos] = k[r];
+            s_v[pos] = v[r];
+        }
+    }
+    __syncthreads();
+    const int cnt = n - base < SORT_TILE ? n - base : SORT_TILE;
+    for (int s = tid; s < cnt; s += BLOCK) {
+        const uint32_t key = s_k[s];
+        const uint32_t g = s_glob[(key >> shift) & 255u] + (uint32_t)s;
+        kout[g] = key;
+        vout[g] = s_v[s];
+    }
+}
+
+// ---- LBVH, bottom-up in one pass -----------------------------------------------------------------
+// delta of adjacent keys k, k+1 (trace_build.wgsl:52-58)
+__device__ __forceinline__ int dadj(const uint32_t *codes, int k)
+{
+    const uint32_t a = codes[k], b = codes[k + 1];
+    return a == b ? 32 + __clz((uint32_t)k ^ (uint32_t)(k + 1)) : __clz(a ^ b);
+}
+
+// is the node over keys [l, r] (not the root) its parent's left child? (its parent splits at r)
+__device__ __forceinline__ bool is_left(const uint32_t *codes, int n, int l, int r)
+{
+    return l == 0 || (r != n - 1 && dadj(codes, r) > dadj(codes, l - 1));
+}
+
+__device__ __forceinline__ void put_node(LrNode *p, float lx, float ly, float lz, uint32_t a, float hx, float hy, float hz, uint32_t b)
+{
+    float4 *q = reinterpret_cast<float4 *>(p);
+    q[0] = make_float4(lx, ly, lz, __uint_as_float(a));
+    q[1] = make_float4(hx, hy, hz, __uint_as_float(b));
+}
+
+// One thread per leaf j: it writes its leaf (leaves, trace_build.wgsl:98-107), then climbs:
+// at each parent the first of the two children to arrive stops, the second joins their
+// boxes (up, :109-122: min / max of left, right), writes the parent with Karras's numbering
+// (internal i at node0 + i, leaf j at node0 + n - 1 + j) and goes on.
+__global__ __launch_bounds__(BLOCK) void k_lbvh(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ items,
+                                                  const float4 *__restrict__ boxes, int n, uint32_t node0, uint32_t ref0,
+                                                  uint32_t ref_add, LrNode *nodes, uint32_t *aux, uint32_t *other,
+                                                  uint16_t *height, uint32_t *depth_out)
+{
+    const int j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t item = items[j];
+    const float4 blo = boxes[2 * (size_t)item], bhi = boxes[2 * (size_t)item + 1];
+    float lx = blo.x, ly = blo.y, lz = blo.z, hx = bhi.x, hy = bhi.y, hz = bhi.z;
+    put_node(&nodes[n - 1 + j], lx, ly, lz, ref0 + (uint32_t)j, hx, hy, hz, 0x80000001u);
+    aux[j] = item + ref_add;
+    if (n == 1) { atomicMax(depth_out, 1u); return; }
+    height[n - 1 + j] = 1;
+    int l = j, r = j;
+    uint32_t h = 1;
+    bool left = is_left(codes, n, l, r);
+    for (;;) {
+        const int p = left ? r : l - 1;             // the parent's split
+        const int end = left ? l : r;
+        __threadfence();                            // this subtree's records before the arrival
+        const uint32_t o = atomicExch(&other[p], (uint32_t)end + 1u);
+        if (o == 0) return;                         // the sibling is not done: it goes on
+        __threadfence();
+        const int L = left ? end : (int)o - 1, R = left ? (int)o - 1 : end;
+        const int lc = L == p ? n - 1 + p : p, rc = p + 1 == R ? n + p : p + 1;
+        const int sib = left ? rc : lc;
+        const float4 *sp = reinterpret_cast<const float4 *>(&nodes[sib]);
+        const float4 s0 = sp[0], s1 = sp[1];
+        const uint32_t sh = height[sib];
+        float nlx, nly, nlz, nhx, nhy, nhz;
+        if (left) {
+            nlx = wmin(lx, s0.x); nly = wmin(ly, s0.y); nlz = wmin(lz, s0.z);
+            nhx = wmax(hx, s1.x); nhy = wmax(hy, s1.y); nhz = wmax(hz, s1.z);
+        } else {
+            nlx = wmin(s0.x, lx); nly = wmin(s0.y, ly); nlz = wmin(s0.z, lz);
+            nhx = wmax(s1.x, hx); nhy = wmax(s1.y, hy); nhz = wmax(s1.z, hz);
+        }
+        lx = nlx; ly = nly; lz = nlz; hx = nhx; hy = nhy; hz = nhz;
+        h = 1 + (h > sh ? h : sh);
+        l = L; r = R;
+        const bool root = l == 0 && r == n - 1;
+        if (!root) left = is_left(codes, n, l, r);
+        const int me = root ? 0 : (left ? r : l);
+        put_node(&nodes[me], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
+        height[me] = (uint16_t)h;
+        if (root) { atomicMax(depth_out, h); return; }
+    }
+}
+
+// ---- PLOC ------------------------------------------------------------------------------------
+struct PlocState {
+    uint32_t nc;          // clusters in the current list
+    uint32_t parity;      // which of the two lists is current
+    uint32_t pad[62];
+    uint32_t ctr[PLOC_MAX_ROUNDS];   // each round's tile counter
+};
+
+// area (trace_ploc.wgsl:18), no fused multiply-add: ((ex*ey) + (ey*ez)) + (ez*ex)
+__device__ __forceinline__ float half_area(float lx, float ly, float lz, float hx, float hy, float hz)
+{
+    const float ex = hx - lx, ey = hy - ly, ez = hz - lz;
+    const float a = ex * ey, b = ey * ez, c = ez * ex;
+    const float s = a + b;
+    return s + c;
+}
+
+// nearer (trace_ploc.wgsl:60-66)
+__device__ __forceinline__ bool nearer(float d, uint32_t i, uint32_t j, float bd, uint32_t bj)
+{
+    if (bj == LR_NONE || d < bd) return true;
+    if (d > bd) return false;
+    const uint32_t mn = min(i, j), bmn = min(i, bj);
+    return mn < bmn || (mn == bmn && max(i, j) < max(i, bj));
+}
+
+// final_node (trace_ploc.wgsl:33-38) of cluster x with box (lo, hi)
+__device__ __forceinline__ void put_final(LrNode *slot, uint32_t x, float4 lo, float4 hi, uint32_t n, uint32_t node0, uint32_t ref0)
+{
+    if (x < n - 1u) put_node(slot, lo.x, lo.y, lo.z, node0 + 1u + 2u * x, hi.x, hi.y, hi.z, node0 + 2u + 2u * x);
+    else put_node(slot, lo.x, lo.y, lo.z, ref0 + x - (n - 1u), hi.x, hi.y, hi.z, 0x80000001u);
+}
+
+// merge (trace_ploc.wgsl:39-46): clusters c (the lower position) and d into internal k
+__device__ __forceinline__ void ploc_merge(uint32_t k, uint32_t c, float4 clo, float4 chi, uint32_t d, float4 dlo, float4 dhi,
+                                           uint32_t n, uint32_t node0, uint32_t ref0, float4 *bx, LrNode *nodes, uint16_t *height)
+{
+    const float4 lo = make_float4(wmin(clo.x, dlo.x), wmin(clo.y, dlo.y), wmin(clo.z, dlo.z), 0.0f);
+    const float4 hi = make_float4(wmax(chi.x, dhi.x), wmax(chi.y, dhi.y), wmax(chi.z, dhi.z), 0.0f);
+    bx[2 * (size_t)k] = lo;
+    bx[2 * (size_t)k + 1] = hi;
+    put_final(&nodes[1 + 2 * (size_t)k], c, clo, chi, n, node0, ref0);
+    put_final(&nodes[2 + 2 * (size_t)k], d, dlo, dhi, n, node0, ref0);
+    const uint32_t hc = height[c], hd = height[d];
+    height[k] = (uint16_t)(1 + (hc > hd ? hc : hd));
+    if (k == 0) put_node(&nodes[0], lo.x, lo.y, lo.z, node0 + 1u, hi.x, hi.y, hi.z, node0 + 2u);
+}
+
+// init (trace_ploc.wgsl:68-82): the leaves in Morton order are the first clusters
+__global__ __launch_bounds__(BLOCK) void k_ploc_init(const uint32_t *__restrict__ items, const float4 *__restrict__ boxes, int n,
+                                                       uint32_t ref0, uint32_t ref_add, uint32_t *aux, float4 *bx, uint32_t *list0,
+                                                       uint16_t *height, PlocState *st, uint64_t *status, size_t status_words)
+{
+    const size_t g = blockIdx.x * (size_t)BLOCK + threadIdx.x;
+    for (size_t i = g; i < status_words; i += (size_t)gridDim.x * BLOCK) status[i] = 0;
+    if (g < sizeof(PlocState) / 4) reinterpret_cast<uint32_t *>(st)[g] = g == 0 ? (uint32_t)n : 0u;
+    if (g >= (size_t)n) return;
+    const int i = (int)g;
+    const uint32_t item = items[i];
+    const size_t x = (size_t)n - 1 + i;
+    bx[2 * x] = boxes[2 * (size_t)item];
+    bx[2 * x + 1] = boxes[2 * (size_t)item + 1];
+    aux[i] = item + ref_add;
+    height[x] = 1;
+    list0[i] = (uint32_t)x;
+}
+
+// the nearest of position i (< nc) among the window held in LDS: wlo/whi[l] is position base - 2R + l
+__device__ __forceinline__ uint32_t nearest_in(int i, int base, uint32_t nc, const float4 *wlo, const float4 *whi)
+{
+    const int li = i - base + 2 * PLOC_R;
+    const float4 lo = wlo[li], hi = whi[li];
+    const int j0 = i - PLOC_R > 0 ? i - PLOC_R : 0;
+    const int j1 = i + PLOC_R < (int)nc - 1 ? i + PLOC_R : (int)nc - 1;
+    float bd = 0.0f;
+    uint32_t bj = LR_NONE;
+    for (int j = j0; j <= j1; j++) {
+        if (j == i) continue;
+        const int lj = j - base + 2 * PLOC_R;
+        const float4 ol = wlo[lj], oh = whi[lj];
+        const float d = half_area(wmin(lo.x, ol.x), wmin(lo.y, ol.y), wmin(lo.z, ol.z),
+                                  wmax(hi.x, oh.x), wmax(hi.y, oh.y), wmax(hi.z, oh.z));
+        if (nearer(d, (uint32_t)i, (uint32_t)j, bd, bj)) { bd = d; bj = (uint32_t)j; }
+    }
+    return bj;
+}
+
+// One PLOC round (nn + mark + top + scatter of trace_ploc.wgsl:84-234 in one kernel): tiles of
+// 256 positions taken in order from a counter; each loads the boxes of its positions +- 2R
+// into LDS, finds the nearest of positions +- R, marks who stays and who merges, takes its
+// place in the next list from the tiles before it (look-back), writes the merged nodes and
+// its part of the next list. Blocks loop over tiles, so any grid size covers the round.
+__global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t small, uint32_t node0, uint32_t ref0,
+                                                        float4 *bx, uint32_t *list0, uint32_t *list1, LrNode *nodes,
+                                                        uint16_t *height, PlocState *st, uint64_t *status0, uint64_t *status1)
+{
+    __shared__ float4 wlo[PLOC_WIN], whi[PLOC_WIN];
+    __shared__ uint32_t wid[PLOC_WIN];
+    __shared__ uint32_t snn[PLOC_NN];
+    __shared__ uint32_t s_wsum[BLOCK / 64];
+    __shared__ uint32_t s_tile, s_ek, s_em;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(&st->ctr[round], 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        const uint32_t nc = ld_dev(&st->nc), p = ld_dev(&st->parity);
+        const uint32_t ntiles = (nc + PLOC_TILE - 1) / PLOC_TILE;
+        if (nc <= small || tile >= ntiles) return;
+        const uint32_t *cur = p ? list1 : list0;
+        uint32_t *nxt = p ? list0 : list1;
+        uint64_t *status = p ? status1 : status0;
+        uint64_t *status_next = p ? status0 : status1;
+        if (tid == 0) status_next[tile] = 0;       // (the next round's, cleared for it)
+        const int base = (int)tile * PLOC_TILE;
+        for (int l = tid; l < PLOC_WIN; l += BLOCK) {
+            const int g = base - 2 * PLOC_R + l;
+            if (g >= 0 && g < (int)nc) {
+                const uint32_t x = cur[g];
+                wid[l] = x;
+                wlo[l] = bx[2 * (size_t)x];
+                whi[l] = bx[2 * (size_t)x + 1];
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < PLOC_NN; q += BLOCK) {
+            const int i = base - PLOC_R + q;
+            snn[q] = (i >= 0 && i < (int)nc) ? nearest_in(i, base, nc, wlo, whi) : LR_NONE;
+        }
+        __syncthreads();
+        // keep_merge (trace_ploc.wgsl:130-145)
+        const int i = base + tid;
+        const bool valid = i < (int)nc;
+        const uint32_t j = valid ? snn[tid + PLOC_R] : LR_NONE;
+        bool mu = false;
+        if (valid && j < nc) mu = snn[(int)j - base + PLOC_R] == (uint32_t)i;
+        const uint32_t keep = valid && !(mu && (uint32_t)i > j), merge = valid && mu && (uint32_t)i < j;
+        uint32_t tk, tm;
+        const uint32_t xk = block_excl_scan<BLOCK / 64>(keep, &tk, s_wsum);
+        const uint32_t xm = block_excl_scan<BLOCK / 64>(merge, &tm, s_wsum);
+        if (tid == 0) {
+            const uint64_t mine = ((uint64_t)tk << 31) | tm;
+            uint64_t ek = 0, em = 0;
+            if (tile == 0) st_dev64(&status[0], (2ull << 62) | mine);
+            else {
+                st_dev64(&status[tile], (1ull << 62) | mine);
+                for (int t = (int)tile - 1;;) {
+                    const uint64_t s = ld_dev64(&status[t]);
+                    const uint64_t f = s >> 62;
+                    if (f == 0) { __builtin_amdgcn_s_sleep(1); continue; }
+                    ek += (s >> 31) & 0x7fffffffull;
+                    em += s & 0x7fffffffull;
+                    if (f == 2) break;
+                    t--;
+                }
+                st_dev64(&status[tile], (2ull << 62) | ((ek + tk) << 31) | (em + tm));
+            }
+            s_ek = (uint32_t)ek;
+            s_em = (uint32_t)em;
+            if (tile == ntiles - 1) {              // the round's last tile: the next count
+                st_dev(&st->nc, (uint32_t)ek + tk);
+                st_dev(&st->parity, 1u - p);
+            }
+        }
+        __syncthreads();
+        if (keep) {
+            const uint32_t dest = s_ek + xk;
+            const int li = tid + 2 * PLOC_R;
+            const uint32_t c = wid[li];
+            uint32_t out = c;
+            if (merge) {
+                const int lj = (int)j - base + 2 * PLOC_R;
+                const uint32_t k = ((uint32_t)n - 2u) - ((uint32_t)n - nc + s_em + xm);
+                ploc_merge(k, c, wlo[li], whi[li], wid[lj], wlo[lj], whi[lj], (uint32_t)n, node0, ref0, bx, nodes, height);
+                out = k;
+            }
+            nxt[dest] = out;
+        }
+        __syncthreads();
+    }
+}
+
+// small (trace_ploc.wgsl:236-301): the rounds left, in one workgroup
+constexpr int SMALL_BLOCK = 1024;
+__global__ __launch_bounds__(SMALL_BLOCK) void k_ploc_small(int n, uint32_t node0, uint32_t ref0, float4 *bx, uint32_t *list0,
+                                                              uint32_t *list1, uint32_t *nnbuf, LrNode *nodes, uint16_t *height,
+                                                              PlocState *st, uint32_t *rounds_out, uint32_t *depth_out)
+{
+    __shared__ uint32_t s_wsum[SMALL_BLOCK / 64];
+    const int tid = threadIdx.x;
+    uint32_t nc = st->nc, p = st->parity;
+    uint32_t rounds = 0;
+    while (nc > 1) {
+        const uint32_t *cur = p ? list1 : list0;
+        uint32_t *nxt = p ? list0 : list1;
+        for (int i = tid; i < (int)nc; i += SMALL_BLOCK) {
+            const uint32_t xi = cur[i];
+            const float4 lo = bx[2 * (size_t)xi], hi = bx[2 * (size_t)xi + 1];
+            const int j0 = i - PLOC_R > 0 ? i - PLOC_R : 0;
+            const int j1 = i + PLOC_R < (int)nc - 1 ? i + PLOC_R : (int)nc - 1;
+            float bd = 0.0f;
+            uint32_t bj = LR_NONE;
+            for (int j = j0; j <= j1; j++) {
+                if (j == i) continue;
+                const uint32_t xj = cur[j];
+                const float4 ol = bx[2 * (size_t)xj], oh = bx[2 * (size_t)xj + 1];
+                const float d = half_area(wmin(lo.x, ol.x), wmin(lo.y, ol.y), wmin(lo.z, ol.z),
+                                          wmax(hi.x, oh.x), wmax(hi.y, oh.y), wmax(hi.z, oh.z));
+                if (nearer(d, (uint32_t)i, (uint32_t)j, bd, bj)) { bd = d; bj = (uint32_t)j; }
+            }
+            nnbuf[i] = bj;
+        }
+        __syncthreads();
+        const int per = ((int)nc + SMALL_BLOCK - 1) / SMALL_BLOCK;
+        const int s0 = min(tid * per, (int)nc), s1 = min(s0 + per, (int)nc);
+        uint32_t ck = 0, cm = 0;
+        for (int i = s0; i < s1; i++) {
+            const uint32_t j = nnbuf[i];
+            const bool mu = j < nc && nnbuf[j] == (uint32_t)i;
+            ck += !(mu && (uint32_t)i > j);
+            cm += mu && (uint32_t)i < j;
+        }
+        uint32_t tk, tm;
+        uint32_t dk = block_excl_scan<SMALL_BLOCK / 64>(ck, &tk, s_wsum);
+        uint32_t dm = block_excl_scan<SMALL_BLOCK / 64>(cm, &tm, s_wsum);
+        for (int i = s0; i < s1; i++) {
+            const uint32_t j = nnbuf[i];
+            const bool mu = j < nc && nnbuf[j] == (uint32_t)i;
+            if (mu && (uint32_t)i > j) continue;
+            const uint32_t c = cur[i];
+            uint32_t out = c;
+            if (mu && (uint32_t)i < j) {
+                const uint32_t d = cur[j];
+                const uint32_t k = ((uint32_t)n - 2u) - ((uint32_t)n - nc + dm);
+                dm++;
+                ploc_merge(k, c, bx[2 * (size_t)c], bx[2 * (size_t)c + 1], d, bx[2 * (size_t)d], bx[2 * (size_t)d + 1],
+                           (uint32_t)n, node0, ref0, bx, nodes, height);
+                out = k;
+            }
+            nxt[dk++] = out;
+        }
+        __syncthreads();
+        // every round merges at least the pair nearest of all (it is mutual); a round that
+        // merges nothing (NaN boxes) ends the loop instead of spinning
+        if (tk >= nc) break;
+        nc = tk;
+        p = 1u - p;
+        rounds++;
+    }
+    if (tid == 0) {
+        *rounds_out = rounds;
+        *depth_out = height[0];
+    }
+}
+
+// ---- the scratch layout of one build --------------------------------------------------------------
+struct Layout {
+    size_t params, bounds_tmp, hist, ticket, bucket, keys0, vals0, keys1, vals1, zero, zero_words, status,
+        ctr, depth, other, height, bx, list0, list1, nnbuf, pst, pstatus, pstatus_words, rounds, total;
+};
+
+Layout layout(int n)
+{
+    Layout L;
+    const size_t N = n > 0 ? (size_t)n : 1;
+    const size_t tiles = (N + SORT_TILE - 1) / SORT_TILE;
+    const size_t ptiles = (N + PLOC_TILE - 1) / PLOC_TILE;
+    size_t o = 0;
+    auto take = [&](size_t bytes) { size_t at = o; o += lr_align(bytes); return at; };
+    L.params = take(sizeof(LrBuildParamsDev));
+    L.bounds_tmp = take(BOUNDS_BLOCKS * 6 * 4 + 64);
+    L.hist = take(HIST_COPIES * SORT_PASSES * 256 * 4);
+    L.ticket = take(64);
+    L.bucket = take(SORT_PASSES * 256 * 4);
+    L.keys0 = take(N * 4);
+    L.vals0 = take(N * 4);
+    L.keys1 = take(N * 4);
+    L.vals1 = take(N * 4);
+    // one region the Morton kernel clears: tile counters, depth, look-back states, arrivals
+    L.zero = o;
+    L.ctr = take(64);
+    L.depth = take(64);
+    L.status = take(SORT_PASSES * tiles * 256 * 4);
+    L.other = take(N * 4);
+    L.zero_words = (o - L.zero) / 4;
+    L.height = take(4 * N);
+    L.bx = take((2 * N) * 32);
+    L.list0 = take(N * 4);
+    L.list1 = take(N * 4);
+    L.nnbuf = take(N * 4);
+    L.pst = take(sizeof(PlocState));
+    L.pstatus_words = 2 * (ptiles + 1);
+    L.pstatus = take(L.pstatus_words * 8);
+    L.rounds = take(64);
+    L.total = o;
+    return L;
+}
+
+// ploc_rounds (trace_gpu.c:563): the rounds from n down to `small`, two more to be safe
+int ploc_rounds(int n, int small)
+{
+    int r = 0;
+    for (int64_t m = n; m > small; m = m * 5 / 6) r++;
+    return r ? r + 2 : 0;
+}
+
+}  // namespace
+
+size_t lr_bounds_tmp_bytes() { return BOUNDS_BLOCKS * 6 * 4 + 64; }
+
+int lr_launch_tri_boxes(LrhContext *c, const float *tris, int n, float *boxes)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_tri_boxes, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, c->stream,
+                       reinterpret_cast<const float4 *>(tris), n, reinterpret_cast<float4 *>(boxes));
+    LR_HIP(hipGetLastError());
+    return 0;
+}
+
+// tmp: BOUNDS_BLOCKS * 6 floats then a zeroed ticket word (at +BOUNDS_BLOCKS*24)
+int lr_launch_bounds(LrhContext *c, const float *boxes, int n, LrBuildParamsDev *out, void *tmp)
+{
+    int blocks = (n + BLOCK - 1) / BLOCK;
+    if (blocks > BOUNDS_BLOCKS) blocks = BOUNDS_BLOCKS;
+    if (blocks < 1) blocks = 1;
+    float *partial = (float *)tmp;
+    uint32_t *ticket = (uint32_t *)((char *)tmp + BOUNDS_BLOCKS * 24);
+    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(BLOCK), 0, c->stream, reinterpret_cast<const float4 *>(boxes), n,
+                       partial, ticket, out);
+    LR_HIP(hipGetLastError());
+    return 0;
+}
+
+size_t lr_build_scratch_bytes(int n) { return layout(n).total; }
+
+int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bounds, int builder, uint32_t node0,
+                 uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux, uint32_t *sorted_codes_out,
+                 uint32_t *sorted_items_out, LrhBuildStats *stats)
+{
+    const Layout L = layout(n);
+    int rc = lr_scratch_reserve(c, LR_SCRATCH_BUILD, L.total);
+    if (rc) return rc;
+    char *S = (char *)c->scratch[LR_SCRATCH_BUILD].base;
+    auto at = [&](size_t off) { return (void *)(S + off); };
+    LrBuildParamsDev *prm = (LrBuildParamsDev *)at(L.params);
+    uint32_t *k0 = (uint32_t *)at(L.keys0), *v0 = (uint32_t *)at(L.vals0);
+    uint32_t *k1 = (uint32_t *)at(L.keys1), *v1 = (uint32_t *)at(L.vals1);
+    LrNode *nd = reinterpret_cast<LrNode *>(nodes) + node0;
+    uint32_t *ax = aux + ref0;
+    hipStream_t s = c->stream;
+    if (stats) LR_HIP(hipEventRecord(c->ev[0], s));
+    if (host_bounds)
+        hipLaunchKernelGGL(k_params, dim3(1), dim3(1), 0, s, prm, host_bounds[0], host_bounds[1], host_bounds[2],
+                           host_bounds[3], host_bounds[4], host_bounds[5]);
+    else if ((rc = lr_launch_bounds(c, boxes, n, prm, at(L.bounds_tmp)))) return rc;
+    const int tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    hipLaunchKernelGGL(k_morton, dim3(tiles), dim3(BLOCK), 0, s, reinterpret_cast<const float4 *>(boxes), n, prm, k0, v0,
+                       (uint32_t *)at(L.hist), (uint32_t *)at(L.ticket), (uint32_t *)at(L.bucket), (uint32_t *)at(L.zero),
+                       L.zero_words);
+    LR_HIP(hipGetLastError());
+    uint32_t *ctr = (uint32_t *)at(L.ctr);
+    for (int pass = 0; pass < SORT_PASSES; pass++) {
+        const uint32_t *ki = pass & 1 ? k1 : k0, *vi = pass & 1 ? v1 : v0;
+        uint32_t *ko = pass & 1 ? k0 : k1, *vo = pass & 1 ? v0 : v1;
+        hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(BLOCK), 0, s, ki, vi, ko, vo, n, 8 * pass,
+                           (const uint32_t *)at(L.bucket) + 256 * pass,
+                           (uint32_t *)at(L.status) + (size_t)pass * tiles * 256, ctr + pass);
+        LR_HIP(hipGetLastError());
+    }
+    // sorted: k0 (codes), v0 (items)
+    if (sorted_codes_out) LR_HIP(hipMemcpyAsync(sorted_codes_out, k0, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    if (sorted_items_out) LR_HIP(hipMemcpyAsync(sorted_items_out, v0, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    uint32_t *depth = (uint32_t *)at(L.depth);
+    uint16_t *height = (uint16_t *)at(L.height);
+    int rounds = 0;
+    const bool ploc = builder == LRH_PLOC && n > 2;
+    if (!ploc) {
+        hipLaunchKernelGGL(k_lbvh, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, k0, v0,
+                           reinterpret_cast<const float4 *>(boxes), n, node0, ref0, ref_add, nd, ax, (uint32_t *)at(L.other),
+                           height, depth);
+        LR_HIP(hipGetLastError());
+    } else {
+        float4 *bx = (float4 *)at(L.bx);
+        uint32_t *l0 = (uint32_t *)at(L.list0), *l1 = (uint32_t *)at(L.list1);
+        PlocState *pst = (PlocState *)at(L.pst);
+        uint64_t *ps = (uint64_t *)at(L.pstatus);
+        const size_t ptiles = L.pstatus_words / 2;
+        hipLaunchKernelGGL(k_ploc_init, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, v0,
+                           reinterpret_cast<const float4 *>(boxes), n, ref0, ref_add, ax, bx, l0, height, pst, ps,
+                           L.pstatus_words);
+        LR_HIP(hipGetLastError());
+        const int small = c->ploc_small > 0 ? c->ploc_small : 1024;
+        int nr = ploc_rounds(n, small);
+        if (nr > PLOC_MAX_ROUNDS) nr = PLOC_MAX_ROUNDS;
+        double est = n;
+        for (int r = 0; r < nr; r++) {
+            int grid = (int)((est + PLOC_TILE - 1) / PLOC_TILE);
+            if (grid < 16) grid = 16;
+            if (grid > 4096) grid = 4096;
+            hipLaunchKernelGGL(k_ploc_round, dim3(grid), dim3(BLOCK), 0, s, r, n, (uint32_t)small, node0, ref0, bx, l0, l1,
+                               nd, height, pst, ps, ps + ptiles);
+            LR_HIP(hipGetLastError());
+            est *= 0.84;
+        }
+        hipLaunchKernelGGL(k_ploc_small, dim3(1), dim3(SMALL_BLOCK), 0, s, n, node0, ref0, bx, l0, l1,
+                           (uint32_t *)at(L.nnbuf), nd, height, pst, (uint32_t *)at(L.rounds), depth);
+        LR_HIP(hipGetLastError());
+        rounds = nr;
+    }
+    if (stats) {
+        LR_HIP(hipEventRecord(c->ev[1], s));
+        LR_HIP(hipEventSynchronize(c->ev[1]));
+        float ms = 0;
+        LR_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        uint32_t dep = 0, small_rounds = 0;
+        LR_HIP(hipMemcpy(&dep, depth, 4, hipMemcpyDeviceToHost));
+        if (ploc) LR_HIP(hipMemcpy(&small_rounds, at(L.rounds), 4, hipMemcpyDeviceToHost));
+        stats->depth = (int)dep;
+        stats->rounds = ploc ? rounds + (int)small_rounds : 0;
+        stats->ms = ms;
+        c->last_build_ms = ms;
+    }
+    return 0;
+}

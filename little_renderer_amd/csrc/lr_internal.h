@@ -1,0 +1,71 @@
+// lr_internal.h - the library's host-side context and the launchers each .hip file provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/lr_hip.h"
+
+// parameters of one build, on the device: the bounds' low corner, 1 / their extent (as
+// build_queue computes it, trace_gpu.c:603-611), and the high corner
+struct LrBuildParamsDev {
+    float lo[4];
+    float ext[4];
+    float hi[4];
+};
+
+// the scratch memory of one context, grown on demand and kept between calls
+struct LrScratch {
+    void *base = nullptr;
+    size_t bytes = 0;
+};
+
+enum { LR_SCRATCH_BUILD = 0, LR_SCRATCH_TRACE = 1, LR_SCRATCH_COUNT = 2 };
+
+struct LrhContext {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    LrScratch scratch[LR_SCRATCH_COUNT];
+    uint32_t *flags = nullptr;   // device words: [0] a walk's stack overflowed (never expected)
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int ploc_small = 1024;
+    float last_build_ms = 0.0f;
+};
+
+// error reporting: a thread-local message, negative codes (lr_hip.h)
+int lr_fail(int code, const char *fmt, ...);
+#define LR_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return lr_fail(LRH_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+// scratch `which` of at least `bytes` (256-byte aligned pieces are carved by the callers);
+// growing it waits for the stream first, so no kernel still uses the old memory
+int lr_scratch_reserve(LrhContext *c, int which, size_t bytes);
+
+static inline size_t lr_align(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// ---- lr_build.hip ----
+int lr_launch_tri_boxes(LrhContext *c, const float *tris, int n, float *boxes);
+int lr_launch_bounds(LrhContext *c, const float *boxes, int n, LrBuildParamsDev *out, void *tmp);
+size_t lr_bounds_tmp_bytes();
+// the build's scratch for n items (both builders)
+size_t lr_build_scratch_bytes(int n);
+int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bounds, int builder,
+                 uint32_t node0, uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux,
+                 uint32_t *sorted_codes_out, uint32_t *sorted_items_out, LrhBuildStats *stats);
+
+// ---- lr_trace.hip ----
+int lr_launch_trace(LrhContext *c, const uint32_t *nodes, uint32_t root, const uint32_t *aux, const float *tris,
+                    const float *rays, int nrays, const uint32_t *count, int any, float *hits, int max_depth,
+                    unsigned long long *counts = nullptr);
+
+// ---- lr_rays.hip ----
+int lr_launch_rays_primary(LrhContext *c, const float *cam, int w, int h, float tmin, float tmax, float *rays);
+int lr_launch_clear(LrhContext *c, uint32_t *p, int n);
+int lr_launch_rays_secondary(LrhContext *c, const float *rays, const float *hits, int n, const float *tris,
+                             const float *light, float eps, uint32_t seed, float *bounce, float *shadow, uint32_t *pix,
+                             uint32_t *count);

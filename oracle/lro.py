@@ -1,0 +1,242 @@
+"""lro.py - ctypes + numpy access to the oracle (liblro.so) and to the reference's own CPU
+engine built here (oracle/_ref/libref_{f,d}.so, when it was built).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by bench.py's
+cpu_baseline leg, and only as the checker or the timed CPU baseline; the product never uses it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liblro.so")
+REF_F = os.path.join(HERE, "_ref", "libref_f.so")
+REF_D = os.path.join(HERE, "_ref", "libref_d.so")
+
+P, I, U, U64, F, D = (ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float,
+                      ctypes.c_double)
+
+
+class LroCount(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_uint64), ("tris", ctypes.c_uint64)]
+
+
+_L = None
+_R = {}
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: make -C oracle")
+        L = ctypes.CDLL(LIB)
+        sig = {
+            "lro_rng_u32": (U, [P]),
+            "lro_rng_seed": (None, [P, U64, U64]),
+            "lro_rng_float": (F, [P]),
+            "lro_rng_double": (D, [P]),
+            "lro_rnd32": (F, [P]),
+            "lro_tri_boxes": (None, [P, I, P]),
+            "lro_union": (None, [P, I, P, P]),
+            "lro_inv_extent": (None, [P, P, P]),
+            "lro_morton": (None, [P, I, P, P, P]),
+            "lro_sort": (None, [P, I, P, P]),
+            "lro_lbvh": (I, [P, P, P, I, U, U, U, P, P]),
+            "lro_lbvh_bottom_up": (I, [P, P, P, I, U, U, U, P, P]),
+            "lro_ploc": (I, [P, P, P, I, U, U, U, P, P, P]),
+            "lro_check_tree": (I, [P, I, U, P, P]),
+            "lro_build_tris": (I, [P, I, I, U, U, U, P, P, P]),
+            "lro_sah_build": (P, [P, I, I, I, I]),
+            "lro_sah_free": (None, [P]),
+            "lro_sah_cost": (D, [P]),
+            "lro_sah_info": (None, [P, P, P, P, P]),
+            "lro_sah_put_nodes": (None, [P, U, U, P, P]),
+            "lro_trace_gpu_layout": (None, [P, U, P, P, P, I, I, P, P]),
+            "lro_trace_sah": (None, [P, P, P, I, I, P]),
+            "lro_trace_brute": (None, [P, I, P, I, I, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _L = L
+    return _L
+
+
+def ref(double=False):
+    """The reference's own CPU engine (float or double build), or None where it was not built."""
+    path = REF_D if double else REF_F
+    if path in _R:
+        return _R[path]
+    if not os.path.exists(path):
+        _R[path] = None
+        return None
+    L = ctypes.CDLL(path)
+    sig = {
+        "refh_pcg32": (None, [U64, U64, I, P]),
+        "refh_rng_real": (None, [U64, U64, I, P]),
+        "refh_sampler_2d": (None, [I, P, P, P, P, U, P]),
+        "refh_bvh_tris": (I, [P, I, I, I, P, P, P, P]),
+        "refh_trace_tris": (None, [P, I, I, P, I, I, P]),
+        "refh_tris_new": (P, [P, I, I]),
+        "refh_tris_free": (None, [P]),
+        "refh_tris_trace": (None, [P, P, I, I, P]),
+        "sampler_hash": (U, [U]),
+        "sampler_owen": (U, [U, U]),
+        "sampler_sobol1": (U, [U]),
+        "blue_noise_mask": (P, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _R[path] = L
+    return L
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+# ---- numpy-level helpers ------------------------------------------------------------------
+def pcg32(seed, stream, n):
+    L = lib()
+    st = (ctypes.c_uint64 * 2)()
+    L.lro_rng_seed(st, seed, stream)
+    return np.array([L.lro_rng_u32(st) for _ in range(n)], dtype=np.uint32)
+
+
+def tri_boxes(tris):
+    tris = np.ascontiguousarray(tris, dtype=np.float32)
+    n = tris.shape[0]
+    out = np.empty((n, 8), dtype=np.float32)
+    lib().lro_tri_boxes(ptr(tris), n, ptr(out))
+    return out
+
+
+def union(boxes):
+    lo = np.empty(3, np.float32)
+    hi = np.empty(3, np.float32)
+    lib().lro_union(ptr(boxes), boxes.shape[0], ptr(lo), ptr(hi))
+    return lo, hi
+
+
+def inv_extent(lo, hi):
+    ext = np.empty(3, np.float32)
+    lib().lro_inv_extent(ptr(np.ascontiguousarray(lo, np.float32)), ptr(np.ascontiguousarray(hi, np.float32)), ptr(ext))
+    return ext
+
+
+def morton(boxes, lo, ext):
+    n = boxes.shape[0]
+    out = np.empty(n, np.uint32)
+    lib().lro_morton(ptr(boxes), n, ptr(np.ascontiguousarray(lo, np.float32)), ptr(np.ascontiguousarray(ext, np.float32)),
+                     ptr(out))
+    return out
+
+
+def sort(codes):
+    codes = np.ascontiguousarray(codes, np.uint32)
+    n = codes.shape[0]
+    sc = np.empty(n, np.uint32)
+    si = np.empty(n, np.uint32)
+    lib().lro_sort(ptr(codes), n, ptr(sc), ptr(si))
+    return sc, si
+
+
+def lbvh(sc, si, boxes, node0=0, ref0=0, ref_add=0, bottom_up=False):
+    n = sc.shape[0]
+    nodes = np.zeros((max(2 * n - 1, 1), 8), np.uint32)
+    aux = np.zeros(max(n, 1), np.uint32)
+    fn = lib().lro_lbvh_bottom_up if bottom_up else lib().lro_lbvh
+    depth = fn(ptr(sc), ptr(si), ptr(boxes), n, node0, ref0, ref_add, ptr(nodes), ptr(aux))
+    return nodes, aux, depth
+
+
+def ploc(sc, si, boxes, node0=0, ref0=0, ref_add=0):
+    n = sc.shape[0]
+    nodes = np.zeros((max(2 * n - 1, 1), 8), np.uint32)
+    aux = np.zeros(max(n, 1), np.uint32)
+    depth = ctypes.c_int()
+    rounds = lib().lro_ploc(ptr(sc), ptr(si), ptr(boxes), n, node0, ref0, ref_add, ptr(nodes), ptr(aux),
+                            ctypes.byref(depth))
+    return nodes, aux, depth.value, rounds
+
+
+def check_tree(nodes, n, node0=0):
+    cost = ctypes.c_double()
+    depth = ctypes.c_int()
+    ok = lib().lro_check_tree(ptr(np.ascontiguousarray(nodes, np.uint32)), n, node0, ctypes.byref(cost),
+                              ctypes.byref(depth))
+    return bool(ok), cost.value, depth.value
+
+
+def build_tris(tris, ploc_=False):
+    """The whole build from triangles: (nodes, aux, sorted codes, sorted items, depth)."""
+    tris = np.ascontiguousarray(tris, np.float32)
+    n = tris.shape[0]
+    nodes = np.zeros((2 * n - 1, 8), np.uint32)
+    aux = np.zeros(n, np.uint32)
+    keys = np.zeros(2 * n, np.uint32)
+    depth = lib().lro_build_tris(ptr(tris), n, 1 if ploc_ else 0, 0, 0, 0, ptr(nodes), ptr(aux), ptr(keys))
+    return nodes, aux, keys[:n].copy(), keys[n:].copy(), depth
+
+
+def trace_gpu_layout(nodes, root, aux, tris, rays, any_hit=False, counts=False):
+    rays = np.ascontiguousarray(rays, np.float32)
+    n = rays.shape[0]
+    hits = np.zeros((n, 4), np.float32)
+    c = LroCount()
+    lib().lro_trace_gpu_layout(ptr(np.ascontiguousarray(nodes, np.uint32)), root, ptr(np.ascontiguousarray(aux, np.uint32)),
+                               ptr(np.ascontiguousarray(tris, np.float32)), ptr(rays), n, 1 if any_hit else 0, ptr(hits),
+                               ctypes.byref(c))
+    return (hits, (c.nodes, c.tris)) if counts else hits
+
+
+class Sah:
+    def __init__(self, boxes, max_leaf=4, bins=16, wide=True):
+        self.h = lib().lro_sah_build(ptr(np.ascontiguousarray(boxes, np.float32)), boxes.shape[0], max_leaf, bins,
+                                     1 if wide else 0)
+
+    def __del__(self):
+        try:
+            lib().lro_sah_free(self.h)
+        except Exception:
+            pass
+
+    def cost(self):
+        return lib().lro_sah_cost(self.h)
+
+    def info(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        lib().lro_sah_info(self.h, *[ctypes.byref(x) for x in v])
+        return dict(nnodes=v[0].value, nrefs=v[1].value, depth=v[2].value, nwide=v[3].value)
+
+    def put_nodes(self, node0=0, ref0=0):
+        inf = self.info()
+        nodes = np.zeros((inf["nnodes"], 8), np.uint32)
+        aux = np.zeros(inf["nrefs"], np.uint32)
+        lib().lro_sah_put_nodes(self.h, node0, ref0, ptr(nodes), ptr(aux))
+        return nodes, aux, inf["depth"]
+
+    def trace(self, tris, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        hits = np.zeros((rays.shape[0], 4), np.float32)
+        lib().lro_trace_sah(self.h, ptr(np.ascontiguousarray(tris, np.float32)), ptr(rays), rays.shape[0],
+                            1 if any_hit else 0, ptr(hits))
+        return hits
+
+
+def trace_brute(tris, rays, any_hit=False):
+    rays = np.ascontiguousarray(rays, np.float32)
+    tris = np.ascontiguousarray(tris, np.float32)
+    hits = np.zeros((rays.shape[0], 4), np.float32)
+    second = np.zeros(rays.shape[0], np.float32)
+    lib().lro_trace_brute(ptr(tris), tris.shape[0], ptr(rays), rays.shape[0], 1 if any_hit else 0, ptr(hits), ptr(second))
+    return hits, second
+
+
+def hit_ids(hits):
+    return np.ascontiguousarray(hits).view(np.uint32)[:, 1]

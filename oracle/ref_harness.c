@@ -1,0 +1,144 @@
+/* ref_harness.c - plain-array entry points into the reference's own CPU engine, compiled
+ * together with its unmodified sources (engine/base, formats, scene, rt) by the "ref" target
+ * of oracle/Makefile into oracle/_ref/. TEST INFRASTRUCTURE ONLY: it lets the tests check the
+ * oracle's restatement against the reference itself, and the benchmark time the reference's
+ * CPU path as its baseline. It calls the reference's functions; it replaces none of them. */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "base/rng.h"
+#include "base/sampler.h"
+#include "rt/bvh.h"
+#include "scene/scene.h"
+
+/* PCG32 outputs (rng.h:13-28) */
+void refh_pcg32(uint64_t seed, uint64_t stream, int n, uint32_t *out)
+{
+    Rng r;
+    rng_seed(&r, seed, stream);
+    for (int i = 0; i < n; i++) out[i] = rng_u32(&r);
+}
+
+/* rng_real (rng.h:31-37) as this build's real, widened to double */
+void refh_rng_real(uint64_t seed, uint64_t stream, int n, double *out)
+{
+    Rng r;
+    rng_seed(&r, seed, stream);
+    for (int i = 0; i < n; i++) out[i] = (double)rng_real(&r);
+}
+
+/* sampler_2d (sampler.c:143-160) for a list of (x, y, index, dim), scramble `seed` */
+void refh_sampler_2d(int n, const int *x, const int *y, const uint32_t *index, const int *dim, uint32_t seed,
+                     double *out)
+{
+    for (int i = 0; i < n; i++) {
+        Rng rng;
+        rng_seed(&rng, 1, 1);
+        Sampler s = sampler_make(&rng, x[i], y[i], index[i], seed, 1);
+        real a, b;
+        sampler_2d(&s, dim[i], &a, &b);
+        out[2 * i] = (double)a;
+        out[2 * i + 1] = (double)b;
+    }
+}
+
+static void tris_in(Scene *s, const float *tris, int n)
+{
+    memset(s, 0, sizeof(*s));
+    s->tris = calloc((size_t)(n > 0 ? n : 1), sizeof(Triangle));
+    for (int i = 0; i < n; i++) {
+        const float *t = tris + 12 * (size_t)i;
+        Triangle *q = &s->tris[i];
+        q->q = vec3(t[0], t[1], t[2]);
+        q->u = vec3(t[4], t[5], t[6]);
+        q->v = vec3(t[8], t[9], t[10]);
+        for (int k = 0; k < 3; k++) { q->n[k] = -1; q->t[k] = -1; }
+        q->material = 0;
+    }
+    s->ntris = n;
+    s->light = -1;
+}
+
+/* bvh_build (bvh.c:447, the scene's shapes = these triangles, their triangle_bounds) with
+ * max_leaf, bins, SAH; its nodes in the GPU layout (put_nodes, trace_gpu.c:883-895) and refs.
+ * Returns the node count; depth and cost (bvh_cost) out. */
+int refh_bvh_tris(const float *tris, int n, int max_leaf, int bins, uint32_t *nodes, int *refs, int *depth, double *cost)
+{
+    Scene s;
+    tris_in(&s, tris, n);
+    Bvh b;
+    bvh_build(&b, &s, max_leaf, bins, BVH_SPLIT_SAH);
+    for (int i = 0; i < b.nnodes; i++) {
+        const BvhNode *nd = &b.nodes[i];
+        float lo[3] = {(float)nd->box.lo.x, (float)nd->box.lo.y, (float)nd->box.lo.z};
+        float hi[3] = {(float)nd->box.hi.x, (float)nd->box.hi.y, (float)nd->box.hi.z};
+        uint32_t *o = nodes + 8 * (size_t)i;
+        memcpy(o, lo, 12);
+        memcpy(o + 4, hi, 12);
+        o[3] = nd->count > 0 ? (uint32_t)nd->start : (uint32_t)nd->left;
+        o[7] = nd->count > 0 ? (0x80000000u | (uint32_t)nd->count) : (uint32_t)nd->right;
+    }
+    for (int i = 0; i < b.nrefs; i++) refs[i] = b.refs[i];
+    *depth = b.depth;
+    *cost = (double)bvh_cost(&b);
+    int nn = b.nnodes;
+    bvh_free(&b);
+    free(s.tris);
+    return nn;
+}
+
+/* the reference CPU's scene of these triangles with its SAH BVH (bvh_build, max leaf 4, 16
+ * bins: accel_default_params, accel.c:28-31), 4-wide when wide (bvh_make_wide) */
+typedef struct { Scene s; Bvh b; } RefTris;
+
+void *refh_tris_new(const float *tris, int n, int wide)
+{
+    RefTris *r = calloc(1, sizeof(RefTris));
+    tris_in(&r->s, tris, n);
+    bvh_build(&r->b, &r->s, 4, 16, BVH_SPLIT_SAH);
+    if (wide) bvh_make_wide(&r->b);
+    return r;
+}
+
+void refh_tris_free(void *h)
+{
+    RefTris *r = h;
+    if (!r) return;
+    bvh_free(&r->b);
+    free(r->s.tris);
+    free(r);
+}
+
+/* bvh_intersect (bvh.c:608-663; wide: wide_intersect :543-604) of rays (o.xyz tmin d.xyz
+ * tmax), OpenMP over the rays: hits (t, id, b1, b2), id 0xffffffff on a miss */
+void refh_tris_trace(void *h, const float *rays, int nrays, int any, float *hits)
+{
+    RefTris *R = h;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < nrays; i++) {
+        const float *r = rays + 8 * (size_t)i;
+        Ray ray = {vec3(r[0], r[1], r[2]), vec3(r[4], r[5], r[6])};
+        Hit hh;
+        float *o = hits + 4 * (size_t)i;
+        if (bvh_intersect(&R->b, &R->s, &ray, r[3], r[7], &hh, any)) {
+            o[0] = (float)hh.t;
+            uint32_t id = (uint32_t)hh.prim;
+            memcpy(&o[1], &id, 4);
+            o[2] = (float)hh.uv.x;
+            o[3] = (float)hh.uv.y;
+        } else {
+            uint32_t none = 0xffffffffu;
+            o[0] = r[7];
+            memcpy(&o[1], &none, 4);
+            o[2] = o[3] = 0;
+        }
+    }
+}
+
+void refh_trace_tris(const float *tris, int n, int wide, const float *rays, int nrays, int any, float *hits)
+{
+    void *h = refh_tris_new(tris, n, wide);
+    refh_tris_trace(h, rays, nrays, any, hits);
+    refh_tris_free(h);
+}
