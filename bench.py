@@ -34,7 +34,9 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-rays", type=int, default=400_000, help="rays in the CPU baseline's sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline: repeat for about this long")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 3 times (calibrates the PMC byte counters) and exit")
     return ap.parse_args()
 
 
@@ -48,6 +50,14 @@ def main():
     import little_renderer_amd as lr
 
     assert torch.cuda.is_available(), "bench.py needs the GPU"
+    if a.copy_probe:
+        src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.copy_probe << 20, "launches": 3}), flush=True)
+        return
     ctx = lr.Context(0)
     stream = torch.cuda.ExternalStream(ctx.stream)
     n, W, H = a.tris, a.width, a.height
@@ -165,56 +175,56 @@ def main():
 
     # ---- the CPU baseline: the reference's own CPU path (SAH build, 4-wide walk) ----
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(tris_np, rays0, bounce, shadow, nsec, a.cpu_rays)
+        out["cpu_baseline"] = cpu_baseline(tris_np, rays0, bounce, shadow, nsec, a.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(tris_np, rays0, bounce, shadow, nsec, sample):
-    """The reference CPU path on this host (all cores, OpenMP), on a bounded sample of the same
-    rays: its binned-SAH tree over the same triangles, then the primaries, bounces and shadow
-    rays of the first `sample` rays in the step's proportions. Falls back to the oracle's port."""
-    import ctypes
-    import torch
+def cpu_baseline(tris_np, rays0, bounce, shadow, nsec, seconds):
+    """The reference CPU path on this host (all cores, OpenMP over the rays as its tracer does
+    over rows): its own binned-SAH BVH over the same triangles (bvh_build, 4-wide), then every
+    ray of one step (primaries, bounces, shadow rays: the same rays the GPU traced), repeated for
+    about `seconds`; the median repetition's rate, builds included (one SAH build per
+    repetition against the GPU's LBVH + PLOC). Falls back to the oracle's port of the same walk
+    where the reference's engine was not built."""
+    import torch  # noqa: F401  (the tensors below are torch's)
     from oracle import lro
 
     n = tris_np.shape[0]
-    npix = rays0.shape[0]
-    k0 = min(npix, max(1, int(sample * npix / (npix + 2 * nsec))))
-    k1 = min(nsec, max(0, (sample - k0) // 2))
-    r0 = rays0[:k0].cpu().numpy()
-    rb = bounce[:k1].cpu().numpy()
-    rs = shadow[:k1].cpu().numpy()
-    cores = os.cpu_count() or 1
+    r0 = np.ascontiguousarray(rays0.cpu().numpy())
+    rb = np.ascontiguousarray(bounce[:nsec].cpu().numpy())
+    rs = np.ascontiguousarray(shadow[:nsec].cpu().numpy())
+    nr = r0.shape[0] + rb.shape[0] + rs.shape[0]
     try:
         cores = len(os.sched_getaffinity(0))
     except Exception:
-        pass
+        cores = os.cpu_count() or 1
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", cores)), cores)
     R = lro.ref()
-    t0 = time.perf_counter()
-    if R is not None:
-        kind = "reference"
-        h = R.refh_tris_new(lro.ptr(tris_np), n, 1)
-        t1 = time.perf_counter()
-        out = np.zeros((max(k0, k1, 1), 4), np.float32)
-        for rr, anyh in ((r0, 0), (rb, 0), (rs, 1)):
-            if rr.shape[0]:
-                R.refh_tris_trace(h, lro.ptr(np.ascontiguousarray(rr)), rr.shape[0], anyh, lro.ptr(out))
-        t2 = time.perf_counter()
-        R.refh_tris_free(h)
-    else:
-        kind = "port"
-        sah = lro.Sah(lro.tri_boxes(tris_np), 4, 16, wide=True)
-        t1 = time.perf_counter()
-        for rr, anyh in ((r0, False), (rb, False), (rs, True)):
-            if rr.shape[0]:
+    kind = "reference" if R is not None else "port"
+    out = np.zeros((r0.shape[0], 4), np.float32)
+    reps, t_all = [], time.perf_counter()
+    while not reps or (time.perf_counter() - t_all < seconds and len(reps) < 50):
+        t0 = time.perf_counter()
+        if R is not None:
+            h = R.refh_tris_new(lro.ptr(tris_np), n, 1)
+            t1 = time.perf_counter()
+            for rr, anyh in ((r0, 0), (rb, 0), (rs, 1)):
+                R.refh_tris_trace(h, lro.ptr(rr), rr.shape[0], anyh, lro.ptr(out))
+            t2 = time.perf_counter()
+            R.refh_tris_free(h)
+        else:
+            sah = lro.Sah(lro.tri_boxes(tris_np), 4, 16, wide=True)
+            t1 = time.perf_counter()
+            for rr, anyh in ((r0, False), (rb, False), (rs, True)):
                 sah.trace(tris_np, rr, any_hit=anyh)
-        t2 = time.perf_counter()
-    nr = k0 + 2 * k1
-    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
-    return {"value": round(nr / (t2 - t1) / 1e6, 4), "unit": "Mrays/s", "cores": min(threads, cores), "kind": kind,
-            "sample": f"{nr} of the step's rays ({k0} primaries, {k1} bounces, {k1} shadow) through the CPU's SAH "
-                      f"tree of the same {n} triangles (built in {t1 - t0:.2f} s, not counted)",
-            "build_s": round(t1 - t0, 3), "trace_s": round(t2 - t1, 3)}
+            t2 = time.perf_counter()
+        reps.append((t2 - t0, t1 - t0, t2 - t1))
+    reps.sort()
+    tot, tb, tt = reps[len(reps) // 2]
+    return {"value": round(nr / tot / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": kind,
+            "sample": f"one whole step's {nr} rays through the CPU's own SAH tree of the same {n} triangles, "
+                      f"build included; median of {len(reps)} repetitions",
+            "build_s": round(tb, 4), "trace_s": round(tt, 4), "trace_only_mrays_s": round(nr / tt / 1e6, 3)}
 
 
 if __name__ == "__main__":

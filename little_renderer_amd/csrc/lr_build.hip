@@ -15,8 +15,10 @@
 namespace {
 
 constexpr int BLOCK = 256;
-constexpr int SORT_ITEMS = 16;
-constexpr int SORT_TILE = BLOCK * SORT_ITEMS;   // 4096 keys per tile
+constexpr int SORT_BLOCK = 1024;                // 16 waves per tile: enough in flight per CU
+constexpr int SORT_ITEMS = 4;
+constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;   // 4096 keys per tile
+constexpr int SORT_WAVES = SORT_BLOCK / 64;
 constexpr int SORT_PASSES = 4;                  // 30-bit codes: bits 0-7, 8-15, 16-23, 24-29
 constexpr int HIST_COPIES = 8;                  // one global histogram per XCD (less contention)
 constexpr int PLOC_R = 16;
@@ -179,23 +181,23 @@ __device__ __forceinline__ uint32_t quant(float c, float lo, float ext)
 // four digits' counts into the histograms; the last block to finish turns them into each
 // digit's start (exclusive scans) and clears them for the next build. Also clears the memory
 // the later kernels expect zeroed (look-back states, tile counters, arrival counters).
-__global__ __launch_bounds__(BLOCK) void k_morton(const float4 *__restrict__ boxes, int n, const LrBuildParamsDev *prm,
+__global__ __launch_bounds__(SORT_BLOCK) void k_morton(const float4 *__restrict__ boxes, int n, const LrBuildParamsDev *prm,
                                                     uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                     uint32_t *hist, uint32_t *ticket, uint32_t *bucket_base,
                                                     uint32_t *zero, size_t zero_words)
 {
     __shared__ uint32_t h[SORT_PASSES][256];
-    __shared__ uint32_t s_wsum[BLOCK / 64];
+    __shared__ uint32_t s_wsum[SORT_WAVES];
     __shared__ bool last;
-    for (int i = threadIdx.x; i < SORT_PASSES * 256; i += BLOCK) (&h[0][0])[i] = 0;
-    for (size_t i = blockIdx.x * (size_t)BLOCK + threadIdx.x; i < zero_words; i += (size_t)gridDim.x * BLOCK) zero[i] = 0;
+    for (int i = threadIdx.x; i < SORT_PASSES * 256; i += SORT_BLOCK) (&h[0][0])[i] = 0;
+    for (size_t i = blockIdx.x * (size_t)SORT_BLOCK + threadIdx.x; i < zero_words; i += (size_t)gridDim.x * SORT_BLOCK) zero[i] = 0;
     __syncthreads();
     const float lx = prm->lo[0], ly = prm->lo[1], lz = prm->lo[2];
     const float ex = prm->ext[0], ey = prm->ext[1], ez = prm->ext[2];
     const int base = blockIdx.x * SORT_TILE;
 #pragma unroll 4
     for (int r = 0; r < SORT_ITEMS; r++) {
-        const int i = base + r * BLOCK + threadIdx.x;
+        const int i = base + r * SORT_BLOCK + threadIdx.x;
         if (i >= n) break;
         const float4 a = boxes[2 * (size_t)i], b = boxes[2 * (size_t)i + 1];
         const float cx = (a.x + b.x) * 0.5f, cy = (a.y + b.y) * 0.5f, cz = (a.z + b.z) * 0.5f;
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(BLOCK) void k_morton(const float4 *__restrict__ box
     }
     __syncthreads();
     uint32_t *mine = hist + (blockIdx.x % HIST_COPIES) * SORT_PASSES * 256;
-    for (int i = threadIdx.x; i < SORT_PASSES * 256; i += BLOCK) {
+    for (int i = threadIdx.x; i < SORT_PASSES * 256; i += SORT_BLOCK) {
         const uint32_t v = (&h[0][0])[i];
         if (v) atomicAdd(&mine[i], v);
     }
@@ -219,14 +221,15 @@ __global__ __launch_bounds__(BLOCK) void k_morton(const float4 *__restrict__ box
     __threadfence();
     for (int p = 0; p < SORT_PASSES; p++) {
         uint32_t v = 0;
-        for (int c = 0; c < HIST_COPIES; c++) {
-            uint32_t *x = &hist[(c * SORT_PASSES + p) * 256 + threadIdx.x];
-            v += ld_dev(x);
-            st_dev(x, 0u);
-        }
+        if (threadIdx.x < 256)
+            for (int c = 0; c < HIST_COPIES; c++) {
+                uint32_t *x = &hist[(c * SORT_PASSES + p) * 256 + threadIdx.x];
+                v += ld_dev(x);
+                st_dev(x, 0u);
+            }
         uint32_t tot;
-        const uint32_t ex_ = block_excl_scan<BLOCK / 64>(v, &tot, s_wsum);
-        bucket_base[p * 256 + threadIdx.x] = ex_;
+        const uint32_t ex_ = block_excl_scan<SORT_WAVES>(v, &tot, s_wsum);
+        if (threadIdx.x < 256) bucket_base[p * 256 + threadIdx.x] = ex_;
     }
     if (threadIdx.x == 0) *ticket = 0;
 }
@@ -237,21 +240,21 @@ __global__ __launch_bounds__(BLOCK) void k_morton(const float4 *__restrict__ box
 // key its place in the tile; the tile's place in each digit's run comes from the tiles before
 // it (look-back over per-tile, per-digit states). The tile is reordered in LDS first, so that
 // the stores to each digit's run are contiguous.
-__global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_pass(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                        uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, int n,
                                                        int shift, const uint32_t *__restrict__ bucket_base,
                                                        uint32_t *status, uint32_t *tile_ctr)
 {
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t wc[BLOCK / 64][256];
+    __shared__ uint32_t wc[SORT_WAVES][256];
     __shared__ uint32_t s_start[256];
     __shared__ uint32_t s_glob[256];
-    __shared__ uint32_t s_wsum[BLOCK / 64];
+    __shared__ uint32_t s_wsum[SORT_WAVES];
     __shared__ uint32_t s_k[SORT_TILE];
     __shared__ uint32_t s_v[SORT_TILE];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
-    for (int i = tid; i < (BLOCK / 64) * 256; i += BLOCK) (&wc[0][0])[i] = 0;
+    for (int i = tid; i < SORT_WAVES * 256; i += SORT_BLOCK) (&wc[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
     const int base = (int)tile * SORT_TILE;
@@ -282,16 +285,18 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint32_t *__restrict_
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    // thread t: digit t
-    uint32_t c[BLOCK / 64], tot = 0;
-#pragma unroll
-    for (int q = 0; q < BLOCK / 64; q++) { c[q] = wc[q][tid]; wc[q][tid] = tot; tot += c[q]; }
-    if (tile == 0) st_dev(&status[tid], FLAG_PREFIX | tot);
-    else st_dev(&status[tile * 256 + tid], FLAG_AGG | tot);
+    // threads 0..255: digit tid
+    const bool dig = tid < 256;
+    uint32_t tot = 0;
+    if (dig) {
+        for (int q = 0; q < SORT_WAVES; q++) { const uint32_t c = wc[q][tid]; wc[q][tid] = tot; tot += c; }
+        if (tile == 0) st_dev(&status[tid], FLAG_PREFIX | tot);
+        else st_dev(&status[tile * 256 + tid], FLAG_AGG | tot);
+    }
     uint32_t all;
-    const uint32_t start = block_excl_scan<BLOCK / 64>(tot, &all, s_wsum);
+    const uint32_t start = block_excl_scan<SORT_WAVES>(tot, &all, s_wsum);
     uint32_t excl = 0;
-    if (tile > 0) {
+    if (dig && tile > 0) {
         int t = (int)tile - 1;
         for (;;) {
             const uint32_t s = ld_dev(&status[t * 256 + tid]);
@@ -302,8 +307,10 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint32_t *__restrict_
         }
         st_dev(&status[tile * 256 + tid], FLAG_PREFIX | (excl + tot));
     }
-    s_start[tid] = start;
-    s_glob[tid] = bucket_base[tid] + excl - start;
+    if (dig) {
+        s_start[tid] = start;
+        s_glob[tid] = bucket_base[tid] + excl - start;
+    }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < SORT_ITEMS; r++) {
@@ -317,7 +324,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint32_t *__restrict_
     }
     __syncthreads();
     const int cnt = n - base < SORT_TILE ? n - base : SORT_TILE;
-    for (int s = tid; s < cnt; s += BLOCK) {
+    for (int s = tid; s < cnt; s += SORT_BLOCK) {
         const uint32_t key = s_k[s];
         const uint32_t g = s_glob[(key >> shift) & 255u] + (uint32_t)s;
         kout[g] = key;
@@ -346,57 +353,86 @@ __device__ __forceinline__ void put_node(LrNode *p, float lx, float ly, float lz
     q[1] = make_float4(hx, hy, hz, __uint_as_float(b));
 }
 
+// a node record written through to the coherence point (sc1 stores), so that another XCD's
+// thread reading it with ld_dev64 sees it without a cache-wide write-back
+__device__ __forceinline__ void put_node_dev(LrNode *p, float lx, float ly, float lz, uint32_t a, float hx, float hy, float hz,
+                                             uint32_t b)
+{
+    uint64_t *q = reinterpret_cast<uint64_t *>(p);
+    st_dev64(q + 0, ((uint64_t)__float_as_uint(ly) << 32) | __float_as_uint(lx));
+    st_dev64(q + 1, ((uint64_t)a << 32) | __float_as_uint(lz));
+    st_dev64(q + 2, ((uint64_t)__float_as_uint(hy) << 32) | __float_as_uint(hx));
+    st_dev64(q + 3, ((uint64_t)b << 32) | __float_as_uint(hz));
+}
+
 // One thread per leaf j: it writes its leaf (leaves, trace_build.wgsl:98-107), then climbs:
 // at each parent the first of the two children to arrive stops, the second joins their
 // boxes (up, :109-122: min / max of left, right), writes the parent with Karras's numbering
 // (internal i at node0 + i, leaf j at node0 + n - 1 + j) and goes on.
+// The hand-over between the two children's threads (often on different XCDs, whose L2s are
+// not coherent) uses no cache-wide fence: a child's record and height are stored through to
+// the coherence point (sc1), the stores are waited for (vmcnt 0), and only then does the
+// thread mark its arrival; the second thread reads them back with sc1 loads.
 __global__ __launch_bounds__(BLOCK) void k_lbvh(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ items,
                                                   const float4 *__restrict__ boxes, int n, uint32_t node0, uint32_t ref0,
                                                   uint32_t ref_add, LrNode *nodes, uint32_t *aux, uint32_t *other,
-                                                  uint16_t *height, uint32_t *depth_out)
+                                                  uint32_t *height, uint32_t *depth_out)
 {
     const int j = blockIdx.x * BLOCK + threadIdx.x;
     if (j >= n) return;
     const uint32_t item = items[j];
     const float4 blo = boxes[2 * (size_t)item], bhi = boxes[2 * (size_t)item + 1];
     float lx = blo.x, ly = blo.y, lz = blo.z, hx = bhi.x, hy = bhi.y, hz = bhi.z;
-    put_node(&nodes[n - 1 + j], lx, ly, lz, ref0 + (uint32_t)j, hx, hy, hz, 0x80000001u);
     aux[j] = item + ref_add;
-    if (n == 1) { atomicMax(depth_out, 1u); return; }
-    height[n - 1 + j] = 1;
+    if (n == 1) {
+        put_node(&nodes[0], lx, ly, lz, ref0, hx, hy, hz, 0x80000001u);
+        atomicMax(depth_out, 1u);
+        return;
+    }
+    put_node_dev(&nodes[n - 1 + j], lx, ly, lz, ref0 + (uint32_t)j, hx, hy, hz, 0x80000001u);
+    st_dev(&height[n - 1 + j], 1u);
     int l = j, r = j;
     uint32_t h = 1;
     bool left = is_left(codes, n, l, r);
     for (;;) {
         const int p = left ? r : l - 1;             // the parent's split
         const int end = left ? l : r;
-        __threadfence();                            // this subtree's records before the arrival
-        const uint32_t o = atomicExch(&other[p], (uint32_t)end + 1u);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);              // this subtree's record is out before the arrival
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const uint32_t o = __hip_atomic_exchange(&other[p], (uint32_t)end + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (o == 0) return;                         // the sibling is not done: it goes on
-        __threadfence();
         const int L = left ? end : (int)o - 1, R = left ? (int)o - 1 : end;
         const int lc = L == p ? n - 1 + p : p, rc = p + 1 == R ? n + p : p + 1;
         const int sib = left ? rc : lc;
-        const float4 *sp = reinterpret_cast<const float4 *>(&nodes[sib]);
-        const float4 s0 = sp[0], s1 = sp[1];
-        const uint32_t sh = height[sib];
+        const uint64_t *sp = reinterpret_cast<const uint64_t *>(&nodes[sib]);
+        const uint64_t q0 = ld_dev64(sp), q1 = ld_dev64(sp + 1), q2 = ld_dev64(sp + 2), q3 = ld_dev64(sp + 3);
+        const float sx = __uint_as_float((uint32_t)q0), sy = __uint_as_float((uint32_t)(q0 >> 32));
+        const float sz = __uint_as_float((uint32_t)q1);
+        const float tx = __uint_as_float((uint32_t)q2), ty = __uint_as_float((uint32_t)(q2 >> 32));
+        const float tz = __uint_as_float((uint32_t)q3);
+        const uint32_t sh = ld_dev(&height[sib]);
         float nlx, nly, nlz, nhx, nhy, nhz;
         if (left) {
-            nlx = wmin(lx, s0.x); nly = wmin(ly, s0.y); nlz = wmin(lz, s0.z);
-            nhx = wmax(hx, s1.x); nhy = wmax(hy, s1.y); nhz = wmax(hz, s1.z);
+            nlx = wmin(lx, sx); nly = wmin(ly, sy); nlz = wmin(lz, sz);
+            nhx = wmax(hx, tx); nhy = wmax(hy, ty); nhz = wmax(hz, tz);
         } else {
-            nlx = wmin(s0.x, lx); nly = wmin(s0.y, ly); nlz = wmin(s0.z, lz);
-            nhx = wmax(s1.x, hx); nhy = wmax(s1.y, hy); nhz = wmax(s1.z, hz);
+            nlx = wmin(sx, lx); nly = wmin(sy, ly); nlz = wmin(sz, lz);
+            nhx = wmax(tx, hx); nhy = wmax(ty, hy); nhz = wmax(tz, hz);
         }
         lx = nlx; ly = nly; lz = nlz; hx = nhx; hy = nhy; hz = nhz;
         h = 1 + (h > sh ? h : sh);
         l = L; r = R;
         const bool root = l == 0 && r == n - 1;
-        if (!root) left = is_left(codes, n, l, r);
-        const int me = root ? 0 : (left ? r : l);
-        put_node(&nodes[me], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
-        height[me] = (uint16_t)h;
-        if (root) { atomicMax(depth_out, h); return; }
+        if (root) {
+            put_node(&nodes[0], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
+            atomicMax(depth_out, h);
+            return;
+        }
+        left = is_left(codes, n, l, r);
+        const int me = left ? r : l;
+        put_node_dev(&nodes[me], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
+        st_dev(&height[me], h);
     }
 }
 
@@ -435,7 +471,7 @@ __device__ __forceinline__ void put_final(LrNode *slot, uint32_t x, float4 lo, f
 
 // merge (trace_ploc.wgsl:39-46): clusters c (the lower position) and d into internal k
 __device__ __forceinline__ void ploc_merge(uint32_t k, uint32_t c, float4 clo, float4 chi, uint32_t d, float4 dlo, float4 dhi,
-                                           uint32_t n, uint32_t node0, uint32_t ref0, float4 *bx, LrNode *nodes, uint16_t *height)
+                                           uint32_t n, uint32_t node0, uint32_t ref0, float4 *bx, LrNode *nodes, uint32_t *height)
 {
     const float4 lo = make_float4(wmin(clo.x, dlo.x), wmin(clo.y, dlo.y), wmin(clo.z, dlo.z), 0.0f);
     const float4 hi = make_float4(wmax(chi.x, dhi.x), wmax(chi.y, dhi.y), wmax(chi.z, dhi.z), 0.0f);
@@ -444,14 +480,14 @@ __device__ __forceinline__ void ploc_merge(uint32_t k, uint32_t c, float4 clo, f
     put_final(&nodes[1 + 2 * (size_t)k], c, clo, chi, n, node0, ref0);
     put_final(&nodes[2 + 2 * (size_t)k], d, dlo, dhi, n, node0, ref0);
     const uint32_t hc = height[c], hd = height[d];
-    height[k] = (uint16_t)(1 + (hc > hd ? hc : hd));
+    height[k] = 1 + (hc > hd ? hc : hd);
     if (k == 0) put_node(&nodes[0], lo.x, lo.y, lo.z, node0 + 1u, hi.x, hi.y, hi.z, node0 + 2u);
 }
 
 // init (trace_ploc.wgsl:68-82): the leaves in Morton order are the first clusters
 __global__ __launch_bounds__(BLOCK) void k_ploc_init(const uint32_t *__restrict__ items, const float4 *__restrict__ boxes, int n,
                                                        uint32_t ref0, uint32_t ref_add, uint32_t *aux, float4 *bx, uint32_t *list0,
-                                                       uint16_t *height, PlocState *st, uint64_t *status, size_t status_words)
+                                                       uint32_t *height, PlocState *st, uint64_t *status, size_t status_words)
 {
     const size_t g = blockIdx.x * (size_t)BLOCK + threadIdx.x;
     for (size_t i = g; i < status_words; i += (size_t)gridDim.x * BLOCK) status[i] = 0;
@@ -494,7 +530,7 @@ __device__ __forceinline__ uint32_t nearest_in(int i, int base, uint32_t nc, con
 // its part of the next list. Blocks loop over tiles, so any grid size covers the round.
 __global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t small, uint32_t node0, uint32_t ref0,
                                                         float4 *bx, uint32_t *list0, uint32_t *list1, LrNode *nodes,
-                                                        uint16_t *height, PlocState *st, uint64_t *status0, uint64_t *status1)
+                                                        uint32_t *height, PlocState *st, uint64_t *status0, uint64_t *status1)
 {
     __shared__ float4 wlo[PLOC_WIN], whi[PLOC_WIN];
     __shared__ uint32_t wid[PLOC_WIN];
@@ -585,7 +621,7 @@ __global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t
 // small (trace_ploc.wgsl:236-301): the rounds left, in one workgroup
 constexpr int SMALL_BLOCK = 1024;
 __global__ __launch_bounds__(SMALL_BLOCK) void k_ploc_small(int n, uint32_t node0, uint32_t ref0, float4 *bx, uint32_t *list0,
-                                                              uint32_t *list1, uint32_t *nnbuf, LrNode *nodes, uint16_t *height,
+                                                              uint32_t *list1, uint32_t *nnbuf, LrNode *nodes, uint32_t *height,
                                                               PlocState *st, uint32_t *rounds_out, uint32_t *depth_out)
 {
     __shared__ uint32_t s_wsum[SMALL_BLOCK / 64];
@@ -685,7 +721,7 @@ Layout layout(int n)
     L.status = take(SORT_PASSES * tiles * 256 * 4);
     L.other = take(N * 4);
     L.zero_words = (o - L.zero) / 4;
-    L.height = take(4 * N);
+    L.height = take(8 * N);
     L.bx = take((2 * N) * 32);
     L.list0 = take(N * 4);
     L.list1 = take(N * 4);
@@ -756,7 +792,7 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
                            host_bounds[3], host_bounds[4], host_bounds[5]);
     else if ((rc = lr_launch_bounds(c, boxes, n, prm, at(L.bounds_tmp)))) return rc;
     const int tiles = (n + SORT_TILE - 1) / SORT_TILE;
-    hipLaunchKernelGGL(k_morton, dim3(tiles), dim3(BLOCK), 0, s, reinterpret_cast<const float4 *>(boxes), n, prm, k0, v0,
+    hipLaunchKernelGGL(k_morton, dim3(tiles), dim3(SORT_BLOCK), 0, s, reinterpret_cast<const float4 *>(boxes), n, prm, k0, v0,
                        (uint32_t *)at(L.hist), (uint32_t *)at(L.ticket), (uint32_t *)at(L.bucket), (uint32_t *)at(L.zero),
                        L.zero_words);
     LR_HIP(hipGetLastError());
@@ -764,7 +800,7 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
     for (int pass = 0; pass < SORT_PASSES; pass++) {
         const uint32_t *ki = pass & 1 ? k1 : k0, *vi = pass & 1 ? v1 : v0;
         uint32_t *ko = pass & 1 ? k0 : k1, *vo = pass & 1 ? v0 : v1;
-        hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(BLOCK), 0, s, ki, vi, ko, vo, n, 8 * pass,
+        hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(SORT_BLOCK), 0, s, ki, vi, ko, vo, n, 8 * pass,
                            (const uint32_t *)at(L.bucket) + 256 * pass,
                            (uint32_t *)at(L.status) + (size_t)pass * tiles * 256, ctr + pass);
         LR_HIP(hipGetLastError());
@@ -773,7 +809,7 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
     if (sorted_codes_out) LR_HIP(hipMemcpyAsync(sorted_codes_out, k0, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
     if (sorted_items_out) LR_HIP(hipMemcpyAsync(sorted_items_out, v0, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
     uint32_t *depth = (uint32_t *)at(L.depth);
-    uint16_t *height = (uint16_t *)at(L.height);
+    uint32_t *height = (uint32_t *)at(L.height);
     int rounds = 0;
     const bool ploc = builder == LRH_PLOC && n > 2;
     if (!ploc) {
