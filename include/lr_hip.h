@@ -108,6 +108,44 @@ int lrh_rays_secondary(LrhContext *c, const float *d_rays, const float *d_hits, 
                        const float *light, float eps, uint32_t seed, float *d_bounce, float *d_shadow, uint32_t *d_pix,
                        uint32_t *d_count);
 
+/* ---- the path tracer (engine/rt/tracer.c; the reference GPU's trace_wave.wgsl) ---- */
+#define LRH_MAT_DIFFUSE 0
+#define LRH_MAT_MIRROR 1
+#define LRH_MAT_GLASS 2
+#define LRH_MAT_LIGHT 3
+#define LRH_MAT_PBR 4
+
+/* a material (Material, engine/scene/scene.h:31-45, untextured) */
+typedef struct {
+    int type;
+    float albedo[3], emission[3];
+    float ior, metallic, roughness, transmission;
+} LrhMaterial;
+
+typedef struct LrhPtScene LrhPtScene;
+
+/* A scene resident on the device (the role of gpu_trace_sync, engine/gpu/trace_gpu.c:1571):
+ * prims (24 floats each, the 96-byte record of trace.wgsl:3-10: centre+radius | corner,
+ * 0 sphere / 1 quad | edge u, material | edge v, area | unit normal, plane_d | (u x v)/|u x v|^2),
+ * then triangles (lr_hip.h's layout, material in the top 6 bits); shape ids are prims first,
+ * then triangles. light: the quad prim sampled as the lamp (Scene.light), or -1. eps: the
+ * rays' minimum distance, margin: how far before the lamp shadow rays stop (RT_EPS,
+ * RT_SHADOW_MARGIN of tracer.c:15-23, in the scene's units). The BVH over all shapes is built
+ * on the device (builder LRH_LBVH or LRH_PLOC). Host arrays; the library keeps copies. */
+int lrh_pt_scene(LrhContext *c, const float *prims, int nprims, const float *tris, int ntris, const LrhMaterial *mats,
+                 int nmats, int light, float eps, float margin, int builder, LrhPtScene **out);
+void lrh_pt_scene_free(LrhPtScene *s);
+
+/* Path traces spp samples (indices s0 .. s0+spp-1) of every pixel of a W x H image and adds
+ * them to d_accum (float4 per pixel, row-major: the sum of the samples' rgb, and their count),
+ * as gpu_frame_trace (engine/app/gpu_frame.c:49-83) and trace_image (engine/rt/tracer.c:329-356)
+ * do: camera cam (lrs_camera_make's 12 floats), NEE to the lamp with MIS (mis = 1) or without,
+ * max_depth bounces, samples brighter than clamp scaled down (0: none), the image's scramble
+ * from seed. A sample's random numbers depend only on (seed, pixel, sample index), so any split
+ * of the samples over calls gives the same sums. Asynchronous on the context's stream. */
+int lrh_pt_render(LrhContext *c, const LrhPtScene *s, const float *cam, int W, int H, int spp, uint32_t s0, uint64_t seed,
+                  int max_depth, int mis, float clamp, float *d_accum);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,8 @@
 
 #include <stdint.h>
 
+#include "lr_hip.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -21,6 +23,24 @@ void lrs_gen_soup(int n, uint64_t seed, int nmat, float *tris);
  * vertical, each 3 floats; computed in double, stored as f32 */
 void lrs_camera_make(const double from[3], const double at[3], const double up[3], double vfov_deg, double aspect,
                      float cam[12]);
+
+/* The sampler's 64 x 64 blue-noise ranks (void-and-cluster, engine/base/sampler.c:86-132, as
+ * the reference's float build makes them) */
+void lrs_blue_noise(uint16_t *rank);
+/* The BSDF's tables (E[32][32], B[32][32], E_avg[32], B_avg[32]; engine/scene/brdf_tables.h's
+ * layout), computed by the quadrature tools/gen_brdf_tables.py documents */
+void lrs_brdf_tables(float *out);
+/* make_quad (engine/scene/scene.c:26-40) into the 24-float prim record of lrh_pt_scene */
+void lrs_make_quad(const float q[3], const float u[3], const float v[3], int material, float prim[24]);
+
+/* The Cornell-style box of the path-tracing workload, in a unit box (the reference's 555-unit
+ * box, engine/scene/scene.c:438-464, scaled by 1/555): 5 walls as 10 triangles, two boxes
+ * turned +15 and -18 degrees about y (24 triangles), each of the 17 flat surfaces Lambertian
+ * with an albedo uniform in [0.2, 0.8] per channel (PCG32 seeded (seed, 1)); a 0.25 x 0.25 quad
+ * lamp of radiance 15 just under the ceiling, facing down. Fills tris (34 x 12 floats), prims
+ * (1 x 24: the lamp), mats (18: the surfaces, then the lamp's), the camera of the reference's
+ * default view ((278, 278, -800) toward (278, 278, 0), vfov 40, scaled) for a w x h image. */
+void lrs_cornell(uint64_t seed, int w, int h, float *tris, float *prims, LrhMaterial *mats, float cam[12]);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,16 @@ class LrhError(RuntimeError):
     pass
 
 
+class Material(ctypes.Structure):
+    """LrhMaterial (Material, engine/scene/scene.h:31-45, untextured)"""
+    _fields_ = [("type", ctypes.c_int), ("albedo", ctypes.c_float * 3), ("emission", ctypes.c_float * 3),
+                ("ior", ctypes.c_float), ("metallic", ctypes.c_float), ("roughness", ctypes.c_float),
+                ("transmission", ctypes.c_float)]
+
+
+MAT_DIFFUSE, MAT_MIRROR, MAT_GLASS, MAT_LIGHT, MAT_PBR = 0, 1, 2, 3, 4
+
+
 class BuildStats(ctypes.Structure):
     _fields_ = [("depth", ctypes.c_int), ("rounds", ctypes.c_int), ("ms", ctypes.c_float)]
 
@@ -50,6 +60,13 @@ def lib():
         "lrh_rays_secondary": (I, [P, P, P, I, P, P, F, U, P, P, P, P]),
         "lrs_gen_soup": (None, [I, ctypes.c_uint64, I, P]),
         "lrs_camera_make": (None, [P, P, P, ctypes.c_double, ctypes.c_double, P]),
+        "lrs_blue_noise": (None, [P]),
+        "lrs_brdf_tables": (None, [P]),
+        "lrs_make_quad": (None, [P, P, P, I, P]),
+        "lrs_cornell": (None, [ctypes.c_uint64, I, I, P, P, P, P]),
+        "lrh_pt_scene": (I, [P, P, I, P, I, P, I, I, F, F, I, ctypes.POINTER(P)]),
+        "lrh_pt_scene_free": (None, [P]),
+        "lrh_pt_render": (I, [P, P, P, I, I, I, U, ctypes.c_uint64, I, I, F, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -89,6 +106,70 @@ def camera_make(frm, at, up, vfov, aspect):
     cam = np.empty(12, dtype=np.float32)
     lib().lrs_camera_make(_ptr(f), _ptr(a), _ptr(u), float(vfov), float(aspect), _ptr(cam))
     return cam
+
+
+def blue_noise():
+    import numpy as np
+    m = np.zeros(4096, np.uint16)
+    lib().lrs_blue_noise(_ptr(m))
+    return m
+
+
+def brdf_tables():
+    import numpy as np
+    t = np.zeros(2112, np.float32)
+    lib().lrs_brdf_tables(_ptr(t))
+    return t
+
+
+def make_quad(q, u, v, material):
+    import numpy as np
+    out = np.zeros(24, np.float32)
+    f = [np.ascontiguousarray(x, np.float32) for x in (q, u, v)]
+    lib().lrs_make_quad(_ptr(f[0]), _ptr(f[1]), _ptr(f[2]), material, _ptr(out))
+    return out
+
+
+def cornell(w, h, seed=0):
+    """The Cornell-style workload scene (lrs_cornell): dict of tris (34, 12), prims (1, 24),
+    mats (18 Material), light 0, cam (12,)"""
+    import numpy as np
+    tris = np.zeros((34, 12), np.float32)
+    prims = np.zeros((1, 24), np.float32)
+    mats = (Material * 18)()
+    cam = np.zeros(12, np.float32)
+    lib().lrs_cornell(seed, w, h, _ptr(tris), _ptr(prims), ctypes.cast(mats, ctypes.c_void_p), _ptr(cam))
+    return {"tris": tris, "prims": prims, "mats": mats, "light": 0, "cam": cam, "eps": 1e-4, "margin": 1e-3}
+
+
+class PtScene:
+    """A scene resident on the device for the path tracer (lrh_pt_scene)."""
+
+    def __init__(self, ctx, prims, tris, mats, light, eps, margin, builder=PLOC):
+        import numpy as np
+        self.ctx = ctx
+        pr = np.ascontiguousarray(prims, np.float32).reshape(-1, 24) if prims is not None else np.zeros((0, 24), np.float32)
+        tr = np.ascontiguousarray(tris, np.float32).reshape(-1, 12) if tris is not None else np.zeros((0, 12), np.float32)
+        self._h = ctypes.c_void_p()
+        _check(lib().lrh_pt_scene(ctx._c, _ptr(pr), pr.shape[0], _ptr(tr), tr.shape[0], ctypes.cast(mats, ctypes.c_void_p),
+                                  len(mats), light, eps, margin, builder, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().lrh_pt_scene_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, cam, w, h, spp, accum, s0=0, seed=0, max_depth=8, mis=True, clamp=0.0):
+        import numpy as np
+        c = np.ascontiguousarray(cam, np.float32)
+        _check(lib().lrh_pt_render(self.ctx._c, self._h, _ptr(c), w, h, spp, s0, seed, max_depth, 1 if mis else 0, clamp,
+                                   _ptr(accum)))
 
 
 class Context:

@@ -33,8 +33,11 @@ int lr_scratch_reserve(LrhContext *c, int which, size_t bytes)
         s.base = nullptr;
         return lr_fail(LRH_ENOMEM, "scratch: %zu bytes of device memory could not be had", want);
     }
-    // the build's counters start at zero (its kernels leave them so)
-    LR_HIP(hipMemset(s.base, 0, want));
+    // the build's counters start at zero (its kernels leave them so). Cleared on the
+    // context's stream, so that the clear is ordered before the kernels that use the memory
+    // (a plain hipMemset runs on the null stream, which a non-blocking stream does not wait for)
+    LR_HIP(hipMemsetAsync(s.base, 0, want, c->stream));
+    LR_HIP(hipStreamSynchronize(c->stream));
     s.bytes = want;
     return 0;
 }
@@ -65,7 +68,8 @@ int lrh_create(LrhContext **out, int device, void *stream)
         c->own_stream = true;
     }
     if (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
-        hipMalloc(&c->flags, 256) != hipSuccess || hipMemset(c->flags, 0, 256) != hipSuccess) {
+        hipMalloc(&c->flags, 256) != hipSuccess || hipMemsetAsync(c->flags, 0, 256, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
         lrh_destroy(c);
         return lr_fail(LRH_EDEVICE, "lrh_create: events or flags could not be made");
     }
@@ -96,7 +100,8 @@ int lrh_sync(LrhContext *c)
     uint32_t f = 0;
     LR_HIP(hipMemcpy(&f, c->flags, 4, hipMemcpyDeviceToHost));
     if (f) {
-        LR_HIP(hipMemset(c->flags, 0, 4));
+        LR_HIP(hipMemsetAsync(c->flags, 0, 4, c->stream));
+        LR_HIP(hipStreamSynchronize(c->stream));
         return lr_fail(LRH_EREFUSED, "a BVH walk needed more stack than the tree's depth allows (depth too small?)");
     }
     return 0;

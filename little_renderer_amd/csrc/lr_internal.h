@@ -21,7 +21,7 @@ struct LrScratch {
     size_t bytes = 0;
 };
 
-enum { LR_SCRATCH_BUILD = 0, LR_SCRATCH_TRACE = 1, LR_SCRATCH_COUNT = 2 };
+enum { LR_SCRATCH_BUILD = 0, LR_SCRATCH_TRACE = 1, LR_SCRATCH_PT = 2, LR_SCRATCH_COUNT = 3 };
 
 struct LrhContext {
     int device = 0;

@@ -1,0 +1,195 @@
+// lr_walk.h - the BVH walk and the shape tests, shared by the trace kernels (lr_trace.hip)
+// and the path tracer's extension and shadow kernels (lr_pt.hip).
+//
+// The walk and its arithmetic are trace.wgsl traverse (:270-392), hit_tri (:159-173),
+// hit_sphere / hit_quad (:130-157) and box_t (:175-184) with safe_inv (media.wgsl:45-48):
+// the same f32 operations in the same order (the library is built without fused
+// multiply-add), so hits equal the oracle's to the bit on the same tree. The stack differs
+// from the reference's 40 entries that drop what does not fit: the first LR_LDS_STACK
+// entries of each ray live in LDS (interleaved by lane: no bank conflicts) and deeper ones
+// spill to device memory, sized from the tree's depth, so nothing is ever dropped.
+#pragma once
+
+#include "lr_device.h"
+
+#define LR_LDS_STACK 24
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 v3of(float4 a) { return V3{a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 vadd(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 vsub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 vmul(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ V3 vscale(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 vneg(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float vdot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 vcross(V3 a, V3 b)
+{
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+__device__ __forceinline__ float safe_inv(float d) { return fabsf(d) > 1e-12f ? 1.0f / d : 1e30f; }
+
+// box_t (trace.wgsl:175-184): the entry distance, -1 on a miss
+__device__ __forceinline__ float box_t(float4 lo, float4 hi, V3 o, V3 inv, float tmin, float tmax)
+{
+    const float t0x = (lo.x - o.x) * inv.x, t0y = (lo.y - o.y) * inv.y, t0z = (lo.z - o.z) * inv.z;
+    const float t1x = (hi.x - o.x) * inv.x, t1y = (hi.y - o.y) * inv.y, t1z = (hi.z - o.z) * inv.z;
+    const float enter = wmax(wmax(wmin(t0x, t1x), wmin(t0y, t1y)), wmax(wmin(t0z, t1z), tmin));
+    const float leave = wmin(wmin(wmax(t0x, t1x), wmax(t0y, t1y)), wmin(wmax(t0z, t1z), tmax));
+    return leave < enter ? -1.0f : enter;
+}
+
+// hit_tri (trace.wgsl:159-173): t in (tmin, tmax) and the barycentrics
+__device__ __forceinline__ bool hit_tri(float4 a, float4 b, float4 c, V3 o, V3 d, float tmin, float tmax, float &t_out,
+                                        float &u_out, float &v_out)
+{
+    const V3 q = v3of(a), u = v3of(b), v = v3of(c);
+    const V3 pvec = vcross(d, v);
+    const float det = vdot(u, pvec);
+    if (fabsf(det) < 1e-12f) return false;
+    const float inv = 1.0f / det;
+    const V3 tvec = vsub(o, q);
+    const float b1 = vdot(tvec, pvec) * inv;
+    if (b1 < 0.0f || b1 > 1.0f) return false;
+    const V3 qvec = vcross(tvec, u);
+    const float b2 = vdot(d, qvec) * inv;
+    if (b2 < 0.0f || b1 + b2 > 1.0f) return false;
+    const float t = vdot(v, qvec) * inv;
+    if (t <= tmin || t >= tmax) return false;
+    t_out = t;
+    u_out = b1;
+    v_out = b2;
+    return true;
+}
+
+// a primitive's 96-byte record (trace.wgsl:3-10): c centre+radius, q corner (w: 0 sphere,
+// 1 quad), u edge (w: material), v edge (w: area), n unit normal (w: plane_d), w = (u x v)/|u x v|^2
+struct Prim {
+    float4 c, q, u, v, n, w;
+};
+
+// hit_sphere (trace.wgsl:130-144)
+__device__ __forceinline__ float hit_sphere(const Prim &p, V3 o, V3 d, float tmin, float tmax)
+{
+    const V3 oc = vsub(v3of(p.c), o);
+    const float a = vdot(d, d), hh = vdot(d, oc), c = vdot(oc, oc) - p.c.w * p.c.w;
+    const float disc = hh * hh - a * c;
+    if (disc < 0.0f) return -1.0f;
+    const float sq = sqrtf(disc);
+    float t = (hh - sq) / a;
+    if (t <= tmin || t >= tmax) {
+        t = (hh + sq) / a;
+        if (t <= tmin || t >= tmax) return -1.0f;
+    }
+    return t;
+}
+
+// hit_quad (trace.wgsl:146-157)
+__device__ __forceinline__ float hit_quad(const Prim &p, V3 o, V3 d, float tmin, float tmax, float &a_out, float &b_out)
+{
+    const V3 n = v3of(p.n);
+    const float denom = vdot(n, d);
+    if (fabsf(denom) < 1e-12f) return -1.0f;
+    const float t = (p.n.w - vdot(n, o)) / denom;
+    if (t <= tmin || t >= tmax) return -1.0f;
+    const V3 rel = vsub(vadd(o, vscale(d, t)), v3of(p.q));
+    const float a = vdot(v3of(p.w), vcross(rel, v3of(p.v)));
+    const float b = vdot(v3of(p.w), vcross(v3of(p.u), rel));
+    if (a < 0.0f || a > 1.0f || b < 0.0f || b > 1.0f) return -1.0f;
+    a_out = a;
+    b_out = b;
+    return t;
+}
+
+__device__ __forceinline__ Prim load_prim(const float4 *prims, uint32_t id)
+{
+    const float4 *p = prims + 6 * (size_t)id;
+    return Prim{p[0], p[1], p[2], p[3], p[4], p[5]};
+}
+
+struct WalkHit {
+    float t;
+    uint32_t id;
+    float b1, b2;
+};
+
+// The stack of one ray: LDS slots stk[k * stride] for k < LR_LDS_STACK, then
+// spill[(k - LR_LDS_STACK) * sstride] up to cap entries in all.
+struct WalkStack {
+    uint32_t *stk;
+    int stride;
+    uint32_t *spill;
+    size_t sstride;
+    int cap;
+};
+
+// The closest hit in (tmin, tmax), or (ANY) the first one, over shape ids taken from the
+// leaves' aux entries: ids below nprims are primitives, the rest triangles (id - nprims).
+// COUNT adds the node records fetched and the shapes tested to cn, ct.
+template <bool ANY, bool PRIMS, bool COUNT>
+__device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32_t root, const uint32_t *__restrict__ aux,
+                                        const float4 *__restrict__ tris, const float4 *__restrict__ prims, uint32_t nprims,
+                                        V3 o, V3 d, float tmin, float tmax, const WalkStack &S, uint32_t *overflow,
+                                        uint32_t &cn, uint32_t &ct)
+{
+    const V3 inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    WalkHit best{tmax, LR_NONE, 0.0f, 0.0f};
+    int sp = 0;
+    auto push = [&](uint32_t x) {
+        if (sp < LR_LDS_STACK) S.stk[sp * S.stride] = x;
+        else S.spill[(size_t)(sp - LR_LDS_STACK) * S.sstride] = x;
+        sp++;
+    };
+    push(root);
+    while (sp > 0) {
+        sp--;
+        const uint32_t ni = sp < LR_LDS_STACK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - LR_LDS_STACK) * S.sstride];
+        const float4 nlo = nodes[2 * (size_t)ni], nhi = nodes[2 * (size_t)ni + 1];
+        const uint32_t na = __float_as_uint(nlo.w), nb = __float_as_uint(nhi.w);
+        if (COUNT) cn++;
+        if (nb & LR_LEAF) {
+            const uint32_t cnt = nb & 0x7fffffffu;
+            for (uint32_t k = 0; k < cnt; k++) {
+                const uint32_t sid = aux[na + k];
+                if (COUNT) ct++;
+                if (PRIMS && sid < nprims) {
+                    const Prim p = load_prim(prims, sid);
+                    float a = 0.0f, b = 0.0f;
+                    const float t = p.q.w < 0.5f ? hit_sphere(p, o, d, tmin, best.t) : hit_quad(p, o, d, tmin, best.t, a, b);
+                    if (t > 0.0f) {
+                        best = WalkHit{t, sid, a, b};
+                        if (ANY) return best;
+                    }
+                    continue;
+                }
+                const uint32_t ti = PRIMS ? sid - nprims : sid;
+                const float4 ta = tris[3 * (size_t)ti], tb = tris[3 * (size_t)ti + 1], tc = tris[3 * (size_t)ti + 2];
+                float t, u, v;
+                if (hit_tri(ta, tb, tc, o, d, tmin, best.t, t, u, v)) {
+                    best = WalkHit{t, sid, u, v};
+                    if (ANY) return best;
+                }
+            }
+            continue;
+        }
+        const float4 alo = nodes[2 * (size_t)na], ahi = nodes[2 * (size_t)na + 1];
+        const float4 blo = nodes[2 * (size_t)nb], bhi = nodes[2 * (size_t)nb + 1];
+        const float tl = box_t(alo, ahi, o, inv, tmin, best.t);
+        const float tr = box_t(blo, bhi, o, inv, tmin, best.t);
+        if (COUNT) cn += 2;
+        if (sp + 2 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
+            atomicOr(overflow, 1u);
+            break;
+        }
+        if (tl >= 0.0f && tr >= 0.0f) {
+            const bool left_first = tl <= tr;
+            push(left_first ? nb : na);
+            push(left_first ? na : nb);
+        } else if (tl >= 0.0f) push(na);
+        else if (tr >= 0.0f) push(nb);
+    }
+    return best;
+}

@@ -3,6 +3,9 @@
 
 #include <math.h>
 #include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
 
 typedef struct { uint64_t state, inc; } Pcg;
 
@@ -92,4 +95,275 @@ void lrs_camera_make(const double from[3], const double at[3], const double up[3
         cam[6 + k] = (float)hor;
         cam[9 + k] = (float)ver;
     }
+}
+
+/* ---- the sampler's blue-noise mask: void-and-cluster (Ulichney 1993) as
+ * engine/base/sampler.c:86-132 makes it: a Gaussian energy (sigma 1.9) that wraps around the
+ * 64 x 64 tile; a tenth of the points placed at random (PCG32 seeded (0x5eedb10e, 7)) and
+ * relaxed (the most crowded moved to the largest gap until nothing moves), ranked by removing
+ * the most crowded first; the rest ranked by filling the largest gap ---- */
+#define BNS 64
+#define BNN (BNS * BNS)
+static void bn_add(float *energy, const float *kern, int p, float sign)
+{
+    const int px = p % BNS, py = p / BNS;
+    for (int y = 0; y < BNS; y++) {
+        const float *k = kern + ((y - py) & (BNS - 1)) * BNS;
+        float *e = energy + y * BNS;
+        for (int x = 0; x < px; x++) e[x] += sign * k[x - px + BNS];
+        for (int x = px; x < BNS; x++) e[x] += sign * k[x - px];
+    }
+}
+static int bn_pick(const float *energy, const unsigned char *on, int want_on, int want_max)
+{
+    int best = -1;
+    for (int i = 0; i < BNN; i++) {
+        if (on[i] != want_on) continue;
+        if (best < 0 || (want_max ? energy[i] > energy[best] : energy[i] < energy[best])) best = i;
+    }
+    return best;
+}
+void lrs_blue_noise(uint16_t *rank)
+{
+    static float kern[BNN], energy[BNN], saved[BNN];
+    static unsigned char on[BNN], on_saved[BNN];
+    const double sigma = (double)1.9f;   /* the float build's literal (-fsingle-precision-constant) */
+    for (int y = 0; y < BNS; y++)
+        for (int x = 0; x < BNS; x++) {
+            const int dx = x < BNS / 2 ? x : x - BNS, dy = y < BNS / 2 ? y : y - BNS;
+            kern[y * BNS + x] = (float)exp(-(dx * dx + dy * dy) / (2 * sigma * sigma));
+        }
+    for (int i = 0; i < BNN; i++) { on[i] = 0; energy[i] = 0.0f; }
+    Pcg r;
+    pcg_seed(&r, 0x5eedb10eULL, 7);
+    const int ones = BNN / 10;
+    for (int n = 0; n < ones;) {
+        const int p = (int)(pcg_u32(&r) % BNN);
+        if (on[p]) continue;
+        on[p] = 1;
+        bn_add(energy, kern, p, 1.0f);
+        n++;
+    }
+    for (int it = 0; it < BNN; it++) {
+        const int c = bn_pick(energy, on, 1, 1);
+        on[c] = 0;
+        bn_add(energy, kern, c, -1.0f);
+        const int v = bn_pick(energy, on, 0, 0);
+        on[v] = 1;
+        bn_add(energy, kern, v, 1.0f);
+        if (v == c) break;
+    }
+    for (int i = 0; i < BNN; i++) { on_saved[i] = on[i]; saved[i] = energy[i]; }
+    for (int k = ones - 1; k >= 0; k--) {
+        const int c = bn_pick(energy, on, 1, 1);
+        on[c] = 0;
+        bn_add(energy, kern, c, -1.0f);
+        rank[c] = (uint16_t)k;
+    }
+    for (int i = 0; i < BNN; i++) { on[i] = on_saved[i]; energy[i] = saved[i]; }
+    for (int k = ones; k < BNN; k++) {
+        const int v = bn_pick(energy, on, 0, 0);
+        on[v] = 1;
+        bn_add(energy, kern, v, 1.0f);
+        rank[v] = (uint16_t)k;
+    }
+}
+
+/* ---- the BSDF's tables (the method tools/gen_brdf_tables.py documents): E(mu, r), the
+ * directional albedo of a white GGX lobe (alpha = r^2, height-correlated Smith) seen at
+ * cosine mu, and B(mu, r), the same with (1 - v.h)^5 inside, on 32 x 32 nodes, each the mean
+ * over 128 x 128 midpoint-stratified visible-normal samples (weight G2 / G1); E_avg, B_avg =
+ * 2 * integral of E(mu) mu dmu by the trapezoid rule on 2049 points, E linearly interpolated
+ * between the nodes. Stored, as the reference's table is, rounded to 6 decimals. ---- */
+static double tab_g1(double c, double a) { c = fabs(c); return 2 * c / (c + sqrt(a * a + (1 - a * a) * c * c)); }
+static double tab_vis(double cv, double cl, double a)
+{
+    const double a2 = a * a;
+    const double d = cl * sqrt(cv * cv * (1 - a2) + a2) + cv * sqrt(cl * cl * (1 - a2) + a2);
+    return 0.5 / d;
+}
+static void tab_albedo(double mu, double r, double *E, double *B)
+{
+    const int S = 128;
+    double a = r * r;
+    if (a < 1e-4) a = 1e-4;
+    if (mu < 1e-4) mu = 1e-4;
+    const double v[3] = {sqrt(1 - mu * mu), 0.0, mu};
+    double vh[3] = {a * v[0], a * v[1], v[2]};
+    const double ln = sqrt(vh[0] * vh[0] + vh[1] * vh[1] + vh[2] * vh[2]);
+    for (int k = 0; k < 3; k++) vh[k] /= ln;
+    double se = 0, sb = 0;
+    for (int i = 0; i < S; i++)
+        for (int j = 0; j < S; j++) {
+            const double u1 = (i + 0.5) / S, u2 = (j + 0.5) / S;
+            const double phi = 2 * M_PI * u1;
+            const double z = (1 - u2) * (1 + vh[2]) - vh[2];
+            const double st = sqrt(fmax(0.0, 1 - z * z));
+            const double hs[3] = {st * cos(phi) + vh[0], st * sin(phi) + vh[1], z + vh[2]};
+            double h[3] = {a * hs[0], a * hs[1], fmax(hs[2], 0.0)};
+            const double hl = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+            for (int k = 0; k < 3; k++) h[k] /= hl;
+            const double vdh = h[0] * v[0] + h[1] * v[1] + h[2] * v[2];
+            const double cl = 2 * vdh * h[2] - v[2];
+            double w = 0;
+            if (cl > 0) w = tab_vis(mu, fmax(cl, 1e-9), a) * 4 * mu * fmax(cl, 0.0) / tab_g1(mu, a);
+            const double x = 1 - fmin(fmax(vdh, 0.0), 1.0), x5 = x * x * x * x * x;
+            se += w;
+            sb += w * x5;
+        }
+    *E = se / (S * S);
+    *B = sb / (S * S);
+}
+static float tab_round(double x)
+{
+    char buf[64];
+    snprintf(buf, sizeof(buf), "%.6f", x);
+    return strtof(buf, NULL);
+}
+static double tab_avg(const double *row)
+{
+    /* 2 * trapezoid of interp(fine, mu, row) * fine over fine = linspace(0, 1, 2049) */
+    const int F = 2049, N = 32;
+    double s = 0, prev = 0;
+    for (int k = 0; k < F; k++) {
+        const double x = (double)k / (F - 1);
+        double pos = x * (N - 1);
+        int i = (int)pos;
+        if (i > N - 2) i = N - 2;
+        const double t = pos - i;
+        const double y = (row[i] + t * (row[i + 1] - row[i])) * x;
+        if (k > 0) s += (prev + y) * 0.5 / (F - 1);
+        prev = y;
+    }
+    return 2 * s;
+}
+void lrs_brdf_tables(float *out)
+{
+    static float cache[2112];
+    static int ready;
+    if (!ready) {
+        double E[32][32], B[32][32];
+        for (int j = 0; j < 32; j++)
+            for (int i = 0; i < 32; i++) tab_albedo(i / 31.0, j / 31.0, &E[j][i], &B[j][i]);
+        for (int j = 0; j < 32; j++)
+            for (int i = 0; i < 32; i++) {
+                cache[j * 32 + i] = tab_round(E[j][i]);
+                cache[1024 + j * 32 + i] = tab_round(B[j][i]);
+            }
+        for (int j = 0; j < 32; j++) {
+            cache[2048 + j] = tab_round(tab_avg(E[j]));
+            cache[2080 + j] = tab_round(tab_avg(B[j]));
+        }
+        ready = 1;
+    }
+    for (int i = 0; i < 2112; i++) out[i] = cache[i];
+}
+
+/* ---- make_quad (scene.c:26-40), in float as the reference's default build ---- */
+void lrs_make_quad(const float q[3], const float u[3], const float v[3], int material, float p[24])
+{
+    for (int k = 0; k < 24; k++) p[k] = 0.0f;
+    const float n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    const float len = sqrtf(nn);
+    float unit[3];
+    for (int k = 0; k < 3; k++) unit[k] = len > 0 ? n[k] * (1.0f / len) : n[k];
+    const float plane_d = unit[0] * q[0] + unit[1] * q[1] + unit[2] * q[2];
+    const float inv = 1.0f / nn;
+    for (int k = 0; k < 3; k++) {
+        p[4 + k] = q[k];
+        p[8 + k] = u[k];
+        p[12 + k] = v[k];
+        p[16 + k] = unit[k];
+        p[20 + k] = n[k] * inv;
+    }
+    p[7] = 1.0f;                  /* a quad */
+    p[11] = (float)material;
+    p[15] = len;                  /* its area */
+    p[19] = plane_d;
+}
+
+/* ---- the Cornell-style box ---- */
+static void put_tri(float *o, const double a[3], const double b[3], const double c[3], int mat)
+{
+    for (int k = 0; k < 3; k++) {
+        o[k] = (float)a[k];
+        o[4 + k] = (float)(b[k] - a[k]);
+        o[8 + k] = (float)(c[k] - a[k]);
+    }
+    uint32_t w[3] = {0xffffffffu, 0u, (uint32_t)(mat & 63) << 26};
+    for (int k = 0; k < 3; k++) { union { uint32_t u; float f; } x; x.u = w[k]; o[3 + 4 * k] = x.f; }
+}
+/* a quad q + a u + b v as two triangles: (q, q+u, q+v) and (q+u+v, q+v, q+u), both facing u x v */
+static int put_quad(float *tris, int at, const double q[3], const double u[3], const double v[3], int mat)
+{
+    double a[3], b[3], c[3], d[3];
+    for (int k = 0; k < 3; k++) { a[k] = q[k]; b[k] = q[k] + u[k]; c[k] = q[k] + v[k]; d[k] = q[k] + u[k] + v[k]; }
+    put_tri(tris + 12 * (size_t)at, a, b, c, mat);
+    put_tri(tris + 12 * (size_t)(at + 1), d, c, b, mat);
+    return at + 2;
+}
+static void rot_y(const double p[3], double deg, double o[3])
+{
+    const double a = deg * M_PI / 180.0, c = cos(a), s = sin(a);
+    o[0] = c * p[0] + s * p[2];
+    o[1] = p[1];
+    o[2] = -s * p[0] + c * p[2];
+}
+/* add_box (scene.c:379-389): 6 faces of a box (0,0,0)..size turned by deg about y, moved by off */
+static int put_box(float *tris, int at, const double size[3], double deg, const double off[3], int *mat, double sc)
+{
+    const double dx[3] = {size[0], 0, 0}, dy[3] = {0, size[1], 0}, dz[3] = {0, 0, size[2]};
+    const double corner[6][3] = {{0, 0, 0}, {0, 0, size[2]}, {0, 0, 0}, {size[0], 0, 0}, {0, 0, 0}, {0, size[1], 0}};
+    const double *eu[6] = {dx, dx, dz, dz, dx, dx}, *ev[6] = {dy, dy, dy, dy, dz, dz};
+    for (int i = 0; i < 6; i++) {
+        double q[3], u[3], v[3];
+        rot_y(corner[i], deg, q);
+        rot_y(eu[i], deg, u);
+        rot_y(ev[i], deg, v);
+        for (int k = 0; k < 3; k++) { q[k] = (q[k] + off[k]) * sc; u[k] *= sc; v[k] *= sc; }
+        at = put_quad(tris, at, q, u, v, (*mat)++);
+    }
+    return at;
+}
+
+void lrs_cornell(uint64_t seed, int w, int h, float *tris, float *prims, LrhMaterial *mats, float cam[12])
+{
+    const double sc = 1.0 / 555.0;
+    Pcg r;
+    pcg_seed(&r, seed, 1);
+    for (int i = 0; i < 17; i++) {
+        LrhMaterial *m = &mats[i];
+        memset(m, 0, sizeof(*m));
+        m->type = LRH_MAT_DIFFUSE;
+        for (int k = 0; k < 3; k++) m->albedo[k] = (float)(0.2 + 0.6 * pcg_unit(&r));
+        m->ior = 1.5f;
+        m->roughness = 1.0f;
+    }
+    LrhMaterial *lm = &mats[17];
+    memset(lm, 0, sizeof(*lm));
+    lm->type = LRH_MAT_LIGHT;
+    lm->emission[0] = lm->emission[1] = lm->emission[2] = 15.0f;
+    lm->ior = 1.0f;
+    lm->roughness = 1.0f;
+    /* the walls (scene.c:449-453): left, right, floor, ceiling, back */
+    const double wq[5][3] = {{555, 0, 0}, {0, 0, 0}, {0, 0, 0}, {555, 555, 555}, {0, 0, 555}};
+    const double wu[5][3] = {{0, 555, 0}, {0, 555, 0}, {555, 0, 0}, {-555, 0, 0}, {555, 0, 0}};
+    const double wv[5][3] = {{0, 0, 555}, {0, 0, 555}, {0, 0, 555}, {0, 0, -555}, {0, 555, 0}};
+    int at = 0;
+    for (int i = 0; i < 5; i++) {
+        double q[3], u[3], v[3];
+        for (int k = 0; k < 3; k++) { q[k] = wq[i][k] * sc; u[k] = wu[i][k] * sc; v[k] = wv[i][k] * sc; }
+        at = put_quad(tris, at, q, u, v, i);
+    }
+    int mat = 5;
+    const double tall[3] = {165, 330, 165}, tall_off[3] = {265, 0, 295};
+    const double shrt[3] = {165, 165, 165}, shrt_off[3] = {130, 0, 65};
+    at = put_box(tris, at, tall, 15.0, tall_off, &mat, sc);
+    at = put_box(tris, at, shrt, -18.0, shrt_off, &mat, sc);
+    /* the lamp: 0.25 x 0.25, centred under the ceiling, u x v pointing down */
+    const float lq[3] = {0.625f, (float)(554.0 * sc), 0.625f}, lu[3] = {-0.25f, 0.0f, 0.0f}, lv[3] = {0.0f, 0.0f, -0.25f};
+    lrs_make_quad(lq, lu, lv, 17, prims);
+    const double from[3] = {278 * sc, 278 * sc, -800 * sc}, to[3] = {278 * sc, 278 * sc, 0}, up[3] = {0, 1, 0};
+    lrs_camera_make(from, to, up, 40.0, (double)w / (double)h, cam);
 }

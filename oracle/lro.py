@@ -56,6 +56,13 @@ def lib():
             "lro_trace_gpu_layout": (None, [P, U, P, P, P, I, I, P, P]),
             "lro_trace_sah": (None, [P, P, P, I, I, P]),
             "lro_trace_brute": (None, [P, I, P, I, I, P, P]),
+            "lro_sampler_hash": (U, [U]),
+            "lro_sampler_owen": (U, [U, U]),
+            "lro_sampler_sobol1": (U, [U]),
+            "lro_blue_noise": (None, [P]),
+            "lro_sampler_2d": (None, [P, I, I, U, U, I, P]),
+            "lro_bsdf_case": (None, [P, I, P, P, P, P, F, F, F, P]),
+            "lro_pt_image": (None, [P, P, I, I, U, I, U64, F, I, I, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -87,6 +94,9 @@ def ref(double=False):
         "sampler_owen": (U, [U, U]),
         "sampler_sobol1": (U, [U]),
         "blue_noise_mask": (P, []),
+        "refh_brdf_tables": (None, [P]),
+        "refh_bsdf_case": (None, [P, I, P, P, P, F, F, F, P]),
+        "refh_render": (None, [P, I, P, I, P, I, P, I, I, I, U64, I, I, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -240,3 +250,73 @@ def trace_brute(tris, rays, any_hit=False):
 
 def hit_ids(hits):
     return np.ascontiguousarray(hits).view(np.uint32)[:, 1]
+
+
+# ---- the path tracer ------------------------------------------------------------------------
+class LroMat(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("albedo", ctypes.c_float * 3), ("emission", ctypes.c_float * 3),
+                ("ior", ctypes.c_float), ("metallic", ctypes.c_float), ("roughness", ctypes.c_float),
+                ("transmission", ctypes.c_float)]
+
+
+class LroScene(ctypes.Structure):
+    _fields_ = [("prims", ctypes.c_void_p), ("nprims", ctypes.c_int), ("tris", ctypes.c_void_p), ("ntris", ctypes.c_int),
+                ("mats", ctypes.c_void_p), ("nmats", ctypes.c_int), ("light", ctypes.c_int), ("eps", ctypes.c_float),
+                ("margin", ctypes.c_float), ("mask", ctypes.c_void_p), ("tab", ctypes.c_void_p)]
+
+
+def blue_noise():
+    m = np.zeros(4096, np.uint16)
+    lib().lro_blue_noise(ptr(m))
+    return m
+
+
+def mats_of(mats):
+    """Materials (any ctypes array of structures laid out as LroMat) as an LroMat array"""
+    out = (LroMat * len(mats))()
+    ctypes.memmove(out, mats, ctypes.sizeof(out))
+    return out
+
+
+def pt_image(scene, W, H, s0, ns, seed=0, clamp=0.0, max_depth=8, mis=True, tab=None, mask=None):
+    """lro_pt_image: the rgb sums of samples s0 .. s0+ns-1 of every pixel, (H, W, 3) float32.
+    scene: dict of prims (n, 24), tris (m, 12), mats, light, cam, eps, margin; tab, mask: the
+    BSDF tables and blue-noise ranks (as the device gets them)"""
+    prims = np.ascontiguousarray(scene["prims"], np.float32)
+    tris = np.ascontiguousarray(scene["tris"], np.float32)
+    mats = mats_of(scene["mats"])
+    keep = [prims, tris, mats, tab, mask]
+    sc = LroScene(ptr(prims), prims.shape[0], ptr(tris), tris.shape[0], ctypes.cast(mats, ctypes.c_void_p), len(mats),
+                  scene["light"], scene["eps"], scene["margin"], ptr(mask), ptr(tab))
+    out = np.zeros((H, W, 3), np.float32)
+    cam = np.ascontiguousarray(scene["cam"], np.float32)
+    lib().lro_pt_image(ctypes.byref(sc), ptr(cam), W, H, s0, ns, seed, clamp, max_depth, 1 if mis else 0, ptr(out))
+    del keep
+    return out
+
+
+def ref_render(scene, W, H, spp, seed=0, max_depth=8, mis=True, double=True):
+    """The reference's own trace_image (its double build by default) on the same scene: the
+    mean of spp samples per pixel, (H, W, 3) float64; None where oracle/_ref was not built"""
+    R = ref(double)
+    if R is None:
+        return None
+    tris = np.ascontiguousarray(scene["tris"], np.float32)
+    pr = np.ascontiguousarray(scene["prims"], np.float32).reshape(-1, 24)
+    lamp = None
+    lamp_mat = -1
+    if scene["light"] >= 0:
+        p = pr[scene["light"]]
+        lamp = np.concatenate([p[4:7], p[8:11], p[12:15]]).astype(np.float32)
+        lamp_mat = int(p[11])
+    m = np.zeros((len(scene["mats"]), 12), np.float32)
+    for i, x in enumerate(scene["mats"]):
+        m[i, 0] = x.type
+        m[i, 1:4] = list(x.albedo)
+        m[i, 4:8] = [x.ior, x.metallic, x.roughness, x.transmission]
+        m[i, 8:11] = list(x.emission)
+    cam = np.ascontiguousarray(scene["cam"], np.float32)
+    px = np.zeros((H, W, 3), np.float64)
+    R.refh_render(ptr(tris), tris.shape[0], ptr(lamp), lamp_mat, ptr(m), m.shape[0], ptr(cam), W, H, spp, seed, max_depth,
+                  1 if mis else 0, ptr(px))
+    return px

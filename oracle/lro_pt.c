@@ -1,0 +1,734 @@
+/* lro_pt.c - the oracle's path tracer: the sampler (engine/base/sampler.c), the BSDF
+ * (engine/scene/brdf.h), next event estimation with MIS and the bounce loop
+ * (engine/rt/tracer.c:30-81, 295-509) and the image loop (tracer.c:329-356), restated in
+ * plain C with `real` = float, the reference's default build (-fsingle-precision-constant:
+ * every literal below is a float). TEST INFRASTRUCTURE ONLY (see lro.h).
+ *
+ * One deliberate difference from trace_image: each (pixel, sample) has its own PCG32 stream
+ * (the pixel's seed of tracer.c:339 xor the sample index times a constant), so that samples can
+ * be traced in any order, on any device, alone or batched, and give the same numbers. The
+ * device path tracer uses the same rule. */
+#include "lro_pt.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef float real;
+typedef struct { real x, y, z; } V;
+static inline V v3(real x, real y, real z) { V r = {x, y, z}; return r; }
+static inline V vadd(V a, V b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V vsub(V a, V b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V vmul(V a, V b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline V vscale(V a, real s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline V vneg(V a) { return v3(-a.x, -a.y, -a.z); }
+static inline real vdot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V vcross(V a, V b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline real rfmin(real a, real b) { return b != b ? a : (a < b ? a : b); }
+static inline real rfmax(real a, real b) { return b != b ? a : (a > b ? a : b); }
+static inline real vlen(V a) { return sqrtf(vdot(a, a)); }
+static inline V vnorm(V a) { real l = vlen(a); return l > 0 ? vscale(a, 1.0f / l) : a; }
+static inline real vmax(V a) { return rfmax(a.x, rfmax(a.y, a.z)); }
+static inline V vreflect(V d, V n) { return vsub(d, vscale(n, 2.0f * vdot(d, n))); }
+static inline V vrefract(V d, V n, real eta)
+{
+    real cos_i = rfmin(vdot(vneg(d), n), 1.0f);
+    V perp = vscale(vadd(d, vscale(n, cos_i)), eta);
+    V para = vscale(n, -sqrtf(fabsf(1.0f - vdot(perp, perp))));
+    return vadd(perp, para);
+}
+static inline V vload(const float *p) { return v3(p[0], p[1], p[2]); }
+
+#define PI_ 3.14159265358979323846
+
+/* ---- the sampler (sampler.c:7-175) ---- */
+uint32_t lro_sampler_hash(uint32_t x)
+{
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+static inline uint32_t reverse_bits(uint32_t x)
+{
+    x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
+    x = ((x >> 2) & 0x33333333u) | ((x & 0x33333333u) << 2);
+    x = ((x >> 4) & 0x0f0f0f0fu) | ((x & 0x0f0f0f0fu) << 4);
+    x = ((x >> 8) & 0x00ff00ffu) | ((x & 0x00ff00ffu) << 8);
+    return (x >> 16) | (x << 16);
+}
+static inline uint32_t owen_reversed(uint32_t x, uint32_t seed)   /* sampler.c:30-37 (Burley 2020) */
+{
+    x ^= x * 0x3d20adeau;
+    x += seed;
+    x *= (seed >> 16) | 1u;
+    x ^= x * 0x05526c56u;
+    x ^= x * 0x53a22864u;
+    return reverse_bits(x);
+}
+uint32_t lro_sampler_owen(uint32_t x, uint32_t seed) { return owen_reversed(reverse_bits(x), seed); }
+static inline uint32_t sobol1_reversed(uint32_t i)                 /* sampler.c:46-54 */
+{
+    i ^= (i >> 1) & 0x55555555u;
+    i ^= (i >> 2) & 0x33333333u;
+    i ^= (i >> 4) & 0x0f0f0f0fu;
+    i ^= (i >> 8) & 0x00ff00ffu;
+    i ^= i >> 16;
+    return i;
+}
+uint32_t lro_sampler_sobol1(uint32_t i) { return reverse_bits(sobol1_reversed(i)); }
+
+/* the blue-noise mask: void-and-cluster (sampler.c:86-132) */
+#define BN 64
+#define BN_N (BN * BN)
+static void bn_update(float *energy, const float *kern, int p, float sign)
+{
+    int px = p % BN, py = p / BN;
+    for (int y = 0; y < BN; y++) {
+        const float *k = kern + ((y - py) & (BN - 1)) * BN;
+        float *e = energy + y * BN;
+        for (int x = 0; x < px; x++) e[x] += sign * k[x - px + BN];
+        for (int x = px; x < BN; x++) e[x] += sign * k[x - px];
+    }
+}
+static int bn_extreme(const float *energy, const unsigned char *on, int want_on, int want_max)
+{
+    int best = -1;
+    for (int i = 0; i < BN_N; i++) {
+        if (on[i] != want_on) continue;
+        if (best < 0 || (want_max ? energy[i] > energy[best] : energy[i] < energy[best])) best = i;
+    }
+    return best;
+}
+void lro_blue_noise(uint16_t *rank)
+{
+    float *kern = malloc(BN_N * 4), *energy = malloc(BN_N * 4), *e0 = malloc(BN_N * 4);
+    unsigned char on[BN_N], on0[BN_N];
+    const double sigma = (double)1.9f;   /* the float build's literal (-fsingle-precision-constant) */
+    for (int y = 0; y < BN; y++)
+        for (int x = 0; x < BN; x++) {
+            int dx = x < BN / 2 ? x : x - BN, dy = y < BN / 2 ? y : y - BN;
+            kern[y * BN + x] = (float)exp(-(dx * dx + dy * dy) / (2 * sigma * sigma));
+        }
+    memset(on, 0, sizeof(on));
+    memset(energy, 0, BN_N * 4);
+    LroRng rng;
+    lro_rng_seed(&rng, 0x5eedb10eULL, 7);
+    const int ones = BN_N / 10;
+    for (int n = 0; n < ones;) {
+        int p = (int)(lro_rng_u32(&rng) % BN_N);
+        if (on[p]) continue;
+        on[p] = 1;
+        bn_update(energy, kern, p, 1);
+        n++;
+    }
+    for (int it = 0; it < BN_N; it++) {
+        int c = bn_extreme(energy, on, 1, 1);
+        on[c] = 0;
+        bn_update(energy, kern, c, -1);
+        int v = bn_extreme(energy, on, 0, 0);
+        on[v] = 1;
+        bn_update(energy, kern, v, 1);
+        if (v == c) break;
+    }
+    memcpy(on0, on, sizeof(on));
+    memcpy(e0, energy, BN_N * 4);
+    for (int r = ones - 1; r >= 0; r--) {
+        int c = bn_extreme(energy, on, 1, 1);
+        on[c] = 0;
+        bn_update(energy, kern, c, -1);
+        rank[c] = (uint16_t)r;
+    }
+    memcpy(on, on0, sizeof(on));
+    memcpy(energy, e0, BN_N * 4);
+    for (int r = ones; r < BN_N; r++) {
+        int v = bn_extreme(energy, on, 0, 0);
+        on[v] = 1;
+        bn_update(energy, kern, v, 1);
+        rank[v] = (uint16_t)r;
+    }
+    free(kern); free(energy); free(e0);
+}
+
+typedef struct {
+    LroRng *rng;
+    uint32_t index, seed;
+    int x, y, on;
+    const uint16_t *mask;
+} Smp;
+
+static inline real bn_value(const uint16_t *m, int x, int y)
+{
+    return ((real)m[(y & (BN - 1)) * BN + (x & (BN - 1))] + 0.5f) * (real)(1.0 / BN_N);
+}
+static inline real unit24(uint32_t v) { return (real)(v >> 8) * (1.0f / 16777216.0f); }
+static inline real wrap01(real t) { return t >= 1 ? t - 1 : t; }
+
+/* sampler_2d (sampler.c:143-160) */
+static void sampler_2d(Smp *s, int dim, real *a, real *b)
+{
+    if (!s->on || dim >= LRO_SAMPLER_PAIRS) {
+        *a = lro_rng_float(s->rng);
+        *b = lro_rng_float(s->rng);
+        return;
+    }
+    uint32_t tile = lro_sampler_hash(((uint32_t)(s->x >> 6) * 73856093u) ^ ((uint32_t)(s->y >> 6) * 19349663u));
+    uint32_t seed = lro_sampler_hash(s->seed ^ lro_sampler_hash(((uint32_t)dim * 0x9e3779b9u) ^ tile));
+    uint32_t i = lro_sampler_owen(s->index, seed);
+    uint32_t xa = owen_reversed(i, lro_sampler_hash(seed ^ 0xa511e9b3u));
+    uint32_t xb = owen_reversed(sobol1_reversed(i), lro_sampler_hash(seed ^ 0x63d83595u));
+    int ox = s->x + 23 * dim, oy = s->y + 41 * dim;
+    *a = wrap01(unit24(xa) + bn_value(s->mask, ox, oy));
+    *b = wrap01(unit24(xb) + bn_value(s->mask, ox + 32, oy + 17));
+}
+
+void lro_sampler_2d(const uint16_t *mask, int x, int y, uint32_t index, uint32_t seed, int dim, float out[2])
+{
+    LroRng rng;
+    lro_rng_seed(&rng, 1, 1);
+    Smp s = {&rng, index, seed, x, y, 1, mask};
+    sampler_2d(&s, dim, &out[0], &out[1]);
+}
+
+/* ---- the BSDF (brdf.h:43-348) ---- */
+enum { SPEC, DIFF, TRANS, MS, LOBES };
+typedef struct {
+    V base;
+    real metallic, transmission, rough, alpha, eta, e_avg, es_avg;
+    int smooth;
+    V ms;
+    real kd_norm, kms_norm;
+    real cv, p[LOBES], es_v, e_v;
+    const float *tab;      /* E[32][32], B[32][32], E_avg[32], B_avg[32] */
+} Bsdf;
+typedef struct { V l, w; real pdf; int delta, through; } BSample;
+
+static inline real clamp01(real x) { return x < 0 ? 0 : x > 1 ? 1 : x; }
+static inline real lum(V c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+static inline real table(const float *t, real rough, real mu)      /* brdf.h:71-80 */
+{
+    real x = rough * (LRO_BRDF_N - 1), y = clamp01(mu) * (LRO_BRDF_N - 1);
+    int j = (int)x, i = (int)y;
+    if (j > LRO_BRDF_N - 2) j = LRO_BRDF_N - 2;
+    if (i > LRO_BRDF_N - 2) i = LRO_BRDF_N - 2;
+    real fx = x - j, fy = y - i;
+    real e0 = t[j * 32 + i] + fy * (t[j * 32 + i + 1] - t[j * 32 + i]);
+    real e1 = t[(j + 1) * 32 + i] + fy * (t[(j + 1) * 32 + i + 1] - t[(j + 1) * 32 + i]);
+    return e0 + fx * (e1 - e0);
+}
+static inline real albedo_e(const Bsdf *b, real mu) { return table(b->tab, b->rough, mu); }
+static inline real albedo_b(const Bsdf *b, real mu) { return table(b->tab + 1024, b->rough, mu); }
+static inline real fresnel_dielectric(real cos_i, real eta)        /* brdf.h:88-98 */
+{
+    cos_i = clamp01(cos_i);
+    real s2 = eta * eta * (1 - cos_i * cos_i);
+    if (s2 >= 1) return 1;
+    real c = eta > 1 ? sqrtf(1 - s2) : cos_i;
+    real f0 = (1 - eta) / (1 + eta);
+    f0 *= f0;
+    real x = 1 - c, x2 = x * x;
+    return f0 + (1 - f0) * x2 * x2 * x;
+}
+static inline real fresnel_dielectric_avg(real eta)               /* brdf.h:103-109 */
+{
+    real e = eta > 1 ? 1 / eta : eta;
+    real f0 = (1 - e) / (1 + e);
+    f0 *= f0;
+    real avg = f0 + (1 - f0) / 21;
+    return eta > 1 ? 1 - (1 - avg) / (eta * eta) : avg;
+}
+static inline real spec_albedo(const Bsdf *b, real mu)            /* brdf.h:114-119 */
+{
+    if (b->smooth || b->eta > 1) return fresnel_dielectric(mu, b->eta);
+    real f0 = (1 - b->eta) / (1 + b->eta), e = albedo_e(b, mu), bb = albedo_b(b, mu);
+    f0 *= f0;
+    return f0 * (e - bb) + bb;
+}
+static inline real spec_albedo_avg(const Bsdf *b)
+{
+    return b->smooth || b->eta > 1 ? fresnel_dielectric_avg(b->eta) : b->es_avg;
+}
+static inline V fresnel_metal(V f0, real c)                       /* brdf.h:125-128 */
+{
+    real x = 1 - clamp01(c), x2 = x * x, x5 = x2 * x2 * x;
+    return v3(f0.x + (1 - f0.x) * x5, f0.y + (1 - f0.y) * x5, f0.z + (1 - f0.z) * x5);
+}
+static inline real ggx_d(real nh, real s2, real a)                /* brdf.h:132-135 */
+{
+    real a2 = a * a, d = s2 + a2 * nh * nh;
+    return a2 / ((real)PI_ * d * d);
+}
+static inline real ggx_s2(V n, V h) { V c = vcross(n, h); return vdot(c, c); }
+static inline real ggx_g1(real c, real a)
+{
+    c = fabsf(c);
+    return 2 * c / (c + sqrtf(a * a + (1 - a * a) * c * c));
+}
+static inline real ggx_vis(real cv, real cl, real a)
+{
+    cv = fabsf(cv); cl = fabsf(cl);
+    real a2 = a * a;
+    real d = cl * sqrtf(cv * cv * (1 - a2) + a2) + cv * sqrtf(cl * cl * (1 - a2) + a2);
+    return d > 0 ? 0.5f / d : 0;
+}
+static void onb(V n, V *t, V *b)                                  /* rng.h:53-59 */
+{
+    real s = copysignf(1.0f, n.z);
+    real a = -1.0f / (s + n.z);
+    real c = n.x * n.y * a;
+    *t = v3(1.0f + s * n.x * n.x * a, s * c, -s * n.x);
+    *b = v3(c, s + n.y * n.y * a, -n.y);
+}
+static V cosine_hemisphere(V n, real u1, real u2)                /* rng.h:63-70 */
+{
+    real phi = 2.0f * (real)PI_ * u1;
+    real rad = sqrtf(u2);
+    real x = rad * cosf(phi), y = rad * sinf(phi), z = sqrtf(rfmax(0.0f, 1.0f - u2));
+    V t, b;
+    onb(n, &t, &b);
+    return vadd(vadd(vscale(t, x), vscale(b, y)), vscale(n, z));
+}
+static V ggx_sample_vndf(V n, V v, real a, real u1, real u2)     /* brdf.h:154-164 */
+{
+    V t, bt;
+    onb(n, &t, &bt);
+    V vh = vnorm(v3(a * vdot(v, t), a * vdot(v, bt), vdot(v, n)));
+    real phi = 2 * (real)PI_ * u1;
+    real z = (1 - u2) * (1 + vh.z) - vh.z;
+    real st = sqrtf(rfmax(0, 1 - z * z));
+    V hs = v3(st * cosf(phi) + vh.x, st * sinf(phi) + vh.y, z + vh.z);
+    V hl = vnorm(v3(a * hs.x, a * hs.y, rfmax(hs.z, 0)));
+    return vadd(vadd(vscale(t, hl.x), vscale(bt, hl.y)), vscale(n, hl.z));
+}
+static V ms_color(const Bsdf *b)                                  /* brdf.h:168-172 */
+{
+    real e = b->e_avg;
+    V fa = v3(b->base.x + (1 - b->base.x) / 21, b->base.y + (1 - b->base.y) / 21, b->base.z + (1 - b->base.z) / 21);
+    return v3(fa.x * fa.x * e / (1 - fa.x * (1 - e)), fa.y * fa.y * e / (1 - fa.y * (1 - e)), fa.z * fa.z * e / (1 - fa.z * (1 - e)));
+}
+
+/* material_resolved (scene.h:80-90) then bsdf_make (brdf.h:176-205) */
+static Bsdf bsdf_make(const LroMat *mat, int outside, const float *tab)
+{
+    real metallic = mat->metallic, roughness = mat->roughness, ior = mat->ior, transmission = mat->transmission;
+    switch (mat->type) {
+    case LRO_MAT_DIFFUSE: metallic = 0; roughness = 1; ior = 1.5f; transmission = 0; break;
+    case LRO_MAT_MIRROR: metallic = 1; roughness = 0; ior = 1.5f; transmission = 0; break;
+    case LRO_MAT_GLASS: metallic = 0; roughness = 0; transmission = 1; if (!(ior > 0)) ior = 1.5f; break;
+    case LRO_MAT_LIGHT: metallic = 0; roughness = 1; transmission = 0; break;
+    default: if (!(ior > 0)) ior = 1.5f; break;
+    }
+    Bsdf b;
+    b.tab = tab;
+    b.base = vload(mat->albedo);
+    b.metallic = clamp01(metallic);
+    b.transmission = clamp01(transmission);
+    real r = clamp01(roughness);
+    b.smooth = r <= 0.02f;
+    b.rough = r;
+    b.alpha = rfmax(r * r, 1e-4f);
+    ior = ior > 0 ? ior : 1.5f;
+    b.eta = b.transmission > 0 && !outside ? ior : 1 / ior;
+    real x = r * (LRO_BRDF_N - 1);
+    int j = (int)x;
+    if (j > LRO_BRDF_N - 2) j = LRO_BRDF_N - 2;
+    const float *EA = tab + 2048, *BA = tab + 2080;
+    b.e_avg = EA[j] + (x - j) * (EA[j + 1] - EA[j]);
+    real b_avg = BA[j] + (x - j) * (BA[j + 1] - BA[j]);
+    real f0 = (1 - b.eta) / (1 + b.eta);
+    f0 *= f0;
+    b.es_avg = f0 * (b.e_avg - b_avg) + b_avg;
+    b.kd_norm = 1 / (1 - spec_albedo_avg(&b));
+    b.kms_norm = 1 / ((real)PI_ * (1 - b.e_avg));
+    b.ms = b.smooth || b.metallic <= 0 ? v3(0, 0, 0) : ms_color(&b);
+    b.cv = -1;
+    b.es_v = b.e_v = 0;
+    for (int i = 0; i < LOBES; i++) b.p[i] = 0;
+    return b;
+}
+
+static void bsdf_view(Bsdf *b, real cv)                           /* brdf.h:209-224 */
+{
+    if (b->cv == cv) return;
+    b->cv = cv;
+    real fd = fresnel_dielectric(cv, b->eta);
+    real fs = b->smooth ? fd : rfmin(fd, 0.9f);
+    real m = b->metallic, t = b->transmission, lb = lum(b->base);
+    b->es_v = (1 - m) * (1 - t) > 0 ? spec_albedo(b, cv) : 0;
+    b->e_v = b->smooth || m <= 0 ? 1 : albedo_e(b, cv);
+    real w[LOBES];
+    w[SPEC] = (1 - m) * fs + m * lum(fresnel_metal(b->base, cv));
+    w[DIFF] = (1 - m) * (1 - t) * (1 - b->es_v) * lb;
+    w[TRANS] = (1 - m) * t * (1 - fs) * lb;
+    w[MS] = m * lum(b->ms) * (1 - b->e_v);
+    real sum = w[0] + w[1] + w[2] + w[3];
+    for (int k = 0; k < LOBES; k++) b->p[k] = sum > 0 ? w[k] / sum : 0;
+}
+
+static V bsdf_eval(Bsdf *b, V n, V v, V l, real *pdf)             /* brdf.h:228-275 */
+{
+    V f = v3(0, 0, 0);
+    *pdf = 0;
+    real cv = vdot(v, n), cl = vdot(l, n);
+    if (cv <= 0) return f;
+    bsdf_view(b, cv);
+    const real *p = b->p;
+    const real m = b->metallic, t = b->transmission, a = b->alpha;
+    if (cl > 0) {
+        V h = vnorm(vadd(v, l));
+        real vh = clamp01(vdot(v, h)), nh = vdot(n, h);
+        real fd = fresnel_dielectric(vh, b->eta);
+        real kd = (1 - m) * (1 - t);
+        if (kd > 0) {
+            kd *= (1 - b->es_v) * (1 - spec_albedo(b, cl)) * b->kd_norm;
+            f = vscale(b->base, kd / (real)PI_);
+            *pdf += p[DIFF] * cl / (real)PI_;
+        }
+        if (!b->smooth && m > 0) {
+            real k = m * (1 - b->e_v) * (1 - albedo_e(b, cl)) * b->kms_norm;
+            f = vadd(f, vscale(b->ms, k));
+            *pdf += p[MS] * cl / (real)PI_;
+        }
+        if (!b->smooth && nh > 0) {
+            real d = ggx_d(nh, ggx_s2(n, h), a);
+            V fm = fresnel_metal(b->base, vh);
+            real s = d * ggx_vis(cv, cl, a);
+            f = vadd(f, v3(((1 - m) * fd + m * fm.x) * s, ((1 - m) * fd + m * fm.y) * s, ((1 - m) * fd + m * fm.z) * s));
+            *pdf += p[SPEC] * ggx_g1(cv, a) * d / (4 * cv);
+        }
+    } else if (cl < 0 && !b->smooth && t > 0 && m < 1) {
+        const real eta = b->eta;
+        V h = vnorm(vneg(vadd(vscale(v, eta), l)));
+        if (vdot(h, n) < 0) h = vneg(h);
+        real vh = vdot(v, h), lh = vdot(l, h), nh = vdot(n, h);
+        real den = eta * vh + lh;
+        den *= den;
+        if (vh > 0 && lh < 0 && nh > 0 && den > 1e-12f) {
+            real fd = fresnel_dielectric(vh, eta);
+            real d = ggx_d(nh, ggx_s2(n, h), a);
+            real g2 = ggx_vis(cv, cl, a) * 4 * cv * -cl;
+            real ft = vh * -lh * (1 - fd) * d * g2 / (cv * -cl * den);
+            f = vscale(b->base, (1 - m) * t * ft);
+            *pdf = p[TRANS] * ggx_g1(cv, a) * vh * d / cv * -lh / den;
+        }
+    }
+    return f;
+}
+
+static int bsdf_pick(const Bsdf *b, real ul)                      /* brdf.h:280-285 */
+{
+    const real *p = b->p;
+    int lobe = 0;
+    for (real acc = p[0]; lobe < LOBES - 1 && (ul >= acc || p[lobe] <= 0); acc += p[++lobe]);
+    return p[lobe] > 0 ? lobe : -1;
+}
+static int bsdf_lobe_delta(const Bsdf *b, int lobe) { return b->smooth && (lobe == SPEC || lobe == TRANS); }
+static int bsdf_nlobes(const Bsdf *b) { return (b->p[0] > 0) + (b->p[1] > 0) + (b->p[2] > 0) + (b->p[3] > 0); }
+
+static int bsdf_sample_lobe(Bsdf *b, int lobe, V n, V v, real u1, real u2, BSample *s)   /* brdf.h:292-340 */
+{
+    real cv = vdot(v, n);
+    if (cv <= 0 || lobe < 0) return 0;
+    bsdf_view(b, cv);
+    const real *p = b->p;
+    if (p[lobe] <= 0) return 0;
+    const real m = b->metallic, t = b->transmission;
+    s->delta = 0;
+    s->through = 0;
+    if (b->smooth && (lobe == SPEC || lobe == TRANS)) {
+        real fd = fresnel_dielectric(cv, b->eta);
+        if (lobe == SPEC) {
+            V fm = fresnel_metal(b->base, cv);
+            real k = 1 / p[lobe];
+            s->l = vreflect(vneg(v), n);
+            s->w = v3(((1 - m) * fd + m * fm.x) * k, ((1 - m) * fd + m * fm.y) * k, ((1 - m) * fd + m * fm.z) * k);
+        } else {
+            if (fd >= 1) return 0;
+            s->l = vrefract(vneg(v), n, b->eta);
+            s->w = vscale(b->base, (1 - m) * t * (1 - fd) / p[lobe]);
+            s->through = 1;
+        }
+        s->pdf = 0;
+        s->delta = 1;
+        return 1;
+    }
+    V l;
+    if (lobe == DIFF || lobe == MS) l = cosine_hemisphere(n, u1, u2);
+    else {
+        V h = ggx_sample_vndf(n, v, b->alpha, u1, u2);
+        real vh = vdot(v, h);
+        if (vh <= 0) return 0;
+        if (lobe == SPEC) l = vreflect(vneg(v), h);
+        else {
+            if (b->eta * b->eta * (1 - vh * vh) >= 1) return 0;
+            l = vrefract(vneg(v), h, b->eta);
+        }
+    }
+    if ((lobe == TRANS) != (vdot(l, n) < 0)) return 0;
+    real pdf;
+    V f = bsdf_eval(b, n, v, l, &pdf);
+    if (!(pdf > 0)) return 0;
+    s->l = l;
+    s->w = vscale(f, fabsf(vdot(l, n)) / pdf);
+    s->pdf = pdf;
+    s->through = vdot(l, n) < 0;
+    return 1;
+}
+
+void lro_bsdf_case(const LroMat *m, int outside, const float *tab, const float n[3], const float v[3], const float l[3],
+                   float ul, float u1, float u2, float out[16])
+{
+    Bsdf b = bsdf_make(m, outside, tab);
+    V N = vload(n), Vv = vload(v), L = vload(l);
+    real pdf;
+    V f = bsdf_eval(&b, N, Vv, L, &pdf);
+    out[0] = f.x; out[1] = f.y; out[2] = f.z; out[3] = pdf;
+    BSample s;
+    memset(&s, 0, sizeof(s));
+    bsdf_view(&b, vdot(Vv, N));
+    int lobe = bsdf_pick(&b, ul);
+    int ok = bsdf_sample_lobe(&b, lobe, N, Vv, u1, u2, &s);
+    out[4] = (float)ok; out[5] = (float)lobe;
+    out[6] = s.l.x; out[7] = s.l.y; out[8] = s.l.z;
+    out[9] = s.w.x; out[10] = s.w.y; out[11] = s.w.z;
+    out[12] = s.pdf; out[13] = (float)s.delta; out[14] = (float)s.through; out[15] = 0;
+}
+
+/* ---- the scene: shapes and their hits (intersect.h:66-112, intersect.c:4-67) ---- */
+typedef struct { real t; V p, n, ng; int front, prim; } Hit;
+
+static int hit_sphere(const LroPrim *s, V o, V d, real tmin, real tmax, real *t_out)
+{
+    V oc = vsub(vload(s->c), o);
+    real a = vdot(d, d), hh = vdot(d, oc), c = vdot(oc, oc) - s->c[3] * s->c[3];
+    real disc = hh * hh - a * c;
+    if (disc < 0) return 0;
+    real sq = sqrtf(disc);
+    real t = (hh - sq) / a;
+    if (t <= tmin || t >= tmax) {
+        t = (hh + sq) / a;
+        if (t <= tmin || t >= tmax) return 0;
+    }
+    *t_out = t;
+    return 1;
+}
+static int hit_quad(const LroPrim *s, V o, V d, real tmin, real tmax, real *t_out)
+{
+    V nn = vload(s->n);
+    real denom = vdot(nn, d);
+    if (fabsf(denom) < 1e-12f) return 0;
+    real t = (s->n[3] - vdot(nn, o)) / denom;
+    if (t <= tmin || t >= tmax) return 0;
+    V p = vadd(o, vscale(d, t));
+    V rel = vsub(p, vload(s->q));
+    real a = vdot(vload(s->w), vcross(rel, vload(s->v)));
+    real b = vdot(vload(s->w), vcross(vload(s->u), rel));
+    if (a < 0 || a > 1 || b < 0 || b > 1) return 0;
+    *t_out = t;
+    return 1;
+}
+static int hit_tri(const float *tr, V o, V d, real tmin, real tmax, real *t_out)
+{
+    V q = vload(tr), u = vload(tr + 4), v = vload(tr + 8);
+    V pvec = vcross(d, v);
+    real det = vdot(u, pvec);
+    if (fabsf(det) < 1e-12f) return 0;
+    real inv = 1.0f / det;
+    V tvec = vsub(o, q);
+    real b1 = vdot(tvec, pvec) * inv;
+    if (b1 < 0 || b1 > 1) return 0;
+    V qvec = vcross(tvec, u);
+    real b2 = vdot(d, qvec) * inv;
+    if (b2 < 0 || b1 + b2 > 1) return 0;
+    real t = vdot(v, qvec) * inv;
+    if (t <= tmin || t >= tmax) return 0;
+    *t_out = t;
+    return 1;
+}
+static int shape_test(const LroScene *s, int id, V o, V d, real tmin, real tmax, real *t)
+{
+    if (id < s->nprims) {
+        const LroPrim *p = &s->prims[id];
+        return p->q[3] < 0.5f ? hit_sphere(p, o, d, tmin, tmax, t) : hit_quad(p, o, d, tmin, tmax, t);
+    }
+    return hit_tri(s->tris + 12 * (size_t)(id - s->nprims), o, d, tmin, tmax, t);
+}
+/* every shape in id order, the closest kept (intersect_world's brute force, accel.c:264-277) */
+static int scene_intersect(const LroScene *s, V o, V d, real tmin, real tmax, Hit *h)
+{
+    int found = -1;
+    int n = s->nprims + s->ntris;
+    for (int i = 0; i < n; i++) {
+        real t;
+        if (shape_test(s, i, o, d, tmin, tmax, &t)) { found = i; tmax = t; }
+    }
+    if (found < 0) return 0;
+    real t = tmax;
+    V outward;
+    if (found < s->nprims) {
+        const LroPrim *p = &s->prims[found];
+        if (p->q[3] < 0.5f) outward = vscale(vsub(vadd(o, vscale(d, t)), vload(p->c)), 1.0f / p->c[3]);
+        else outward = vload(p->n);
+    } else {
+        const float *tr = s->tris + 12 * (size_t)(found - s->nprims);
+        outward = vnorm(vcross(vload(tr + 4), vload(tr + 8)));
+    }
+    h->t = t;
+    h->p = vadd(o, vscale(d, t));
+    h->front = vdot(d, outward) < 0;
+    h->n = h->front ? outward : vneg(outward);
+    h->ng = h->n;
+    h->prim = found;
+    return 1;
+}
+static int scene_occluded(const LroScene *s, V o, V d, real tmin, real tmax)
+{
+    int n = s->nprims + s->ntris;
+    for (int i = 0; i < n; i++) {
+        real t;
+        if (shape_test(s, i, o, d, tmin, tmax, &t)) return 1;
+    }
+    return 0;
+}
+static int shape_material(const LroScene *s, int id)
+{
+    if (id < s->nprims) return (int)s->prims[id].u[3];
+    uint32_t w;
+    memcpy(&w, s->tris + 12 * (size_t)(id - s->nprims) + 11, 4);
+    return (int)(w >> 26);
+}
+
+/* ---- next event estimation to the quad lamp (tracer.c:30-81) ---- */
+static inline real power_heuristic(real pa, real pb)
+{
+    real a = pa * pa, b = pb * pb;
+    return a + b > 0 ? a / (a + b) : 0;
+}
+static inline real light_pdf(const LroPrim *light, real dist2, real cos_light) { return dist2 / (cos_light * light->v[3]); }
+
+static V sample_direct_light(const LroScene *s, const Hit *h, Bsdf *b, V v, int mis, real ua, real ub)
+{
+    const LroPrim *light = &s->prims[s->light];
+    V le = vload(s->mats[(int)light->u[3]].emission);
+    V xl = vadd(vload(light->q), vadd(vscale(vload(light->u), ua), vscale(vload(light->v), ub)));
+    V to_light = vsub(xl, h->p);
+    real dist2 = vdot(to_light, to_light);
+    real dist = sqrtf(dist2);
+    V wi = vscale(to_light, 1.0f / dist);
+    real cos_surf = vdot(h->n, wi);
+    real cos_light = -vdot(vload(light->n), wi);
+    if (cos_light <= 0) return v3(0, 0, 0);
+    real pdf;
+    V f = bsdf_eval(b, h->n, v, wi, &pdf);
+    if (!(vmax(f) > 0)) return v3(0, 0, 0);
+    if (scene_occluded(s, h->p, wi, s->eps, dist - s->margin)) return v3(0, 0, 0);
+    real g = fabsf(cos_surf) * cos_light * light->v[3] / dist2;
+    if (mis) g *= power_heuristic(light_pdf(light, dist2, cos_light), pdf);
+    return vscale(vmul(f, le), g);
+}
+
+static int rough_part(const Bsdf *b) { return !b->smooth || (1 - b->metallic) * (1 - b->transmission) > 0; }
+
+enum { EMIT_FULL, EMIT_NONE, EMIT_MIS };
+
+/* trace_path_sampled (tracer.c:364-509) without media, sky, glowing shapes or textures */
+static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis)
+{
+    V radiance = v3(0, 0, 0), throughput = v3(1, 1, 1);
+    int emit = EMIT_FULL;
+    real bounce_pdf = 0;
+    if (mis) d = vnorm(d);
+    for (int depth = 0;; depth++) {
+        Hit h;
+        if (!scene_intersect(s, o, d, s->eps, 1e30f, &h)) break;
+        const LroMat *m = &s->mats[shape_material(s, h.prim)];
+        if (m->type == LRO_MAT_LIGHT) {
+            if (h.front) {
+                if (emit == EMIT_FULL) radiance = vadd(radiance, vmul(throughput, vload(m->emission)));
+                else if (emit == EMIT_MIS && h.prim == s->light) {
+                    const LroPrim *l = &s->prims[s->light];
+                    real cos_light = -vdot(vload(l->n), d);
+                    if (cos_light > 0) {
+                        real w = power_heuristic(bounce_pdf, light_pdf(l, h.t * h.t, cos_light));
+                        radiance = vadd(radiance, vmul(throughput, vscale(vload(m->emission), w)));
+                    }
+                }
+            }
+            break;
+        }
+        if (depth >= max_depth) break;
+        V v = vneg(vnorm(d));
+        Bsdf b = bsdf_make(m, h.front, s->tab);
+        if (rough_part(&b) && s->light >= 0) {
+            real la, lb;
+            sampler_2d(sm, LRO_SAMPLE_LIGHT(depth), &la, &lb);
+            radiance = vadd(radiance, vmul(throughput, sample_direct_light(s, &h, &b, v, mis, lb, la)));
+        }
+        bsdf_view(&b, vdot(v, h.n));
+        real ur = 0, ul = 0, da = 0, db = 0;
+        if (depth >= 3 || bsdf_nlobes(&b) > 1) sampler_2d(sm, LRO_SAMPLE_MISC(depth), &ur, &ul);
+        int lobe = bsdf_pick(&b, ul);
+        if (lobe >= 0 && !bsdf_lobe_delta(&b, lobe)) sampler_2d(sm, LRO_SAMPLE_DIR(depth), &da, &db);
+        BSample bs;
+        if (!bsdf_sample_lobe(&b, lobe, h.n, v, da, db, &bs)) break;
+        throughput = vmul(throughput, bs.w);
+        if (bs.delta) emit = EMIT_FULL;
+        else {
+            bounce_pdf = bs.pdf;
+            emit = mis ? EMIT_MIS : EMIT_NONE;
+        }
+        if (depth >= 3) {                                    /* russian_roulette_u (tracer.c:295-301) */
+            real pr = rfmin(vmax(throughput), 0.95f);
+            if (ur >= pr) break;
+            throughput = vscale(throughput, 1.0f / pr);
+        }
+        o = h.p;
+        d = mis ? vnorm(bs.l) : bs.l;
+    }
+    return radiance;
+}
+
+/* camera_ray (camera.h:38-42) */
+static void camera_ray(const float *cam, real s, real t, V *o, V *d)
+{
+    V target = vsub(vadd(vload(cam + 3), vscale(vload(cam + 6), s)), vscale(vload(cam + 9), t));
+    *o = vload(cam);
+    *d = vnorm(vsub(target, *o));
+}
+
+void lro_pt_seed(LroRng *rng, uint64_t pixel, uint32_t k, uint64_t seed)
+{
+    lro_rng_seed(rng, 0x853c49e6748fea9bULL ^ (pixel * 0x9E3779B97F4A7C15ULL) ^ (seed * 0xD1B54A32D192ED03ULL) ^
+                          ((uint64_t)k * 0xA0761D6478BD642FULL), pixel);
+}
+
+/* trace_image (tracer.c:329-356): the sum of samples s0 .. s0+ns-1 of every pixel into sum
+ * (rgb, float, the samples added in order; NaNs skipped, clamp applied) */
+void lro_pt_image(const LroScene *s, const float cam[12], int W, int H, uint32_t s0, int ns, uint64_t seed, float clamp,
+                  int max_depth, int mis, float *sum)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            uint64_t pixel = (uint64_t)y * W + x;
+            V acc = v3(0, 0, 0);
+            for (int q = 0; q < ns; q++) {
+                uint32_t k = s0 + (uint32_t)q;
+                LroRng rng;
+                lro_pt_seed(&rng, pixel, k, seed);
+                Smp sm = {&rng, k, (uint32_t)seed * 0x9e3779b9u, x, y, 1, s->mask};
+                real jx, jy;
+                sampler_2d(&sm, 0, &jx, &jy);
+                real u = (x + jx) / W, v = (y + jy) / H;
+                V o, d;
+                camera_ray(cam, u, v, &o, &d);
+                V c = trace_path(s, o, d, &sm, max_depth, mis);
+                if (!(c.x == c.x && c.y == c.y && c.z == c.z)) continue;
+                if (clamp > 0 && vmax(c) > clamp) c = vscale(c, clamp / vmax(c));
+                acc = vadd(acc, c);
+            }
+            float *o3 = sum + 3 * pixel;
+            o3[0] += acc.x; o3[1] += acc.y; o3[2] += acc.z;
+        }
+}

@@ -1,0 +1,63 @@
+/* lro_pt.h - the oracle's path tracer (lro_pt.c). TEST INFRASTRUCTURE ONLY (see lro.h). */
+#ifndef LRO_PT_H
+#define LRO_PT_H
+
+#include <stdint.h>
+
+#include "lro.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LRO_SAMPLER_PAIRS 13                 /* sampler.h:29-33 */
+#define LRO_SAMPLE_LIGHT(d) (1 + 3 * (d))
+#define LRO_SAMPLE_DIR(d) (2 + 3 * (d))
+#define LRO_SAMPLE_MISC(d) (3 + 3 * (d))
+#define LRO_BRDF_N 32
+
+enum { LRO_MAT_DIFFUSE, LRO_MAT_MIRROR, LRO_MAT_GLASS, LRO_MAT_LIGHT, LRO_MAT_PBR };   /* scene.h:23-29 */
+
+/* a material (scene.h:31-45, without textures) */
+typedef struct {
+    int type;
+    float albedo[3], emission[3];
+    float ior, metallic, roughness, transmission;
+} LroMat;
+
+/* a primitive in the GPU's 96-byte record (trace.wgsl:3-10): c = centre, radius; q = corner,
+ * w 0 sphere / 1 quad; u = edge, w material; v = edge, w area; n = unit normal, w plane_d;
+ * w = (u x v) / |u x v|^2 */
+typedef struct {
+    float c[4], q[4], u[4], v[4], n[4], w[4];
+} LroPrim;
+
+typedef struct {
+    const LroPrim *prims; int nprims;
+    const float *tris; int ntris;        /* 12 floats each, material in the top 6 bits of word 11 */
+    const LroMat *mats; int nmats;
+    int light;                            /* the lamp: a quad prim, or -1 */
+    float eps, margin;                    /* ray offset and shadow margin (tracer.c:15-23) */
+    const uint16_t *mask;                 /* the blue-noise ranks (64 x 64) */
+    const float *tab;                     /* the BSDF tables: E[32][32], B[32][32], E_avg[32], B_avg[32] */
+} LroScene;
+
+uint32_t lro_sampler_hash(uint32_t x);
+uint32_t lro_sampler_owen(uint32_t x, uint32_t seed);
+uint32_t lro_sampler_sobol1(uint32_t i);
+void lro_blue_noise(uint16_t *rank);
+void lro_sampler_2d(const uint16_t *mask, int x, int y, uint32_t index, uint32_t seed, int dim, float out[2]);
+/* one BSDF case: eval (f.rgb, pdf) then pick(ul) + sample_lobe(u1, u2): ok, lobe, l.xyz, w.rgb,
+ * pdf, delta, through */
+void lro_bsdf_case(const LroMat *m, int outside, const float *tab, const float n[3], const float v[3], const float l[3],
+                   float ul, float u1, float u2, float out[16]);
+/* the per-(pixel, sample) PCG32 stream */
+void lro_pt_seed(LroRng *rng, uint64_t pixel, uint32_t k, uint64_t seed);
+/* adds samples s0 .. s0+ns-1 of every pixel (rgb sums) into sum */
+void lro_pt_image(const LroScene *s, const float cam[12], int W, int H, uint32_t s0, int ns, uint64_t seed, float clamp,
+                  int max_depth, int mis, float *sum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

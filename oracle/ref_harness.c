@@ -142,3 +142,92 @@ void refh_trace_tris(const float *tris, int n, int wide, const float *rays, int 
     refh_tris_trace(h, rays, nrays, any, hits);
     refh_tris_free(h);
 }
+
+/* ---- the path tracer's pieces ---- */
+#include "scene/brdf.h"
+#include "rt/accel.h"
+#include "rt/tracer.h"
+
+/* the BSDF tables the reference was built with (brdf_tables.h): E, B, E_avg, B_avg */
+void refh_brdf_tables(float *out)
+{
+    memcpy(out, BRDF_E, sizeof(BRDF_E));
+    memcpy(out + 1024, BRDF_B, sizeof(BRDF_B));
+    memcpy(out + 2048, BRDF_E_AVG, sizeof(BRDF_E_AVG));
+    memcpy(out + 2080, BRDF_B_AVG, sizeof(BRDF_B_AVG));
+}
+
+/* one BSDF case through brdf.h (the layout of lro_bsdf_case): material m = type, albedo.rgb,
+ * ior, metallic, roughness, transmission (9 floats) */
+void refh_bsdf_case(const float *m, int outside, const float *n, const float *v, const float *l, float ul, float u1,
+                    float u2, double *out)
+{
+    Material mat;
+    memset(&mat, 0, sizeof(mat));
+    mat.type = (MatType)(int)m[0];
+    mat.albedo = vec3(m[1], m[2], m[3]);
+    mat.ior = m[4]; mat.metallic = m[5]; mat.roughness = m[6]; mat.transmission = m[7];
+    Bsdf b = bsdf_make(&mat, outside);
+    Vec3 N = vec3(n[0], n[1], n[2]), V = vec3(v[0], v[1], v[2]), L = vec3(l[0], l[1], l[2]);
+    real pdf;
+    Vec3 f = bsdf_eval(&b, N, V, L, &pdf);
+    out[0] = f.x; out[1] = f.y; out[2] = f.z; out[3] = pdf;
+    BsdfSample s;
+    memset(&s, 0, sizeof(s));
+    bsdf_view(&b, v_dot(V, N));
+    int lobe = bsdf_pick(&b, ul);
+    int ok = bsdf_sample_lobe(&b, lobe, N, V, u1, u2, &s);
+    out[4] = ok; out[5] = lobe;
+    out[6] = s.l.x; out[7] = s.l.y; out[8] = s.l.z;
+    out[9] = s.w.x; out[10] = s.w.y; out[11] = s.w.z;
+    out[12] = s.pdf; out[13] = s.delta; out[14] = s.through; out[15] = 0;
+}
+
+/* trace_image (tracer.c:329-356) of a scene of triangles (12 floats each, material in the top
+ * 6 bits of word 11) and one quad lamp (q, u, v: 9 floats, material lamp_mat), materials as
+ * refh_bsdf_case's 9 floats + emission.rgb (12 floats), built through the reference's own
+ * scene API (scene_add_triangle, make_quad, scene_set_accel), camera 12 floats (origin,
+ * upper_left, horizontal, vertical); the mean of spp samples per pixel into px (rgb). */
+void refh_render(const float *tris, int ntris, const float *lamp, int lamp_mat, const float *mats, int nmats,
+                 const float *cam, int W, int H, int spp, uint64_t seed, int max_depth, int mis, double *px)
+{
+    Scene s;
+    memset(&s, 0, sizeof(s));
+    for (int i = 0; i < nmats; i++) {
+        const float *m = mats + 12 * (size_t)i;
+        Material mat;
+        memset(&mat, 0, sizeof(mat));
+        mat.type = (MatType)(int)m[0];
+        mat.albedo = vec3(m[1], m[2], m[3]);
+        mat.ior = m[4]; mat.metallic = m[5]; mat.roughness = m[6]; mat.transmission = m[7];
+        mat.emission = vec3(m[8], m[9], m[10]);
+        scene_add_material(&s, mat);
+    }
+    s.light = -1;
+    if (lamp)
+        s.light = scene_add_prim(&s, make_quad(vec3(lamp[0], lamp[1], lamp[2]), vec3(lamp[3], lamp[4], lamp[5]),
+                                               vec3(lamp[6], lamp[7], lamp[8]), lamp_mat));
+    for (int i = 0; i < ntris; i++) {
+        const float *t = tris + 12 * (size_t)i;
+        uint32_t w;
+        memcpy(&w, t + 11, 4);
+        Vec3 a = vec3(t[0], t[1], t[2]);
+        Vec3 b = v_add(a, vec3(t[4], t[5], t[6])), c = v_add(a, vec3(t[8], t[9], t[10]));
+        scene_add_triangle(&s, a, b, c, NULL, (int)(w >> 26));
+    }
+    AccelParams ap = accel_default_params();
+    scene_set_accel(&s, ACCEL_BVH, &ap);
+    Camera c;
+    c.origin = vec3(cam[0], cam[1], cam[2]);
+    c.upper_left = vec3(cam[3], cam[4], cam[5]);
+    c.horizontal = vec3(cam[6], cam[7], cam[8]);
+    c.vertical = vec3(cam[9], cam[10], cam[11]);
+    TraceParams tp = trace_default_params();
+    tp.max_depth = max_depth;
+    tp.mis = mis;
+    Vec3 *out = calloc((size_t)W * H, sizeof(Vec3));
+    trace_image(&s, &c, &tp, W, H, spp, seed, 0, out, NULL, NULL);
+    for (int i = 0; i < W * H; i++) { px[3 * i] = out[i].x; px[3 * i + 1] = out[i].y; px[3 * i + 2] = out[i].z; }
+    free(out);
+    scene_free(&s);
+}
