@@ -16,6 +16,7 @@
 #ifndef LR_HIP_H
 #define LR_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -142,9 +143,125 @@ void lrh_pt_scene_free(LrhPtScene *s);
  * do: camera cam (lrs_camera_make's 12 floats), NEE to the lamp with MIS (mis = 1) or without,
  * max_depth bounces, samples brighter than clamp scaled down (0: none), the image's scramble
  * from seed. A sample's random numbers depend only on (seed, pixel, sample index), so any split
- * of the samples over calls gives the same sums. Asynchronous on the context's stream. */
+ * of the samples over calls gives the same sums. Returns when d_accum holds the result (the
+ * samples run on the context's engine, a one-job batch). */
 int lrh_pt_render(LrhContext *c, const LrhPtScene *s, const float *cam, int W, int H, int spp, uint32_t s0, uint64_t seed,
                   int max_depth, int mis, float clamp, float *d_accum);
+
+/* ---- the engine: many render jobs on one device ----
+ * The role of the reference's GpuView trace loop (gpu_view_trace_start/more/passes/read,
+ * engine/gpu/view_gpu.h:71-124, and gpu_frame_trace, engine/app/gpu_frame.h:14-26) for any
+ * number of jobs at once. An engine holds a pool of 2^slots_log2 path slots and an image arena;
+ * jobs (each a resident scene, a camera, a resolution, samples, a seed) are admitted while their
+ * images fit, their paths fill the free slots as others end, and one hipGraph of several
+ * wavefront iterations is replayed until the work is done. A job's image depends only on the
+ * job: it is the same to the bit alone, among any others, and in the plain per-job form. */
+#define LRH_EAGAIN (-5)     /* not now: no room for the job until others finish (or: timed out) */
+
+typedef struct LrhEngine LrhEngine;
+
+/* one render job: as gpu_frame_trace's arguments (gpu_frame.h:14-26) */
+typedef struct {
+    const LrhPtScene *scene;
+    float cam[12];          /* lrs_camera_make's 12 floats */
+    int w, h, spp;
+    uint32_t s0;            /* the first sample index */
+    uint64_t seed;          /* the image's scramble */
+    int max_depth, mis;
+    float clamp;            /* 0: none */
+} LrhJob;
+
+/* The engine's launch shape (what lrh_engine_autotune picks): work-group sizes of the walk
+ * kernels (extend, shadow) and of the others (regen, shade), each 64, 128 or 256; the walk's
+ * LDS stack (16, 24 or 32 entries; deeper entries spill to device memory); persistent
+ * work-groups per compute unit (1..16); the queue tile a work-group claims at a time, in
+ * work-group widths (1, 2, 4 or 8); wavefront iterations per captured graph (1..64). */
+typedef struct {
+    int block_trace, stack, block_shade, blocks_per_cu, tile, iters_per_graph;
+} LrhTune;
+
+#define LRH_JOB_RUNNING 1
+#define LRH_JOB_DONE 2
+
+typedef struct {
+    int state;              /* LRH_JOB_RUNNING or LRH_JOB_DONE */
+    int w, h, spp;
+    uint64_t paths;         /* w * h * spp */
+    double t_submit_ms;     /* when it was admitted (lrh_now_ms's clock) */
+    double t_done_ms;       /* when its last sample ended, from the device's clock (0 while running) */
+} LrhJobInfo;
+
+typedef struct {
+    uint64_t rays_ext;      /* closest-hit rays traced (every queue entry walked) */
+    uint64_t rays_shadow;   /* any-hit shadow rays traced */
+    uint64_t iterations;    /* wavefront iterations run */
+    uint64_t jobs_done;
+    uint64_t node_fetches;  /* with counting on: node records read by the extension walks */
+    uint64_t shape_tests;   /* and shapes tested */
+} LrhEngineStats;
+
+/* host clock (steady, ms) that job times are given in */
+double lrh_now_ms(void);
+
+/* slots_log2 in 10..22; max_tree_depth: the deepest tree any job's scene may have (its
+ * walk stack); image_bytes: the arena for job images (24 bytes a pixel). Own stream. */
+int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t image_bytes, LrhEngine **out);
+void lrh_engine_destroy(LrhEngine *e);
+int lrh_engine_tune_set(LrhEngine *e, const LrhTune *t);
+int lrh_engine_tune_get(const LrhEngine *e, LrhTune *t);
+/* counting on: the extension walks also count node records and shapes (a slower kernel) */
+int lrh_engine_counting(LrhEngine *e, int on);
+/* Admits a job (its image cleared, its paths queued after those already queued): LRH_EAGAIN
+ * if its image or a job slot cannot be had until others are released. *id names it. */
+int lrh_engine_submit(LrhEngine *e, const LrhJob *job, int64_t *id);
+/* Runs up to `graphs` captured graphs of the engine's iterations (fewer if the work ends) and
+ * takes in the progress they report. Returns the number of jobs still running (>= 0). */
+int lrh_engine_pump(LrhEngine *e, int graphs);
+/* pump until every admitted job is done */
+int lrh_engine_drain(LrhEngine *e);
+/* The reference's shape (gpu_frame_trace over one view, engine/app/gpu_frame.c:49-83): the
+ * job alone, one pass of one sample per pixel after another, each pass a grid over its paths,
+ * the host waiting for the job's end. The engine must be idle. */
+int lrh_engine_plain(LrhEngine *e, const LrhJob *job, int64_t *id);
+int lrh_engine_job(LrhEngine *e, int64_t id, LrhJobInfo *info);
+/* the job's image into d_out: float4 per pixel (the sum of its samples' rgb, their count),
+ * stored (add = 0) or added (add = 1); returns when d_out holds it */
+int lrh_engine_image(LrhEngine *e, int64_t id, float *d_out, int add);
+/* forgets a finished job: its image memory and slot go back to the engine */
+int lrh_engine_release(LrhEngine *e, int64_t id);
+int lrh_engine_stats(LrhEngine *e, LrhEngineStats *st);
+/* Times the sample jobs (their spp as given: keep it small) under candidate launch shapes,
+ * one choice at a time (walk work-group size x stack, then work-groups per CU, then the tile,
+ * then the shading work-group size), keeps the fastest in rays per second and sets it. */
+int lrh_engine_autotune(LrhEngine *e, const LrhJob *sample, int n, LrhTune *best, double *mrays);
+
+/* ---- serving: a request loop over an engine ----
+ * Scenes registered with the server stay resident; their bytes and the images of admitted jobs
+ * count against a budget of device memory. Requests (thread-safe, from any host thread) wait
+ * in order until their image fits, then join the running batch at once: their paths take free
+ * slots as others end (continuous batching). A worker thread drives the engine. */
+typedef struct LrhServer LrhServer;
+
+typedef struct {
+    int64_t submitted, admitted, done, waiting;
+    uint64_t rays;              /* rays traced (closest-hit + shadow) */
+    double busy_ms;             /* time with jobs on the device */
+    double p50_ms, p99_ms;      /* latency (submit -> last sample) of the jobs done */
+    size_t budget, in_use;      /* bytes: the budget; scenes + images admitted now */
+} LrhServerStats;
+
+int lrh_server_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t budget_bytes, const LrhTune *tune,
+                      LrhServer **out);
+/* a resident scene (LRH_EREFUSED if it does not fit the budget) */
+int lrh_server_add_scene(LrhServer *s, const LrhPtScene *scene);
+int lrh_server_submit(LrhServer *s, const LrhJob *job, int64_t *id);
+/* waits up to timeout_ms (< 0: no limit) for the job: 0 done, LRH_EAGAIN not yet */
+int lrh_server_wait(LrhServer *s, int64_t id, double timeout_ms, LrhJobInfo *info);
+/* a finished job's image into host memory h_out (float4 per pixel: sum, count) */
+int lrh_server_image(LrhServer *s, int64_t id, float *h_out);
+int lrh_server_release(LrhServer *s, int64_t id);
+int lrh_server_stats(LrhServer *s, LrhServerStats *st);
+void lrh_server_destroy(LrhServer *s);
 
 #ifdef __cplusplus
 }

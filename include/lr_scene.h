@@ -42,6 +42,24 @@ void lrs_make_quad(const float q[3], const float u[3], const float v[3], int mat
  * default view ((278, 278, -800) toward (278, 278, 0), vfov 40, scaled) for a w x h image. */
 void lrs_cornell(uint64_t seed, int w, int h, float *tris, float *prims, LrhMaterial *mats, float cam[12]);
 
+/* ---- the batched (many jobs in one launch) and serving workloads ---- */
+#define LRS_BATCH_SOUP_MATS 14
+
+/* A scene of the batched workload: the soup of lrs_gen_soup (n triangles, seed) over 14 GGX
+ * materials drawn from PCG32 (seed, stream 3): base colour uniform in [0.2, 0.9] per channel,
+ * roughness uniform in [0.05, 1], metallic with probability 0.4, and two of them (every 7th)
+ * clear glass of IOR 1.5; a floor quad (2 triangles, material 14, diffuse 0.5) at y = -0.05 over
+ * [-2, 3]^2; a 0.5 x 0.5 quad lamp of radiance 15 (material 15) at y = 1.6 over the cube's
+ * centre, facing down. tris: (n + 2) x 12 floats; prims: 1 x 24 (the lamp, prim 0); mats: 16. */
+void lrs_batch_scene(int n, uint64_t seed, float *tris, float *prims, LrhMaterial *mats);
+/* a job's camera: from a direction drawn from PCG32 (seed, stream 4) with y >= 0.15, 2.2 from
+ * the soup's centre (0.5, 0.5, 0.5), looking at it; vfov 40, aspect w / h */
+void lrs_batch_camera(uint64_t seed, int w, int h, float cam[12]);
+/* The job mix, from PCG32 (seed, stream 5): per job a triangle count log-uniform in [1000,
+ * 500000], a resolution uniform over 256^2, 512^2, 1280x720, 1920x1080, and spp uniform over
+ * {1, 4, 16, 64} (spp_lo = 4: over {4, 16, 64}, the serving mix). */
+void lrs_batch_plan(int njobs, uint64_t seed, int spp_lo, int *ntris, int *w, int *h, int *spp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,43 @@ class BuildStats(ctypes.Structure):
     _fields_ = [("depth", ctypes.c_int), ("rounds", ctypes.c_int), ("ms", ctypes.c_float)]
 
 
+class Tune(ctypes.Structure):
+    """LrhTune: the engine's launch shape"""
+    _fields_ = [("block_trace", ctypes.c_int), ("stack", ctypes.c_int), ("block_shade", ctypes.c_int),
+                ("blocks_per_cu", ctypes.c_int), ("tile", ctypes.c_int), ("iters_per_graph", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Job(ctypes.Structure):
+    """LrhJob: one render job (gpu_frame_trace's arguments)"""
+    _fields_ = [("scene", ctypes.c_void_p), ("cam", ctypes.c_float * 12), ("w", ctypes.c_int), ("h", ctypes.c_int),
+                ("spp", ctypes.c_int), ("s0", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("max_depth", ctypes.c_int),
+                ("mis", ctypes.c_int), ("clamp", ctypes.c_float)]
+
+
+class JobInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("w", ctypes.c_int), ("h", ctypes.c_int), ("spp", ctypes.c_int),
+                ("paths", ctypes.c_uint64), ("t_submit_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double)]
+
+
+class EngineStats(ctypes.Structure):
+    _fields_ = [("rays_ext", ctypes.c_uint64), ("rays_shadow", ctypes.c_uint64), ("iterations", ctypes.c_uint64),
+                ("jobs_done", ctypes.c_uint64), ("node_fetches", ctypes.c_uint64), ("shape_tests", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("admitted", ctypes.c_int64), ("done", ctypes.c_int64),
+                ("waiting", ctypes.c_int64), ("rays", ctypes.c_uint64), ("busy_ms", ctypes.c_double),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("budget", ctypes.c_size_t),
+                ("in_use", ctypes.c_size_t)]
+
+
+JOB_RUNNING, JOB_DONE = 1, 2
+EAGAIN = -5
+
+
 def lib():
     """The loaded library; raises if it was not built (no fallback of any kind)."""
     global _lib
@@ -67,6 +104,32 @@ def lib():
         "lrh_pt_scene": (I, [P, P, I, P, I, P, I, I, F, F, I, ctypes.POINTER(P)]),
         "lrh_pt_scene_free": (None, [P]),
         "lrh_pt_render": (I, [P, P, P, I, I, I, U, ctypes.c_uint64, I, I, F, P]),
+        "lrs_batch_scene": (None, [I, ctypes.c_uint64, P, P, P]),
+        "lrs_batch_camera": (None, [ctypes.c_uint64, I, I, P]),
+        "lrs_batch_plan": (None, [I, ctypes.c_uint64, I, P, P, P, P]),
+        "lrh_now_ms": (ctypes.c_double, []),
+        "lrh_engine_create": (I, [P, I, I, ctypes.c_size_t, ctypes.POINTER(P)]),
+        "lrh_engine_destroy": (None, [P]),
+        "lrh_engine_tune_set": (I, [P, ctypes.POINTER(Tune)]),
+        "lrh_engine_tune_get": (I, [P, ctypes.POINTER(Tune)]),
+        "lrh_engine_counting": (I, [P, I]),
+        "lrh_engine_submit": (I, [P, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_int64)]),
+        "lrh_engine_pump": (I, [P, I]),
+        "lrh_engine_drain": (I, [P]),
+        "lrh_engine_plain": (I, [P, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_int64)]),
+        "lrh_engine_job": (I, [P, ctypes.c_int64, ctypes.POINTER(JobInfo)]),
+        "lrh_engine_image": (I, [P, ctypes.c_int64, P, I]),
+        "lrh_engine_release": (I, [P, ctypes.c_int64]),
+        "lrh_engine_stats": (I, [P, ctypes.POINTER(EngineStats)]),
+        "lrh_engine_autotune": (I, [P, ctypes.POINTER(Job), I, ctypes.POINTER(Tune), ctypes.POINTER(ctypes.c_double)]),
+        "lrh_server_create": (I, [P, I, I, ctypes.c_size_t, ctypes.POINTER(Tune), ctypes.POINTER(P)]),
+        "lrh_server_add_scene": (I, [P, P]),
+        "lrh_server_submit": (I, [P, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_int64)]),
+        "lrh_server_wait": (I, [P, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(JobInfo)]),
+        "lrh_server_image": (I, [P, ctypes.c_int64, P]),
+        "lrh_server_release": (I, [P, ctypes.c_int64]),
+        "lrh_server_stats": (I, [P, ctypes.POINTER(ServerStats)]),
+        "lrh_server_destroy": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -236,3 +299,185 @@ class Context:
         lt = np.ascontiguousarray(light, dtype=np.float32)
         _check(lib().lrh_rays_secondary(self._c, _ptr(rays), _ptr(hits), n, _ptr(tris), _ptr(lt), eps, seed,
                                         _ptr(bounce), _ptr(shadow), _ptr(pix), _ptr(count)))
+
+
+def now_ms():
+    return lib().lrh_now_ms()
+
+
+def batch_scene(n, seed):
+    """A scene of the batched workload (lrs_batch_scene): dict like cornell's"""
+    import numpy as np
+    tris = np.zeros((n + 2, 12), np.float32)
+    prims = np.zeros((1, 24), np.float32)
+    mats = (Material * 16)()
+    lib().lrs_batch_scene(n, seed, _ptr(tris), _ptr(prims), ctypes.cast(mats, ctypes.c_void_p))
+    return {"tris": tris, "prims": prims, "mats": mats, "light": 0, "eps": 1e-4, "margin": 1e-3}
+
+
+def batch_camera(seed, w, h):
+    import numpy as np
+    cam = np.zeros(12, np.float32)
+    lib().lrs_batch_camera(seed, w, h, _ptr(cam))
+    return cam
+
+
+def batch_plan(njobs, seed, spp_lo=1):
+    """The job mix (lrs_batch_plan): list of (ntris, w, h, spp)"""
+    import numpy as np
+    a = [np.zeros(njobs, np.int32) for _ in range(4)]
+    lib().lrs_batch_plan(njobs, seed, spp_lo, *[_ptr(x) for x in a])
+    return [tuple(int(x[i]) for x in a) for i in range(njobs)]
+
+
+def make_job(scene, cam, w, h, spp, s0=0, seed=0, max_depth=8, mis=True, clamp=0.0):
+    """an LrhJob for a PtScene"""
+    j = Job()
+    j.scene = scene._h.value if isinstance(scene, PtScene) else scene
+    for k in range(12):
+        j.cam[k] = float(cam[k])
+    j.w, j.h, j.spp, j.s0, j.seed = w, h, spp, s0, seed
+    j.max_depth, j.mis, j.clamp = max_depth, 1 if mis else 0, clamp
+    return j
+
+
+class Engine:
+    """An engine (lrh_engine_*): many jobs over one pool of path slots."""
+
+    def __init__(self, ctx, slots_log2=22, max_tree_depth=64, image_bytes=1 << 30):
+        self.ctx = ctx
+        self._e = ctypes.c_void_p()
+        _check(lib().lrh_engine_create(ctx._c, slots_log2, max_tree_depth, image_bytes, ctypes.byref(self._e)))
+
+    def close(self):
+        if self._e:
+            lib().lrh_engine_destroy(self._e)
+            self._e = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def tune(self):
+        t = Tune()
+        _check(lib().lrh_engine_tune_get(self._e, ctypes.byref(t)))
+        return t
+
+    @tune.setter
+    def tune(self, t):
+        if isinstance(t, dict):
+            x = Tune()
+            for k, v in t.items():
+                setattr(x, k, v)
+            t = x
+        _check(lib().lrh_engine_tune_set(self._e, ctypes.byref(t)))
+
+    def counting(self, on):
+        _check(lib().lrh_engine_counting(self._e, 1 if on else 0))
+
+    def submit(self, job):
+        """the job's id, or None if it does not fit now"""
+        i = ctypes.c_int64()
+        rc = lib().lrh_engine_submit(self._e, ctypes.byref(job), ctypes.byref(i))
+        if rc == EAGAIN:
+            return None
+        _check(rc)
+        return i.value
+
+    def pump(self, graphs=1):
+        rc = lib().lrh_engine_pump(self._e, graphs)
+        if rc < 0:
+            _check(rc)
+        return rc
+
+    def drain(self):
+        _check(lib().lrh_engine_drain(self._e))
+
+    def plain(self, job):
+        i = ctypes.c_int64()
+        _check(lib().lrh_engine_plain(self._e, ctypes.byref(job), ctypes.byref(i)))
+        return i.value
+
+    def job(self, i):
+        info = JobInfo()
+        _check(lib().lrh_engine_job(self._e, i, ctypes.byref(info)))
+        return info
+
+    def image(self, i, out, add=False):
+        """the job's image into a device tensor (h, w, 4): sum of the samples' rgb, count"""
+        _check(lib().lrh_engine_image(self._e, i, _ptr(out), 1 if add else 0))
+
+    def release(self, i):
+        _check(lib().lrh_engine_release(self._e, i))
+
+    def stats(self):
+        st = EngineStats()
+        _check(lib().lrh_engine_stats(self._e, ctypes.byref(st)))
+        return st
+
+    def autotune(self, jobs):
+        arr = (Job * len(jobs))(*jobs)
+        best = Tune()
+        mr = ctypes.c_double()
+        _check(lib().lrh_engine_autotune(self._e, arr, len(jobs), ctypes.byref(best), ctypes.byref(mr)))
+        return best, mr.value
+
+
+class Server:
+    """The serving loop (lrh_server_*): resident scenes, requests admitted by memory budget."""
+
+    def __init__(self, ctx, slots_log2=22, max_tree_depth=64, budget=8 << 30, tune=None):
+        self.ctx = ctx
+        self._s = ctypes.c_void_p()
+        t = None
+        if tune is not None:
+            t = Tune()
+            for k, v in (tune.as_dict() if isinstance(tune, Tune) else tune).items():
+                setattr(t, k, v)
+        _check(lib().lrh_server_create(ctx._c, slots_log2, max_tree_depth, budget,
+                                       ctypes.byref(t) if t is not None else None, ctypes.byref(self._s)))
+
+    def close(self):
+        if self._s:
+            lib().lrh_server_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, scene):
+        _check(lib().lrh_server_add_scene(self._s, scene._h))
+
+    def submit(self, job):
+        i = ctypes.c_int64()
+        _check(lib().lrh_server_submit(self._s, ctypes.byref(job), ctypes.byref(i)))
+        return i.value
+
+    def wait(self, i, timeout_ms=-1.0):
+        """the job's JobInfo when done, None on time-out"""
+        info = JobInfo()
+        rc = lib().lrh_server_wait(self._s, i, timeout_ms, ctypes.byref(info))
+        if rc == EAGAIN:
+            return None
+        _check(rc)
+        return info
+
+    def image(self, i, w, h):
+        import numpy as np
+        out = np.zeros((h, w, 4), np.float32)
+        _check(lib().lrh_server_image(self._s, i, _ptr(out)))
+        return out
+
+    def release(self, i):
+        _check(lib().lrh_server_release(self._s, i))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().lrh_server_stats(self._s, ctypes.byref(st)))
+        return st

@@ -81,6 +81,7 @@ void lrh_destroy(LrhContext *c)
 {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    lrh_engine_destroy(c->engine);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &s : c->scratch)
         if (s.base) (void)hipFree(s.base);

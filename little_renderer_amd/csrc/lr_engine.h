@@ -1,0 +1,118 @@
+// lr_engine.h - the path-tracing engine's device state: many jobs (each its own resident
+// scene, camera, image and sample range) share one pool of path slots that the wavefront
+// kernels keep full.
+//
+// One iteration is four kernels: regen (start new paths in free slots), extend (closest hits
+// of the ray queue), shade (emission, next-event shadow ray, BSDF sample, Russian roulette),
+// shadow (any hit before the lamp). A path that ends gives its slot back to a ring of free
+// slots; regen fills free slots from the global path index space, in which job j owns
+// [start_j, start_j + w*h*spp) (sample-major: q = sample * w*h + pixel). The same kernels serve
+// the reference's one-job-at-a-time shape (a grid per pass, passes in order) and the batched
+// form (persistent work-groups, one hipGraph of several iterations replayed until the work is
+// done), so a job's image is the same to the bit in either, alone or among others: each
+// sample depends only on (job seed, pixel, sample index) and is added to its pixel as a
+// fixed-point integer (2^-28 units), whose sum does not depend on order.
+#pragma once
+
+#include "lr_internal.h"
+
+#define LR_SLOT_BITS 22                   // path slots: at most 2^22 (queue ids: job << 22 | slot)
+#define LR_SLOT_MASK ((1u << LR_SLOT_BITS) - 1u)
+#define LR_JOBS 1024                      // job descriptors held on the device at once
+#define LR_ISSUE 4096                     // issue records (a ring, power of two)
+#define LR_FIX_SHIFT 28                   // a sample's radiance in units of 2^-28
+#define LR_FIX_CAP 16777216.0f            // and saturated at 2^24 (a pixel can hold 2^11 such)
+
+// one job on the device (written by a kernel with the descriptor as its argument, so the
+// host's copy is never read late)
+struct JobDev {
+    const float4 *nodes;
+    const uint32_t *aux;
+    const float4 *tris;
+    const float4 *prims;
+    const float4 *mats;       // 3 float4 per material: albedo.rgb type | emission.rgb ior | metallic roughness transmission 0
+    long long *accum;         // 3 x int64 per pixel, fixed point
+    unsigned long long start, total;
+    float cam[12];
+    uint32_t root, nprims;
+    int W, H, npix, light;
+    int max_depth, mis;
+    uint32_t s0, seed_lo, seed_hi;
+    float clamp, eps, margin;
+};
+
+enum { K_REGEN = 0, K_EXTEND = 1, K_SHADE = 2, K_SHADOW = 3 };
+
+// the engine's counters (device memory, followed by each job slot's progress)
+struct EngCtl {
+    unsigned long long head, tail;          // the free-slot ring holds slots [head, tail)
+    unsigned long long next, issue_end;     // the next path index to start; the end of those issued
+    unsigned long long iq_lo, iq_hi;        // the issue records in use
+    unsigned long long rays_ext, rays_shadow;
+    unsigned long long iter;                // iterations run (its parity picks the ray queue)
+    unsigned long long claimed;             // paths the running iteration's regen started
+    unsigned int count[4];                  // ray queues [0], [1]; shadow queue [2]
+    unsigned int tiles[4];                  // each kernel's tile counter (work-groups claim tiles)
+    unsigned long long pad[4];
+};
+
+struct EngProgress {
+    EngCtl ctl;
+    unsigned long long jret[LR_JOBS];       // paths retired per job slot
+    unsigned long long jdone[LR_JOBS];      // device clock when the slot's last path retired
+};
+
+// everything a kernel needs, passed by value
+struct EngArgs {
+    EngCtl *ctl;
+    const JobDev *jobs;
+    unsigned long long *jret, *jdone;
+    const unsigned long long *iq_start;     // issue record: the path index its job starts at
+    const uint32_t *iq_job;                 // and the job slot
+    float4 *thr;                            // throughput.rgb, the last bounce's pdf
+    float4 *rad;                            // radiance.rgb, the emission state
+    uint64_t *rng;                          // PCG32 state (stream: the pixel)
+    uint2 *meta;                            // q (index in the job), depth
+    uint32_t *ring;                         // free slots
+    float4 *qray[2];                        // o.xyz tmin | d.xyz tmax
+    uint32_t *qid[2];                       // job << 22 | slot
+    float4 *hits;                           // t, id, b1, b2 (by queue position)
+    float4 *sray;
+    uint32_t *sid;
+    float4 *scon;                           // contribution.rgb, 1 if the path ends after it
+    uint32_t *spill;                        // walk stacks past the LDS part, by thread
+    unsigned long long spill_stride;        // threads of the launch
+    int spill_per;
+    uint32_t *overflow;
+    const float *tab;                       // BSDF tables
+    const uint16_t *mask;                   // blue-noise ranks
+    unsigned long long *counts;             // node records and shapes walked (counting extend only)
+    uint32_t cmask;                         // slots - 1
+    int mult;                               // a tile is mult work-groups' widths of entries
+};
+
+typedef void (*LrEngFn)(dim3 grid, hipStream_t s, const EngArgs &A);
+
+// the kernels, by work-group size (64, 128, 256: index 0..2) and LDS stack (16, 24, 32)
+struct LrEngTable {
+    LrEngFn regen[3], shade[3];
+    LrEngFn extend[3][3], extend_count[3][3], shadow[3][3];
+};
+void lr_eng_fill_b64(LrEngTable &t);
+void lr_eng_fill_b128(LrEngTable &t);
+void lr_eng_fill_b256(LrEngTable &t);
+
+static inline int lr_blk_index(int b) { return b == 64 ? 0 : b == 128 ? 1 : b == 256 ? 2 : -1; }
+static inline int lr_stk_index(int s) { return s == 16 ? 0 : s == 24 ? 1 : s == 32 ? 2 : -1; }
+
+// a scene resident on the device (lrh_pt_scene)
+struct LrhPtScene {
+    LrhContext *ctx;
+    float4 *prims = nullptr, *tris = nullptr, *mats = nullptr, *boxes = nullptr;
+    uint32_t *nodes = nullptr, *aux = nullptr;
+    float *tab = nullptr;
+    uint16_t *mask = nullptr;
+    int nprims = 0, ntris = 0, nmats = 0, light = -1, depth = 1;
+    float eps = 1e-4f, margin = 1e-3f;
+    size_t bytes = 0;                       // device memory it holds
+};

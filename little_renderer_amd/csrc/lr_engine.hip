@@ -1,0 +1,829 @@
+// lr_engine.hip - the engine's host side (include/lr_hip.h: lrh_engine_*, lrh_pt_render): the
+// device state, job admission and its image arena, the captured graph of iterations and the
+// progress it reports, the plain per-job form, and the autotuner.
+//
+// Progress: after each graph the engine's counters and every job slot's retired-path count
+// are copied to pinned host memory (two buffers, so the device never waits for the host); a
+// job is done when all its paths have retired, at the device clock its last retire recorded.
+#include <string.h>
+
+#include <chrono>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "lr_engine.h"
+
+extern "C" void lrs_blue_noise(uint16_t *rank);
+extern "C" void lrs_brdf_tables(float *out);
+
+namespace {
+
+constexpr int SBLOCK = 256;
+
+__global__ __launch_bounds__(SBLOCK) void k_eng_init(EngProgress *prog, uint32_t *ring, uint32_t C)
+{
+    const size_t nw = sizeof(EngProgress) / 8;
+    unsigned long long *w = (unsigned long long *)prog;
+    for (size_t i = (size_t)blockIdx.x * SBLOCK + threadIdx.x; i < (size_t)C || i < nw; i += (size_t)gridDim.x * SBLOCK) {
+        if (i < (size_t)C) ring[i] = (uint32_t)i;
+        if (i < nw) w[i] = (i == offsetof(EngCtl, tail) / 8) ? (unsigned long long)C : 0ull;
+    }
+}
+
+__global__ void k_job_put(JobDev *jobs, unsigned long long *jret, unsigned long long *jdone, int slot, JobDev jd)
+{
+    jobs[slot] = jd;
+    jret[slot] = 0;
+    jdone[slot] = 0;
+}
+
+__global__ __launch_bounds__(SBLOCK) void k_zero64(long long *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * SBLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * SBLOCK) p[i] = 0;
+}
+
+// an issue record (rec = ~0: none) and the new end of the issued paths
+__global__ void k_issue(EngCtl *ctl, unsigned long long *iq_start, uint32_t *iq_job, unsigned long long rec,
+                        unsigned long long start, uint32_t jslot, unsigned long long issue_end, unsigned long long lo)
+{
+    if (rec != ~0ull) {
+        iq_start[rec & (LR_ISSUE - 1)] = start;
+        iq_job[rec & (LR_ISSUE - 1)] = jslot;
+        ctl->iq_hi = rec + 1;
+    }
+    ctl->issue_end = issue_end;
+    ctl->iq_lo = lo;
+}
+
+// wf_resolve's output (trace_wave.wgsl:357-378): float4 (sum, count) per pixel
+__global__ __launch_bounds__(SBLOCK) void k_resolve(const long long *acc, int npix, float count, float4 *out, int add)
+{
+    const int p = blockIdx.x * SBLOCK + threadIdx.x;
+    if (p >= npix) return;
+    const double u = 1.0 / (double)(1u << LR_FIX_SHIFT);
+    const float4 v = make_float4((float)((double)acc[3 * (size_t)p] * u), (float)((double)acc[3 * (size_t)p + 1] * u),
+                                 (float)((double)acc[3 * (size_t)p + 2] * u), count);
+    if (add) {
+        float4 o = out[p];
+        o.x += v.x;
+        o.y += v.y;
+        o.z += v.z;
+        o.w += v.w;
+        out[p] = o;
+    } else out[p] = v;
+}
+
+__global__ void k_clock(unsigned long long *out) { *out = (unsigned long long)wall_clock64(); }
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+const LrEngTable &table()
+{
+    static LrEngTable t = [] {
+        LrEngTable x;
+        memset(&x, 0, sizeof(x));
+        lr_eng_fill_b64(x);
+        lr_eng_fill_b128(x);
+        lr_eng_fill_b256(x);
+        return x;
+    }();
+    return t;
+}
+
+struct HostJob {
+    int state = 0;              // 0 free, LRH_JOB_RUNNING, LRH_JOB_DONE
+    int64_t id = -1;
+    LrhJob desc{};
+    size_t off = 0, bytes = 0;  // its image in the arena
+    unsigned long long start = 0, total = 0, rec = 0;
+    double t_submit = 0, t_done = 0;
+    unsigned long long min_seq = 0;   // the first progress snapshot taken after its admission
+};
+
+struct IssueRec {
+    unsigned long long rec, start, end;
+};
+
+}  // namespace
+
+struct LrhEngine {
+    LrhContext *ctx = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t C = 0;
+    int ncu = 0, max_tree_depth = 0;
+    size_t threads_cap = 0;
+    int spill_cap = 0;                         // spill entries per thread (for a 16-entry LDS stack)
+    // device memory
+    void *dmem = nullptr;
+    char *arena = nullptr;
+    size_t arena_bytes = 0;
+    EngProgress *prog = nullptr;
+    JobDev *jobs = nullptr;
+    unsigned long long *iq_start = nullptr, *counts = nullptr, *clk = nullptr;
+    uint32_t *iq_job = nullptr, *overflow = nullptr;
+    EngArgs base{};
+    // host
+    EngProgress *pin[2] = {nullptr, nullptr};
+    hipEvent_t pev[2] = {nullptr, nullptr};
+    bool pend[2] = {false, false};
+    unsigned long long seq[2] = {0, 0}, seq_next = 0;
+    int pk = 0;
+    EngProgress last{};
+    HostJob hj[LR_JOBS];
+    std::map<size_t, size_t> free_blocks;      // arena: offset -> bytes
+    std::map<int64_t, int> by_id;
+    std::deque<IssueRec> recs;
+    int64_t next_id = 1;
+    unsigned long long issue_end = 0, iq_hi = 0;
+    int running = 0;
+    uint64_t jobs_done = 0;
+    LrhTune tune{128, 24, 128, 4, 1, 8};
+    bool counting = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_ok = false;
+    double clk_khz = 100000.0, clk_host0 = 0.0;
+    unsigned long long clk_dev0 = 0;
+};
+
+namespace {
+
+int eng_fail_dev(const char *what, hipError_t e)
+{
+    return lr_fail(LRH_EDEVICE, "engine: %s: %s", what, hipGetErrorString(e));
+}
+#define ENG_HIP(call)                                          \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return eng_fail_dev(#call, e_);  \
+    } while (0)
+
+bool tune_valid(const LrhTune &t)
+{
+    return lr_blk_index(t.block_trace) >= 0 && lr_blk_index(t.block_shade) >= 0 && lr_stk_index(t.stack) >= 0 &&
+           t.blocks_per_cu >= 1 && t.blocks_per_cu <= 16 && (t.tile == 1 || t.tile == 2 || t.tile == 4 || t.tile == 8) &&
+           t.iters_per_graph >= 1 && t.iters_per_graph <= 64;
+}
+
+// the kernels' arguments for a launch whose walk kernels run `threads` threads
+EngArgs args_for(const LrhEngine *e, int stack, size_t threads, int mult)
+{
+    EngArgs A = e->base;
+    A.spill_per = e->max_tree_depth + 2 - stack > 0 ? e->max_tree_depth + 2 - stack : 0;
+    A.spill_stride = threads;
+    A.mult = mult;
+    return A;
+}
+
+// the persistent grids of the batched form
+void launch_iteration(LrhEngine *e, const LrhTune &t)
+{
+    const LrEngTable &T = table();
+    const int bt = lr_blk_index(t.block_trace), bs = lr_blk_index(t.block_shade), sk = lr_stk_index(t.stack);
+    const dim3 gt(e->ncu * t.blocks_per_cu), gs(e->ncu * t.blocks_per_cu);
+    const EngArgs At = args_for(e, t.stack, (size_t)gt.x * t.block_trace, t.tile);
+    const EngArgs As = args_for(e, t.stack, 0, t.tile);
+    T.regen[bs](gs, e->stream, As);
+    (e->counting ? T.extend_count : T.extend)[bt][sk](gt, e->stream, At);
+    T.shade[bs](gs, e->stream, As);
+    T.shadow[bt][sk](gt, e->stream, At);
+}
+
+int make_graph(LrhEngine *e)
+{
+    if (e->graph_ok) return 0;
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->gexec = nullptr;
+    e->graph = nullptr;
+    ENG_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < e->tune.iters_per_graph; i++) launch_iteration(e, e->tune);
+    hipError_t le = hipGetLastError();
+    hipError_t ce = hipStreamEndCapture(e->stream, &e->graph);
+    if (le != hipSuccess) return eng_fail_dev("graph capture (launch)", le);
+    if (ce != hipSuccess) return eng_fail_dev("graph capture", ce);
+    ENG_HIP(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+    e->graph_ok = true;
+    return 0;
+}
+
+double dev_ms(const LrhEngine *e, unsigned long long clk)
+{
+    return e->clk_host0 + ((double)clk - (double)e->clk_dev0) / e->clk_khz;
+}
+
+// takes in a progress snapshot: jobs whose paths have all retired are done (a snapshot copied
+// before a job's admission may still show the slot's previous job: it is not looked at)
+void absorb(LrhEngine *e, const EngProgress &p, unsigned long long seq)
+{
+    e->last = p;
+    for (int s = 0; s < LR_JOBS; s++) {
+        HostJob &j = e->hj[s];
+        if (j.state != LRH_JOB_RUNNING || seq < j.min_seq) continue;
+        if (p.jret[s] >= j.total && j.total > 0) {
+            j.state = LRH_JOB_DONE;
+            j.t_done = dev_ms(e, p.jdone[s]);
+            e->running--;
+            e->jobs_done++;
+        }
+    }
+    while (!e->recs.empty() && e->recs.front().end <= p.ctl.next) e->recs.pop_front();
+}
+
+unsigned long long issue_lo(const LrhEngine *e) { return e->recs.empty() ? e->iq_hi : e->recs.front().rec; }
+
+int progress_enqueue(LrhEngine *e)
+{
+    const int b = e->pk;
+    ENG_HIP(hipMemcpyAsync(e->pin[b], e->prog, sizeof(EngProgress), hipMemcpyDeviceToHost, e->stream));
+    ENG_HIP(hipEventRecord(e->pev[b], e->stream));
+    e->pend[b] = true;
+    e->seq[b] = e->seq_next++;
+    e->pk ^= 1;
+    return 0;
+}
+
+int progress_take(LrhEngine *e, int b)
+{
+    if (!e->pend[b]) return 0;
+    ENG_HIP(hipEventSynchronize(e->pev[b]));
+    e->pend[b] = false;
+    absorb(e, *e->pin[b], e->seq[b]);
+    return 0;
+}
+
+// every pending snapshot taken in, oldest first
+int progress_flush(LrhEngine *e)
+{
+    int rc = progress_take(e, e->pk);
+    if (!rc) rc = progress_take(e, e->pk ^ 1);
+    return rc;
+}
+
+int check_overflow(LrhEngine *e)
+{
+    uint32_t f = 0;
+    ENG_HIP(hipMemcpyAsync(&f, e->overflow, 4, hipMemcpyDeviceToHost, e->stream));
+    ENG_HIP(hipStreamSynchronize(e->stream));
+    if (f) return lr_fail(LRH_EREFUSED, "engine: a walk needed more stack than the engine's tree depth allows");
+    return 0;
+}
+
+size_t arena_take(LrhEngine *e, size_t bytes)
+{
+    for (auto it = e->free_blocks.begin(); it != e->free_blocks.end(); ++it) {
+        if (it->second < bytes) continue;
+        const size_t off = it->first, left = it->second - bytes;
+        e->free_blocks.erase(it);
+        if (left) e->free_blocks[off + bytes] = left;
+        return off;
+    }
+    return (size_t)-1;
+}
+
+void arena_give(LrhEngine *e, size_t off, size_t bytes)
+{
+    auto it = e->free_blocks.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != e->free_blocks.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        e->free_blocks.erase(nx);
+    }
+    if (it != e->free_blocks.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            e->free_blocks.erase(it);
+        }
+    }
+}
+
+int job_check(const LrhEngine *e, const LrhJob *j)
+{
+    if (!j || !j->scene || j->w <= 0 || j->h <= 0 || j->spp <= 0 || j->max_depth < 0)
+        return lr_fail(LRH_EINVAL, "engine: bad job");
+    const unsigned long long npix = (unsigned long long)j->w * (unsigned long long)j->h;
+    if (npix * (unsigned long long)j->spp >= (1ull << 32))
+        return lr_fail(LRH_EREFUSED, "engine: %d x %d x %d paths: at most 2^32 per job", j->w, j->h, j->spp);
+    if (j->scene->depth > e->max_tree_depth)
+        return lr_fail(LRH_EREFUSED, "engine: the scene's tree is %d deep, the engine's walks hold %d", j->scene->depth,
+                       e->max_tree_depth);
+    return 0;
+}
+
+// admits a job: its descriptor, its image cleared, and (issue) its paths queued
+int job_admit(LrhEngine *e, const LrhJob *j, bool issue, int *slot_out)
+{
+    int rc = job_check(e, j);
+    if (rc) return rc;
+    int slot = -1;
+    for (int s = 0; s < LR_JOBS; s++)
+        if (e->hj[s].state == 0) {
+            slot = s;
+            break;
+        }
+    if (slot < 0) return lr_fail(LRH_EAGAIN, "engine: all %d job slots are in use", LR_JOBS);
+    if (e->iq_hi - issue_lo(e) >= LR_ISSUE - 1) return lr_fail(LRH_EAGAIN, "engine: too many jobs queued");
+    const size_t npix = (size_t)j->w * (size_t)j->h;
+    const size_t bytes = lr_align(npix * 24);
+    const size_t off = arena_take(e, bytes);
+    if (off == (size_t)-1) return lr_fail(LRH_EAGAIN, "engine: the image (%zu bytes) does not fit now", bytes);
+    HostJob &h = e->hj[slot];
+    h = HostJob();
+    h.state = LRH_JOB_RUNNING;
+    h.id = e->next_id++;
+    h.desc = *j;
+    h.off = off;
+    h.bytes = bytes;
+    h.total = (unsigned long long)npix * (unsigned long long)j->spp;
+    h.start = e->issue_end;
+    h.rec = e->iq_hi;
+    h.t_submit = now_ms();
+    h.min_seq = e->seq_next;
+    const LrhPtScene *s = j->scene;
+    JobDev jd;
+    memset(&jd, 0, sizeof(jd));
+    jd.nodes = (const float4 *)s->nodes;
+    jd.aux = s->aux;
+    jd.tris = s->tris;
+    jd.prims = s->prims;
+    jd.mats = s->mats;
+    jd.accum = (long long *)(e->arena + off);
+    jd.start = h.start;
+    jd.total = h.total;
+    memcpy(jd.cam, j->cam, sizeof(jd.cam));
+    jd.root = 0;
+    jd.nprims = (uint32_t)s->nprims;
+    jd.W = j->w;
+    jd.H = j->h;
+    jd.npix = (int)npix;
+    jd.light = s->light;
+    jd.max_depth = j->max_depth;
+    jd.mis = j->mis;
+    jd.s0 = j->s0;
+    jd.seed_lo = (uint32_t)j->seed;
+    jd.seed_hi = (uint32_t)(j->seed >> 32);
+    jd.clamp = j->clamp;
+    jd.eps = s->eps;
+    jd.margin = s->margin;
+    hipLaunchKernelGGL(k_job_put, dim3(1), dim3(1), 0, e->stream, e->jobs, e->prog->jret, e->prog->jdone, slot, jd);
+    const size_t words = npix * 3;
+    hipLaunchKernelGGL(k_zero64, dim3((unsigned)((words + SBLOCK - 1) / SBLOCK < 4096 ? (words + SBLOCK - 1) / SBLOCK : 4096)),
+                       dim3(SBLOCK), 0, e->stream, jd.accum, words);
+    e->iq_hi++;
+    e->issue_end += issue ? h.total : 0;
+    e->recs.push_back(IssueRec{h.rec, h.start, h.start + h.total});
+    hipLaunchKernelGGL(k_issue, dim3(1), dim3(1), 0, e->stream, &e->prog->ctl, e->iq_start, e->iq_job, h.rec, h.start,
+                       (uint32_t)slot, e->issue_end, issue_lo(e));
+    ENG_HIP(hipGetLastError());
+    e->by_id[h.id] = slot;
+    e->running++;
+    *slot_out = slot;
+    return 0;
+}
+
+int job_slot(const LrhEngine *e, int64_t id)
+{
+    auto it = e->by_id.find(id);
+    return it == e->by_id.end() ? -1 : it->second;
+}
+
+}  // namespace
+
+extern "C" {
+
+double lrh_now_ms(void) { return now_ms(); }
+
+void lrh_engine_destroy(LrhEngine *e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (int b = 0; b < 2; b++) {
+        if (e->pin[b]) (void)hipHostFree(e->pin[b]);
+        if (e->pev[b]) (void)hipEventDestroy(e->pev[b]);
+    }
+    if (e->dmem) (void)hipFree(e->dmem);
+    if (e->arena) (void)hipFree(e->arena);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t image_bytes, LrhEngine **out)
+{
+    if (!c || !out || slots_log2 < 10 || slots_log2 > LR_SLOT_BITS || max_tree_depth < 1 || max_tree_depth > 1024 ||
+        image_bytes < 256)
+        return lr_fail(LRH_EINVAL, "lrh_engine_create: bad arguments");
+    *out = nullptr;
+    LR_HIP(hipSetDevice(c->device));
+    LrhEngine *e = new LrhEngine();
+    e->ctx = c;
+    e->device = c->device;
+    e->C = 1u << slots_log2;
+    e->max_tree_depth = max_tree_depth;
+    int ncu = 0, khz = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu <= 0) ncu = 256;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) == hipSuccess && khz > 0) e->clk_khz = khz;
+    e->ncu = ncu;
+    // walk threads: every slot (the plain form's grids) or every persistent work-group's lane
+    e->threads_cap = (size_t)e->C + 256;
+    if ((size_t)ncu * 16 * 256 > e->threads_cap) e->threads_cap = (size_t)ncu * 16 * 256;
+    e->spill_cap = max_tree_depth + 2 - 16 > 0 ? max_tree_depth + 2 - 16 : 0;
+    const size_t C = e->C;
+    const size_t sizes[] = {
+        sizeof(EngProgress), LR_JOBS * sizeof(JobDev), LR_ISSUE * 8, LR_ISSUE * 4,   // 0..3
+        C * 16, C * 16, C * 8, C * 8, C * 4,                                         // 4..8 thr rad rng meta ring
+        C * 32, C * 32, C * 4, C * 4, C * 16,                                        // 9..13 qray qray qid qid hits
+        C * 32, C * 4, C * 16,                                                       // 14..16 sray sid scon
+        e->threads_cap * (size_t)(e->spill_cap > 0 ? e->spill_cap : 1) * 4,          // 17 spill
+        2112 * 4, 4096 * 2, 256, 256, 256};                                          // 18..22 tab mask counts clk overflow
+    constexpr int NS = sizeof(sizes) / sizeof(sizes[0]);
+    size_t total = 0;
+    for (size_t z : sizes) total += lr_align(z);
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess || hipMalloc(&e->dmem, total) != hipSuccess ||
+        hipMalloc((void **)&e->arena, image_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        lrh_engine_destroy(e);
+        return lr_fail(LRH_ENOMEM, "lrh_engine_create: %zu + %zu bytes of device memory could not be had", total, image_bytes);
+    }
+    e->arena_bytes = image_bytes & ~(size_t)255;
+    e->free_blocks[0] = e->arena_bytes;
+    char *p[NS];
+    char *at = (char *)e->dmem;
+    for (int i = 0; i < NS; i++) {
+        p[i] = at;
+        at += lr_align(sizes[i]);
+    }
+    e->prog = (EngProgress *)p[0];
+    e->jobs = (JobDev *)p[1];
+    e->iq_start = (unsigned long long *)p[2];
+    e->iq_job = (uint32_t *)p[3];
+    e->counts = (unsigned long long *)p[20];
+    e->clk = (unsigned long long *)p[21];
+    e->overflow = (uint32_t *)p[22];
+    EngArgs &A = e->base;
+    A.ctl = &e->prog->ctl;
+    A.jobs = e->jobs;
+    A.jret = e->prog->jret;
+    A.jdone = e->prog->jdone;
+    A.iq_start = e->iq_start;
+    A.iq_job = e->iq_job;
+    A.thr = (float4 *)p[4];
+    A.rad = (float4 *)p[5];
+    A.rng = (uint64_t *)p[6];
+    A.meta = (uint2 *)p[7];
+    A.ring = (uint32_t *)p[8];
+    A.qray[0] = (float4 *)p[9];
+    A.qray[1] = (float4 *)p[10];
+    A.qid[0] = (uint32_t *)p[11];
+    A.qid[1] = (uint32_t *)p[12];
+    A.hits = (float4 *)p[13];
+    A.sray = (float4 *)p[14];
+    A.sid = (uint32_t *)p[15];
+    A.scon = (float4 *)p[16];
+    A.spill = (uint32_t *)p[17];
+    A.tab = (const float *)p[18];
+    A.mask = (const uint16_t *)p[19];
+    A.counts = e->counts;
+    A.overflow = e->overflow;
+    A.cmask = e->C - 1;
+    A.mult = 1;
+    std::vector<float> tab(2112);
+    std::vector<uint16_t> mask(4096);
+    lrs_brdf_tables(tab.data());
+    lrs_blue_noise(mask.data());
+    for (int b = 0; b < 2; b++) {
+        if (hipHostMalloc((void **)&e->pin[b], sizeof(EngProgress), hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&e->pev[b], hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            lrh_engine_destroy(e);
+            return lr_fail(LRH_ENOMEM, "lrh_engine_create: pinned memory or events could not be had");
+        }
+    }
+    int rc = 0;
+    const double t0 = now_ms();
+    if (hipMemsetAsync(e->dmem, 0, total, e->stream) != hipSuccess ||
+        hipMemcpyAsync(p[18], tab.data(), 2112 * 4, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+        hipMemcpyAsync(p[19], mask.data(), 4096 * 2, hipMemcpyHostToDevice, e->stream) != hipSuccess)
+        rc = -1;
+    if (!rc) {
+        hipLaunchKernelGGL(k_eng_init, dim3(1024), dim3(SBLOCK), 0, e->stream, e->prog, A.ring, e->C);
+        hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, e->stream, e->clk);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(e->stream) != hipSuccess) rc = -1;
+    }
+    const double t1 = now_ms();
+    if (!rc && hipMemcpy(&e->clk_dev0, e->clk, 8, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+    if (rc) {
+        (void)hipGetLastError();
+        lrh_engine_destroy(e);
+        return lr_fail(LRH_EDEVICE, "lrh_engine_create: the engine's state could not be set up");
+    }
+    e->clk_host0 = 0.5 * (t0 + t1);
+    memset(&e->last, 0, sizeof(e->last));
+    e->last.ctl.tail = e->C;
+    *out = e;
+    return 0;
+}
+
+int lrh_engine_tune_set(LrhEngine *e, const LrhTune *t)
+{
+    if (!e || !t || !tune_valid(*t)) return lr_fail(LRH_EINVAL, "lrh_engine_tune_set: bad launch shape");
+    if (memcmp(&e->tune, t, sizeof(LrhTune)) != 0) {
+        e->tune = *t;
+        e->graph_ok = false;
+    }
+    return 0;
+}
+
+int lrh_engine_tune_get(const LrhEngine *e, LrhTune *t)
+{
+    if (!e || !t) return lr_fail(LRH_EINVAL, "lrh_engine_tune_get: bad arguments");
+    *t = e->tune;
+    return 0;
+}
+
+int lrh_engine_counting(LrhEngine *e, int on)
+{
+    if (!e) return lr_fail(LRH_EINVAL, "lrh_engine_counting: no engine");
+    if ((bool)on != e->counting) {
+        e->counting = on != 0;
+        e->graph_ok = false;
+    }
+    return 0;
+}
+
+int lrh_engine_submit(LrhEngine *e, const LrhJob *job, int64_t *id)
+{
+    if (!e || !job || !id) return lr_fail(LRH_EINVAL, "lrh_engine_submit: bad arguments");
+    (void)hipSetDevice(e->device);
+    int slot;
+    int rc = job_admit(e, job, true, &slot);
+    if (rc) return rc;
+    *id = e->hj[slot].id;
+    return 0;
+}
+
+int lrh_engine_pump(LrhEngine *e, int graphs)
+{
+    if (!e || graphs < 0) return lr_fail(LRH_EINVAL, "lrh_engine_pump: bad arguments");
+    (void)hipSetDevice(e->device);
+    int rc = make_graph(e);
+    if (rc) return rc;
+    for (int g = 0; g < graphs && e->running > 0; g++) {
+        ENG_HIP(hipGraphLaunch(e->gexec, e->stream));
+        rc = progress_enqueue(e);
+        if (!rc) rc = progress_take(e, e->pk);   // the snapshot before this one
+        if (rc) return rc;
+    }
+    return e->running;
+}
+
+int lrh_engine_drain(LrhEngine *e)
+{
+    if (!e) return lr_fail(LRH_EINVAL, "lrh_engine_drain: no engine");
+    unsigned long long seen = ~0ull;
+    int still = 0;
+    while (e->running > 0) {
+        int rc = lrh_engine_pump(e, 1);
+        if (rc < 0) return rc;
+        // a stall (no path retired over many graphs) would be a defect: said, not waited on
+        const unsigned long long t = e->last.ctl.tail;
+        if (t == seen) {
+            if (++still > 4096) return lr_fail(LRH_EDEVICE, "lrh_engine_drain: no progress");
+        } else {
+            seen = t;
+            still = 0;
+        }
+    }
+    int rc = progress_flush(e);
+    if (!rc) rc = check_overflow(e);
+    return rc;
+}
+
+int lrh_engine_plain(LrhEngine *e, const LrhJob *job, int64_t *id)
+{
+    if (!e || !job || !id) return lr_fail(LRH_EINVAL, "lrh_engine_plain: bad arguments");
+    if (e->running > 0) return lr_fail(LRH_EINVAL, "lrh_engine_plain: the engine is busy");
+    (void)hipSetDevice(e->device);
+    const size_t npix = (size_t)job->w * (size_t)job->h;
+    if (npix > e->C) return lr_fail(LRH_EREFUSED, "lrh_engine_plain: %zu pixels a pass, %u slots", npix, e->C);
+    int slot;
+    int rc = job_admit(e, job, false, &slot);
+    if (rc) return rc;
+    HostJob &h = e->hj[slot];
+    const LrEngTable &T = table();
+    const LrhTune &t = e->tune;
+    const int bt = lr_blk_index(t.block_trace), bs = lr_blk_index(t.block_shade), sk = lr_stk_index(t.stack);
+    const dim3 gt((unsigned)((npix + t.block_trace - 1) / t.block_trace)), gs((unsigned)((npix + t.block_shade - 1) / t.block_shade));
+    const EngArgs At = args_for(e, t.stack, (size_t)gt.x * t.block_trace, 1);
+    const EngArgs As = args_for(e, t.stack, 0, 1);
+    for (int s = 0; s < job->spp; s++) {
+        // the pass: paths [start + s * npix, start + (s + 1) * npix), one per pixel
+        hipLaunchKernelGGL(k_issue, dim3(1), dim3(1), 0, e->stream, &e->prog->ctl, e->iq_start, e->iq_job, ~0ull, 0ull, 0u,
+                           h.start + (unsigned long long)(s + 1) * npix, issue_lo(e));
+        T.regen[bs](gs, e->stream, As);
+        for (int d = 0; d <= job->max_depth; d++) {
+            (e->counting ? T.extend_count : T.extend)[bt][sk](gt, e->stream, At);
+            T.shade[bs](gs, e->stream, As);
+            T.shadow[bt][sk](gt, e->stream, At);
+        }
+    }
+    ENG_HIP(hipGetLastError());
+    e->issue_end = h.start + h.total;
+    ENG_HIP(hipStreamSynchronize(e->stream));    // the host waits for the job (gpu_wait)
+    rc = progress_enqueue(e);
+    if (!rc) rc = progress_flush(e);
+    if (!rc) rc = check_overflow(e);
+    if (rc) return rc;
+    if (h.state != LRH_JOB_DONE) return lr_fail(LRH_EDEVICE, "lrh_engine_plain: the job did not finish");
+    *id = h.id;
+    return 0;
+}
+
+int lrh_engine_job(LrhEngine *e, int64_t id, LrhJobInfo *info)
+{
+    if (!e || !info) return lr_fail(LRH_EINVAL, "lrh_engine_job: bad arguments");
+    const int s = job_slot(e, id);
+    if (s < 0) return lr_fail(LRH_EINVAL, "lrh_engine_job: no job %lld", (long long)id);
+    const HostJob &h = e->hj[s];
+    info->state = h.state;
+    info->w = h.desc.w;
+    info->h = h.desc.h;
+    info->spp = h.desc.spp;
+    info->paths = h.total;
+    info->t_submit_ms = h.t_submit;
+    info->t_done_ms = h.state == LRH_JOB_DONE ? h.t_done : 0.0;
+    return 0;
+}
+
+int lrh_engine_image(LrhEngine *e, int64_t id, float *d_out, int add)
+{
+    if (!e || !d_out) return lr_fail(LRH_EINVAL, "lrh_engine_image: bad arguments");
+    const int s = job_slot(e, id);
+    if (s < 0) return lr_fail(LRH_EINVAL, "lrh_engine_image: no job %lld", (long long)id);
+    const HostJob &h = e->hj[s];
+    if (h.state != LRH_JOB_DONE) return lr_fail(LRH_EAGAIN, "lrh_engine_image: job %lld is not done", (long long)id);
+    const int npix = h.desc.w * h.desc.h;
+    hipLaunchKernelGGL(k_resolve, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, e->stream,
+                       (const long long *)(e->arena + h.off), npix, (float)h.desc.spp, (float4 *)d_out, add);
+    ENG_HIP(hipGetLastError());
+    ENG_HIP(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int lrh_engine_release(LrhEngine *e, int64_t id)
+{
+    if (!e) return lr_fail(LRH_EINVAL, "lrh_engine_release: no engine");
+    const int s = job_slot(e, id);
+    if (s < 0) return lr_fail(LRH_EINVAL, "lrh_engine_release: no job %lld", (long long)id);
+    HostJob &h = e->hj[s];
+    if (h.state != LRH_JOB_DONE) return lr_fail(LRH_EAGAIN, "lrh_engine_release: job %lld is running", (long long)id);
+    arena_give(e, h.off, h.bytes);
+    e->by_id.erase(id);
+    h = HostJob();
+    return 0;
+}
+
+int lrh_engine_stats(LrhEngine *e, LrhEngineStats *st)
+{
+    if (!e || !st) return lr_fail(LRH_EINVAL, "lrh_engine_stats: bad arguments");
+    int rc = progress_flush(e);
+    if (rc) return rc;
+    unsigned long long cnt[2] = {0, 0};
+    ENG_HIP(hipMemcpyAsync(cnt, e->counts, 16, hipMemcpyDeviceToHost, e->stream));
+    ENG_HIP(hipStreamSynchronize(e->stream));
+    st->rays_ext = e->last.ctl.rays_ext;
+    st->rays_shadow = e->last.ctl.rays_shadow;
+    st->iterations = e->last.ctl.iter;
+    st->jobs_done = e->jobs_done;
+    st->node_fetches = cnt[0];
+    st->shape_tests = cnt[1];
+    return 0;
+}
+
+int lrh_engine_autotune(LrhEngine *e, const LrhJob *sample, int n, LrhTune *best, double *mrays)
+{
+    if (!e || !sample || n <= 0) return lr_fail(LRH_EINVAL, "lrh_engine_autotune: bad arguments");
+    if (e->running > 0) return lr_fail(LRH_EINVAL, "lrh_engine_autotune: the engine is busy");
+    hipEvent_t ev[2];
+    ENG_HIP(hipEventCreate(&ev[0]));
+    ENG_HIP(hipEventCreate(&ev[1]));
+    auto measure = [&](const LrhTune &t, double &score) -> int {
+        int rc = lrh_engine_tune_set(e, &t);
+        if (!rc) rc = make_graph(e);
+        if (rc) return rc;
+        const unsigned long long r0 = e->last.ctl.rays_ext + e->last.ctl.rays_shadow;
+        std::vector<int64_t> ids;
+        ENG_HIP(hipEventRecord(ev[0], e->stream));
+        for (int i = 0; i < n; i++) {
+            int64_t id;
+            rc = lrh_engine_submit(e, &sample[i], &id);
+            if (rc == LRH_EAGAIN) break;
+            if (rc) return rc;
+            ids.push_back(id);
+        }
+        if (ids.empty()) return lr_fail(LRH_EREFUSED, "lrh_engine_autotune: no sample job fits the engine");
+        rc = lrh_engine_drain(e);
+        if (rc) return rc;
+        ENG_HIP(hipEventRecord(ev[1], e->stream));
+        ENG_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0.0f;
+        ENG_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        for (int64_t id : ids) lrh_engine_release(e, id);
+        const unsigned long long r1 = e->last.ctl.rays_ext + e->last.ctl.rays_shadow;
+        score = ms > 0.0f ? (double)(r1 - r0) / (ms * 1e3) : 0.0;
+        return 0;
+    };
+    LrhTune cur = e->tune, win = cur;
+    double top = 0.0, s = 0.0;
+    int rc = measure(cur, s);   // a first run warms the caches and the code up
+    if (!rc) rc = measure(cur, top);
+    const int blks[3] = {64, 128, 256}, stks[3] = {16, 24, 32}, bpcs[5] = {1, 2, 4, 8, 16}, tiles[4] = {1, 2, 4, 8};
+    auto consider = [&](const LrhTune &t) {
+        if (rc || memcmp(&t, &win, sizeof(LrhTune)) == 0) return;
+        double sc = 0.0;
+        rc = measure(t, sc);
+        if (!rc && sc > top) {
+            top = sc;
+            win = t;
+        }
+    };
+    for (int b : blks)
+        for (int k : stks) {
+            LrhTune t = win;
+            t.block_trace = b;
+            t.stack = k;
+            consider(t);
+        }
+    for (int v : bpcs) {
+        LrhTune t = win;
+        t.blocks_per_cu = v;
+        consider(t);
+    }
+    for (int v : tiles) {
+        LrhTune t = win;
+        t.tile = v;
+        consider(t);
+    }
+    for (int b : blks) {
+        LrhTune t = win;
+        t.block_shade = b;
+        consider(t);
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc) return rc;
+    rc = lrh_engine_tune_set(e, &win);
+    if (rc) return rc;
+    if (best) *best = win;
+    if (mrays) *mrays = top;
+    return 0;
+}
+
+// lrh_pt_render: a one-job batch on the context's own engine (made, or made larger, on demand)
+int lrh_pt_render(LrhContext *c, const LrhPtScene *s, const float *cam, int W, int H, int spp, uint32_t s0, uint64_t seed,
+                  int max_depth, int mis, float clamp, float *d_accum)
+{
+    if (!c || !s || !cam || W <= 0 || H <= 0 || spp <= 0 || max_depth < 0 || !d_accum)
+        return lr_fail(LRH_EINVAL, "lrh_pt_render: bad arguments");
+    const size_t need = lr_align((size_t)W * H * 24);
+    LrhEngine *e = c->engine;
+    if (e && (e->max_tree_depth < s->depth || e->arena_bytes < need)) {
+        lrh_engine_destroy(e);
+        c->engine = e = nullptr;
+    }
+    if (!e) {
+        const size_t img = need > ((size_t)64 << 20) ? need : ((size_t)64 << 20);
+        int rc = lrh_engine_create(c, 20, s->depth > 64 ? s->depth : 64, img, &e);
+        if (rc) return rc;
+        c->engine = e;
+    }
+    // the caller's stream may still be writing d_accum or the scene
+    LR_HIP(hipStreamSynchronize(c->stream));
+    LrhJob j;
+    j.scene = s;
+    memcpy(j.cam, cam, sizeof(j.cam));
+    j.w = W;
+    j.h = H;
+    j.spp = spp;
+    j.s0 = s0;
+    j.seed = seed;
+    j.max_depth = max_depth;
+    j.mis = mis;
+    j.clamp = clamp;
+    int64_t id;
+    int rc = lrh_engine_submit(e, &j, &id);
+    if (!rc) rc = lrh_engine_drain(e);
+    if (!rc) rc = lrh_engine_image(e, id, d_accum, 1);
+    if (!rc) rc = lrh_engine_release(e, id);
+    return rc;
+}
+
+}  // extern "C"

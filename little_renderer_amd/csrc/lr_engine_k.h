@@ -1,0 +1,434 @@
+// lr_engine_k.h - the engine's four wavefront kernels, templated on the work-group size (and
+// the walks on their LDS stack depth). Included once per work-group size by
+// lr_engine_b64.hip, lr_engine_b128.hip and lr_engine_b256.hip, each filling the launch table
+// (lr_engine.h) for its size.
+//
+// Light transport: trace_path_sampled (engine/rt/tracer.c:364-509) and trace_image
+// (tracer.c:329-356) in the wavefront split of trace_wave.wgsl (wf_gen :51-70, wf_ext :72-84,
+// wf_shade :191-310, wf_shadow :335-355, wf_resolve :357-378).
+#pragma once
+
+#include "lr_engine.h"
+#include "lr_pt_dev.h"
+
+namespace {
+
+// The entries [0, n) in tiles of BLK * mult, claimed by the work-groups one at a time through
+// a counter (persistent work-groups loop; a grid that covers n takes one tile each). body(i,
+// live) runs for every lane, live or not, so that it may use wave-wide ballots.
+template <int BLK, class F>
+__device__ __forceinline__ void for_tiles(unsigned int *ctr, uint32_t n, int mult, F &&body)
+{
+    __shared__ uint32_t s_tile;
+    const uint32_t width = (uint32_t)BLK * (uint32_t)mult;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_tile = atomicAdd(ctr, 1u);
+        __syncthreads();
+        const uint64_t base = (uint64_t)s_tile * width;
+        if (base >= n) break;
+        for (int j = 0; j < mult; j++) {
+            const uint64_t i = base + (uint64_t)j * BLK + threadIdx.x;
+            body((uint32_t)i, i < n);
+        }
+    }
+}
+
+// Each kernel clears the other kernels' tile counters as it starts (kernels of one stream run
+// one after another, and no kernel follows itself), so every kernel finds its own at zero.
+__device__ __forceinline__ void reset_tiles(EngCtl *ctl, int self)
+{
+    for (int k = 0; k < 4; k++)
+        if (k != self) ctl->tiles[k] = 0;
+}
+
+__device__ __forceinline__ long long to_fixed(float x)
+{
+    x = x > LR_FIX_CAP ? LR_FIX_CAP : (x < -LR_FIX_CAP ? -LR_FIX_CAP : x);
+    return __float2ll_rn(x * (float)(1u << LR_FIX_SHIFT));
+}
+
+// A path ends: its sample joins its pixel (NaN samples skipped and the clamp applied as
+// trace_image does, tracer.c:347-349), its slot goes back to the ring, its job counts it. Every
+// lane of the wave calls this (on: the lane's path ends).
+__device__ __forceinline__ void retire(const EngArgs &A, bool on, uint32_t jslot, uint32_t slot, uint32_t q, float4 r)
+{
+    if (on) {
+        const JobDev &J = A.jobs[jslot];
+        V3 c = v3(r.x, r.y, r.z);
+        if (c.x == c.x && c.y == c.y && c.z == c.z) {
+            const float m = vmax3(c);
+            if (J.clamp > 0.0f && m > J.clamp) c = vscale(c, J.clamp / m);
+            long long *px = J.accum + 3 * (size_t)(q % (uint32_t)J.npix);
+            const long long fx = to_fixed(c.x), fy = to_fixed(c.y), fz = to_fixed(c.z);
+            if (fx) atomicAdd((unsigned long long *)&px[0], (unsigned long long)fx);
+            if (fy) atomicAdd((unsigned long long *)&px[1], (unsigned long long)fy);
+            if (fz) atomicAdd((unsigned long long *)&px[2], (unsigned long long)fz);
+        }
+    }
+    const uint64_t m = __ballot(on);
+    if (m == 0) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned long long base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(&A.ctl->tail, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (on) A.ring[(uint32_t)(base + (unsigned long long)__popcll(m & lanemask_lt())) & A.cmask] = slot;
+    // the job's count of retired paths: one atomic per job present in the wave
+    uint64_t pend = m;
+    while (pend) {
+        const int l = __ffsll((unsigned long long)pend) - 1;
+        const uint32_t j = __shfl(jslot, l, 64);
+        const uint64_t same = __ballot(on && jslot == j);
+        pend &= ~same;
+        if ((int)lane_id() == l) {
+            const unsigned long long k = (unsigned long long)__popcll(same);
+            const unsigned long long old = atomicAdd(&A.jret[j], k);
+            if (old + k == A.jobs[j].total) A.jdone[j] = wall_clock64();
+        }
+    }
+}
+
+// regen (wf_gen, trace_wave.wgsl:51-70, with trace_image's seeds, tracer.c:335-346): the free
+// slots [head, head + claimed) start paths next .. next + claimed - 1, each a camera ray
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_regen(EngArgs A)
+{
+    EngCtl *ctl = A.ctl;
+    const unsigned long long head = ctl->head, tail = ctl->tail, next = ctl->next, iend = ctl->issue_end;
+    const unsigned long long lo = ctl->iq_lo, hi = ctl->iq_hi;
+    const unsigned long long free_n = tail - head, todo = iend - next;
+    const uint32_t n = (uint32_t)(free_n < todo ? free_n : todo);
+    const int cur = (int)(ctl->iter & 1ull);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->claimed = n;
+        reset_tiles(ctl, K_REGEN);
+    }
+    for_tiles<BLK>(&ctl->tiles[K_REGEN], n, A.mult, [&](uint32_t i, bool live) {
+        uint32_t id = 0;
+        float4 r0, r1;
+        if (live) {
+            const uint32_t slot = A.ring[(uint32_t)(head + i) & A.cmask];
+            const unsigned long long g = next + i;
+            // the issue record holding g: the last one that starts at or before it
+            unsigned long long a = lo, b = hi;   // records [a, b); the answer in [a, b)
+            while (b - a > 1) {
+                const unsigned long long mid = a + (b - a) / 2;
+                if (A.iq_start[mid & (LR_ISSUE - 1)] <= g) a = mid;
+                else b = mid;
+            }
+            const uint32_t jslot = A.iq_job[a & (LR_ISSUE - 1)] & (LR_JOBS - 1);
+            const JobDev &J = A.jobs[jslot];
+            const uint32_t q = (uint32_t)(g - J.start);
+            const uint32_t pixel = q % (uint32_t)J.npix;
+            const uint32_t k = J.s0 + q / (uint32_t)J.npix;
+            const int x = (int)(pixel % (uint32_t)J.W), y = (int)(pixel / (uint32_t)J.W);
+            const uint64_t seed = ((uint64_t)J.seed_hi << 32) | J.seed_lo;
+            uint64_t st = pcg_seed((uint64_t)pixel, k, seed);
+            const uint64_t inc = ((uint64_t)pixel << 1u) | 1u;
+            const Smp sm{x, y, k, (uint32_t)seed * 0x9e3779b9u, A.mask, &st, inc};
+            float jx, jy;
+            s2(sm, 0, jx, jy);
+            const float u = ((float)x + jx) / (float)J.W, v = ((float)y + jy) / (float)J.H;
+            // camera_ray (camera.h:38-42)
+            const V3 org = v3(J.cam[0], J.cam[1], J.cam[2]);
+            const V3 ul = v3(J.cam[3], J.cam[4], J.cam[5]), hz = v3(J.cam[6], J.cam[7], J.cam[8]),
+                     vt = v3(J.cam[9], J.cam[10], J.cam[11]);
+            const V3 target = vsub(vadd(ul, vscale(hz, u)), vscale(vt, v));
+            const V3 d = vnorm(vsub(target, org));
+            r0 = make_float4(org.x, org.y, org.z, J.eps);
+            r1 = make_float4(d.x, d.y, d.z, 1e30f);
+            A.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            A.rad[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(EMIT_FULL));
+            A.rng[slot] = st;
+            A.meta[slot] = make_uint2(q, 0u);
+            id = (jslot << LR_SLOT_BITS) | slot;
+        }
+        const uint32_t at = wave_append(live, &ctl->count[cur]);
+        if (live) {
+            A.qray[cur][2 * (size_t)at] = r0;
+            A.qray[cur][2 * (size_t)at + 1] = r1;
+            A.qid[cur][at] = id;
+        }
+    });
+}
+
+// extend (wf_ext, trace_wave.wgsl:72-84): the closest hit of every queued ray, each through
+// its own job's tree. Also applies regen's claim to the ring and the issue counter, and clears
+// the queues the shade kernel appends to.
+template <int BLK, int STK, bool COUNT>
+__global__ __launch_bounds__(BLK) void k_extend(EngArgs A)
+{
+    __shared__ uint32_t stk[STK][BLK];
+    EngCtl *ctl = A.ctl;
+    const int cur = (int)(ctl->iter & 1ull);
+    const uint32_t n = ctl->count[cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned long long c = ctl->claimed;
+        ctl->head += c;
+        ctl->next += c;
+        ctl->claimed = 0;
+        ctl->rays_ext += n;
+        ctl->count[1 - cur] = 0;
+        ctl->count[2] = 0;
+        reset_tiles(ctl, K_EXTEND);
+    }
+    const size_t gtid = (size_t)blockIdx.x * BLK + threadIdx.x;
+    const WalkStack S{&stk[0][threadIdx.x], BLK, A.spill + gtid, (size_t)A.spill_stride, STK + A.spill_per};
+    uint32_t cn = 0, ct = 0;
+    for_tiles<BLK>(&ctl->tiles[K_EXTEND], n, A.mult, [&](uint32_t i, bool live) {
+        if (!live) return;
+        const uint32_t id = A.qid[cur][i];
+        const JobDev &J = A.jobs[id >> LR_SLOT_BITS];
+        const float4 r0 = A.qray[cur][2 * (size_t)i], r1 = A.qray[cur][2 * (size_t)i + 1];
+        const WalkHit h = walk<false, true, COUNT, STK>(J.nodes, J.root, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
+                                                        v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+        A.hits[i] = make_float4(h.t, __uint_as_float(h.id), h.b1, h.b2);
+    });
+    if (COUNT) {
+        atomicAdd(&A.counts[0], (unsigned long long)cn);
+        atomicAdd(&A.counts[1], (unsigned long long)ct);
+    }
+}
+
+// shade (wf_shade, trace_wave.wgsl:191-310): the bounce of trace_path_sampled (tracer.c:382-507)
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
+{
+    EngCtl *ctl = A.ctl;
+    const int cur = (int)(ctl->iter & 1ull);
+    const uint32_t n = ctl->count[cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) reset_tiles(ctl, K_SHADE);
+    for_tiles<BLK>(&ctl->tiles[K_SHADE], n, A.mult, [&](uint32_t i, bool live) {
+        bool next = false, shadow = false;
+        float4 nr0, nr1, sr0, sr1, scon;
+        uint32_t id = 0, slot = 0, jslot = 0, q = 0;
+        float4 rad_out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (live) {
+            id = A.qid[cur][i];
+            slot = id & LR_SLOT_MASK;
+            jslot = id >> LR_SLOT_BITS;
+            const JobDev &J = A.jobs[jslot];
+            const uint2 mt = A.meta[slot];
+            q = mt.x;
+            const int depth = (int)mt.y;
+            const float4 r0 = A.qray[cur][2 * (size_t)i], r1 = A.qray[cur][2 * (size_t)i + 1];
+            const float4 hv = A.hits[i];
+            const float4 thr4 = A.thr[slot], rad4 = A.rad[slot];
+            V3 thr = v3of(thr4), rad = v3of(rad4);
+            float bounce_pdf = thr4.w;
+            int emit = __float_as_int(rad4.w);
+            const uint32_t hid = __float_as_uint(hv.y);
+            const V3 d = v3of(r1), o = v3of(r0);
+            const uint32_t pixel = q % (uint32_t)J.npix;
+            const uint32_t k = J.s0 + q / (uint32_t)J.npix;
+            uint64_t st = A.rng[slot];
+            const uint64_t inc = ((uint64_t)pixel << 1u) | 1u;
+            const uint64_t seed = ((uint64_t)J.seed_hi << 32) | J.seed_lo;
+            const Smp sm{(int)(pixel % (uint32_t)J.W), (int)(pixel / (uint32_t)J.W), k, (uint32_t)seed * 0x9e3779b9u,
+                         A.mask, &st, inc};
+            if (hid != LR_NONE) {
+                // the hit (set_hit, intersect.c:4-10): point, the normal facing the ray
+                const float t = hv.x;
+                const V3 hp = vadd(o, vscale(d, t));
+                V3 outward;
+                int mi;
+                if (hid < J.nprims) {
+                    const Prim pr = load_prim(J.prims, hid);
+                    outward = pr.q.w < 0.5f ? vscale(vsub(hp, v3of(pr.c)), 1.0f / pr.c.w) : v3of(pr.n);
+                    mi = (int)pr.u.w;
+                } else {
+                    const size_t ti = hid - J.nprims;
+                    const float4 tu = J.tris[3 * ti + 1], tv = J.tris[3 * ti + 2];
+                    outward = vnorm(vcross(v3of(tu), v3of(tv)));
+                    mi = (int)(__float_as_uint(tv.w) >> 26);
+                }
+                const bool front = vdot(d, outward) < 0.0f;
+                const V3 nrm = front ? outward : vneg(outward);
+                int type;
+                Bsdf b = bsdf_make(J.mats, mi, front, A.tab, type);
+                if (type == MAT_LIGHT) {
+                    if (front) {
+                        const V3 em = v3of(J.mats[3 * mi + 1]);
+                        if (emit == EMIT_FULL) rad = vadd(rad, vmul(thr, em));
+                        else if (emit == EMIT_MIS && (int)hid == J.light) {
+                            const Prim l = load_prim(J.prims, (uint32_t)J.light);
+                            const float cos_light = -vdot(v3of(l.n), d);
+                            if (cos_light > 0.0f) {
+                                const float w = power_heuristic(bounce_pdf, (t * t) / (cos_light * l.v.w));
+                                rad = vadd(rad, vmul(thr, vscale(em, w)));
+                            }
+                        }
+                    }
+                } else if (depth < J.max_depth) {
+                    const V3 v = vneg(vnorm(d));
+                    // 1. next event estimation to the lamp (sample_direct_light, tracer.c:56-81),
+                    //    its visibility left to the shadow kernel
+                    const bool rough_part = !b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f;
+                    if (rough_part && J.light >= 0) {
+                        float la, lb;
+                        s2(sm, 1 + 3 * depth, la, lb);
+                        const float ua = lb, ub = la;   // the pair's second number first (tracer.c:50-53)
+                        const Prim l = load_prim(J.prims, (uint32_t)J.light);
+                        const V3 le = v3of(J.mats[3 * (int)l.u.w + 1]);
+                        const V3 xl = vadd(v3of(l.q), vadd(vscale(v3of(l.u), ua), vscale(v3of(l.v), ub)));
+                        const V3 to_light = vsub(xl, hp);
+                        const float dist2 = vdot(to_light, to_light);
+                        const float dist = sqrtf(dist2);
+                        const V3 wi = vscale(to_light, 1.0f / dist);
+                        const float cos_surf = vdot(nrm, wi);
+                        const float cos_light = -vdot(v3of(l.n), wi);
+                        if (cos_light > 0.0f) {
+                            float bpdf;
+                            const V3 f = bsdf_eval(b, A.tab, nrm, v, wi, bpdf);
+                            if (vmax3(f) > 0.0f) {
+                                float g = fabsf(cos_surf) * cos_light * l.v.w / dist2;
+                                if (J.mis) g *= power_heuristic(dist2 / (cos_light * l.v.w), bpdf);
+                                const V3 c = vmul(thr, vscale(vmul(f, le), g));
+                                shadow = true;
+                                sr0 = make_float4(hp.x, hp.y, hp.z, J.eps);
+                                sr1 = make_float4(wi.x, wi.y, wi.z, dist - J.margin);
+                                scon = make_float4(c.x, c.y, c.z, 0.0f);
+                            }
+                        }
+                    }
+                    // 2. the next direction from the BSDF, Russian roulette from depth 3
+                    bsdf_view(b, A.tab, vdot(v, nrm));
+                    float ur = 0.0f, ul = 0.0f, da = 0.0f, db = 0.0f;
+                    const int nl = (b.p[0] > 0.0f) + (b.p[1] > 0.0f) + (b.p[2] > 0.0f) + (b.p[3] > 0.0f);
+                    if (depth >= 3 || nl > 1) s2(sm, 3 + 3 * depth, ur, ul);
+                    const int lobe = bsdf_pick(b, ul);
+                    if (lobe >= 0 && !(b.smooth && (lobe == SPEC || lobe == TRANS))) s2(sm, 2 + 3 * depth, da, db);
+                    BSample bs;
+                    if (bsdf_sample_lobe(b, A.tab, lobe, nrm, v, da, db, bs)) {
+                        thr = vmul(thr, bs.w);
+                        if (bs.delta) emit = EMIT_FULL;
+                        else {
+                            bounce_pdf = bs.pdf;
+                            emit = J.mis ? EMIT_MIS : EMIT_NONE;
+                        }
+                        bool go = true;
+                        if (depth >= 3) {
+                            const float pr = rfmin(vmax3(thr), 0.95f);
+                            if (ur >= pr) go = false;
+                            else thr = vscale(thr, 1.0f / pr);
+                        }
+                        if (go) {
+                            const V3 nd = J.mis ? vnorm(bs.l) : bs.l;
+                            next = true;
+                            nr0 = make_float4(hp.x, hp.y, hp.z, J.eps);
+                            nr1 = make_float4(nd.x, nd.y, nd.z, 1e30f);
+                        }
+                    }
+                }
+            }
+            rad_out = make_float4(rad.x, rad.y, rad.z, __int_as_float(emit));
+            if (next) {
+                A.thr[slot] = make_float4(thr.x, thr.y, thr.z, bounce_pdf);
+                A.rng[slot] = st;
+                A.meta[slot] = make_uint2(q, (uint32_t)depth + 1u);
+            }
+            if (next || shadow) A.rad[slot] = rad_out;
+            if (shadow && !next) scon.w = 1.0f;   // the shadow kernel ends the path
+        }
+        const uint32_t at = wave_append(next, &ctl->count[1 - cur]);
+        if (next) {
+            A.qray[1 - cur][2 * (size_t)at] = nr0;
+            A.qray[1 - cur][2 * (size_t)at + 1] = nr1;
+            A.qid[1 - cur][at] = id;
+        }
+        const uint32_t ss = wave_append(shadow, &ctl->count[2]);
+        if (shadow) {
+            A.sray[2 * (size_t)ss] = sr0;
+            A.sray[2 * (size_t)ss + 1] = sr1;
+            A.sid[ss] = id;
+            A.scon[ss] = scon;
+        }
+        retire(A, live && !next && !shadow, jslot, slot, q, rad_out);
+    });
+}
+
+// shadow (wf_shadow, trace_wave.wgsl:335-355): any hit before the lamp; if none, the
+// contribution joins the path's radiance. Paths whose last vertex made the shadow ray end here.
+template <int BLK, int STK>
+__global__ __launch_bounds__(BLK) void k_shadow(EngArgs A)
+{
+    __shared__ uint32_t stk[STK][BLK];
+    EngCtl *ctl = A.ctl;
+    const uint32_t n = ctl->count[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        reset_tiles(ctl, K_SHADOW);
+        ctl->rays_shadow += n;
+        ctl->iter += 1;
+    }
+    const size_t gtid = (size_t)blockIdx.x * BLK + threadIdx.x;
+    const WalkStack S{&stk[0][threadIdx.x], BLK, A.spill + gtid, (size_t)A.spill_stride, STK + A.spill_per};
+    for_tiles<BLK>(&ctl->tiles[K_SHADOW], n, A.mult, [&](uint32_t i, bool live) {
+        bool ends = false;
+        uint32_t slot = 0, jslot = 0, q = 0;
+        float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (live) {
+            const uint32_t id = A.sid[i];
+            slot = id & LR_SLOT_MASK;
+            jslot = id >> LR_SLOT_BITS;
+            const JobDev &J = A.jobs[jslot];
+            const float4 r0 = A.sray[2 * (size_t)i], r1 = A.sray[2 * (size_t)i + 1];
+            uint32_t cn, ct;
+            const WalkHit h = walk<true, true, false, STK>(J.nodes, J.root, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
+                                                           v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+            const float4 c = A.scon[i];
+            ends = c.w != 0.0f;
+            const bool lit = h.id == LR_NONE;
+            if (lit || ends) {
+                r = A.rad[slot];
+                if (lit) {
+                    r.x += c.x;
+                    r.y += c.y;
+                    r.z += c.z;
+                }
+                if (!ends) A.rad[slot] = r;
+                else q = A.meta[slot].x;
+            }
+        }
+        retire(A, ends, jslot, slot, q, r);
+    });
+}
+
+template <int BLK>
+void L_regen(dim3 g, hipStream_t s, const EngArgs &A)
+{
+    hipLaunchKernelGGL(k_regen<BLK>, g, dim3(BLK), 0, s, A);
+}
+template <int BLK>
+void L_shade(dim3 g, hipStream_t s, const EngArgs &A)
+{
+    hipLaunchKernelGGL(k_shade<BLK>, g, dim3(BLK), 0, s, A);
+}
+template <int BLK, int STK, bool COUNT>
+void L_extend(dim3 g, hipStream_t s, const EngArgs &A)
+{
+    hipLaunchKernelGGL((k_extend<BLK, STK, COUNT>), g, dim3(BLK), 0, s, A);
+}
+template <int BLK, int STK>
+void L_shadow(dim3 g, hipStream_t s, const EngArgs &A)
+{
+    hipLaunchKernelGGL((k_shadow<BLK, STK>), g, dim3(BLK), 0, s, A);
+}
+
+template <int BLK>
+void fill_table(LrEngTable &t)
+{
+    const int b = lr_blk_index(BLK);
+    t.regen[b] = L_regen<BLK>;
+    t.shade[b] = L_shade<BLK>;
+    t.extend[b][0] = L_extend<BLK, 16, false>;
+    t.extend[b][1] = L_extend<BLK, 24, false>;
+    t.extend[b][2] = L_extend<BLK, 32, false>;
+    t.extend_count[b][0] = L_extend<BLK, 16, true>;
+    t.extend_count[b][1] = L_extend<BLK, 24, true>;
+    t.extend_count[b][2] = L_extend<BLK, 32, true>;
+    t.shadow[b][0] = L_shadow<BLK, 16>;
+    t.shadow[b][1] = L_shadow<BLK, 24>;
+    t.shadow[b][2] = L_shadow<BLK, 32>;
+}
+
+}  // namespace

@@ -32,6 +32,7 @@ struct LrhContext {
     hipEvent_t ev[2] = {nullptr, nullptr};
     int ploc_small = 1024;
     float last_build_ms = 0.0f;
+    LrhEngine *engine = nullptr;  // lrh_pt_render's engine, made on first use
 };
 
 // error reporting: a thread-local message, negative codes (lr_hip.h)

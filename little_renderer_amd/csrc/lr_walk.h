@@ -116,8 +116,8 @@ struct WalkHit {
     float b1, b2;
 };
 
-// The stack of one ray: LDS slots stk[k * stride] for k < LR_LDS_STACK, then
-// spill[(k - LR_LDS_STACK) * sstride] up to cap entries in all.
+// The stack of one ray: LDS slots stk[k * stride] for k < STK (the walk's template
+// argument), then spill[(k - STK) * sstride] up to cap entries in all.
 struct WalkStack {
     uint32_t *stk;
     int stride;
@@ -129,7 +129,7 @@ struct WalkStack {
 // The closest hit in (tmin, tmax), or (ANY) the first one, over shape ids taken from the
 // leaves' aux entries: ids below nprims are primitives, the rest triangles (id - nprims).
 // COUNT adds the node records fetched and the shapes tested to cn, ct.
-template <bool ANY, bool PRIMS, bool COUNT>
+template <bool ANY, bool PRIMS, bool COUNT, int STK = LR_LDS_STACK>
 __device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32_t root, const uint32_t *__restrict__ aux,
                                         const float4 *__restrict__ tris, const float4 *__restrict__ prims, uint32_t nprims,
                                         V3 o, V3 d, float tmin, float tmax, const WalkStack &S, uint32_t *overflow,
@@ -139,14 +139,14 @@ __device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32
     WalkHit best{tmax, LR_NONE, 0.0f, 0.0f};
     int sp = 0;
     auto push = [&](uint32_t x) {
-        if (sp < LR_LDS_STACK) S.stk[sp * S.stride] = x;
-        else S.spill[(size_t)(sp - LR_LDS_STACK) * S.sstride] = x;
+        if (sp < STK) S.stk[sp * S.stride] = x;
+        else S.spill[(size_t)(sp - STK) * S.sstride] = x;
         sp++;
     };
     push(root);
     while (sp > 0) {
         sp--;
-        const uint32_t ni = sp < LR_LDS_STACK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - LR_LDS_STACK) * S.sstride];
+        const uint32_t ni = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
         const float4 nlo = nodes[2 * (size_t)ni], nhi = nodes[2 * (size_t)ni + 1];
         const uint32_t na = __float_as_uint(nlo.w), nb = __float_as_uint(nhi.w);
         if (COUNT) cn++;

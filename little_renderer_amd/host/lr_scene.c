@@ -367,3 +367,69 @@ void lrs_cornell(uint64_t seed, int w, int h, float *tris, float *prims, LrhMate
     const double from[3] = {278 * sc, 278 * sc, -800 * sc}, to[3] = {278 * sc, 278 * sc, 0}, up[3] = {0, 1, 0};
     lrs_camera_make(from, to, up, 40.0, (double)w / (double)h, cam);
 }
+
+/* ---- the batched and serving workloads ---- */
+void lrs_batch_scene(int n, uint64_t seed, float *tris, float *prims, LrhMaterial *mats)
+{
+    Pcg r;
+    pcg_seed(&r, seed, 3);
+    for (int i = 0; i < LRS_BATCH_SOUP_MATS; i++) {
+        LrhMaterial *m = &mats[i];
+        memset(m, 0, sizeof(*m));
+        for (int k = 0; k < 3; k++) m->albedo[k] = (float)(0.2 + 0.7 * pcg_unit(&r));
+        m->roughness = (float)(0.05 + 0.95 * pcg_unit(&r));
+        m->metallic = pcg_unit(&r) < 0.4 ? 1.0f : 0.0f;
+        m->ior = 1.5f;
+        m->type = LRH_MAT_PBR;
+        if (i % 7 == 6) {           /* two of the fourteen: clear glass */
+            m->type = LRH_MAT_GLASS;
+            m->albedo[0] = m->albedo[1] = m->albedo[2] = 0.95f;
+            m->metallic = 0.0f;
+            m->roughness = 0.0f;
+            m->transmission = 1.0f;
+        }
+    }
+    LrhMaterial *fl = &mats[LRS_BATCH_SOUP_MATS], *lm = &mats[LRS_BATCH_SOUP_MATS + 1];
+    memset(fl, 0, sizeof(*fl));
+    fl->type = LRH_MAT_DIFFUSE;
+    fl->albedo[0] = fl->albedo[1] = fl->albedo[2] = 0.5f;
+    fl->ior = 1.5f;
+    fl->roughness = 1.0f;
+    memset(lm, 0, sizeof(*lm));
+    lm->type = LRH_MAT_LIGHT;
+    lm->emission[0] = lm->emission[1] = lm->emission[2] = 15.0f;
+    lm->ior = 1.0f;
+    lm->roughness = 1.0f;
+    lrs_gen_soup(n, seed, LRS_BATCH_SOUP_MATS, tris);
+    const double fq[3] = {-2.0, -0.05, -2.0}, fu[3] = {0.0, 0.0, 5.0}, fv[3] = {5.0, 0.0, 0.0};
+    put_quad(tris, n, fq, fu, fv, LRS_BATCH_SOUP_MATS);
+    const float lq[3] = {0.75f, 1.6f, 0.75f}, lu[3] = {-0.5f, 0.0f, 0.0f}, lv[3] = {0.0f, 0.0f, -0.5f};
+    lrs_make_quad(lq, lu, lv, LRS_BATCH_SOUP_MATS + 1, prims);
+}
+
+void lrs_batch_camera(uint64_t seed, int w, int h, float cam[12])
+{
+    Pcg r;
+    pcg_seed(&r, seed, 4);
+    double d[3];
+    do random_dir(&r, d); while (d[1] < 0.15);
+    const double at[3] = {0.5, 0.5, 0.5}, up[3] = {0, 1, 0};
+    const double from[3] = {0.5 + 2.2 * d[0], 0.5 + 2.2 * d[1], 0.5 + 2.2 * d[2]};
+    lrs_camera_make(from, at, up, 40.0, (double)w / (double)h, cam);
+}
+
+void lrs_batch_plan(int njobs, uint64_t seed, int spp_lo, int *ntris, int *w, int *h, int *spp)
+{
+    static const int res[4][2] = {{256, 256}, {512, 512}, {1280, 720}, {1920, 1080}};
+    static const int spps[4] = {1, 4, 16, 64};
+    Pcg r;
+    pcg_seed(&r, seed, 5);
+    const int first = spp_lo == 4 ? 1 : 0;
+    for (int i = 0; i < njobs; i++) {
+        ntris[i] = (int)floor(exp(log(1000.0) + (log(500000.0) - log(1000.0)) * pcg_unit(&r)) + 0.5);
+        const int k = (int)(pcg_u32(&r) % 4u);
+        w[i] = res[k][0];
+        h[i] = res[k][1];
+        spp[i] = spps[first + (int)(pcg_u32(&r) % (uint32_t)(4 - first))];
+    }
+}

@@ -1,0 +1,217 @@
+"""GPU tests of the engine (many jobs over one pool of path slots) and of the serving loop.
+
+A job's samples depend only on (its seed, pixel, sample index), and each sample joins its pixel as
+a fixed-point integer, so a job's image must be the same to the bit whatever else runs beside it,
+however the jobs are launched (the plain per-job form, the batched graph, any launch shape) and
+however few slots the engine has. These tests check exactly that (bit-identical images), that the
+engine's image equals the oracle's sample for sample, and that serving completes every request
+with the same images under a memory budget that admits jobs one at a time.
+"""
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+import little_renderer_amd as lr
+from oracle import lro
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    assert torch.cuda.is_available(), "these tests need the GPU"
+    c = lr.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def scenes(ctx):
+    """the Cornell box and three small scenes of the batched workload"""
+    out = []
+    cb = lr.cornell(64, 64)
+    out.append((lr.PtScene(ctx, cb["prims"], cb["tris"], cb["mats"], 0, cb["eps"], cb["margin"]), cb))
+    for n, seed in ((1000, 11), (6000, 12), (30000, 13)):
+        sc = lr.batch_scene(n, seed)
+        out.append((lr.PtScene(ctx, sc["prims"], sc["tris"], sc["mats"], 0, sc["eps"], sc["margin"]), sc))
+    yield out
+    for ps, _ in out:
+        ps.close()
+
+
+def job_set(scenes):
+    """a mix of scenes, resolutions, sample counts, seeds and depths"""
+    jobs = []
+    specs = [(0, 64, 64, 8, 0, 8), (1, 96, 40, 4, 3, 8), (2, 50, 70, 16, 5, 6), (3, 128, 128, 2, 9, 8),
+             (0, 33, 17, 5, 2, 3), (2, 200, 120, 1, 7, 8), (1, 64, 64, 3, 1, 0)]
+    for k, (si, w, h, spp, seed, depth) in enumerate(specs):
+        ps, sc = scenes[si]
+        cam = lr.cornell(w, h)["cam"] if si == 0 else lr.batch_camera(100 + k, w, h)
+        jobs.append(lr.make_job(ps, cam, w, h, spp, s0=k, seed=seed, max_depth=depth, mis=(k % 3 != 2)))
+    return jobs
+
+
+def images(eng, ids, jobs):
+    out = []
+    for i, j in zip(ids, jobs):
+        t = torch.zeros((j.h, j.w, 4), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        eng.image(i, t)
+        out.append(t.cpu().numpy())
+        eng.release(i)
+    return out
+
+
+def run_batched(eng, jobs):
+    ids = [eng.submit(j) for j in jobs]
+    assert all(i is not None for i in ids)
+    eng.drain()
+    return images(eng, ids, jobs)
+
+
+def run_plain(eng, jobs):
+    return images(eng, [eng.plain(j) for j in jobs], jobs)
+
+
+def same(a, b):
+    return all(np.array_equal(x.view(np.uint32), y.view(np.uint32)) for x, y in zip(a, b))
+
+
+def test_engine_matches_oracle_sample_for_sample(ctx, scenes):
+    ps, cb = scenes[0]
+    say("engine: Cornell 48x48x16 vs oracle")
+    eng = lr.Engine(ctx, slots_log2=16, max_tree_depth=64, image_bytes=64 << 20)
+    job = lr.make_job(ps, lr.cornell(48, 48)["cam"], 48, 48, 16)
+    g = run_batched(eng, [job])[0]
+    eng.close()
+    assert np.all(g[..., 3] == 16)
+    o = lro.pt_image(lr.cornell(48, 48), 48, 48, 0, 16, tab=lr.brdf_tables(), mask=lro.blue_noise())
+    rel = np.abs(g[..., :3] - o) / np.maximum(np.abs(o), 1e-3)
+    say(f"  median rel {np.median(rel):.2e}, max {rel.max():.2e}")
+    assert np.median(rel) < 1e-5 and np.mean(rel > 1e-3) < 0.01
+
+
+def test_batched_equals_plain_bit_for_bit(ctx, scenes):
+    jobs = job_set(scenes)
+    say(f"engine: {len(jobs)} jobs batched vs plain")
+    eng = lr.Engine(ctx, slots_log2=17, max_tree_depth=64, image_bytes=256 << 20)
+    a = run_batched(eng, jobs)
+    s1 = eng.stats()
+    b = run_plain(eng, jobs)
+    s2 = eng.stats()
+    eng.close()
+    assert same(a, b)
+    for j, im in zip(jobs, a):
+        assert np.all(im[..., 3] == j.spp)
+        assert np.isfinite(im).all()
+        assert j.max_depth == 0 or im[..., :3].sum() > 0   # depth 0: only a lamp seen directly
+    # the same paths: the same rays, counted on the device
+    assert s2.rays_ext - s1.rays_ext == s1.rays_ext and s2.rays_shadow - s1.rays_shadow == s1.rays_shadow
+    say(f"  rays {s1.rays_ext} + {s1.rays_shadow} shadow")
+
+
+def test_alone_and_in_the_mix(ctx, scenes):
+    jobs = job_set(scenes)
+    say("engine: each job alone vs all together")
+    eng = lr.Engine(ctx, slots_log2=16, max_tree_depth=64, image_bytes=256 << 20)
+    mix = run_batched(eng, jobs)
+    alone = [run_batched(eng, [j])[0] for j in jobs]
+    # and in another order, the pool small enough that paths of many jobs share every iteration
+    rev = run_batched(eng, jobs[::-1])[::-1]
+    eng.close()
+    assert same(mix, alone)
+    assert same(mix, rev)
+
+
+def test_few_slots_and_every_launch_shape(ctx, scenes):
+    jobs = job_set(scenes)[:4]
+    say("engine: launch shapes and a pool of 1024 slots")
+    big = lr.Engine(ctx, slots_log2=18, max_tree_depth=64, image_bytes=128 << 20)
+    ref = run_batched(big, jobs)
+    big.close()
+    small = lr.Engine(ctx, slots_log2=10, max_tree_depth=64, image_bytes=128 << 20)
+    assert same(ref, run_batched(small, jobs))
+    small.close()
+    eng = lr.Engine(ctx, slots_log2=16, max_tree_depth=64, image_bytes=128 << 20)
+    shapes = [dict(block_trace=64, stack=16, block_shade=256, blocks_per_cu=1, tile=1, iters_per_graph=1),
+              dict(block_trace=256, stack=32, block_shade=64, blocks_per_cu=16, tile=8, iters_per_graph=64),
+              dict(block_trace=128, stack=24, block_shade=128, blocks_per_cu=2, tile=4, iters_per_graph=3)]
+    for t in shapes:
+        eng.tune = t
+        assert same(ref, run_batched(eng, jobs)), t
+        assert same(ref, run_plain(eng, jobs)), t
+    eng.close()
+
+
+def test_autotune_keeps_images(ctx, scenes):
+    jobs = job_set(scenes)
+    say("engine: autotune")
+    eng = lr.Engine(ctx, slots_log2=16, max_tree_depth=64, image_bytes=256 << 20)
+    ref = run_batched(eng, jobs)
+    sample = [lr.make_job(scenes[3][0], lr.batch_camera(7, 128, 128), 128, 128, 1, seed=4)]
+    best, mrays = eng.autotune(sample)
+    say(f"  best {best.as_dict()} at {mrays:.1f} Mrays/s")
+    assert mrays > 0 and eng.tune.as_dict() == best.as_dict()
+    assert same(ref, run_batched(eng, jobs))
+    eng.close()
+
+
+def test_counting_walks(ctx, scenes):
+    jobs = job_set(scenes)[:3]
+    eng = lr.Engine(ctx, slots_log2=16, max_tree_depth=64, image_bytes=64 << 20)
+    ref = run_batched(eng, jobs)
+    eng.counting(True)
+    got = run_batched(eng, jobs)
+    st = eng.stats()
+    eng.close()
+    assert same(ref, got)
+    assert st.node_fetches > st.rays_ext // 2 and st.shape_tests > 0
+
+
+def test_serving_same_results_under_a_tight_budget(ctx, scenes):
+    jobs = job_set(scenes)
+    say("serving: threads submit, a budget that fits one image at a time")
+    eng = lr.Engine(ctx, slots_log2=16, max_tree_depth=64, image_bytes=256 << 20)
+    ref = run_batched(eng, jobs)
+    eng.close()
+    biggest = max(j.w * j.h * 24 for j in jobs)
+    srv = lr.Server(ctx, slots_log2=16, max_tree_depth=64, budget=1 << 30)
+    for ps, _ in scenes:
+        srv.add_scene(ps)
+    st0 = srv.stats()
+    srv.close()
+    # the budget: the scenes plus room for the largest image (aligned) and a little less than two
+    budget = st0.in_use + biggest + 4096
+    srv = lr.Server(ctx, slots_log2=16, max_tree_depth=64, budget=budget)
+    for ps, _ in scenes:
+        srv.add_scene(ps)
+    ids = [None] * len(jobs)
+    got = [None] * len(jobs)
+
+    def client(k):
+        ids[k] = srv.submit(jobs[k])
+        info = srv.wait(ids[k], 120000.0)
+        assert info is not None and info.state == lr.JOB_DONE
+        assert info.t_done_ms >= info.t_submit_ms
+        got[k] = srv.image(ids[k], jobs[k].w, jobs[k].h)
+        srv.release(ids[k])
+
+    th = [threading.Thread(target=client, args=(k,)) for k in range(len(jobs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(180)
+    st = srv.stats()
+    srv.close()
+    say(f"  done {st.done}, p50 {st.p50_ms:.2f} ms, p99 {st.p99_ms:.2f} ms, rays {st.rays}")
+    assert st.done == len(jobs) and st.submitted == len(jobs)
+    assert st.p99_ms >= st.p50_ms > 0
+    assert same(ref, got)
