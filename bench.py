@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
 """bench.py - little_renderer's path-tracing hot path on one MI355X.
 
-Workload (one step, all inputs resident in HBM before the timed region): the BVH-stress scene
-of 1,000,000 seeded triangles (centroids uniform in the unit cube, log-normal edges around
-0.002): an LBVH build and a PLOC build (radius 16) from the triangles' boxes, then 1920x1080
-pinhole primaries traced (closest hit) through the PLOC tree, and for each primary hit one
-cosine-weighted diffuse bounce (closest hit) and one shadow ray toward a fixed point light
-(any hit). value = rays traced per second over the whole step (builds included), in Mrays/s.
+Default workload (--workload batched): the metric's own form. One step = a batch of 64 render
+jobs, each its own resident scene (seeded triangle soup of 1k-500k triangles, 14 GGX materials,
+floor, quad lamp), camera, resolution (256^2 .. 1920x1080) and spp (1 .. 64), traced together
+by the engine (persistent wavefront kernels replayed as one hipGraph, autotuned launch shape).
+value = rays traced (closest-hit + shadow, counted on the device) per second over the timed
+batches. Beside it in the same JSON line: the same jobs in the plain per-job form (the
+reference's shape) with the images compared bit for bit, p50/p99 job latency of both, each
+kernel's time from HIP events on the engine's stream, the roofline of the dominant kernel (the
+extension walk: algorithmic bytes from its own counters over its time), serving under Poisson
+arrivals at several rates, and the reference's CPU path tracer (compiled from its sources into
+oracle/_ref) on a sample of the same jobs on this host's cores.
 
-Prints one JSON line (rank 0). Also: each kernel's time from HIP events recorded on the stream
-the kernels run on, the roofline of the dominant kernel (its algorithmic bytes from the walk's
-own fetch counters / its average time), and the reference's CPU path (its own SAH build and
-4-wide walk, compiled from its sources into oracle/_ref) timed on a bounded sample of the same
-rays on this host's cores.
+--workload bvh_stress: the 1M-triangle BVH-stress step (LBVH + PLOC builds, 1920x1080 primaries,
+a diffuse bounce and a shadow ray per hit), with its own roofline and CPU baseline.
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -28,8 +32,12 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (batched: 2 batches; bvh_stress: 10)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (batched: 1; bvh_stress: 3)")
+    ap.add_argument("--workload", choices=["batched", "bvh_stress"], default="batched")
+    ap.add_argument("--jobs", type=int, default=64, help="batched: jobs (and scenes) in a batch")
+    ap.add_argument("--serve-requests", type=int, default=40, help="serving: requests per arrival rate (0: skip)")
+    ap.add_argument("--serve-rates", default="50,100,200,400", help="serving: Poisson arrival rates, jobs/s")
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
@@ -43,8 +51,8 @@ def parse():
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (the path would shard by "
-              f"scene or image tile, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (the work would shard by "
+              f"resident scene or image tile across GPUs; only the one-GPU path is built)", file=sys.stderr)
         sys.exit(2)
     import torch
     import little_renderer_amd as lr
@@ -58,6 +66,17 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.copy_probe << 20, "launches": 3}), flush=True)
         return
+    if a.workload == "bvh_stress":
+        out = bench_bvh_stress(a, torch, lr)
+    else:
+        out = bench_batched(a, torch, lr)
+    print(json.dumps(out), flush=True)
+
+
+def bench_bvh_stress(a, torch, lr):
+    """the BVH-stress workload: builds and one plain launch sequence of traces per step"""
+    a.steps = a.steps or 10
+    a.warmup = 3 if a.warmup is None else a.warmup
     ctx = lr.Context(0)
     stream = torch.cuda.ExternalStream(ctx.stream)
     n, W, H = a.tris, a.width, a.height
@@ -176,7 +195,285 @@ def main():
     # ---- the CPU baseline: the reference's own CPU path (SAH build, 4-wide walk) ----
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tris_np, rays0, bounce, shadow, nsec, a.cpu_seconds)
-    print(json.dumps(out), flush=True)
+    return out
+
+
+METRIC = "batched Mrays/s + p50/p99 job latency vs plain per-job launch, equal RMSE"
+WORKLOAD_BATCHED = ("batched: 64 seeded scenes (triangle soup of 1k-500k triangles, log-uniform; 14 GGX materials "
+                    "incl. glass; floor; quad lamp), one job each at a resolution from {256^2, 512^2, 1280x720, "
+                    "1920x1080} and spp from {1, 4, 16, 64}, max depth 8, NEE + MIS; all jobs in one engine "
+                    "(persistent wavefront kernels, hipGraph of iterations, autotuned launch shape)")
+
+
+def _align(x):
+    return (x + 255) & ~255
+
+
+def percentile(v, p):
+    return float(np.percentile(np.asarray(v, np.float64), p)) if len(v) else None
+
+
+def build_pool(ctx, lr, plan):
+    """one resident scene per job of the plan (scene i: lrs_batch_scene(ntris_i, seed i))"""
+    pool = []
+    for i, (n, w, h, spp) in enumerate(plan):
+        sc = lr.batch_scene(n, i)
+        pool.append(lr.PtScene(ctx, sc["prims"], sc["tris"], sc["mats"], 0, sc["eps"], sc["margin"]))
+    return pool
+
+
+def bench_batched(a, torch, lr):
+    """The batched workload: one step = every job of the batch through the engine (submit all,
+    drain); value = rays traced (closest-hit + shadow, counted on the device) per second of
+    wall time over the timed steps. Beside it: the same jobs in the plain per-job form (the
+    reference's shape: one job at a time, 1-spp passes, the host waiting for each job), checked
+    bit-identical image by image; job latencies; the kernels' times and the roofline of the
+    dominant one; serving under Poisson arrivals; the reference CPU path tracer on a sample."""
+    steps = a.steps or 2
+    warm = 1 if a.warmup is None else a.warmup
+    ctx = lr.Context(0)
+    plan = lr.batch_plan(a.jobs, 7)
+    t0 = time.perf_counter()
+    pool = build_pool(ctx, lr, plan)
+    build_s = time.perf_counter() - t0
+    info = [ps.info() for ps in pool]
+    depth = max(d for d, _, _ in info)
+    jobs = [lr.make_job(pool[i], lr.batch_camera(1000 + i, w, h), w, h, spp, seed=i, max_depth=8)
+            for i, (n, w, h, spp) in enumerate(plan)]
+    paths = sum(w * h * spp for (_, w, h, spp) in plan)
+    img_bytes = sum(_align(w * h * 24) for (_, w, h, _) in plan)
+    big = max(_align(w * h * 24) for (_, w, h, _) in plan)
+    eng = lr.Engine(ctx, slots_log2=22, max_tree_depth=depth, image_bytes=img_bytes + big + (64 << 20))
+    sample = [lr.make_job(pool[i], lr.batch_camera(1000 + i, w, h), w, h, 1, seed=i, max_depth=8)
+              for i, (n, w, h, spp) in enumerate(plan[:16])]
+    t0 = time.perf_counter()
+    tune, tune_mr = eng.autotune(sample)
+    tune_s = time.perf_counter() - t0
+
+    def batch(keep=False):
+        s0 = eng.stats()
+        t = lr.now_ms()
+        ids = [eng.submit(j) for j in jobs]
+        assert all(i is not None for i in ids)
+        eng.drain()
+        t1 = lr.now_ms()
+        s1 = eng.stats()
+        lat = [eng.job(i).t_done_ms - t for i in ids]
+        if not keep:
+            for i in ids:
+                eng.release(i)
+        rays = (s1.rays_ext - s0.rays_ext) + (s1.rays_shadow - s0.rays_shadow)
+        return ids, t1 - t, rays, s1.rays_ext - s0.rays_ext, lat, s1.iterations - s0.iterations
+
+    for _ in range(warm):
+        batch()
+    # ---- the timed region: exactly K batches between full synchronizations ----
+    torch.cuda.synchronize()
+    rays = 0
+    lats, iters = [], 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ids, ms, r, rext, lat, it = batch(keep=(k == steps - 1))
+        rays += r
+        lats += lat
+        iters += it
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = rays / dt / 1e6
+    rays_step = rays // steps
+
+    # ---- plain per-job launches, and the images compared bit for bit ----
+    same = 0
+    plain_lat, pr0, plain_s = [], eng.stats(), 0.0
+    for j, i in zip(jobs, ids):
+        ta = time.perf_counter()
+        pid = eng.plain(j)                      # returns when the job is done (the host waited)
+        plain_s += time.perf_counter() - ta
+        plain_lat.append(plain_s * 1e3)        # one job after another: latency from the batch's start
+        x = torch.empty((j.h, j.w, 4), dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        eng.image(i, x)
+        eng.image(pid, y)
+        same += int(torch.equal(x.view(torch.int32), y.view(torch.int32)))
+        eng.release(pid)
+        eng.release(i)
+    pr1 = eng.stats()
+    plain_rays = (pr1.rays_ext - pr0.rays_ext) + (pr1.rays_shadow - pr0.rays_shadow)
+    plain_mrays = plain_rays / plain_s / 1e6
+    # the plain form in shortest-first order instead of the batch's (the same per-job times)
+    dur = np.diff(np.concatenate([[0.0], plain_lat]))
+    order = np.argsort([w * h * spp for (_, w, h, spp) in plan], kind="stable")
+    plain_sjf = np.cumsum(dur[order])
+
+    # ---- kernel times (events around each kernel, iterations launched without the graph) and the
+    # roofline of the dominant kernel: the extension walk's algorithmic bytes from its own counters
+    # (node records x 32 B, shapes x 52 B, ray 32 B + hit 16 B per ray) over its time ----
+    eng.profiling(True)
+    k0 = eng.stats()
+    batch()
+    k1 = eng.stats()
+    eng.profiling(False)
+    names = ["regen", "extend", "shade", "shadow"]
+    kms = {n: k1.kernel_ms[q] - k0.kernel_ms[q] for q, n in enumerate(names)}
+    kl = {n: k1.kernel_launches[q] - k0.kernel_launches[q] for q, n in enumerate(names)}
+    eng.counting(True)
+    c0 = eng.stats()
+    batch()
+    c1 = eng.stats()
+    eng.counting(False)
+    nodes = c1.node_fetches - c0.node_fetches
+    shapes = c1.shape_tests - c0.shape_tests
+    rext = c1.rays_ext - c0.rays_ext
+    alg = 48 * rext + 32 * nodes + 52 * shapes
+    dom = max(kms, key=kms.get)
+    achieved = alg / (kms["extend"] * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": round(dt * 1e3 / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scenes, cameras and job mix generated on the host; random-init materials)",
+        "config": {"workload": WORKLOAD_BATCHED, "jobs": len(jobs), "paths_per_step": paths,
+                   "rays_per_step": rays_step, "triangles": sum(n for n, _, _, _ in plan), "max_tree_depth": depth,
+                   "slots": 1 << 22, "tune": tune.as_dict(), "tune_mrays_s": round(tune_mr, 1),
+                   "tune_s": round(tune_s, 2), "scene_build_s": round(build_s, 2)},
+        "batched": {"mrays_s": round(value, 3), "mpaths_s": round(paths * steps / dt / 1e6, 3),
+                    "p50_ms": round(percentile(lats, 50), 2), "p99_ms": round(percentile(lats, 99), 2),
+                    "iterations_per_step": iters // steps},
+        "plain": {"mrays_s": round(plain_mrays, 3), "s": round(plain_s, 3), "p50_ms": round(percentile(plain_lat, 50), 2),
+                  "p99_ms": round(percentile(plain_lat, 99), 2), "rays": plain_rays,
+                  "shortest_first_p50_ms": round(percentile(plain_sjf, 50), 2),
+                  "shortest_first_p99_ms": round(percentile(plain_sjf, 99), 2)},
+        "speedup_vs_plain": round(value / plain_mrays, 3),
+        "equal_error": {"images_bit_identical": same, "of": len(jobs),
+                        "note": "same samples per (job, pixel, index), fixed-point sums: equal RMSE by construction"},
+        "kernel_ms_per_launch": {n: round(kms[n] / max(kl[n], 1), 4) for n in names},
+        "kernel_ms_total": {n: round(kms[n], 2) for n in names},
+        "roofline": {"bound": "hbm", "kernel": "extend", "dominant": dom, "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": alg // max(kl["extend"], 1), "node_fetches": nodes, "shape_tests": shapes,
+                     "rays": rext, "launches": kl["extend"]},
+    }
+    if a.serve_requests > 0:
+        rates = [float(x) for x in a.serve_rates.split(",") if x]
+        # serving answers sooner with shorter graphs (a new job joins at the next graph)
+        stune = dict(tune.as_dict(), iters_per_graph=2)
+        out["serving"] = bench_serving(ctx, lr, pool[:32], depth, stune, rates, a.serve_requests)
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_batched(lr, eng, plan, jobs, a.cpu_seconds)
+    eng.close()
+    for ps in pool:
+        ps.close()
+    return out
+
+
+def bench_serving(ctx, lr, pool, depth, tune, rates, nreq):
+    """Serving: the pool's scenes resident in a server; for each arrival rate, nreq requests
+    (a random scene and camera, a resolution from the batched mix, spp from {4, 16, 64}) arrive
+    as a Poisson stream; each is waited for and released by a client thread. Per rate: the rays
+    traced per second from the first arrival to the last completion, the completed jobs per
+    second, and p50/p99 latency (submission to the last sample)."""
+    import threading
+    import queue
+    budget = sum(ps.info()[2] for ps in pool) + (24 << 30)
+    srv = lr.Server(ctx, slots_log2=22, max_tree_depth=depth, budget=budget, tune=tune)
+    for ps in pool:
+        srv.add_scene(ps)
+    res = []
+    for ri, lam in enumerate(rates):
+        rng = np.random.default_rng(1000 + ri)
+        mix = lr.batch_plan(nreq, 200 + ri, spp_lo=4)
+        gaps = rng.exponential(1.0 / lam, nreq)
+        picks = rng.integers(0, len(pool), nreq)
+        q, lat, done_t = queue.Queue(), [], []
+
+        def collect():
+            while True:
+                i = q.get()
+                if i is None:
+                    return
+                info = srv.wait(i, 600000.0)
+                lat.append(info.t_done_ms - info.t_submit_ms)
+                done_t.append(info.t_done_ms)
+                srv.release(i)
+
+        th = threading.Thread(target=collect)
+        th.start()
+        r0 = srv.stats().rays
+        t_start = lr.now_ms()
+        t_next = time.perf_counter()
+        for k in range(nreq):
+            t_next += gaps[k]
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            _, w, h, spp = mix[k]
+            j = lr.make_job(pool[int(picks[k])], lr.batch_camera(5000 + 1000 * ri + k, w, h), w, h, spp,
+                            seed=10000 + k, max_depth=8)
+            q.put(srv.submit(j))
+        q.put(None)
+        th.join()
+        r1 = srv.stats().rays
+        span = max(done_t) - t_start
+        res.append({"rate_jobs_s": lam, "requests": nreq, "mrays_s": round((r1 - r0) / span / 1e3, 2),
+                    "jobs_s": round(nreq / span * 1e3, 2), "p50_ms": round(percentile(lat, 50), 1),
+                    "p99_ms": round(percentile(lat, 99), 1), "span_s": round(span / 1e3, 3)})
+    srv.close()
+    sat = [r["rate_jobs_s"] for r in res if r["jobs_s"] >= 0.9 * r["rate_jobs_s"]]
+    return {"scenes": len(pool), "rates": res,
+            "sustained_rate_jobs_s": max(sat) if sat else None,
+            "capacity_jobs_s": max(r["jobs_s"] for r in res)}
+
+
+def cpu_baseline_batched(lr, eng, plan, jobs, seconds):
+    """The reference's own CPU path tracer (trace_image of its float build, compiled from its
+    sources into oracle/_ref, OpenMP over rows on every host core) on the batched workload's jobs
+    at 1 spp, in the batch's order, until about `seconds` of tracing (scene builds excluded). Its
+    paths per second times the rays per path the device traced on the same jobs at 1 spp gives
+    its Mrays/s (the CPU path does not count rays; both follow the same bounce and shadow rules)."""
+    from oracle import lro
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        cores = os.cpu_count() or 1
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", cores)), cores)
+    if lro.ref(False) is None:
+        return {"value": None, "unit": "Mrays/s", "cores": threads, "kind": "reference",
+                "sample": "oracle/_ref not built: no CPU baseline"}
+    cpu_ms, cpu_paths, used = 0.0, 0, 0
+    for i, (n, w, h, spp) in enumerate(plan):
+        if cpu_ms >= seconds * 1e3:
+            break
+        sc = lr.batch_scene(n, i)
+        sc["cam"] = np.asarray(jobs[i].cam[:], np.float32)
+        _, ms = lro.ref_render(sc, w, h, 1, seed=i, max_depth=8, mis=True, double=False, timed=True)
+        cpu_ms += ms
+        cpu_paths += w * h
+        used += 1
+    s0 = eng.stats()
+    ids = []
+    for j in jobs[:used]:
+        one = lr.make_job(j.scene, j.cam[:], j.w, j.h, 1, seed=j.seed, max_depth=8)
+        ids.append(eng.submit(one))
+    eng.drain()
+    s1 = eng.stats()
+    for i in ids:
+        eng.release(i)
+    rpp = ((s1.rays_ext - s0.rays_ext) + (s1.rays_shadow - s0.rays_shadow)) / cpu_paths
+    mpaths = cpu_paths / cpu_ms / 1e3
+    return {"value": round(mpaths * rpp, 4), "unit": "Mrays/s", "cores": threads, "kind": "reference",
+            "sample": f"the first {used} jobs of the batch at 1 spp ({cpu_paths} paths, {cpu_ms / 1e3:.1f} s of "
+                      f"trace_image); Mrays/s = its Mpaths/s x the device's {rpp:.3f} rays per path on the same jobs",
+            "mpaths_s": round(mpaths, 4)}
 
 
 def cpu_baseline(tris_np, rays0, bounce, shadow, nsec, seconds):

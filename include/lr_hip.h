@@ -136,6 +136,9 @@ typedef struct LrhPtScene LrhPtScene;
 int lrh_pt_scene(LrhContext *c, const float *prims, int nprims, const float *tris, int ntris, const LrhMaterial *mats,
                  int nmats, int light, float eps, float margin, int builder, LrhPtScene **out);
 void lrh_pt_scene_free(LrhPtScene *s);
+/* the scene's tree depth (what an engine's max_tree_depth must cover), its shape count, and
+ * the device memory it holds (what a server's budget counts) */
+int lrh_pt_scene_info(const LrhPtScene *s, int *depth, int *nshapes, size_t *bytes);
 
 /* Path traces spp samples (indices s0 .. s0+spp-1) of every pixel of a W x H image and adds
  * them to d_accum (float4 per pixel, row-major: the sum of the samples' rgb, and their count),
@@ -198,6 +201,8 @@ typedef struct {
     uint64_t jobs_done;
     uint64_t node_fetches;  /* with counting on: node records read by the extension walks */
     uint64_t shape_tests;   /* and shapes tested */
+    double kernel_ms[4];    /* with profiling on: device time of regen, extend, shade, shadow */
+    uint64_t kernel_launches[4];
 } LrhEngineStats;
 
 /* host clock (steady, ms) that job times are given in */
@@ -211,8 +216,13 @@ int lrh_engine_tune_set(LrhEngine *e, const LrhTune *t);
 int lrh_engine_tune_get(const LrhEngine *e, LrhTune *t);
 /* counting on: the extension walks also count node records and shapes (a slower kernel) */
 int lrh_engine_counting(LrhEngine *e, int on);
-/* Admits a job (its image cleared, its paths queued after those already queued): LRH_EAGAIN
- * if its image or a job slot cannot be had until others are released. *id names it. */
+/* profiling on: lrh_engine_pump launches the iterations kernel by kernel between hipEvents on
+ * the engine's stream (no graph) and sums each kernel's time into the stats */
+int lrh_engine_profiling(LrhEngine *e, int on);
+/* Admits a job (its image cleared): LRH_EAGAIN if its image or a job slot cannot be had until
+ * others are released. *id names it. Its paths are issued to the device in chunks of at most
+ * one pool of slots, always from the admitted job with the fewest paths left to issue
+ * (shortest remaining first), just ahead of what the device consumes. */
 int lrh_engine_submit(LrhEngine *e, const LrhJob *job, int64_t *id);
 /* Runs up to `graphs` captured graphs of the engine's iterations (fewer if the work ends) and
  * takes in the progress they report. Returns the number of jobs still running (>= 0). */

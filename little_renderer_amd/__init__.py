@@ -58,7 +58,8 @@ class JobInfo(ctypes.Structure):
 
 class EngineStats(ctypes.Structure):
     _fields_ = [("rays_ext", ctypes.c_uint64), ("rays_shadow", ctypes.c_uint64), ("iterations", ctypes.c_uint64),
-                ("jobs_done", ctypes.c_uint64), ("node_fetches", ctypes.c_uint64), ("shape_tests", ctypes.c_uint64)]
+                ("jobs_done", ctypes.c_uint64), ("node_fetches", ctypes.c_uint64), ("shape_tests", ctypes.c_uint64),
+                ("kernel_ms", ctypes.c_double * 4), ("kernel_launches", ctypes.c_uint64 * 4)]
 
 
 class ServerStats(ctypes.Structure):
@@ -103,6 +104,8 @@ def lib():
         "lrs_cornell": (None, [ctypes.c_uint64, I, I, P, P, P, P]),
         "lrh_pt_scene": (I, [P, P, I, P, I, P, I, I, F, F, I, ctypes.POINTER(P)]),
         "lrh_pt_scene_free": (None, [P]),
+        "lrh_pt_scene_info": (I, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_size_t)]),
         "lrh_pt_render": (I, [P, P, P, I, I, I, U, ctypes.c_uint64, I, I, F, P]),
         "lrs_batch_scene": (None, [I, ctypes.c_uint64, P, P, P]),
         "lrs_batch_camera": (None, [ctypes.c_uint64, I, I, P]),
@@ -113,6 +116,7 @@ def lib():
         "lrh_engine_tune_set": (I, [P, ctypes.POINTER(Tune)]),
         "lrh_engine_tune_get": (I, [P, ctypes.POINTER(Tune)]),
         "lrh_engine_counting": (I, [P, I]),
+        "lrh_engine_profiling": (I, [P, I]),
         "lrh_engine_submit": (I, [P, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_int64)]),
         "lrh_engine_pump": (I, [P, I]),
         "lrh_engine_drain": (I, [P]),
@@ -216,6 +220,12 @@ class PtScene:
         self._h = ctypes.c_void_p()
         _check(lib().lrh_pt_scene(ctx._c, _ptr(pr), pr.shape[0], _ptr(tr), tr.shape[0], ctypes.cast(mats, ctypes.c_void_p),
                                   len(mats), light, eps, margin, builder, ctypes.byref(self._h)))
+
+    def info(self):
+        """(tree depth, shapes, device bytes)"""
+        d, n, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        _check(lib().lrh_pt_scene_info(self._h, ctypes.byref(d), ctypes.byref(n), ctypes.byref(b)))
+        return d.value, n.value, b.value
 
     def close(self):
         if self._h:
@@ -377,6 +387,9 @@ class Engine:
 
     def counting(self, on):
         _check(lib().lrh_engine_counting(self._e, 1 if on else 0))
+
+    def profiling(self, on):
+        _check(lib().lrh_engine_profiling(self._e, 1 if on else 0))
 
     def submit(self, job):
         """the job's id, or None if it does not fit now"""

@@ -5,8 +5,9 @@
 // One iteration is four kernels: regen (start new paths in free slots), extend (closest hits
 // of the ray queue), shade (emission, next-event shadow ray, BSDF sample, Russian roulette),
 // shadow (any hit before the lamp). A path that ends gives its slot back to a ring of free
-// slots; regen fills free slots from the global path index space, in which job j owns
-// [start_j, start_j + w*h*spp) (sample-major: q = sample * w*h + pixel). The same kernels serve
+// slots; regen fills free slots from the global path index space, in which each job owns
+// chunks of [0, w*h*spp) (sample-major: q = sample * w*h + pixel), each chunk an issue record
+// mapping a range of global path indices to (job, first q). The same kernels serve
 // the reference's one-job-at-a-time shape (a grid per pass, passes in order) and the batched
 // form (persistent work-groups, one hipGraph of several iterations replayed until the work is
 // done), so a job's image is the same to the bit in either, alone or among others: each
@@ -32,7 +33,7 @@ struct JobDev {
     const float4 *prims;
     const float4 *mats;       // 3 float4 per material: albedo.rgb type | emission.rgb ior | metallic roughness transmission 0
     long long *accum;         // 3 x int64 per pixel, fixed point
-    unsigned long long start, total;
+    unsigned long long total; // w * h * spp
     float cam[12];
     uint32_t root, nprims;
     int W, H, npix, light;
@@ -67,8 +68,9 @@ struct EngArgs {
     EngCtl *ctl;
     const JobDev *jobs;
     unsigned long long *jret, *jdone;
-    const unsigned long long *iq_start;     // issue record: the path index its job starts at
-    const uint32_t *iq_job;                 // and the job slot
+    const unsigned long long *iq_start;     // issue record: the first global path index it covers,
+    const uint32_t *iq_job;                 // its job slot,
+    const uint32_t *iq_q0;                  // and the index in the job of its first path
     float4 *thr;                            // throughput.rgb, the last bounce's pdf
     float4 *rad;                            // radiance.rgb, the emission state
     uint64_t *rng;                          // PCG32 state (stream: the pixel)

@@ -43,15 +43,15 @@ __global__ __launch_bounds__(SBLOCK) void k_zero64(long long *p, size_t n)
     for (size_t i = (size_t)blockIdx.x * SBLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * SBLOCK) p[i] = 0;
 }
 
-// an issue record (rec = ~0: none) and the new end of the issued paths
-__global__ void k_issue(EngCtl *ctl, unsigned long long *iq_start, uint32_t *iq_job, unsigned long long rec,
-                        unsigned long long start, uint32_t jslot, unsigned long long issue_end, unsigned long long lo)
+// an issue record: global path indices [start, issue_end) are the job's q0, q0 + 1, ...
+__global__ void k_issue(EngCtl *ctl, unsigned long long *iq_start, uint32_t *iq_job, uint32_t *iq_q0,
+                        unsigned long long rec, unsigned long long start, uint32_t jslot, uint32_t q0,
+                        unsigned long long issue_end, unsigned long long lo)
 {
-    if (rec != ~0ull) {
-        iq_start[rec & (LR_ISSUE - 1)] = start;
-        iq_job[rec & (LR_ISSUE - 1)] = jslot;
-        ctl->iq_hi = rec + 1;
-    }
+    iq_start[rec & (LR_ISSUE - 1)] = start;
+    iq_job[rec & (LR_ISSUE - 1)] = jslot;
+    iq_q0[rec & (LR_ISSUE - 1)] = q0;
+    ctl->iq_hi = rec + 1;
     ctl->issue_end = issue_end;
     ctl->iq_lo = lo;
 }
@@ -99,7 +99,7 @@ struct HostJob {
     int64_t id = -1;
     LrhJob desc{};
     size_t off = 0, bytes = 0;  // its image in the arena
-    unsigned long long start = 0, total = 0, rec = 0;
+    unsigned long long total = 0, issued = 0;   // its paths, and those issued so far
     double t_submit = 0, t_done = 0;
     unsigned long long min_seq = 0;   // the first progress snapshot taken after its admission
 };
@@ -125,7 +125,7 @@ struct LrhEngine {
     EngProgress *prog = nullptr;
     JobDev *jobs = nullptr;
     unsigned long long *iq_start = nullptr, *counts = nullptr, *clk = nullptr;
-    uint32_t *iq_job = nullptr, *overflow = nullptr;
+    uint32_t *iq_job = nullptr, *iq_q0 = nullptr, *overflow = nullptr;
     EngArgs base{};
     // host
     EngProgress *pin[2] = {nullptr, nullptr};
@@ -144,6 +144,11 @@ struct LrhEngine {
     uint64_t jobs_done = 0;
     LrhTune tune{128, 24, 128, 4, 1, 8};
     bool counting = false;
+    // profiling: iterations launched one kernel at a time between events (no graph)
+    bool profiling = false;
+    std::vector<hipEvent_t> pevs;
+    double kms[4] = {0, 0, 0, 0};
+    uint64_t klaunch[4] = {0, 0, 0, 0};
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool graph_ok = false;
@@ -180,18 +185,45 @@ EngArgs args_for(const LrhEngine *e, int stack, size_t threads, int mult)
     return A;
 }
 
-// the persistent grids of the batched form
-void launch_iteration(LrhEngine *e, const LrhTune &t)
+// the persistent grids of the batched form; ev (profiling): 5 events around the 4 kernels
+void launch_iteration(LrhEngine *e, const LrhTune &t, hipEvent_t *ev = nullptr)
 {
     const LrEngTable &T = table();
     const int bt = lr_blk_index(t.block_trace), bs = lr_blk_index(t.block_shade), sk = lr_stk_index(t.stack);
     const dim3 gt(e->ncu * t.blocks_per_cu), gs(e->ncu * t.blocks_per_cu);
     const EngArgs At = args_for(e, t.stack, (size_t)gt.x * t.block_trace, t.tile);
     const EngArgs As = args_for(e, t.stack, 0, t.tile);
+    if (ev) (void)hipEventRecord(ev[0], e->stream);
     T.regen[bs](gs, e->stream, As);
+    if (ev) (void)hipEventRecord(ev[1], e->stream);
     (e->counting ? T.extend_count : T.extend)[bt][sk](gt, e->stream, At);
+    if (ev) (void)hipEventRecord(ev[2], e->stream);
     T.shade[bs](gs, e->stream, As);
+    if (ev) (void)hipEventRecord(ev[3], e->stream);
     T.shadow[bt][sk](gt, e->stream, At);
+    if (ev) (void)hipEventRecord(ev[4], e->stream);
+}
+
+// profiling: one graph's worth of iterations, each kernel between events; waits for them
+int launch_profiled(LrhEngine *e)
+{
+    const int G = e->tune.iters_per_graph;
+    while ((int)e->pevs.size() < 5 * G) {
+        hipEvent_t ev;
+        ENG_HIP(hipEventCreate(&ev));
+        e->pevs.push_back(ev);
+    }
+    for (int i = 0; i < G; i++) launch_iteration(e, e->tune, &e->pevs[5 * i]);
+    ENG_HIP(hipGetLastError());
+    ENG_HIP(hipEventSynchronize(e->pevs[5 * G - 1]));
+    for (int i = 0; i < G; i++)
+        for (int k = 0; k < 4; k++) {
+            float ms = 0.0f;
+            ENG_HIP(hipEventElapsedTime(&ms, e->pevs[5 * i + k], e->pevs[5 * i + k + 1]));
+            e->kms[k] += ms;
+            e->klaunch[k]++;
+        }
+    return 0;
 }
 
 int make_graph(LrhEngine *e)
@@ -316,8 +348,8 @@ int job_check(const LrhEngine *e, const LrhJob *j)
     return 0;
 }
 
-// admits a job: its descriptor, its image cleared, and (issue) its paths queued
-int job_admit(LrhEngine *e, const LrhJob *j, bool issue, int *slot_out)
+// admits a job: its descriptor and its image cleared; its paths are issued later, in chunks
+int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
 {
     int rc = job_check(e, j);
     if (rc) return rc;
@@ -328,7 +360,6 @@ int job_admit(LrhEngine *e, const LrhJob *j, bool issue, int *slot_out)
             break;
         }
     if (slot < 0) return lr_fail(LRH_EAGAIN, "engine: all %d job slots are in use", LR_JOBS);
-    if (e->iq_hi - issue_lo(e) >= LR_ISSUE - 1) return lr_fail(LRH_EAGAIN, "engine: too many jobs queued");
     const size_t npix = (size_t)j->w * (size_t)j->h;
     const size_t bytes = lr_align(npix * 24);
     const size_t off = arena_take(e, bytes);
@@ -341,8 +372,6 @@ int job_admit(LrhEngine *e, const LrhJob *j, bool issue, int *slot_out)
     h.off = off;
     h.bytes = bytes;
     h.total = (unsigned long long)npix * (unsigned long long)j->spp;
-    h.start = e->issue_end;
-    h.rec = e->iq_hi;
     h.t_submit = now_ms();
     h.min_seq = e->seq_next;
     const LrhPtScene *s = j->scene;
@@ -354,7 +383,6 @@ int job_admit(LrhEngine *e, const LrhJob *j, bool issue, int *slot_out)
     jd.prims = s->prims;
     jd.mats = s->mats;
     jd.accum = (long long *)(e->arena + off);
-    jd.start = h.start;
     jd.total = h.total;
     memcpy(jd.cam, j->cam, sizeof(jd.cam));
     jd.root = 0;
@@ -375,15 +403,47 @@ int job_admit(LrhEngine *e, const LrhJob *j, bool issue, int *slot_out)
     const size_t words = npix * 3;
     hipLaunchKernelGGL(k_zero64, dim3((unsigned)((words + SBLOCK - 1) / SBLOCK < 4096 ? (words + SBLOCK - 1) / SBLOCK : 4096)),
                        dim3(SBLOCK), 0, e->stream, jd.accum, words);
-    e->iq_hi++;
-    e->issue_end += issue ? h.total : 0;
-    e->recs.push_back(IssueRec{h.rec, h.start, h.start + h.total});
-    hipLaunchKernelGGL(k_issue, dim3(1), dim3(1), 0, e->stream, &e->prog->ctl, e->iq_start, e->iq_job, h.rec, h.start,
-                       (uint32_t)slot, e->issue_end, issue_lo(e));
     ENG_HIP(hipGetLastError());
     e->by_id[h.id] = slot;
     e->running++;
     *slot_out = slot;
+    return 0;
+}
+
+// issues the next n paths of job `slot` as one record
+int issue_chunk(LrhEngine *e, int slot, unsigned long long n)
+{
+    HostJob &h = e->hj[slot];
+    const unsigned long long rec = e->iq_hi++, start = e->issue_end;
+    e->issue_end += n;
+    e->recs.push_back(IssueRec{rec, start, start + n});
+    hipLaunchKernelGGL(k_issue, dim3(1), dim3(1), 0, e->stream, &e->prog->ctl, e->iq_start, e->iq_job, e->iq_q0, rec,
+                       start, (uint32_t)slot, (uint32_t)h.issued, e->issue_end, issue_lo(e));
+    h.issued += n;
+    ENG_HIP(hipGetLastError());
+    return 0;
+}
+
+// Keeps enough paths issued ahead of the device for the next graphs (the host sees the device's
+// progress one graph late), chunk by chunk, from the job with the fewest paths left to issue
+// (shortest remaining first: small jobs finish early, the device stays full either way).
+int top_up(LrhEngine *e)
+{
+    const unsigned long long target = (unsigned long long)(e->tune.iters_per_graph + 2) * e->C;
+    while (e->issue_end - e->last.ctl.next < target && e->iq_hi - issue_lo(e) < LR_ISSUE - 2) {
+        int best = -1;
+        unsigned long long left = ~0ull;
+        for (int s = 0; s < LR_JOBS; s++) {
+            const HostJob &h = e->hj[s];
+            if (h.state == LRH_JOB_RUNNING && h.issued < h.total && h.total - h.issued < left) {
+                left = h.total - h.issued;
+                best = s;
+            }
+        }
+        if (best < 0) break;
+        int rc = issue_chunk(e, best, left < e->C ? left : e->C);
+        if (rc) return rc;
+    }
     return 0;
 }
 
@@ -406,6 +466,7 @@ void lrh_engine_destroy(LrhEngine *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (hipEvent_t ev : e->pevs) (void)hipEventDestroy(ev);
     for (int b = 0; b < 2; b++) {
         if (e->pin[b]) (void)hipHostFree(e->pin[b]);
         if (e->pev[b]) (void)hipEventDestroy(e->pev[b]);
@@ -443,7 +504,8 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
         C * 32, C * 32, C * 4, C * 4, C * 16,                                        // 9..13 qray qray qid qid hits
         C * 32, C * 4, C * 16,                                                       // 14..16 sray sid scon
         e->threads_cap * (size_t)(e->spill_cap > 0 ? e->spill_cap : 1) * 4,          // 17 spill
-        2112 * 4, 4096 * 2, 256, 256, 256};                                          // 18..22 tab mask counts clk overflow
+        2112 * 4, 4096 * 2, 256, 256, 256,                                           // 18..22 tab mask counts clk overflow
+        LR_ISSUE * 4};                                                               // 23 iq_q0
     constexpr int NS = sizeof(sizes) / sizeof(sizes[0]);
     size_t total = 0;
     for (size_t z : sizes) total += lr_align(z);
@@ -465,6 +527,7 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
     e->jobs = (JobDev *)p[1];
     e->iq_start = (unsigned long long *)p[2];
     e->iq_job = (uint32_t *)p[3];
+    e->iq_q0 = (uint32_t *)p[23];
     e->counts = (unsigned long long *)p[20];
     e->clk = (unsigned long long *)p[21];
     e->overflow = (uint32_t *)p[22];
@@ -475,6 +538,7 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
     A.jdone = e->prog->jdone;
     A.iq_start = e->iq_start;
     A.iq_job = e->iq_job;
+    A.iq_q0 = e->iq_q0;
     A.thr = (float4 *)p[4];
     A.rad = (float4 *)p[5];
     A.rng = (uint64_t *)p[6];
@@ -564,7 +628,7 @@ int lrh_engine_submit(LrhEngine *e, const LrhJob *job, int64_t *id)
     if (!e || !job || !id) return lr_fail(LRH_EINVAL, "lrh_engine_submit: bad arguments");
     (void)hipSetDevice(e->device);
     int slot;
-    int rc = job_admit(e, job, true, &slot);
+    int rc = job_admit(e, job, &slot);
     if (rc) return rc;
     *id = e->hj[slot].id;
     return 0;
@@ -577,8 +641,14 @@ int lrh_engine_pump(LrhEngine *e, int graphs)
     int rc = make_graph(e);
     if (rc) return rc;
     for (int g = 0; g < graphs && e->running > 0; g++) {
-        ENG_HIP(hipGraphLaunch(e->gexec, e->stream));
-        rc = progress_enqueue(e);
+        rc = top_up(e);
+        if (rc) return rc;
+        if (e->profiling) rc = launch_profiled(e);
+        else {
+            hipError_t le = hipGraphLaunch(e->gexec, e->stream);
+            if (le != hipSuccess) return eng_fail_dev("hipGraphLaunch", le);
+        }
+        if (!rc) rc = progress_enqueue(e);
         if (!rc) rc = progress_take(e, e->pk);   // the snapshot before this one
         if (rc) return rc;
     }
@@ -614,8 +684,10 @@ int lrh_engine_plain(LrhEngine *e, const LrhJob *job, int64_t *id)
     (void)hipSetDevice(e->device);
     const size_t npix = (size_t)job->w * (size_t)job->h;
     if (npix > e->C) return lr_fail(LRH_EREFUSED, "lrh_engine_plain: %zu pixels a pass, %u slots", npix, e->C);
+    if (e->iq_hi - issue_lo(e) + (unsigned long long)job->spp >= LR_ISSUE - 2)
+        return lr_fail(LRH_EAGAIN, "lrh_engine_plain: too many issue records in use");
     int slot;
-    int rc = job_admit(e, job, false, &slot);
+    int rc = job_admit(e, job, &slot);
     if (rc) return rc;
     HostJob &h = e->hj[slot];
     const LrEngTable &T = table();
@@ -625,9 +697,9 @@ int lrh_engine_plain(LrhEngine *e, const LrhJob *job, int64_t *id)
     const EngArgs At = args_for(e, t.stack, (size_t)gt.x * t.block_trace, 1);
     const EngArgs As = args_for(e, t.stack, 0, 1);
     for (int s = 0; s < job->spp; s++) {
-        // the pass: paths [start + s * npix, start + (s + 1) * npix), one per pixel
-        hipLaunchKernelGGL(k_issue, dim3(1), dim3(1), 0, e->stream, &e->prog->ctl, e->iq_start, e->iq_job, ~0ull, 0ull, 0u,
-                           h.start + (unsigned long long)(s + 1) * npix, issue_lo(e));
+        // the pass: the job's paths s * npix .. (s + 1) * npix - 1, one per pixel
+        rc = issue_chunk(e, slot, npix);
+        if (rc) return rc;
         T.regen[bs](gs, e->stream, As);
         for (int d = 0; d <= job->max_depth; d++) {
             (e->counting ? T.extend_count : T.extend)[bt][sk](gt, e->stream, At);
@@ -636,7 +708,6 @@ int lrh_engine_plain(LrhEngine *e, const LrhJob *job, int64_t *id)
         }
     }
     ENG_HIP(hipGetLastError());
-    e->issue_end = h.start + h.total;
     ENG_HIP(hipStreamSynchronize(e->stream));    // the host waits for the job (gpu_wait)
     rc = progress_enqueue(e);
     if (!rc) rc = progress_flush(e);
@@ -705,6 +776,17 @@ int lrh_engine_stats(LrhEngine *e, LrhEngineStats *st)
     st->jobs_done = e->jobs_done;
     st->node_fetches = cnt[0];
     st->shape_tests = cnt[1];
+    for (int k = 0; k < 4; k++) {
+        st->kernel_ms[k] = e->kms[k];
+        st->kernel_launches[k] = e->klaunch[k];
+    }
+    return 0;
+}
+
+int lrh_engine_profiling(LrhEngine *e, int on)
+{
+    if (!e) return lr_fail(LRH_EINVAL, "lrh_engine_profiling: no engine");
+    e->profiling = on != 0;
     return 0;
 }
 
@@ -715,7 +797,8 @@ int lrh_engine_autotune(LrhEngine *e, const LrhJob *sample, int n, LrhTune *best
     hipEvent_t ev[2];
     ENG_HIP(hipEventCreate(&ev[0]));
     ENG_HIP(hipEventCreate(&ev[1]));
-    auto measure = [&](const LrhTune &t, double &score) -> int {
+    // one timed run of the sample jobs under launch shape t: rays per microsecond
+    auto run_once = [&](const LrhTune &t, double &score) -> int {
         int rc = lrh_engine_tune_set(e, &t);
         if (!rc) rc = make_graph(e);
         if (rc) return rc;
@@ -740,6 +823,14 @@ int lrh_engine_autotune(LrhEngine *e, const LrhJob *sample, int n, LrhTune *best
         const unsigned long long r1 = e->last.ctl.rays_ext + e->last.ctl.rays_shadow;
         score = ms > 0.0f ? (double)(r1 - r0) / (ms * 1e3) : 0.0;
         return 0;
+    };
+    // the better of two runs (one run of a short sample is at the mercy of clocks and caches)
+    auto measure = [&](const LrhTune &t, double &score) -> int {
+        double a = 0.0, b = 0.0;
+        int rc = run_once(t, a);
+        if (!rc) rc = run_once(t, b);
+        score = a > b ? a : b;
+        return rc;
     };
     LrhTune cur = e->tune, win = cur;
     double top = 0.0, s = 0.0;

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(BLK) void k_regen(EngArgs A)
             }
             const uint32_t jslot = A.iq_job[a & (LR_ISSUE - 1)] & (LR_JOBS - 1);
             const JobDev &J = A.jobs[jslot];
-            const uint32_t q = (uint32_t)(g - J.start);
+            const uint32_t q = A.iq_q0[a & (LR_ISSUE - 1)] + (uint32_t)(g - A.iq_start[a & (LR_ISSUE - 1)]);
             const uint32_t pixel = q % (uint32_t)J.npix;
             const uint32_t k = J.s0 + q / (uint32_t)J.npix;
             const int x = (int)(pixel % (uint32_t)J.W), y = (int)(pixel / (uint32_t)J.W);

@@ -133,3 +133,12 @@ extern "C" void lrh_pt_scene_free(LrhPtScene *s)
     if (s && s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
     pt_scene_release(s);
 }
+
+extern "C" int lrh_pt_scene_info(const LrhPtScene *s, int *depth, int *nshapes, size_t *bytes)
+{
+    if (!s) return lr_fail(LRH_EINVAL, "lrh_pt_scene_info: no scene");
+    if (depth) *depth = s->depth;
+    if (nshapes) *nshapes = s->nprims + s->ntris;
+    if (bytes) *bytes = s->bytes;
+    return 0;
+}

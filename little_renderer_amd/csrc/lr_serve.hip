@@ -38,7 +38,9 @@ struct SrvJob {
 struct LrhServer {
     LrhEngine *eng = nullptr;
     LrhContext *ctx = nullptr;
-    std::mutex mu;
+    // emu: the engine (the worker holds it while a graph runs); mu: the requests and counters
+    // (held briefly: a submission never waits for the device). Order: emu, then mu.
+    std::mutex emu, mu;
     std::condition_variable cv_work, cv_done;
     std::thread worker;
     bool stop = false;
@@ -122,17 +124,22 @@ void collect(LrhServer *s)
 
 void work(LrhServer *s)
 {
-    std::unique_lock<std::mutex> lk(s->mu);
     (void)hipSetDevice(s->ctx->device);
-    while (!s->stop) {
+    for (;;) {
+        std::unique_lock<std::mutex> ek(s->emu);
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (s->stop) break;
         admit(s);
         if (s->running.empty()) {
+            ek.unlock();
             const uint64_t seen = s->changes;
             s->cv_work.wait(lk, [&] { return s->stop || s->changes != seen; });
             continue;
         }
+        lk.unlock();
         const double t0 = lrh_now_ms();
-        const int rc = lrh_engine_pump(s->eng, 1);
+        const int rc = lrh_engine_pump(s->eng, 1);   // the engine's lock only: requests keep coming
+        lk.lock();
         s->busy_ms += lrh_now_ms() - t0;
         if (rc < 0) {
             // the device failed: every job still open fails with it
@@ -147,14 +154,14 @@ void work(LrhServer *s)
             s->waiting.clear();
             s->running.clear();
             s->cv_done.notify_all();
+            ek.unlock();
             s->cv_work.wait(lk, [&] { return s->stop; });
             break;
         }
         collect(s);
-        // let requests and readers in between graphs
         lk.unlock();
-        std::this_thread::yield();
-        lk.lock();
+        ek.unlock();
+        std::this_thread::yield();   // readers and releasers get the engine between graphs
     }
 }
 
@@ -238,6 +245,7 @@ int lrh_server_wait(LrhServer *s, int64_t id, double timeout_ms, LrhJobInfo *inf
 int lrh_server_image(LrhServer *s, int64_t id, float *h_out)
 {
     if (!s || !h_out) return lr_fail(LRH_EINVAL, "lrh_server_image: bad arguments");
+    std::lock_guard<std::mutex> eg(s->emu);
     std::lock_guard<std::mutex> g(s->mu);
     auto it = s->jobs.find(id);
     if (it == s->jobs.end()) return lr_fail(LRH_EINVAL, "lrh_server_image: no job %lld", (long long)id);
@@ -259,6 +267,7 @@ int lrh_server_image(LrhServer *s, int64_t id, float *h_out)
 int lrh_server_release(LrhServer *s, int64_t id)
 {
     if (!s) return lr_fail(LRH_EINVAL, "lrh_server_release: no server");
+    std::lock_guard<std::mutex> eg(s->emu);
     std::lock_guard<std::mutex> g(s->mu);
     auto it = s->jobs.find(id);
     if (it == s->jobs.end()) return lr_fail(LRH_EINVAL, "lrh_server_release: no job %lld", (long long)id);
@@ -278,6 +287,7 @@ int lrh_server_release(LrhServer *s, int64_t id)
 int lrh_server_stats(LrhServer *s, LrhServerStats *st)
 {
     if (!s || !st) return lr_fail(LRH_EINVAL, "lrh_server_stats: bad arguments");
+    std::lock_guard<std::mutex> eg(s->emu);
     std::lock_guard<std::mutex> g(s->mu);
     LrhEngineStats es;
     int rc = lrh_engine_stats(s->eng, &es);

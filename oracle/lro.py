@@ -97,6 +97,7 @@ def ref(double=False):
         "refh_brdf_tables": (None, [P]),
         "refh_bsdf_case": (None, [P, I, P, P, P, F, F, F, P]),
         "refh_render": (None, [P, I, P, I, P, I, P, I, I, I, U64, I, I, P]),
+        "refh_render_t": (None, [P, I, P, I, P, I, P, I, I, I, U64, I, I, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -295,9 +296,10 @@ def pt_image(scene, W, H, s0, ns, seed=0, clamp=0.0, max_depth=8, mis=True, tab=
     return out
 
 
-def ref_render(scene, W, H, spp, seed=0, max_depth=8, mis=True, double=True):
+def ref_render(scene, W, H, spp, seed=0, max_depth=8, mis=True, double=True, timed=False):
     """The reference's own trace_image (its double build by default) on the same scene: the
-    mean of spp samples per pixel, (H, W, 3) float64; None where oracle/_ref was not built"""
+    mean of spp samples per pixel, (H, W, 3) float64; None where oracle/_ref was not built.
+    timed: also the milliseconds trace_image took (its BVH build excluded): (px, ms)"""
     R = ref(double)
     if R is None:
         return None
@@ -317,6 +319,7 @@ def ref_render(scene, W, H, spp, seed=0, max_depth=8, mis=True, double=True):
         m[i, 8:11] = list(x.emission)
     cam = np.ascontiguousarray(scene["cam"], np.float32)
     px = np.zeros((H, W, 3), np.float64)
-    R.refh_render(ptr(tris), tris.shape[0], ptr(lamp), lamp_mat, ptr(m), m.shape[0], ptr(cam), W, H, spp, seed, max_depth,
-                  1 if mis else 0, ptr(px))
-    return px
+    ms = np.zeros(1, np.float64)
+    R.refh_render_t(ptr(tris), tris.shape[0], ptr(lamp), lamp_mat, ptr(m), m.shape[0], ptr(cam), W, H, spp, seed,
+                    max_depth, 1 if mis else 0, ptr(px), ptr(ms))
+    return (px, float(ms[0])) if timed else px

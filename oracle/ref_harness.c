@@ -6,6 +6,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "base/rng.h"
 #include "base/sampler.h"
@@ -188,8 +189,27 @@ void refh_bsdf_case(const float *m, int outside, const float *n, const float *v,
  * refh_bsdf_case's 9 floats + emission.rgb (12 floats), built through the reference's own
  * scene API (scene_add_triangle, make_quad, scene_set_accel), camera 12 floats (origin,
  * upper_left, horizontal, vertical); the mean of spp samples per pixel into px (rgb). */
+static double refh_now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+/* refh_render, also giving the time trace_image took (the scene's build excluded) */
+void refh_render_t(const float *tris, int ntris, const float *lamp, int lamp_mat, const float *mats, int nmats,
+                   const float *cam, int W, int H, int spp, uint64_t seed, int max_depth, int mis, double *px,
+                   double *trace_ms);
+
 void refh_render(const float *tris, int ntris, const float *lamp, int lamp_mat, const float *mats, int nmats,
                  const float *cam, int W, int H, int spp, uint64_t seed, int max_depth, int mis, double *px)
+{
+    refh_render_t(tris, ntris, lamp, lamp_mat, mats, nmats, cam, W, H, spp, seed, max_depth, mis, px, NULL);
+}
+
+void refh_render_t(const float *tris, int ntris, const float *lamp, int lamp_mat, const float *mats, int nmats,
+                   const float *cam, int W, int H, int spp, uint64_t seed, int max_depth, int mis, double *px,
+                   double *trace_ms)
 {
     Scene s;
     memset(&s, 0, sizeof(s));
@@ -226,7 +246,9 @@ void refh_render(const float *tris, int ntris, const float *lamp, int lamp_mat, 
     tp.max_depth = max_depth;
     tp.mis = mis;
     Vec3 *out = calloc((size_t)W * H, sizeof(Vec3));
+    const double t0 = refh_now_ms();
     trace_image(&s, &c, &tp, W, H, spp, seed, 0, out, NULL, NULL);
+    if (trace_ms) *trace_ms = refh_now_ms() - t0;
     for (int i = 0; i < W * H; i++) { px[3 * i] = out[i].x; px[3 * i + 1] = out[i].y; px[3 * i + 2] = out[i].z; }
     free(out);
     scene_free(&s);
