@@ -31,8 +31,6 @@ uint32_t lro_rng_u32(LroRng *r);                                  /* rng.h:13-20
 void lro_rng_seed(LroRng *r, uint64_t seed, uint64_t stream);     /* rng.h:22-28 */
 float lro_rng_float(LroRng *r);                                   /* rng.h:31-37 (float build) */
 double lro_rng_double(LroRng *r);                                 /* rng.h:31-37 (double build) */
-/* the GPU's 32-bit PCG-RXS-M-XS (trace.wgsl:74-79) */
-float lro_rnd32(uint32_t *s);
 
 /* ---- BVH builds (trace_build.wgsl, trace_ploc.wgsl, trace_gpu.c:545-823) ---- */
 #define LRO_LEAF 0x80000000u

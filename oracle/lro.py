@@ -37,7 +37,6 @@ def lib():
             "lro_rng_seed": (None, [P, U64, U64]),
             "lro_rng_float": (F, [P]),
             "lro_rng_double": (D, [P]),
-            "lro_rnd32": (F, [P]),
             "lro_tri_boxes": (None, [P, I, P]),
             "lro_union": (None, [P, I, P, P]),
             "lro_inv_extent": (None, [P, P, P]),

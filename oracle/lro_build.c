@@ -28,15 +28,6 @@ void lro_rng_seed(LroRng *r, uint64_t seed, uint64_t stream)
 float lro_rng_float(LroRng *r) { return (float)(lro_rng_u32(r) >> 8) * (1.0f / 16777216.0f); }
 double lro_rng_double(LroRng *r) { return lro_rng_u32(r) * (1.0 / 4294967296.0); }
 
-/* trace.wgsl:74-79: the GPU's 32-bit PCG (RXS-M-XS output) */
-float lro_rnd32(uint32_t *s)
-{
-    uint32_t old = *s;
-    *s = old * 747796405u + 2891336453u;
-    uint32_t w = ((old >> ((old >> 28u) + 4u)) ^ old) * 277803737u;
-    return (float)(((w >> 22u) ^ w) >> 8u) * (1.0f / 16777216.0f);
-}
-
 /* WGSL's min / max on f32 (the builders' boxes): e2 if e2 < e1 (resp. >), else e1 */
 static inline float wmin(float a, float b) { return b < a ? b : a; }
 static inline float wmax(float a, float b) { return b > a ? b : a; }
