@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--jobs", type=int, default=64, help="batched: jobs (and scenes) in a batch")
     ap.add_argument("--serve-requests", type=int, default=40, help="serving: requests per arrival rate (0: skip)")
     ap.add_argument("--serve-rates", default="50,100,200,400", help="serving: Poisson arrival rates, jobs/s")
+    ap.add_argument("--tune", default=None, metavar="BT,STK,BS,BPC,TILE,IPG",
+                    help="batched: this launch shape instead of the autotuner's")
+    ap.add_argument("--skip-plain", action="store_true", help="batched: leave out the plain per-job comparison")
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
@@ -209,6 +212,10 @@ def _align(x):
     return (x + 255) & ~255
 
 
+def rnd(x, k):
+    return None if x is None else round(float(x), k)
+
+
 def percentile(v, p):
     return float(np.percentile(np.asarray(v, np.float64), p)) if len(v) else None
 
@@ -220,6 +227,21 @@ def build_pool(ctx, lr, plan):
         sc = lr.batch_scene(n, i)
         pool.append(lr.PtScene(ctx, sc["prims"], sc["tris"], sc["mats"], 0, sc["eps"], sc["margin"]))
     return pool
+
+
+def measured_traffic(tune):
+    """HBM bytes per launch of the extension kernel from the committed PMC summary
+    (profiles/batched_extend_traffic.txt: two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE,
+    calibrated on a device copy), when this run uses the same kernel variant; else None"""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "batched_extend_traffic.txt")
+    try:
+        rec = json.load(open(path))["batched"]
+    except Exception:
+        return None
+    t = tune.as_dict() if hasattr(tune, "as_dict") else dict(tune)
+    if [t["block_trace"], t["stack"]] != rec["tune"]:
+        return None
+    return rec["bytes_per_launch"]
 
 
 def bench_batched(a, torch, lr):
@@ -247,7 +269,12 @@ def bench_batched(a, torch, lr):
     sample = [lr.make_job(pool[i], lr.batch_camera(1000 + i, w, h), w, h, 1, seed=i, max_depth=8)
               for i, (n, w, h, spp) in enumerate(plan[:16])]
     t0 = time.perf_counter()
-    tune, tune_mr = eng.autotune(sample)
+    if a.tune:
+        f = [int(x) for x in a.tune.split(",")]
+        eng.tune = dict(zip(["block_trace", "stack", "block_shade", "blocks_per_cu", "tile", "iters_per_graph"], f))
+        tune, tune_mr = eng.tune, 0.0
+    else:
+        tune, tune_mr = eng.autotune(sample)
     tune_s = time.perf_counter() - t0
 
     def batch(keep=False):
@@ -285,7 +312,7 @@ def bench_batched(a, torch, lr):
     # ---- plain per-job launches, and the images compared bit for bit ----
     same = 0
     plain_lat, pr0, plain_s = [], eng.stats(), 0.0
-    for j, i in zip(jobs, ids):
+    for j, i in zip([] if a.skip_plain else jobs, ids):
         ta = time.perf_counter()
         pid = eng.plain(j)                      # returns when the job is done (the host waited)
         plain_s += time.perf_counter() - ta
@@ -298,13 +325,16 @@ def bench_batched(a, torch, lr):
         same += int(torch.equal(x.view(torch.int32), y.view(torch.int32)))
         eng.release(pid)
         eng.release(i)
+    if a.skip_plain:
+        for i in ids:
+            eng.release(i)
     pr1 = eng.stats()
     plain_rays = (pr1.rays_ext - pr0.rays_ext) + (pr1.rays_shadow - pr0.rays_shadow)
-    plain_mrays = plain_rays / plain_s / 1e6
+    plain_mrays = plain_rays / plain_s / 1e6 if plain_s > 0 else None
     # the plain form in shortest-first order instead of the batch's (the same per-job times)
     dur = np.diff(np.concatenate([[0.0], plain_lat]))
     order = np.argsort([w * h * spp for (_, w, h, spp) in plan], kind="stable")
-    plain_sjf = np.cumsum(dur[order])
+    plain_sjf = np.cumsum(dur[order]) if len(dur) == len(plan) else []
 
     # ---- kernel times (events around each kernel, iterations launched without the graph) and the
     # roofline of the dominant kernel: the extension walk's algorithmic bytes from its own counters
@@ -349,17 +379,19 @@ def bench_batched(a, torch, lr):
         "batched": {"mrays_s": round(value, 3), "mpaths_s": round(paths * steps / dt / 1e6, 3),
                     "p50_ms": round(percentile(lats, 50), 2), "p99_ms": round(percentile(lats, 99), 2),
                     "iterations_per_step": iters // steps},
-        "plain": {"mrays_s": round(plain_mrays, 3), "s": round(plain_s, 3), "p50_ms": round(percentile(plain_lat, 50), 2),
-                  "p99_ms": round(percentile(plain_lat, 99), 2), "rays": plain_rays,
-                  "shortest_first_p50_ms": round(percentile(plain_sjf, 50), 2),
-                  "shortest_first_p99_ms": round(percentile(plain_sjf, 99), 2)},
-        "speedup_vs_plain": round(value / plain_mrays, 3),
-        "equal_error": {"images_bit_identical": same, "of": len(jobs),
+        "plain": None if a.skip_plain else {
+            "mrays_s": rnd(plain_mrays, 3), "s": rnd(plain_s, 3), "p50_ms": rnd(percentile(plain_lat, 50), 2),
+            "p99_ms": rnd(percentile(plain_lat, 99), 2), "rays": plain_rays,
+            "shortest_first_p50_ms": rnd(percentile(plain_sjf, 50), 2),
+            "shortest_first_p99_ms": rnd(percentile(plain_sjf, 99), 2)},
+        "speedup_vs_plain": rnd(value / plain_mrays, 3) if plain_mrays else None,
+        "equal_error": {"images_bit_identical": same, "of": 0 if a.skip_plain else len(jobs),
                         "note": "same samples per (job, pixel, index), fixed-point sums: equal RMSE by construction"},
         "kernel_ms_per_launch": {n: round(kms[n] / max(kl[n], 1), 4) for n in names},
         "kernel_ms_total": {n: round(kms[n], 2) for n in names},
         "roofline": {"bound": "hbm", "kernel": "extend", "dominant": dom, "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(tune),
                      "alg_bytes_per_launch": alg // max(kl["extend"], 1), "node_fetches": nodes, "shape_tests": shapes,
                      "rays": rext, "launches": kl["extend"]},
     }
