@@ -338,7 +338,7 @@ def bench_batched(a, torch, lr):
 
     # ---- kernel times (events around each kernel, iterations launched without the graph) and the
     # roofline of the dominant kernel: the extension walk's algorithmic bytes from its own counters
-    # (node records x 32 B, shapes x 52 B, ray 32 B + hit 16 B per ray) over its time ----
+    # (64-byte node records x 64 B, shapes x 52 B, ray 32 B + hit 16 B per ray) over its time ----
     eng.profiling(True)
     k0 = eng.stats()
     batch()
@@ -355,7 +355,7 @@ def bench_batched(a, torch, lr):
     nodes = c1.node_fetches - c0.node_fetches
     shapes = c1.shape_tests - c0.shape_tests
     rext = c1.rays_ext - c0.rays_ext
-    alg = 48 * rext + 32 * nodes + 52 * shapes
+    alg = 48 * rext + 64 * nodes + 52 * shapes   # the engine's node records hold both children: 64 B
     dom = max(kms, key=kms.get)
     achieved = alg / (kms["extend"] * 1e-3) / 1e9
 

@@ -60,6 +60,20 @@ void lrs_batch_camera(uint64_t seed, int w, int h, float cam[12]);
  * {1, 4, 16, 64} (spp_lo = 4: over {4, 16, 64}, the serving mix). */
 void lrs_batch_plan(int njobs, uint64_t seed, int spp_lo, int *ntris, int *w, int *h, int *spp);
 
+/* ---- the sky (engine/scene/env.c) ---- */
+/* env_procedural_sky (env.c:510-537) with the sun's radiance a parameter: a w x w/2 equirect
+ * map (rows top to bottom, u = 0.5 looking along +z), blue above with haze at the horizon, a
+ * glow and a 1-degree sun at (sun_elev, sun_azim) degrees, a grey ground below; f32 as the
+ * reference's float build computes it. sun_le = (2600, 2400, 2100) is the reference's. */
+void lrs_sky(int w, float sun_elev, float sun_azim, const float sun_le[3], float *rgb);
+/* The sky's sampling tables (make_weights and prepare_base, env.c:125-164, 335-364): marg
+ * (h + 1), cond (h x (w + 1)), prob (w x h: each pixel's probability, pixel_prob), and the
+ * weights' sum. 0, or -1 on bad arguments or no memory. */
+int lrs_sky_tables(const float *rgb, int w, int h, float *marg, float *cond, float *prob, double *sum);
+
+/* the inverse of an affine 3 x 4 transform (rows), in double, stored as f32: -1 if singular */
+int lrs_affine_inverse(const float fwd[12], float inv[12]);
+
 #ifdef __cplusplus
 }
 #endif

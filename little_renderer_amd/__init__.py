@@ -110,6 +110,9 @@ def lib():
         "lrs_batch_scene": (None, [I, ctypes.c_uint64, P, P, P]),
         "lrs_batch_camera": (None, [ctypes.c_uint64, I, I, P]),
         "lrs_batch_plan": (None, [I, ctypes.c_uint64, I, P, P, P, P]),
+        "lrs_sky": (None, [I, F, F, P, P]),
+        "lrs_sky_tables": (I, [P, I, I, P, P, P, ctypes.POINTER(ctypes.c_double)]),
+        "lrs_affine_inverse": (I, [P, P]),
         "lrh_now_ms": (ctypes.c_double, []),
         "lrh_engine_create": (I, [P, I, I, ctypes.c_size_t, ctypes.POINTER(P)]),
         "lrh_engine_destroy": (None, [P]),
@@ -494,3 +497,33 @@ class Server:
         st = ServerStats()
         _check(lib().lrh_server_stats(self._s, ctypes.byref(st)))
         return st
+
+
+def sky(w, sun_elev=35.0, sun_azim=60.0, sun_le=(2600.0, 2400.0, 2100.0)):
+    """The procedural sky (lrs_sky): (w/2, w, 3) float32"""
+    import numpy as np
+    out = np.zeros((w // 2, w, 3), np.float32)
+    le = np.ascontiguousarray(sun_le, np.float32)
+    lib().lrs_sky(w, sun_elev, sun_azim, _ptr(le), _ptr(out))
+    return out
+
+
+def sky_tables(rgb):
+    """lrs_sky_tables: marg (h+1), cond (h*(w+1)), prob (w*h), sum"""
+    import numpy as np
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    h, w = rgb.shape[:2]
+    marg, cond, prob = np.zeros(h + 1, np.float32), np.zeros(h * (w + 1), np.float32), np.zeros(w * h, np.float32)
+    sm = ctypes.c_double()
+    if lib().lrs_sky_tables(_ptr(rgb), w, h, _ptr(marg), _ptr(cond), _ptr(prob), ctypes.byref(sm)) != 0:
+        raise LrhError("lrs_sky_tables failed")
+    return marg, cond, prob, sm.value
+
+
+def affine_inverse(fwd):
+    import numpy as np
+    f = np.ascontiguousarray(fwd, np.float32).reshape(12)
+    inv = np.zeros(12, np.float32)
+    if lib().lrs_affine_inverse(_ptr(f), _ptr(inv)) != 0:
+        raise LrhError("singular transform")
+    return inv

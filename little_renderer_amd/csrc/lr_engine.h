@@ -27,7 +27,7 @@
 // one job on the device (written by a kernel with the descriptor as its argument, so the
 // host's copy is never read late)
 struct JobDev {
-    const float4 *nodes;
+    const float4 *nodes;      // 64-byte records, children's boxes in the parent (walk2)
     const uint32_t *aux;
     const float4 *tris;
     const float4 *prims;
@@ -112,6 +112,7 @@ struct LrhPtScene {
     LrhContext *ctx;
     float4 *prims = nullptr, *tris = nullptr, *mats = nullptr, *boxes = nullptr;
     uint32_t *nodes = nullptr, *aux = nullptr;
+    float4 *wnodes = nullptr;                 // the same tree, children's boxes in the parent (walk2)
     float *tab = nullptr;
     uint16_t *mask = nullptr;
     int nprims = 0, ntris = 0, nmats = 0, light = -1, depth = 1;

@@ -377,7 +377,7 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
     const LrhPtScene *s = j->scene;
     JobDev jd;
     memset(&jd, 0, sizeof(jd));
-    jd.nodes = (const float4 *)s->nodes;
+    jd.nodes = s->wnodes;
     jd.aux = s->aux;
     jd.tris = s->tris;
     jd.prims = s->prims;

@@ -180,8 +180,8 @@ __global__ __launch_bounds__(BLK) void k_extend(EngArgs A)
         const uint32_t id = A.qid[cur][i];
         const JobDev &J = A.jobs[id >> LR_SLOT_BITS];
         const float4 r0 = A.qray[cur][2 * (size_t)i], r1 = A.qray[cur][2 * (size_t)i + 1];
-        const WalkHit h = walk<false, true, COUNT, STK>(J.nodes, J.root, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
-                                                        v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+        const WalkHit h = walk2<false, COUNT, STK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w,
+                                                   r1.w, S, A.overflow, cn, ct);
         A.hits[i] = make_float4(h.t, __uint_as_float(h.id), h.b1, h.b2);
     });
     if (COUNT) {
@@ -373,8 +373,8 @@ __global__ __launch_bounds__(BLK) void k_shadow(EngArgs A)
             const JobDev &J = A.jobs[jslot];
             const float4 r0 = A.sray[2 * (size_t)i], r1 = A.sray[2 * (size_t)i + 1];
             uint32_t cn, ct;
-            const WalkHit h = walk<true, true, false, STK>(J.nodes, J.root, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
-                                                           v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+            const WalkHit h = walk2<true, false, STK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w,
+                                                      r1.w, S, A.overflow, cn, ct);
             const float4 c = A.scon[i];
             ends = c.w != 0.0f;
             const bool lit = h.id == LR_NONE;

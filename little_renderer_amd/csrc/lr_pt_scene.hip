@@ -5,12 +5,42 @@
 
 #include <vector>
 
+#include "lr_device.h"
 #include "lr_engine.h"
+
+// The engine's node records (walk2, lr_walk.h): for each inner node of the reference layout,
+// both children's boxes and (index | leaf record). A one-shape tree (its root a leaf) gets a
+// record whose left child is that leaf and whose right child is an empty leaf.
+__global__ __launch_bounds__(256) void k_child_boxes(const float4 *__restrict__ nodes, int nn, float4 *__restrict__ w)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    const float4 lo = nodes[2 * (size_t)i], hi = nodes[2 * (size_t)i + 1];
+    const uint32_t b = __float_as_uint(hi.w);
+    if (b & LR_LEAF) {
+        if (i != 0) return;
+        w[0] = lo;
+        w[1] = hi;
+        w[2] = lo;
+        w[3] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(LR_LEAF));
+        return;
+    }
+    const uint32_t c[2] = {__float_as_uint(lo.w), b};
+    for (int k = 0; k < 2; k++) {
+        float4 clo = nodes[2 * (size_t)c[k]], chi = nodes[2 * (size_t)c[k] + 1];
+        if (!(__float_as_uint(chi.w) & LR_LEAF)) {
+            clo.w = __uint_as_float(c[k]);
+            chi.w = __uint_as_float(0u);
+        }
+        w[4 * (size_t)i + 2 * k] = clo;
+        w[4 * (size_t)i + 2 * k + 1] = chi;
+    }
+}
 
 static void pt_scene_release(LrhPtScene *s)
 {
     if (!s) return;
-    void *ps[] = {s->prims, s->tris, s->mats, s->boxes, s->nodes, s->aux, s->tab, s->mask};
+    void *ps[] = {s->prims, s->tris, s->mats, s->boxes, s->nodes, s->aux, s->tab, s->mask, s->wnodes};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete s;
@@ -115,8 +145,16 @@ extern "C" int lrh_pt_scene(LrhContext *c, const float *prims, int nprims, const
         return rc;
     }
     s->depth = st.depth;
+    const size_t nn = 2 * (size_t)n - 1;
+    if (hipMalloc((void **)&s->wnodes, nn * 64) != hipSuccess) {
+        (void)hipGetLastError();
+        pt_scene_release(s);
+        return lr_fail(LRH_ENOMEM, "lrh_pt_scene: device memory for the scene could not be had");
+    }
+    hipLaunchKernelGGL(k_child_boxes, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream,
+                       (const float4 *)s->nodes, (int)nn, s->wnodes);
     s->bytes = (size_t)nprims * 96 + (size_t)ntris * 48 + hm.size() * 4 + boxes.size() * 4 + (2 * (size_t)n - 1) * 32 +
-               (size_t)n * 4 + tab.size() * 4 + mask.size() * 2;
+               (size_t)n * 4 + tab.size() * 4 + mask.size() * 2 + (2 * (size_t)n - 1) * 64;
     // the tree is built on the context's stream; the engines that render the scene run on
     // streams of their own, so the scene is complete before it is handed out
     if (hipStreamSynchronize(c->stream) != hipSuccess) {

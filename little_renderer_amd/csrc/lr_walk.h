@@ -193,3 +193,98 @@ __device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32
     }
     return best;
 }
+
+// ---- the engine's walk: both children's boxes in the parent --------------------------------
+// A 64-byte record per inner node: the left child's (lo, a) (hi, b), then the right child's,
+// where an inner child's a is its node index (b = 0) and a leaf child's (a, b) is its leaf
+// record (first aux entry, LR_LEAF | count). Made from the reference's 32-byte nodes by
+// k_child_boxes (lr_pt_scene.hip); the same tree, so the same hits, with one dependent load per
+// level instead of two (the node, then its children).
+__device__ __forceinline__ void leaf_hits_(const uint32_t *__restrict__ aux, const float4 *__restrict__ tris,
+                                           const float4 *__restrict__ prims, uint32_t nprims, uint32_t first,
+                                           uint32_t cnt, V3 o, V3 d, float tmin, WalkHit &best, bool any, bool &done,
+                                           uint32_t &ct, bool count)
+{
+    for (uint32_t k = 0; k < cnt; k++) {
+        const uint32_t sid = aux[first + k];
+        if (count) ct++;
+        if (sid < nprims) {
+            const Prim p = load_prim(prims, sid);
+            float a = 0.0f, b = 0.0f;
+            const float t = p.q.w < 0.5f ? hit_sphere(p, o, d, tmin, best.t) : hit_quad(p, o, d, tmin, best.t, a, b);
+            if (t > 0.0f) {
+                best = WalkHit{t, sid, a, b};
+                if (any) {
+                    done = true;
+                    return;
+                }
+            }
+            continue;
+        }
+        const uint32_t ti = sid - nprims;
+        const float4 ta = tris[3 * (size_t)ti], tb = tris[3 * (size_t)ti + 1], tc = tris[3 * (size_t)ti + 2];
+        float t, u, v;
+        if (hit_tri(ta, tb, tc, o, d, tmin, best.t, t, u, v)) {
+            best = WalkHit{t, sid, u, v};
+            if (any) {
+                done = true;
+                return;
+            }
+        }
+    }
+}
+
+template <bool ANY, bool COUNT, int STK>
+__device__ __forceinline__ WalkHit walk2(const float4 *__restrict__ wnodes, const uint32_t *__restrict__ aux,
+                                         const float4 *__restrict__ tris, const float4 *__restrict__ prims,
+                                         uint32_t nprims, V3 o, V3 d, float tmin, float tmax, const WalkStack &S,
+                                         uint32_t *overflow, uint32_t &cn, uint32_t &ct)
+{
+    const V3 inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    WalkHit best{tmax, LR_NONE, 0.0f, 0.0f};
+    int sp = 0;
+    auto push = [&](uint32_t x) {
+        if (sp < STK) S.stk[sp * S.stride] = x;
+        else S.spill[(size_t)(sp - STK) * S.sstride] = x;
+        sp++;
+    };
+    push(0u);
+    bool done = false;
+    while (sp > 0) {
+        sp--;
+        const uint32_t ni = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+        const float4 l0 = wnodes[4 * (size_t)ni], l1 = wnodes[4 * (size_t)ni + 1];
+        const float4 r0 = wnodes[4 * (size_t)ni + 2], r1 = wnodes[4 * (size_t)ni + 3];
+        if (COUNT) cn++;
+        const float tl = box_t(l0, l1, o, inv, tmin, best.t);
+        const float tr = box_t(r0, r1, o, inv, tmin, best.t);
+        const uint32_t la = __float_as_uint(l0.w), lb = __float_as_uint(l1.w);
+        const uint32_t ra = __float_as_uint(r0.w), rb = __float_as_uint(r1.w);
+        const bool lleaf = (lb & LR_LEAF) != 0, rleaf = (rb & LR_LEAF) != 0;
+        // leaves at once, the nearer first (their hits shorten best.t for what follows)
+        const bool l_first = tl <= tr;
+        if (tl >= 0.0f && lleaf && l_first) {
+            leaf_hits_(aux, tris, prims, nprims, la, lb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
+            if (ANY && done) return best;
+        }
+        if (tr >= 0.0f && rleaf) {
+            leaf_hits_(aux, tris, prims, nprims, ra, rb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
+            if (ANY && done) return best;
+        }
+        if (tl >= 0.0f && lleaf && !l_first) {
+            leaf_hits_(aux, tris, prims, nprims, la, lb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
+            if (ANY && done) return best;
+        }
+        const bool gl = tl >= 0.0f && !lleaf && tl <= best.t, gr = tr >= 0.0f && !rleaf && tr <= best.t;
+        if (sp + 2 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
+            atomicOr(overflow, 1u);
+            break;
+        }
+        if (gl && gr) {
+            push(l_first ? ra : la);
+            push(l_first ? la : ra);
+        } else if (gl) push(la);
+        else if (gr) push(ra);
+    }
+    return best;
+}

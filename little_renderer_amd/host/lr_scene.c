@@ -433,3 +433,143 @@ void lrs_batch_plan(int njobs, uint64_t seed, int spp_lo, int *ntris, int *w, in
         spp[i] = spps[first + (int)(pcg_u32(&r) % (uint32_t)(4 - first))];
     }
 }
+
+/* ---- the sky (engine/scene/env.c) ---- */
+static const float SKY_PI = 3.14159265358979323846f;
+
+/* map_dir (env.c:48-52): the direction of map point (u, v), y up, u = 0.5 looking along +z */
+static void sky_dir(float u, float v, float d[3])
+{
+    const float ph = (u - 0.5f) * (2.0f * SKY_PI), th = v * SKY_PI;
+    const float st = sinf(th);
+    d[0] = st * sinf(ph);
+    d[1] = cosf(th);
+    d[2] = st * cosf(ph);
+}
+
+void lrs_sky(int w, float sun_elev, float sun_azim, const float sun_le[3], float *rgb)
+{
+    const int h = w / 2;
+    const float el = sun_elev * SKY_PI / 180.0f, az = sun_azim * SKY_PI / 180.0f;
+    const float sun[3] = {cosf(el) * sinf(az), sinf(el), cosf(el) * cosf(az)};
+    const float sun_cos = cosf(1.0f * SKY_PI / 180.0f);
+    const float zen[3] = {0.18f, 0.32f, 0.75f}, hor[3] = {0.75f, 0.8f, 0.85f}, glow[3] = {1.0f, 0.9f, 0.7f};
+    const float gnd_a[3] = {0.7f, 0.72f, 0.75f}, gnd_b[3] = {0.12f, 0.11f, 0.1f};
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            float d[3], c[3];
+            sky_dir((x + 0.5f) / w, (y + 0.5f) / h, d);
+            if (d[1] >= 0) {
+                const float t = 1 - d[1], t4 = t * t * t * t;
+                const float cs = d[0] * sun[0] + d[1] * sun[1] + d[2] * sun[2];
+                const float g = 0.6f * powf(cs > 0 ? cs : 0, 32);
+                for (int k = 0; k < 3; k++) {
+                    c[k] = zen[k] * (1 - t4) + hor[k] * t4;
+                    c[k] = c[k] + glow[k] * g;
+                    if (cs > sun_cos) c[k] = c[k] + sun_le[k];
+                }
+            } else {
+                float t = -d[1] * 4;
+                t = t < 1 ? t : 1;
+                for (int k = 0; k < 3; k++) c[k] = gnd_a[k] * (1 - t) + gnd_b[k] * t;
+            }
+            float *o = rgb + 3 * ((size_t)y * w + x);
+            o[0] = c[0];
+            o[1] = c[1];
+            o[2] = c[2];
+        }
+}
+
+int lrs_sky_tables(const float *rgb, int w, int h, float *marg, float *cond, float *prob, double *sum_out)
+{
+    if (!rgb || w < 2 || h < 1 || !marg || !cond || !prob) return -1;
+    float *lum = malloc(sizeof(float) * (size_t)w * h), *wm = malloc(sizeof(float) * (size_t)w * h);
+    float *rs = malloc(sizeof(float) * (size_t)h);
+    double *rows = malloc(sizeof(double) * (size_t)h);
+    if (!lum || !wm || !rs || !rows) {
+        free(lum); free(wm); free(rs); free(rows);
+        return -1;
+    }
+    /* make_weights (env.c:125-162): the largest luminance of each pixel's 3 x 3 (wrapping
+       across, stopping at the poles), and each row's sin(theta) */
+    for (int y = 0; y < h; y++) {
+        const float *p = rgb + 3 * (size_t)y * w;
+        float *l = lum + (size_t)y * w, *o = wm + (size_t)y * w;
+        for (int x = 0; x < w; x++) l[x] = 0.2126f * p[3 * x] + 0.7152f * p[3 * x + 1] + 0.0722f * p[3 * x + 2];
+        for (int x = 0; x < w; x++) {
+            const float a = l[x == 0 ? w - 1 : x - 1], b = l[x], c = l[x + 1 == w ? 0 : x + 1];
+            const float m = a > b ? a : b;
+            o[x] = m > c ? m : c;
+        }
+        rs[y] = sinf((y + 0.5f) * SKY_PI / h);
+    }
+    memcpy(lum, wm, sizeof(float) * (size_t)w * h);
+    for (int y = 0; y < h; y++) {
+        const float *a = lum + (size_t)(y > 0 ? y - 1 : y) * w, *b = lum + (size_t)y * w;
+        const float *c = lum + (size_t)(y + 1 < h ? y + 1 : y) * w;
+        float *o = wm + (size_t)y * w;
+        for (int x = 0; x < w; x++) {
+            float m = a[x] > b[x] ? a[x] : b[x];
+            m = m > c[x] ? m : c[x];
+            o[x] = m > 0 ? m : 0;
+        }
+    }
+    /* prepare_base's sampling tables (env.c:343-364): each row's cumulative distribution over
+       its pixels, the rows' over the map, weights lum_max * sin(theta) summed in double */
+    double sum = 0;
+    for (int y = 0; y < h; y++) {
+        float *c = cond + (size_t)y * (w + 1);
+        double acc = 0;
+        c[0] = 0;
+        for (int x = 0; x < w; x++) {
+            acc += (double)(wm[(size_t)y * w + x] * rs[y]);
+            c[x + 1] = (float)acc;
+        }
+        rows[y] = acc;
+        for (int x = 1; x <= w; x++) c[x] = acc > 0 ? (float)(c[x] / acc) : (float)x / w;
+        c[w] = 1;
+        sum += acc;
+    }
+    double acc = 0;
+    marg[0] = 0;
+    for (int y = 0; y < h; y++) {
+        acc += rows[y];
+        marg[y + 1] = sum > 0 ? (float)(acc / sum) : (float)(y + 1) / h;
+    }
+    marg[h] = 1;
+    /* pixel_prob (env.c:164-167), per pixel */
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++)
+            prob[(size_t)y * w + x] = sum > 0 ? (float)((double)(wm[(size_t)y * w + x] * rs[y]) / sum) : 0.0f;
+    if (sum_out) *sum_out = sum;
+    free(lum); free(wm); free(rs); free(rows);
+    return 0;
+}
+
+int lrs_affine_inverse(const float fwd[12], float inv[12])
+{
+    /* [A | t]^-1 = [A^-1 | -A^-1 t], in double, by the adjugate */
+    double a[3][3], t[3];
+    for (int r = 0; r < 3; r++) {
+        for (int c = 0; c < 3; c++) a[r][c] = fwd[4 * r + c];
+        t[r] = fwd[4 * r + 3];
+    }
+    const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                       a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    if (!(fabs(det) > 1e-30)) return -1;
+    double b[3][3];
+    b[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+    b[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+    b[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+    b[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+    b[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+    b[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+    b[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+    b[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+    b[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+    for (int r = 0; r < 3; r++) {
+        for (int c = 0; c < 3; c++) inv[4 * r + c] = (float)b[r][c];
+        inv[4 * r + 3] = (float)(-(b[r][0] * t[0] + b[r][1] * t[1] + b[r][2] * t[2]));
+    }
+    return 0;
+}

@@ -62,6 +62,12 @@ def lib():
             "lro_sampler_2d": (None, [P, I, I, U, U, I, P]),
             "lro_bsdf_case": (None, [P, I, P, P, P, P, F, F, F, P]),
             "lro_pt_image": (None, [P, P, I, I, U, I, U64, F, I, I, P]),
+            "lro_env_prepare": (I, [P, P, I, I, F, F]),
+            "lro_env_free": (None, [P]),
+            "lro_env_eval": (None, [P, P, P]),
+            "lro_env_sample": (None, [P, F, F, P, P, P]),
+            "lro_env_pdf": (F, [P, P]),
+            "lro_scene_trace": (None, [P, P, I, I, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -97,6 +103,13 @@ def ref(double=False):
         "refh_bsdf_case": (None, [P, I, P, P, P, F, F, F, P]),
         "refh_render": (None, [P, I, P, I, P, I, P, I, I, I, U64, I, I, P]),
         "refh_render_t": (None, [P, I, P, I, P, I, P, I, I, I, U64, I, I, P, P]),
+        "refh_env_new": (P, [P, I, I, F, F]),
+        "refh_env_free": (None, [P]),
+        "refh_env_tables": (None, [P, P, P, P, P]),
+        "refh_env_eval": (None, [P, P, P]),
+        "refh_env_sample": (None, [P, F, F, P, P, P]),
+        "refh_env_pdf": (F, [P, P]),
+        "refh_sky": (None, [I, F, F, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -259,10 +272,106 @@ class LroMat(ctypes.Structure):
                 ("transmission", ctypes.c_float)]
 
 
+class LroEnvS(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_int), ("h", ctypes.c_int), ("rgb", ctypes.c_void_p), ("marg", ctypes.c_void_p),
+                ("cond", ctypes.c_void_p), ("prob", ctypes.c_void_p), ("sum", ctypes.c_double), ("scale", ctypes.c_float),
+                ("rot", ctypes.c_float), ("rot_c", ctypes.c_float), ("rot_s", ctypes.c_float)]
+
+
+# an instance (LroInst): world -> object rows, kind (mesh >= 0, -1 unit sphere, -2 unit box), material
+INST_DTYPE = np.dtype([("inv", np.float32, 12), ("kind", np.int32), ("material", np.int32)])
+INST_SPHERE, INST_BOX = -1, -2
+
+
 class LroScene(ctypes.Structure):
     _fields_ = [("prims", ctypes.c_void_p), ("nprims", ctypes.c_int), ("tris", ctypes.c_void_p), ("ntris", ctypes.c_int),
                 ("mats", ctypes.c_void_p), ("nmats", ctypes.c_int), ("light", ctypes.c_int), ("eps", ctypes.c_float),
-                ("margin", ctypes.c_float), ("mask", ctypes.c_void_p), ("tab", ctypes.c_void_p)]
+                ("margin", ctypes.c_float), ("mask", ctypes.c_void_p), ("tab", ctypes.c_void_p), ("env", ctypes.c_void_p),
+                ("mtris", ctypes.c_void_p), ("mesh_first", ctypes.c_void_p), ("nmeshes", ctypes.c_int),
+                ("inst", ctypes.c_void_p), ("ninst", ctypes.c_int)]
+
+
+class Env:
+    """The oracle's sky (lro_env_prepare): rgb (h, w, 3) float32, its tables"""
+
+    def __init__(self, rgb, scale=1.0, rot=0.0):
+        self.rgb = np.ascontiguousarray(rgb, np.float32)
+        self.h, self.w = self.rgb.shape[:2]
+        self.s = LroEnvS()
+        assert lib().lro_env_prepare(ctypes.byref(self.s), ptr(self.rgb), self.w, self.h, scale, rot) == 0
+
+    def tables(self):
+        n = (self.h + 1, self.h * (self.w + 1), self.w * self.h)
+        out = []
+        for src, k in ((self.s.marg, n[0]), (self.s.cond, n[1]), (self.s.prob, n[2])):
+            out.append(np.ctypeslib.as_array(ctypes.cast(src, ctypes.POINTER(ctypes.c_float)), shape=(k,)).copy())
+        return out + [self.s.sum]
+
+    def eval(self, d):
+        o = np.zeros(3, np.float32)
+        lib().lro_env_eval(ctypes.byref(self.s), ptr(np.ascontiguousarray(d, np.float32)), ptr(o))
+        return o
+
+    def sample(self, u1, u2):
+        d, le, pdf = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(1, np.float32)
+        lib().lro_env_sample(ctypes.byref(self.s), u1, u2, ptr(d), ptr(pdf), ptr(le))
+        return d, float(pdf[0]), le
+
+    def pdf(self, d):
+        return lib().lro_env_pdf(ctypes.byref(self.s), ptr(np.ascontiguousarray(d, np.float32)))
+
+    def __del__(self):
+        try:
+            lib().lro_env_free(ctypes.byref(self.s))
+        except Exception:
+            pass
+
+
+class RefEnv:
+    """The reference's own sky (env_from_rgb of its float build), or None if not built"""
+
+    def __init__(self, rgb, scale=1.0, rot=0.0, double=False):
+        self.R = ref(double)
+        self.rgb = np.ascontiguousarray(rgb, np.float32)
+        self.h, self.w = self.rgb.shape[:2]
+        self.e = self.R.refh_env_new(ptr(self.rgb), self.w, self.h, scale, rot)
+
+    def tables(self):
+        marg = np.zeros(self.h + 1, np.float32)
+        cond = np.zeros(self.h * (self.w + 1), np.float32)
+        prob = np.zeros(self.w * self.h, np.float32)
+        sm = np.zeros(1, np.float64)
+        self.R.refh_env_tables(self.e, ptr(marg), ptr(cond), ptr(prob), ptr(sm))
+        return marg, cond, prob, float(sm[0])
+
+    def eval(self, d):
+        o = np.zeros(3, np.float32)
+        self.R.refh_env_eval(self.e, ptr(np.ascontiguousarray(d, np.float32)), ptr(o))
+        return o
+
+    def sample(self, u1, u2):
+        d, le, pdf = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(1, np.float32)
+        self.R.refh_env_sample(self.e, u1, u2, ptr(d), ptr(pdf), ptr(le))
+        return d, float(pdf[0]), le
+
+    def pdf(self, d):
+        return self.R.refh_env_pdf(self.e, ptr(np.ascontiguousarray(d, np.float32)))
+
+    def __del__(self):
+        try:
+            self.R.refh_env_free(self.e)
+        except Exception:
+            pass
+
+
+def ref_sky(w, elev, azim):
+    """env_procedural_sky of the reference's float build: (w/2, w, 3) float32, or None"""
+    R = ref(False)
+    if R is None:
+        return None
+    out = np.zeros((w // 2, w, 3), np.float32)
+    R.refh_sky(w, elev, azim, ptr(out))
+    return out
 
 
 def blue_noise():
@@ -282,15 +391,41 @@ def pt_image(scene, W, H, s0, ns, seed=0, clamp=0.0, max_depth=8, mis=True, tab=
     """lro_pt_image: the rgb sums of samples s0 .. s0+ns-1 of every pixel, (H, W, 3) float32.
     scene: dict of prims (n, 24), tris (m, 12), mats, light, cam, eps, margin; tab, mask: the
     BSDF tables and blue-noise ranks (as the device gets them)"""
-    prims = np.ascontiguousarray(scene["prims"], np.float32)
-    tris = np.ascontiguousarray(scene["tris"], np.float32)
-    mats = mats_of(scene["mats"])
-    keep = [prims, tris, mats, tab, mask]
-    sc = LroScene(ptr(prims), prims.shape[0], ptr(tris), tris.shape[0], ctypes.cast(mats, ctypes.c_void_p), len(mats),
-                  scene["light"], scene["eps"], scene["margin"], ptr(mask), ptr(tab))
+    sc, keep = scene_struct(scene, tab, mask)
     out = np.zeros((H, W, 3), np.float32)
     cam = np.ascontiguousarray(scene["cam"], np.float32)
     lib().lro_pt_image(ctypes.byref(sc), ptr(cam), W, H, s0, ns, seed, clamp, max_depth, 1 if mis else 0, ptr(out))
+    del keep
+    return out
+
+
+def scene_struct(scene, tab=None, mask=None):
+    """an LroScene for a scene dict (prims, tris, mats, light, eps, margin; optional "sky": an Env,
+    "meshes": (tris (m, 12), first (nmeshes + 1,)), "instances": an INST_DTYPE array); and the
+    arrays it points into, to keep alive"""
+    prims = np.ascontiguousarray(scene.get("prims", np.zeros((0, 24), np.float32)), np.float32).reshape(-1, 24)
+    tris = np.ascontiguousarray(scene.get("tris", np.zeros((0, 12), np.float32)), np.float32).reshape(-1, 12)
+    mats = mats_of(scene["mats"])
+    env = scene.get("sky")
+    mt, mf, inst = None, None, None
+    if scene.get("instances") is not None:
+        mt = np.ascontiguousarray(scene["meshes"][0], np.float32)
+        mf = np.ascontiguousarray(scene["meshes"][1], np.int32)
+        inst = np.ascontiguousarray(scene["instances"], INST_DTYPE)
+    keep = [prims, tris, mats, tab, mask, env, mt, mf, inst]
+    sc = LroScene(ptr(prims), prims.shape[0], ptr(tris), tris.shape[0], ctypes.cast(mats, ctypes.c_void_p), len(mats),
+                  scene.get("light", -1), scene["eps"], scene.get("margin", 1e-3), ptr(mask), ptr(tab),
+                  ctypes.addressof(env.s) if env is not None else None, ptr(mt), ptr(mf),
+                  0 if mf is None else len(mf) - 1, ptr(inst), 0 if inst is None else len(inst))
+    return sc, keep
+
+
+def scene_trace(scene, rays, any_hit=False):
+    """lro_scene_trace: brute-force hits (n, 4): t, shape id (u32), instance (u32), 0"""
+    sc, keep = scene_struct(scene)
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    out = np.zeros((r.shape[0], 4), np.float32)
+    lib().lro_scene_trace(ctypes.byref(sc), ptr(r), r.shape[0], 1 if any_hit else 0, ptr(out))
     del keep
     return out
 

@@ -496,7 +496,7 @@ void lro_bsdf_case(const LroMat *m, int outside, const float *tab, const float n
 }
 
 /* ---- the scene: shapes and their hits (intersect.h:66-112, intersect.c:4-67) ---- */
-typedef struct { real t; V p, n, ng; int front, prim; } Hit;
+typedef struct { real t; V p, n, ng; int front, prim, inst; } Hit;
 
 static int hit_sphere(const LroPrim *s, V o, V d, real tmin, real tmax, real *t_out)
 {
@@ -554,19 +554,119 @@ static int shape_test(const LroScene *s, int id, V o, V d, real tmin, real tmax,
     }
     return hit_tri(s->tris + 12 * (size_t)(id - s->nprims), o, d, tmin, tmax, t);
 }
-/* every shape in id order, the closest kept (intersect_world's brute force, accel.c:264-277) */
+/* ---- instances (trace.wgsl:186-262: to_object, hit_unit_sphere, hit_unit_box,
+   normal_to_world) ---- */
+static void to_object(const LroInst *in, V o, V d, V *oo, V *od)
+{
+    const float *a = in->inv, *b = in->inv + 4, *c = in->inv + 8;
+    *oo = v3(a[0] * o.x + a[1] * o.y + a[2] * o.z + a[3], b[0] * o.x + b[1] * o.y + b[2] * o.z + b[3],
+             c[0] * o.x + c[1] * o.y + c[2] * o.z + c[3]);
+    *od = v3(a[0] * d.x + a[1] * d.y + a[2] * d.z, b[0] * d.x + b[1] * d.y + b[2] * d.z,
+             c[0] * d.x + c[1] * d.y + c[2] * d.z);
+}
+/* an object-space normal in the world: by the inverse transpose, normalised */
+static V normal_to_world(const LroInst *in, V n)
+{
+    const float *a = in->inv, *b = in->inv + 4, *c = in->inv + 8;
+    return vnorm(v3(a[0] * n.x + b[0] * n.y + c[0] * n.z, a[1] * n.x + b[1] * n.y + c[1] * n.z,
+                    a[2] * n.x + b[2] * n.y + c[2] * n.z));
+}
+static int hit_unit_sphere(V o, V d, real tmin, real tmax, real *t_out)
+{
+    real a = vdot(d, d), hh = -vdot(d, o), c = vdot(o, o) - 1.0f;
+    real disc = hh * hh - a * c;
+    if (disc < 0) return 0;
+    real sq = sqrtf(disc);
+    real t = (hh - sq) / a;
+    if (t <= tmin || t >= tmax) {
+        t = (hh + sq) / a;
+        if (t <= tmin || t >= tmax) return 0;
+    }
+    *t_out = t;
+    return 1;
+}
+static inline real safe_inv(real x) { return fabsf(x) > 1e-12f ? 1.0f / x : 1e30f; }
+static int hit_unit_box(V o, V d, real tmin, real tmax, real *t_out)
+{
+    V inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    real t0x = (-1.0f - o.x) * inv.x, t0y = (-1.0f - o.y) * inv.y, t0z = (-1.0f - o.z) * inv.z;
+    real t1x = (1.0f - o.x) * inv.x, t1y = (1.0f - o.y) * inv.y, t1z = (1.0f - o.z) * inv.z;
+    real ax = t0x < t1x ? t0x : t1x, ay = t0y < t1y ? t0y : t1y, az = t0z < t1z ? t0z : t1z;
+    real bx = t0x > t1x ? t0x : t1x, by = t0y > t1y ? t0y : t1y, bz = t0z > t1z ? t0z : t1z;
+    real enter = ax > ay ? ax : ay;
+    enter = enter > az ? enter : az;
+    real leave = bx < by ? bx : by;
+    leave = leave < bz ? leave : bz;
+    if (leave < enter) return 0;
+    if (enter > tmin && enter < tmax) { *t_out = enter; return 1; }
+    if (leave > tmin && leave < tmax) { *t_out = leave; return 1; }
+    return 0;
+}
+/* the object-space outward normal of the unit box at p: the axis p lies farthest along */
+static V unit_box_normal(V p)
+{
+    real ax = fabsf(p.x), ay = fabsf(p.y), az = fabsf(p.z);
+    if (ax >= ay && ax >= az) return v3(p.x < 0 ? -1.0f : 1.0f, 0, 0);
+    if (ay >= az) return v3(0, p.y < 0 ? -1.0f : 1.0f, 0);
+    return v3(0, 0, p.z < 0 ? -1.0f : 1.0f);
+}
+/* the closest (any: the first) hit among instance k's shapes, in its space: local id */
+static int inst_test(const LroScene *s, int k, V o, V d, real tmin, real tmax, int any, real *t, int *local)
+{
+    const LroInst *in = &s->inst[k];
+    V oo, od;
+    to_object(in, o, d, &oo, &od);
+    if (in->kind == LRO_INST_SPHERE || in->kind == LRO_INST_BOX) {
+        real tt;
+        int h = in->kind == LRO_INST_SPHERE ? hit_unit_sphere(oo, od, tmin, tmax, &tt) : hit_unit_box(oo, od, tmin, tmax, &tt);
+        if (h) { *t = tt; *local = in->kind; }
+        return h;
+    }
+    int found = 0;
+    for (int i = s->mesh_first[in->kind]; i < s->mesh_first[in->kind + 1]; i++) {
+        real tt;
+        if (hit_tri(s->mtris + 12 * (size_t)i, oo, od, tmin, tmax, &tt)) {
+            found = 1;
+            tmax = tt;
+            *t = tt;
+            *local = i - s->mesh_first[in->kind];
+            if (any) return 1;
+        }
+    }
+    return found;
+}
+
+/* every shape in id order, then every instance, the closest kept (intersect_world's brute
+   force, accel.c:264-277) */
 static int scene_intersect(const LroScene *s, V o, V d, real tmin, real tmax, Hit *h)
 {
-    int found = -1;
+    int found = -1, finst = -1;
     int n = s->nprims + s->ntris;
     for (int i = 0; i < n; i++) {
         real t;
         if (shape_test(s, i, o, d, tmin, tmax, &t)) { found = i; tmax = t; }
     }
-    if (found < 0) return 0;
+    for (int k = 0; k < s->ninst; k++) {
+        real t;
+        int local;
+        if (inst_test(s, k, o, d, tmin, tmax, 0, &t, &local)) { found = local; finst = k; tmax = t; }
+    }
+    if (found < 0 && finst < 0) return 0;
     real t = tmax;
     V outward;
-    if (found < s->nprims) {
+    if (finst >= 0) {
+        const LroInst *in = &s->inst[finst];
+        V oo, od;
+        to_object(in, o, d, &oo, &od);
+        V no;
+        if (in->kind == LRO_INST_SPHERE) no = vadd(oo, vscale(od, t));
+        else if (in->kind == LRO_INST_BOX) no = unit_box_normal(vadd(oo, vscale(od, t)));
+        else {
+            const float *tr = s->mtris + 12 * (size_t)(s->mesh_first[in->kind] + found);
+            no = vcross(vload(tr + 4), vload(tr + 8));
+        }
+        outward = normal_to_world(in, no);
+    } else if (found < s->nprims) {
         const LroPrim *p = &s->prims[found];
         if (p->q[3] < 0.5f) outward = vscale(vsub(vadd(o, vscale(d, t)), vload(p->c)), 1.0f / p->c[3]);
         else outward = vload(p->n);
@@ -580,6 +680,7 @@ static int scene_intersect(const LroScene *s, V o, V d, real tmin, real tmax, Hi
     h->n = h->front ? outward : vneg(outward);
     h->ng = h->n;
     h->prim = found;
+    h->inst = finst;
     return 1;
 }
 static int scene_occluded(const LroScene *s, V o, V d, real tmin, real tmax)
@@ -588,6 +689,11 @@ static int scene_occluded(const LroScene *s, V o, V d, real tmin, real tmax)
     for (int i = 0; i < n; i++) {
         real t;
         if (shape_test(s, i, o, d, tmin, tmax, &t)) return 1;
+    }
+    for (int k = 0; k < s->ninst; k++) {
+        real t;
+        int local;
+        if (inst_test(s, k, o, d, tmin, tmax, 1, &t, &local)) return 1;
     }
     return 0;
 }
@@ -598,6 +704,226 @@ static int shape_material(const LroScene *s, int id)
     memcpy(&w, s->tris + 12 * (size_t)(id - s->nprims) + 11, 4);
     return (int)(w >> 26);
 }
+/* an instance's material, else its triangle's own */
+static int hit_material(const LroScene *s, const Hit *h)
+{
+    if (h->inst < 0) return shape_material(s, h->prim);
+    const LroInst *in = &s->inst[h->inst];
+    if (in->material >= 0 || in->kind < 0) return in->material >= 0 ? in->material : 0;
+    uint32_t w;
+    memcpy(&w, s->mtris + 12 * (size_t)(s->mesh_first[in->kind] + h->prim) + 11, 4);
+    return (int)(w >> 26);
+}
+
+void lro_scene_trace(const LroScene *s, const float *rays, int n, int any, float *hits)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; i++) {
+        const float *r = rays + 8 * (size_t)i;
+        V o = v3(r[0], r[1], r[2]), d = v3(r[4], r[5], r[6]);
+        float *o4 = hits + 4 * (size_t)i;
+        uint32_t none = 0xffffffffu, id, ins;
+        if (any) {
+            int hit = scene_occluded(s, o, d, r[3], r[7]);
+            o4[0] = hit ? 0.0f : r[7];
+            id = hit ? 0u : none;
+            ins = none;
+        } else {
+            Hit h;
+            if (scene_intersect(s, o, d, r[3], r[7], &h)) {
+                o4[0] = h.t;
+                /* a unit sphere or box instance: 0xfffffff0 / 0xfffffff1 (trace.wgsl:188-190) */
+                id = h.prim >= 0 ? (uint32_t)h.prim : 0xfffffff0u + (uint32_t)(-1 - h.prim);
+                ins = h.inst >= 0 ? (uint32_t)h.inst : none;
+            } else {
+                o4[0] = r[7];
+                id = none;
+                ins = none;
+            }
+        }
+        memcpy(&o4[1], &id, 4);
+        memcpy(&o4[2], &ins, 4);
+        o4[3] = 0;
+    }
+}
+
+/* ---- the sky (env.c) ---- */
+static const float SKYPI = 3.14159265358979323846f;
+static void env_uv(const LroEnv *e, V d, real *u, real *v)   /* env_uv + map_uv (env.c:36-57) */
+{
+    if (e->rot != 0) d = v3(d.x * e->rot_c - d.z * e->rot_s, d.y, d.x * e->rot_s + d.z * e->rot_c);
+    real y = d.y < -1 ? -1 : d.y > 1 ? 1 : d.y;
+    real uu = 0.5f + atan2f(d.x, d.z) * (0.5f / SKYPI);
+    *u = uu - floorf(uu);
+    *v = acosf(y) * (1.0f / SKYPI);
+}
+static V env_dir(const LroEnv *e, real u, real v)            /* env_dir + map_dir (env.c:48-68) */
+{
+    real ph = (u - 0.5f) * (2.0f * SKYPI), th = v * SKYPI;
+    real st = sinf(th);
+    V d = v3(st * sinf(ph), cosf(th), st * cosf(ph));
+    return e->rot != 0 ? v3(d.x * e->rot_c + d.z * e->rot_s, d.y, -d.x * e->rot_s + d.z * e->rot_c) : d;
+}
+static V env_bilerp(const LroEnv *e, real u, real v)         /* bilerp (env.c:70-88) */
+{
+    const int w = e->w, h = e->h;
+    real x = u * w - 0.5f, y = v * h - 0.5f;
+    int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    real fx = x - x0, fy = y - y0;
+    int y1 = y0 + 1;
+    if (y0 < 0) y0 = 0;
+    if (y1 > h - 1) y1 = h - 1;
+    if (y0 > h - 1) y0 = h - 1;
+    int xa = x0 < 0 ? x0 + w : (x0 >= w ? x0 - w : x0);
+    if (xa < 0 || xa >= w) xa = ((xa % w) + w) % w;
+    int xb = xa + 1 == w ? 0 : xa + 1;
+    const float *a = e->rgb + 3 * ((size_t)y0 * w + xa), *b = e->rgb + 3 * ((size_t)y0 * w + xb);
+    const float *c = e->rgb + 3 * ((size_t)y1 * w + xa), *dd = e->rgb + 3 * ((size_t)y1 * w + xb);
+    return v3((1 - fy) * ((1 - fx) * a[0] + fx * b[0]) + fy * ((1 - fx) * c[0] + fx * dd[0]),
+              (1 - fy) * ((1 - fx) * a[1] + fx * b[1]) + fy * ((1 - fx) * c[1] + fx * dd[1]),
+              (1 - fy) * ((1 - fx) * a[2] + fx * b[2]) + fy * ((1 - fx) * c[2] + fx * dd[2]));
+}
+static V env_eval(const LroEnv *e, V d)
+{
+    real u, v;
+    env_uv(e, d, &u, &v);
+    return vscale(env_bilerp(e, u, v), e->scale);
+}
+static int env_upper_bound(const float *c, int n, real x)    /* the last i with c[i] <= x (env.c:105-112) */
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (c[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+static V env_sample(const LroEnv *e, real u1, real u2, V *d, real *pdf)   /* env.c:171-189 */
+{
+    *pdf = 0;
+    if (!(e->sum > 0)) return v3(0, 0, 0);
+    int y = env_upper_bound(e->marg, e->h, u2);
+    real m0 = e->marg[y], m1 = e->marg[y + 1];
+    real dv = m1 > m0 ? (u2 - m0) / (m1 - m0) : 0.5f;
+    const float *c = e->cond + (size_t)y * (e->w + 1);
+    int x = env_upper_bound(c, e->w, u1);
+    real c0 = c[x], c1 = c[x + 1];
+    real du = c1 > c0 ? (u1 - c0) / (c1 - c0) : 0.5f;
+    if (dv > 0.9999f) dv = 0.9999f;
+    if (du > 0.9999f) du = 0.9999f;
+    real u = (x + du) / e->w, v = (y + dv) / e->h;
+    real st = sinf(v * SKYPI);
+    if (st < 1e-6f) return v3(0, 0, 0);
+    *d = env_dir(e, u, v);
+    *pdf = e->prob[(size_t)y * e->w + x] * e->w * e->h / ((2.0f * SKYPI * SKYPI) * st);
+    return env_eval(e, *d);
+}
+static real env_pdf(const LroEnv *e, V d)                    /* env.c:191-203 */
+{
+    if (!(e->sum > 0)) return 0;
+    real u, v;
+    env_uv(e, d, &u, &v);
+    real st = sinf(v * SKYPI);
+    if (st < 1e-6f) return 0;
+    int x = (int)(u * e->w), y = (int)(v * e->h);
+    if (x > e->w - 1) x = e->w - 1;
+    if (y > e->h - 1) y = e->h - 1;
+    return e->prob[(size_t)y * e->w + x] * e->w * e->h / ((2.0f * SKYPI * SKYPI) * st);
+}
+
+int lro_env_prepare(LroEnv *e, const float *rgb, int w, int h, float scale, float rot)
+{
+    memset(e, 0, sizeof(*e));
+    e->w = w;
+    e->h = h;
+    e->rgb = rgb;
+    e->scale = scale;
+    e->rot = rot;
+    e->rot_c = cosf(rot);
+    e->rot_s = sinf(rot);
+    e->marg = malloc(sizeof(float) * (size_t)(h + 1));
+    e->cond = malloc(sizeof(float) * (size_t)h * (w + 1));
+    e->prob = malloc(sizeof(float) * (size_t)w * h);
+    float *lum = malloc(sizeof(float) * (size_t)w * h), *wm = malloc(sizeof(float) * (size_t)w * h);
+    float *rs = malloc(sizeof(float) * (size_t)h);
+    double *rows = malloc(sizeof(double) * (size_t)h);
+    if (!e->marg || !e->cond || !e->prob || !lum || !wm || !rs || !rows) {
+        free(lum); free(wm); free(rs); free(rows);
+        lro_env_free(e);
+        return -1;
+    }
+    /* make_weights (env.c:125-162) */
+    for (int y = 0; y < h; y++) {
+        const float *p = rgb + 3 * (size_t)y * w;
+        float *l = lum + (size_t)y * w, *o = wm + (size_t)y * w;
+        for (int x = 0; x < w; x++) l[x] = 0.2126f * p[3 * x] + 0.7152f * p[3 * x + 1] + 0.0722f * p[3 * x + 2];
+        for (int x = 0; x < w; x++) {
+            float a = l[x == 0 ? w - 1 : x - 1], b = l[x], c = l[x + 1 == w ? 0 : x + 1];
+            float m = a > b ? a : b;
+            o[x] = m > c ? m : c;
+        }
+        rs[y] = sinf((y + 0.5f) * SKYPI / h);
+    }
+    memcpy(lum, wm, sizeof(float) * (size_t)w * h);
+    for (int y = 0; y < h; y++) {
+        const float *a = lum + (size_t)(y > 0 ? y - 1 : y) * w, *b = lum + (size_t)y * w;
+        const float *c = lum + (size_t)(y + 1 < h ? y + 1 : y) * w;
+        float *o = wm + (size_t)y * w;
+        for (int x = 0; x < w; x++) {
+            float m = a[x] > b[x] ? a[x] : b[x];
+            m = m > c[x] ? m : c[x];
+            o[x] = m > 0 ? m : 0;
+        }
+    }
+    /* the sampling tables of prepare_base (env.c:343-364) */
+    e->sum = 0;
+    for (int y = 0; y < h; y++) {
+        float *c = e->cond + (size_t)y * (w + 1);
+        double acc = 0;
+        c[0] = 0;
+        for (int x = 0; x < w; x++) {
+            acc += wm[(size_t)y * w + x] * rs[y];
+            c[x + 1] = (float)acc;
+        }
+        rows[y] = acc;
+        for (int x = 1; x <= w; x++) c[x] = acc > 0 ? (float)(c[x] / acc) : (float)x / w;
+        c[w] = 1;
+        e->sum += acc;
+    }
+    double acc = 0;
+    e->marg[0] = 0;
+    for (int y = 0; y < h; y++) {
+        acc += rows[y];
+        e->marg[y + 1] = e->sum > 0 ? (float)(acc / e->sum) : (float)(y + 1) / h;
+    }
+    e->marg[h] = 1;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++)
+            e->prob[(size_t)y * w + x] = e->sum > 0 ? (float)((float)(wm[(size_t)y * w + x] * rs[y]) / e->sum) : 0.0f;
+    free(lum); free(wm); free(rs); free(rows);
+    return 0;
+}
+void lro_env_free(LroEnv *e)
+{
+    free(e->marg); free(e->cond); free(e->prob);
+    e->marg = e->cond = e->prob = NULL;
+}
+void lro_env_eval(const LroEnv *e, const float d[3], float out[3])
+{
+    V c = env_eval(e, v3(d[0], d[1], d[2]));
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+}
+void lro_env_sample(const LroEnv *e, float u1, float u2, float d[3], float *pdf, float le[3])
+{
+    V dd = v3(0, 0, 0);
+    real p;
+    V c = env_sample(e, u1, u2, &dd, &p);
+    d[0] = dd.x; d[1] = dd.y; d[2] = dd.z;
+    *pdf = p;
+    le[0] = c.x; le[1] = c.y; le[2] = c.z;
+}
+float lro_env_pdf(const LroEnv *e, const float d[3]) { return env_pdf(e, v3(d[0], d[1], d[2])); }
 
 /* ---- next event estimation to the quad lamp (tracer.c:30-81) ---- */
 static inline real power_heuristic(real pa, real pb)
@@ -630,9 +956,25 @@ static V sample_direct_light(const LroScene *s, const Hit *h, Bsdf *b, V v, int 
 
 static int rough_part(const Bsdf *b) { return !b->smooth || (1 - b->metallic) * (1 - b->transmission) > 0; }
 
+/* next event estimation towards the sky (sample_direct_env, tracer.c:87-101) */
+static V sample_direct_env(const LroScene *s, const Hit *h, Bsdf *b, V v, int mis, real ua, real ub)
+{
+    V wi;
+    real pdf;
+    V le = env_sample(s->env, ua, ub, &wi, &pdf);
+    if (!(pdf > 0) || !(vmax(le) > 0)) return v3(0, 0, 0);
+    real bpdf;
+    V f = bsdf_eval(b, h->n, v, wi, &bpdf);
+    if (!(vmax(f) > 0)) return v3(0, 0, 0);
+    if (scene_occluded(s, h->p, wi, s->eps, 1e30f)) return v3(0, 0, 0);
+    real g = fabsf(vdot(h->n, wi)) / pdf;
+    if (mis) g *= power_heuristic(pdf, bpdf);
+    return vscale(vmul(f, le), g);
+}
+
 enum { EMIT_FULL, EMIT_NONE, EMIT_MIS };
 
-/* trace_path_sampled (tracer.c:364-509) without media, sky, glowing shapes or textures */
+/* trace_path_sampled (tracer.c:364-509) without media, glowing shapes or textures */
 static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis)
 {
     V radiance = v3(0, 0, 0), throughput = v3(1, 1, 1);
@@ -641,8 +983,15 @@ static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis
     if (mis) d = vnorm(d);
     for (int depth = 0;; depth++) {
         Hit h;
-        if (!scene_intersect(s, o, d, s->eps, 1e30f, &h)) break;
-        const LroMat *m = &s->mats[shape_material(s, h.prim)];
+        if (!scene_intersect(s, o, d, s->eps, 1e30f, &h)) {   /* escaped: the sky (tracer.c:415-421) */
+            if (s->env && emit != EMIT_NONE) {
+                V le = env_eval(s->env, d);
+                if (emit == EMIT_MIS) le = vscale(le, power_heuristic(bounce_pdf, env_pdf(s->env, d)));
+                radiance = vadd(radiance, vmul(throughput, le));
+            }
+            break;
+        }
+        const LroMat *m = &s->mats[hit_material(s, &h)];
         if (m->type == LRO_MAT_LIGHT) {
             if (h.front) {
                 if (emit == EMIT_FULL) radiance = vadd(radiance, vmul(throughput, vload(m->emission)));
@@ -660,10 +1009,20 @@ static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis
         if (depth >= max_depth) break;
         V v = vneg(vnorm(d));
         Bsdf b = bsdf_make(m, h.front, s->tab);
-        if (rough_part(&b) && s->light >= 0) {
+        if (rough_part(&b)) {
             real la, lb;
-            sampler_2d(sm, LRO_SAMPLE_LIGHT(depth), &la, &lb);
-            radiance = vadd(radiance, vmul(throughput, sample_direct_light(s, &h, &b, v, mis, lb, la)));
+            if (s->light >= 0) {
+                sampler_2d(sm, LRO_SAMPLE_LIGHT(depth), &la, &lb);
+                radiance = vadd(radiance, vmul(throughput, sample_direct_light(s, &h, &b, v, mis, lb, la)));
+            }
+            if (s->env) {   /* env_numbers (tracer.c:232-235) */
+                if (s->light < 0) sampler_2d(sm, LRO_SAMPLE_LIGHT(depth), &la, &lb);
+                else {
+                    la = lro_rng_float(sm->rng);
+                    lb = lro_rng_float(sm->rng);
+                }
+                radiance = vadd(radiance, vmul(throughput, sample_direct_env(s, &h, &b, v, mis, la, lb)));
+            }
         }
         bsdf_view(&b, vdot(v, h.n));
         real ur = 0, ul = 0, da = 0, db = 0;

@@ -32,6 +32,32 @@ typedef struct {
     float c[4], q[4], u[4], v[4], n[4], w[4];
 } LroPrim;
 
+/* the sky (engine/scene/env.h): an equirect map w x h (w = 2h) and its sampling tables */
+typedef struct {
+    int w, h;
+    const float *rgb;                     /* w * h * 3, rows top to bottom */
+    float *marg, *cond, *prob;            /* h + 1; h * (w + 1); w * h (pixel_prob) */
+    double sum;
+    float scale, rot, rot_c, rot_s;
+} LroEnv;
+/* the tables of env_prepare's sampling part (env.c:125-164, 335-364); 0 or -1 */
+int lro_env_prepare(LroEnv *e, const float *rgb, int w, int h, float scale, float rot);
+void lro_env_free(LroEnv *e);
+/* env_eval, env_sample, env_pdf (env.c:54-203) */
+void lro_env_eval(const LroEnv *e, const float d[3], float out[3]);
+void lro_env_sample(const LroEnv *e, float u1, float u2, float d[3], float *pdf, float le[3]);
+float lro_env_pdf(const LroEnv *e, const float d[3]);
+
+/* an instance of a two-level scene: world -> object rows (3 x 4), its kind (a mesh index >= 0,
+ * LRO_INST_SPHERE: the unit sphere, LRO_INST_BOX: the box [-1, 1]^3), its material (-1: the
+ * mesh triangles' own) */
+#define LRO_INST_SPHERE (-1)
+#define LRO_INST_BOX (-2)
+typedef struct {
+    float inv[12];
+    int kind, material;
+} LroInst;
+
 typedef struct {
     const LroPrim *prims; int nprims;
     const float *tris; int ntris;        /* 12 floats each, material in the top 6 bits of word 11 */
@@ -40,7 +66,18 @@ typedef struct {
     float eps, margin;                    /* ray offset and shadow margin (tracer.c:15-23) */
     const uint16_t *mask;                 /* the blue-noise ranks (64 x 64) */
     const float *tab;                     /* the BSDF tables: E[32][32], B[32][32], E_avg[32], B_avg[32] */
+    const LroEnv *env;                    /* the sky, or NULL (black) */
+    /* instances (a two-level scene): the meshes' triangles in object space (mesh m: triangles
+       mesh_first[m] .. mesh_first[m + 1] - 1 of mtris) and the instances, besides the shapes above */
+    const float *mtris; const int *mesh_first; int nmeshes;
+    const LroInst *inst; int ninst;
 } LroScene;
+
+/* The closest hit of n rays (8 floats each) in a scene, by brute force over every shape and
+ * every instance's shapes: hit = t, shape id (a world shape, or the instance's local triangle,
+ * 0xfffffff0 its unit sphere, 0xfffffff1 its unit box; 0xffffffff: no hit), instance (u32,
+ * 0xffffffff: none), unused */
+void lro_scene_trace(const LroScene *s, const float *rays, int n, int any, float *hits);
 
 uint32_t lro_sampler_hash(uint32_t x);
 uint32_t lro_sampler_owen(uint32_t x, uint32_t seed);

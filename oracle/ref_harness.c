@@ -253,3 +253,50 @@ void refh_render_t(const float *tris, int ntris, const float *lamp, int lamp_mat
     free(out);
     scene_free(&s);
 }
+
+/* ---- the reference's sky (engine/scene/env.c), for pinning the oracle's ---- */
+#include "scene/env.h"
+
+void *refh_env_new(const float *rgb, int w, int h, float scale, float rot)
+{
+    float *c = malloc(sizeof(float) * 3 * (size_t)w * h);
+    if (!c) return NULL;
+    memcpy(c, rgb, sizeof(float) * 3 * (size_t)w * h);
+    Env *e = env_from_rgb(c, w, h);
+    if (!e) return NULL;
+    e->scale = scale;
+    env_set_rot(e, rot);
+    return e;
+}
+void refh_env_free(void *e) { env_free((Env *)e); }
+void refh_env_tables(void *ev, float *marg, float *cond, float *prob, double *sum)
+{
+    Env *e = (Env *)ev;
+    memcpy(marg, e->marg, sizeof(float) * (size_t)(e->h + 1));
+    memcpy(cond, e->cond, sizeof(float) * (size_t)e->h * (e->w + 1));
+    for (int y = 0; y < e->h; y++)
+        for (int x = 0; x < e->w; x++) prob[(size_t)y * e->w + x] = (float)env_pixel_prob(e, x, y);
+    *sum = e->sum;
+}
+void refh_env_eval(void *e, const float *d, float *out)
+{
+    Vec3 c = env_eval((Env *)e, vec3(d[0], d[1], d[2]));
+    out[0] = (float)c.x; out[1] = (float)c.y; out[2] = (float)c.z;
+}
+void refh_env_sample(void *e, float u1, float u2, float *d, float *pdf, float *le)
+{
+    Vec3 dd = vec3(0, 0, 0);
+    real p = 0;
+    Vec3 c = env_sample((Env *)e, u1, u2, &dd, &p);
+    d[0] = (float)dd.x; d[1] = (float)dd.y; d[2] = (float)dd.z;
+    *pdf = (float)p;
+    le[0] = (float)c.x; le[1] = (float)c.y; le[2] = (float)c.z;
+}
+float refh_env_pdf(void *e, const float *d) { return (float)env_pdf((Env *)e, vec3(d[0], d[1], d[2])); }
+void refh_sky(int w, float elev, float azim, float *rgb)
+{
+    Env *e = env_procedural_sky(w, elev, azim);
+    if (!e) return;
+    memcpy(rgb, e->rgb, sizeof(float) * 3 * (size_t)e->w * e->h);
+    env_free(e);
+}
