@@ -140,6 +140,41 @@ void lrh_pt_scene_free(LrhPtScene *s);
  * the device memory it holds (what a server's budget counts) */
 int lrh_pt_scene_info(const LrhPtScene *s, int *depth, int *nshapes, size_t *bytes);
 
+/* The sky (env.h: Scene.env): an equirect map of w x h radiance triples (w = 2h, rows top to
+ * bottom, u = 0.5 looking along +z), times scale, turned by rot radians about y; rays that leave
+ * the scene see it (env_eval) and next event estimation samples it by luminance x sin(theta)
+ * with MIS (env_sample, env_pdf, sample_direct_env: engine/scene/env.c:54-203,
+ * engine/rt/tracer.c:87-101, 415-421). rgb = NULL removes it. A scene with a lamp takes none
+ * here (one light per scene). Host memory; the library keeps a copy and the sampling tables. */
+int lrh_pt_scene_sky(LrhPtScene *s, const float *rgb, int w, int h, float scale, float rot);
+
+/* Closest (any = 0) or first (any = 1) hits of n rays (d_rays, lr_hip.h's ray layout) in a
+ * resident scene, through the walk its jobs use (scene_intersect / scene_occluded,
+ * engine/rt/accel.h:43-45, for a batch): hit t, shape id (an instanced scene: the global
+ * triangle index, 0xfffffff0 / 0xfffffff1 for a unit sphere / box), then the instance (u32;
+ * other scenes: b1), b2. Asynchronous on the context's stream. */
+int lrh_pt_scene_trace(LrhContext *c, const LrhPtScene *s, const float *d_rays, int n, int any, float *d_hits);
+
+/* ---- two-level scenes (trace.wgsl:186-392: instances of meshes, unit spheres and boxes) ---- */
+#define LRH_INST_SPHERE (-1)   /* the unit sphere */
+#define LRH_INST_BOX (-2)      /* the box [-1, 1]^3 */
+/* an instance: object -> world transform (3 x 4 rows), its kind (a mesh index, or the above),
+ * its material (-1: the mesh triangles' own) */
+typedef struct {
+    float xform[12];
+    int kind, material;
+} LrhInstance;
+/* A scene of instances: the meshes' triangles in object space (mesh m: triangles
+ * mesh_first[m] .. mesh_first[m + 1] - 1 of mtris, lr_hip.h's layout), one BVH per mesh built
+ * on the device (LRH_LBVH or LRH_PLOC), and a top-level LBVH over the instances' world boxes
+ * (inverse transforms in double on the host). No lamp: light it with lrh_pt_scene_sky. */
+int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const int *mesh_first, int nmeshes, const LrhMaterial *mats,
+                           int nmats, const LrhInstance *inst, int ninst, float eps, int builder, LrhPtScene **out);
+/* New poses for the same instances: the top level is rebuilt on the device (per frame, as the
+ * reference rebuilds its TLAS, trace_gpu.c:1371-1379). No job of the scene may be running.
+ * stats (optional): the top level's build. */
+int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int ninst, int builder, LrhBuildStats *stats);
+
 /* Path traces spp samples (indices s0 .. s0+spp-1) of every pixel of a W x H image and adds
  * them to d_accum (float4 per pixel, row-major: the sum of the samples' rgb, and their count),
  * as gpu_frame_trace (engine/app/gpu_frame.c:49-83) and trace_image (engine/rt/tracer.c:329-356)

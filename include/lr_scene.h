@@ -71,6 +71,18 @@ void lrs_sky(int w, float sun_elev, float sun_azim, const float sun_le[3], float
  * weights' sum. 0, or -1 on bad arguments or no memory. */
 int lrs_sky_tables(const float *rgb, int w, int h, float *marg, float *cond, float *prob, double *sum);
 
+/* ---- meshes for instancing (object space; each returns its triangle count) ---- */
+/* a unit UV sphere: 2 x slices x (stacks - 1) triangles facing out (slices 30, stacks 17: 960) */
+int lrs_uv_sphere(int slices, int stacks, int material, float *tris);
+/* the box [-1, 1]^3: 12 triangles facing out */
+int lrs_box_mesh(int material, float *tris);
+/* the convex hull of npts points drawn on the unit sphere from PCG32 (seed, stream 6): 2 npts - 4
+ * triangles facing out (64 points: 124); -1 if out of memory */
+int lrs_hull(int npts, uint64_t seed, int material, float *tris);
+/* a seeded blob: a UV sphere of about ntris triangles (room for ntris + 4 * sqrt(ntris) + 64),
+ * its radius bumped by four waves drawn from PCG32 (seed, stream 7) */
+int lrs_blob(int ntris, uint64_t seed, int material, float *tris);
+
 /* the inverse of an affine 3 x 4 transform (rows), in double, stored as f32: -1 if singular */
 int lrs_affine_inverse(const float fwd[12], float inv[12]);
 

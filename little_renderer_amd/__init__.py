@@ -104,6 +104,10 @@ def lib():
         "lrs_cornell": (None, [ctypes.c_uint64, I, I, P, P, P, P]),
         "lrh_pt_scene": (I, [P, P, I, P, I, P, I, I, F, F, I, ctypes.POINTER(P)]),
         "lrh_pt_scene_free": (None, [P]),
+        "lrh_pt_scene_sky": (I, [P, P, I, I, F, F]),
+        "lrh_pt_scene_instanced": (I, [P, P, P, I, P, I, P, I, F, I, ctypes.POINTER(P)]),
+        "lrh_pt_scene_update": (I, [P, P, I, I, ctypes.POINTER(BuildStats)]),
+        "lrh_pt_scene_trace": (I, [P, P, P, I, I, P]),
         "lrh_pt_scene_info": (I, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_size_t)]),
         "lrh_pt_render": (I, [P, P, P, I, I, I, U, ctypes.c_uint64, I, I, F, P]),
@@ -113,6 +117,10 @@ def lib():
         "lrs_sky": (None, [I, F, F, P, P]),
         "lrs_sky_tables": (I, [P, I, I, P, P, P, ctypes.POINTER(ctypes.c_double)]),
         "lrs_affine_inverse": (I, [P, P]),
+        "lrs_uv_sphere": (I, [I, I, I, P]),
+        "lrs_box_mesh": (I, [I, P]),
+        "lrs_hull": (I, [I, ctypes.c_uint64, I, P]),
+        "lrs_blob": (I, [I, ctypes.c_uint64, I, P]),
         "lrh_now_ms": (ctypes.c_double, []),
         "lrh_engine_create": (I, [P, I, I, ctypes.c_size_t, ctypes.POINTER(P)]),
         "lrh_engine_destroy": (None, [P]),
@@ -212,8 +220,34 @@ def cornell(w, h, seed=0):
     return {"tris": tris, "prims": prims, "mats": mats, "light": 0, "cam": cam, "eps": 1e-4, "margin": 1e-3}
 
 
+INST_SPHERE, INST_BOX = -1, -2
+
+
+def _inst_dtype():
+    import numpy as np
+    return np.dtype([("xform", np.float32, 12), ("kind", np.int32), ("material", np.int32)])
+
+
+INSTANCE_DTYPE = _inst_dtype()   # an LrhInstance: xform (object -> world, 3 x 4 rows), kind, material
+
+
 class PtScene:
-    """A scene resident on the device for the path tracer (lrh_pt_scene)."""
+    """A scene resident on the device for the path tracer (lrh_pt_scene, lrh_pt_scene_instanced)."""
+
+    @classmethod
+    def instanced(cls, ctx, mtris, mesh_first, mats, instances, eps=1e-4, builder=PLOC):
+        """lrh_pt_scene_instanced: meshes (triangles in object space, mesh_first: nmeshes + 1
+        offsets) and an INSTANCE_DTYPE array of instances"""
+        import numpy as np
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        mt = np.ascontiguousarray(mtris, np.float32).reshape(-1, 12)
+        mf = np.ascontiguousarray(mesh_first, np.int32)
+        inst = np.ascontiguousarray(instances, INSTANCE_DTYPE)
+        self._h = ctypes.c_void_p()
+        _check(lib().lrh_pt_scene_instanced(ctx._c, _ptr(mt), _ptr(mf), len(mf) - 1, ctypes.cast(mats, ctypes.c_void_p),
+                                            len(mats), _ptr(inst), len(inst), eps, builder, ctypes.byref(self._h)))
+        return self
 
     def __init__(self, ctx, prims, tris, mats, light, eps, margin, builder=PLOC):
         import numpy as np
@@ -223,6 +257,34 @@ class PtScene:
         self._h = ctypes.c_void_p()
         _check(lib().lrh_pt_scene(ctx._c, _ptr(pr), pr.shape[0], _ptr(tr), tr.shape[0], ctypes.cast(mats, ctypes.c_void_p),
                                   len(mats), light, eps, margin, builder, ctypes.byref(self._h)))
+
+    def set_sky(self, rgb, scale=1.0, rot=0.0):
+        """lrh_pt_scene_sky: rgb (h, 2h, 3) float32, or None to remove it"""
+        import numpy as np
+        if rgb is None:
+            _check(lib().lrh_pt_scene_sky(self._h, None, 0, 0, 1.0, 0.0))
+            return
+        m = np.ascontiguousarray(rgb, np.float32)
+        _check(lib().lrh_pt_scene_sky(self._h, _ptr(m), m.shape[1], m.shape[0], scale, rot))
+
+    def update(self, instances, builder=LBVH):
+        """lrh_pt_scene_update: new poses (an INSTANCE_DTYPE array); the top level's BuildStats"""
+        import numpy as np
+        inst = np.ascontiguousarray(instances, INSTANCE_DTYPE)
+        st = BuildStats()
+        _check(lib().lrh_pt_scene_update(self._h, _ptr(inst), len(inst), builder, ctypes.byref(st)))
+        return st
+
+    def trace(self, rays, any_hit=False):
+        """lrh_pt_scene_trace on host rays (n, 8): hits (n, 4) as a numpy array"""
+        import numpy as np
+        import torch
+        r = torch.from_numpy(np.ascontiguousarray(rays, np.float32).reshape(-1, 8)).cuda()
+        h = torch.zeros((r.shape[0], 4), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        _check(lib().lrh_pt_scene_trace(self.ctx._c, self._h, _ptr(r), r.shape[0], 1 if any_hit else 0, _ptr(h)))
+        self.ctx.sync()
+        return h.cpu().numpy()
 
     def info(self):
         """(tree depth, shapes, device bytes)"""
@@ -527,3 +589,98 @@ def affine_inverse(fwd):
     if lib().lrs_affine_inverse(_ptr(f), _ptr(inv)) != 0:
         raise LrhError("singular transform")
     return inv
+
+
+def uv_sphere(slices=30, stacks=17, material=0):
+    import numpy as np
+    out = np.zeros((2 * slices * (stacks - 1), 12), np.float32)
+    n = lib().lrs_uv_sphere(slices, stacks, material, _ptr(out))
+    return out[:n]
+
+
+def box_mesh(material=0):
+    import numpy as np
+    out = np.zeros((12, 12), np.float32)
+    lib().lrs_box_mesh(material, _ptr(out))
+    return out
+
+
+def hull(npts, seed, material=0):
+    import numpy as np
+    out = np.zeros((npts * npts, 12), np.float32)
+    n = lib().lrs_hull(npts, seed, material, _ptr(out))
+    if n < 0:
+        raise LrhError("lrs_hull: out of memory")
+    return out[:n].copy()
+
+
+def blob(ntris, seed, material=0):
+    import numpy as np
+    out = np.zeros((ntris + 4 * int(ntris ** 0.5) + 64, 12), np.float32)
+    n = lib().lrs_blob(ntris, seed, material, _ptr(out))
+    return out[:n].copy()
+
+
+def pose(rng, scale, at):
+    """an object -> world transform (3 x 4): a uniformly random rotation, scale (3,), translation"""
+    import numpy as np
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    r = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                  [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                  [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    return np.hstack([r @ np.diag(scale), np.asarray(at, np.float64)[:, None]]).astype(np.float32)
+
+
+def instanced_workload(seed=0, frame=0, n_dynamic=2000, n_static=3000, n_props=20, n_hulls=8):
+    """The instanced workload: meshes (ground 40 x 40, a 960-triangle sphere, a box, n_hulls
+    64-point convex hulls, n_props seeded blobs of 500-20k triangles, log-uniform), 16
+    materials (14 GGX drawn as the batched scenes', two of them glass; the ground; one spare),
+    n_dynamic instances (spheres, boxes, hulls) at poses of this frame and n_static props.
+    Returns dict(mtris, mesh_first, mats, instances, cam for 1280 x 720)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    meshes = []
+    ground = np.zeros((2, 12), np.float32)
+    q = np.array([-20.0, 0.0, -20.0])
+    u, v = np.array([0.0, 0.0, 40.0]), np.array([40.0, 0.0, 0.0])
+    for k, (a, b, c) in enumerate(((q, q + u, q + v), (q + u + v, q + v, q + u))):
+        ground[k, 0:3], ground[k, 4:7], ground[k, 8:11] = a, b - a, c - a
+        ground[k, 3], ground[k, 7] = np.frombuffer(np.uint32(0xffffffff).tobytes(), np.float32)[0], 0.0
+        ground[k, 11] = np.frombuffer(np.uint32(14 << 26).tobytes(), np.float32)[0]
+    meshes.append(ground)
+    meshes.append(uv_sphere(30, 17))
+    meshes.append(box_mesh())
+    for h in range(n_hulls):
+        meshes.append(hull(64, 100 + h))
+    for p_ in range(n_props):
+        n = int(round(np.exp(rng.uniform(np.log(500), np.log(20000)))))
+        meshes.append(blob(n, 200 + p_))
+    first = np.cumsum([0] + [len(m) for m in meshes]).astype(np.int32)
+    mtris = np.concatenate(meshes)
+    sc = batch_scene(1000, seed)          # its 14 GGX materials, the floor's diffuse, a spare
+    mats = (Material * 16)()
+    ctypes.memmove(mats, sc["mats"], ctypes.sizeof(mats))
+    mats[15] = mats[14]
+    inst = np.zeros(1 + n_dynamic + n_static, INSTANCE_DTYPE)
+    inst[0]["xform"] = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)
+    inst[0]["kind"], inst[0]["material"] = 0, 14
+    prng = np.random.default_rng(seed + 1)
+    for i in range(n_dynamic):
+        kind = (1, 2, 3 + i % n_hulls)[i % 3]
+        s = prng.uniform(0.3, 1.0)
+        x, z = prng.uniform(-19, 19, 2)
+        phase, speed = prng.uniform(0, 2 * np.pi), prng.uniform(0.5, 2.0)
+        y = s + 2.0 * abs(np.sin(phase + speed * 0.1 * frame))       # bouncing with the frame
+        rot = np.random.default_rng([seed, i, frame])
+        inst[1 + i]["xform"] = pose(rot, (s, s, s), (x, y, z)).reshape(12)
+        inst[1 + i]["kind"], inst[1 + i]["material"] = kind, int(prng.integers(0, 14))
+    for i in range(n_static):
+        kind = 3 + n_hulls + int(prng.integers(0, n_props))
+        s = prng.uniform(0.5, 3.0)
+        x, z = prng.uniform(-19, 19, 2)
+        inst[1 + n_dynamic + i]["xform"] = pose(prng, (s, s, s), (x, s * 0.8, z)).reshape(12)
+        inst[1 + n_dynamic + i]["kind"], inst[1 + n_dynamic + i]["material"] = kind, int(prng.integers(0, 14))
+    cam = camera_make((0.0, 14.0, -30.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0, 1280 / 720)
+    return {"mtris": mtris, "mesh_first": first, "mats": mats, "instances": inst, "cam": cam}

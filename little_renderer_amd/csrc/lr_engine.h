@@ -40,7 +40,26 @@ struct JobDev {
     int max_depth, mis;
     uint32_t s0, seed_lo, seed_hi;
     float clamp, eps, margin;
+    // the sky (none: env_w = 0)
+    const float *env_rgb, *env_marg, *env_cond, *env_prob;
+    int env_w, env_h;
+    float env_scale, env_rot, env_rot_c, env_rot_s;
+    // a two-level scene: nodes/aux above are the top level over the instances (aux: instance
+    // ids); the meshes' node records and aux (global triangle ids into tris); per instance 4
+    // float4: world -> object rows, then (kind, material, mesh root, 0) as u32
+    const float4 *mnodes;
+    const uint32_t *maux;
+    const float4 *inst;
+    int two_level;
 };
+
+// instance kinds on the device: a mesh index, or these (trace.wgsl:188-190)
+#define LR_UNIT_SPHERE 0xfffffff0u
+#define LR_UNIT_BOX 0xfffffff1u
+// stack entries of the two-level walk: a top-level node k (< 2^30), INST | instance, BLAS | node
+#define LR_WALK_INST 0x40000000u
+#define LR_WALK_BLAS 0x80000000u
+#define LR_WALK_EXIT 0xffffffffu
 
 enum { K_REGEN = 0, K_EXTEND = 1, K_SHADE = 2, K_SHADOW = 3 };
 
@@ -113,9 +132,22 @@ struct LrhPtScene {
     float4 *prims = nullptr, *tris = nullptr, *mats = nullptr, *boxes = nullptr;
     uint32_t *nodes = nullptr, *aux = nullptr;
     float4 *wnodes = nullptr;                 // the same tree, children's boxes in the parent (walk2)
+    // the sky
+    float *env_rgb = nullptr, *env_marg = nullptr, *env_cond = nullptr, *env_prob = nullptr;
+    int env_w = 0, env_h = 0;
+    float env_scale = 1.0f, env_rot = 0.0f;
+    // a two-level scene (lrh_pt_scene_instanced): meshes, instances, the top level
+    int two_level = 0, nmeshes = 0, ninst = 0, mesh_depth = 0, tlas_depth = 0;
+    float4 *mtris = nullptr, *mboxes = nullptr, *mwnodes = nullptr, *inst = nullptr, *ifwd = nullptr;
+    uint32_t *mnodes = nullptr, *maux = nullptr, *mroot = nullptr;
+    float4 *tboxes = nullptr;
+    uint32_t *tnodes = nullptr, *taux = nullptr;
     float *tab = nullptr;
     uint16_t *mask = nullptr;
     int nprims = 0, ntris = 0, nmats = 0, light = -1, depth = 1;
     float eps = 1e-4f, margin = 1e-3f;
     size_t bytes = 0;                       // device memory it holds
 };
+
+// a job descriptor's scene part (the shapes, trees, materials, sky) from a resident scene
+void lr_job_scene(JobDev &jd, const LrhPtScene *s);

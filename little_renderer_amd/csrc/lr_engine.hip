@@ -377,28 +377,19 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
     const LrhPtScene *s = j->scene;
     JobDev jd;
     memset(&jd, 0, sizeof(jd));
-    jd.nodes = s->wnodes;
-    jd.aux = s->aux;
-    jd.tris = s->tris;
-    jd.prims = s->prims;
-    jd.mats = s->mats;
+    lr_job_scene(jd, s);
     jd.accum = (long long *)(e->arena + off);
     jd.total = h.total;
     memcpy(jd.cam, j->cam, sizeof(jd.cam));
-    jd.root = 0;
-    jd.nprims = (uint32_t)s->nprims;
     jd.W = j->w;
     jd.H = j->h;
     jd.npix = (int)npix;
-    jd.light = s->light;
     jd.max_depth = j->max_depth;
     jd.mis = j->mis;
     jd.s0 = j->s0;
     jd.seed_lo = (uint32_t)j->seed;
     jd.seed_hi = (uint32_t)(j->seed >> 32);
     jd.clamp = j->clamp;
-    jd.eps = s->eps;
-    jd.margin = s->margin;
     hipLaunchKernelGGL(k_job_put, dim3(1), dim3(1), 0, e->stream, e->jobs, e->prog->jret, e->prog->jdone, slot, jd);
     const size_t words = npix * 3;
     hipLaunchKernelGGL(k_zero64, dim3((unsigned)((words + SBLOCK - 1) / SBLOCK < 4096 ? (words + SBLOCK - 1) / SBLOCK : 4096)),

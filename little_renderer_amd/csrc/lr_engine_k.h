@@ -88,6 +88,161 @@ __device__ __forceinline__ void retire(const EngArgs &A, bool on, uint32_t jslot
     }
 }
 
+// ---- two-level scenes (trace.wgsl:186-392: instances, to_object, unit sphere and box) ----
+__device__ __forceinline__ void to_object(const float4 *rows, V3 o, V3 d, V3 &oo, V3 &od)
+{
+    const float4 a = rows[0], b = rows[1], c = rows[2];
+    oo = v3(a.x * o.x + a.y * o.y + a.z * o.z + a.w, b.x * o.x + b.y * o.y + b.z * o.z + b.w,
+            c.x * o.x + c.y * o.y + c.z * o.z + c.w);
+    od = v3(a.x * d.x + a.y * d.y + a.z * d.z, b.x * d.x + b.y * d.y + b.z * d.z, c.x * d.x + c.y * d.y + c.z * d.z);
+}
+// an object-space normal in the world: by the inverse transpose, normalised
+__device__ __forceinline__ V3 normal_to_world(const float4 *rows, V3 n)
+{
+    const float4 a = rows[0], b = rows[1], c = rows[2];
+    return vnorm(v3(a.x * n.x + b.x * n.y + c.x * n.z, a.y * n.x + b.y * n.y + c.y * n.z,
+                    a.z * n.x + b.z * n.y + c.z * n.z));
+}
+__device__ __forceinline__ float hit_unit_sphere(V3 o, V3 d, float tmin, float tmax)
+{
+    const float a = vdot(d, d), hh = -vdot(d, o), c = vdot(o, o) - 1.0f;
+    const float disc = hh * hh - a * c;
+    if (disc < 0.0f) return -1.0f;
+    const float sq = sqrtf(disc);
+    float t = (hh - sq) / a;
+    if (t <= tmin || t >= tmax) {
+        t = (hh + sq) / a;
+        if (t <= tmin || t >= tmax) return -1.0f;
+    }
+    return t;
+}
+__device__ __forceinline__ float hit_unit_box(V3 o, V3 d, float tmin, float tmax)
+{
+    const V3 inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    const float t0x = (-1.0f - o.x) * inv.x, t0y = (-1.0f - o.y) * inv.y, t0z = (-1.0f - o.z) * inv.z;
+    const float t1x = (1.0f - o.x) * inv.x, t1y = (1.0f - o.y) * inv.y, t1z = (1.0f - o.z) * inv.z;
+    const float ax = t0x < t1x ? t0x : t1x, ay = t0y < t1y ? t0y : t1y, az = t0z < t1z ? t0z : t1z;
+    const float bx = t0x > t1x ? t0x : t1x, by = t0y > t1y ? t0y : t1y, bz = t0z > t1z ? t0z : t1z;
+    float enter = ax > ay ? ax : ay;
+    enter = enter > az ? enter : az;
+    float leave = bx < by ? bx : by;
+    leave = leave < bz ? leave : bz;
+    if (leave < enter) return -1.0f;
+    if (enter > tmin && enter < tmax) return enter;
+    if (leave > tmin && leave < tmax) return leave;
+    return -1.0f;
+}
+__device__ __forceinline__ V3 unit_box_normal(V3 p)
+{
+    const float ax = fabsf(p.x), ay = fabsf(p.y), az = fabsf(p.z);
+    if (ax >= ay && ax >= az) return v3(p.x < 0.0f ? -1.0f : 1.0f, 0.0f, 0.0f);
+    if (ay >= az) return v3(0.0f, p.y < 0.0f ? -1.0f : 1.0f, 0.0f);
+    return v3(0.0f, 0.0f, p.z < 0.0f ? -1.0f : 1.0f);
+}
+
+// The closest (ANY: the first) hit in a two-level scene: the top level's records (children's
+// boxes in the parent, as walk2), whose leaves hold instances; a mesh instance is entered (the
+// ray taken to its space, EXIT pushed under its mesh's root), a unit sphere or box is tested on
+// the spot. The hit's b1 carries the instance (as u32).
+template <bool ANY, bool COUNT, int STK>
+__device__ WalkHit walk_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax, const WalkStack &S, uint32_t *overflow,
+                           uint32_t &cn, uint32_t &ct)
+{
+    V3 ro = o, rd = d;
+    const V3 inv_w = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    V3 inv = inv_w;
+    uint32_t cur = LR_NONE;
+    WalkHit best{tmax, LR_NONE, __uint_as_float(LR_NONE), 0.0f};
+    int sp = 0;
+    auto push = [&](uint32_t x) -> bool {
+        if (sp + 1 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
+            atomicOr(overflow, 1u);
+            return false;
+        }
+        if (sp < STK) S.stk[sp * S.stride] = x;
+        else S.spill[(size_t)(sp - STK) * S.sstride] = x;
+        sp++;
+        return true;
+    };
+    push(0u);
+    while (sp > 0) {
+        sp--;
+        const uint32_t e = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+        if (e == LR_WALK_EXIT) {
+            cur = LR_NONE;
+            ro = o;
+            rd = d;
+            inv = inv_w;
+            continue;
+        }
+        if ((e & 0xc0000000u) == LR_WALK_INST) {
+            const uint32_t i = e & 0x3fffffffu;
+            to_object(J.inst + 4 * (size_t)i, o, d, ro, rd);
+            inv = v3(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
+            cur = i;
+            if (!push(LR_WALK_EXIT) || !push(LR_WALK_BLAS | __float_as_uint(J.inst[4 * (size_t)i + 3].z))) break;
+            continue;
+        }
+        const bool blas = (e & LR_WALK_BLAS) != 0;
+        const uint32_t k = e & 0x7fffffffu;
+        const float4 *rec = (blas ? J.mnodes : J.nodes) + 4 * (size_t)k;
+        const float4 l0 = rec[0], l1 = rec[1], r0 = rec[2], r1 = rec[3];
+        if (COUNT) cn++;
+        const float tl = box_t(l0, l1, ro, inv, tmin, best.t);
+        const float tr = box_t(r0, r1, ro, inv, tmin, best.t);
+        const uint32_t ch[2] = {__float_as_uint(l0.w), __float_as_uint(r0.w)};
+        const uint32_t cb[2] = {__float_as_uint(l1.w), __float_as_uint(r1.w)};
+        const float ct2[2] = {tl, tr};
+        const int first = tl <= tr ? 0 : 1;
+        bool inner[2] = {false, false};
+        for (int q = 0; q < 2; q++) {
+            const int c = q == 0 ? first : 1 - first;
+            if (!(ct2[c] >= 0.0f)) continue;
+            if (!(cb[c] & LR_LEAF)) {
+                inner[c] = true;
+                continue;
+            }
+            const uint32_t cnt = cb[c] & 0x7fffffffu;
+            for (uint32_t j = 0; j < cnt; j++) {
+                if (COUNT) ct++;
+                if (blas) {
+                    const uint32_t sid = J.maux[ch[c] + j];
+                    const float4 ta = J.tris[3 * (size_t)sid], tb = J.tris[3 * (size_t)sid + 1], tc = J.tris[3 * (size_t)sid + 2];
+                    float t, u, v;
+                    if (hit_tri(ta, tb, tc, ro, rd, tmin, best.t, t, u, v)) {
+                        best = WalkHit{t, sid, __uint_as_float(cur), 0.0f};
+                        if (ANY) return best;
+                    }
+                    continue;
+                }
+                const uint32_t i = J.aux[ch[c] + j];
+                const uint32_t kind = __float_as_uint(J.inst[4 * (size_t)i + 3].x);
+                if (kind >= LR_UNIT_SPHERE) {
+                    V3 oo, od;
+                    to_object(J.inst + 4 * (size_t)i, o, d, oo, od);
+                    const float t = kind == LR_UNIT_SPHERE ? hit_unit_sphere(oo, od, tmin, best.t) : hit_unit_box(oo, od, tmin, best.t);
+                    if (t > 0.0f) {
+                        best = WalkHit{t, kind, __uint_as_float(i), 0.0f};
+                        if (ANY) return best;
+                    }
+                    continue;
+                }
+                if (!push(LR_WALK_INST | i)) return best;
+            }
+        }
+        const bool gl = inner[0] && tl <= best.t, gr = inner[1] && tr <= best.t;
+        const uint32_t tag = blas ? LR_WALK_BLAS : 0u;
+        if (gl && gr) {
+            if (!push(tag | ch[1 - first]) || !push(tag | ch[first])) break;
+        } else if (gl) {
+            if (!push(tag | ch[0])) break;
+        } else if (gr) {
+            if (!push(tag | ch[1])) break;
+        }
+    }
+    return best;
+}
+
 // regen (wf_gen, trace_wave.wgsl:51-70, with trace_image's seeds, tracer.c:335-346): the free
 // slots [head, head + claimed) start paths next .. next + claimed - 1, each a camera ray
 template <int BLK>
@@ -180,8 +335,9 @@ __global__ __launch_bounds__(BLK) void k_extend(EngArgs A)
         const uint32_t id = A.qid[cur][i];
         const JobDev &J = A.jobs[id >> LR_SLOT_BITS];
         const float4 r0 = A.qray[cur][2 * (size_t)i], r1 = A.qray[cur][2 * (size_t)i + 1];
-        const WalkHit h = walk2<false, COUNT, STK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w,
-                                                   r1.w, S, A.overflow, cn, ct);
+        const WalkHit h = J.two_level ? walk_tl<false, COUNT, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct)
+                                      : walk2<false, COUNT, STK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
+                                                                 v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
         A.hits[i] = make_float4(h.t, __uint_as_float(h.id), h.b1, h.b2);
     });
     if (COUNT) {
@@ -226,13 +382,40 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
             const uint64_t seed = ((uint64_t)J.seed_hi << 32) | J.seed_lo;
             const Smp sm{(int)(pixel % (uint32_t)J.W), (int)(pixel / (uint32_t)J.W), k, (uint32_t)seed * 0x9e3779b9u,
                          A.mask, &st, inc};
-            if (hid != LR_NONE) {
+            const EnvRef env{J.env_rgb, J.env_marg, J.env_cond, J.env_prob, J.env_w, J.env_h, J.env_scale, J.env_rot,
+                             J.env_rot_c, J.env_rot_s};
+            if (hid == LR_NONE) {
+                // escaped: the sky (tracer.c:415-421), black without one
+                if (J.env_w > 0 && emit != EMIT_NONE) {
+                    V3 le = env_eval(env, d);
+                    if (emit == EMIT_MIS) le = vscale(le, power_heuristic(bounce_pdf, env_pdf(env, d)));
+                    rad = vadd(rad, vmul(thr, le));
+                }
+            } else {
                 // the hit (set_hit, intersect.c:4-10): point, the normal facing the ray
                 const float t = hv.x;
                 const V3 hp = vadd(o, vscale(d, t));
                 V3 outward;
                 int mi;
-                if (hid < J.nprims) {
+                if (J.two_level) {
+                    // an instance's hit: the normal by the inverse transpose, its material or the triangle's
+                    const uint32_t ii = __float_as_uint(hv.z);
+                    const float4 *rows = J.inst + 4 * (size_t)ii;
+                    const float4 info = rows[3];
+                    const uint32_t kind = __float_as_uint(info.x), imat = __float_as_uint(info.y);
+                    V3 oo, od, no;
+                    to_object(rows, o, d, oo, od);
+                    int tmat = 0;
+                    if (kind == LR_UNIT_SPHERE) no = vadd(oo, vscale(od, t));
+                    else if (kind == LR_UNIT_BOX) no = unit_box_normal(vadd(oo, vscale(od, t)));
+                    else {
+                        const float4 tu = J.tris[3 * (size_t)hid + 1], tv = J.tris[3 * (size_t)hid + 2];
+                        no = vcross(v3of(tu), v3of(tv));
+                        tmat = (int)(__float_as_uint(tv.w) >> 26);
+                    }
+                    outward = normal_to_world(rows, no);
+                    mi = imat != LR_NONE ? (int)imat : tmat;
+                } else if (hid < J.nprims) {
                     const Prim pr = load_prim(J.prims, hid);
                     outward = pr.q.w < 0.5f ? vscale(vsub(hp, v3of(pr.c)), 1.0f / pr.c.w) : v3of(pr.n);
                     mi = (int)pr.u.w;
@@ -264,6 +447,27 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
                     // 1. next event estimation to the lamp (sample_direct_light, tracer.c:56-81),
                     //    its visibility left to the shadow kernel
                     const bool rough_part = !b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f;
+                    if (rough_part && J.light < 0 && J.env_w > 0) {
+                        // towards the sky (sample_direct_env, tracer.c:87-101; env_numbers :232-235)
+                        float ua, ub;
+                        s2(sm, 1 + 3 * depth, ua, ub);
+                        V3 wi;
+                        float pdf;
+                        const V3 le = env_sample(env, ua, ub, wi, pdf);
+                        if (pdf > 0.0f && vmax3(le) > 0.0f) {
+                            float bpdf;
+                            const V3 f = bsdf_eval(b, A.tab, nrm, v, wi, bpdf);
+                            if (vmax3(f) > 0.0f) {
+                                float g = fabsf(vdot(nrm, wi)) / pdf;
+                                if (J.mis) g *= power_heuristic(pdf, bpdf);
+                                const V3 c = vmul(thr, vscale(vmul(f, le), g));
+                                shadow = true;
+                                sr0 = make_float4(hp.x, hp.y, hp.z, J.eps);
+                                sr1 = make_float4(wi.x, wi.y, wi.z, 1e30f);
+                                scon = make_float4(c.x, c.y, c.z, 0.0f);
+                            }
+                        }
+                    }
                     if (rough_part && J.light >= 0) {
                         float la, lb;
                         s2(sm, 1 + 3 * depth, la, lb);
@@ -373,8 +577,9 @@ __global__ __launch_bounds__(BLK) void k_shadow(EngArgs A)
             const JobDev &J = A.jobs[jslot];
             const float4 r0 = A.sray[2 * (size_t)i], r1 = A.sray[2 * (size_t)i + 1];
             uint32_t cn, ct;
-            const WalkHit h = walk2<true, false, STK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w,
-                                                      r1.w, S, A.overflow, cn, ct);
+            const WalkHit h = J.two_level ? walk_tl<true, false, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct)
+                                          : walk2<true, false, STK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
+                                                                    v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
             const float4 c = A.scon[i];
             ends = c.w != 0.0f;
             const bool lit = h.id == LR_NONE;

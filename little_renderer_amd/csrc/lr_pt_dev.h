@@ -418,3 +418,100 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter)
 }
 
 }  // namespace
+
+// ---- the sky (engine/scene/env.c:36-203; trace.wgsl:700-783) ----
+namespace {
+struct EnvRef {
+    const float *rgb, *marg, *cond, *prob;   // w*h*3; h+1; h*(w+1); w*h (each pixel's probability)
+    int w, h;
+    float scale, rot, rot_c, rot_s;
+};
+constexpr float SKY_PI = 3.14159265358979323846f;
+
+// env_uv + map_uv (env.c:36-57): map coordinates of direction d
+__device__ __forceinline__ void env_uv(const EnvRef &e, V3 d, float &u, float &v)
+{
+    if (e.rot != 0.0f) d = v3(d.x * e.rot_c - d.z * e.rot_s, d.y, d.x * e.rot_s + d.z * e.rot_c);
+    const float y = d.y < -1.0f ? -1.0f : d.y > 1.0f ? 1.0f : d.y;
+    const float uu = 0.5f + atan2f(d.x, d.z) * (0.5f / SKY_PI);
+    u = uu - floorf(uu);
+    v = acosf(y) * (1.0f / SKY_PI);
+}
+// env_dir + map_dir (env.c:48-68)
+__device__ __forceinline__ V3 env_dir(const EnvRef &e, float u, float v)
+{
+    const float ph = (u - 0.5f) * (2.0f * SKY_PI), th = v * SKY_PI;
+    const float st = sinf(th);
+    const V3 d = v3(st * sinf(ph), cosf(th), st * cosf(ph));
+    return e.rot != 0.0f ? v3(d.x * e.rot_c + d.z * e.rot_s, d.y, -d.x * e.rot_s + d.z * e.rot_c) : d;
+}
+// bilerp (env.c:70-88): across wraps around, down stops at the poles
+__device__ __forceinline__ V3 env_bilerp(const EnvRef &e, float u, float v)
+{
+    const int w = e.w, h = e.h;
+    const float x = u * w - 0.5f, y = v * h - 0.5f;
+    const int x0 = (int)floorf(x);
+    int y0 = (int)floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    int y1 = y0 + 1;
+    if (y0 < 0) y0 = 0;
+    if (y1 > h - 1) y1 = h - 1;
+    if (y0 > h - 1) y0 = h - 1;
+    int xa = x0 < 0 ? x0 + w : (x0 >= w ? x0 - w : x0);
+    if (xa < 0 || xa >= w) xa = ((xa % w) + w) % w;
+    const int xb = xa + 1 == w ? 0 : xa + 1;
+    const float *a = e.rgb + 3 * ((size_t)y0 * w + xa), *b = e.rgb + 3 * ((size_t)y0 * w + xb);
+    const float *c = e.rgb + 3 * ((size_t)y1 * w + xa), *dd = e.rgb + 3 * ((size_t)y1 * w + xb);
+    return v3((1 - fy) * ((1 - fx) * a[0] + fx * b[0]) + fy * ((1 - fx) * c[0] + fx * dd[0]),
+              (1 - fy) * ((1 - fx) * a[1] + fx * b[1]) + fy * ((1 - fx) * c[1] + fx * dd[1]),
+              (1 - fy) * ((1 - fx) * a[2] + fx * b[2]) + fy * ((1 - fx) * c[2] + fx * dd[2]));
+}
+__device__ __forceinline__ V3 env_eval(const EnvRef &e, V3 d)
+{
+    float u, v;
+    env_uv(e, d, u, v);
+    return vscale(env_bilerp(e, u, v), e.scale);
+}
+// the last i in 0..n-1 with c[i] <= x (env.c:105-112)
+__device__ __forceinline__ int env_upper_bound(const float *c, int n, float x)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (c[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+// env_sample (env.c:171-189): a direction by the map's luminance x sin(theta), its density
+__device__ __forceinline__ V3 env_sample(const EnvRef &e, float u1, float u2, V3 &d, float &pdf)
+{
+    pdf = 0.0f;
+    const int y = env_upper_bound(e.marg, e.h, u2);
+    const float m0 = e.marg[y], m1 = e.marg[y + 1];
+    float dv = m1 > m0 ? (u2 - m0) / (m1 - m0) : 0.5f;
+    const float *c = e.cond + (size_t)y * (e.w + 1);
+    const int x = env_upper_bound(c, e.w, u1);
+    const float c0 = c[x], c1 = c[x + 1];
+    float du = c1 > c0 ? (u1 - c0) / (c1 - c0) : 0.5f;
+    if (dv > 0.9999f) dv = 0.9999f;
+    if (du > 0.9999f) du = 0.9999f;
+    const float u = (x + du) / e.w, v = (y + dv) / e.h;
+    const float st = sinf(v * SKY_PI);
+    if (st < 1e-6f) return v3(0.0f, 0.0f, 0.0f);
+    d = env_dir(e, u, v);
+    pdf = e.prob[(size_t)y * e.w + x] * e.w * e.h / ((2.0f * SKY_PI * SKY_PI) * st);
+    return env_eval(e, d);
+}
+__device__ __forceinline__ float env_pdf(const EnvRef &e, V3 d)   // env.c:191-203
+{
+    float u, v;
+    env_uv(e, d, u, v);
+    const float st = sinf(v * SKY_PI);
+    if (st < 1e-6f) return 0.0f;
+    int x = (int)(u * e.w), y = (int)(v * e.h);
+    if (x > e.w - 1) x = e.w - 1;
+    if (y > e.h - 1) y = e.h - 1;
+    return e.prob[(size_t)y * e.w + x] * e.w * e.h / ((2.0f * SKY_PI * SKY_PI) * st);
+}
+}  // namespace

@@ -6,7 +6,7 @@
 #include <vector>
 
 #include "lr_device.h"
-#include "lr_engine.h"
+#include "lr_engine_k.h"
 
 // The engine's node records (walk2, lr_walk.h): for each inner node of the reference layout,
 // both children's boxes and (index | leaf record). A one-shape tree (its root a leaf) gets a
@@ -17,14 +17,7 @@ __global__ __launch_bounds__(256) void k_child_boxes(const float4 *__restrict__ 
     if (i >= nn) return;
     const float4 lo = nodes[2 * (size_t)i], hi = nodes[2 * (size_t)i + 1];
     const uint32_t b = __float_as_uint(hi.w);
-    if (b & LR_LEAF) {
-        if (i != 0) return;
-        w[0] = lo;
-        w[1] = hi;
-        w[2] = lo;
-        w[3] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(LR_LEAF));
-        return;
-    }
+    if (b & LR_LEAF) return;
     const uint32_t c[2] = {__float_as_uint(lo.w), b};
     for (int k = 0; k < 2; k++) {
         float4 clo = nodes[2 * (size_t)c[k]], chi = nodes[2 * (size_t)c[k] + 1];
@@ -37,10 +30,64 @@ __global__ __launch_bounds__(256) void k_child_boxes(const float4 *__restrict__ 
     }
 }
 
+// A tree whose root is a leaf (one shape) gets a record with that leaf on the left and an
+// empty leaf on the right.
+__global__ void k_root_leaf(const float4 *__restrict__ nodes, uint32_t r, float4 *__restrict__ w)
+{
+    const float4 lo = nodes[2 * (size_t)r], hi = nodes[2 * (size_t)r + 1];
+    if (!(__float_as_uint(hi.w) & LR_LEAF)) return;
+    w[4 * (size_t)r] = lo;
+    w[4 * (size_t)r + 1] = hi;
+    w[4 * (size_t)r + 2] = lo;
+    w[4 * (size_t)r + 3] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(LR_LEAF));
+}
+
+// Each instance's world box: the 8 corners of its object box (its mesh's root box, or [-1, 1]^3)
+// moved to the world, padded by a little more than their rounding.
+__global__ __launch_bounds__(256) void k_inst_boxes(const float4 *__restrict__ fwd, const float4 *__restrict__ inst,
+                                                    const uint32_t *__restrict__ mroot, const float4 *__restrict__ mnodes,
+                                                    int n, float4 *__restrict__ boxes)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t kind = __float_as_uint(inst[4 * (size_t)i + 3].x);
+    float lo[3] = {-1.0f, -1.0f, -1.0f}, hi[3] = {1.0f, 1.0f, 1.0f};
+    if (kind < LR_UNIT_SPHERE) {
+        const uint32_t r = mroot[kind];
+        const float4 a = mnodes[2 * (size_t)r], b = mnodes[2 * (size_t)r + 1];
+        lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
+        hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
+    }
+    const float4 rows[3] = {fwd[3 * (size_t)i], fwd[3 * (size_t)i + 1], fwd[3 * (size_t)i + 2]};
+    float wlo[3] = {1e30f, 1e30f, 1e30f}, whi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int c = 0; c < 8; c++) {
+        const float p[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+        for (int k = 0; k < 3; k++) {
+            const float4 r = rows[k];
+            const float v = r.x * p[0] + r.y * p[1] + r.z * p[2] + r.w;
+            wlo[k] = fminf(wlo[k], v);
+            whi[k] = fmaxf(whi[k], v);
+        }
+    }
+    float4 bl, bh;
+    float *ol = &bl.x, *oh = &bh.x;
+    for (int k = 0; k < 3; k++) {
+        const float pad = 1e-5f * fmaxf(whi[k] - wlo[k], fmaxf(fabsf(wlo[k]), fabsf(whi[k]))) + 1e-6f;
+        ol[k] = wlo[k] - pad;
+        oh[k] = whi[k] + pad;
+    }
+    bl.w = 0.0f;
+    bh.w = 0.0f;
+    boxes[2 * (size_t)i] = bl;
+    boxes[2 * (size_t)i + 1] = bh;
+}
+
 static void pt_scene_release(LrhPtScene *s)
 {
     if (!s) return;
-    void *ps[] = {s->prims, s->tris, s->mats, s->boxes, s->nodes, s->aux, s->tab, s->mask, s->wnodes};
+    void *ps[] = {s->prims, s->tris, s->mats, s->boxes, s->nodes, s->aux, s->tab, s->mask, s->wnodes, s->env_rgb,
+                  s->env_marg, s->env_cond, s->env_prob, s->mboxes, s->mwnodes, s->inst, s->ifwd, s->mnodes, s->maux,
+                  s->mroot};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete s;
@@ -153,6 +200,8 @@ extern "C" int lrh_pt_scene(LrhContext *c, const float *prims, int nprims, const
     }
     hipLaunchKernelGGL(k_child_boxes, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream,
                        (const float4 *)s->nodes, (int)nn, s->wnodes);
+    if (n == 1)   // the root is the one shape's leaf
+        hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, c->stream, (const float4 *)s->nodes, 0u, s->wnodes);
     s->bytes = (size_t)nprims * 96 + (size_t)ntris * 48 + hm.size() * 4 + boxes.size() * 4 + (2 * (size_t)n - 1) * 32 +
                (size_t)n * 4 + tab.size() * 4 + mask.size() * 2 + (2 * (size_t)n - 1) * 64;
     // the tree is built on the context's stream; the engines that render the scene run on
@@ -178,5 +227,271 @@ extern "C" int lrh_pt_scene_info(const LrhPtScene *s, int *depth, int *nshapes, 
     if (depth) *depth = s->depth;
     if (nshapes) *nshapes = s->nprims + s->ntris;
     if (bytes) *bytes = s->bytes;
+    return 0;
+}
+
+extern "C" int lrs_sky_tables(const float *rgb, int w, int h, float *marg, float *cond, float *prob, double *sum);
+extern "C" int lrs_affine_inverse(const float fwd[12], float inv[12]);
+
+static bool dev_upload(void **dst, const void *src, size_t bytes)
+{
+    if (hipMalloc(dst, bytes ? bytes : 16) != hipSuccess) return false;
+    if (src && bytes && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return false;
+    return true;
+}
+
+extern "C" int lrh_pt_scene_sky(LrhPtScene *s, const float *rgb, int w, int h, float scale, float rot)
+{
+    if (!s || (rgb && (w < 2 || h < 1 || w != 2 * h)) || !(scale >= 0.0f))
+        return lr_fail(LRH_EINVAL, "lrh_pt_scene_sky: bad arguments (the map is w x w/2)");
+    if (rgb && s->light >= 0)
+        return lr_fail(LRH_EINVAL, "lrh_pt_scene_sky: a scene with a lamp takes no sky here (one light per scene)");
+    (void)hipStreamSynchronize(s->ctx->stream);
+    void *old[] = {s->env_rgb, s->env_marg, s->env_cond, s->env_prob};
+    for (void *p : old)
+        if (p) (void)hipFree(p);
+    s->env_rgb = s->env_marg = s->env_cond = s->env_prob = nullptr;
+    s->env_w = s->env_h = 0;
+    if (!rgb) return 0;
+    std::vector<float> marg(h + 1), cond((size_t)h * (w + 1)), prob((size_t)w * h);
+    double sum = 0;
+    if (lrs_sky_tables(rgb, w, h, marg.data(), cond.data(), prob.data(), &sum) != 0)
+        return lr_fail(LRH_ENOMEM, "lrh_pt_scene_sky: host memory for the tables could not be had");
+    if (!dev_upload((void **)&s->env_rgb, rgb, (size_t)w * h * 12) || !dev_upload((void **)&s->env_marg, marg.data(), marg.size() * 4) ||
+        !dev_upload((void **)&s->env_cond, cond.data(), cond.size() * 4) ||
+        !dev_upload((void **)&s->env_prob, prob.data(), prob.size() * 4)) {
+        (void)hipGetLastError();
+        return lr_fail(LRH_ENOMEM, "lrh_pt_scene_sky: device memory for the sky could not be had");
+    }
+    s->env_w = w;
+    s->env_h = h;
+    s->env_scale = scale;
+    s->env_rot = rot;
+    s->bytes += (size_t)w * h * 12 + marg.size() * 4 + cond.size() * 4 + prob.size() * 4;
+    return 0;
+}
+
+// the instances' device records (world -> object rows, then kind, material, mesh root) and
+// their object -> world rows, from the caller's transforms
+static int pack_instances(const LrhPtScene *s, const LrhInstance *in, int n, const std::vector<uint32_t> &roots,
+                          std::vector<float> &rec, std::vector<float> &fwd)
+{
+    rec.assign((size_t)n * 16, 0.0f);
+    fwd.assign((size_t)n * 12, 0.0f);
+    for (int i = 0; i < n; i++) {
+        const LrhInstance &x = in[i];
+        if (x.kind < LRH_INST_BOX || x.kind >= s->nmeshes) return lr_fail(LRH_EINVAL, "instance %d: kind %d", i, x.kind);
+        if (x.material < -1 || x.material >= s->nmats) return lr_fail(LRH_EINVAL, "instance %d: material %d", i, x.material);
+        float inv[12];
+        if (lrs_affine_inverse(x.xform, inv) != 0) return lr_fail(LRH_EINVAL, "instance %d: its transform is singular", i);
+        float *o = &rec[(size_t)i * 16];
+        memcpy(o, inv, 48);
+        const uint32_t kind = x.kind == LRH_INST_SPHERE ? LR_UNIT_SPHERE : x.kind == LRH_INST_BOX ? LR_UNIT_BOX : (uint32_t)x.kind;
+        const uint32_t mat = x.material < 0 ? LR_NONE : (uint32_t)x.material;
+        const uint32_t root = x.kind >= 0 ? roots[x.kind] : 0u;
+        memcpy(o + 12, &kind, 4);
+        memcpy(o + 13, &mat, 4);
+        memcpy(o + 14, &root, 4);
+        memcpy(&fwd[(size_t)i * 12], x.xform, 48);
+    }
+    return 0;
+}
+
+extern "C" int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int ninst, int builder, LrhBuildStats *stats)
+{
+    if (!s || !s->two_level || !inst || ninst != s->ninst || (builder != LRH_LBVH && builder != LRH_PLOC))
+        return lr_fail(LRH_EINVAL, "lrh_pt_scene_update: bad arguments (an instanced scene, the same instance count)");
+    LrhContext *c = s->ctx;
+    std::vector<uint32_t> roots(s->nmeshes);
+    if (s->nmeshes) LR_HIP(hipMemcpy(roots.data(), s->mroot, (size_t)s->nmeshes * 4, hipMemcpyDeviceToHost));
+    std::vector<float> rec, fwd;
+    int rc = pack_instances(s, inst, ninst, roots, rec, fwd);
+    if (rc) return rc;
+    LR_HIP(hipStreamSynchronize(c->stream));
+    LR_HIP(hipMemcpy(s->inst, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
+    LR_HIP(hipMemcpy(s->ifwd, fwd.data(), fwd.size() * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_inst_boxes, dim3((unsigned)((ninst + 255) / 256)), dim3(256), 0, c->stream, s->ifwd, s->inst,
+                       s->mroot, (const float4 *)s->mnodes, ninst, s->tboxes);
+    LR_HIP(hipGetLastError());
+    LrhBuildStats st;
+    rc = lrh_build(c, (const float *)s->tboxes, ninst, nullptr, builder, 0, 0, 0, s->tnodes, s->taux, nullptr, &st);
+    if (rc) return rc;
+    const size_t nn = 2 * (size_t)ninst - 1;
+    hipLaunchKernelGGL(k_child_boxes, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, (const float4 *)s->tnodes,
+                       (int)nn, s->wnodes);
+    if (ninst == 1) hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, c->stream, (const float4 *)s->tnodes, 0u, s->wnodes);
+    LR_HIP(hipGetLastError());
+    LR_HIP(hipStreamSynchronize(c->stream));
+    s->tlas_depth = st.depth;
+    // the two-level walk's stack: a pending sibling and an instance to enter per top-level
+    // level, EXIT, the mesh's walk
+    s->depth = 2 * s->tlas_depth + s->mesh_depth + 6;
+    if (stats) *stats = st;
+    return 0;
+}
+
+extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const int *mesh_first, int nmeshes,
+                                      const LrhMaterial *mats, int nmats, const LrhInstance *inst, int ninst, float eps,
+                                      int builder, LrhPtScene **out)
+{
+    if (!c || !out || nmeshes < 0 || !mesh_first || ninst <= 0 || !inst || !mats || nmats <= 0 || nmats > 64 ||
+        (builder != LRH_LBVH && builder != LRH_PLOC))
+        return lr_fail(LRH_EINVAL, "lrh_pt_scene_instanced: bad arguments");
+    if (mesh_first[0] != 0) return lr_fail(LRH_EINVAL, "lrh_pt_scene_instanced: mesh_first[0] must be 0");
+    for (int m = 0; m < nmeshes; m++)
+        if (mesh_first[m + 1] <= mesh_first[m]) return lr_fail(LRH_EINVAL, "lrh_pt_scene_instanced: mesh %d is empty", m);
+    const int M = mesh_first[nmeshes];
+    if (M > 0 && !mtris) return lr_fail(LRH_EINVAL, "lrh_pt_scene_instanced: no triangles");
+    for (int i = 0; i < M; i++) {
+        uint32_t w;
+        memcpy(&w, mtris + 12 * (size_t)i + 11, 4);
+        if ((int)(w >> 26) >= nmats) return lr_fail(LRH_EINVAL, "lrh_pt_scene_instanced: triangle %d: material %u", i, w >> 26);
+    }
+    for (int i = 0; i < nmats; i++)
+        if (mats[i].type < 0 || mats[i].type > LRH_MAT_PBR) return lr_fail(LRH_EINVAL, "material %d: type %d", i, mats[i].type);
+    *out = nullptr;
+    LrhPtScene *s = new LrhPtScene();
+    s->ctx = c;
+    s->two_level = 1;
+    s->nmeshes = nmeshes;
+    s->ninst = ninst;
+    s->ntris = M;
+    s->nmats = nmats;
+    s->light = -1;
+    s->eps = eps;
+    s->margin = 0.0f;
+    size_t nnodes = 0;
+    std::vector<uint32_t> roots(nmeshes);
+    for (int m = 0; m < nmeshes; m++) {
+        roots[m] = (uint32_t)nnodes;
+        nnodes += 2 * (size_t)(mesh_first[m + 1] - mesh_first[m]) - 1;
+    }
+    std::vector<float> hm(12 * (size_t)nmats);
+    for (int i = 0; i < nmats; i++) {
+        const LrhMaterial &m = mats[i];
+        float *o = &hm[12 * (size_t)i];
+        o[0] = m.albedo[0]; o[1] = m.albedo[1]; o[2] = m.albedo[2]; o[3] = (float)m.type;
+        o[4] = m.emission[0]; o[5] = m.emission[1]; o[6] = m.emission[2]; o[7] = m.ior;
+        o[8] = m.metallic; o[9] = m.roughness; o[10] = m.transmission; o[11] = 0.0f;
+    }
+    std::vector<float> tab(2112);
+    std::vector<uint16_t> mask(4096);
+    lrs_brdf_tables(tab.data());
+    lrs_blue_noise(mask.data());
+    const size_t tn = 2 * (size_t)ninst - 1;
+    bool ok = dev_upload((void **)&s->tris, mtris, (size_t)M * 48) && dev_upload((void **)&s->mboxes, nullptr, (size_t)M * 32) &&
+              dev_upload((void **)&s->mnodes, nullptr, nnodes * 32) && dev_upload((void **)&s->maux, nullptr, (size_t)M * 4) &&
+              dev_upload((void **)&s->mwnodes, nullptr, nnodes * 64) &&
+              dev_upload((void **)&s->mroot, roots.data(), roots.size() * 4) &&
+              dev_upload((void **)&s->mats, hm.data(), hm.size() * 4) && dev_upload((void **)&s->tab, tab.data(), tab.size() * 4) &&
+              dev_upload((void **)&s->mask, mask.data(), mask.size() * 2) &&
+              dev_upload((void **)&s->inst, nullptr, (size_t)ninst * 64) && dev_upload((void **)&s->ifwd, nullptr, (size_t)ninst * 48) &&
+              dev_upload((void **)&s->tboxes, nullptr, (size_t)ninst * 32) && dev_upload((void **)&s->tnodes, nullptr, tn * 32) &&
+              dev_upload((void **)&s->taux, nullptr, (size_t)ninst * 4) && dev_upload((void **)&s->wnodes, nullptr, tn * 64);
+    if (!ok) {
+        (void)hipGetLastError();
+        pt_scene_release(s);
+        return lr_fail(LRH_ENOMEM, "lrh_pt_scene_instanced: device memory for the scene could not be had");
+    }
+    s->nodes = s->tnodes;
+    s->aux = s->taux;
+    s->boxes = s->tboxes;
+    int rc = lrh_tri_boxes(c, mtris ? (const float *)s->tris : nullptr, M, (float *)s->mboxes);
+    int depth = 0;
+    for (int m = 0; !rc && m < nmeshes; m++) {
+        const int f = mesh_first[m], n = mesh_first[m + 1] - f;
+        LrhBuildStats st;
+        rc = lrh_build(c, (const float *)(s->mboxes + 2 * (size_t)f), n, nullptr, builder, roots[m], (uint32_t)f, (uint32_t)f,
+                       s->mnodes, s->maux, nullptr, &st);
+        if (!rc && st.depth > depth) depth = st.depth;
+        if (!rc && n == 1)
+            hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, c->stream, (const float4 *)s->mnodes, roots[m], s->mwnodes);
+    }
+    if (!rc && nnodes)
+        hipLaunchKernelGGL(k_child_boxes, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, c->stream,
+                           (const float4 *)s->mnodes, (int)nnodes, s->mwnodes);
+    if (rc) {
+        pt_scene_release(s);
+        return rc;
+    }
+    // the one-shape meshes' root records were written before the pass over all inner nodes,
+    // which skips leaves: they stay
+    s->mesh_depth = depth;
+    s->bytes = (size_t)M * (48 + 32 + 4) + nnodes * (32 + 64) + hm.size() * 4 + tab.size() * 4 + mask.size() * 2 +
+               (size_t)ninst * (64 + 48 + 32 + 4) + tn * (32 + 64);
+    rc = lrh_pt_scene_update(s, inst, ninst, LRH_LBVH, nullptr);
+    if (rc) {
+        pt_scene_release(s);
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+void lr_job_scene(JobDev &jd, const LrhPtScene *s)
+{
+    jd.nodes = s->wnodes;
+    jd.aux = s->aux;
+    jd.tris = s->tris;
+    jd.prims = s->prims;
+    jd.mats = s->mats;
+    jd.root = 0;
+    jd.nprims = (uint32_t)s->nprims;
+    jd.light = s->light;
+    jd.eps = s->eps;
+    jd.margin = s->margin;
+    jd.env_rgb = s->env_rgb;
+    jd.env_marg = s->env_marg;
+    jd.env_cond = s->env_cond;
+    jd.env_prob = s->env_prob;
+    jd.env_w = s->env_w;
+    jd.env_h = s->env_h;
+    jd.env_scale = s->env_scale;
+    jd.env_rot = s->env_rot;
+    jd.env_rot_c = cosf(s->env_rot);
+    jd.env_rot_s = sinf(s->env_rot);
+    jd.two_level = s->two_level;
+    jd.mnodes = s->mwnodes;
+    jd.maux = s->maux;
+    jd.inst = s->inst;
+}
+
+// a batch of rays through a resident scene's own walk (the engine's: walk2, or the two-level
+// walk of an instanced scene); hits t, id, b1 (an instanced scene: the instance, as u32), b2
+template <bool ANY>
+__global__ __launch_bounds__(128) void k_scene_trace(JobDev J, const float4 *__restrict__ rays, int n, float4 *__restrict__ hits,
+                                                     uint32_t *spill, int spill_per, uint32_t *overflow)
+{
+    __shared__ uint32_t stk[LR_LDS_STACK][128];
+    const int i = blockIdx.x * 128 + threadIdx.x;
+    if (i >= n) return;
+    const float4 r0 = rays[2 * (size_t)i], r1 = rays[2 * (size_t)i + 1];
+    const WalkStack S{&stk[0][threadIdx.x], 128, spill + i, (size_t)n, LR_LDS_STACK + spill_per};
+    uint32_t cn = 0, ct = 0;
+    const WalkHit h = J.two_level ? walk_tl<ANY, false, LR_LDS_STACK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, overflow, cn, ct)
+                                  : walk2<ANY, false, LR_LDS_STACK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
+                                                                    v3of(r1), r0.w, r1.w, S, overflow, cn, ct);
+    hits[i] = make_float4(h.t, __uint_as_float(h.id), h.b1, h.b2);
+}
+
+extern "C" int lrh_pt_scene_trace(LrhContext *c, const LrhPtScene *s, const float *d_rays, int n, int any, float *d_hits)
+{
+    if (!c || !s || n < 0 || (n > 0 && (!d_rays || !d_hits))) return lr_fail(LRH_EINVAL, "lrh_pt_scene_trace: bad arguments");
+    if (n == 0) return 0;
+    const int spill_per = s->depth + 2 > LR_LDS_STACK ? s->depth + 2 - LR_LDS_STACK : 0;
+    int rc = lr_scratch_reserve(c, LR_SCRATCH_TRACE, lr_align((size_t)spill_per * n * 4 + 4));
+    if (rc) return rc;
+    JobDev jd;
+    memset(&jd, 0, sizeof(jd));
+    lr_job_scene(jd, s);
+    const dim3 grid((n + 127) / 128);
+    uint32_t *spill = (uint32_t *)c->scratch[LR_SCRATCH_TRACE].base;
+    if (any)
+        hipLaunchKernelGGL(k_scene_trace<true>, grid, dim3(128), 0, c->stream, jd, (const float4 *)d_rays, n, (float4 *)d_hits,
+                           spill, spill_per, c->flags);
+    else
+        hipLaunchKernelGGL(k_scene_trace<false>, grid, dim3(128), 0, c->stream, jd, (const float4 *)d_rays, n, (float4 *)d_hits,
+                           spill, spill_per, c->flags);
+    LR_HIP(hipGetLastError());
     return 0;
 }

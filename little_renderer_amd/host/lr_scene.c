@@ -573,3 +573,112 @@ int lrs_affine_inverse(const float fwd[12], float inv[12])
     }
     return 0;
 }
+
+/* ---- meshes of the instanced workload (object space, material in the top bits) ---- */
+static void put_tri_f(float *o, const double a[3], const double b[3], const double c[3], int mat) { put_tri(o, a, b, c, mat); }
+
+int lrs_uv_sphere(int slices, int stacks, int mat, float *tris)
+{
+    int at = 0;
+    for (int j = 0; j < stacks; j++) {
+        const double t0 = M_PI * j / stacks, t1 = M_PI * (j + 1) / stacks;
+        for (int i = 0; i < slices; i++) {
+            const double p0 = 2 * M_PI * i / slices, p1 = 2 * M_PI * (i + 1) / slices;
+            const double a[3] = {sin(t0) * cos(p0), cos(t0), sin(t0) * sin(p0)}, b[3] = {sin(t0) * cos(p1), cos(t0), sin(t0) * sin(p1)};
+            const double c[3] = {sin(t1) * cos(p0), cos(t1), sin(t1) * sin(p0)}, d[3] = {sin(t1) * cos(p1), cos(t1), sin(t1) * sin(p1)};
+            /* outward (u x v away from the centre) */
+            if (j > 0) put_tri_f(tris + 12 * (size_t)at++, a, b, c, mat);
+            if (j < stacks - 1) put_tri_f(tris + 12 * (size_t)at++, b, d, c, mat);
+        }
+    }
+    return at;
+}
+
+int lrs_box_mesh(int mat, float *tris)
+{
+    int at = 0;
+    for (int ax = 0; ax < 3; ax++)
+        for (int s = -1; s <= 1; s += 2) {
+            const int u = (ax + 1) % 3, v = (ax + 2) % 3;
+            double q[3] = {0, 0, 0}, du[3] = {0, 0, 0}, dv[3] = {0, 0, 0};
+            q[ax] = s;
+            q[u] = -1;
+            q[v] = -1;
+            du[u] = 2;
+            dv[v] = 2;
+            if (s < 0) {   /* keep u x v pointing out */
+                double t[3];
+                memcpy(t, du, sizeof(t));
+                memcpy(du, dv, sizeof(t));
+                memcpy(dv, t, sizeof(t));
+            }
+            at = put_quad(tris, at, q, du, dv, mat);
+        }
+    return at;
+}
+
+int lrs_hull(int npts, uint64_t seed, int mat, float *tris)
+{
+    /* the convex hull of npts points drawn uniformly on the unit sphere: with every point on
+       the hull, a face is a triple with all the other points on one side (O(n^4), n small) */
+    Pcg r;
+    pcg_seed(&r, seed, 6);
+    double (*p)[3] = malloc(sizeof(double) * 3 * (size_t)npts);
+    if (!p) return -1;
+    for (int i = 0; i < npts; i++) random_dir(&r, p[i]);
+    int at = 0;
+    for (int i = 0; i < npts; i++)
+        for (int j = i + 1; j < npts; j++)
+            for (int k = j + 1; k < npts; k++) {
+                double e1[3], e2[3], n[3];
+                for (int q = 0; q < 3; q++) { e1[q] = p[j][q] - p[i][q]; e2[q] = p[k][q] - p[i][q]; }
+                cross3(e1, e2, n);
+                const double d0 = n[0] * p[i][0] + n[1] * p[i][1] + n[2] * p[i][2];
+                int pos = 0, neg = 0;
+                for (int m = 0; m < npts && !(pos && neg); m++) {
+                    if (m == i || m == j || m == k) continue;
+                    const double s = n[0] * p[m][0] + n[1] * p[m][1] + n[2] * p[m][2] - d0;
+                    if (s > 0) pos = 1;
+                    else neg = 1;
+                }
+                if (pos && neg) continue;
+                if (pos) put_tri_f(tris + 12 * (size_t)at++, p[i], p[k], p[j], mat);   /* the others above: flip */
+                else put_tri_f(tris + 12 * (size_t)at++, p[i], p[j], p[k], mat);
+            }
+    free(p);
+    return at;
+}
+
+int lrs_blob(int ntris, uint64_t seed, int mat, float *tris)
+{
+    /* a UV sphere of about ntris triangles, its radius bumped by a few seeded waves */
+    int stacks = (int)floor(sqrt(ntris / 4.0) + 0.5) + 1;
+    if (stacks < 3) stacks = 3;
+    int slices = (int)floor(ntris / (2.0 * (stacks - 1)) + 0.5);
+    if (slices < 3) slices = 3;
+    const int n = lrs_uv_sphere(slices, stacks, mat, tris);
+    Pcg r;
+    pcg_seed(&r, seed, 7);
+    double k[4][3], ph[4], amp[4];
+    for (int w = 0; w < 4; w++) {
+        random_dir(&r, k[w]);
+        const double f = 1.0 + 3.0 * pcg_unit(&r);
+        for (int q = 0; q < 3; q++) k[w][q] *= f;
+        ph[w] = 2 * M_PI * pcg_unit(&r);
+        amp[w] = 0.05 + 0.1 * pcg_unit(&r);
+    }
+    for (int t = 0; t < n; t++) {
+        float *o = tris + 12 * (size_t)t;
+        double v[3][3];
+        for (int q = 0; q < 3; q++) { v[0][q] = o[q]; v[1][q] = o[q] + o[4 + q]; v[2][q] = o[q] + o[8 + q]; }
+        for (int c = 0; c < 3; c++) {
+            double s = 1.0;
+            for (int w = 0; w < 4; w++) s += amp[w] * sin(k[w][0] * v[c][0] + k[w][1] * v[c][1] + k[w][2] * v[c][2] + ph[w]);
+            for (int q = 0; q < 3; q++) v[c][q] *= s;
+        }
+        uint32_t wd;
+        memcpy(&wd, &o[11], 4);
+        put_tri_f(o, v[0], v[1], v[2], (int)(wd >> 26));
+    }
+    return n;
+}
