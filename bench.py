@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--tune", default=None, metavar="BT,STK,BS,BPC,TILE,IPG",
                     help="batched: this launch shape instead of the autotuner's")
     ap.add_argument("--skip-plain", action="store_true", help="batched: leave out the plain per-job comparison")
+    ap.add_argument("--tlas", choices=["lbvh", "ploc"], default="ploc", help="instanced: the per-frame top-level builder")
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
@@ -483,7 +484,8 @@ def bench_instanced(a, torch, lr):
     ps = lr.PtScene.instanced(ctx, w0["mtris"], w0["mesh_first"], w0["mats"], w0["instances"])
     ps.set_sky(lr.sky(2048, 35.0, 60.0, sun_le=(5e4, 4.6e4, 4e4)))
     depth, nshapes, nbytes = ps.info()
-    eng = lr.Engine(ctx, slots_log2=22, max_tree_depth=depth, image_bytes=_align(W * H * 24) + (16 << 20))
+    # the top level is rebuilt every frame: its depth moves a little, so the walks get room
+    eng = lr.Engine(ctx, slots_log2=22, max_tree_depth=depth + 32, image_bytes=_align(W * H * 24) + (16 << 20))
     if a.tune:
         f = [int(x) for x in a.tune.split(",")]
         eng.tune = dict(zip(["block_trace", "stack", "block_shade", "blocks_per_cu", "tile", "iters_per_graph"], f))
@@ -492,7 +494,7 @@ def bench_instanced(a, torch, lr):
     tune = eng.tune
 
     def frame(k, keep=False):
-        st = ps.update(frames[k]["instances"])
+        st = ps.update(frames[k]["instances"], builder=lr.PLOC if a.tlas == "ploc" else lr.LBVH)
         i = eng.submit(lr.make_job(ps, frames[k]["cam"], W, H, 1, seed=k, max_depth=8))
         eng.drain()
         if not keep:
@@ -545,6 +547,7 @@ def bench_instanced(a, torch, lr):
                                "14 GGX materials, procedural 2048x1024 sky (5e4 sun), TLAS rebuilt per frame, "
                                "1280x720, 1 spp, 8 bounces", "instances": ninst, "mesh_triangles": int(nshapes),
                    "scene_bytes": int(nbytes), "walk_depth": depth, "tune": tune.as_dict()},
+        "tlas_builder": a.tlas,
         "tlas_build_ms": round(float(np.mean(builds)), 4),
         "tlas_build_mprims_s": round(ninst / (float(np.mean(builds)) * 1e3), 2),
         "kernel_ms_per_launch": {n: round(kms[n] / max(kl[n], 1), 4) for n in names},

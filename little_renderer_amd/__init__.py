@@ -637,7 +637,8 @@ def instanced_workload(seed=0, frame=0, n_dynamic=2000, n_static=3000, n_props=2
     """The instanced workload: meshes (ground 40 x 40, a 960-triangle sphere, a box, n_hulls
     64-point convex hulls, n_props seeded blobs of 500-20k triangles, log-uniform), 16
     materials (14 GGX drawn as the batched scenes', two of them glass; the ground; one spare),
-    n_dynamic instances (spheres, boxes, hulls) at poses of this frame and n_static props.
+    n_dynamic instances (spheres, boxes, hulls; scale 0.2-0.6) bouncing with the frame, and n_static
+    props (scale 0.15-0.6), all at uniformly random rotations over the ground.
     Returns dict(mtris, mesh_first, mats, instances, cam for 1280 x 720)."""
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -669,7 +670,7 @@ def instanced_workload(seed=0, frame=0, n_dynamic=2000, n_static=3000, n_props=2
     prng = np.random.default_rng(seed + 1)
     for i in range(n_dynamic):
         kind = (1, 2, 3 + i % n_hulls)[i % 3]
-        s = prng.uniform(0.3, 1.0)
+        s = prng.uniform(0.2, 0.6)
         x, z = prng.uniform(-19, 19, 2)
         phase, speed = prng.uniform(0, 2 * np.pi), prng.uniform(0.5, 2.0)
         y = s + 2.0 * abs(np.sin(phase + speed * 0.1 * frame))       # bouncing with the frame
@@ -678,7 +679,7 @@ def instanced_workload(seed=0, frame=0, n_dynamic=2000, n_static=3000, n_props=2
         inst[1 + i]["kind"], inst[1 + i]["material"] = kind, int(prng.integers(0, 14))
     for i in range(n_static):
         kind = 3 + n_hulls + int(prng.integers(0, n_props))
-        s = prng.uniform(0.5, 3.0)
+        s = prng.uniform(0.15, 0.6)
         x, z = prng.uniform(-19, 19, 2)
         inst[1 + n_dynamic + i]["xform"] = pose(prng, (s, s, s), (x, s * 0.8, z)).reshape(12)
         inst[1 + n_dynamic + i]["kind"], inst[1 + n_dynamic + i]["material"] = kind, int(prng.integers(0, 14))

@@ -73,7 +73,9 @@ struct EngCtl {
     unsigned long long claimed;             // paths the running iteration's regen started
     unsigned int count[4];                  // ray queues [0], [1]; shadow queue [2]
     unsigned int tiles[4];                  // each kernel's tile counter (work-groups claim tiles)
-    unsigned long long pad[4];
+    unsigned int regen_done;                // regen's work-groups that have finished
+    unsigned int pad0;
+    unsigned long long pad[3];
 };
 
 struct EngProgress {
@@ -108,6 +110,7 @@ struct EngArgs {
     const float *tab;                       // BSDF tables
     const uint16_t *mask;                   // blue-noise ranks
     unsigned long long *counts;             // node records and shapes walked (counting extend only)
+    unsigned long long *ostat;              // shade's tiles: look-back status of its ordered appends
     uint32_t cmask;                         // slots - 1
     int mult;                               // a tile is mult work-groups' widths of entries
 };

@@ -496,7 +496,7 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
         C * 32, C * 4, C * 16,                                                       // 14..16 sray sid scon
         e->threads_cap * (size_t)(e->spill_cap > 0 ? e->spill_cap : 1) * 4,          // 17 spill
         2112 * 4, 4096 * 2, 256, 256, 256,                                           // 18..22 tab mask counts clk overflow
-        LR_ISSUE * 4};                                                               // 23 iq_q0
+        LR_ISSUE * 4, (C / 64 + 4) * 8};                                             // 23 iq_q0, 24 ostat
     constexpr int NS = sizeof(sizes) / sizeof(sizes[0]);
     size_t total = 0;
     for (size_t z : sizes) total += lr_align(z);
@@ -547,6 +547,7 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
     A.tab = (const float *)p[18];
     A.mask = (const uint16_t *)p[19];
     A.counts = e->counts;
+    A.ostat = (unsigned long long *)p[24];
     A.overflow = e->overflow;
     A.cmask = e->C - 1;
     A.mult = 1;

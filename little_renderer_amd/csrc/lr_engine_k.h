@@ -48,6 +48,61 @@ __device__ __forceinline__ long long to_fixed(float x)
     return __float2ll_rn(x * (float)(1u << LR_FIX_SHIFT));
 }
 
+// Positions, in entry order, of a tile's outputs to two queues: a scan across the work-group
+// and a decoupled look-back over the tiles before it. A tile's status word: 2-bit flag (1: its
+// own counts, 2: the counts of all tiles up to it), then two 31-bit counts. Tiles are claimed
+// in order by running work-groups, so every tile waited on is being worked on. The last tile
+// writes the queues' lengths.
+template <int BLK>
+__device__ __forceinline__ void ordered_append2(unsigned long long *stat, uint32_t t, uint32_t ntiles, bool fa, bool fb,
+                                                uint32_t &pa, uint32_t &pb, unsigned int *len_a, unsigned int *len_b)
+{
+    constexpr int NW = BLK / 64;
+    __shared__ uint32_t s_w[2][NW];
+    __shared__ uint32_t s_base[2];
+    const uint64_t ma = __ballot(fa), mb = __ballot(fb);
+    const int w = (int)(threadIdx.x / 64);
+    if (lane_id() == 0) {
+        s_w[0][w] = (uint32_t)__popcll(ma);
+        s_w[1][w] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    uint32_t pre_a = 0, pre_b = 0, tot_a = 0, tot_b = 0;
+    for (int k = 0; k < NW; k++) {
+        if (k < w) {
+            pre_a += s_w[0][k];
+            pre_b += s_w[1][k];
+        }
+        tot_a += s_w[0][k];
+        tot_b += s_w[1][k];
+    }
+    if (threadIdx.x == 0) {
+        unsigned long long ea = 0, eb = 0;
+        if (t > 0) {
+            st_dev64((uint64_t *)&stat[t], (1ull << 62) | ((unsigned long long)tot_a << 31) | tot_b);
+            for (long j = (long)t - 1; j >= 0;) {
+                const unsigned long long v = ld_dev64((const uint64_t *)&stat[j]);
+                const unsigned f = (unsigned)(v >> 62);
+                if (f == 0) continue;
+                ea += (v >> 31) & 0x7fffffffull;
+                eb += v & 0x7fffffffull;
+                if (f == 2) break;
+                j--;
+            }
+        }
+        st_dev64((uint64_t *)&stat[t], (2ull << 62) | ((ea + tot_a) << 31) | (eb + tot_b));
+        s_base[0] = (uint32_t)ea;
+        s_base[1] = (uint32_t)eb;
+        if (t == ntiles - 1) {
+            *len_a = (unsigned int)(ea + tot_a);
+            *len_b = (unsigned int)(eb + tot_b);
+        }
+    }
+    __syncthreads();
+    pa = s_base[0] + pre_a + (uint32_t)__popcll(ma & lanemask_lt());
+    pb = s_base[1] + pre_b + (uint32_t)__popcll(mb & lanemask_lt());
+}
+
 // A path ends: its sample joins its pixel (NaN samples skipped and the clamp applied as
 // trace_image does, tracer.c:347-349), its slot goes back to the ring, its job counts it. Every
 // lane of the wave calls this (on: the lane's path ends).
@@ -164,15 +219,21 @@ __device__ WalkHit walk_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax, 
         sp++;
         return true;
     };
-    push(0u);
-    while (sp > 0) {
+    auto pop = [&](uint32_t &x) -> bool {
+        if (sp == 0) return false;
         sp--;
-        const uint32_t e = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+        x = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+        return true;
+    };
+    // the next entry stays in a register; only deferred ones go on the stack
+    uint32_t e = 0u;
+    for (;;) {
         if (e == LR_WALK_EXIT) {
             cur = LR_NONE;
             ro = o;
             rd = d;
             inv = inv_w;
+            if (!pop(e)) break;
             continue;
         }
         if ((e & 0xc0000000u) == LR_WALK_INST) {
@@ -180,7 +241,8 @@ __device__ WalkHit walk_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax, 
             to_object(J.inst + 4 * (size_t)i, o, d, ro, rd);
             inv = v3(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
             cur = i;
-            if (!push(LR_WALK_EXIT) || !push(LR_WALK_BLAS | __float_as_uint(J.inst[4 * (size_t)i + 3].z))) break;
+            if (!push(LR_WALK_EXIT)) break;
+            e = LR_WALK_BLAS | __float_as_uint(J.inst[4 * (size_t)i + 3].z);
             continue;
         }
         const bool blas = (e & LR_WALK_BLAS) != 0;
@@ -233,12 +295,11 @@ __device__ WalkHit walk_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax, 
         const bool gl = inner[0] && tl <= best.t, gr = inner[1] && tr <= best.t;
         const uint32_t tag = blas ? LR_WALK_BLAS : 0u;
         if (gl && gr) {
-            if (!push(tag | ch[1 - first]) || !push(tag | ch[first])) break;
-        } else if (gl) {
-            if (!push(tag | ch[0])) break;
-        } else if (gr) {
-            if (!push(tag | ch[1])) break;
-        }
+            if (!push(tag | ch[1 - first])) break;
+            e = tag | ch[first];
+        } else if (gl) e = tag | ch[0];
+        else if (gr) e = tag | ch[1];
+        else if (!pop(e)) break;
     }
     return best;
 }
@@ -254,6 +315,7 @@ __global__ __launch_bounds__(BLK) void k_regen(EngArgs A)
     const unsigned long long free_n = tail - head, todo = iend - next;
     const uint32_t n = (uint32_t)(free_n < todo ? free_n : todo);
     const int cur = (int)(ctl->iter & 1ull);
+    const uint32_t start = ctl->count[cur];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->claimed = n;
         reset_tiles(ctl, K_REGEN);
@@ -298,13 +360,20 @@ __global__ __launch_bounds__(BLK) void k_regen(EngArgs A)
             A.meta[slot] = make_uint2(q, 0u);
             id = (jslot << LR_SLOT_BITS) | slot;
         }
-        const uint32_t at = wave_append(live, &ctl->count[cur]);
+        // every started path has its camera ray: in claim order, after the queue's paths
         if (live) {
+            const uint32_t at = start + i;
             A.qray[cur][2 * (size_t)at] = r0;
             A.qray[cur][2 * (size_t)at + 1] = r1;
             A.qid[cur][at] = id;
         }
     });
+    // the last work-group to finish sets the queue's length (all have read the old one)
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(&ctl->regen_done, 1u) == gridDim.x - 1) {
+        ctl->count[cur] = start + n;
+        ctl->regen_done = 0;
+    }
 }
 
 // extend (wf_ext, trace_wave.wgsl:72-84): the closest hit of every queued ray, each through
@@ -328,6 +397,8 @@ __global__ __launch_bounds__(BLK) void k_extend(EngArgs A)
         reset_tiles(ctl, K_EXTEND);
     }
     const size_t gtid = (size_t)blockIdx.x * BLK + threadIdx.x;
+    // the shade kernel's look-back status words, one per tile of at least 64 entries
+    for (size_t k = gtid; k < (size_t)n / 64 + 2; k += (size_t)gridDim.x * BLK) A.ostat[k] = 0ull;
     const WalkStack S{&stk[0][threadIdx.x], BLK, A.spill + gtid, (size_t)A.spill_stride, STK + A.spill_per};
     uint32_t cn = 0, ct = 0;
     for_tiles<BLK>(&ctl->tiles[K_EXTEND], n, A.mult, [&](uint32_t i, bool live) {
@@ -354,7 +425,7 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
     const int cur = (int)(ctl->iter & 1ull);
     const uint32_t n = ctl->count[cur];
     if (blockIdx.x == 0 && threadIdx.x == 0) reset_tiles(ctl, K_SHADE);
-    for_tiles<BLK>(&ctl->tiles[K_SHADE], n, A.mult, [&](uint32_t i, bool live) {
+    for_tiles<BLK>(&ctl->tiles[K_SHADE], n, 1, [&](uint32_t i, bool live) {
         bool next = false, shadow = false;
         float4 nr0, nr1, sr0, sr1, scon;
         uint32_t id = 0, slot = 0, jslot = 0, q = 0;
@@ -534,13 +605,14 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
             if (next || shadow) A.rad[slot] = rad_out;
             if (shadow && !next) scon.w = 1.0f;   // the shadow kernel ends the path
         }
-        const uint32_t at = wave_append(next, &ctl->count[1 - cur]);
+        uint32_t at, ss;
+        ordered_append2<BLK>(A.ostat, i / BLK, (n + BLK - 1) / BLK, next, shadow, at, ss, &ctl->count[1 - cur],
+                             &ctl->count[2]);
         if (next) {
             A.qray[1 - cur][2 * (size_t)at] = nr0;
             A.qray[1 - cur][2 * (size_t)at + 1] = nr1;
             A.qid[1 - cur][at] = id;
         }
-        const uint32_t ss = wave_append(shadow, &ctl->count[2]);
         if (shadow) {
             A.sray[2 * (size_t)ss] = sr0;
             A.sray[2 * (size_t)ss + 1] = sr1;

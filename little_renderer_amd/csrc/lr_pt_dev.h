@@ -340,11 +340,30 @@ __device__ V3 bsdf_eval(Bsdf &b, const float *tab, V3 n, V3 v, V3 l, float &pdf)
     return f;
 }
 
+// p[lobe] without dynamic indexing (the array then stays in registers)
+__device__ __forceinline__ float lobe_p(const Bsdf &b, int lobe)
+{
+    return lobe == 0 ? b.p[0] : lobe == 1 ? b.p[1] : lobe == 2 ? b.p[2] : b.p[3];
+}
+
 __device__ __forceinline__ int bsdf_pick(const Bsdf &b, float ul)   // brdf.h:280-285
 {
     int lobe = 0;
-    for (float acc = b.p[0]; lobe < LOBES - 1 && (ul >= acc || b.p[lobe] <= 0.0f); acc += b.p[++lobe]);
-    return b.p[lobe] > 0.0f ? lobe : -1;
+    // unrolled, so that the probabilities stay in registers: the sums in the same order
+    float acc = b.p[0];
+    if (ul >= acc || b.p[0] <= 0.0f) {
+        lobe = 1;
+        acc += b.p[1];
+        if (ul >= acc || b.p[1] <= 0.0f) {
+            lobe = 2;
+            acc += b.p[2];
+            if (ul >= acc || b.p[2] <= 0.0f) {
+                lobe = 3;
+                acc += b.p[3];
+            }
+        }
+    }
+    return lobe_p(b, lobe) > 0.0f ? lobe : -1;
 }
 
 struct BSample {
@@ -358,20 +377,21 @@ __device__ bool bsdf_sample_lobe(Bsdf &b, const float *tab, int lobe, V3 n, V3 v
     const float cv = vdot(v, n);
     if (cv <= 0.0f || lobe < 0) return false;
     bsdf_view(b, tab, cv);
-    if (b.p[lobe] <= 0.0f) return false;
+    const float plobe = lobe_p(b, lobe);
+    if (plobe <= 0.0f) return false;
     const float m = b.metallic, t = b.transmission;
     s.delta = false;
     if (b.smooth && (lobe == SPEC || lobe == TRANS)) {
         const float fd = fresnel_dielectric(cv, b.eta);
         if (lobe == SPEC) {
             const V3 fm = fresnel_metal(b.base, cv);
-            const float k = 1.0f / b.p[lobe];
+            const float k = 1.0f / plobe;
             s.l = vreflect(vneg(v), n);
             s.w = v3(((1.0f - m) * fd + m * fm.x) * k, ((1.0f - m) * fd + m * fm.y) * k, ((1.0f - m) * fd + m * fm.z) * k);
         } else {
             if (fd >= 1.0f) return false;
             s.l = vrefract(vneg(v), n, b.eta);
-            s.w = vscale(b.base, (1.0f - m) * t * (1.0f - fd) / b.p[lobe]);
+            s.w = vscale(b.base, (1.0f - m) * t * (1.0f - fd) / plobe);
         }
         s.pdf = 0.0f;
         s.delta = true;

@@ -248,11 +248,10 @@ __device__ __forceinline__ WalkHit walk2(const float4 *__restrict__ wnodes, cons
         else S.spill[(size_t)(sp - STK) * S.sstride] = x;
         sp++;
     };
-    push(0u);
     bool done = false;
-    while (sp > 0) {
-        sp--;
-        const uint32_t ni = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+    // the next node stays in a register; only the farther child of two goes on the stack
+    uint32_t ni = 0u;
+    for (;;) {
         const float4 l0 = wnodes[4 * (size_t)ni], l1 = wnodes[4 * (size_t)ni + 1];
         const float4 r0 = wnodes[4 * (size_t)ni + 2], r1 = wnodes[4 * (size_t)ni + 3];
         if (COUNT) cn++;
@@ -276,15 +275,20 @@ __device__ __forceinline__ WalkHit walk2(const float4 *__restrict__ wnodes, cons
             if (ANY && done) return best;
         }
         const bool gl = tl >= 0.0f && !lleaf && tl <= best.t, gr = tr >= 0.0f && !rleaf && tr <= best.t;
-        if (sp + 2 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
-            atomicOr(overflow, 1u);
-            break;
-        }
         if (gl && gr) {
+            if (sp + 1 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
+                atomicOr(overflow, 1u);
+                break;
+            }
             push(l_first ? ra : la);
-            push(l_first ? la : ra);
-        } else if (gl) push(la);
-        else if (gr) push(ra);
+            ni = l_first ? la : ra;
+        } else if (gl) ni = la;
+        else if (gr) ni = ra;
+        else {
+            if (sp == 0) break;
+            sp--;
+            ni = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+        }
     }
     return best;
 }
