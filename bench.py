@@ -341,7 +341,7 @@ def bench_batched(a, torch, lr):
 
     # ---- kernel times (events around each kernel, iterations launched without the graph) and the
     # roofline of the dominant kernel: the extension walk's algorithmic bytes from its own counters
-    # (64-byte node records x 64 B, shapes x 52 B, ray 32 B + hit 16 B per ray) over its time ----
+    # (4-wide nodes x 128 B, shapes x 52 B, ray 32 B + hit 16 B per ray) over its time ----
     eng.profiling(True)
     k0 = eng.stats()
     batch()
@@ -358,7 +358,7 @@ def bench_batched(a, torch, lr):
     nodes = c1.node_fetches - c0.node_fetches
     shapes = c1.shape_tests - c0.shape_tests
     rext = c1.rays_ext - c0.rays_ext
-    alg = 48 * rext + 64 * nodes + 52 * shapes   # the engine's node records hold both children: 64 B
+    alg = 48 * rext + 128 * nodes + 52 * shapes   # the engine walks 4-wide nodes: 4 x 32-byte slots
     dom = max(kms, key=kms.get)
     achieved = alg / (kms["extend"] * 1e-3) / 1e9
 

@@ -27,7 +27,8 @@
 // one job on the device (written by a kernel with the descriptor as its argument, so the
 // host's copy is never read late)
 struct JobDev {
-    const float4 *nodes;      // 64-byte records, children's boxes in the parent (walk2)
+    const float4 *nodes;      // 4-wide nodes (walk4); a two-level scene: its top level (walk_tl)
+    const float4 *bnodes;     // the same tree's binary records, children's boxes in the parent (walk2)
     const uint32_t *aux;
     const float4 *tris;
     const float4 *prims;
@@ -135,6 +136,7 @@ struct LrhPtScene {
     float4 *prims = nullptr, *tris = nullptr, *mats = nullptr, *boxes = nullptr;
     uint32_t *nodes = nullptr, *aux = nullptr;
     float4 *wnodes = nullptr;                 // the same tree, children's boxes in the parent (walk2)
+    float4 *w4nodes = nullptr;                // the same tree, 4-wide (walk4)
     // the sky
     float *env_rgb = nullptr, *env_marg = nullptr, *env_cond = nullptr, *env_prob = nullptr;
     int env_w = 0, env_h = 0;

@@ -407,7 +407,7 @@ __global__ __launch_bounds__(BLK) void k_extend(EngArgs A)
         const JobDev &J = A.jobs[id >> LR_SLOT_BITS];
         const float4 r0 = A.qray[cur][2 * (size_t)i], r1 = A.qray[cur][2 * (size_t)i + 1];
         const WalkHit h = J.two_level ? walk_tl<false, COUNT, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct)
-                                      : walk2<false, COUNT, STK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
+                                      : walk4<false, COUNT, STK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
                                                                  v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
         A.hits[i] = make_float4(h.t, __uint_as_float(h.id), h.b1, h.b2);
     });
@@ -650,7 +650,7 @@ __global__ __launch_bounds__(BLK) void k_shadow(EngArgs A)
             const float4 r0 = A.sray[2 * (size_t)i], r1 = A.sray[2 * (size_t)i + 1];
             uint32_t cn, ct;
             const WalkHit h = J.two_level ? walk_tl<true, false, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct)
-                                          : walk2<true, false, STK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
+                                          : walk2<true, false, STK>(J.bnodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
                                                                     v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
             const float4 c = A.scon[i];
             ends = c.w != 0.0f;

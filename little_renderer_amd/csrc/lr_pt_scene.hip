@@ -82,10 +82,103 @@ __global__ __launch_bounds__(256) void k_inst_boxes(const float4 *__restrict__ f
     boxes[2 * (size_t)i + 1] = bh;
 }
 
+// ---- the 4-wide tree (bvh_make_wide / collapse, engine/rt/bvh.c:487-539; put_wide,
+// trace_gpu.c:867-881): level by level, one wide node per thread ----
+struct WideTask {
+    uint32_t node, wide;   // the binary node the wide node stands for, its index
+};
+
+__device__ __forceinline__ float node_area(const float4 *nodes, uint32_t i)   // box_area (bvh.c:30-34), halved
+{
+    const float4 lo = nodes[2 * (size_t)i], hi = nodes[2 * (size_t)i + 1];
+    const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+    if (dx < 0.0f || dy < 0.0f || dz < 0.0f) return 0.0f;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ void k_collapse_init(WideTask *q0, uint32_t *cnt, int levels, uint32_t *nwide, uint32_t root)
+{
+    for (int l = 0; l <= levels + 1; l++) cnt[l] = 0;
+    q0[0] = WideTask{root, 0u};
+    cnt[0] = 1;
+    *nwide = 1;
+}
+
+__global__ __launch_bounds__(256) void k_collapse(const float4 *__restrict__ nodes, const WideTask *__restrict__ qin,
+                                                  const uint32_t *cin, WideTask *__restrict__ qout, uint32_t *cout,
+                                                  uint32_t *nwide, float4 *__restrict__ w4)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= *cin) return;
+    const WideTask e = qin[i];
+    uint32_t kids[4];
+    int n = 0;
+    const uint32_t rb = __float_as_uint(nodes[2 * (size_t)e.node + 1].w);
+    if (rb & LR_LEAF) kids[n++] = e.node;   // a tree that is one leaf
+    else {
+        kids[n++] = __float_as_uint(nodes[2 * (size_t)e.node].w);
+        kids[n++] = rb;
+    }
+    // open the inner child of largest area until there are four (or only leaves)
+    while (n < 4) {
+        int best = -1;
+        float area = -1.0f;
+        for (int k = 0; k < n; k++) {
+            const bool inner = !(__float_as_uint(nodes[2 * (size_t)kids[k] + 1].w) & LR_LEAF);
+            const float a = node_area(nodes, kids[k]);
+            if (inner && a > area) {
+                area = a;
+                best = k;
+            }
+        }
+        if (best < 0) break;
+        const uint32_t c = kids[best];
+        kids[best] = __float_as_uint(nodes[2 * (size_t)c].w);
+        kids[n++] = __float_as_uint(nodes[2 * (size_t)c + 1].w);
+    }
+    for (int k = 0; k < 4; k++) {
+        float4 lo = make_float4(1e30f, 1e30f, 1e30f, __uint_as_float(0u));
+        float4 hi = make_float4(1e30f, 1e30f, 1e30f, __uint_as_float(LR_NONE));
+        if (k < n) {
+            lo = nodes[2 * (size_t)kids[k]];
+            hi = nodes[2 * (size_t)kids[k] + 1];
+            if (!(__float_as_uint(hi.w) & LR_LEAF)) {
+                const uint32_t w = atomicAdd(nwide, 1u);
+                qout[atomicAdd(cout, 1u)] = WideTask{kids[k], w};
+                lo.w = __uint_as_float(4u * w);
+                hi.w = __uint_as_float(0u);
+            }
+        }
+        w4[2 * (4 * (size_t)e.wide + k)] = lo;
+        w4[2 * (4 * (size_t)e.wide + k) + 1] = hi;
+    }
+}
+
+// the wide tree of a binary tree (root `root`, n items, depth levels) into w4 (room for n wide
+// nodes); waits for it
+static int collapse_wide(LrhContext *c, const uint32_t *nodes, uint32_t root, int n, int depth, float4 *w4)
+{
+    const size_t qbytes = lr_align((size_t)(n > 1 ? n : 2) * sizeof(WideTask));
+    int rc = lr_scratch_reserve(c, LR_SCRATCH_TRACE, 2 * qbytes + lr_align((size_t)(depth + 4) * 4) + 256);
+    if (rc) return rc;
+    char *base = (char *)c->scratch[LR_SCRATCH_TRACE].base;
+    WideTask *q[2] = {(WideTask *)base, (WideTask *)(base + qbytes)};
+    uint32_t *cnt = (uint32_t *)(base + 2 * qbytes);
+    uint32_t *nwide = cnt + depth + 3;
+    hipLaunchKernelGGL(k_collapse_init, dim3(1), dim3(1), 0, c->stream, q[0], cnt, depth, nwide, root);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    for (int l = 0; l <= depth; l++)
+        hipLaunchKernelGGL(k_collapse, grid, dim3(256), 0, c->stream, (const float4 *)nodes, (const WideTask *)q[l & 1],
+                           (const uint32_t *)(cnt + l), q[(l + 1) & 1], cnt + l + 1, nwide, w4);
+    LR_HIP(hipGetLastError());
+    LR_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 static void pt_scene_release(LrhPtScene *s)
 {
     if (!s) return;
-    void *ps[] = {s->prims, s->tris, s->mats, s->boxes, s->nodes, s->aux, s->tab, s->mask, s->wnodes, s->env_rgb,
+    void *ps[] = {s->prims, s->tris, s->mats, s->boxes, s->nodes, s->aux, s->tab, s->mask, s->wnodes, s->w4nodes, s->env_rgb,
                   s->env_marg, s->env_cond, s->env_prob, s->mboxes, s->mwnodes, s->inst, s->ifwd, s->mnodes, s->maux,
                   s->mroot};
     for (void *p : ps)
@@ -202,8 +295,20 @@ extern "C" int lrh_pt_scene(LrhContext *c, const float *prims, int nprims, const
                        (const float4 *)s->nodes, (int)nn, s->wnodes);
     if (n == 1)   // the root is the one shape's leaf
         hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, c->stream, (const float4 *)s->nodes, 0u, s->wnodes);
+    // the 4-wide tree the jobs walk; its stack: up to 3 entries per wide level
+    if (hipMalloc((void **)&s->w4nodes, (size_t)n * 128) != hipSuccess) {
+        (void)hipGetLastError();
+        pt_scene_release(s);
+        return lr_fail(LRH_ENOMEM, "lrh_pt_scene: device memory for the scene could not be had");
+    }
+    rc = collapse_wide(c, s->nodes, 0u, n, st.depth, s->w4nodes);
+    if (rc) {
+        pt_scene_release(s);
+        return rc;
+    }
+    s->depth = 2 * st.depth + 4;
     s->bytes = (size_t)nprims * 96 + (size_t)ntris * 48 + hm.size() * 4 + boxes.size() * 4 + (2 * (size_t)n - 1) * 32 +
-               (size_t)n * 4 + tab.size() * 4 + mask.size() * 2 + (2 * (size_t)n - 1) * 64;
+               (size_t)n * 4 + tab.size() * 4 + mask.size() * 2 + (2 * (size_t)n - 1) * 64 + (size_t)n * 128;
     // the tree is built on the context's stream; the engines that render the scene run on
     // streams of their own, so the scene is complete before it is handed out
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -430,7 +535,8 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
 
 void lr_job_scene(JobDev &jd, const LrhPtScene *s)
 {
-    jd.nodes = s->wnodes;
+    jd.nodes = s->two_level ? s->wnodes : s->w4nodes;
+    jd.bnodes = s->wnodes;
     jd.aux = s->aux;
     jd.tris = s->tris;
     jd.prims = s->prims;
@@ -469,7 +575,7 @@ __global__ __launch_bounds__(128) void k_scene_trace(JobDev J, const float4 *__r
     const WalkStack S{&stk[0][threadIdx.x], 128, spill + i, (size_t)n, LR_LDS_STACK + spill_per};
     uint32_t cn = 0, ct = 0;
     const WalkHit h = J.two_level ? walk_tl<ANY, false, LR_LDS_STACK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, overflow, cn, ct)
-                                  : walk2<ANY, false, LR_LDS_STACK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
+                                  : walk4<ANY, false, LR_LDS_STACK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
                                                                     v3of(r1), r0.w, r1.w, S, overflow, cn, ct);
     hits[i] = make_float4(h.t, __uint_as_float(h.id), h.b1, h.b2);
 }

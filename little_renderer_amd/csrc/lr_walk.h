@@ -292,3 +292,99 @@ __device__ __forceinline__ WalkHit walk2(const float4 *__restrict__ wnodes, cons
     }
     return best;
 }
+
+// ---- the engine's 4-wide walk ---------------------------------------------------------------
+// A wide node is 4 consecutive 32-byte slots, one per child (the reference's put_wide layout,
+// trace_gpu.c:867-881): the child's box, then (a, b): a leaf child (first aux entry,
+// LR_LEAF | count), an inner child (its own first slot, 0), or no child (b = LR_NONE). Made from
+// the binary tree by k_collapse (lr_pt_scene.hip) with the collapse rule of bvh.c:497-530.
+
+// the entry distance of a box, or -1: the slab test with the ray's o * inv folded into one fused
+// multiply-add per plane (a culling test: its last bit does not decide any hit)
+__device__ __forceinline__ float box_t_fma(float4 lo, float4 hi, V3 oi, V3 inv, float tmin, float tmax)
+{
+    const float t0x = __builtin_fmaf(lo.x, inv.x, -oi.x), t0y = __builtin_fmaf(lo.y, inv.y, -oi.y);
+    const float t0z = __builtin_fmaf(lo.z, inv.z, -oi.z);
+    const float t1x = __builtin_fmaf(hi.x, inv.x, -oi.x), t1y = __builtin_fmaf(hi.y, inv.y, -oi.y);
+    const float t1z = __builtin_fmaf(hi.z, inv.z, -oi.z);
+    const float enter = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+    const float leave = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+    return leave < enter ? -1.0f : enter;
+}
+
+__device__ __forceinline__ void cswap_(float &ta, uint32_t &ea, float &tb, uint32_t &eb)
+{
+    const bool s = tb < ta;
+    const float t = s ? tb : ta;
+    const uint32_t e = s ? eb : ea;
+    tb = s ? ta : tb;
+    eb = s ? ea : eb;
+    ta = t;
+    ea = e;
+}
+
+template <bool ANY, bool COUNT, int STK>
+__device__ __forceinline__ WalkHit walk4(const float4 *__restrict__ w4, const uint32_t *__restrict__ aux,
+                                         const float4 *__restrict__ tris, const float4 *__restrict__ prims,
+                                         uint32_t nprims, V3 o, V3 d, float tmin, float tmax, const WalkStack &S,
+                                         uint32_t *overflow, uint32_t &cn, uint32_t &ct)
+{
+    const V3 inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    const V3 oi = v3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+    WalkHit best{tmax, LR_NONE, 0.0f, 0.0f};
+    int sp = 0;
+    bool done = false;
+    uint32_t cur = 0u;   // the wide node's first slot; the next one stays in a register
+    for (;;) {
+        if (COUNT) cn++;
+        float t[4];
+        uint32_t ea[4], eb[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float4 lo = w4[2 * (size_t)(cur + k)], hi = w4[2 * (size_t)(cur + k) + 1];
+            ea[k] = __float_as_uint(lo.w);
+            eb[k] = __float_as_uint(hi.w);
+            t[k] = eb[k] == LR_NONE ? -1.0f : box_t_fma(lo, hi, oi, inv, tmin, best.t);
+        }
+        // the leaves hit, in slot order (their hits shorten best.t for the boxes that follow)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (t[k] >= 0.0f && (eb[k] & LR_LEAF) && t[k] <= best.t) {
+                leaf_hits_(aux, tris, prims, nprims, ea[k], eb[k] & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
+                if (ANY && done) return best;
+            }
+        }
+        // the inner children hit, nearest first: a sorting network over 4
+        float s0 = t[0] >= 0.0f && !(eb[0] & LR_LEAF) && t[0] <= best.t ? t[0] : 3e38f;
+        float s1 = t[1] >= 0.0f && !(eb[1] & LR_LEAF) && t[1] <= best.t ? t[1] : 3e38f;
+        float s2 = t[2] >= 0.0f && !(eb[2] & LR_LEAF) && t[2] <= best.t ? t[2] : 3e38f;
+        float s3 = t[3] >= 0.0f && !(eb[3] & LR_LEAF) && t[3] <= best.t ? t[3] : 3e38f;
+        uint32_t c0 = ea[0], c1 = ea[1], c2 = ea[2], c3 = ea[3];
+        cswap_(s0, c0, s1, c1);
+        cswap_(s2, c2, s3, c3);
+        cswap_(s0, c0, s2, c2);
+        cswap_(s1, c1, s3, c3);
+        cswap_(s1, c1, s2, c2);
+        const int m = (s0 < 3e38f) + (s1 < 3e38f) + (s2 < 3e38f) + (s3 < 3e38f);
+        if (m == 0) {
+            if (sp == 0) break;
+            sp--;
+            cur = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+            continue;
+        }
+        if (sp + m - 1 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
+            atomicOr(overflow, 1u);
+            break;
+        }
+        auto push = [&](uint32_t x) {
+            if (sp < STK) S.stk[sp * S.stride] = x;
+            else S.spill[(size_t)(sp - STK) * S.sstride] = x;
+            sp++;
+        };
+        if (m > 3) push(c3);
+        if (m > 2) push(c2);
+        if (m > 1) push(c1);
+        cur = c0;
+    }
+    return best;
+}
