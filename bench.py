@@ -358,7 +358,7 @@ def bench_batched(a, torch, lr):
     nodes = c1.node_fetches - c0.node_fetches
     shapes = c1.shape_tests - c0.shape_tests
     rext = c1.rays_ext - c0.rays_ext
-    alg = 48 * rext + 128 * nodes + 52 * shapes   # the engine walks 4-wide nodes: 4 x 32-byte slots
+    alg = 48 * rext + 64 * nodes + 52 * shapes   # the engine walks 64-byte (compressed 4-wide) nodes
     dom = max(kms, key=kms.get)
     achieved = alg / (kms["extend"] * 1e-3) / 1e9
 
@@ -527,7 +527,7 @@ def bench_instanced(a, torch, lr):
     c1 = eng.stats()
     eng.counting(False)
     nodes, shapes, rext = c1.node_fetches - c0.node_fetches, c1.shape_tests - c0.shape_tests, c1.rays_ext - c0.rays_ext
-    alg = 48 * rext + 64 * nodes + 52 * shapes
+    alg = 48 * rext + 64 * nodes + 52 * shapes   # 64-byte 4-wide nodes at both levels
     achieved = alg / (kms["extend"] * 1e-3) / 1e9
     ninst = len(w0["instances"])
     out = {

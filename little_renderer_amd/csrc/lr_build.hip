@@ -195,12 +195,21 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_morton(const float4 *__restrict_
     const float lx = prm->lo[0], ly = prm->lo[1], lz = prm->lo[2];
     const float ex = prm->ext[0], ey = prm->ext[1], ez = prm->ext[2];
     const int base = blockIdx.x * SORT_TILE;
-#pragma unroll 4
+    // all the tile's boxes in flight at once, then the codes
+    float4 a[SORT_ITEMS], b[SORT_ITEMS];
+#pragma unroll
     for (int r = 0; r < SORT_ITEMS; r++) {
         const int i = base + r * SORT_BLOCK + threadIdx.x;
-        if (i >= n) break;
-        const float4 a = boxes[2 * (size_t)i], b = boxes[2 * (size_t)i + 1];
-        const float cx = (a.x + b.x) * 0.5f, cy = (a.y + b.y) * 0.5f, cz = (a.z + b.z) * 0.5f;
+        if (i < n) {
+            a[r] = boxes[2 * (size_t)i];
+            b[r] = boxes[2 * (size_t)i + 1];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        const int i = base + r * SORT_BLOCK + threadIdx.x;
+        if (i >= n) continue;
+        const float cx = (a[r].x + b[r].x) * 0.5f, cy = (a[r].y + b[r].y) * 0.5f, cz = (a[r].z + b[r].z) * 0.5f;
         const uint32_t code = (spread(quant(cx, lx, ex)) << 2u) | (spread(quant(cy, ly, ey)) << 1u) | spread(quant(cz, lz, ez));
         keys[i] = code;
         vals[i] = (uint32_t)i;
@@ -219,18 +228,26 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_morton(const float4 *__restrict_
     __syncthreads();
     if (!last) return;
     __threadfence();
-    for (int p = 0; p < SORT_PASSES; p++) {
-        uint32_t v = 0;
-        if (threadIdx.x < 256)
-            for (int c = 0; c < HIST_COPIES; c++) {
-                uint32_t *x = &hist[(c * SORT_PASSES + p) * 256 + threadIdx.x];
-                v += ld_dev(x);
-                st_dev(x, 0u);
-            }
-        uint32_t tot;
-        const uint32_t ex_ = block_excl_scan<SORT_WAVES>(v, &tot, s_wsum);
-        if (threadIdx.x < 256) bucket_base[p * 256 + threadIdx.x] = ex_;
+    // the last block: thread t sums digit t % 256 of pass t / 256 over the copies (all the
+    // copies' loads in flight together), clears them, and the passes are scanned side by side
+    // (four waves per pass)
+    static_assert(SORT_BLOCK == SORT_PASSES * 256, "one thread per (pass, digit)");
+    uint32_t c8[HIST_COPIES];
+#pragma unroll
+    for (int c = 0; c < HIST_COPIES; c++) c8[c] = ld_dev(&hist[c * SORT_PASSES * 256 + threadIdx.x]);
+    uint32_t v = 0;
+#pragma unroll
+    for (int c = 0; c < HIST_COPIES; c++) {
+        v += c8[c];
+        st_dev(&hist[c * SORT_PASSES * 256 + threadIdx.x], 0u);
     }
+    const int w = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if ((threadIdx.x & 63) == 63) s_wsum[w] = inc;
+    __syncthreads();
+    uint32_t off = 0;
+    for (int k = w & ~3; k < w; k++) off += s_wsum[k];
+    bucket_base[threadIdx.x] = off + inc - v;
     if (threadIdx.x == 0) *ticket = 0;
 }
 

@@ -27,10 +27,11 @@
 // one job on the device (written by a kernel with the descriptor as its argument, so the
 // host's copy is never read late)
 struct JobDev {
-    const float4 *nodes;      // 4-wide nodes (walk4); a two-level scene: its top level (walk_tl)
-    const float4 *bnodes;     // the same tree's binary records, children's boxes in the parent (walk2)
+    const float4 *nodes;      // 4-wide nodes (walk4); a two-level scene: its top level, 4-wide (walk4_tl)
+    const float4 *bnodes;     // the same tree's binary records, children's boxes in the parent (walk2, walk_tl)
     const uint32_t *aux;
     const float4 *tris;
+    const float4 *ltris;      // the triangles again, in leaf order: ltris[3 e..] for aux entry e (the walks)
     const float4 *prims;
     const float4 *mats;       // 3 float4 per material: albedo.rgb type | emission.rgb ior | metallic roughness transmission 0
     long long *accum;         // 3 x int64 per pixel, fixed point
@@ -45,19 +46,26 @@ struct JobDev {
     const float *env_rgb, *env_marg, *env_cond, *env_prob;
     int env_w, env_h;
     float env_scale, env_rot, env_rot_c, env_rot_s;
-    // a two-level scene: nodes/aux above are the top level over the instances (aux: instance
-    // ids); the meshes' node records and aux (global triangle ids into tris); per instance 4
-    // float4: world -> object rows, then (kind, material, mesh root, 0) as u32
+    // a two-level scene: nodes/bnodes/aux above are the top level over the instances (aux:
+    // instance ids); the meshes' binary records, 4-wide nodes and aux (global triangle ids into
+    // tris; ltris then holds the meshes' triangles in maux order); per instance 4 float4: world
+    // -> object rows, then (kind, material, mesh root record, mesh root wide slot) as u32
     const float4 *mnodes;
+    const float4 *mnodes4;
     const uint32_t *maux;
     const float4 *inst;
     int two_level;
 };
 
+// the compressed 4-wide nodes' leaf references hold a first aux entry in 24 bits (all ones is
+// kept free: a two-level walk's leaf entry then never equals its EXIT entry)
+#define LR_WIDE_MAX_SHAPES ((1 << 24) - 1)
+
 // instance kinds on the device: a mesh index, or these (trace.wgsl:188-190)
 #define LR_UNIT_SPHERE 0xfffffff0u
 #define LR_UNIT_BOX 0xfffffff1u
-// stack entries of the two-level walk: a top-level node k (< 2^30), INST | instance, BLAS | node
+// stack entries of the two-level walks: a top-level node k (< 2^30), INST | instance, BLAS | node
+// (and, in the 4-wide walk, a mesh leaf: 0xc0000000 | (count - 1) << 24 | first maux entry)
 #define LR_WALK_INST 0x40000000u
 #define LR_WALK_BLAS 0x80000000u
 #define LR_WALK_EXIT 0xffffffffu
@@ -118,10 +126,12 @@ struct EngArgs {
 
 typedef void (*LrEngFn)(dim3 grid, hipStream_t s, const EngArgs &A);
 
-// the kernels, by work-group size (64, 128, 256: index 0..2) and LDS stack (16, 24, 32)
+// the kernels, by work-group size (64, 128, 256: index 0..2), LDS stack (16, 24, 32) and, for the
+// walks, whether they take two-level scenes too (0: one-level scenes only; the two-level walks'
+// registers would lower the one-level walks' occupancy)
 struct LrEngTable {
     LrEngFn regen[3], shade[3];
-    LrEngFn extend[3][3], extend_count[3][3], shadow[3][3];
+    LrEngFn extend[3][3][2], extend_count[3][3][2], shadow[3][3][2];
 };
 void lr_eng_fill_b64(LrEngTable &t);
 void lr_eng_fill_b128(LrEngTable &t);
@@ -137,6 +147,7 @@ struct LrhPtScene {
     uint32_t *nodes = nullptr, *aux = nullptr;
     float4 *wnodes = nullptr;                 // the same tree, children's boxes in the parent (walk2)
     float4 *w4nodes = nullptr;                // the same tree, 4-wide (walk4)
+    float4 *ltris = nullptr;                  // the triangles in leaf order (zeros at primitives' entries)
     // the sky
     float *env_rgb = nullptr, *env_marg = nullptr, *env_cond = nullptr, *env_prob = nullptr;
     int env_w = 0, env_h = 0;
@@ -144,6 +155,8 @@ struct LrhPtScene {
     // a two-level scene (lrh_pt_scene_instanced): meshes, instances, the top level
     int two_level = 0, nmeshes = 0, ninst = 0, mesh_depth = 0, tlas_depth = 0;
     float4 *mtris = nullptr, *mboxes = nullptr, *mwnodes = nullptr, *inst = nullptr, *ifwd = nullptr;
+    float4 *mw4nodes = nullptr, *tw4nodes = nullptr;   // the meshes' and the top level's 4-wide nodes
+    uint32_t *mroot4 = nullptr;                        // each mesh's root wide slot
     uint32_t *mnodes = nullptr, *maux = nullptr, *mroot = nullptr;
     float4 *tboxes = nullptr;
     uint32_t *tnodes = nullptr, *taux = nullptr;

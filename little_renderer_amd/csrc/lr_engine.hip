@@ -152,6 +152,7 @@ struct LrhEngine {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool graph_ok = false;
+    bool two_level = false;     // a two-level scene has been submitted: the walks that take both
     double clk_khz = 100000.0, clk_host0 = 0.0;
     unsigned long long clk_dev0 = 0;
 };
@@ -196,11 +197,12 @@ void launch_iteration(LrhEngine *e, const LrhTune &t, hipEvent_t *ev = nullptr)
     if (ev) (void)hipEventRecord(ev[0], e->stream);
     T.regen[bs](gs, e->stream, As);
     if (ev) (void)hipEventRecord(ev[1], e->stream);
-    (e->counting ? T.extend_count : T.extend)[bt][sk](gt, e->stream, At);
+    const int tl = e->two_level ? 1 : 0;
+    (e->counting ? T.extend_count : T.extend)[bt][sk][tl](gt, e->stream, At);
     if (ev) (void)hipEventRecord(ev[2], e->stream);
     T.shade[bs](gs, e->stream, As);
     if (ev) (void)hipEventRecord(ev[3], e->stream);
-    T.shadow[bt][sk](gt, e->stream, At);
+    T.shadow[bt][sk][tl](gt, e->stream, At);
     if (ev) (void)hipEventRecord(ev[4], e->stream);
 }
 
@@ -302,6 +304,7 @@ int check_overflow(LrhEngine *e)
     uint32_t f = 0;
     ENG_HIP(hipMemcpyAsync(&f, e->overflow, 4, hipMemcpyDeviceToHost, e->stream));
     ENG_HIP(hipStreamSynchronize(e->stream));
+    if (f & 2u) return lr_fail(LRH_EDEVICE, "engine: a two-level scene's ray reached a one-level walk");
     if (f) return lr_fail(LRH_EREFUSED, "engine: a walk needed more stack than the engine's tree depth allows");
     return 0;
 }
@@ -375,6 +378,12 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
     h.t_submit = now_ms();
     h.min_seq = e->seq_next;
     const LrhPtScene *s = j->scene;
+    if (s->two_level && !e->two_level) {
+        // from now on the walks that take two-level scenes too; the graphs already launched
+        // run before this job's first paths are issued (the same stream)
+        e->two_level = true;
+        e->graph_ok = false;
+    }
     JobDev jd;
     memset(&jd, 0, sizeof(jd));
     lr_job_scene(jd, s);
@@ -688,15 +697,16 @@ int lrh_engine_plain(LrhEngine *e, const LrhJob *job, int64_t *id)
     const dim3 gt((unsigned)((npix + t.block_trace - 1) / t.block_trace)), gs((unsigned)((npix + t.block_shade - 1) / t.block_shade));
     const EngArgs At = args_for(e, t.stack, (size_t)gt.x * t.block_trace, 1);
     const EngArgs As = args_for(e, t.stack, 0, 1);
+    const int tl = e->two_level ? 1 : 0;
     for (int s = 0; s < job->spp; s++) {
         // the pass: the job's paths s * npix .. (s + 1) * npix - 1, one per pixel
         rc = issue_chunk(e, slot, npix);
         if (rc) return rc;
         T.regen[bs](gs, e->stream, As);
         for (int d = 0; d <= job->max_depth; d++) {
-            (e->counting ? T.extend_count : T.extend)[bt][sk](gt, e->stream, At);
+            (e->counting ? T.extend_count : T.extend)[bt][sk][tl](gt, e->stream, At);
             T.shade[bs](gs, e->stream, As);
-            T.shadow[bt][sk](gt, e->stream, At);
+            T.shadow[bt][sk][tl](gt, e->stream, At);
         }
     }
     ENG_HIP(hipGetLastError());

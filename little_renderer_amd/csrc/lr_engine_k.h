@@ -195,8 +195,8 @@ __device__ __forceinline__ V3 unit_box_normal(V3 p)
     return v3(0.0f, 0.0f, p.z < 0.0f ? -1.0f : 1.0f);
 }
 
-// The closest (ANY: the first) hit in a two-level scene: the top level's records (children's
-// boxes in the parent, as walk2), whose leaves hold instances; a mesh instance is entered (the
+// The closest (ANY: the first) hit in a two-level scene: the top level's binary records
+// (children's boxes in the parent, as walk2), whose leaves hold instances; a mesh instance is entered (the
 // ray taken to its space, EXIT pushed under its mesh's root), a unit sphere or box is tested on
 // the spot. The hit's b1 carries the instance (as u32).
 template <bool ANY, bool COUNT, int STK>
@@ -247,7 +247,7 @@ __device__ WalkHit walk_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax, 
         }
         const bool blas = (e & LR_WALK_BLAS) != 0;
         const uint32_t k = e & 0x7fffffffu;
-        const float4 *rec = (blas ? J.mnodes : J.nodes) + 4 * (size_t)k;
+        const float4 *rec = (blas ? J.mnodes : J.bnodes) + 4 * (size_t)k;
         const float4 l0 = rec[0], l1 = rec[1], r0 = rec[2], r1 = rec[3];
         if (COUNT) cn++;
         const float tl = box_t(l0, l1, ro, inv, tmin, best.t);
@@ -300,6 +300,150 @@ __device__ WalkHit walk_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax, 
         } else if (gl) e = tag | ch[0];
         else if (gr) e = tag | ch[1];
         else if (!pop(e)) break;
+    }
+    return best;
+}
+
+// The same on the 4-wide trees (walk4's compressed nodes, one entry at a time): the top level's
+// wide nodes, whose leaf references hold instances, and each mesh's. A node's children hit are
+// sorted by their boxes' entry distance, whatever they are (a node, a mesh's leaf, an instance
+// to enter), so all are visited near to far; unit spheres and boxes are tested on the spot. A
+// mesh's triangles are read in leaf order (ltris) beside their aux entries.
+template <bool ANY, bool COUNT, int STK>
+__device__ WalkHit walk4_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax, const WalkStack &S, uint32_t *overflow,
+                            uint32_t &cn, uint32_t &ct)
+{
+    // the ray in the space being walked (the world's, or an instance's); the world's inverse
+    // direction is computed again on leaving an instance rather than held
+    V3 ro = o, rd = d;
+    V3 inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    V3 oi = v3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+    uint32_t cur = LR_NONE;
+    WalkHit best{tmax, LR_NONE, __uint_as_float(LR_NONE), 0.0f};
+    int sp = 0;
+    auto push = [&](uint32_t x) {
+        if (sp < STK) S.stk[sp * S.stride] = x;
+        else S.spill[(size_t)(sp - STK) * S.sstride] = x;
+        sp++;
+    };
+    auto pop = [&](uint32_t &x) -> bool {
+        if (sp == 0) return false;
+        sp--;
+        x = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+        return true;
+    };
+    // the entry being visited: a top-level node k, INST | i, BLAS | mesh node, MLEAF | (count -
+    // 1) << 24 | first maux entry, EXIT
+    constexpr uint32_t MLEAF = 0xc0000000u;
+    uint32_t e = 0u;
+    for (;;) {
+        if (e == LR_WALK_EXIT) {
+            cur = LR_NONE;
+            ro = o;
+            rd = d;
+            inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+            oi = v3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+            if (!pop(e)) break;
+            continue;
+        }
+        const uint32_t tag = e & 0xc0000000u;
+        if (tag == LR_WALK_INST) {
+            const uint32_t i = e & 0x3fffffffu;
+            const uint32_t kind = __float_as_uint(J.inst[4 * (size_t)i + 3].x);
+            if (kind >= LR_UNIT_SPHERE) {   // a unit sphere or box: tested in its space, the ray kept
+                if (COUNT) ct++;
+                V3 oo, od;
+                to_object(J.inst + 4 * (size_t)i, o, d, oo, od);
+                const float th = kind == LR_UNIT_SPHERE ? hit_unit_sphere(oo, od, tmin, best.t) : hit_unit_box(oo, od, tmin, best.t);
+                if (th > 0.0f) {
+                    best = WalkHit{th, kind, __uint_as_float(i), 0.0f};
+                    if (ANY) return best;
+                }
+                if (!pop(e)) break;
+                continue;
+            }
+            to_object(J.inst + 4 * (size_t)i, o, d, ro, rd);
+            inv = v3(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
+            oi = v3(ro.x * inv.x, ro.y * inv.y, ro.z * inv.z);
+            cur = i;
+            if (sp + 1 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
+                atomicOr(overflow, 1u);
+                break;
+            }
+            push(LR_WALK_EXIT);
+            e = LR_WALK_BLAS | __float_as_uint(J.inst[4 * (size_t)i + 3].w);
+            continue;
+        }
+        if (tag == MLEAF) {   // a mesh's leaf, in the instance's space
+            const uint32_t cnt = ((e >> 24) & 0x3fu) + 1u, first = e & 0xffffffu;
+            for (uint32_t j = 0; j < cnt; j++) {
+                if (COUNT) ct++;
+                const uint32_t x = first + j;
+                const uint32_t sid = J.maux[x];
+                const float4 ta = J.ltris[3 * (size_t)x], tb = J.ltris[3 * (size_t)x + 1], tc = J.ltris[3 * (size_t)x + 2];
+                float th, u, v;
+                if (hit_tri(ta, tb, tc, ro, rd, tmin, best.t, th, u, v)) {
+                    best = WalkHit{th, sid, __uint_as_float(cur), 0.0f};
+                    if (ANY) return best;
+                }
+            }
+            if (!pop(e)) break;
+            continue;
+        }
+        const bool blas = tag == LR_WALK_BLAS;
+        if (COUNT) cn++;
+        float t[4];
+        uint32_t ref[4];
+        wide_test(blas ? J.mnodes4 : J.nodes, e & 0x3fffffffu, oi, inv, tmin, best.t, t, ref);
+        float s[4] = {3e38f, 3e38f, 3e38f, 3e38f};
+        uint32_t c[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!(t[k] >= 0.0f)) continue;
+            if (!(ref[k] & LR_LEAF)) {
+                c[k] = (blas ? LR_WALK_BLAS : 0u) | ref[k];
+                s[k] = t[k];
+                continue;
+            }
+            if (blas) {
+                c[k] = MLEAF | (ref[k] & 0x3fffffffu);
+                s[k] = t[k];
+                continue;
+            }
+            // a top-level leaf: its instances, entries (one leaf, one instance: sorted with the rest)
+            const uint32_t cnt = ((ref[k] >> 24) & 0x3fu) + 1u, first = ref[k] & 0xffffffu;
+            if (cnt == 1) {
+                c[k] = LR_WALK_INST | J.aux[first];
+                s[k] = t[k];
+                continue;
+            }
+            if (sp + cnt > (uint32_t)S.cap) {   // cannot happen with a correct depth: said, never silently dropped
+                atomicOr(overflow, 1u);
+                return best;
+            }
+            for (uint32_t j = 0; j < cnt; j++) push(LR_WALK_INST | J.aux[first + j]);
+        }
+        // nearest first
+        float s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
+        uint32_t c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+        cswap_(s0, c0, s1, c1);
+        cswap_(s2, c2, s3, c3);
+        cswap_(s0, c0, s2, c2);
+        cswap_(s1, c1, s3, c3);
+        cswap_(s1, c1, s2, c2);
+        const int m = (s0 < 3e38f) + (s1 < 3e38f) + (s2 < 3e38f) + (s3 < 3e38f);
+        if (m == 0) {
+            if (!pop(e)) break;
+            continue;
+        }
+        if (sp + m - 1 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
+            atomicOr(overflow, 1u);
+            break;
+        }
+        if (m > 3) push(c3);
+        if (m > 2) push(c2);
+        if (m > 1) push(c1);
+        e = c0;
     }
     return best;
 }
@@ -378,8 +522,9 @@ __global__ __launch_bounds__(BLK) void k_regen(EngArgs A)
 
 // extend (wf_ext, trace_wave.wgsl:72-84): the closest hit of every queued ray, each through
 // its own job's tree. Also applies regen's claim to the ring and the issue counter, and clears
-// the queues the shade kernel appends to.
-template <int BLK, int STK, bool COUNT>
+// the queues the shade kernel appends to. TL: two-level scenes may be queued (without it, such
+// a ray is not walked: flagged, so that the host reports it).
+template <int BLK, int STK, bool COUNT, bool TL>
 __global__ __launch_bounds__(BLK) void k_extend(EngArgs A)
 {
     __shared__ uint32_t stk[STK][BLK];
@@ -406,9 +551,15 @@ __global__ __launch_bounds__(BLK) void k_extend(EngArgs A)
         const uint32_t id = A.qid[cur][i];
         const JobDev &J = A.jobs[id >> LR_SLOT_BITS];
         const float4 r0 = A.qray[cur][2 * (size_t)i], r1 = A.qray[cur][2 * (size_t)i + 1];
-        const WalkHit h = J.two_level ? walk_tl<false, COUNT, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct)
-                                      : walk4<false, COUNT, STK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
-                                                                 v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+        WalkHit h;
+        if (TL && J.two_level) h = walk4_tl<false, COUNT, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+        else if (!J.two_level)
+            h = walk4<false, COUNT, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w, r1.w, S,
+                                         A.overflow, cn, ct);
+        else {
+            atomicOr(A.overflow, 2u);
+            h = WalkHit{r1.w, LR_NONE, 0.0f, 0.0f};
+        }
         A.hits[i] = make_float4(h.t, __uint_as_float(h.id), h.b1, h.b2);
     });
     if (COUNT) {
@@ -625,7 +776,8 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
 
 // shadow (wf_shadow, trace_wave.wgsl:335-355): any hit before the lamp; if none, the
 // contribution joins the path's radiance. Paths whose last vertex made the shadow ray end here.
-template <int BLK, int STK>
+// TL as for extend.
+template <int BLK, int STK, bool TL>
 __global__ __launch_bounds__(BLK) void k_shadow(EngArgs A)
 {
     __shared__ uint32_t stk[STK][BLK];
@@ -649,9 +801,15 @@ __global__ __launch_bounds__(BLK) void k_shadow(EngArgs A)
             const JobDev &J = A.jobs[jslot];
             const float4 r0 = A.sray[2 * (size_t)i], r1 = A.sray[2 * (size_t)i + 1];
             uint32_t cn, ct;
-            const WalkHit h = J.two_level ? walk_tl<true, false, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct)
-                                          : walk2<true, false, STK>(J.bnodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
-                                                                    v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+            WalkHit h;
+            if (TL && J.two_level) h = walk4_tl<true, false, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+            else if (!J.two_level)
+                h = walk4<true, false, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w, r1.w, S,
+                                            A.overflow, cn, ct);
+            else {
+                atomicOr(A.overflow, 2u);
+                h = WalkHit{0.0f, 0u, 0.0f, 0.0f};   // occluded: adds nothing
+            }
             const float4 c = A.scon[i];
             ends = c.w != 0.0f;
             const bool lit = h.id == LR_NONE;
@@ -680,15 +838,27 @@ void L_shade(dim3 g, hipStream_t s, const EngArgs &A)
 {
     hipLaunchKernelGGL(k_shade<BLK>, g, dim3(BLK), 0, s, A);
 }
-template <int BLK, int STK, bool COUNT>
+template <int BLK, int STK, bool COUNT, bool TL>
 void L_extend(dim3 g, hipStream_t s, const EngArgs &A)
 {
-    hipLaunchKernelGGL((k_extend<BLK, STK, COUNT>), g, dim3(BLK), 0, s, A);
+    hipLaunchKernelGGL((k_extend<BLK, STK, COUNT, TL>), g, dim3(BLK), 0, s, A);
 }
-template <int BLK, int STK>
+template <int BLK, int STK, bool TL>
 void L_shadow(dim3 g, hipStream_t s, const EngArgs &A)
 {
-    hipLaunchKernelGGL((k_shadow<BLK, STK>), g, dim3(BLK), 0, s, A);
+    hipLaunchKernelGGL((k_shadow<BLK, STK, TL>), g, dim3(BLK), 0, s, A);
+}
+
+template <int BLK, int STK, int K>
+void fill_walks(LrEngTable &t)
+{
+    const int b = lr_blk_index(BLK);
+    t.extend[b][K][0] = L_extend<BLK, STK, false, false>;
+    t.extend[b][K][1] = L_extend<BLK, STK, false, true>;
+    t.extend_count[b][K][0] = L_extend<BLK, STK, true, false>;
+    t.extend_count[b][K][1] = L_extend<BLK, STK, true, true>;
+    t.shadow[b][K][0] = L_shadow<BLK, STK, false>;
+    t.shadow[b][K][1] = L_shadow<BLK, STK, true>;
 }
 
 template <int BLK>
@@ -697,15 +867,9 @@ void fill_table(LrEngTable &t)
     const int b = lr_blk_index(BLK);
     t.regen[b] = L_regen<BLK>;
     t.shade[b] = L_shade<BLK>;
-    t.extend[b][0] = L_extend<BLK, 16, false>;
-    t.extend[b][1] = L_extend<BLK, 24, false>;
-    t.extend[b][2] = L_extend<BLK, 32, false>;
-    t.extend_count[b][0] = L_extend<BLK, 16, true>;
-    t.extend_count[b][1] = L_extend<BLK, 24, true>;
-    t.extend_count[b][2] = L_extend<BLK, 32, true>;
-    t.shadow[b][0] = L_shadow<BLK, 16>;
-    t.shadow[b][1] = L_shadow<BLK, 24>;
-    t.shadow[b][2] = L_shadow<BLK, 32>;
+    fill_walks<BLK, 16, 0>(t);
+    fill_walks<BLK, 24, 1>(t);
+    fill_walks<BLK, 32, 2>(t);
 }
 
 }  // namespace

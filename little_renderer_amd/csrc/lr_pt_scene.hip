@@ -30,6 +30,26 @@ __global__ __launch_bounds__(256) void k_child_boxes(const float4 *__restrict__ 
     }
 }
 
+// The triangles in leaf order: entry e of aux names shape aux[e]; a triangle's 48 bytes go to
+// ltris[3 e..], a primitive's entry gets zeros (the walks test primitives from prims).
+__global__ __launch_bounds__(256) void k_leaf_tris(const uint32_t *__restrict__ aux, int n, uint32_t nprims,
+                                                   const float4 *__restrict__ tris, float4 *__restrict__ ltris)
+{
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t sid = aux[e];
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a, c = a;
+    if (sid >= nprims) {
+        const size_t t = sid - nprims;
+        a = tris[3 * t];
+        b = tris[3 * t + 1];
+        c = tris[3 * t + 2];
+    }
+    ltris[3 * (size_t)e] = a;
+    ltris[3 * (size_t)e + 1] = b;
+    ltris[3 * (size_t)e + 2] = c;
+}
+
 // A tree whose root is a leaf (one shape) gets a record with that leaf on the left and an
 // empty leaf on the right.
 __global__ void k_root_leaf(const float4 *__restrict__ nodes, uint32_t r, float4 *__restrict__ w)
@@ -82,8 +102,17 @@ __global__ __launch_bounds__(256) void k_inst_boxes(const float4 *__restrict__ f
     boxes[2 * (size_t)i + 1] = bh;
 }
 
-// ---- the 4-wide tree (bvh_make_wide / collapse, engine/rt/bvh.c:487-539; put_wide,
-// trace_gpu.c:867-881): level by level, one wide node per thread ----
+// ---- the 4-wide tree (bvh_make_wide / collapse, engine/rt/bvh.c:487-539): level by level, one
+// wide node per thread. The children are chosen as the reference chooses them; the node is
+// stored compressed (64 bytes instead of put_wide's four 32-byte slots, trace_gpu.c:867-881):
+//   float4 0: the node's box corner lo.xyz, then the three axes' scale exponents (8 bits each,
+//             biased by 127: the scale is 2^(e - 127))
+//   float4 1: the four children's references: a wide node's index; a leaf 0x80000000 |
+//             (count - 1) << 24 | its first aux entry; no child LR_NONE
+//   float4 2: the children's lo.x, lo.y, lo.z, hi.x as 8-bit steps from the corner (byte k:
+//             child k), float4 3: hi.y, hi.z, 0, 0
+// Each child's box is rounded outwards to the steps (lo down, hi up, in exact arithmetic), so it
+// holds the child's own box: the walk visits a few more nodes, never misses a hit. ----
 struct WideTask {
     uint32_t node, wide;   // the binary node the wide node stands for, its index
 };
@@ -96,12 +125,12 @@ __device__ __forceinline__ float node_area(const float4 *nodes, uint32_t i)   //
     return dx * dy + dy * dz + dz * dx;
 }
 
-__global__ void k_collapse_init(WideTask *q0, uint32_t *cnt, int levels, uint32_t *nwide, uint32_t root)
+__global__ void k_collapse_init(WideTask *q0, uint32_t *cnt, int levels, uint32_t *nwide, uint32_t root, uint32_t w0)
 {
     for (int l = 0; l <= levels + 1; l++) cnt[l] = 0;
-    q0[0] = WideTask{root, 0u};
+    q0[0] = WideTask{root, w0};
     cnt[0] = 1;
-    *nwide = 1;
+    *nwide = w0 + 1;
 }
 
 __global__ __launch_bounds__(256) void k_collapse(const float4 *__restrict__ nodes, const WideTask *__restrict__ qin,
@@ -136,27 +165,52 @@ __global__ __launch_bounds__(256) void k_collapse(const float4 *__restrict__ nod
         kids[best] = __float_as_uint(nodes[2 * (size_t)c].w);
         kids[n++] = __float_as_uint(nodes[2 * (size_t)c + 1].w);
     }
-    for (int k = 0; k < 4; k++) {
-        float4 lo = make_float4(1e30f, 1e30f, 1e30f, __uint_as_float(0u));
-        float4 hi = make_float4(1e30f, 1e30f, 1e30f, __uint_as_float(LR_NONE));
-        if (k < n) {
-            lo = nodes[2 * (size_t)kids[k]];
-            hi = nodes[2 * (size_t)kids[k] + 1];
-            if (!(__float_as_uint(hi.w) & LR_LEAF)) {
-                const uint32_t w = atomicAdd(nwide, 1u);
-                qout[atomicAdd(cout, 1u)] = WideTask{kids[k], w};
-                lo.w = __uint_as_float(4u * w);
-                hi.w = __uint_as_float(0u);
-            }
+    // the node's box (the union of its children's) and, per axis, the smallest power of two s
+    // with extent / s < 255
+    const float4 plo = nodes[2 * (size_t)e.node], phi = nodes[2 * (size_t)e.node + 1];
+    const float pl[3] = {plo.x, plo.y, plo.z}, ph[3] = {phi.x, phi.y, phi.z};
+    int ex[3];
+    for (int a = 0; a < 3; a++) {
+        const double ext = (double)ph[a] - (double)pl[a];
+        int k = -126;
+        if (ext > 0.0) {
+            (void)frexp(ext / 255.0, &k);   // ext / 255 < 2^k
+            k = k < -126 ? -126 : (k > 127 ? 127 : k);
         }
-        w4[2 * (4 * (size_t)e.wide + k)] = lo;
-        w4[2 * (4 * (size_t)e.wide + k) + 1] = hi;
+        ex[a] = k;
     }
+    uint32_t ref[4] = {LR_NONE, LR_NONE, LR_NONE, LR_NONE}, qlo[3] = {0u, 0u, 0u}, qhi[3] = {0u, 0u, 0u};
+    for (int k = 0; k < n; k++) {
+        const float4 lo = nodes[2 * (size_t)kids[k]], hi = nodes[2 * (size_t)kids[k] + 1];
+        const uint32_t b = __float_as_uint(hi.w);
+        if (b & LR_LEAF) ref[k] = LR_LEAF | (((b & 0x7fffffffu) - 1u) << 24) | __float_as_uint(lo.w);
+        else {
+            const uint32_t w = atomicAdd(nwide, 1u);
+            qout[atomicAdd(cout, 1u)] = WideTask{kids[k], w};
+            ref[k] = w;
+        }
+        const float cl[3] = {lo.x, lo.y, lo.z}, ch[3] = {hi.x, hi.y, hi.z};
+        for (int a = 0; a < 3; a++) {
+            // steps from the corner, exact in double (a float difference, a power-of-two scale)
+            const double fl = floor(ldexp((double)cl[a] - (double)pl[a], -ex[a]));
+            const double fh = ceil(ldexp((double)ch[a] - (double)pl[a], -ex[a]));
+            const uint32_t l = fl < 0.0 ? 0u : (fl > 255.0 ? 255u : (uint32_t)fl);
+            const uint32_t h = fh < 0.0 ? 0u : (fh > 255.0 ? 255u : (uint32_t)fh);
+            qlo[a] |= l << (8 * k);
+            qhi[a] |= h << (8 * k);
+        }
+    }
+    const uint32_t eb = (uint32_t)(ex[0] + 127) | ((uint32_t)(ex[1] + 127) << 8) | ((uint32_t)(ex[2] + 127) << 16);
+    float4 *o = w4 + 4 * (size_t)e.wide;
+    o[0] = make_float4(plo.x, plo.y, plo.z, __uint_as_float(eb));
+    o[1] = make_float4(__uint_as_float(ref[0]), __uint_as_float(ref[1]), __uint_as_float(ref[2]), __uint_as_float(ref[3]));
+    o[2] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]), __uint_as_float(qhi[0]));
+    o[3] = make_float4(__uint_as_float(qhi[1]), __uint_as_float(qhi[2]), 0.0f, 0.0f);
 }
 
-// the wide tree of a binary tree (root `root`, n items, depth levels) into w4 (room for n wide
-// nodes); waits for it
-static int collapse_wide(LrhContext *c, const uint32_t *nodes, uint32_t root, int n, int depth, float4 *w4)
+// the wide tree of a binary tree (root `root`, n items, depth levels) into w4, its wide nodes
+// numbered from w0 (room for n of them from there); waits for it
+static int collapse_wide(LrhContext *c, const uint32_t *nodes, uint32_t root, int n, int depth, float4 *w4, uint32_t w0 = 0)
 {
     const size_t qbytes = lr_align((size_t)(n > 1 ? n : 2) * sizeof(WideTask));
     int rc = lr_scratch_reserve(c, LR_SCRATCH_TRACE, 2 * qbytes + lr_align((size_t)(depth + 4) * 4) + 256);
@@ -165,7 +219,7 @@ static int collapse_wide(LrhContext *c, const uint32_t *nodes, uint32_t root, in
     WideTask *q[2] = {(WideTask *)base, (WideTask *)(base + qbytes)};
     uint32_t *cnt = (uint32_t *)(base + 2 * qbytes);
     uint32_t *nwide = cnt + depth + 3;
-    hipLaunchKernelGGL(k_collapse_init, dim3(1), dim3(1), 0, c->stream, q[0], cnt, depth, nwide, root);
+    hipLaunchKernelGGL(k_collapse_init, dim3(1), dim3(1), 0, c->stream, q[0], cnt, depth, nwide, root, w0);
     const dim3 grid((unsigned)((n + 255) / 256));
     for (int l = 0; l <= depth; l++)
         hipLaunchKernelGGL(k_collapse, grid, dim3(256), 0, c->stream, (const float4 *)nodes, (const WideTask *)q[l & 1],
@@ -178,9 +232,9 @@ static int collapse_wide(LrhContext *c, const uint32_t *nodes, uint32_t root, in
 static void pt_scene_release(LrhPtScene *s)
 {
     if (!s) return;
-    void *ps[] = {s->prims, s->tris, s->mats, s->boxes, s->nodes, s->aux, s->tab, s->mask, s->wnodes, s->w4nodes, s->env_rgb,
+    void *ps[] = {s->prims, s->tris, s->mats, s->boxes, s->nodes, s->aux, s->tab, s->mask, s->wnodes, s->w4nodes, s->ltris, s->env_rgb,
                   s->env_marg, s->env_cond, s->env_prob, s->mboxes, s->mwnodes, s->inst, s->ifwd, s->mnodes, s->maux,
-                  s->mroot};
+                  s->mroot, s->mw4nodes, s->tw4nodes, s->mroot4};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete s;
@@ -216,6 +270,9 @@ extern "C" int lrh_pt_scene(LrhContext *c, const float *prims, int nprims, const
         (nprims > 0 && !prims) || (ntris > 0 && !tris) || light >= nprims)
         return lr_fail(LRH_EINVAL, "lrh_pt_scene: bad arguments");
     if (light >= 0 && prims[24 * light + 7] < 0.5f) return lr_fail(LRH_EINVAL, "lrh_pt_scene: the lamp must be a quad");
+    if ((int64_t)nprims + ntris > LR_WIDE_MAX_SHAPES)
+        return lr_fail(LRH_EREFUSED, "lrh_pt_scene: %d shapes, the wide nodes' leaf references take at most %d",
+                       nprims + ntris, LR_WIDE_MAX_SHAPES);
     for (int i = 0; i < nprims; i++) {
         const float m = prims[24 * (size_t)i + 11];
         if (!(m >= 0.0f && m < (float)nmats)) return lr_fail(LRH_EINVAL, "lrh_pt_scene: prim %d: material %g of %d", i, m, nmats);
@@ -296,7 +353,7 @@ extern "C" int lrh_pt_scene(LrhContext *c, const float *prims, int nprims, const
     if (n == 1)   // the root is the one shape's leaf
         hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, c->stream, (const float4 *)s->nodes, 0u, s->wnodes);
     // the 4-wide tree the jobs walk; its stack: up to 3 entries per wide level
-    if (hipMalloc((void **)&s->w4nodes, (size_t)n * 128) != hipSuccess) {
+    if (hipMalloc((void **)&s->w4nodes, (size_t)n * 64) != hipSuccess) {
         (void)hipGetLastError();
         pt_scene_release(s);
         return lr_fail(LRH_ENOMEM, "lrh_pt_scene: device memory for the scene could not be had");
@@ -306,9 +363,19 @@ extern "C" int lrh_pt_scene(LrhContext *c, const float *prims, int nprims, const
         pt_scene_release(s);
         return rc;
     }
+    // the triangles in leaf order: a leaf's triangle is then one load after its node, beside
+    // (not after) its aux entry
+    if (hipMalloc((void **)&s->ltris, (size_t)n * 48) != hipSuccess) {
+        (void)hipGetLastError();
+        pt_scene_release(s);
+        return lr_fail(LRH_ENOMEM, "lrh_pt_scene: device memory for the scene could not be had");
+    }
+    hipLaunchKernelGGL(k_leaf_tris, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, s->aux, n, (uint32_t)nprims,
+                       (const float4 *)s->tris, s->ltris);
     s->depth = 2 * st.depth + 4;
     s->bytes = (size_t)nprims * 96 + (size_t)ntris * 48 + hm.size() * 4 + boxes.size() * 4 + (2 * (size_t)n - 1) * 32 +
-               (size_t)n * 4 + tab.size() * 4 + mask.size() * 2 + (2 * (size_t)n - 1) * 64 + (size_t)n * 128;
+               (size_t)n * 4 + tab.size() * 4 + mask.size() * 2 + (2 * (size_t)n - 1) * 64 + (size_t)n * 64 +
+               (size_t)n * 48;
     // the tree is built on the context's stream; the engines that render the scene run on
     // streams of their own, so the scene is complete before it is handed out
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -379,7 +446,7 @@ extern "C" int lrh_pt_scene_sky(LrhPtScene *s, const float *rgb, int w, int h, f
 // the instances' device records (world -> object rows, then kind, material, mesh root) and
 // their object -> world rows, from the caller's transforms
 static int pack_instances(const LrhPtScene *s, const LrhInstance *in, int n, const std::vector<uint32_t> &roots,
-                          std::vector<float> &rec, std::vector<float> &fwd)
+                          const std::vector<uint32_t> &roots4, std::vector<float> &rec, std::vector<float> &fwd)
 {
     rec.assign((size_t)n * 16, 0.0f);
     fwd.assign((size_t)n * 12, 0.0f);
@@ -393,10 +460,11 @@ static int pack_instances(const LrhPtScene *s, const LrhInstance *in, int n, con
         memcpy(o, inv, 48);
         const uint32_t kind = x.kind == LRH_INST_SPHERE ? LR_UNIT_SPHERE : x.kind == LRH_INST_BOX ? LR_UNIT_BOX : (uint32_t)x.kind;
         const uint32_t mat = x.material < 0 ? LR_NONE : (uint32_t)x.material;
-        const uint32_t root = x.kind >= 0 ? roots[x.kind] : 0u;
+        const uint32_t root = x.kind >= 0 ? roots[x.kind] : 0u, root4 = x.kind >= 0 ? roots4[x.kind] : 0u;
         memcpy(o + 12, &kind, 4);
         memcpy(o + 13, &mat, 4);
         memcpy(o + 14, &root, 4);
+        memcpy(o + 15, &root4, 4);
         memcpy(&fwd[(size_t)i * 12], x.xform, 48);
     }
     return 0;
@@ -407,10 +475,13 @@ extern "C" int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int n
     if (!s || !s->two_level || !inst || ninst != s->ninst || (builder != LRH_LBVH && builder != LRH_PLOC))
         return lr_fail(LRH_EINVAL, "lrh_pt_scene_update: bad arguments (an instanced scene, the same instance count)");
     LrhContext *c = s->ctx;
-    std::vector<uint32_t> roots(s->nmeshes);
-    if (s->nmeshes) LR_HIP(hipMemcpy(roots.data(), s->mroot, (size_t)s->nmeshes * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> roots(s->nmeshes), roots4(s->nmeshes);
+    if (s->nmeshes) {
+        LR_HIP(hipMemcpy(roots.data(), s->mroot, (size_t)s->nmeshes * 4, hipMemcpyDeviceToHost));
+        LR_HIP(hipMemcpy(roots4.data(), s->mroot4, (size_t)s->nmeshes * 4, hipMemcpyDeviceToHost));
+    }
     std::vector<float> rec, fwd;
-    int rc = pack_instances(s, inst, ninst, roots, rec, fwd);
+    int rc = pack_instances(s, inst, ninst, roots, roots4, rec, fwd);
     if (rc) return rc;
     LR_HIP(hipStreamSynchronize(c->stream));
     LR_HIP(hipMemcpy(s->inst, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
@@ -426,11 +497,13 @@ extern "C" int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int n
                        (int)nn, s->wnodes);
     if (ninst == 1) hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, c->stream, (const float4 *)s->tnodes, 0u, s->wnodes);
     LR_HIP(hipGetLastError());
-    LR_HIP(hipStreamSynchronize(c->stream));
+    // the top level 4-wide (the closest-hit walk's); collapse_wide waits for it
+    rc = collapse_wide(c, s->tnodes, 0u, ninst, st.depth, s->tw4nodes);
+    if (rc) return rc;
     s->tlas_depth = st.depth;
-    // the two-level walk's stack: a pending sibling and an instance to enter per top-level
-    // level, EXIT, the mesh's walk
-    s->depth = 2 * s->tlas_depth + s->mesh_depth + 6;
+    // the two-level walks' stacks: per wide level of either tree at most three pending
+    // entries (the binary walk: a sibling and an instance per top-level level), and EXIT
+    s->depth = 3 * (s->tlas_depth + s->mesh_depth) + 8;
     if (stats) *stats = st;
     return 0;
 }
@@ -447,6 +520,9 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
         if (mesh_first[m + 1] <= mesh_first[m]) return lr_fail(LRH_EINVAL, "lrh_pt_scene_instanced: mesh %d is empty", m);
     const int M = mesh_first[nmeshes];
     if (M > 0 && !mtris) return lr_fail(LRH_EINVAL, "lrh_pt_scene_instanced: no triangles");
+    if (M > LR_WIDE_MAX_SHAPES || ninst > LR_WIDE_MAX_SHAPES)
+        return lr_fail(LRH_EREFUSED, "lrh_pt_scene_instanced: the wide nodes' leaf references take at most %d shapes",
+                       LR_WIDE_MAX_SHAPES);
     for (int i = 0; i < M; i++) {
         uint32_t w;
         memcpy(&w, mtris + 12 * (size_t)i + 11, 4);
@@ -492,7 +568,10 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
               dev_upload((void **)&s->mask, mask.data(), mask.size() * 2) &&
               dev_upload((void **)&s->inst, nullptr, (size_t)ninst * 64) && dev_upload((void **)&s->ifwd, nullptr, (size_t)ninst * 48) &&
               dev_upload((void **)&s->tboxes, nullptr, (size_t)ninst * 32) && dev_upload((void **)&s->tnodes, nullptr, tn * 32) &&
-              dev_upload((void **)&s->taux, nullptr, (size_t)ninst * 4) && dev_upload((void **)&s->wnodes, nullptr, tn * 64);
+              dev_upload((void **)&s->taux, nullptr, (size_t)ninst * 4) && dev_upload((void **)&s->wnodes, nullptr, tn * 64) &&
+              dev_upload((void **)&s->mw4nodes, nullptr, (size_t)M * 64) && dev_upload((void **)&s->ltris, nullptr, (size_t)M * 48) &&
+              dev_upload((void **)&s->tw4nodes, nullptr, (size_t)ninst * 64) &&
+              dev_upload((void **)&s->mroot4, nullptr, (size_t)nmeshes * 4);
     if (!ok) {
         (void)hipGetLastError();
         pt_scene_release(s);
@@ -503,11 +582,13 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
     s->boxes = s->tboxes;
     int rc = lrh_tri_boxes(c, mtris ? (const float *)s->tris : nullptr, M, (float *)s->mboxes);
     int depth = 0;
+    std::vector<int> mdepth(nmeshes, 1);
     for (int m = 0; !rc && m < nmeshes; m++) {
         const int f = mesh_first[m], n = mesh_first[m + 1] - f;
         LrhBuildStats st;
         rc = lrh_build(c, (const float *)(s->mboxes + 2 * (size_t)f), n, nullptr, builder, roots[m], (uint32_t)f, (uint32_t)f,
                        s->mnodes, s->maux, nullptr, &st);
+        if (!rc) mdepth[m] = st.depth;
         if (!rc && st.depth > depth) depth = st.depth;
         if (!rc && n == 1)
             hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, c->stream, (const float4 *)s->mnodes, roots[m], s->mwnodes);
@@ -515,6 +596,21 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
     if (!rc && nnodes)
         hipLaunchKernelGGL(k_child_boxes, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, c->stream,
                            (const float4 *)s->mnodes, (int)nnodes, s->mwnodes);
+    // each mesh 4-wide, its wide nodes from the mesh's first triangle on (a mesh of n
+    // triangles has at most n wide nodes), and the triangles in the meshes' leaf order
+    std::vector<uint32_t> roots4(nmeshes);
+    for (int m = 0; !rc && m < nmeshes; m++) {
+        const int f = mesh_first[m], n = mesh_first[m + 1] - f;
+        roots4[m] = (uint32_t)f;
+        rc = collapse_wide(c, s->mnodes, roots[m], n, mdepth[m], s->mw4nodes, (uint32_t)f);
+    }
+    if (!rc && M) {
+        hipLaunchKernelGGL(k_leaf_tris, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream, s->maux, M, 0u,
+                           (const float4 *)s->tris, s->ltris);
+        if (hipGetLastError() != hipSuccess) rc = lr_fail(LRH_EDEVICE, "lrh_pt_scene_instanced: a launch failed");
+    }
+    if (!rc && nmeshes && hipMemcpy(s->mroot4, roots4.data(), (size_t)nmeshes * 4, hipMemcpyHostToDevice) != hipSuccess)
+        rc = lr_fail(LRH_EDEVICE, "lrh_pt_scene_instanced: a copy failed");
     if (rc) {
         pt_scene_release(s);
         return rc;
@@ -522,8 +618,8 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
     // the one-shape meshes' root records were written before the pass over all inner nodes,
     // which skips leaves: they stay
     s->mesh_depth = depth;
-    s->bytes = (size_t)M * (48 + 32 + 4) + nnodes * (32 + 64) + hm.size() * 4 + tab.size() * 4 + mask.size() * 2 +
-               (size_t)ninst * (64 + 48 + 32 + 4) + tn * (32 + 64);
+    s->bytes = (size_t)M * (48 + 32 + 4 + 64 + 48) + nnodes * (32 + 64) + hm.size() * 4 + tab.size() * 4 + mask.size() * 2 +
+               (size_t)ninst * (64 + 48 + 32 + 4 + 64) + tn * (32 + 64) + (size_t)nmeshes * 8;
     rc = lrh_pt_scene_update(s, inst, ninst, LRH_LBVH, nullptr);
     if (rc) {
         pt_scene_release(s);
@@ -535,10 +631,11 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
 
 void lr_job_scene(JobDev &jd, const LrhPtScene *s)
 {
-    jd.nodes = s->two_level ? s->wnodes : s->w4nodes;
+    jd.nodes = s->two_level ? s->tw4nodes : s->w4nodes;
     jd.bnodes = s->wnodes;
     jd.aux = s->aux;
     jd.tris = s->tris;
+    jd.ltris = s->ltris;
     jd.prims = s->prims;
     jd.mats = s->mats;
     jd.root = 0;
@@ -558,12 +655,13 @@ void lr_job_scene(JobDev &jd, const LrhPtScene *s)
     jd.env_rot_s = sinf(s->env_rot);
     jd.two_level = s->two_level;
     jd.mnodes = s->mwnodes;
+    jd.mnodes4 = s->mw4nodes;
     jd.maux = s->maux;
     jd.inst = s->inst;
 }
 
-// a batch of rays through a resident scene's own walk (the engine's: walk2, or the two-level
-// walk of an instanced scene); hits t, id, b1 (an instanced scene: the instance, as u32), b2
+// a batch of rays through a resident scene's own walk (the engine's closest-hit walk: walk4, or
+// walk4_tl for an instanced scene); hits t, id, b1 (an instanced scene: the instance, as u32), b2
 template <bool ANY>
 __global__ __launch_bounds__(128) void k_scene_trace(JobDev J, const float4 *__restrict__ rays, int n, float4 *__restrict__ hits,
                                                      uint32_t *spill, int spill_per, uint32_t *overflow)
@@ -574,8 +672,8 @@ __global__ __launch_bounds__(128) void k_scene_trace(JobDev J, const float4 *__r
     const float4 r0 = rays[2 * (size_t)i], r1 = rays[2 * (size_t)i + 1];
     const WalkStack S{&stk[0][threadIdx.x], 128, spill + i, (size_t)n, LR_LDS_STACK + spill_per};
     uint32_t cn = 0, ct = 0;
-    const WalkHit h = J.two_level ? walk_tl<ANY, false, LR_LDS_STACK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, overflow, cn, ct)
-                                  : walk4<ANY, false, LR_LDS_STACK>(J.nodes, J.aux, J.tris, J.prims, J.nprims, v3of(r0),
+    const WalkHit h = J.two_level ? walk4_tl<ANY, false, LR_LDS_STACK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, overflow, cn, ct)
+                                  : walk4<ANY, false, LR_LDS_STACK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0),
                                                                     v3of(r1), r0.w, r1.w, S, overflow, cn, ct);
     hits[i] = make_float4(h.t, __uint_as_float(h.id), h.b1, h.b2);
 }

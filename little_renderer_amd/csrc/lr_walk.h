@@ -110,6 +110,34 @@ __device__ __forceinline__ Prim load_prim(const float4 *prims, uint32_t id)
     return Prim{p[0], p[1], p[2], p[3], p[4], p[5]};
 }
 
+// hit_sphere / hit_quad on a record read as it is needed (the quad's edges only once its plane
+// is hit): the same arithmetic, fewer registers held in the walks
+__device__ __forceinline__ float hit_prim(const float4 *__restrict__ prims, uint32_t id, V3 o, V3 d, float tmin, float tmax,
+                                          float &a_out, float &b_out)
+{
+    const float4 *p = prims + 6 * (size_t)id;
+    const float4 q = p[1];
+    if (q.w < 0.5f) {
+        Prim s;
+        s.c = p[0];
+        return hit_sphere(s, o, d, tmin, tmax);
+    }
+    const float4 nn = p[4];
+    const V3 n = v3of(nn);
+    const float denom = vdot(n, d);
+    if (fabsf(denom) < 1e-12f) return -1.0f;
+    const float t = (nn.w - vdot(n, o)) / denom;
+    if (t <= tmin || t >= tmax) return -1.0f;
+    const float4 u = p[2], v = p[3], w = p[5];
+    const V3 rel = vsub(vadd(o, vscale(d, t)), v3of(q));
+    const float a = vdot(v3of(w), vcross(rel, v3of(v)));
+    const float b = vdot(v3of(w), vcross(v3of(u), rel));
+    if (a < 0.0f || a > 1.0f || b < 0.0f || b > 1.0f) return -1.0f;
+    a_out = a;
+    b_out = b;
+    return t;
+}
+
 struct WalkHit {
     float t;
     uint32_t id;
@@ -200,18 +228,21 @@ __device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32
 // record (first aux entry, LR_LEAF | count). Made from the reference's 32-byte nodes by
 // k_child_boxes (lr_pt_scene.hip); the same tree, so the same hits, with one dependent load per
 // level instead of two (the node, then its children).
-__device__ __forceinline__ void leaf_hits_(const uint32_t *__restrict__ aux, const float4 *__restrict__ tris,
+// ltris: the triangles in leaf order (aux entry e's triangle at ltris[3 e..]), so that the
+// triangle's load does not wait for the aux entry's.
+__device__ __forceinline__ void leaf_hits_(const uint32_t *__restrict__ aux, const float4 *__restrict__ ltris,
                                            const float4 *__restrict__ prims, uint32_t nprims, uint32_t first,
                                            uint32_t cnt, V3 o, V3 d, float tmin, WalkHit &best, bool any, bool &done,
                                            uint32_t &ct, bool count)
 {
     for (uint32_t k = 0; k < cnt; k++) {
-        const uint32_t sid = aux[first + k];
+        const uint32_t e = first + k;
+        const uint32_t sid = aux[e];
+        const float4 ta = ltris[3 * (size_t)e], tb = ltris[3 * (size_t)e + 1], tc = ltris[3 * (size_t)e + 2];
         if (count) ct++;
         if (sid < nprims) {
-            const Prim p = load_prim(prims, sid);
             float a = 0.0f, b = 0.0f;
-            const float t = p.q.w < 0.5f ? hit_sphere(p, o, d, tmin, best.t) : hit_quad(p, o, d, tmin, best.t, a, b);
+            const float t = hit_prim(prims, sid, o, d, tmin, best.t, a, b);
             if (t > 0.0f) {
                 best = WalkHit{t, sid, a, b};
                 if (any) {
@@ -221,8 +252,6 @@ __device__ __forceinline__ void leaf_hits_(const uint32_t *__restrict__ aux, con
             }
             continue;
         }
-        const uint32_t ti = sid - nprims;
-        const float4 ta = tris[3 * (size_t)ti], tb = tris[3 * (size_t)ti + 1], tc = tris[3 * (size_t)ti + 2];
         float t, u, v;
         if (hit_tri(ta, tb, tc, o, d, tmin, best.t, t, u, v)) {
             best = WalkHit{t, sid, u, v};
@@ -236,7 +265,7 @@ __device__ __forceinline__ void leaf_hits_(const uint32_t *__restrict__ aux, con
 
 template <bool ANY, bool COUNT, int STK>
 __device__ __forceinline__ WalkHit walk2(const float4 *__restrict__ wnodes, const uint32_t *__restrict__ aux,
-                                         const float4 *__restrict__ tris, const float4 *__restrict__ prims,
+                                         const float4 *__restrict__ ltris, const float4 *__restrict__ prims,
                                          uint32_t nprims, V3 o, V3 d, float tmin, float tmax, const WalkStack &S,
                                          uint32_t *overflow, uint32_t &cn, uint32_t &ct)
 {
@@ -263,15 +292,15 @@ __device__ __forceinline__ WalkHit walk2(const float4 *__restrict__ wnodes, cons
         // leaves at once, the nearer first (their hits shorten best.t for what follows)
         const bool l_first = tl <= tr;
         if (tl >= 0.0f && lleaf && l_first) {
-            leaf_hits_(aux, tris, prims, nprims, la, lb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
+            leaf_hits_(aux, ltris, prims, nprims, la, lb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
             if (ANY && done) return best;
         }
         if (tr >= 0.0f && rleaf) {
-            leaf_hits_(aux, tris, prims, nprims, ra, rb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
+            leaf_hits_(aux, ltris, prims, nprims, ra, rb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
             if (ANY && done) return best;
         }
         if (tl >= 0.0f && lleaf && !l_first) {
-            leaf_hits_(aux, tris, prims, nprims, la, lb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
+            leaf_hits_(aux, ltris, prims, nprims, la, lb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
             if (ANY && done) return best;
         }
         const bool gl = tl >= 0.0f && !lleaf && tl <= best.t, gr = tr >= 0.0f && !rleaf && tr <= best.t;
@@ -294,22 +323,47 @@ __device__ __forceinline__ WalkHit walk2(const float4 *__restrict__ wnodes, cons
 }
 
 // ---- the engine's 4-wide walk ---------------------------------------------------------------
-// A wide node is 4 consecutive 32-byte slots, one per child (the reference's put_wide layout,
-// trace_gpu.c:867-881): the child's box, then (a, b): a leaf child (first aux entry,
-// LR_LEAF | count), an inner child (its own first slot, 0), or no child (b = LR_NONE). Made from
-// the binary tree by k_collapse (lr_pt_scene.hip) with the collapse rule of bvh.c:497-530.
+// A wide node is 64 bytes (made by k_collapse, lr_pt_scene.hip, from the binary tree with the
+// collapse rule of bvh.c:497-530): its box corner and per-axis power-of-two scales, the four
+// children's references (a wide node's index; a leaf LR_LEAF | (count - 1) << 24 | first aux
+// entry; none LR_NONE), and the children's boxes as 8-bit steps from the corner, rounded
+// outwards.
 
-// the entry distance of a box, or -1: the slab test with the ray's o * inv folded into one fused
-// multiply-add per plane (a culling test: its last bit does not decide any hit)
-__device__ __forceinline__ float box_t_fma(float4 lo, float4 hi, V3 oi, V3 inv, float tmin, float tmax)
+// the entry distances (-1: missed, or no child) and references of a wide node's children: the
+// slab test with each plane's t as one fused multiply-add of its step, fma(q, s * inv, (corner -
+// o) * inv) (a culling test on boxes that hold the children's: its last bit decides no hit)
+__device__ __forceinline__ void wide_test(const float4 *__restrict__ w, uint32_t node, V3 oi, V3 inv, float tmin, float tmax,
+                                          float t[4], uint32_t ref[4])
 {
-    const float t0x = __builtin_fmaf(lo.x, inv.x, -oi.x), t0y = __builtin_fmaf(lo.y, inv.y, -oi.y);
-    const float t0z = __builtin_fmaf(lo.z, inv.z, -oi.z);
-    const float t1x = __builtin_fmaf(hi.x, inv.x, -oi.x), t1y = __builtin_fmaf(hi.y, inv.y, -oi.y);
-    const float t1z = __builtin_fmaf(hi.z, inv.z, -oi.z);
-    const float enter = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-    const float leave = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
-    return leave < enter ? -1.0f : enter;
+    const float4 n0 = w[4 * (size_t)node], n1 = w[4 * (size_t)node + 1];
+    const float4 n2 = w[4 * (size_t)node + 2], n3 = w[4 * (size_t)node + 3];
+    const uint32_t eb = __float_as_uint(n0.w);
+    const float ax = __uint_as_float((eb & 0xffu) << 23) * inv.x;
+    const float ay = __uint_as_float(((eb >> 8) & 0xffu) << 23) * inv.y;
+    const float az = __uint_as_float(((eb >> 16) & 0xffu) << 23) * inv.z;
+    const float bx = __builtin_fmaf(n0.x, inv.x, -oi.x), by = __builtin_fmaf(n0.y, inv.y, -oi.y);
+    const float bz = __builtin_fmaf(n0.z, inv.z, -oi.z);
+    const uint32_t lx = __float_as_uint(n2.x), ly = __float_as_uint(n2.y), lz = __float_as_uint(n2.z);
+    const uint32_t hx = __float_as_uint(n2.w), hy = __float_as_uint(n3.x), hz = __float_as_uint(n3.y);
+    ref[0] = __float_as_uint(n1.x);
+    ref[1] = __float_as_uint(n1.y);
+    ref[2] = __float_as_uint(n1.z);
+    ref[3] = __float_as_uint(n1.w);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int sh = 8 * k;
+        const float t0x = __builtin_fmaf((float)((lx >> sh) & 0xffu), ax, bx);
+        const float t0y = __builtin_fmaf((float)((ly >> sh) & 0xffu), ay, by);
+        const float t0z = __builtin_fmaf((float)((lz >> sh) & 0xffu), az, bz);
+        const float t1x = __builtin_fmaf((float)((hx >> sh) & 0xffu), ax, bx);
+        const float t1y = __builtin_fmaf((float)((hy >> sh) & 0xffu), ay, by);
+        const float t1z = __builtin_fmaf((float)((hz >> sh) & 0xffu), az, bz);
+        const float enter = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+        const float leave = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+        t[k] = ref[k] == LR_NONE || leave < enter ? -1.0f : enter;
+        // one child after the other: interleaving the four holds all 24 planes at once
+        __builtin_amdgcn_sched_barrier(0);
+    }
 }
 
 __device__ __forceinline__ void cswap_(float &ta, uint32_t &ea, float &tb, uint32_t &eb)
@@ -323,9 +377,15 @@ __device__ __forceinline__ void cswap_(float &ta, uint32_t &ea, float &tb, uint3
     ea = e;
 }
 
+// One entry at a time: a wide node (its children's boxes tested, the children hit pushed
+// farthest first, leaves among them) or a leaf (its shapes tested). Leaves and nodes are thus
+// visited near to far, and the leaf test is one copy of its code, not one per child slot: fewer
+// registers held, more waves in flight. The loop is written leaves-inside: a wave tests the
+// leaves its lanes hold, then takes its next node step together (the other nesting, node
+// steps inside until every lane holds a leaf, measured twice as slow).
 template <bool ANY, bool COUNT, int STK>
 __device__ __forceinline__ WalkHit walk4(const float4 *__restrict__ w4, const uint32_t *__restrict__ aux,
-                                         const float4 *__restrict__ tris, const float4 *__restrict__ prims,
+                                         const float4 *__restrict__ ltris, const float4 *__restrict__ prims,
                                          uint32_t nprims, V3 o, V3 d, float tmin, float tmax, const WalkStack &S,
                                          uint32_t *overflow, uint32_t &cn, uint32_t &ct)
 {
@@ -334,32 +394,24 @@ __device__ __forceinline__ WalkHit walk4(const float4 *__restrict__ w4, const ui
     WalkHit best{tmax, LR_NONE, 0.0f, 0.0f};
     int sp = 0;
     bool done = false;
-    uint32_t cur = 0u;   // the wide node's first slot; the next one stays in a register
+    uint32_t cur = 0u;   // the entry being visited (the root node first); the next one stays in a register
     for (;;) {
+        while (cur & LR_LEAF) {
+            leaf_hits_(aux, ltris, prims, nprims, cur & 0xffffffu, ((cur >> 24) & 0x3fu) + 1u, o, d, tmin, best, ANY, done,
+                       ct, COUNT);
+            if (ANY && done) return best;
+            if (sp == 0) return best;
+            sp--;
+            cur = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+        }
         if (COUNT) cn++;
         float t[4];
-        uint32_t ea[4], eb[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const float4 lo = w4[2 * (size_t)(cur + k)], hi = w4[2 * (size_t)(cur + k) + 1];
-            ea[k] = __float_as_uint(lo.w);
-            eb[k] = __float_as_uint(hi.w);
-            t[k] = eb[k] == LR_NONE ? -1.0f : box_t_fma(lo, hi, oi, inv, tmin, best.t);
-        }
-        // the leaves hit, in slot order (their hits shorten best.t for the boxes that follow)
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (t[k] >= 0.0f && (eb[k] & LR_LEAF) && t[k] <= best.t) {
-                leaf_hits_(aux, tris, prims, nprims, ea[k], eb[k] & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
-                if (ANY && done) return best;
-            }
-        }
-        // the inner children hit, nearest first: a sorting network over 4
-        float s0 = t[0] >= 0.0f && !(eb[0] & LR_LEAF) && t[0] <= best.t ? t[0] : 3e38f;
-        float s1 = t[1] >= 0.0f && !(eb[1] & LR_LEAF) && t[1] <= best.t ? t[1] : 3e38f;
-        float s2 = t[2] >= 0.0f && !(eb[2] & LR_LEAF) && t[2] <= best.t ? t[2] : 3e38f;
-        float s3 = t[3] >= 0.0f && !(eb[3] & LR_LEAF) && t[3] <= best.t ? t[3] : 3e38f;
-        uint32_t c0 = ea[0], c1 = ea[1], c2 = ea[2], c3 = ea[3];
+        uint32_t ref[4];
+        wide_test(w4, cur, oi, inv, tmin, best.t, t, ref);
+        // the children hit, nearest first: a sorting network over 4
+        float s0 = t[0] >= 0.0f ? t[0] : 3e38f, s1 = t[1] >= 0.0f ? t[1] : 3e38f;
+        float s2 = t[2] >= 0.0f ? t[2] : 3e38f, s3 = t[3] >= 0.0f ? t[3] : 3e38f;
+        uint32_t c0 = ref[0], c1 = ref[1], c2 = ref[2], c3 = ref[3];
         cswap_(s0, c0, s1, c1);
         cswap_(s2, c2, s3, c3);
         cswap_(s0, c0, s2, c2);
