@@ -26,7 +26,9 @@ import time
 
 import numpy as np
 
-HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
+from little_renderer_amd import roofline as rf
+
+HBM_PEAK_GBS = rf.HBM_PEAK_GBS   # MI355X HBM3E, published
 
 
 def parse():
@@ -170,8 +172,9 @@ def bench_bvh_stress(a, torch, lr):
     ctx.sync()
     cn, ct = [int(x) for x in counts.cpu().tolist()]
     nr = npix if c is None else nsec
-    alg_bytes = 48 * nr + 32 * cn + 52 * ct
-    achieved = alg_bytes / (kms[dom] * 1e-3) / 1e9
+    alg_bytes = rf.walk_bytes(nr, cn, ct, rf.NODE_BYTES_BINARY)
+    achieved, _ = rf.roofline(alg_bytes, kms[dom])
+    lb_gbs, lb_frac = rf.roofline(rf.lbvh_bytes(n), kms["build_lbvh"])
 
     out = {
         "metric": "batched Mrays/s + p50/p99 job latency vs plain per-job launch, equal RMSE",
@@ -193,6 +196,9 @@ def bench_bvh_stress(a, torch, lr):
         "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
         "build_mprims_s": {"lbvh": round(n / (kms["build_lbvh"] * 1e3), 2),
                            "ploc": round(n / (kms["build_ploc"] * 1e3), 2)},
+        "build_roofline": {"kernel": "build_lbvh (bounds, morton, 4 sort passes, lbvh)", "bound": "hbm",
+                           "alg_bytes": rf.lbvh_bytes(n), "achieved": round(lb_gbs, 2), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(lb_frac, 4)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "alg_bytes": alg_bytes, "node_fetches": cn, "tri_tests": ct, "rays": nr},
@@ -269,8 +275,10 @@ def bench_batched(a, torch, lr):
     img_bytes = sum(_align(w * h * 24) for (_, w, h, _) in plan)
     big = max(_align(w * h * 24) for (_, w, h, _) in plan)
     eng = lr.Engine(ctx, slots_log2=22, max_tree_depth=depth, image_bytes=img_bytes + big + (64 << 20))
-    sample = [lr.make_job(pool[i], lr.batch_camera(1000 + i, w, h), w, h, 1, seed=i, max_depth=8)
-              for i, (n, w, h, spp) in enumerate(plan[:16])]
+    # the autotuner's sample: the first 24 jobs at 4 spp (about 80M paths, 0.1 s a run), enough for
+    # the steady state of the wavefront to outweigh its start and tail, as in the whole batch
+    sample = [lr.make_job(pool[i], lr.batch_camera(1000 + i, w, h), w, h, 4, seed=i, max_depth=8)
+              for i, (n, w, h, spp) in enumerate(plan[:24])]
     t0 = time.perf_counter()
     if a.tune:
         f = [int(x) for x in a.tune.split(",")]
@@ -358,7 +366,7 @@ def bench_batched(a, torch, lr):
     nodes = c1.node_fetches - c0.node_fetches
     shapes = c1.shape_tests - c0.shape_tests
     rext = c1.rays_ext - c0.rays_ext
-    alg = 48 * rext + 64 * nodes + 52 * shapes   # the engine walks 64-byte (compressed 4-wide) nodes
+    alg = rf.walk_bytes(rext, nodes, shapes, rf.NODE_BYTES_WIDE)
     dom = max(kms, key=kms.get)
     achieved = alg / (kms["extend"] * 1e-3) / 1e9
 
@@ -527,7 +535,7 @@ def bench_instanced(a, torch, lr):
     c1 = eng.stats()
     eng.counting(False)
     nodes, shapes, rext = c1.node_fetches - c0.node_fetches, c1.shape_tests - c0.shape_tests, c1.rays_ext - c0.rays_ext
-    alg = 48 * rext + 64 * nodes + 52 * shapes   # 64-byte 4-wide nodes at both levels
+    alg = rf.walk_bytes(rext, nodes, shapes, rf.NODE_BYTES_WIDE)
     achieved = alg / (kms["extend"] * 1e-3) / 1e9
     ninst = len(w0["instances"])
     out = {

@@ -142,7 +142,7 @@ struct LrhEngine {
     unsigned long long issue_end = 0, iq_hi = 0;
     int running = 0;
     uint64_t jobs_done = 0;
-    LrhTune tune{128, 24, 128, 4, 1, 8};
+    LrhTune tune{256, 16, 256, 8, 1, 8};   // the best measured on the batched workload (MI355X)
     bool counting = false;
     // profiling: iterations launched one kernel at a time between events (no graph)
     bool profiling = false;

@@ -1,0 +1,41 @@
+"""Algorithmic bytes of the hot kernels: one formula per kernel, the bytes the algorithm must move
+between the compute units and memory, whatever the caches do. A kernel's roofline figure is these
+bytes over its measured time, against the HBM peak.
+
+Walks (k_trace, k_extend, k_shadow, k_scene_trace): per ray its 32-byte ray read and its 16-byte
+hit written, per node record fetched its size (32 bytes for the reference's binary node, 64 for
+the compressed 4-wide node), per shape tested its 48-byte triangle and 4-byte aux entry. The node
+and shape counts are the walks' own counters on the same tree and rays.
+
+LBVH build (k_bounds, k_morton, k_sort_pass x 4, k_lbvh), per primitive:
+  bounds    the box read, 32
+  morton    the box read, code and item written, 32 + 8
+  sort      four 8-bit passes, each (code, item) read and written, 4 x 16
+  lbvh      item 4 and box 32 read, leaf node 32 and aux 4 written, and per inner node (n - 1 of
+            them, counted per primitive) the sibling's record read and the node written, 32 + 32
+"""
+
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
+
+NODE_BYTES_BINARY = 32          # the reference's node (put_nodes)
+NODE_BYTES_WIDE = 64            # the compressed 4-wide node
+RAY_BYTES, HIT_BYTES, SHAPE_BYTES = 32, 16, 52
+
+LBVH_BYTES = {"bounds": 32, "morton": 40, "sort": 64, "lbvh": 4 + 32 + 32 + 4 + 64}
+
+
+def walk_bytes(rays, node_fetches, shape_tests, node_bytes):
+    """a walk kernel's algorithmic bytes"""
+    return (RAY_BYTES + HIT_BYTES) * rays + node_bytes * node_fetches + SHAPE_BYTES * shape_tests
+
+
+def lbvh_bytes(n, with_bounds=True):
+    """an LBVH build of n primitives (the bounds pass included unless the caller gives them)"""
+    per = sum(v for k, v in LBVH_BYTES.items() if with_bounds or k != "bounds")
+    return per * n
+
+
+def roofline(alg_bytes, ms):
+    """(achieved GB/s, fraction of the HBM peak) of alg_bytes moved in ms milliseconds"""
+    gbs = alg_bytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return gbs, gbs / HBM_PEAK_GBS
