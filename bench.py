@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--skip-plain", action="store_true", help="batched: leave out the plain per-job comparison")
     ap.add_argument("--tlas", choices=["lbvh", "ploc"], default="ploc", help="instanced: the per-frame top-level builder")
     ap.add_argument("--tris", type=int, default=1_000_000)
+    ap.add_argument("--build-sizes", default="4000000,16000000",
+                    help="bvh_stress: also time the builds alone at these sizes (the build kernels' roofline at scale)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -203,6 +205,34 @@ def bench_bvh_stress(a, torch, lr):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "alg_bytes": alg_bytes, "node_fetches": cn, "tri_tests": ct, "rays": nr},
     }
+
+    # ---- the builds alone at larger sizes: where launch and dependency latency stop dominating ----
+    scaling = []
+    for m in [int(x) for x in a.build_sizes.split(",") if x]:
+        t_m = torch.from_numpy(lr.gen_soup(m, 3)).to(dev)
+        b_m = torch.empty((m, 8), dtype=f32, device=dev)
+        nd_m = torch.empty((2 * m - 1, 8), dtype=i32, device=dev)
+        ax_m = torch.empty(m, dtype=i32, device=dev)
+        ctx.tri_boxes(t_m, b_m)
+        row = {"prims": m}
+        for name, bld in (("lbvh", lr.LBVH), ("ploc", lr.PLOC)):
+            ctx.build(b_m, m, nd_m, ax_m, bld, stats=False)   # warm-up
+            ts = []
+            for _ in range(5):
+                s_, e_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                s_.record(stream)
+                ctx.build(b_m, m, nd_m, ax_m, bld, stats=False)
+                e_.record(stream)
+                ts.append((s_, e_))
+            ctx.sync()
+            ms = float(np.mean([x.elapsed_time(y) for x, y in ts]))
+            row[name + "_ms"] = round(ms, 4)
+            row[name + "_mprims_s"] = round(m / (ms * 1e3), 1)
+        gbs, frac = rf.roofline(rf.lbvh_bytes(m), row["lbvh_ms"])
+        row["lbvh_roofline"] = {"achieved": round(gbs, 1), "unit": "GB/s", "frac": round(frac, 4)}
+        scaling.append(row)
+        del t_m, b_m, nd_m, ax_m
+    out["build_scaling"] = scaling
 
     # ---- the CPU baseline: the reference's own CPU path (SAH build, 4-wide walk) ----
     if not a.no_cpu:

@@ -14,6 +14,17 @@
 
 namespace {
 
+// Wait until this thread's memory operations are done. Device-scope atomics and stores are
+// coherent across the XCDs once done, so a hand-off needs this wait before its flag, not a
+// __threadfence (on gfx950 a write-back of the whole L2, here once per wave)
+__device__ __forceinline__ void wait_mem()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+
 constexpr int BLOCK = 256;
 constexpr int SORT_BLOCK = 1024;                // 16 waves per tile: enough in flight per CU
 constexpr int SORT_ITEMS = 4;
@@ -122,12 +133,11 @@ __global__ __launch_bounds__(BLOCK) void k_bounds(const float4 *__restrict__ box
     block_min6(v, s);
     if (threadIdx.x == 0) {
         for (int k = 0; k < 6; k++) st_dev((uint32_t *)&partial[6 * blockIdx.x + k], __float_as_uint(v[k]));
-        __threadfence();
+        wait_mem();   // the partials are out (device-scope stores) before the ticket
         last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
     float w[6] = {1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f};
     for (int b = threadIdx.x; b < (int)gridDim.x; b += BLOCK) {
         float t[6];
@@ -222,12 +232,11 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_morton(const float4 *__restrict_
         const uint32_t v = (&h[0][0])[i];
         if (v) atomicAdd(&mine[i], v);
     }
-    __threadfence();
+    wait_mem();   // this thread's histogram atomics are done before the ticket
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     __syncthreads();
     if (!last) return;
-    __threadfence();
     // the last block: thread t sums digit t % 256 of pass t / 256 over the copies (all the
     // copies' loads in flight together), clears them, and the passes are scanned side by side
     // (four waves per pass)
@@ -382,18 +391,40 @@ __device__ __forceinline__ void put_node_dev(LrNode *p, float lx, float ly, floa
     st_dev64(q + 3, ((uint64_t)b << 32) | __float_as_uint(hz));
 }
 
+// The same through a buffer resource: a record as two 16-byte stores (and loads), sc1 (device
+// scope, through to the coherence point) as st_dev64 / ld_dev64 but half the transactions. The
+// resource covers 2 GiB from the records' base (the host takes this path only below that).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int CPOL_SC1 = 16;   // the sc1 cache-policy bit of gfx950's memory instructions
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t node_rsrc(LrNode *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ void put_node_buf(__amdgpu_buffer_rsrc_t rs, int idx, float lx, float ly, float lz, uint32_t a,
+                                             float hx, float hy, float hz, uint32_t b)
+{
+    const u32x4 lo = {__float_as_uint(lx), __float_as_uint(ly), __float_as_uint(lz), a};
+    const u32x4 hi = {__float_as_uint(hx), __float_as_uint(hy), __float_as_uint(hz), b};
+    __builtin_amdgcn_raw_buffer_store_b128(lo, rs, idx * 32, 0, CPOL_SC1);
+    __builtin_amdgcn_raw_buffer_store_b128(hi, rs, idx * 32 + 16, 0, CPOL_SC1);
+}
+
 // One thread per leaf j: it writes its leaf (leaves, trace_build.wgsl:98-107), then climbs:
 // at each parent the first of the two children to arrive stops, the second joins their
 // boxes (up, :109-122: min / max of left, right), writes the parent with Karras's numbering
 // (internal i at node0 + i, leaf j at node0 + n - 1 + j) and goes on.
 // The hand-over between the two children's threads (often on different XCDs, whose L2s are
-// not coherent) uses no cache-wide fence: a child's record and height are stored through to
-// the coherence point (sc1), the stores are waited for (vmcnt 0), and only then does the
-// thread mark its arrival; the second thread reads them back with sc1 loads.
+// not coherent) uses no cache-wide fence: an inner child's record is stored through to the
+// coherence point (sc1), the stores are waited for (vmcnt 0), and only then does the thread
+// mark its arrival, with its subtree's far end and height in the one 64-bit word it swaps in;
+// the second thread reads the record back with sc1 loads. A leaf child's box is read from the
+// input boxes instead (written before the kernel), so leaves are plain stores.
+// BUF: the inner records through a buffer resource (16-byte device-scope accesses)
+template <bool BUF>
 __global__ __launch_bounds__(BLOCK) void k_lbvh(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ items,
                                                   const float4 *__restrict__ boxes, int n, uint32_t node0, uint32_t ref0,
-                                                  uint32_t ref_add, LrNode *nodes, uint32_t *aux, uint32_t *other,
-                                                  uint32_t *height, uint32_t *depth_out)
+                                                  uint32_t ref_add, LrNode *nodes, uint32_t *aux, unsigned long long *other,
+                                                  uint32_t *depth_out)
 {
     const int j = blockIdx.x * BLOCK + threadIdx.x;
     if (j >= n) return;
@@ -406,29 +437,41 @@ __global__ __launch_bounds__(BLOCK) void k_lbvh(const uint32_t *__restrict__ cod
         atomicMax(depth_out, 1u);
         return;
     }
-    put_node_dev(&nodes[n - 1 + j], lx, ly, lz, ref0 + (uint32_t)j, hx, hy, hz, 0x80000001u);
-    st_dev(&height[n - 1 + j], 1u);
+    put_node(&nodes[n - 1 + j], lx, ly, lz, ref0 + (uint32_t)j, hx, hy, hz, 0x80000001u);
+    const __amdgpu_buffer_rsrc_t rs = node_rsrc(nodes);
     int l = j, r = j;
     uint32_t h = 1;
     bool left = is_left(codes, n, l, r);
     for (;;) {
         const int p = left ? r : l - 1;             // the parent's split
         const int end = left ? l : r;
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);              // this subtree's record is out before the arrival
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        const uint32_t o = __hip_atomic_exchange(&other[p], (uint32_t)end + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        wait_mem();                                 // this subtree's record is out before the arrival
+        const unsigned long long mine = ((unsigned long long)h << 32) | ((uint32_t)end + 1u);
+        const unsigned long long o = __hip_atomic_exchange(&other[p], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (o == 0) return;                         // the sibling is not done: it goes on
-        const int L = left ? end : (int)o - 1, R = left ? (int)o - 1 : end;
+        const int oend = (int)(uint32_t)o - 1;
+        const uint32_t sh = (uint32_t)(o >> 32);
+        const int L = left ? end : oend, R = left ? oend : end;
         const int lc = L == p ? n - 1 + p : p, rc = p + 1 == R ? n + p : p + 1;
         const int sib = left ? rc : lc;
-        const uint64_t *sp = reinterpret_cast<const uint64_t *>(&nodes[sib]);
-        const uint64_t q0 = ld_dev64(sp), q1 = ld_dev64(sp + 1), q2 = ld_dev64(sp + 2), q3 = ld_dev64(sp + 3);
-        const float sx = __uint_as_float((uint32_t)q0), sy = __uint_as_float((uint32_t)(q0 >> 32));
-        const float sz = __uint_as_float((uint32_t)q1);
-        const float tx = __uint_as_float((uint32_t)q2), ty = __uint_as_float((uint32_t)(q2 >> 32));
-        const float tz = __uint_as_float((uint32_t)q3);
-        const uint32_t sh = ld_dev(&height[sib]);
+        float sx, sy, sz, tx, ty, tz;
+        if (sib >= n - 1) {   // a leaf: its box is the input's
+            const uint32_t it = items[sib - (n - 1)];
+            const float4 a = boxes[2 * (size_t)it], b = boxes[2 * (size_t)it + 1];
+            sx = a.x; sy = a.y; sz = a.z; tx = b.x; ty = b.y; tz = b.z;
+        } else if (BUF) {
+            const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, sib * 32, 0, CPOL_SC1);
+            const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, sib * 32 + 16, 0, CPOL_SC1);
+            sx = __uint_as_float(a.x); sy = __uint_as_float(a.y); sz = __uint_as_float(a.z);
+            tx = __uint_as_float(b.x); ty = __uint_as_float(b.y); tz = __uint_as_float(b.z);
+        } else {
+            const uint64_t *sp = reinterpret_cast<const uint64_t *>(&nodes[sib]);
+            const uint64_t q0 = ld_dev64(sp), q1 = ld_dev64(sp + 1), q2 = ld_dev64(sp + 2), q3 = ld_dev64(sp + 3);
+            sx = __uint_as_float((uint32_t)q0); sy = __uint_as_float((uint32_t)(q0 >> 32));
+            sz = __uint_as_float((uint32_t)q1);
+            tx = __uint_as_float((uint32_t)q2); ty = __uint_as_float((uint32_t)(q2 >> 32));
+            tz = __uint_as_float((uint32_t)q3);
+        }
         float nlx, nly, nlz, nhx, nhy, nhz;
         if (left) {
             nlx = wmin(lx, sx); nly = wmin(ly, sy); nlz = wmin(lz, sz);
@@ -448,8 +491,8 @@ __global__ __launch_bounds__(BLOCK) void k_lbvh(const uint32_t *__restrict__ cod
         }
         left = is_left(codes, n, l, r);
         const int me = left ? r : l;
-        put_node_dev(&nodes[me], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
-        st_dev(&height[me], h);
+        if (BUF) put_node_buf(rs, me, lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
+        else put_node_dev(&nodes[me], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
     }
 }
 
@@ -736,7 +779,7 @@ Layout layout(int n)
     L.ctr = take(64);
     L.depth = take(64);
     L.status = take(SORT_PASSES * tiles * 256 * 4);
-    L.other = take(N * 4);
+    L.other = take(N * 8);
     L.zero_words = (o - L.zero) / 4;
     L.height = take(8 * N);
     L.bx = take((2 * N) * 32);
@@ -830,9 +873,15 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
     int rounds = 0;
     const bool ploc = builder == LRH_PLOC && n > 2;
     if (!ploc) {
-        hipLaunchKernelGGL(k_lbvh, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, k0, v0,
-                           reinterpret_cast<const float4 *>(boxes), n, node0, ref0, ref_add, nd, ax, (uint32_t *)at(L.other),
-                           height, depth);
+        // the buffer path while the inner records lie within 2 GiB of the array's base
+        if ((2 * (size_t)n - 1) * 32 < 0x7fffffffu)
+            hipLaunchKernelGGL(k_lbvh<true>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, k0, v0,
+                               reinterpret_cast<const float4 *>(boxes), n, node0, ref0, ref_add, nd, ax,
+                               (unsigned long long *)at(L.other), depth);
+        else
+            hipLaunchKernelGGL(k_lbvh<false>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, k0, v0,
+                               reinterpret_cast<const float4 *>(boxes), n, node0, ref0, ref_add, nd, ax,
+                               (unsigned long long *)at(L.other), depth);
         LR_HIP(hipGetLastError());
     } else {
         float4 *bx = (float4 *)at(L.bx);

@@ -131,10 +131,10 @@ def test_build_offsets(ctx, builder):
     assert not ga[:ref0].any() and np.array_equal(ga[ref0:ref0 + n], aux)
 
 
+@pytest.mark.parametrize("n", [1_000_000, 4_000_000], ids=["1M", "4M"])
 @pytest.mark.parametrize("builder", [lr.LBVH, lr.PLOC], ids=["lbvh", "ploc"])
-def test_build_full_size_and_deterministic(ctx, builder):
-    n = 1_000_000
-    say(f"build n=1e6 builder={builder}")
+def test_build_full_size_and_deterministic(ctx, builder, n):
+    say(f"build n={n} builder={builder}")
     tris = soup(n, 2)
     a = gpu_build(ctx, tris, builder)
     b = gpu_build(ctx, tris, builder)
