@@ -41,6 +41,7 @@ constexpr int BOUNDS_BLOCKS = 512;
 
 constexpr uint32_t FLAG_AGG = 1u << 30;
 constexpr uint32_t FLAG_PREFIX = 2u << 30;
+constexpr int LOOK_BACK = 8;                     // predecessor tiles read at once in a sort pass's look-back
 constexpr uint32_t VALUE_MASK = (1u << 30) - 1u;
 
 // ---- block-wide scans (256 threads, wave64) ------------------------------------------------
@@ -323,13 +324,32 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_sort_pass(const uint32_t *__rest
     const uint32_t start = block_excl_scan<SORT_WAVES>(tot, &all, s_wsum);
     uint32_t excl = 0;
     if (dig && tile > 0) {
+        // LOOK_BACK tiles at a time (their loads in flight together): the nearest prefix among
+        // them ends the walk; a tile not yet posted means the same window again
         int t = (int)tile - 1;
         for (;;) {
-            const uint32_t s = ld_dev(&status[t * 256 + tid]);
-            if ((s & ~VALUE_MASK) == 0) { __builtin_amdgcn_s_sleep(1); continue; }
-            excl += s & VALUE_MASK;
-            if ((s & ~VALUE_MASK) == FLAG_PREFIX) break;
-            t--;
+            uint32_t sv[LOOK_BACK];
+#pragma unroll
+            for (int k = 0; k < LOOK_BACK; k++) sv[k] = t - k >= 0 ? ld_dev(&status[(t - k) * 256 + tid]) : FLAG_PREFIX;
+            bool wait = false, done = false;
+            uint32_t add = 0;
+#pragma unroll
+            for (int k = 0; k < LOOK_BACK; k++) {
+                if (done || wait) continue;
+                const uint32_t f = sv[k] & ~VALUE_MASK;
+                if (f == 0) wait = true;
+                else {
+                    add += sv[k] & VALUE_MASK;
+                    done = f == FLAG_PREFIX;
+                }
+            }
+            if (wait) {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += add;
+            if (done) break;
+            t -= LOOK_BACK;
         }
         st_dev(&status[tile * 256 + tid], FLAG_PREFIX | (excl + tot));
     }
@@ -636,28 +656,48 @@ __global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t
         uint32_t tk, tm;
         const uint32_t xk = block_excl_scan<BLOCK / 64>(keep, &tk, s_wsum);
         const uint32_t xm = block_excl_scan<BLOCK / 64>(merge, &tm, s_wsum);
-        if (tid == 0) {
+        if (tid < 64) {
+            // the look-back by the first wave, 64 predecessor tiles at once (lane l: tile t - l):
+            // the nearest inclusive prefix among them ends it; a needed tile not yet posted means
+            // the same window again
             const uint64_t mine = ((uint64_t)tk << 31) | tm;
             uint64_t ek = 0, em = 0;
-            if (tile == 0) st_dev64(&status[0], (2ull << 62) | mine);
-            else {
-                st_dev64(&status[tile], (1ull << 62) | mine);
+            if (tile == 0) {
+                if (tid == 0) st_dev64(&status[0], (2ull << 62) | mine);
+            } else {
+                if (tid == 0) st_dev64(&status[tile], (1ull << 62) | mine);
                 for (int t = (int)tile - 1;;) {
-                    const uint64_t s = ld_dev64(&status[t]);
-                    const uint64_t f = s >> 62;
-                    if (f == 0) { __builtin_amdgcn_s_sleep(1); continue; }
-                    ek += (s >> 31) & 0x7fffffffull;
-                    em += s & 0x7fffffffull;
-                    if (f == 2) break;
-                    t--;
+                    const int q = t - tid;
+                    const uint64_t sv = q >= 0 ? ld_dev64(&status[q]) : (2ull << 62);
+                    const uint64_t f = sv >> 62;
+                    const uint64_t inc = __ballot(f == 2), zero = __ballot(f == 0);
+                    const int stop = inc ? __ffsll((unsigned long long)inc) - 1 : 63;
+                    const uint64_t need = stop == 63 ? ~0ull : ((2ull << stop) - 1ull);
+                    if (zero & need) {
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    uint64_t ak = tid <= stop ? (sv >> 31) & 0x7fffffffull : 0ull;
+                    uint64_t am = tid <= stop ? sv & 0x7fffffffull : 0ull;
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) {
+                        ak += __shfl_xor(ak, o, 64);
+                        am += __shfl_xor(am, o, 64);
+                    }
+                    ek += ak;
+                    em += am;
+                    if (inc) break;
+                    t -= 64;
                 }
-                st_dev64(&status[tile], (2ull << 62) | ((ek + tk) << 31) | (em + tm));
+                if (tid == 0) st_dev64(&status[tile], (2ull << 62) | ((ek + tk) << 31) | (em + tm));
             }
-            s_ek = (uint32_t)ek;
-            s_em = (uint32_t)em;
-            if (tile == ntiles - 1) {              // the round's last tile: the next count
-                st_dev(&st->nc, (uint32_t)ek + tk);
-                st_dev(&st->parity, 1u - p);
+            if (tid == 0) {
+                s_ek = (uint32_t)ek;
+                s_em = (uint32_t)em;
+                if (tile == ntiles - 1) {              // the round's last tile: the next count
+                    st_dev(&st->nc, (uint32_t)ek + tk);
+                    st_dev(&st->parity, 1u - p);
+                }
             }
         }
         __syncthreads();
