@@ -39,11 +39,13 @@ def parse():
     ap.add_argument("--workload", choices=["batched", "bvh_stress", "instanced"], default="batched")
     ap.add_argument("--jobs", type=int, default=64, help="batched: jobs (and scenes) in a batch")
     ap.add_argument("--serve-requests", type=int, default=40, help="serving: requests per arrival rate (0: skip)")
-    ap.add_argument("--serve-rates", default="50,100,200,400", help="serving: Poisson arrival rates, jobs/s")
+    ap.add_argument("--serve-rates", default="10,25,50,100,200,400",
+                    help="serving: Poisson arrival rates, jobs/s (the workload's 50-400, and two below saturation)")
     ap.add_argument("--tune", default=None, metavar="BT,STK,BS,BPC,TILE,IPG",
                     help="batched: this launch shape instead of the autotuner's")
     ap.add_argument("--skip-plain", action="store_true", help="batched: leave out the plain per-job comparison")
     ap.add_argument("--tlas", choices=["lbvh", "ploc"], default="ploc", help="instanced: the per-frame top-level builder")
+    ap.add_argument("--frames-in-flight", type=int, default=2, help="instanced: frames rendering at once")
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--build-sizes", default="4000000,16000000",
                     help="bvh_stress: also time the builds alone at these sizes (the build kernels' roofline at scale)")
@@ -512,18 +514,26 @@ def bench_instanced(a, torch, lr):
     hulls) bouncing over a 40 x 40 ground, 3000 static props from 20 seeded blobs of 500-20k
     triangles, 14 GGX materials (two glass), a procedural 2048 x 1024 sky with a 5e4 sun. One
     step = one frame: new poses, the top-level BVH rebuilt on the device, a 1280 x 720, 1-spp,
-    8-bounce image. value = rays traced per second of frame time."""
+    8-bounce image. value = rays traced per second of frame time. --frames-in-flight F keeps F
+    frames rendering at once, each with its own copy of the scene (frame k's top level is built
+    while frame k - 1 renders): more rays per iteration, the frames' latency F times one frame's."""
     steps = a.steps or 10
     warm = 2 if a.warmup is None else a.warmup
     ctx = lr.Context(0)
     W, H = 1280, 720
     frames = [lr.instanced_workload(0, f) for f in range(warm + steps)]
     w0 = frames[0]
-    ps = lr.PtScene.instanced(ctx, w0["mtris"], w0["mesh_first"], w0["mats"], w0["instances"])
-    ps.set_sky(lr.sky(2048, 35.0, 60.0, sun_le=(5e4, 4.6e4, 4e4)))
+    F = max(1, a.frames_in_flight)
+    sky = lr.sky(2048, 35.0, 60.0, sun_le=(5e4, 4.6e4, 4e4))
+    scenes = []
+    for _ in range(F):
+        s_ = lr.PtScene.instanced(ctx, w0["mtris"], w0["mesh_first"], w0["mats"], w0["instances"])
+        s_.set_sky(sky)
+        scenes.append(s_)
+    ps = scenes[0]
     depth, nshapes, nbytes = ps.info()
     # the top level is rebuilt every frame: its depth moves a little, so the walks get room
-    eng = lr.Engine(ctx, slots_log2=22, max_tree_depth=depth + 32, image_bytes=_align(W * H * 24) + (16 << 20))
+    eng = lr.Engine(ctx, slots_log2=22, max_tree_depth=depth + 32, image_bytes=F * _align(W * H * 24) + (16 << 20))
     if a.tune:
         f = [int(x) for x in a.tune.split(",")]
         eng.tune = dict(zip(["block_trace", "stack", "block_shade", "blocks_per_cu", "tile", "iters_per_graph"], f))
@@ -531,29 +541,47 @@ def bench_instanced(a, torch, lr):
         eng.autotune([lr.make_job(ps, w0["cam"], W, H, 1, seed=99)])
     tune = eng.tune
 
-    def frame(k, keep=False):
-        st = ps.update(frames[k]["instances"], builder=lr.PLOC if a.tlas == "ploc" else lr.LBVH)
-        i = eng.submit(lr.make_job(ps, frames[k]["cam"], W, H, 1, seed=k, max_depth=8))
-        eng.drain()
-        if not keep:
-            eng.release(i)
-        return st, i
+    builder = lr.PLOC if a.tlas == "ploc" else lr.LBVH
+    in_flight = {}   # scene copy -> the job rendering it
+
+    def settle(si):
+        """wait for the frame rendering scene copy si, then let it go"""
+        i = in_flight.pop(si, None)
+        if i is None:
+            return
+        while eng.job(i).state != lr.JOB_DONE:
+            eng.pump(1)
+        eng.release(i)
+
+    def frame(k):
+        si = k % F
+        settle(si)
+        st = scenes[si].update(frames[k]["instances"], builder=builder)
+        in_flight[si] = eng.submit(lr.make_job(scenes[si], frames[k]["cam"], W, H, 1, seed=k, max_depth=8))
+        eng.pump(1)
+        return st
+
+    def finish():
+        for si in list(in_flight):
+            settle(si)
 
     for k in range(warm):
         frame(k)
+    finish()
     torch.cuda.synchronize()
     s0 = eng.stats()
     builds = []
     t0 = time.perf_counter()
     for k in range(warm, warm + steps):
-        st, _ = frame(k)
-        builds.append(st.ms)
+        builds.append(frame(k).ms)
+    finish()
     dt = time.perf_counter() - t0
     s1 = eng.stats()
     rays = (s1.rays_ext - s0.rays_ext) + (s1.rays_shadow - s0.rays_shadow)
     eng.profiling(True)
     k0 = eng.stats()
     frame(0)
+    finish()
     k1 = eng.stats()
     eng.profiling(False)
     names = ["regen", "extend", "shade", "shadow"]
@@ -562,6 +590,7 @@ def bench_instanced(a, torch, lr):
     eng.counting(True)
     c0 = eng.stats()
     frame(0)
+    finish()
     c1 = eng.stats()
     eng.counting(False)
     nodes, shapes, rext = c1.node_fetches - c0.node_fetches, c1.shape_tests - c0.shape_tests, c1.rays_ext - c0.rays_ext
@@ -584,7 +613,8 @@ def bench_instanced(a, torch, lr):
         "config": {"workload": "instanced: 2000 dynamic + 3000 static instances of 23 meshes over a 40x40 ground, "
                                "14 GGX materials, procedural 2048x1024 sky (5e4 sun), TLAS rebuilt per frame, "
                                "1280x720, 1 spp, 8 bounces", "instances": ninst, "mesh_triangles": int(nshapes),
-                   "scene_bytes": int(nbytes), "walk_depth": depth, "tune": tune.as_dict()},
+                   "scene_bytes": int(nbytes), "walk_depth": depth, "tune": tune.as_dict(),
+                   "frames_in_flight": F},
         "tlas_builder": a.tlas,
         "tlas_build_ms": round(float(np.mean(builds)), 4),
         "tlas_build_mprims_s": round(ninst / (float(np.mean(builds)) * 1e3), 2),
@@ -598,7 +628,8 @@ def bench_instanced(a, torch, lr):
                          "sample": "not run for this workload: the reference's instanced CPU path is not built here"},
     }
     eng.close()
-    ps.close()
+    for s_ in scenes:
+        s_.close()
     return out
 
 
