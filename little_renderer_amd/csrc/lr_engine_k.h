@@ -195,117 +195,10 @@ __device__ __forceinline__ V3 unit_box_normal(V3 p)
     return v3(0.0f, 0.0f, p.z < 0.0f ? -1.0f : 1.0f);
 }
 
-// The closest (ANY: the first) hit in a two-level scene: the top level's binary records
-// (children's boxes in the parent, as walk2), whose leaves hold instances; a mesh instance is entered (the
-// ray taken to its space, EXIT pushed under its mesh's root), a unit sphere or box is tested on
-// the spot. The hit's b1 carries the instance (as u32).
-template <bool ANY, bool COUNT, int STK>
-__device__ WalkHit walk_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax, const WalkStack &S, uint32_t *overflow,
-                           uint32_t &cn, uint32_t &ct)
-{
-    V3 ro = o, rd = d;
-    const V3 inv_w = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
-    V3 inv = inv_w;
-    uint32_t cur = LR_NONE;
-    WalkHit best{tmax, LR_NONE, __uint_as_float(LR_NONE), 0.0f};
-    int sp = 0;
-    auto push = [&](uint32_t x) -> bool {
-        if (sp + 1 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
-            atomicOr(overflow, 1u);
-            return false;
-        }
-        if (sp < STK) S.stk[sp * S.stride] = x;
-        else S.spill[(size_t)(sp - STK) * S.sstride] = x;
-        sp++;
-        return true;
-    };
-    auto pop = [&](uint32_t &x) -> bool {
-        if (sp == 0) return false;
-        sp--;
-        x = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
-        return true;
-    };
-    // the next entry stays in a register; only deferred ones go on the stack
-    uint32_t e = 0u;
-    for (;;) {
-        if (e == LR_WALK_EXIT) {
-            cur = LR_NONE;
-            ro = o;
-            rd = d;
-            inv = inv_w;
-            if (!pop(e)) break;
-            continue;
-        }
-        if ((e & 0xc0000000u) == LR_WALK_INST) {
-            const uint32_t i = e & 0x3fffffffu;
-            to_object(J.inst + 4 * (size_t)i, o, d, ro, rd);
-            inv = v3(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
-            cur = i;
-            if (!push(LR_WALK_EXIT)) break;
-            e = LR_WALK_BLAS | __float_as_uint(J.inst[4 * (size_t)i + 3].z);
-            continue;
-        }
-        const bool blas = (e & LR_WALK_BLAS) != 0;
-        const uint32_t k = e & 0x7fffffffu;
-        const float4 *rec = (blas ? J.mnodes : J.bnodes) + 4 * (size_t)k;
-        const float4 l0 = rec[0], l1 = rec[1], r0 = rec[2], r1 = rec[3];
-        if (COUNT) cn++;
-        const float tl = box_t(l0, l1, ro, inv, tmin, best.t);
-        const float tr = box_t(r0, r1, ro, inv, tmin, best.t);
-        const uint32_t ch[2] = {__float_as_uint(l0.w), __float_as_uint(r0.w)};
-        const uint32_t cb[2] = {__float_as_uint(l1.w), __float_as_uint(r1.w)};
-        const float ct2[2] = {tl, tr};
-        const int first = tl <= tr ? 0 : 1;
-        bool inner[2] = {false, false};
-        for (int q = 0; q < 2; q++) {
-            const int c = q == 0 ? first : 1 - first;
-            if (!(ct2[c] >= 0.0f)) continue;
-            if (!(cb[c] & LR_LEAF)) {
-                inner[c] = true;
-                continue;
-            }
-            const uint32_t cnt = cb[c] & 0x7fffffffu;
-            for (uint32_t j = 0; j < cnt; j++) {
-                if (COUNT) ct++;
-                if (blas) {
-                    const uint32_t sid = J.maux[ch[c] + j];
-                    const float4 ta = J.tris[3 * (size_t)sid], tb = J.tris[3 * (size_t)sid + 1], tc = J.tris[3 * (size_t)sid + 2];
-                    float t, u, v;
-                    if (hit_tri(ta, tb, tc, ro, rd, tmin, best.t, t, u, v)) {
-                        best = WalkHit{t, sid, __uint_as_float(cur), 0.0f};
-                        if (ANY) return best;
-                    }
-                    continue;
-                }
-                const uint32_t i = J.aux[ch[c] + j];
-                const uint32_t kind = __float_as_uint(J.inst[4 * (size_t)i + 3].x);
-                if (kind >= LR_UNIT_SPHERE) {
-                    V3 oo, od;
-                    to_object(J.inst + 4 * (size_t)i, o, d, oo, od);
-                    const float t = kind == LR_UNIT_SPHERE ? hit_unit_sphere(oo, od, tmin, best.t) : hit_unit_box(oo, od, tmin, best.t);
-                    if (t > 0.0f) {
-                        best = WalkHit{t, kind, __uint_as_float(i), 0.0f};
-                        if (ANY) return best;
-                    }
-                    continue;
-                }
-                if (!push(LR_WALK_INST | i)) return best;
-            }
-        }
-        const bool gl = inner[0] && tl <= best.t, gr = inner[1] && tr <= best.t;
-        const uint32_t tag = blas ? LR_WALK_BLAS : 0u;
-        if (gl && gr) {
-            if (!push(tag | ch[1 - first])) break;
-            e = tag | ch[first];
-        } else if (gl) e = tag | ch[0];
-        else if (gr) e = tag | ch[1];
-        else if (!pop(e)) break;
-    }
-    return best;
-}
-
-// The same on the 4-wide trees (walk4's compressed nodes, one entry at a time): the top level's
-// wide nodes, whose leaf references hold instances, and each mesh's. A node's children hit are
+// The closest (ANY: the first) hit in a two-level scene, on the 4-wide trees (walk4's compressed
+// nodes, one entry at a time): the top level's wide nodes, whose leaf references hold
+// instances, and each mesh's. A mesh instance is entered (the ray taken to its space, EXIT
+// pushed under its mesh's root); the hit's b1 carries the instance (as u32). A node's children hit are
 // sorted by their boxes' entry distance, whatever they are (a node, a mesh's leaf, an instance
 // to enter), so all are visited near to far; unit spheres and boxes are tested on the spot. A
 // mesh's triangles are read in leaf order (ltris) beside their aux entries.

@@ -222,12 +222,7 @@ __device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32
     return best;
 }
 
-// ---- the engine's walk: both children's boxes in the parent --------------------------------
-// A 64-byte record per inner node: the left child's (lo, a) (hi, b), then the right child's,
-// where an inner child's a is its node index (b = 0) and a leaf child's (a, b) is its leaf
-// record (first aux entry, LR_LEAF | count). Made from the reference's 32-byte nodes by
-// k_child_boxes (lr_pt_scene.hip); the same tree, so the same hits, with one dependent load per
-// level instead of two (the node, then its children).
+// ---- the engine's walks: leaves -------------------------------------------------------------
 // ltris: the triangles in leaf order (aux entry e's triangle at ltris[3 e..]), so that the
 // triangle's load does not wait for the aux entry's.
 __device__ __forceinline__ void leaf_hits_(const uint32_t *__restrict__ aux, const float4 *__restrict__ ltris,
@@ -263,64 +258,6 @@ __device__ __forceinline__ void leaf_hits_(const uint32_t *__restrict__ aux, con
     }
 }
 
-template <bool ANY, bool COUNT, int STK>
-__device__ __forceinline__ WalkHit walk2(const float4 *__restrict__ wnodes, const uint32_t *__restrict__ aux,
-                                         const float4 *__restrict__ ltris, const float4 *__restrict__ prims,
-                                         uint32_t nprims, V3 o, V3 d, float tmin, float tmax, const WalkStack &S,
-                                         uint32_t *overflow, uint32_t &cn, uint32_t &ct)
-{
-    const V3 inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
-    WalkHit best{tmax, LR_NONE, 0.0f, 0.0f};
-    int sp = 0;
-    auto push = [&](uint32_t x) {
-        if (sp < STK) S.stk[sp * S.stride] = x;
-        else S.spill[(size_t)(sp - STK) * S.sstride] = x;
-        sp++;
-    };
-    bool done = false;
-    // the next node stays in a register; only the farther child of two goes on the stack
-    uint32_t ni = 0u;
-    for (;;) {
-        const float4 l0 = wnodes[4 * (size_t)ni], l1 = wnodes[4 * (size_t)ni + 1];
-        const float4 r0 = wnodes[4 * (size_t)ni + 2], r1 = wnodes[4 * (size_t)ni + 3];
-        if (COUNT) cn++;
-        const float tl = box_t(l0, l1, o, inv, tmin, best.t);
-        const float tr = box_t(r0, r1, o, inv, tmin, best.t);
-        const uint32_t la = __float_as_uint(l0.w), lb = __float_as_uint(l1.w);
-        const uint32_t ra = __float_as_uint(r0.w), rb = __float_as_uint(r1.w);
-        const bool lleaf = (lb & LR_LEAF) != 0, rleaf = (rb & LR_LEAF) != 0;
-        // leaves at once, the nearer first (their hits shorten best.t for what follows)
-        const bool l_first = tl <= tr;
-        if (tl >= 0.0f && lleaf && l_first) {
-            leaf_hits_(aux, ltris, prims, nprims, la, lb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
-            if (ANY && done) return best;
-        }
-        if (tr >= 0.0f && rleaf) {
-            leaf_hits_(aux, ltris, prims, nprims, ra, rb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
-            if (ANY && done) return best;
-        }
-        if (tl >= 0.0f && lleaf && !l_first) {
-            leaf_hits_(aux, ltris, prims, nprims, la, lb & 0x7fffffffu, o, d, tmin, best, ANY, done, ct, COUNT);
-            if (ANY && done) return best;
-        }
-        const bool gl = tl >= 0.0f && !lleaf && tl <= best.t, gr = tr >= 0.0f && !rleaf && tr <= best.t;
-        if (gl && gr) {
-            if (sp + 1 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
-                atomicOr(overflow, 1u);
-                break;
-            }
-            push(l_first ? ra : la);
-            ni = l_first ? la : ra;
-        } else if (gl) ni = la;
-        else if (gr) ni = ra;
-        else {
-            if (sp == 0) break;
-            sp--;
-            ni = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
-        }
-    }
-    return best;
-}
 
 // ---- the engine's 4-wide walk ---------------------------------------------------------------
 // A wide node is 64 bytes (made by k_collapse, lr_pt_scene.hip, from the binary tree with the
