@@ -28,7 +28,7 @@
 // host's copy is never read late)
 struct JobDev {
     const float4 *nodes;      // 4-wide nodes (walk4); a two-level scene: its top level, 4-wide (walk4_tl)
-    const float4 *bnodes;     // the same tree's binary records, children's boxes in the parent (walk2, walk_tl)
+    const float4 *bnodes;     // a two-level scene's top level as binary records (k_child_boxes; not walked now)
     const uint32_t *aux;
     const float4 *tris;
     const float4 *ltris;      // the triangles again, in leaf order: ltris[3 e..] for aux entry e (the walks)
@@ -145,7 +145,7 @@ struct LrhPtScene {
     LrhContext *ctx;
     float4 *prims = nullptr, *tris = nullptr, *mats = nullptr, *boxes = nullptr;
     uint32_t *nodes = nullptr, *aux = nullptr;
-    float4 *wnodes = nullptr;                 // the same tree, children's boxes in the parent (walk2)
+    float4 *wnodes = nullptr;                 // a two-level scene's top level as binary records (k_child_boxes)
     float4 *w4nodes = nullptr;                // the same tree, 4-wide (walk4)
     float4 *ltris = nullptr;                  // the triangles in leaf order (zeros at primitives' entries)
     // the sky

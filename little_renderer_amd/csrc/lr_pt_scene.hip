@@ -8,9 +8,10 @@
 #include "lr_device.h"
 #include "lr_engine_k.h"
 
-// The engine's node records (walk2, lr_walk.h): for each inner node of the reference layout,
-// both children's boxes and (index | leaf record). A one-shape tree (its root a leaf) gets a
-// record whose left child is that leaf and whose right child is an empty leaf.
+// Binary records with both children's boxes in the parent: for each inner node of the reference
+// layout, both children's boxes and (index | leaf record); a one-shape tree (its root a leaf)
+// gets a record whose left child is that leaf and whose right child is an empty leaf. Made for
+// the two-level scenes' top level and meshes (JobDev bnodes, mnodes); no walk reads them now.
 __global__ __launch_bounds__(256) void k_child_boxes(const float4 *__restrict__ nodes, int nn, float4 *__restrict__ w)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -342,16 +343,6 @@ extern "C" int lrh_pt_scene(LrhContext *c, const float *prims, int nprims, const
         return rc;
     }
     s->depth = st.depth;
-    const size_t nn = 2 * (size_t)n - 1;
-    if (hipMalloc((void **)&s->wnodes, nn * 64) != hipSuccess) {
-        (void)hipGetLastError();
-        pt_scene_release(s);
-        return lr_fail(LRH_ENOMEM, "lrh_pt_scene: device memory for the scene could not be had");
-    }
-    hipLaunchKernelGGL(k_child_boxes, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream,
-                       (const float4 *)s->nodes, (int)nn, s->wnodes);
-    if (n == 1)   // the root is the one shape's leaf
-        hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, c->stream, (const float4 *)s->nodes, 0u, s->wnodes);
     // the 4-wide tree the jobs walk; its stack: up to 3 entries per wide level
     if (hipMalloc((void **)&s->w4nodes, (size_t)n * 64) != hipSuccess) {
         (void)hipGetLastError();
@@ -374,8 +365,7 @@ extern "C" int lrh_pt_scene(LrhContext *c, const float *prims, int nprims, const
                        (const float4 *)s->tris, s->ltris);
     s->depth = 2 * st.depth + 4;
     s->bytes = (size_t)nprims * 96 + (size_t)ntris * 48 + hm.size() * 4 + boxes.size() * 4 + (2 * (size_t)n - 1) * 32 +
-               (size_t)n * 4 + tab.size() * 4 + mask.size() * 2 + (2 * (size_t)n - 1) * 64 + (size_t)n * 64 +
-               (size_t)n * 48;
+               (size_t)n * 4 + tab.size() * 4 + mask.size() * 2 + (size_t)n * 64 + (size_t)n * 48;
     // the tree is built on the context's stream; the engines that render the scene run on
     // streams of their own, so the scene is complete before it is handed out
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
