@@ -87,9 +87,10 @@ int lrh_trace_queue(LrhContext *c, const uint32_t *d_nodes, uint32_t root, const
                     const float *d_rays, int max_n, const uint32_t *d_count, int any, float *d_hits, int max_depth);
 
 /* lrh_trace (d_count NULL) or lrh_trace_queue, also adding to d_counts[0] the node records
- * the walk fetched (each popped node, and an inner node's two children: 32 bytes each) and to
- * d_counts[1] the triangles it tested (48 bytes + a 4-byte aux entry each): the algorithmic
- * traffic the roofline is computed from. Slower (atomics): for measurement, not timing. */
+ * the walk fetched (the root's, an inner node's two children's, and each node popped from
+ * the stack, whose two child words it reads again: 32 bytes each) and to d_counts[1] the
+ * triangles it tested (48 bytes + a 4-byte aux entry each): the algorithmic traffic the
+ * roofline is computed from. Slower (atomics): for measurement, not timing. */
 int lrh_trace_count(LrhContext *c, const uint32_t *d_nodes, uint32_t root, const uint32_t *d_aux, const float *d_tris,
                     const float *d_rays, int max_n, const uint32_t *d_count, int any, float *d_hits, int max_depth,
                     uint64_t *d_counts);

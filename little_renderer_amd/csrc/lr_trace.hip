@@ -8,8 +8,8 @@ namespace {
 
 constexpr int TBLOCK = 128;
 
-// COUNT: also sums the node records fetched (the popped node, and an inner node's two
-// children) and the triangles tested, into counts[0], counts[1] (the algorithmic traffic)
+// COUNT: also sums the node records fetched (the root's, an inner node's two children's, a
+// popped node's) and the triangles tested, into counts[0], counts[1] (the algorithmic traffic)
 template <bool ANY, bool COUNT>
 __global__ __launch_bounds__(TBLOCK) void k_trace(const float4 *__restrict__ nodes, uint32_t root, const uint32_t *__restrict__ aux,
                                                     const float4 *__restrict__ tris, const float4 *__restrict__ rays, int n,

@@ -1,5 +1,5 @@
 // lr_walk.h - the BVH walk and the shape tests, shared by the trace kernels (lr_trace.hip)
-// and the path tracer's extension and shadow kernels (lr_pt.hip).
+// and the path tracer's extension and shadow kernels (lr_engine_k.h).
 //
 // The walk and its arithmetic are trace.wgsl traverse (:270-392), hit_tri (:159-173),
 // hit_sphere / hit_quad (:130-157) and box_t (:175-184) with safe_inv (media.wgsl:45-48):
@@ -7,12 +7,14 @@
 // multiply-add), so hits equal the oracle's to the bit on the same tree. The stack differs
 // from the reference's 40 entries that drop what does not fit: the first LR_LDS_STACK
 // entries of each ray live in LDS (interleaved by lane: no bank conflicts) and deeper ones
-// spill to device memory, sized from the tree's depth, so nothing is ever dropped.
+// spill to device memory, sized from the tree's depth, so nothing is ever dropped. Sixteen
+// LDS entries (8 KB for a work-group of 128) leave the walk kernels at their register-bound
+// occupancy; the binary walk pushes only the farther child, so deep entries are rare.
 #pragma once
 
 #include "lr_device.h"
 
-#define LR_LDS_STACK 24
+#define LR_LDS_STACK 16
 
 struct V3 {
     float x, y, z;
@@ -156,7 +158,15 @@ struct WalkStack {
 
 // The closest hit in (tmin, tmax), or (ANY) the first one, over shape ids taken from the
 // leaves' aux entries: ids below nprims are primitives, the rest triangles (id - nprims).
-// COUNT adds the node records fetched and the shapes tested to cn, ct.
+//
+// The nodes are visited in the reference's order (pop a node; test its two children's boxes;
+// push the farther hit child, then the nearer, and pop again), but the nearer child is not
+// pushed and popped: it stays in registers as the next entry, with its two child fields
+// (lo.w, hi.w), which its parent's box test has already loaded. Only a popped (farther) child
+// fetches those two words of its record; the reference fetches the whole record of every node
+// it pops. Same visits, same tests, same hits: one dependent load fewer on each step down.
+// COUNT adds the node records fetched (the root's, each inner node's two children, each
+// popped node's) and the shapes tested to cn, ct.
 template <bool ANY, bool PRIMS, bool COUNT, int STK = LR_LDS_STACK>
 __device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32_t root, const uint32_t *__restrict__ aux,
                                         const float4 *__restrict__ tris, const float4 *__restrict__ prims, uint32_t nprims,
@@ -165,20 +175,14 @@ __device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32
 {
     const V3 inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
     WalkHit best{tmax, LR_NONE, 0.0f, 0.0f};
+    const float *words = reinterpret_cast<const float *>(nodes);
+    // the entry being visited: its record's lo.w and hi.w (inner: the children; leaf: the
+    // first aux entry, then LR_LEAF | count)
+    uint32_t na = __float_as_uint(words[8 * (size_t)root + 3]), nb = __float_as_uint(words[8 * (size_t)root + 7]);
+    if (COUNT) cn++;
     int sp = 0;
-    auto push = [&](uint32_t x) {
-        if (sp < STK) S.stk[sp * S.stride] = x;
-        else S.spill[(size_t)(sp - STK) * S.sstride] = x;
-        sp++;
-    };
-    push(root);
-    while (sp > 0) {
-        sp--;
-        const uint32_t ni = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
-        const float4 nlo = nodes[2 * (size_t)ni], nhi = nodes[2 * (size_t)ni + 1];
-        const uint32_t na = __float_as_uint(nlo.w), nb = __float_as_uint(nhi.w);
-        if (COUNT) cn++;
-        if (nb & LR_LEAF) {
+    for (;;) {
+        while (nb & LR_LEAF) {
             const uint32_t cnt = nb & 0x7fffffffu;
             for (uint32_t k = 0; k < cnt; k++) {
                 const uint32_t sid = aux[na + k];
@@ -201,26 +205,48 @@ __device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32
                     if (ANY) return best;
                 }
             }
-            continue;
+            if (sp == 0) return best;
+            sp--;
+            const uint32_t ni = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+            na = __float_as_uint(words[8 * (size_t)ni + 3]);
+            nb = __float_as_uint(words[8 * (size_t)ni + 7]);
+            if (COUNT) cn++;
         }
         const float4 alo = nodes[2 * (size_t)na], ahi = nodes[2 * (size_t)na + 1];
         const float4 blo = nodes[2 * (size_t)nb], bhi = nodes[2 * (size_t)nb + 1];
         const float tl = box_t(alo, ahi, o, inv, tmin, best.t);
         const float tr = box_t(blo, bhi, o, inv, tmin, best.t);
         if (COUNT) cn += 2;
-        if (sp + 2 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
-            atomicOr(overflow, 1u);
-            break;
-        }
         if (tl >= 0.0f && tr >= 0.0f) {
+            if (sp + 1 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
+                atomicOr(overflow, 1u);
+                break;
+            }
             const bool left_first = tl <= tr;
-            push(left_first ? nb : na);
-            push(left_first ? na : nb);
-        } else if (tl >= 0.0f) push(na);
-        else if (tr >= 0.0f) push(nb);
+            const uint32_t far = left_first ? nb : na;
+            if (sp < STK) S.stk[sp * S.stride] = far;
+            else S.spill[(size_t)(sp - STK) * S.sstride] = far;
+            sp++;
+            na = __float_as_uint(left_first ? alo.w : blo.w);
+            nb = __float_as_uint(left_first ? ahi.w : bhi.w);
+        } else if (tl >= 0.0f) {
+            na = __float_as_uint(alo.w);
+            nb = __float_as_uint(ahi.w);
+        } else if (tr >= 0.0f) {
+            na = __float_as_uint(blo.w);
+            nb = __float_as_uint(bhi.w);
+        } else {
+            if (sp == 0) break;
+            sp--;
+            const uint32_t ni = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+            na = __float_as_uint(words[8 * (size_t)ni + 3]);
+            nb = __float_as_uint(words[8 * (size_t)ni + 7]);
+            if (COUNT) cn++;
+        }
     }
     return best;
 }
+
 
 // ---- the engine's walks: leaves -------------------------------------------------------------
 // ltris: the triangles in leaf order (aux entry e's triangle at ltris[3 e..]), so that the
