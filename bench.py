@@ -200,7 +200,7 @@ def bench_bvh_stress(a, torch, lr):
         "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
         "build_mprims_s": {"lbvh": round(n / (kms["build_lbvh"] * 1e3), 2),
                            "ploc": round(n / (kms["build_ploc"] * 1e3), 2)},
-        "build_roofline": {"kernel": "build_lbvh (bounds, morton, 4 sort passes, lbvh)", "bound": "hbm",
+        "build_roofline": {"kernel": "build_lbvh (bounds, morton, 4 sort passes, lbvh, lbvh_top)", "bound": "hbm",
                            "alg_bytes": rf.lbvh_bytes(n), "achieved": round(lb_gbs, 2), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(lb_frac, 4)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

@@ -7,6 +7,7 @@
 // passes, its 54 refit passes one bottom-up pass with per-node arrival counters, and each
 // PLOC round (4 dispatches there) one kernel whose cluster windows sit in LDS.
 #include <stdarg.h>
+#include <algorithm>
 #include <stdio.h>
 
 #include "lr_device.h"
@@ -429,90 +430,219 @@ __device__ __forceinline__ void put_node_buf(__amdgpu_buffer_rsrc_t rs, int idx,
     __builtin_amdgcn_raw_buffer_store_b128(hi, rs, idx * 32 + 16, 0, CPOL_SC1);
 }
 
-// One thread per leaf j: it writes its leaf (leaves, trace_build.wgsl:98-107), then climbs:
-// at each parent the first of the two children to arrive stops, the second joins their
-// boxes (up, :109-122: min / max of left, right), writes the parent with Karras's numbering
-// (internal i at node0 + i, leaf j at node0 + n - 1 + j) and goes on.
-// The hand-over between the two children's threads (often on different XCDs, whose L2s are
-// not coherent) uses no cache-wide fence: an inner child's record is stored through to the
-// coherence point (sc1), the stores are waited for (vmcnt 0), and only then does the thread
-// mark its arrival, with its subtree's far end and height in the one 64-bit word it swaps in;
-// the second thread reads the record back with sc1 loads. A leaf child's box is read from the
-// input boxes instead (written before the kernel), so leaves are plain stores.
-// BUF: the inner records through a buffer resource (16-byte device-scope accesses)
-template <bool BUF>
-__global__ __launch_bounds__(BLOCK) void k_lbvh(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ items,
-                                                  const float4 *__restrict__ boxes, int n, uint32_t node0, uint32_t ref0,
-                                                  uint32_t ref_add, LrNode *nodes, uint32_t *aux, unsigned long long *other,
-                                                  uint32_t *depth_out)
+// min-scans over a block of 256 (4 waves): forward (the minimum of this and every earlier
+// thread's value) and backward (this and every later thread's)
+__device__ __forceinline__ int block_min_scan(int v, bool backward, int *s_w)
 {
-    const int j = blockIdx.x * BLOCK + threadIdx.x;
-    if (j >= n) return;
-    const uint32_t item = items[j];
-    const float4 blo = boxes[2 * (size_t)item], bhi = boxes[2 * (size_t)item + 1];
-    float lx = blo.x, ly = blo.y, lz = blo.z, hx = bhi.x, hy = bhi.y, hz = bhi.z;
-    aux[j] = item + ref_add;
-    if (n == 1) {
-        put_node(&nodes[0], lx, ly, lz, ref0, hx, hy, hz, 0x80000001u);
-        atomicMax(depth_out, 1u);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = backward ? __shfl_down(v, o, 64) : __shfl_up(v, o, 64);
+        if (backward ? lane + o < 64 : lane >= o) v = t < v ? t : v;
+    }
+    if (lane == (backward ? 0 : 63)) s_w[w] = v;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BLOCK / 64; k++) {
+        const int t = s_w[k];
+        if (backward ? k > w : k < w) v = t < v ? t : v;
+    }
+    __syncthreads();
+    return v;
+}
+
+// The LBVH's nodes bottom-up (leaves, trace_build.wgsl:98-107; up, :109-122): from each
+// leaf a thread climbs; at each parent the first of the two children to arrive stops, the
+// second joins their boxes (min / max of left, right), writes the parent with Karras's
+// numbering (internal i at node0 + i, leaf j at node0 + n - 1 + j) and goes on.
+//
+// Two kernels. k_lbvh: a work-group's 256 leaves climb through the nodes whose key ranges lie
+// inside the group's: the two children's threads are then both in the group, and hand their
+// boxes over in LDS (a slot per side, an LDS arrival counter). Whether a node's parent lies
+// inside is decided from the adjacent-key deltas d(k) alone: a left child [l, p]'s parent ends
+// at the first k > p with d(k) < d(p), a right child [p + 1, r]'s starts after the last k < p
+// with d(k) < d(p); a min-scan of the group's deltas each way answers both with one
+// comparison. A node whose parent lies outside (the root of a largest subtree inside the
+// group: at most 256 of them, about 8 on random keys) is written and listed in the group's
+// slots. k_lbvh_top then climbs from the listed nodes, one wave per group's list, through
+// device memory: the two children's threads are on any XCDs, whose L2s are not coherent, so a
+// record is stored through to the coherence point (sc1), the stores are waited for (vmcnt 0),
+// and only then does the thread mark its arrival, with its subtree's far end and height in the
+// one 64-bit word it swaps in; the second thread reads the record back with sc1 loads (a leaf
+// child's box from the input boxes). Kept apart, the few long climbs through device memory
+// do not hold the groups of the first kernel, and their LDS, on the compute units.
+constexpr int LBVH_SLOTS = BLOCK;   // listed nodes per group, at most
+
+__global__ __launch_bounds__(BLOCK) void k_lbvh(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ items,
+                                                const float4 *__restrict__ boxes, int n, uint32_t node0, uint32_t ref0,
+                                                uint32_t ref_add, LrNode *__restrict__ nodes, uint32_t *__restrict__ aux,
+                                                uint4 *__restrict__ esc, uint32_t *__restrict__ esc_count,
+                                                uint32_t *depth_out)
+{
+    __shared__ int s_d[BLOCK + 1];        // d(B0 - 1 + i); -1 (below every delta) off the keys' ends
+    __shared__ int s_pmin[BLOCK + 1];     // min of s_d[0 .. i]
+    __shared__ int s_smin[BLOCK + 1];     // min of s_d[i .. BLOCK] (i >= 1)
+    __shared__ float s_box[BLOCK][2][6];  // per parent split, per side: the arriving child's box
+    __shared__ uint32_t s_end[BLOCK][2], s_h[BLOCK][2], s_arrive[BLOCK];
+    __shared__ int s_w[BLOCK / 64];
+    __shared__ uint32_t s_nesc;
+    const int t = threadIdx.x;
+    const int B0 = blockIdx.x * BLOCK;
+    const int j = B0 + t;
+    const bool live = j < n;
+    float lx = 0.0f, ly = 0.0f, lz = 0.0f, hx = 0.0f, hy = 0.0f, hz = 0.0f;
+    if (live) {
+        const uint32_t item = items[j];
+        const float4 blo = boxes[2 * (size_t)item], bhi = boxes[2 * (size_t)item + 1];
+        lx = blo.x; ly = blo.y; lz = blo.z; hx = bhi.x; hy = bhi.y; hz = bhi.z;
+        aux[j] = item + ref_add;
+        if (n > 1) put_node(&nodes[n - 1 + j], lx, ly, lz, ref0 + (uint32_t)j, hx, hy, hz, 0x80000001u);
+        else {
+            put_node(&nodes[0], lx, ly, lz, ref0, hx, hy, hz, 0x80000001u);
+            atomicMax(depth_out, 1u);
+        }
+    }
+    if (n == 1) {   // (uniform: the one group returns before any barrier)
+        if (t == 0) esc_count[0] = 0u;
         return;
     }
-    put_node(&nodes[n - 1 + j], lx, ly, lz, ref0 + (uint32_t)j, hx, hy, hz, 0x80000001u);
-    const __amdgpu_buffer_rsrc_t rs = node_rsrc(nodes);
-    int l = j, r = j;
+    const int dj = j < n - 1 ? dadj(codes, j) : -1;
+    s_d[t + 1] = dj;
+    if (t == 0) {
+        s_d[0] = B0 > 0 ? dadj(codes, B0 - 1) : -1;
+        s_nesc = 0u;
+    }
+    s_arrive[t] = 0u;
+    __syncthreads();
+    const int d0 = s_d[0];
+    const int pm = block_min_scan(dj, false, s_w);
+    const int sm = block_min_scan(dj, true, s_w);
+    s_pmin[t + 1] = pm < d0 ? pm : d0;
+    s_smin[t + 1] = sm;
+    if (t == 0) s_pmin[0] = d0;
+    __syncthreads();
+
+    // d(k) = s_d[k - B0 + 1] for B0 - 1 <= k <= B0 + BLOCK - 1
+    int l = j, r = j;      // the node reached: keys [l, r] (a leaf while l == r)
+    int lc = 0, rc = 0;    // an inner node's children
     uint32_t h = 1;
-    bool left = is_left(codes, n, l, r);
-    for (;;) {
-        const int p = left ? r : l - 1;             // the parent's split
-        const int end = left ? l : r;
-        wait_mem();                                 // this subtree's record is out before the arrival
-        const unsigned long long mine = ((unsigned long long)h << 32) | ((uint32_t)end + 1u);
-        const unsigned long long o = __hip_atomic_exchange(&other[p], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (o == 0) return;                         // the sibling is not done: it goes on
-        const int oend = (int)(uint32_t)o - 1;
-        const uint32_t sh = (uint32_t)(o >> 32);
-        const int L = left ? end : oend, R = left ? oend : end;
-        const int lc = L == p ? n - 1 + p : p, rc = p + 1 == R ? n + p : p + 1;
-        const int sib = left ? rc : lc;
-        float sx, sy, sz, tx, ty, tz;
-        if (sib >= n - 1) {   // a leaf: its box is the input's
-            const uint32_t it = items[sib - (n - 1)];
-            const float4 a = boxes[2 * (size_t)it], b = boxes[2 * (size_t)it + 1];
-            sx = a.x; sy = a.y; sz = a.z; tx = b.x; ty = b.y; tz = b.z;
-        } else if (BUF) {
-            const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, sib * 32, 0, CPOL_SC1);
-            const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, sib * 32 + 16, 0, CPOL_SC1);
-            sx = __uint_as_float(a.x); sy = __uint_as_float(a.y); sz = __uint_as_float(a.z);
-            tx = __uint_as_float(b.x); ty = __uint_as_float(b.y); tz = __uint_as_float(b.z);
-        } else {
-            const uint64_t *sp = reinterpret_cast<const uint64_t *>(&nodes[sib]);
-            const uint64_t q0 = ld_dev64(sp), q1 = ld_dev64(sp + 1), q2 = ld_dev64(sp + 2), q3 = ld_dev64(sp + 3);
-            sx = __uint_as_float((uint32_t)q0); sy = __uint_as_float((uint32_t)(q0 >> 32));
-            sz = __uint_as_float((uint32_t)q1);
-            tx = __uint_as_float((uint32_t)q2); ty = __uint_as_float((uint32_t)(q2 >> 32));
-            tz = __uint_as_float((uint32_t)q3);
+    bool climbing = live;
+    bool left = live && s_d[r - B0 + 1] > s_d[l - B0];
+    while (climbing) {
+        const int p = left ? r : l - 1;
+        const int dp = s_d[p - B0 + 1];
+        const bool inside = left ? (p <= B0 + BLOCK - 2 && s_smin[p - B0 + 2] < dp) : (p >= B0 && s_pmin[p - B0] < dp);
+        if (l != r) put_node(&nodes[left ? r : l], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
+        if (!inside) {   // listed for k_lbvh_top (its record, or its leaf's, holds its box)
+            const uint32_t k = atomicAdd(&s_nesc, 1u);
+            esc[(size_t)blockIdx.x * LBVH_SLOTS + k] = make_uint4((uint32_t)l, (uint32_t)r, h, 0u);
+            break;
         }
-        float nlx, nly, nlz, nhx, nhy, nhz;
+        const int sl = p - B0, side = left ? 0 : 1;
+        float *bx = s_box[sl][side];
+        bx[0] = lx; bx[1] = ly; bx[2] = lz; bx[3] = hx; bx[4] = hy; bx[5] = hz;
+        s_end[sl][side] = (uint32_t)(left ? l : r);
+        s_h[sl][side] = h;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (atomicAdd(&s_arrive[sl], 1u) == 0u) break;   // the sibling's thread goes on
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const float *ob = s_box[sl][side ^ 1];
+        const int oend = (int)s_end[sl][side ^ 1];
+        const uint32_t sh = s_h[sl][side ^ 1];
+        const int L = left ? l : oend, R = left ? oend : r;
         if (left) {
-            nlx = wmin(lx, sx); nly = wmin(ly, sy); nlz = wmin(lz, sz);
-            nhx = wmax(hx, tx); nhy = wmax(hy, ty); nhz = wmax(hz, tz);
+            lx = wmin(lx, ob[0]); ly = wmin(ly, ob[1]); lz = wmin(lz, ob[2]);
+            hx = wmax(hx, ob[3]); hy = wmax(hy, ob[4]); hz = wmax(hz, ob[5]);
         } else {
-            nlx = wmin(sx, lx); nly = wmin(sy, ly); nlz = wmin(sz, lz);
-            nhx = wmax(tx, hx); nhy = wmax(ty, hy); nhz = wmax(tz, hz);
+            lx = wmin(ob[0], lx); ly = wmin(ob[1], ly); lz = wmin(ob[2], lz);
+            hx = wmax(ob[3], hx); hy = wmax(ob[4], hy); hz = wmax(ob[5], hz);
         }
-        lx = nlx; ly = nly; lz = nlz; hx = nhx; hy = nhy; hz = nhz;
         h = 1 + (h > sh ? h : sh);
+        lc = L == p ? n - 1 + p : p;
+        rc = p + 1 == R ? n + p : p + 1;
         l = L; r = R;
-        const bool root = l == 0 && r == n - 1;
-        if (root) {
+        if (l == 0 && r == n - 1) {
             put_node(&nodes[0], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
             atomicMax(depth_out, h);
-            return;
+            break;
         }
-        left = is_left(codes, n, l, r);
-        const int me = left ? r : l;
-        if (BUF) put_node_buf(rs, me, lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
-        else put_node_dev(&nodes[me], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
+        left = s_d[r - B0 + 1] > s_d[l - B0];
+    }
+    __syncthreads();
+    if (t == 0) esc_count[blockIdx.x] = s_nesc;
+}
+
+// BUF: the records through a buffer resource (16-byte device-scope accesses)
+template <bool BUF>
+__global__ __launch_bounds__(BLOCK) void k_lbvh_top(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ items,
+                                                      const float4 *__restrict__ boxes, int n, int groups, uint32_t node0,
+                                                      LrNode *nodes, unsigned long long *other, const uint4 *__restrict__ esc,
+                                                      const uint32_t *__restrict__ esc_count, uint32_t *depth_out)
+{
+    const int g = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);   // one wave per group's list
+    if (g >= groups) return;
+    const uint32_t cnt = esc_count[g];
+    const __amdgpu_buffer_rsrc_t rs = node_rsrc(nodes);
+    for (uint32_t k = threadIdx.x & 63; k < cnt; k += 64) {
+        const uint4 q = esc[(size_t)g * LBVH_SLOTS + k];
+        int l = (int)q.x, r = (int)q.y;
+        uint32_t h = q.z;
+        bool left = is_left(codes, n, l, r);
+        const int me = l == r ? n - 1 + l : (left ? r : l);
+        const float4 *rec = reinterpret_cast<const float4 *>(&nodes[me]);   // written by k_lbvh
+        const float4 blo = rec[0], bhi = rec[1];
+        float lx = blo.x, ly = blo.y, lz = blo.z, hx = bhi.x, hy = bhi.y, hz = bhi.z;
+        for (;;) {
+            const int p = left ? r : l - 1;             // the parent's split
+            const int end = left ? l : r;
+            wait_mem();                                 // this subtree's record is out before the arrival
+            const unsigned long long mine = ((unsigned long long)h << 32) | ((uint32_t)end + 1u);
+            const unsigned long long o = __hip_atomic_exchange(&other[p], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (o == 0) break;                          // the sibling is not done: it goes on
+            const int oend = (int)(uint32_t)o - 1;
+            const uint32_t sh = (uint32_t)(o >> 32);
+            const int L = left ? end : oend, R = left ? oend : end;
+            const int lc = L == p ? n - 1 + p : p, rc = p + 1 == R ? n + p : p + 1;
+            const int sib = left ? rc : lc;
+            float sx, sy, sz, tx, ty, tz;
+            if (sib >= n - 1) {   // a leaf: its box is the input's
+                const uint32_t it = items[sib - (n - 1)];
+                const float4 a = boxes[2 * (size_t)it], b = boxes[2 * (size_t)it + 1];
+                sx = a.x; sy = a.y; sz = a.z; tx = b.x; ty = b.y; tz = b.z;
+            } else if (BUF) {
+                const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, sib * 32, 0, CPOL_SC1);
+                const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, sib * 32 + 16, 0, CPOL_SC1);
+                sx = __uint_as_float(a.x); sy = __uint_as_float(a.y); sz = __uint_as_float(a.z);
+                tx = __uint_as_float(b.x); ty = __uint_as_float(b.y); tz = __uint_as_float(b.z);
+            } else {
+                const uint64_t *sp = reinterpret_cast<const uint64_t *>(&nodes[sib]);
+                const uint64_t q0 = ld_dev64(sp), q1 = ld_dev64(sp + 1), q2 = ld_dev64(sp + 2), q3 = ld_dev64(sp + 3);
+                sx = __uint_as_float((uint32_t)q0); sy = __uint_as_float((uint32_t)(q0 >> 32));
+                sz = __uint_as_float((uint32_t)q1);
+                tx = __uint_as_float((uint32_t)q2); ty = __uint_as_float((uint32_t)(q2 >> 32));
+                tz = __uint_as_float((uint32_t)q3);
+            }
+            float nlx, nly, nlz, nhx, nhy, nhz;
+            if (left) {
+                nlx = wmin(lx, sx); nly = wmin(ly, sy); nlz = wmin(lz, sz);
+                nhx = wmax(hx, tx); nhy = wmax(hy, ty); nhz = wmax(hz, tz);
+            } else {
+                nlx = wmin(sx, lx); nly = wmin(sy, ly); nlz = wmin(sz, lz);
+                nhx = wmax(tx, hx); nhy = wmax(ty, hy); nhz = wmax(tz, hz);
+            }
+            lx = nlx; ly = nly; lz = nlz; hx = nhx; hy = nhy; hz = nhz;
+            h = 1 + (h > sh ? h : sh);
+            l = L; r = R;
+            if (l == 0 && r == n - 1) {
+                put_node(&nodes[0], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
+                atomicMax(depth_out, h);
+                break;
+            }
+            left = is_left(codes, n, l, r);
+            const int mi = left ? r : l;
+            if (BUF) put_node_buf(rs, mi, lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
+            else put_node_dev(&nodes[mi], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
+        }
     }
 }
 
@@ -821,8 +951,10 @@ Layout layout(int n)
     L.status = take(SORT_PASSES * tiles * 256 * 4);
     L.other = take(N * 8);
     L.zero_words = (o - L.zero) / 4;
-    L.height = take(8 * N);
-    L.bx = take((2 * N) * 32);
+    // PLOC's; an LBVH build's listed nodes (k_lbvh's slots, 16 bytes each) and their counts
+    const size_t groups = (N + BLOCK - 1) / BLOCK;
+    L.height = take(std::max(8 * N, groups * 4));
+    L.bx = take(std::max((2 * N) * 32, groups * LBVH_SLOTS * 16));
     L.list0 = take(N * 4);
     L.list1 = take(N * 4);
     L.nnbuf = take(N * 4);
@@ -913,15 +1045,21 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
     int rounds = 0;
     const bool ploc = builder == LRH_PLOC && n > 2;
     if (!ploc) {
-        // the buffer path while the inner records lie within 2 GiB of the array's base
+        const int groups = (n + BLOCK - 1) / BLOCK;
+        uint4 *esc = (uint4 *)at(L.bx);
+        uint32_t *esc_count = (uint32_t *)at(L.height);
+        hipLaunchKernelGGL(k_lbvh, dim3(groups), dim3(BLOCK), 0, s, k0, v0, reinterpret_cast<const float4 *>(boxes), n, node0,
+                           ref0, ref_add, nd, ax, esc, esc_count, depth);
+        LR_HIP(hipGetLastError());
+        // the climbs above the groups; the buffer path while the records lie within 2 GiB of
+        // the array's base
+        const dim3 tgrid((unsigned)((groups + BLOCK / 64 - 1) / (BLOCK / 64)));
         if ((2 * (size_t)n - 1) * 32 < 0x7fffffffu)
-            hipLaunchKernelGGL(k_lbvh<true>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, k0, v0,
-                               reinterpret_cast<const float4 *>(boxes), n, node0, ref0, ref_add, nd, ax,
-                               (unsigned long long *)at(L.other), depth);
+            hipLaunchKernelGGL(k_lbvh_top<true>, tgrid, dim3(BLOCK), 0, s, k0, v0, reinterpret_cast<const float4 *>(boxes), n,
+                               groups, node0, nd, (unsigned long long *)at(L.other), esc, esc_count, depth);
         else
-            hipLaunchKernelGGL(k_lbvh<false>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, k0, v0,
-                               reinterpret_cast<const float4 *>(boxes), n, node0, ref0, ref_add, nd, ax,
-                               (unsigned long long *)at(L.other), depth);
+            hipLaunchKernelGGL(k_lbvh_top<false>, tgrid, dim3(BLOCK), 0, s, k0, v0, reinterpret_cast<const float4 *>(boxes), n,
+                               groups, node0, nd, (unsigned long long *)at(L.other), esc, esc_count, depth);
         LR_HIP(hipGetLastError());
     } else {
         float4 *bx = (float4 *)at(L.bx);

@@ -7,7 +7,7 @@ hit written, per node record fetched its size (32 bytes for the reference's bina
 the compressed 4-wide node), per shape tested its 48-byte triangle and 4-byte aux entry. The node
 and shape counts are the walks' own counters on the same tree and rays.
 
-LBVH build (k_bounds, k_morton, k_sort_pass x 4, k_lbvh), per primitive:
+LBVH build (k_bounds, k_morton, k_sort_pass x 4, k_lbvh and k_lbvh_top), per primitive:
   bounds    the box read, 32
   morton    the box read, code and item written, 32 + 8
   sort      four 8-bit passes, each (code, item) read and written, 4 x 16
