@@ -179,6 +179,7 @@ def bench_bvh_stress(a, torch, lr):
     alg_bytes = rf.walk_bytes(nr, cn, ct, rf.NODE_BYTES_BINARY)
     achieved, _ = rf.roofline(alg_bytes, kms[dom])
     lb_gbs, lb_frac = rf.roofline(rf.lbvh_bytes(n), kms["build_lbvh"])
+    pl_gbs, pl_frac = rf.roofline(rf.ploc_bytes(n, st_p.clusters), kms["build_ploc"])
 
     out = {
         "metric": "batched Mrays/s + p50/p99 job latency vs plain per-job launch, equal RMSE",
@@ -202,7 +203,9 @@ def bench_bvh_stress(a, torch, lr):
                            "ploc": round(n / (kms["build_ploc"] * 1e3), 2)},
         "build_roofline": {"kernel": "build_lbvh (bounds, morton, 4 sort passes, lbvh, lbvh_top)", "bound": "hbm",
                            "alg_bytes": rf.lbvh_bytes(n), "achieved": round(lb_gbs, 2), "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": round(lb_frac, 4)},
+                           "unit": "GB/s", "frac": round(lb_frac, 4),
+                           "ploc": {"alg_bytes": rf.ploc_bytes(n, st_p.clusters), "live": st_p.clusters,
+                                    "achieved": round(pl_gbs, 2), "frac": round(pl_frac, 4)}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "alg_bytes": alg_bytes, "node_fetches": cn, "tri_tests": ct, "rays": nr},
@@ -232,6 +235,9 @@ def bench_bvh_stress(a, torch, lr):
             row[name + "_mprims_s"] = round(m / (ms * 1e3), 1)
         gbs, frac = rf.roofline(rf.lbvh_bytes(m), row["lbvh_ms"])
         row["lbvh_roofline"] = {"achieved": round(gbs, 1), "unit": "GB/s", "frac": round(frac, 4)}
+        live = ctx.build(b_m, m, nd_m, ax_m, lr.PLOC).clusters     # the rounds' live clusters
+        gbs, frac = rf.roofline(rf.ploc_bytes(m, live), row["ploc_ms"])
+        row["ploc_roofline"] = {"achieved": round(gbs, 1), "unit": "GB/s", "frac": round(frac, 4), "live": live}
         scaling.append(row)
         del t_m, b_m, nd_m, ax_m
     out["build_scaling"] = scaling

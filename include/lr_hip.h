@@ -38,6 +38,8 @@ typedef struct {
     int depth;       /* the tree's depth in levels (check_tree's, trace_gpu.c:768-805) */
     int rounds;      /* PLOC rounds (0 for an LBVH) */
     float ms;        /* device time of the build, hipEvent-timed on the context's stream */
+    uint64_t clusters; /* PLOC: the live clusters summed over its rounds, the count its traffic
+                          is priced by (0 for an LBVH) */
 } LrhBuildStats;
 
 /* The message of the last failed call on this host thread. */

@@ -32,7 +32,8 @@ MAT_DIFFUSE, MAT_MIRROR, MAT_GLASS, MAT_LIGHT, MAT_PBR = 0, 1, 2, 3, 4
 
 
 class BuildStats(ctypes.Structure):
-    _fields_ = [("depth", ctypes.c_int), ("rounds", ctypes.c_int), ("ms", ctypes.c_float)]
+    _fields_ = [("depth", ctypes.c_int), ("rounds", ctypes.c_int), ("ms", ctypes.c_float),
+                ("clusters", ctypes.c_uint64)]
 
 
 class Tune(ctypes.Structure):

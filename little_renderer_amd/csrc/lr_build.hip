@@ -7,6 +7,7 @@
 // passes, its 54 refit passes one bottom-up pass with per-node arrival counters, and each
 // PLOC round (4 dispatches there) one kernel whose cluster windows sit in LDS.
 #include <stdarg.h>
+#include <stddef.h>
 #include <algorithm>
 #include <stdio.h>
 
@@ -650,7 +651,8 @@ __global__ __launch_bounds__(BLOCK) void k_lbvh_top(const uint32_t *__restrict__
 struct PlocState {
     uint32_t nc;          // clusters in the current list
     uint32_t parity;      // which of the two lists is current
-    uint32_t pad[62];
+    uint64_t live;        // the live clusters summed over the rounds (the build's traffic)
+    uint32_t pad[60];
     uint32_t ctr[PLOC_MAX_ROUNDS];   // each round's tile counter
 };
 
@@ -827,6 +829,7 @@ __global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t
                 if (tile == ntiles - 1) {              // the round's last tile: the next count
                     st_dev(&st->nc, (uint32_t)ek + tk);
                     st_dev(&st->parity, 1u - p);
+                    st_dev64(&st->live, ld_dev64(&st->live) + nc);   // (one writer a round)
                 }
             }
         }
@@ -848,17 +851,47 @@ __global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t
     }
 }
 
-// small (trace_ploc.wgsl:236-301): the rounds left, in one workgroup
+// small (trace_ploc.wgsl:236-301): the rounds left, in one workgroup. While more than
+// SMALL_MAX clusters are left (only when the rounds before could not merge enough: boxes that
+// tie everywhere merge one pair a round) a round goes through device memory; from SMALL_MAX
+// down, the list, its boxes (six arrays of floats) and heights live in LDS, the nearest search
+// reads its 2R neighbours there, and only the merged nodes' records go out.
 constexpr int SMALL_BLOCK = 1024;
+constexpr int SMALL_MAX = 4096;
+constexpr int SMALL_PER = SMALL_MAX / SMALL_BLOCK;   // positions a thread holds in an LDS round
+
+// the nearest of position i among the LDS list of nc (positions +- R; the tie rule of nearer)
+__device__ __forceinline__ uint32_t nearest_lds(int i, int nc, const float *lx, const float *ly, const float *lz,
+                                                const float *hx, const float *hy, const float *hz)
+{
+    const float ax = lx[i], ay = ly[i], az = lz[i], bx_ = hx[i], by = hy[i], bz = hz[i];
+    const int j0 = i - PLOC_R > 0 ? i - PLOC_R : 0;
+    const int j1 = i + PLOC_R < nc - 1 ? i + PLOC_R : nc - 1;
+    float bd = 0.0f;
+    uint32_t bj = LR_NONE;
+    for (int j = j0; j <= j1; j++) {
+        if (j == i) continue;
+        const float d = half_area(wmin(ax, lx[j]), wmin(ay, ly[j]), wmin(az, lz[j]), wmax(bx_, hx[j]), wmax(by, hy[j]),
+                                  wmax(bz, hz[j]));
+        if (nearer(d, (uint32_t)i, (uint32_t)j, bd, bj)) { bd = d; bj = (uint32_t)j; }
+    }
+    return bj;
+}
+
 __global__ __launch_bounds__(SMALL_BLOCK) void k_ploc_small(int n, uint32_t node0, uint32_t ref0, float4 *bx, uint32_t *list0,
                                                               uint32_t *list1, uint32_t *nnbuf, LrNode *nodes, uint32_t *height,
                                                               PlocState *st, uint32_t *rounds_out, uint32_t *depth_out)
 {
     __shared__ uint32_t s_wsum[SMALL_BLOCK / 64];
+    __shared__ float s_lx[SMALL_MAX], s_ly[SMALL_MAX], s_lz[SMALL_MAX], s_hx[SMALL_MAX], s_hy[SMALL_MAX], s_hz[SMALL_MAX];
+    __shared__ uint32_t s_id[SMALL_MAX], s_nn[SMALL_MAX], s_h[SMALL_MAX];
     const int tid = threadIdx.x;
     uint32_t nc = st->nc, p = st->parity;
     uint32_t rounds = 0;
-    while (nc > 1) {
+    uint64_t live = 0;
+    // rounds through device memory while the list does not fit in LDS
+    while (nc > (uint32_t)SMALL_MAX) {
+        live += nc;
         const uint32_t *cur = p ? list1 : list0;
         uint32_t *nxt = p ? list0 : list1;
         for (int i = tid; i < (int)nc; i += SMALL_BLOCK) {
@@ -910,14 +943,95 @@ __global__ __launch_bounds__(SMALL_BLOCK) void k_ploc_small(int n, uint32_t node
         __syncthreads();
         // every round merges at least the pair nearest of all (it is mutual); a round that
         // merges nothing (NaN boxes) ends the loop instead of spinning
-        if (tk >= nc) break;
+        if (tk >= nc) {
+            nc = 0;
+            break;
+        }
         nc = tk;
         p = 1u - p;
         rounds++;
     }
+    // the rest in LDS
+    if (nc > 0) {
+        const uint32_t *cur = p ? list1 : list0;
+        for (int i = tid; i < (int)nc; i += SMALL_BLOCK) {
+            const uint32_t x = cur[i];
+            const float4 lo = bx[2 * (size_t)x], hi = bx[2 * (size_t)x + 1];
+            s_id[i] = x;
+            s_lx[i] = lo.x; s_ly[i] = lo.y; s_lz[i] = lo.z;
+            s_hx[i] = hi.x; s_hy[i] = hi.y; s_hz[i] = hi.z;
+            s_h[i] = height[x];
+        }
+        __syncthreads();
+    }
+    while (nc > 1) {
+        live += nc;
+        for (int i = tid; i < (int)nc; i += SMALL_BLOCK) s_nn[i] = nearest_lds(i, (int)nc, s_lx, s_ly, s_lz, s_hx, s_hy, s_hz);
+        __syncthreads();
+        // each thread takes a run of positions (the list's order is kept by the scans), holds
+        // what it writes in registers, and writes after every read of the round is done
+        const int per = ((int)nc + SMALL_BLOCK - 1) / SMALL_BLOCK;
+        const int s0 = min(tid * per, (int)nc), s1 = min(s0 + per, (int)nc);
+        uint32_t ck = 0, cm = 0;
+        for (int i = s0; i < s1; i++) {
+            const uint32_t j = s_nn[i];
+            const bool mu = j < nc && s_nn[j] == (uint32_t)i;
+            ck += !(mu && (uint32_t)i > j);
+            cm += mu && (uint32_t)i < j;
+        }
+        uint32_t tk, tm;
+        const uint32_t dk0 = block_excl_scan<SMALL_BLOCK / 64>(ck, &tk, s_wsum);
+        uint32_t dm = block_excl_scan<SMALL_BLOCK / 64>(cm, &tm, s_wsum);
+        uint32_t o_id[SMALL_PER], o_h[SMALL_PER];
+        float o_b[SMALL_PER][6];
+        int m = 0;
+#pragma unroll
+        for (int q = 0; q < SMALL_PER; q++) {
+            const int i = s0 + q;
+            if (i >= s1) break;
+            const uint32_t j = s_nn[i];
+            const bool mu = j < nc && s_nn[j] == (uint32_t)i;
+            if (mu && (uint32_t)i > j) continue;
+            float b0 = s_lx[i], b1 = s_ly[i], b2 = s_lz[i], b3 = s_hx[i], b4 = s_hy[i], b5 = s_hz[i];
+            uint32_t id = s_id[i], h = s_h[i];
+            if (mu && (uint32_t)i < j) {
+                const uint32_t k = ((uint32_t)n - 2u) - ((uint32_t)n - nc + dm);
+                dm++;
+                const float4 clo = make_float4(b0, b1, b2, 0.0f), chi = make_float4(b3, b4, b5, 0.0f);
+                const float4 dlo = make_float4(s_lx[j], s_ly[j], s_lz[j], 0.0f), dhi = make_float4(s_hx[j], s_hy[j], s_hz[j], 0.0f);
+                b0 = wmin(clo.x, dlo.x); b1 = wmin(clo.y, dlo.y); b2 = wmin(clo.z, dlo.z);
+                b3 = wmax(chi.x, dhi.x); b4 = wmax(chi.y, dhi.y); b5 = wmax(chi.z, dhi.z);
+                put_final(&nodes[1 + 2 * (size_t)k], id, clo, chi, (uint32_t)n, node0, ref0);
+                put_final(&nodes[2 + 2 * (size_t)k], s_id[j], dlo, dhi, (uint32_t)n, node0, ref0);
+                const uint32_t hd = s_h[j];
+                h = 1 + (h > hd ? h : hd);
+                if (k == 0) put_node(&nodes[0], b0, b1, b2, node0 + 1u, b3, b4, b5, node0 + 2u);
+                id = k;
+            }
+            o_id[m] = id;
+            o_h[m] = h;
+            o_b[m][0] = b0; o_b[m][1] = b1; o_b[m][2] = b2; o_b[m][3] = b3; o_b[m][4] = b4; o_b[m][5] = b5;
+            m++;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < SMALL_PER; q++) {
+            if (q >= m) break;
+            const uint32_t d = dk0 + (uint32_t)q;
+            s_id[d] = o_id[q];
+            s_h[d] = o_h[q];
+            s_lx[d] = o_b[q][0]; s_ly[d] = o_b[q][1]; s_lz[d] = o_b[q][2];
+            s_hx[d] = o_b[q][3]; s_hy[d] = o_b[q][4]; s_hz[d] = o_b[q][5];
+        }
+        __syncthreads();
+        if (tk >= nc) break;   // merged nothing (NaN boxes): ends instead of spinning
+        nc = tk;
+        rounds++;
+    }
     if (tid == 0) {
         *rounds_out = rounds;
-        *depth_out = height[0];
+        *depth_out = nc == 1 ? s_h[0] : height[0];
+        st->live += live;
     }
 }
 
@@ -1071,7 +1185,7 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
                            reinterpret_cast<const float4 *>(boxes), n, ref0, ref_add, ax, bx, l0, height, pst, ps,
                            L.pstatus_words);
         LR_HIP(hipGetLastError());
-        const int small = c->ploc_small > 0 ? c->ploc_small : 1024;
+        const int small = std::min(c->ploc_small > 0 ? c->ploc_small : SMALL_MAX, SMALL_MAX);
         int nr = ploc_rounds(n, small);
         if (nr > PLOC_MAX_ROUNDS) nr = PLOC_MAX_ROUNDS;
         double est = n;
@@ -1095,8 +1209,13 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
         float ms = 0;
         LR_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
         uint32_t dep = 0, small_rounds = 0;
+        uint64_t live = 0;
         LR_HIP(hipMemcpy(&dep, depth, 4, hipMemcpyDeviceToHost));
-        if (ploc) LR_HIP(hipMemcpy(&small_rounds, at(L.rounds), 4, hipMemcpyDeviceToHost));
+        if (ploc) {
+            LR_HIP(hipMemcpy(&small_rounds, at(L.rounds), 4, hipMemcpyDeviceToHost));
+            LR_HIP(hipMemcpy(&live, (const char *)at(L.pst) + offsetof(PlocState, live), 8, hipMemcpyDeviceToHost));
+        }
+        stats->clusters = live;
         stats->depth = (int)dep;
         stats->rounds = ploc ? rounds + (int)small_rounds : 0;
         stats->ms = ms;

@@ -13,6 +13,15 @@ LBVH build (k_bounds, k_morton, k_sort_pass x 4, k_lbvh and k_lbvh_top), per pri
   sort      four 8-bit passes, each (code, item) read and written, 4 x 16
   lbvh      item 4 and box 32 read, leaf node 32 and aux 4 written, and per inner node (n - 1 of
             them, counted per primitive) the sibling's record read and the node written, 32 + 32
+
+PLOC build (k_ploc_init, k_ploc_round x rounds, k_ploc_small), the bounds, Morton and sort
+passes before it priced as for the LBVH (bounds 32, morton 40, sort 64 per primitive):
+  init      per primitive: item 4 and box 32 read, cluster box 32, aux 4, height 4 and list
+            entry 4 written, 80
+  rounds    per live cluster of each round: its list entry 4 and box 32 read, the next list's
+            entry 4 written, 40 (live: the clusters summed over the rounds, the build's count)
+  merges    n - 1 of them: the merged box 32, the two children's records 64 written, their
+            heights 8 read and the merged one 4 written, 108
 """
 
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
@@ -33,6 +42,15 @@ def lbvh_bytes(n, with_bounds=True):
     """an LBVH build of n primitives (the bounds pass included unless the caller gives them)"""
     per = sum(v for k, v in LBVH_BYTES.items() if with_bounds or k != "bounds")
     return per * n
+
+
+PLOC_INIT, PLOC_PER_LIVE, PLOC_PER_MERGE = 80, 40, 108
+
+
+def ploc_bytes(n, live, with_bounds=True):
+    """a PLOC build of n primitives whose rounds held `live` clusters in all"""
+    pre = sum(v for k, v in LBVH_BYTES.items() if k != "lbvh" and (with_bounds or k != "bounds"))
+    return (pre + PLOC_INIT) * n + PLOC_PER_LIVE * live + PLOC_PER_MERGE * max(n - 1, 0)
 
 
 def roofline(alg_bytes, ms):

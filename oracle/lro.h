@@ -53,9 +53,11 @@ int lro_lbvh(const uint32_t *sorted_codes, const uint32_t *sorted_items, const f
 int lro_lbvh_bottom_up(const uint32_t *sorted_codes, const uint32_t *sorted_items, const float *boxes, int n,
                        uint32_t node0, uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux);
 /* PLOC (trace_ploc.wgsl:1-301) with radius 16 over the Morton-ordered leaves. Returns the
- * number of rounds; *depth the tree's depth. n <= 2 builds an LBVH, as build_queue does. */
+ * number of rounds; *depth the tree's depth; *live (optional) the clusters summed over the
+ * rounds. n <= 2 builds an LBVH, as build_queue does. */
 int lro_ploc(const uint32_t *sorted_codes, const uint32_t *sorted_items, const float *boxes, int n,
-             uint32_t node0, uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux, int *depth);
+             uint32_t node0, uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux, int *depth,
+             uint64_t *live);
 /* check_tree (trace_gpu.c:768-805): 1 if nodes are one tree over exactly n leaves; its cost
  * (sum of areas / root's, in double) and depth */
 int lro_check_tree(const uint32_t *nodes, int n, uint32_t node0, double *cost, int *depth);

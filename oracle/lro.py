@@ -44,7 +44,7 @@ def lib():
             "lro_sort": (None, [P, I, P, P]),
             "lro_lbvh": (I, [P, P, P, I, U, U, U, P, P]),
             "lro_lbvh_bottom_up": (I, [P, P, P, I, U, U, U, P, P]),
-            "lro_ploc": (I, [P, P, P, I, U, U, U, P, P, P]),
+            "lro_ploc": (I, [P, P, P, I, U, U, U, P, P, P, P]),
             "lro_check_tree": (I, [P, I, U, P, P]),
             "lro_build_tris": (I, [P, I, I, U, U, U, P, P, P]),
             "lro_sah_build": (P, [P, I, I, I, I]),
@@ -178,14 +178,15 @@ def lbvh(sc, si, boxes, node0=0, ref0=0, ref_add=0, bottom_up=False):
     return nodes, aux, depth
 
 
-def ploc(sc, si, boxes, node0=0, ref0=0, ref_add=0):
+def ploc(sc, si, boxes, node0=0, ref0=0, ref_add=0, live=False):
     n = sc.shape[0]
     nodes = np.zeros((max(2 * n - 1, 1), 8), np.uint32)
     aux = np.zeros(max(n, 1), np.uint32)
     depth = ctypes.c_int()
+    lv = ctypes.c_uint64()
     rounds = lib().lro_ploc(ptr(sc), ptr(si), ptr(boxes), n, node0, ref0, ref_add, ptr(nodes), ptr(aux),
-                            ctypes.byref(depth))
-    return nodes, aux, depth.value, rounds
+                            ctypes.byref(depth), ctypes.byref(lv))
+    return (nodes, aux, depth.value, rounds, lv.value) if live else (nodes, aux, depth.value, rounds)
 
 
 def check_tree(nodes, n, node0=0):

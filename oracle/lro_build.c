@@ -279,8 +279,10 @@ static int nearer(float d, uint32_t i, uint32_t j, float bd, uint32_t bj)
 }
 
 int lro_ploc(const uint32_t *codes, const uint32_t *items, const float *boxes, int n,
-             uint32_t node0, uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux, int *depth)
+             uint32_t node0, uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux, int *depth,
+             uint64_t *live)
 {
+    if (live) *live = 0;
     if (n <= 2) {                                 /* build_queue: PLOC only above 2 (trace_gpu.c:615) */
         int d = lro_lbvh(codes, items, boxes, n, node0, ref0, ref_add, nodes, aux);
         if (depth) *depth = d;
@@ -302,6 +304,7 @@ int lro_ploc(const uint32_t *codes, const uint32_t *items, const float *boxes, i
     uint32_t nc = (uint32_t)n;
     int rounds = 0;
     while (nc > 1) {
+        if (live) *live += nc;
         /* nn (trace_ploc.wgsl:99-128) */
         for (uint32_t i = 0; i < nc; i++) {
             const float *bi = bx + 6 * (size_t)list[i];
@@ -412,7 +415,7 @@ int lro_build_tris(const float *tris, int n, int ploc, uint32_t node0, uint32_t 
     lro_morton(boxes, n, lo, ext, codes);
     lro_sort(codes, n, sc, si);
     int depth = 0;
-    if (ploc) lro_ploc(sc, si, boxes, n, node0, ref0, ref_add, nodes, aux, &depth);
+    if (ploc) lro_ploc(sc, si, boxes, n, node0, ref0, ref_add, nodes, aux, &depth, NULL);
     else depth = lro_lbvh(sc, si, boxes, n, node0, ref0, ref_add, nodes, aux);
     if (keys_out) { memcpy(keys_out, sc, (size_t)n * 4); memcpy(keys_out + n, si, (size_t)n * 4); }
     free(boxes); free(codes); free(sc); free(si);

@@ -95,6 +95,10 @@ def test_build_bit_exact(ctx, builder, n):
     assert g["st"].depth == depth
     ok, cost, d = lro.check_tree(got, n)
     assert ok and d == depth
+    if builder == lr.PLOC:   # the live clusters over the rounds (the roofline's count)
+        assert g["st"].clusters == lro.ploc(sc, si, lro.tri_boxes(tris), live=True)[4]
+    else:
+        assert g["st"].clusters == 0
 
 
 @pytest.mark.parametrize("builder", [lr.LBVH, lr.PLOC], ids=["lbvh", "ploc"])

@@ -182,12 +182,12 @@ def half_area32(lo, hi):
     return np.float32(np.float32(a + b) + c)
 
 
-def ploc_global(boxes_sorted, n):
+def ploc_global(boxes_sorted, n, live=None):
     """Agglomerative clustering where every cluster sees every other (n <= R + 1 = 17, so the
     radius of 16 covers the whole list): each round, each cluster's nearest over the whole list
     (ties: the lower pair of positions), mutual pairs merge into the lower one's place, the
     list keeps its order; a merge makes internal node (n-2) - (merges before it). Returns the
-    node array as trace_ploc.wgsl lays it out."""
+    node array as trace_ploc.wgsl lays it out; `live` (a list) gets each round's cluster count."""
     bx = {}
     for j in range(n):
         bx[n - 1 + j] = (boxes_sorted[j, 0:3].copy(), boxes_sorted[j, 4:7].copy())
@@ -207,6 +207,8 @@ def ploc_global(boxes_sorted, n):
 
     while len(lst) > 1:
         nc = len(lst)
+        if live is not None:
+            live.append(nc)
         nn = []
         for i in range(nc):
             best = None
@@ -248,9 +250,12 @@ def test_ploc_small_is_global_agglomeration(n):
         t = soup(n, seed=100 + seed)
         nodes, aux, sc, si, depth = lro.build_tris(t, ploc_=True)
         boxes = lro.tri_boxes(t)
-        want = ploc_global(boxes[si], n)
+        counts = []
+        want = ploc_global(boxes[si], n, counts)
         assert np.array_equal(nodes, want), (n, seed)
         assert np.array_equal(aux, si)
+        # the clusters summed over the rounds (the build's traffic count) agree too
+        assert lro.ploc(sc, si, boxes, live=True)[4] == sum(counts), (n, seed)
 
 
 def test_ploc_ties_go_to_the_lower_pair():
@@ -260,9 +265,11 @@ def test_ploc_ties_go_to_the_lower_pair():
     boxes = np.tile(np.array([0, 0, 0, 0, 1, 1, 1, 0], np.float32), (n, 1))
     sc = np.zeros(n, np.uint32)
     si = np.arange(n, dtype=np.uint32)
-    nodes, aux, depth, rounds = lro.ploc(sc, si, boxes)
+    nodes, aux, depth, rounds, live = lro.ploc(sc, si, boxes, live=True)
     want = ploc_global(boxes, n)
     assert np.array_equal(nodes, want)
+    # one pair a round (the lowest): n, n - 1, ..., 2 clusters live
+    assert rounds == n - 1 and live == n * (n + 1) // 2 - 1
     ok, cost, d = lro.check_tree(nodes, n)
     assert ok and d == depth
 

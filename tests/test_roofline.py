@@ -19,6 +19,14 @@ def test_lbvh_bytes_by_hand():
     assert rf.lbvh_bytes(10, with_bounds=False) == 10 * (per - 32)
 
 
+def test_ploc_bytes_by_hand():
+    # 3 primitives, 5 live clusters over the rounds: bounds 32 + morton 40 + sort 64 and init
+    # 80 per primitive, 40 per live cluster, 108 for each of the 2 merges
+    assert rf.ploc_bytes(3, 5) == 3 * (32 + 40 + 64 + 80) + 5 * 40 + 2 * 108
+    assert rf.ploc_bytes(3, 5, with_bounds=False) == rf.ploc_bytes(3, 5) - 3 * 32
+    assert rf.ploc_bytes(1, 0) == 32 + 40 + 64 + 80
+
+
 def test_roofline_fraction():
     gbs, frac = rf.roofline(8e9, 1.0)           # 8 GB in a millisecond: 8000 GB/s, the peak
     assert abs(gbs - 8000.0) < 1e-6 and abs(frac - 1.0) < 1e-12
