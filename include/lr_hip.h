@@ -177,6 +177,21 @@ int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const int *mesh_fi
  * reference rebuilds its TLAS, trace_gpu.c:1371-1379). No job of the scene may be running.
  * stats (optional): the top level's build. */
 int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int ninst, int builder, LrhBuildStats *stats);
+/* Only the instances that moved since the last update: inst[k] is instance idx[k] (n of them,
+ * each index once); their records alone are sent, in one upload scattered into place on the
+ * device, and the top level is rebuilt over all the instances. The reference's
+ * upload_instances sends the runs of records changed since its copy's version
+ * (trace_gpu.c:1391-1452; Instance.version, scene_changed_run, scene.h:207, 359-363): its
+ * caller passes those runs' indices here. n = 0: nothing is sent or rebuilt. The result equals
+ * lrh_pt_scene_update with every pose. No job of the scene may be running. */
+int lrh_pt_scene_move(LrhPtScene *s, const int *idx, const LrhInstance *inst, int n, int builder, LrhBuildStats *stats);
+/* A dynamic mesh's new triangles (scene_mesh_changed, scene.h:385-386): ntris must be the
+ * mesh's count. They are uploaded in its place, its tree is built again on the device in its
+ * place (with its 4-wide copy and leaf-order triangles), and the top level over the instances
+ * is rebuilt, as their world boxes follow the mesh's (upload_instances' dynamic meshes,
+ * trace_gpu.c:1486-1512). The result equals a scene made with these triangles. No job of the
+ * scene may be running. stats (optional): the mesh's build. */
+int lrh_pt_scene_mesh(LrhPtScene *s, int mesh, const float *tris, int ntris, int builder, LrhBuildStats *stats);
 
 /* Path traces spp samples (indices s0 .. s0+spp-1) of every pixel of a W x H image and adds
  * them to d_accum (float4 per pixel, row-major: the sum of the samples' rgb, and their count),

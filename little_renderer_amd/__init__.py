@@ -108,6 +108,8 @@ def lib():
         "lrh_pt_scene_sky": (I, [P, P, I, I, F, F]),
         "lrh_pt_scene_instanced": (I, [P, P, P, I, P, I, P, I, F, I, ctypes.POINTER(P)]),
         "lrh_pt_scene_update": (I, [P, P, I, I, ctypes.POINTER(BuildStats)]),
+        "lrh_pt_scene_move": (I, [P, P, P, I, I, ctypes.POINTER(BuildStats)]),
+        "lrh_pt_scene_mesh": (I, [P, I, P, I, I, ctypes.POINTER(BuildStats)]),
         "lrh_pt_scene_trace": (I, [P, P, P, I, I, P]),
         "lrh_pt_scene_info": (I, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_size_t)]),
@@ -274,6 +276,26 @@ class PtScene:
         inst = np.ascontiguousarray(instances, INSTANCE_DTYPE)
         st = BuildStats()
         _check(lib().lrh_pt_scene_update(self._h, _ptr(inst), len(inst), builder, ctypes.byref(st)))
+        return st
+
+    def move(self, idx, instances, builder=LBVH):
+        """lrh_pt_scene_move: new poses for the instances idx (each once; an INSTANCE_DTYPE array of
+        as many); the top level's BuildStats (zeros when nothing moved)"""
+        import numpy as np
+        ix = np.ascontiguousarray(idx, np.int32)
+        inst = np.ascontiguousarray(instances, INSTANCE_DTYPE)
+        if len(ix) != len(inst):
+            raise ValueError("one pose per index")
+        st = BuildStats()
+        _check(lib().lrh_pt_scene_move(self._h, _ptr(ix), _ptr(inst), len(ix), builder, ctypes.byref(st)))
+        return st
+
+    def mesh(self, m, tris, builder=PLOC):
+        """lrh_pt_scene_mesh: mesh m's new triangles (as many as it has); the mesh build's BuildStats"""
+        import numpy as np
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 12)
+        st = BuildStats()
+        _check(lib().lrh_pt_scene_mesh(self._h, m, _ptr(t), len(t), builder, ctypes.byref(st)))
         return st
 
     def trace(self, rays, any_hit=False):

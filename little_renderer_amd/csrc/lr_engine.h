@@ -15,6 +15,8 @@
 // fixed-point integer (2^-28 units), whose sum does not depend on order.
 #pragma once
 
+#include <vector>
+
 #include "lr_internal.h"
 
 #define LR_SLOT_BITS 22                   // path slots: at most 2^22 (queue ids: job << 22 | slot)
@@ -154,12 +156,17 @@ struct LrhPtScene {
     float env_scale = 1.0f, env_rot = 0.0f;
     // a two-level scene (lrh_pt_scene_instanced): meshes, instances, the top level
     int two_level = 0, nmeshes = 0, ninst = 0, mesh_depth = 0, tlas_depth = 0;
+    int top_builder = 0;                      // the top level's last builder (a mesh's update builds it again so)
     float4 *mtris = nullptr, *mboxes = nullptr, *mwnodes = nullptr, *inst = nullptr, *ifwd = nullptr;
     float4 *mw4nodes = nullptr, *tw4nodes = nullptr;   // the meshes' and the top level's 4-wide nodes
     uint32_t *mroot4 = nullptr;                        // each mesh's root wide slot
     uint32_t *mnodes = nullptr, *maux = nullptr, *mroot = nullptr;
     float4 *tboxes = nullptr;
     uint32_t *tnodes = nullptr, *taux = nullptr;
+    // host copies of the meshes' layout: first triangle (nmeshes + 1 of them), root record,
+    // root wide slot and tree depth of each mesh
+    std::vector<int> mfirst, mdepth;
+    std::vector<uint32_t> mroot_h, mroot4_h;
     float *tab = nullptr;
     uint16_t *mask = nullptr;
     int nprims = 0, ntris = 0, nmats = 0, light = -1, depth = 1;

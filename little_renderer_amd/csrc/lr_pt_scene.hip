@@ -12,10 +12,10 @@
 // layout, both children's boxes and (index | leaf record); a one-shape tree (its root a leaf)
 // gets a record whose left child is that leaf and whose right child is an empty leaf. Made for
 // the two-level scenes' top level and meshes (JobDev bnodes, mnodes); no walk reads them now.
-__global__ __launch_bounds__(256) void k_child_boxes(const float4 *__restrict__ nodes, int nn, float4 *__restrict__ w)
+__global__ __launch_bounds__(256) void k_child_boxes(const float4 *__restrict__ nodes, int i0, int nn, float4 *__restrict__ w)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nn) return;
+    const int i = i0 + (int)(blockIdx.x * 256 + threadIdx.x);   // records i0 .. i0 + nn - 1
+    if (i >= i0 + nn) return;
     const float4 lo = nodes[2 * (size_t)i], hi = nodes[2 * (size_t)i + 1];
     const uint32_t b = __float_as_uint(hi.w);
     if (b & LR_LEAF) return;
@@ -460,30 +460,21 @@ static int pack_instances(const LrhPtScene *s, const LrhInstance *in, int n, con
     return 0;
 }
 
-extern "C" int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int ninst, int builder, LrhBuildStats *stats)
+// The top level over the instances again, from their records on the device: their world boxes,
+// the tree over them, its binary child-box records and its 4-wide copy; the walks' stack depth.
+// Waits for it (collapse_wide does).
+static int rebuild_top(LrhPtScene *s, int builder, LrhBuildStats *stats)
 {
-    if (!s || !s->two_level || !inst || ninst != s->ninst || (builder != LRH_LBVH && builder != LRH_PLOC))
-        return lr_fail(LRH_EINVAL, "lrh_pt_scene_update: bad arguments (an instanced scene, the same instance count)");
     LrhContext *c = s->ctx;
-    std::vector<uint32_t> roots(s->nmeshes), roots4(s->nmeshes);
-    if (s->nmeshes) {
-        LR_HIP(hipMemcpy(roots.data(), s->mroot, (size_t)s->nmeshes * 4, hipMemcpyDeviceToHost));
-        LR_HIP(hipMemcpy(roots4.data(), s->mroot4, (size_t)s->nmeshes * 4, hipMemcpyDeviceToHost));
-    }
-    std::vector<float> rec, fwd;
-    int rc = pack_instances(s, inst, ninst, roots, roots4, rec, fwd);
-    if (rc) return rc;
-    LR_HIP(hipStreamSynchronize(c->stream));
-    LR_HIP(hipMemcpy(s->inst, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
-    LR_HIP(hipMemcpy(s->ifwd, fwd.data(), fwd.size() * 4, hipMemcpyHostToDevice));
+    const int ninst = s->ninst;
     hipLaunchKernelGGL(k_inst_boxes, dim3((unsigned)((ninst + 255) / 256)), dim3(256), 0, c->stream, s->ifwd, s->inst,
                        s->mroot, (const float4 *)s->mnodes, ninst, s->tboxes);
     LR_HIP(hipGetLastError());
     LrhBuildStats st;
-    rc = lrh_build(c, (const float *)s->tboxes, ninst, nullptr, builder, 0, 0, 0, s->tnodes, s->taux, nullptr, &st);
+    int rc = lrh_build(c, (const float *)s->tboxes, ninst, nullptr, builder, 0, 0, 0, s->tnodes, s->taux, nullptr, &st);
     if (rc) return rc;
     const size_t nn = 2 * (size_t)ninst - 1;
-    hipLaunchKernelGGL(k_child_boxes, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, (const float4 *)s->tnodes,
+    hipLaunchKernelGGL(k_child_boxes, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, (const float4 *)s->tnodes, 0,
                        (int)nn, s->wnodes);
     if (ninst == 1) hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, c->stream, (const float4 *)s->tnodes, 0u, s->wnodes);
     LR_HIP(hipGetLastError());
@@ -491,9 +482,125 @@ extern "C" int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int n
     rc = collapse_wide(c, s->tnodes, 0u, ninst, st.depth, s->tw4nodes);
     if (rc) return rc;
     s->tlas_depth = st.depth;
+    s->top_builder = builder;
     // the two-level walks' stacks: per wide level of either tree at most three pending
     // entries (the binary walk: a sibling and an instance per top-level level), and EXIT
     s->depth = 3 * (s->tlas_depth + s->mesh_depth) + 8;
+    if (stats) *stats = st;
+    return 0;
+}
+
+extern "C" int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int ninst, int builder, LrhBuildStats *stats)
+{
+    if (!s || !s->two_level || !inst || ninst != s->ninst || (builder != LRH_LBVH && builder != LRH_PLOC))
+        return lr_fail(LRH_EINVAL, "lrh_pt_scene_update: bad arguments (an instanced scene, the same instance count)");
+    LrhContext *c = s->ctx;
+    std::vector<float> rec, fwd;
+    int rc = pack_instances(s, inst, ninst, s->mroot_h, s->mroot4_h, rec, fwd);
+    if (rc) return rc;
+    LR_HIP(hipStreamSynchronize(c->stream));
+    LR_HIP(hipMemcpy(s->inst, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
+    LR_HIP(hipMemcpy(s->ifwd, fwd.data(), fwd.size() * 4, hipMemcpyHostToDevice));
+    return rebuild_top(s, builder, stats);
+}
+
+// moved instances' records, packed in one upload (index, 4 float4 record, 3 float4 rows each),
+// into their places
+__global__ __launch_bounds__(256) void k_scatter_inst(const int *__restrict__ idx, const float4 *__restrict__ rec,
+                                                      const float4 *__restrict__ fwd, int n, float4 *__restrict__ inst,
+                                                      float4 *__restrict__ ifwd)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const size_t i = (size_t)idx[k];
+    for (int q = 0; q < 4; q++) inst[4 * i + q] = rec[4 * (size_t)k + q];
+    for (int q = 0; q < 3; q++) ifwd[3 * i + q] = fwd[3 * (size_t)k + q];
+}
+
+extern "C" int lrh_pt_scene_move(LrhPtScene *s, const int *idx, const LrhInstance *inst, int n, int builder,
+                                 LrhBuildStats *stats)
+{
+    if (!s || !s->two_level || n < 0 || (n > 0 && (!idx || !inst)) || (builder != LRH_LBVH && builder != LRH_PLOC))
+        return lr_fail(LRH_EINVAL, "lrh_pt_scene_move: bad arguments (an instanced scene, n >= 0)");
+    std::vector<char> seen((size_t)s->ninst, 0);
+    for (int k = 0; k < n; k++) {
+        if (idx[k] < 0 || idx[k] >= s->ninst) return lr_fail(LRH_EINVAL, "lrh_pt_scene_move: instance %d of %d", idx[k], s->ninst);
+        if (seen[(size_t)idx[k]]) return lr_fail(LRH_EINVAL, "lrh_pt_scene_move: instance %d given twice", idx[k]);
+        seen[(size_t)idx[k]] = 1;
+    }
+    if (n == 0) {   // nothing moved: nothing sent, the top level stands
+        if (stats) memset(stats, 0, sizeof(*stats));
+        return 0;
+    }
+    std::vector<float> rec, fwd;
+    int rc = pack_instances(s, inst, n, s->mroot_h, s->mroot4_h, rec, fwd);
+    if (rc) return rc;
+    const size_t bi = lr_align((size_t)n * 4), br = (size_t)n * 64, bf = (size_t)n * 48;
+    std::vector<char> host(bi + br + bf);
+    memcpy(host.data(), idx, (size_t)n * 4);
+    memcpy(host.data() + bi, rec.data(), br);
+    memcpy(host.data() + bi + br, fwd.data(), bf);
+    LrhContext *c = s->ctx;
+    LR_HIP(hipStreamSynchronize(c->stream));
+    char *stage = nullptr;
+    if (hipMalloc((void **)&stage, host.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        return lr_fail(LRH_ENOMEM, "lrh_pt_scene_move: device memory for %d records could not be had", n);
+    }
+    rc = 0;
+    if (hipMemcpy(stage, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
+        rc = lr_fail(LRH_EDEVICE, "lrh_pt_scene_move: the upload failed");
+    if (!rc) {
+        hipLaunchKernelGGL(k_scatter_inst, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (const int *)stage,
+                           (const float4 *)(stage + bi), (const float4 *)(stage + bi + br), n, s->inst, s->ifwd);
+        if (hipGetLastError() != hipSuccess) rc = lr_fail(LRH_EDEVICE, "lrh_pt_scene_move: a launch failed");
+    }
+    if (!rc) rc = rebuild_top(s, builder, stats);   // (waits for the stream)
+    else (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(stage);
+    return rc;
+}
+
+extern "C" int lrh_pt_scene_mesh(LrhPtScene *s, int mesh, const float *tris, int ntris, int builder, LrhBuildStats *stats)
+{
+    if (!s || !s->two_level || mesh < 0 || mesh >= s->nmeshes || !tris || (builder != LRH_LBVH && builder != LRH_PLOC))
+        return lr_fail(LRH_EINVAL, "lrh_pt_scene_mesh: bad arguments (an instanced scene, one of its meshes)");
+    const int f = s->mfirst[(size_t)mesh], n = s->mfirst[(size_t)mesh + 1] - f;
+    if (ntris != n) return lr_fail(LRH_EINVAL, "lrh_pt_scene_mesh: mesh %d has %d triangles, not %d", mesh, n, ntris);
+    for (int i = 0; i < n; i++) {
+        uint32_t w;
+        memcpy(&w, tris + 12 * (size_t)i + 11, 4);
+        if ((int)(w >> 26) >= s->nmats) return lr_fail(LRH_EINVAL, "lrh_pt_scene_mesh: triangle %d: material %u", i, w >> 26);
+    }
+    LrhContext *c = s->ctx;
+    const uint32_t root = s->mroot_h[(size_t)mesh];
+    LR_HIP(hipStreamSynchronize(c->stream));
+    LR_HIP(hipMemcpy(s->tris + 3 * (size_t)f, tris, (size_t)n * 48, hipMemcpyHostToDevice));
+    // its boxes, its tree in its place (records from its root, aux entries from its first
+    // triangle), the records' child boxes, its 4-wide copy from its first wide slot, its
+    // triangles in leaf order
+    int rc = lrh_tri_boxes(c, (const float *)(s->tris + 3 * (size_t)f), n, (float *)(s->mboxes + 2 * (size_t)f));
+    LrhBuildStats st;
+    if (!rc) rc = lrh_build(c, (const float *)(s->mboxes + 2 * (size_t)f), n, nullptr, builder, root, (uint32_t)f, (uint32_t)f,
+                            s->mnodes, s->maux, nullptr, &st);
+    if (rc) return rc;
+    const int nn = 2 * n - 1;
+    if (n == 1) hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, c->stream, (const float4 *)s->mnodes, root, s->mwnodes);
+    else
+        hipLaunchKernelGGL(k_child_boxes, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, (const float4 *)s->mnodes,
+                           (int)root, nn, s->mwnodes);
+    LR_HIP(hipGetLastError());
+    rc = collapse_wide(c, s->mnodes, root, n, st.depth, s->mw4nodes, (uint32_t)f);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_leaf_tris, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, s->maux + f, n, 0u,
+                       (const float4 *)s->tris, s->ltris + 3 * (size_t)f);
+    LR_HIP(hipGetLastError());
+    s->mdepth[(size_t)mesh] = st.depth;
+    s->mesh_depth = 0;
+    for (int d : s->mdepth) s->mesh_depth = d > s->mesh_depth ? d : s->mesh_depth;
+    // the instances' world boxes follow the mesh's: the top level again, with its own builder
+    rc = rebuild_top(s, s->top_builder, nullptr);
+    if (rc) return rc;
     if (stats) *stats = st;
     return 0;
 }
@@ -585,7 +692,7 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
     }
     if (!rc && nnodes)
         hipLaunchKernelGGL(k_child_boxes, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, c->stream,
-                           (const float4 *)s->mnodes, (int)nnodes, s->mwnodes);
+                           (const float4 *)s->mnodes, 0, (int)nnodes, s->mwnodes);
     // each mesh 4-wide, its wide nodes from the mesh's first triangle on (a mesh of n
     // triangles has at most n wide nodes), and the triangles in the meshes' leaf order
     std::vector<uint32_t> roots4(nmeshes);
@@ -608,6 +715,10 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
     // the one-shape meshes' root records were written before the pass over all inner nodes,
     // which skips leaves: they stay
     s->mesh_depth = depth;
+    s->mfirst.assign(mesh_first, mesh_first + nmeshes + 1);
+    s->mdepth = mdepth;
+    s->mroot_h = roots;
+    s->mroot4_h = roots4;
     s->bytes = (size_t)M * (48 + 32 + 4 + 64 + 48) + nnodes * (32 + 64) + hm.size() * 4 + tab.size() * 4 + mask.size() * 2 +
                (size_t)ninst * (64 + 48 + 32 + 4 + 64) + tn * (32 + 64) + (size_t)nmeshes * 8;
     rc = lrh_pt_scene_update(s, inst, ninst, LRH_LBVH, nullptr);
