@@ -519,7 +519,7 @@ def bench_instanced(a, torch, lr):
     """The instanced workload: 2000 dynamic instances (960-triangle spheres, boxes, 64-point
     hulls) bouncing over a 40 x 40 ground, 3000 static props from 20 seeded blobs of 500-20k
     triangles, 14 GGX materials (two glass), a procedural 2048 x 1024 sky with a 5e4 sun. One
-    step = one frame: new poses, the top-level BVH rebuilt on the device, a 1280 x 720, 1-spp,
+    step = one frame: new poses (the moving instances' records only), the top-level BVH rebuilt on the device, a 1280 x 720, 1-spp,
     8-bounce image. value = rays traced per second of frame time. --frames-in-flight F keeps F
     frames rendering at once, each with its own copy of the scene (frame k's top level is built
     while frame k - 1 renders): more rays per iteration, the frames' latency F times one frame's."""
@@ -549,6 +549,14 @@ def bench_instanced(a, torch, lr):
 
     builder = lr.PLOC if a.tlas == "ploc" else lr.LBVH
     in_flight = {}   # scene copy -> the job rendering it
+    # the instances that move (the dynamic ones; the props and the ground keep their poses): a
+    # frame sends only their records (lrh_pt_scene_move), then rebuilds the top level
+    i0 = w0["instances"]
+    moving = np.zeros(len(i0), bool)
+    for fr in frames[1:]:
+        x = fr["instances"]
+        moving |= (x["xform"] != i0["xform"]).any(1) | (x["kind"] != i0["kind"]) | (x["material"] != i0["material"])
+    moving = np.nonzero(moving)[0].astype(np.int32)
 
     def settle(si):
         """wait for the frame rendering scene copy si, then let it go"""
@@ -562,7 +570,7 @@ def bench_instanced(a, torch, lr):
     def frame(k):
         si = k % F
         settle(si)
-        st = scenes[si].update(frames[k]["instances"], builder=builder)
+        st = scenes[si].move(moving, frames[k]["instances"][moving], builder=builder)
         in_flight[si] = eng.submit(lr.make_job(scenes[si], frames[k]["cam"], W, H, 1, seed=k, max_depth=8))
         eng.pump(1)
         return st
@@ -622,6 +630,7 @@ def bench_instanced(a, torch, lr):
                    "scene_bytes": int(nbytes), "walk_depth": depth, "tune": tune.as_dict(),
                    "frames_in_flight": F},
         "tlas_builder": a.tlas,
+        "records_sent_per_frame": int(len(moving)),
         "tlas_build_ms": round(float(np.mean(builds)), 4),
         "tlas_build_mprims_s": round(ninst / (float(np.mean(builds)) * 1e3), 2),
         "kernel_ms_per_launch": {n: round(kms[n] / max(kl[n], 1), 4) for n in names},

@@ -851,11 +851,13 @@ __global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t
     }
 }
 
-// small (trace_ploc.wgsl:236-301): the rounds left, in one workgroup. While more than
-// SMALL_MAX clusters are left (only when the rounds before could not merge enough: boxes that
-// tie everywhere merge one pair a round) a round goes through device memory; from SMALL_MAX
-// down, the list, its boxes (six arrays of floats) and heights live in LDS, the nearest search
-// reads its 2R neighbours there, and only the merged nodes' records go out.
+// small (trace_ploc.wgsl:236-301): the rounds left, in one workgroup, from 1024 clusters (the
+// context's ploc_small: below it a round's launch costs more than its work; above it the
+// nearest search, bound by one compute unit's VALU, is better spread over the chip). While
+// more than SMALL_MAX clusters are left (only when the rounds before could not merge enough:
+// boxes that tie everywhere merge one pair a round) a round goes through device memory; from
+// SMALL_MAX down, the list, its boxes (six arrays of floats) and heights live in LDS, the
+// nearest search reads its 2R neighbours there, and only the merged nodes' records go out.
 constexpr int SMALL_BLOCK = 1024;
 constexpr int SMALL_MAX = 4096;
 constexpr int SMALL_PER = SMALL_MAX / SMALL_BLOCK;   // positions a thread holds in an LDS round
@@ -1185,7 +1187,7 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
                            reinterpret_cast<const float4 *>(boxes), n, ref0, ref_add, ax, bx, l0, height, pst, ps,
                            L.pstatus_words);
         LR_HIP(hipGetLastError());
-        const int small = std::min(c->ploc_small > 0 ? c->ploc_small : SMALL_MAX, SMALL_MAX);
+        const int small = std::min(c->ploc_small > 0 ? c->ploc_small : 1024, SMALL_MAX);
         int nr = ploc_rounds(n, small);
         if (nr > PLOC_MAX_ROUNDS) nr = PLOC_MAX_ROUNDS;
         double est = n;

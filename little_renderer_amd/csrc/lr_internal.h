@@ -30,7 +30,7 @@ struct LrhContext {
     LrScratch scratch[LR_SCRATCH_COUNT];
     uint32_t *flags = nullptr;   // device words: [0] a walk's stack overflowed (never expected)
     hipEvent_t ev[2] = {nullptr, nullptr};
-    int ploc_small = 4096;   // PLOC: the clusters from which the rounds run in one work-group
+    int ploc_small = 1024;   // PLOC: the clusters from which the rounds run in one work-group
     float last_build_ms = 0.0f;
     LrhEngine *engine = nullptr;  // lrh_pt_render's engine, made on first use
 };
