@@ -431,8 +431,9 @@ __device__ __forceinline__ void put_node_buf(__amdgpu_buffer_rsrc_t rs, int idx,
     __builtin_amdgcn_raw_buffer_store_b128(hi, rs, idx * 32 + 16, 0, CPOL_SC1);
 }
 
-// min-scans over a block of 256 (4 waves): forward (the minimum of this and every earlier
-// thread's value) and backward (this and every later thread's)
+// min-scans over a group of NT threads (NT / 64 waves): forward (the minimum of this and every
+// earlier thread's value) and backward (this and every later thread's)
+template <int NT>
 __device__ __forceinline__ int block_min_scan(int v, bool backward, int *s_w)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -444,7 +445,7 @@ __device__ __forceinline__ int block_min_scan(int v, bool backward, int *s_w)
     if (lane == (backward ? 0 : 63)) s_w[w] = v;
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < BLOCK / 64; k++) {
+    for (int k = 0; k < NT / 64; k++) {
         const int t = s_w[k];
         if (backward ? k > w : k < w) v = t < v ? t : v;
     }
@@ -457,14 +458,14 @@ __device__ __forceinline__ int block_min_scan(int v, bool backward, int *s_w)
 // second joins their boxes (min / max of left, right), writes the parent with Karras's
 // numbering (internal i at node0 + i, leaf j at node0 + n - 1 + j) and goes on.
 //
-// Two kernels. k_lbvh: a work-group's 256 leaves climb through the nodes whose key ranges lie
+// Two kernels. k_lbvh: a work-group's LBVH_GROUP leaves climb through the nodes whose key ranges lie
 // inside the group's: the two children's threads are then both in the group, and hand their
 // boxes over in LDS (a slot per side, an LDS arrival counter). Whether a node's parent lies
 // inside is decided from the adjacent-key deltas d(k) alone: a left child [l, p]'s parent ends
 // at the first k > p with d(k) < d(p), a right child [p + 1, r]'s starts after the last k < p
 // with d(k) < d(p); a min-scan of the group's deltas each way answers both with one
 // comparison. A node whose parent lies outside (the root of a largest subtree inside the
-// group: at most 256 of them, about 8 on random keys) is written and listed in the group's
+// group: at most LBVH_GROUP of them, about 10 on random keys) is written and listed in the group's
 // slots. k_lbvh_top then climbs from the listed nodes, one wave per group's list, through
 // device memory: the two children's threads are on any XCDs, whose L2s are not coherent, so a
 // record is stored through to the coherence point (sc1), the stores are waited for (vmcnt 0),
@@ -472,23 +473,24 @@ __device__ __forceinline__ int block_min_scan(int v, bool backward, int *s_w)
 // one 64-bit word it swaps in; the second thread reads the record back with sc1 loads (a leaf
 // child's box from the input boxes). Kept apart, the few long climbs through device memory
 // do not hold the groups of the first kernel, and their LDS, on the compute units.
-constexpr int LBVH_SLOTS = BLOCK;   // listed nodes per group, at most
+constexpr int LBVH_GROUP = 512;      // leaves per work-group of k_lbvh (1024 measured 13 % slower: one per CU)
+constexpr int LBVH_SLOTS = LBVH_GROUP;   // listed nodes per group, at most
 
-__global__ __launch_bounds__(BLOCK) void k_lbvh(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ items,
+__global__ __launch_bounds__(LBVH_GROUP) void k_lbvh(const uint32_t *__restrict__ codes, const uint32_t *__restrict__ items,
                                                 const float4 *__restrict__ boxes, int n, uint32_t node0, uint32_t ref0,
                                                 uint32_t ref_add, LrNode *__restrict__ nodes, uint32_t *__restrict__ aux,
                                                 uint4 *__restrict__ esc, uint32_t *__restrict__ esc_count,
                                                 uint32_t *depth_out)
 {
-    __shared__ int s_d[BLOCK + 1];        // d(B0 - 1 + i); -1 (below every delta) off the keys' ends
-    __shared__ int s_pmin[BLOCK + 1];     // min of s_d[0 .. i]
-    __shared__ int s_smin[BLOCK + 1];     // min of s_d[i .. BLOCK] (i >= 1)
-    __shared__ float s_box[BLOCK][2][6];  // per parent split, per side: the arriving child's box
-    __shared__ uint32_t s_end[BLOCK][2], s_h[BLOCK][2], s_arrive[BLOCK];
-    __shared__ int s_w[BLOCK / 64];
+    __shared__ int s_d[LBVH_GROUP + 1];        // d(B0 - 1 + i); -1 (below every delta) off the keys' ends
+    __shared__ int s_pmin[LBVH_GROUP + 1];     // min of s_d[0 .. i]
+    __shared__ int s_smin[LBVH_GROUP + 1];     // min of s_d[i .. LBVH_GROUP] (i >= 1)
+    __shared__ float s_box[LBVH_GROUP][2][6];  // per parent split, per side: the arriving child's box
+    __shared__ uint32_t s_end[LBVH_GROUP][2], s_h[LBVH_GROUP][2], s_arrive[LBVH_GROUP];
+    __shared__ int s_w[LBVH_GROUP / 64];
     __shared__ uint32_t s_nesc;
     const int t = threadIdx.x;
-    const int B0 = blockIdx.x * BLOCK;
+    const int B0 = blockIdx.x * LBVH_GROUP;
     const int j = B0 + t;
     const bool live = j < n;
     float lx = 0.0f, ly = 0.0f, lz = 0.0f, hx = 0.0f, hy = 0.0f, hz = 0.0f;
@@ -516,14 +518,14 @@ __global__ __launch_bounds__(BLOCK) void k_lbvh(const uint32_t *__restrict__ cod
     s_arrive[t] = 0u;
     __syncthreads();
     const int d0 = s_d[0];
-    const int pm = block_min_scan(dj, false, s_w);
-    const int sm = block_min_scan(dj, true, s_w);
+    const int pm = block_min_scan<LBVH_GROUP>(dj, false, s_w);
+    const int sm = block_min_scan<LBVH_GROUP>(dj, true, s_w);
     s_pmin[t + 1] = pm < d0 ? pm : d0;
     s_smin[t + 1] = sm;
     if (t == 0) s_pmin[0] = d0;
     __syncthreads();
 
-    // d(k) = s_d[k - B0 + 1] for B0 - 1 <= k <= B0 + BLOCK - 1
+    // d(k) = s_d[k - B0 + 1] for B0 - 1 <= k <= B0 + LBVH_GROUP - 1
     int l = j, r = j;      // the node reached: keys [l, r] (a leaf while l == r)
     int lc = 0, rc = 0;    // an inner node's children
     uint32_t h = 1;
@@ -532,7 +534,7 @@ __global__ __launch_bounds__(BLOCK) void k_lbvh(const uint32_t *__restrict__ cod
     while (climbing) {
         const int p = left ? r : l - 1;
         const int dp = s_d[p - B0 + 1];
-        const bool inside = left ? (p <= B0 + BLOCK - 2 && s_smin[p - B0 + 2] < dp) : (p >= B0 && s_pmin[p - B0] < dp);
+        const bool inside = left ? (p <= B0 + LBVH_GROUP - 2 && s_smin[p - B0 + 2] < dp) : (p >= B0 && s_pmin[p - B0] < dp);
         if (l != r) put_node(&nodes[left ? r : l], lx, ly, lz, node0 + (uint32_t)lc, hx, hy, hz, node0 + (uint32_t)rc);
         if (!inside) {   // listed for k_lbvh_top (its record, or its leaf's, holds its box)
             const uint32_t k = atomicAdd(&s_nesc, 1u);
@@ -1068,7 +1070,7 @@ Layout layout(int n)
     L.other = take(N * 8);
     L.zero_words = (o - L.zero) / 4;
     // PLOC's; an LBVH build's listed nodes (k_lbvh's slots, 16 bytes each) and their counts
-    const size_t groups = (N + BLOCK - 1) / BLOCK;
+    const size_t groups = (N + LBVH_GROUP - 1) / LBVH_GROUP;
     L.height = take(std::max(8 * N, groups * 4));
     L.bx = take(std::max((2 * N) * 32, groups * LBVH_SLOTS * 16));
     L.list0 = take(N * 4);
@@ -1161,10 +1163,10 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
     int rounds = 0;
     const bool ploc = builder == LRH_PLOC && n > 2;
     if (!ploc) {
-        const int groups = (n + BLOCK - 1) / BLOCK;
+        const int groups = (n + LBVH_GROUP - 1) / LBVH_GROUP;
         uint4 *esc = (uint4 *)at(L.bx);
         uint32_t *esc_count = (uint32_t *)at(L.height);
-        hipLaunchKernelGGL(k_lbvh, dim3(groups), dim3(BLOCK), 0, s, k0, v0, reinterpret_cast<const float4 *>(boxes), n, node0,
+        hipLaunchKernelGGL(k_lbvh, dim3(groups), dim3(LBVH_GROUP), 0, s, k0, v0, reinterpret_cast<const float4 *>(boxes), n, node0,
                            ref0, ref_add, nd, ax, esc, esc_count, depth);
         LR_HIP(hipGetLastError());
         // the climbs above the groups; the buffer path while the records lie within 2 GiB of
