@@ -417,8 +417,11 @@ __global__ __launch_bounds__(BLK) void k_regen(EngArgs A)
 // its own job's tree. Also applies regen's claim to the ring and the issue counter, and clears
 // the queues the shade kernel appends to. TL: two-level scenes may be queued (without it, such
 // a ray is not walked: flagged, so that the host reports it).
+// Without TL the kernel is held to 64 VGPRs, 8 waves per SIMD: a few registers spill (12 bytes a
+// lane), and the walk, bound by the latency of its node fetches, gains more from the waves (the
+// whole persistent grid of 8 work-groups per compute unit is then resident).
 template <int BLK, int STK, bool COUNT, bool TL>
-__global__ __launch_bounds__(BLK) void k_extend(EngArgs A)
+__global__ __launch_bounds__(BLK, TL ? 1 : 8) void k_extend(EngArgs A)
 {
     __shared__ uint32_t stk[STK][BLK];
     EngCtl *ctl = A.ctl;
@@ -669,9 +672,9 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
 
 // shadow (wf_shadow, trace_wave.wgsl:335-355): any hit before the lamp; if none, the
 // contribution joins the path's radiance. Paths whose last vertex made the shadow ray end here.
-// TL as for extend.
+// TL and the register bound as for extend.
 template <int BLK, int STK, bool TL>
-__global__ __launch_bounds__(BLK) void k_shadow(EngArgs A)
+__global__ __launch_bounds__(BLK, TL ? 1 : 8) void k_shadow(EngArgs A)
 {
     __shared__ uint32_t stk[STK][BLK];
     EngCtl *ctl = A.ctl;
