@@ -258,7 +258,7 @@ typedef struct {
     uint64_t kernel_launches[4];
 } LrhEngineStats;
 
-/* host clock (steady, ms) that job times are given in */
+/* host clock (steady, ms) that job times are given in (clock_ms, engine/base/clock.h:6) */
 double lrh_now_ms(void);
 
 /* slots_log2 in 10..22; max_tree_depth: the deepest tree any job's scene may have (its
