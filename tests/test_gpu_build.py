@@ -150,6 +150,34 @@ def test_build_full_size_and_deterministic(ctx, builder, n):
     say(f"  depth {depth} build {a['st'].ms:.3f} ms")
 
 
+def test_lbvh_past_the_buffer_path(ctx):
+    """34M primitives: (2n - 1) x 32 bytes of records past 2 GiB, where the climbs above the
+    work-groups hand records over with 8-byte device-scope accesses instead of the 16-byte buffer
+    ones. The sorted output is checked as such (order, a permutation, each code the oracle's
+    Morton code of its item) and the tree against the oracle's LBVH over it, bit for bit."""
+    n = 34_000_000
+    assert (2 * n - 1) * 32 >= 2 ** 31
+    say(f"LBVH n={n}")
+    tris = soup(n, 41)
+    g = gpu_build(ctx, tris, lr.LBVH)
+    srt = u32(g["srt"])
+    sc, si = srt[:n].copy(), srt[n:].copy()
+    del srt
+    boxes = host(g["boxes"])
+    assert np.all(sc[1:] >= sc[:-1])
+    tie = sc[1:] == sc[:-1]
+    assert np.all(si[1:][tie] > si[:-1][tie])
+    assert np.array_equal(np.bincount(si, minlength=n), np.ones(n, np.int64))
+    lo, hi = lro.union(boxes)
+    assert np.array_equal(lro.morton(boxes, lo, lro.inv_extent(lo, hi))[si], sc)
+    nodes, aux, depth = lro.lbvh(sc, si, boxes)
+    assert np.array_equal(u32(g["aux"]), aux)
+    got = u32(g["nodes"])
+    assert np.array_equal(got, nodes), f"nodes differ at {np.nonzero((got != nodes).any(1))[0][:8]}"
+    assert g["st"].depth == depth
+    say(f"  depth {depth} build {g['st'].ms:.3f} ms")
+
+
 def aimed_rays(tris, n, seed):
     rng = np.random.default_rng(seed)
     o = rng.uniform(-0.3, 1.3, size=(n, 3)).astype(np.float32)
