@@ -44,7 +44,8 @@ def parse():
     ap.add_argument("--tune", default=None, metavar="BT,STK,BS,BPC,TILE,IPG",
                     help="batched: this launch shape instead of the autotuner's")
     ap.add_argument("--skip-plain", action="store_true", help="batched: leave out the plain per-job comparison")
-    ap.add_argument("--tlas", choices=["lbvh", "ploc"], default="ploc", help="instanced: the per-frame top-level builder")
+    ap.add_argument("--tlas", choices=["lbvh", "ploc", "refit"], default="ploc",
+                    help="instanced: the per-frame top-level builder (refit: the same tree while its cost stays within 1.1)")
     ap.add_argument("--frames-in-flight", type=int, default=2, help="instanced: frames rendering at once")
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--build-sizes", default="4000000,16000000",
@@ -547,7 +548,7 @@ def bench_instanced(a, torch, lr):
         eng.autotune([lr.make_job(ps, w0["cam"], W, H, 1, seed=99)])
     tune = eng.tune
 
-    builder = lr.PLOC if a.tlas == "ploc" else lr.LBVH
+    builder = {"ploc": lr.PLOC, "lbvh": lr.LBVH, "refit": lr.REFIT}[a.tlas]
     in_flight = {}   # scene copy -> the job rendering it
     # the instances that move (the dynamic ones; the props and the ground keep their poses): a
     # frame sends only their records (lrh_pt_scene_move), then rebuilds the top level
@@ -557,6 +558,9 @@ def bench_instanced(a, torch, lr):
         x = fr["instances"]
         moving |= (x["xform"] != i0["xform"]).any(1) | (x["kind"] != i0["kind"]) | (x["material"] != i0["material"])
     moving = np.nonzero(moving)[0].astype(np.int32)
+    if builder == lr.REFIT:   # refits start from (and fall back to) a PLOC top level
+        for s_ in scenes:
+            s_.move(moving, w0["instances"][moving], builder=lr.PLOC)
 
     def settle(si):
         """wait for the frame rendering scene copy si, then let it go"""
@@ -584,10 +588,12 @@ def bench_instanced(a, torch, lr):
     finish()
     torch.cuda.synchronize()
     s0 = eng.stats()
-    builds = []
+    builds, refits = [], 0
     t0 = time.perf_counter()
     for k in range(warm, warm + steps):
-        builds.append(frame(k).ms)
+        st = frame(k)
+        builds.append(st.ms)
+        refits += st.refitted
     finish()
     dt = time.perf_counter() - t0
     s1 = eng.stats()
@@ -631,6 +637,7 @@ def bench_instanced(a, torch, lr):
                    "frames_in_flight": F},
         "tlas_builder": a.tlas,
         "records_sent_per_frame": int(len(moving)),
+        "tlas_refitted_frames": refits,
         "tlas_build_ms": round(float(np.mean(builds)), 4),
         "tlas_build_mprims_s": round(ninst / (float(np.mean(builds)) * 1e3), 2),
         "kernel_ms_per_launch": {n: round(kms[n] / max(kl[n], 1), 4) for n in names},

@@ -31,6 +31,10 @@ extern "C" {
 
 #define LRH_LBVH 0
 #define LRH_PLOC 1
+/* an instanced scene's top level only: the same tree over the instances' new boxes while its cost
+ * stays within 1.1 of its last build's, else built again with that build's builder (LR_TLAS_REFIT,
+ * engine/rt/accel.c:152-173) */
+#define LRH_REFIT 2
 
 typedef struct LrhContext LrhContext;
 
@@ -40,6 +44,7 @@ typedef struct {
     float ms;        /* device time of the build, hipEvent-timed on the context's stream */
     uint64_t clusters; /* PLOC: the live clusters summed over its rounds, the count its traffic
                           is priced by (0 for an LBVH) */
+    int refitted;    /* LRH_REFIT: 1 when the tree was kept and refitted, 0 when built again */
 } LrhBuildStats;
 
 /* The message of the last failed call on this host thread. */
@@ -174,8 +179,8 @@ typedef struct {
 int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const int *mesh_first, int nmeshes, const LrhMaterial *mats,
                            int nmats, const LrhInstance *inst, int ninst, float eps, int builder, LrhPtScene **out);
 /* New poses for the same instances: the top level is rebuilt on the device (per frame, as the
- * reference rebuilds its TLAS, trace_gpu.c:1371-1379). No job of the scene may be running.
- * stats (optional): the top level's build. */
+ * reference rebuilds its TLAS, trace_gpu.c:1371-1379), or refitted (builder LRH_REFIT). No job
+ * of the scene may be running. stats (optional): the top level's build. */
 int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int ninst, int builder, LrhBuildStats *stats);
 /* Only the instances that moved since the last update: inst[k] is instance idx[k] (n of them,
  * each index once); their records alone are sent, in one upload scattered into place on the
@@ -192,6 +197,10 @@ int lrh_pt_scene_move(LrhPtScene *s, const int *idx, const LrhInstance *inst, in
  * trace_gpu.c:1486-1512). The result equals a scene made with these triangles. No job of the
  * scene may be running. stats (optional): the mesh's build. */
 int lrh_pt_scene_mesh(LrhPtScene *s, int mesh, const float *tris, int ntris, int builder, LrhBuildStats *stats);
+/* The top level of an instanced scene to the host: its 2 ninst - 1 binary records (lr_hip.h's node
+ * layout) and ninst aux entries (instance ids), and its cost (bvh_cost, engine/rt/bvh.c:464-469:
+ * the records' box areas over the root's); any of the three may be NULL. */
+int lrh_pt_scene_top(LrhPtScene *s, uint32_t *h_nodes, uint32_t *h_aux, double *cost);
 
 /* Path traces spp samples (indices s0 .. s0+spp-1) of every pixel of a W x H image and adds
  * them to d_accum (float4 per pixel, row-major: the sum of the samples' rgb, and their count),

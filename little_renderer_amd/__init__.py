@@ -13,6 +13,7 @@ __all__ = ["lib", "Context", "LrhError", "gen_soup", "camera_make", "LBVH", "PLO
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblr_hip.so")
 LBVH, PLOC = 0, 1
+REFIT = 2        # an instanced scene's top level: refitted while its cost stays within 1.1 of its build's
 
 _lib = None
 
@@ -33,7 +34,7 @@ MAT_DIFFUSE, MAT_MIRROR, MAT_GLASS, MAT_LIGHT, MAT_PBR = 0, 1, 2, 3, 4
 
 class BuildStats(ctypes.Structure):
     _fields_ = [("depth", ctypes.c_int), ("rounds", ctypes.c_int), ("ms", ctypes.c_float),
-                ("clusters", ctypes.c_uint64)]
+                ("clusters", ctypes.c_uint64), ("refitted", ctypes.c_int)]
 
 
 class Tune(ctypes.Structure):
@@ -110,6 +111,7 @@ def lib():
         "lrh_pt_scene_update": (I, [P, P, I, I, ctypes.POINTER(BuildStats)]),
         "lrh_pt_scene_move": (I, [P, P, P, I, I, ctypes.POINTER(BuildStats)]),
         "lrh_pt_scene_mesh": (I, [P, I, P, I, I, ctypes.POINTER(BuildStats)]),
+        "lrh_pt_scene_top": (I, [P, P, P, ctypes.POINTER(ctypes.c_double)]),
         "lrh_pt_scene_trace": (I, [P, P, P, I, I, P]),
         "lrh_pt_scene_info": (I, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_size_t)]),
@@ -250,6 +252,7 @@ class PtScene:
         self._h = ctypes.c_void_p()
         _check(lib().lrh_pt_scene_instanced(ctx._c, _ptr(mt), _ptr(mf), len(mf) - 1, ctypes.cast(mats, ctypes.c_void_p),
                                             len(mats), _ptr(inst), len(inst), eps, builder, ctypes.byref(self._h)))
+        self.ninst = len(inst)
         return self
 
     def __init__(self, ctx, prims, tris, mats, light, eps, margin, builder=PLOC):
@@ -297,6 +300,16 @@ class PtScene:
         st = BuildStats()
         _check(lib().lrh_pt_scene_mesh(self._h, m, _ptr(t), len(t), builder, ctypes.byref(st)))
         return st
+
+    def top(self):
+        """lrh_pt_scene_top: (records (2 ninst - 1, 8) as u32, aux (ninst,) as u32, cost)"""
+        import numpy as np
+        n = self.ninst
+        nodes = np.zeros((2 * n - 1, 8), np.uint32)
+        aux = np.zeros(n, np.uint32)
+        cost = ctypes.c_double()
+        _check(lib().lrh_pt_scene_top(self._h, _ptr(nodes), _ptr(aux), ctypes.byref(cost)))
+        return nodes, aux, cost.value
 
     def trace(self, rays, any_hit=False):
         """lrh_pt_scene_trace on host rays (n, 8): hits (n, 4) as a numpy array"""

@@ -1220,6 +1220,7 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
             LR_HIP(hipMemcpy(&live, (const char *)at(L.pst) + offsetof(PlocState, live), 8, hipMemcpyDeviceToHost));
         }
         stats->clusters = live;
+        stats->refitted = 0;
         stats->depth = (int)dep;
         stats->rounds = ploc ? rounds + (int)small_rounds : 0;
         stats->ms = ms;

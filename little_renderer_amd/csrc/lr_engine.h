@@ -157,6 +157,9 @@ struct LrhPtScene {
     // a two-level scene (lrh_pt_scene_instanced): meshes, instances, the top level
     int two_level = 0, nmeshes = 0, ninst = 0, mesh_depth = 0, tlas_depth = 0;
     int top_builder = 0;                      // the top level's last builder (a mesh's update builds it again so)
+    uint32_t *tparent = nullptr, *tarrive = nullptr;   // the top level's records' parents; refit arrivals
+    double *tcost_dev = nullptr;              // a cost computed on the device
+    double tcost_build = 0.0;                 // the top level's cost at its last build (the refit rule's)
     float4 *mtris = nullptr, *mboxes = nullptr, *mwnodes = nullptr, *inst = nullptr, *ifwd = nullptr;
     float4 *mw4nodes = nullptr, *tw4nodes = nullptr;   // the meshes' and the top level's 4-wide nodes
     uint32_t *mroot4 = nullptr;                        // each mesh's root wide slot

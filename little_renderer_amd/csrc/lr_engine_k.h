@@ -420,8 +420,15 @@ __global__ __launch_bounds__(BLK) void k_regen(EngArgs A)
 // Without TL the kernel is held to 64 VGPRs, 8 waves per SIMD: a few registers spill (12 bytes a
 // lane), and the walk, bound by the latency of its node fetches, gains more from the waves (the
 // whole persistent grid of 8 work-groups per compute unit is then resident).
+// waves per SIMD a walk kernel is held to: 8, or fewer where its LDS stack (160 KB per compute
+// unit) allows fewer anyway
+constexpr int walk_waves(int blk, int stk)
+{
+    return (163840 / (stk * blk * 4 + 4)) * (blk / 64) / 4 < 8 ? (163840 / (stk * blk * 4 + 4)) * (blk / 64) / 4 : 8;
+}
+
 template <int BLK, int STK, bool COUNT, bool TL>
-__global__ __launch_bounds__(BLK, TL ? 1 : 8) void k_extend(EngArgs A)
+__global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_extend(EngArgs A)
 {
     __shared__ uint32_t stk[STK][BLK];
     EngCtl *ctl = A.ctl;
@@ -674,7 +681,7 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
 // contribution joins the path's radiance. Paths whose last vertex made the shadow ray end here.
 // TL and the register bound as for extend.
 template <int BLK, int STK, bool TL>
-__global__ __launch_bounds__(BLK, TL ? 1 : 8) void k_shadow(EngArgs A)
+__global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(EngArgs A)
 {
     __shared__ uint32_t stk[STK][BLK];
     EngCtl *ctl = A.ctl;

@@ -235,7 +235,7 @@ static void pt_scene_release(LrhPtScene *s)
     if (!s) return;
     void *ps[] = {s->prims, s->tris, s->mats, s->boxes, s->nodes, s->aux, s->tab, s->mask, s->wnodes, s->w4nodes, s->ltris, s->env_rgb,
                   s->env_marg, s->env_cond, s->env_prob, s->mboxes, s->mwnodes, s->inst, s->ifwd, s->mnodes, s->maux,
-                  s->mroot, s->mw4nodes, s->tw4nodes, s->mroot4};
+                  s->mroot, s->mw4nodes, s->tw4nodes, s->mroot4, s->tparent, s->tarrive, s->tcost_dev};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete s;
@@ -460,6 +460,121 @@ static int pack_instances(const LrhPtScene *s, const LrhInstance *in, int n, con
     return 0;
 }
 
+// ---- the top level refitted (LRH_REFIT: bvh_refit_boxes and the refit rule of build_tlas,
+// engine/rt/bvh.c:464-483, engine/rt/accel.c:152-173): the same tree over the instances' new
+// boxes while its cost (the nodes' areas over the root's) stays within 1.1 of its last build's ----
+constexpr double LR_REFIT_GROWTH = 1.1;
+
+// each record's parent (the root's: LR_NONE)
+__global__ __launch_bounds__(256) void k_parents(const float4 *__restrict__ nodes, int nn, uint32_t *__restrict__ parent)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    if (i == 0) parent[0] = LR_NONE;
+    const uint32_t a = __float_as_uint(nodes[2 * (size_t)i].w), b = __float_as_uint(nodes[2 * (size_t)i + 1].w);
+    if (b & LR_LEAF) return;
+    parent[a] = (uint32_t)i;
+    parent[b] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_clear32(uint32_t *p, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// From each leaf record (its box: its instances' world boxes) up: at each inner record the first
+// of its two children's threads to arrive stops, the second joins their boxes (left, then right)
+// into it and goes on. Records are handed over through the coherence point (sc1 stores, waited
+// for before the arrival; sc1 loads after it), as the LBVH's climbs are.
+__global__ __launch_bounds__(256) void k_refit(float4 *nodes, int nn, const uint32_t *__restrict__ aux,
+                                               const float4 *__restrict__ boxes, const uint32_t *__restrict__ parent,
+                                               uint32_t *arrive)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    const float4 r0 = nodes[2 * (size_t)i], r1 = nodes[2 * (size_t)i + 1];
+    const uint32_t b = __float_as_uint(r1.w);
+    if (!(b & LR_LEAF)) return;
+    const uint32_t first = __float_as_uint(r0.w), cnt = b & 0x7fffffffu;
+    float lx = 3e38f, ly = 3e38f, lz = 3e38f, hx = -3e38f, hy = -3e38f, hz = -3e38f;
+    for (uint32_t k = 0; k < cnt; k++) {
+        const uint32_t it = aux[first + k];
+        const float4 blo = boxes[2 * (size_t)it], bhi = boxes[2 * (size_t)it + 1];
+        lx = k ? wmin(lx, blo.x) : blo.x; ly = k ? wmin(ly, blo.y) : blo.y; lz = k ? wmin(lz, blo.z) : blo.z;
+        hx = k ? wmax(hx, bhi.x) : bhi.x; hy = k ? wmax(hy, bhi.y) : bhi.y; hz = k ? wmax(hz, bhi.z) : bhi.z;
+    }
+    uint64_t *w = reinterpret_cast<uint64_t *>(nodes);
+    auto put = [&](uint32_t x, uint32_t a, uint32_t bb) {
+        st_dev64(w + 4 * (size_t)x, ((uint64_t)__float_as_uint(ly) << 32) | __float_as_uint(lx));
+        st_dev64(w + 4 * (size_t)x + 1, ((uint64_t)a << 32) | __float_as_uint(lz));
+        st_dev64(w + 4 * (size_t)x + 2, ((uint64_t)__float_as_uint(hy) << 32) | __float_as_uint(hx));
+        st_dev64(w + 4 * (size_t)x + 3, ((uint64_t)bb << 32) | __float_as_uint(hz));
+    };
+    put((uint32_t)i, first, b);
+    uint32_t cur = (uint32_t)i;
+    for (;;) {
+        const uint32_t p = parent[cur];
+        if (p == LR_NONE) return;                   // the root
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);              // this record is out before the arrival
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (__hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+        const uint32_t pa = __float_as_uint(nodes[2 * (size_t)p].w), pb = __float_as_uint(nodes[2 * (size_t)p + 1].w);
+        const uint32_t sib = pa == cur ? pb : pa;
+        const uint64_t q0 = ld_dev64(w + 4 * (size_t)sib), q1 = ld_dev64(w + 4 * (size_t)sib + 1);
+        const uint64_t q2 = ld_dev64(w + 4 * (size_t)sib + 2), q3 = ld_dev64(w + 4 * (size_t)sib + 3);
+        const float sx = __uint_as_float((uint32_t)q0), sy = __uint_as_float((uint32_t)(q0 >> 32));
+        const float sz = __uint_as_float((uint32_t)q1);
+        const float tx = __uint_as_float((uint32_t)q2), ty = __uint_as_float((uint32_t)(q2 >> 32));
+        const float tz = __uint_as_float((uint32_t)q3);
+        if (pa == cur) {   // this one is the left child
+            lx = wmin(lx, sx); ly = wmin(ly, sy); lz = wmin(lz, sz);
+            hx = wmax(hx, tx); hy = wmax(hy, ty); hz = wmax(hz, tz);
+        } else {
+            lx = wmin(sx, lx); ly = wmin(sy, ly); lz = wmin(sz, lz);
+            hx = wmax(tx, hx); hy = wmax(ty, hy); hz = wmax(tz, hz);
+        }
+        put(p, pa, pb);
+        cur = p;
+    }
+}
+
+// the tree's cost (bvh_cost, engine/rt/bvh.c:464-469): the records' box areas over the root's, in
+// double, one work-group
+__global__ __launch_bounds__(1024) void k_cost(const float4 *__restrict__ nodes, int nn, double *out)
+{
+    __shared__ double s_sum[1024];
+    double sum = 0.0;
+    for (int i = threadIdx.x; i < nn; i += 1024) {
+        const float4 lo = nodes[2 * (size_t)i], hi = nodes[2 * (size_t)i + 1];
+        const double dx = (double)hi.x - lo.x, dy = (double)hi.y - lo.y, dz = (double)hi.z - lo.z;
+        if (dx >= 0.0 && dy >= 0.0 && dz >= 0.0) sum += dx * dy + dy * dz + dz * dx;
+    }
+    s_sum[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float4 lo = nodes[0], hi = nodes[1];
+        const double dx = (double)hi.x - lo.x, dy = (double)hi.y - lo.y, dz = (double)hi.z - lo.z;
+        const double root = dx * dy + dy * dz + dz * dx;
+        *out = root > 0.0 ? s_sum[0] / root : 0.0;
+    }
+}
+
+static int top_cost(LrhPtScene *s, double *cost)
+{
+    LrhContext *c = s->ctx;
+    hipLaunchKernelGGL(k_cost, dim3(1), dim3(1024), 0, c->stream, (const float4 *)s->tnodes, 2 * s->ninst - 1, s->tcost_dev);
+    LR_HIP(hipGetLastError());
+    LR_HIP(hipMemcpyAsync(cost, s->tcost_dev, 8, hipMemcpyDeviceToHost, c->stream));
+    LR_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 // The top level over the instances again, from their records on the device: their world boxes,
 // the tree over them, its binary child-box records and its 4-wide copy; the walks' stack depth.
 // Waits for it (collapse_wide does).
@@ -481,18 +596,66 @@ static int rebuild_top(LrhPtScene *s, int builder, LrhBuildStats *stats)
     // the top level 4-wide (the closest-hit walk's); collapse_wide waits for it
     rc = collapse_wide(c, s->tnodes, 0u, ninst, st.depth, s->tw4nodes);
     if (rc) return rc;
+    // for later refits: each record's parent, and this build's cost
+    hipLaunchKernelGGL(k_parents, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, (const float4 *)s->tnodes,
+                       (int)nn, s->tparent);
+    LR_HIP(hipGetLastError());
+    rc = top_cost(s, &s->tcost_build);
+    if (rc) return rc;
     s->tlas_depth = st.depth;
     s->top_builder = builder;
     // the two-level walks' stacks: per wide level of either tree at most three pending
     // entries (the binary walk: a sibling and an instance per top-level level), and EXIT
     s->depth = 3 * (s->tlas_depth + s->mesh_depth) + 8;
+    st.refitted = 0;
     if (stats) *stats = st;
     return 0;
 }
 
+// The top level refitted to the instances' records (LRH_REFIT), kept while its cost stays within
+// LR_REFIT_GROWTH of its last build's, else built again with that build's builder
+static int refit_top(LrhPtScene *s, LrhBuildStats *stats)
+{
+    LrhContext *c = s->ctx;
+    const int ninst = s->ninst, nn = 2 * ninst - 1;
+    LR_HIP(hipEventRecord(c->ev[0], c->stream));
+    hipLaunchKernelGGL(k_inst_boxes, dim3((unsigned)((ninst + 255) / 256)), dim3(256), 0, c->stream, s->ifwd, s->inst,
+                       s->mroot, (const float4 *)s->mnodes, ninst, s->tboxes);
+    hipLaunchKernelGGL(k_clear32, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, s->tarrive, nn);
+    hipLaunchKernelGGL(k_refit, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, (float4 *)s->tnodes, nn, s->taux,
+                       (const float4 *)s->tboxes, (const uint32_t *)s->tparent, s->tarrive);
+    LR_HIP(hipGetLastError());
+    double cost = 0.0;
+    int rc = top_cost(s, &cost);
+    if (rc) return rc;
+    if (!(cost <= LR_REFIT_GROWTH * s->tcost_build)) return rebuild_top(s, s->top_builder, stats);
+    hipLaunchKernelGGL(k_child_boxes, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, (const float4 *)s->tnodes, 0,
+                       nn, s->wnodes);
+    if (ninst == 1) hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, c->stream, (const float4 *)s->tnodes, 0u, s->wnodes);
+    LR_HIP(hipGetLastError());
+    rc = collapse_wide(c, s->tnodes, 0u, ninst, s->tlas_depth, s->tw4nodes);   // (waits for it)
+    if (rc) return rc;
+    LR_HIP(hipEventRecord(c->ev[1], c->stream));
+    LR_HIP(hipEventSynchronize(c->ev[1]));
+    float ms = 0.0f;
+    LR_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->depth = s->tlas_depth;
+        stats->ms = ms;
+        stats->refitted = 1;
+    }
+    return 0;
+}
+
+static int top_again(LrhPtScene *s, int builder, LrhBuildStats *stats)
+{
+    return builder == LRH_REFIT ? refit_top(s, stats) : rebuild_top(s, builder, stats);
+}
+
 extern "C" int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int ninst, int builder, LrhBuildStats *stats)
 {
-    if (!s || !s->two_level || !inst || ninst != s->ninst || (builder != LRH_LBVH && builder != LRH_PLOC))
+    if (!s || !s->two_level || !inst || ninst != s->ninst || (builder != LRH_LBVH && builder != LRH_PLOC && builder != LRH_REFIT))
         return lr_fail(LRH_EINVAL, "lrh_pt_scene_update: bad arguments (an instanced scene, the same instance count)");
     LrhContext *c = s->ctx;
     std::vector<float> rec, fwd;
@@ -501,7 +664,7 @@ extern "C" int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int n
     LR_HIP(hipStreamSynchronize(c->stream));
     LR_HIP(hipMemcpy(s->inst, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
     LR_HIP(hipMemcpy(s->ifwd, fwd.data(), fwd.size() * 4, hipMemcpyHostToDevice));
-    return rebuild_top(s, builder, stats);
+    return top_again(s, builder, stats);
 }
 
 // moved instances' records, packed in one upload (index, 4 float4 record, 3 float4 rows each),
@@ -520,7 +683,8 @@ __global__ __launch_bounds__(256) void k_scatter_inst(const int *__restrict__ id
 extern "C" int lrh_pt_scene_move(LrhPtScene *s, const int *idx, const LrhInstance *inst, int n, int builder,
                                  LrhBuildStats *stats)
 {
-    if (!s || !s->two_level || n < 0 || (n > 0 && (!idx || !inst)) || (builder != LRH_LBVH && builder != LRH_PLOC))
+    if (!s || !s->two_level || n < 0 || (n > 0 && (!idx || !inst)) ||
+        (builder != LRH_LBVH && builder != LRH_PLOC && builder != LRH_REFIT))
         return lr_fail(LRH_EINVAL, "lrh_pt_scene_move: bad arguments (an instanced scene, n >= 0)");
     std::vector<char> seen((size_t)s->ninst, 0);
     for (int k = 0; k < n; k++) {
@@ -555,10 +719,22 @@ extern "C" int lrh_pt_scene_move(LrhPtScene *s, const int *idx, const LrhInstanc
                            (const float4 *)(stage + bi), (const float4 *)(stage + bi + br), n, s->inst, s->ifwd);
         if (hipGetLastError() != hipSuccess) rc = lr_fail(LRH_EDEVICE, "lrh_pt_scene_move: a launch failed");
     }
-    if (!rc) rc = rebuild_top(s, builder, stats);   // (waits for the stream)
+    if (!rc) rc = top_again(s, builder, stats);   // (waits for the stream)
     else (void)hipStreamSynchronize(c->stream);
     (void)hipFree(stage);
     return rc;
+}
+
+extern "C" int lrh_pt_scene_top(LrhPtScene *s, uint32_t *nodes, uint32_t *aux, double *cost)
+{
+    if (!s || !s->two_level) return lr_fail(LRH_EINVAL, "lrh_pt_scene_top: an instanced scene");
+    LrhContext *c = s->ctx;
+    const size_t nn = 2 * (size_t)s->ninst - 1;
+    LR_HIP(hipStreamSynchronize(c->stream));
+    if (nodes) LR_HIP(hipMemcpy(nodes, s->tnodes, nn * 32, hipMemcpyDeviceToHost));
+    if (aux) LR_HIP(hipMemcpy(aux, s->taux, (size_t)s->ninst * 4, hipMemcpyDeviceToHost));
+    if (cost) return top_cost(s, cost);
+    return 0;
 }
 
 extern "C" int lrh_pt_scene_mesh(LrhPtScene *s, int mesh, const float *tris, int ntris, int builder, LrhBuildStats *stats)
@@ -668,7 +844,8 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
               dev_upload((void **)&s->taux, nullptr, (size_t)ninst * 4) && dev_upload((void **)&s->wnodes, nullptr, tn * 64) &&
               dev_upload((void **)&s->mw4nodes, nullptr, (size_t)M * 64) && dev_upload((void **)&s->ltris, nullptr, (size_t)M * 48) &&
               dev_upload((void **)&s->tw4nodes, nullptr, (size_t)ninst * 64) &&
-              dev_upload((void **)&s->mroot4, nullptr, (size_t)nmeshes * 4);
+              dev_upload((void **)&s->mroot4, nullptr, (size_t)nmeshes * 4) && dev_upload((void **)&s->tparent, nullptr, tn * 4) &&
+              dev_upload((void **)&s->tarrive, nullptr, tn * 4) && dev_upload((void **)&s->tcost_dev, nullptr, 8);
     if (!ok) {
         (void)hipGetLastError();
         pt_scene_release(s);
@@ -720,7 +897,7 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
     s->mroot_h = roots;
     s->mroot4_h = roots4;
     s->bytes = (size_t)M * (48 + 32 + 4 + 64 + 48) + nnodes * (32 + 64) + hm.size() * 4 + tab.size() * 4 + mask.size() * 2 +
-               (size_t)ninst * (64 + 48 + 32 + 4 + 64) + tn * (32 + 64) + (size_t)nmeshes * 8;
+               (size_t)ninst * (64 + 48 + 32 + 4 + 64) + tn * (32 + 64 + 8) + (size_t)nmeshes * 8 + 8;
     rc = lrh_pt_scene_update(s, inst, ninst, LRH_LBVH, nullptr);
     if (rc) {
         pt_scene_release(s);

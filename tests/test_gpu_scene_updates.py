@@ -143,3 +143,75 @@ def test_mesh_update_equals_a_scene_made_with_the_new_triangles(ctx):
     same_scene(ctx, a, b, cam, "new mesh")
     with pytest.raises(lr.LrhError):
         a.mesh(3, mtris1[f:l - 1])
+
+
+def leaf_boxes(nodes, aux):
+    """instance id -> its leaf record's box (lo.xyz, hi.xyz as float32)"""
+    f = nodes.view(np.float32)
+    leaf = (nodes[:, 7] & 0x80000000) != 0
+    out = {}
+    for r in np.nonzero(leaf)[0]:
+        for k in range(int(nodes[r, 7] & 0x7FFFFFFF)):
+            out[int(aux[nodes[r, 3] + k])] = np.concatenate([f[r, 0:3], f[r, 4:7]])
+    return out
+
+
+def cost_of(nodes):
+    f = nodes.view(np.float32).astype(np.float64)
+    d = f[:, 4:7] - f[:, 0:3]
+    ok = (d >= 0).all(1)
+    a = np.where(ok, d[:, 0] * d[:, 1] + d[:, 1] * d[:, 2] + d[:, 2] * d[:, 0], 0.0)
+    return a.sum() / a[0]
+
+
+def test_refit_keeps_the_tree_while_it_stays_good(ctx):
+    mtris, first, mats, inst0, cam = scene_parts(13)
+    a = lr.PtScene.instanced(ctx, mtris, first, mats, inst0)
+    # a small move: every instance but the ground shifted a little
+    inst1 = inst0.copy()
+    rng = np.random.default_rng(5)
+    for i in range(1, len(inst1)):
+        x = inst1[i]["xform"].reshape(3, 4).copy()
+        x[:, 3] += rng.uniform(-0.05, 0.05, 3)
+        inst1[i]["xform"] = x.reshape(12)
+    st = a.update(inst1, builder=lr.REFIT)
+    say(f"refit: refitted {st.refitted}, {st.ms:.3f} ms")
+    assert st.refitted == 1 and st.depth >= 1
+    b = lr.PtScene.instanced(ctx, mtris, first, mats, inst1)
+    na, xa, ca = a.top()
+    nb, xb, cb = b.top()
+    # every inner record's box is its children's union, every leaf's its instance's world box
+    f = na.view(np.float32)
+    inner = np.nonzero((na[:, 7] & 0x80000000) == 0)[0]
+    for r in inner:
+        lc, rc = na[r, 3], na[r, 7]
+        assert np.array_equal(f[r, 0:3], np.minimum(f[lc, 0:3], f[rc, 0:3]))
+        assert np.array_equal(f[r, 4:7], np.maximum(f[lc, 4:7], f[rc, 4:7]))
+    la, lb = leaf_boxes(na, xa), leaf_boxes(nb, xb)
+    assert sorted(la) == list(range(len(inst1)))
+    for i in la:
+        assert np.array_equal(la[i], lb[i]), i
+    assert abs(ca - cost_of(na)) <= 1e-9 * ca and ca >= 1.0
+    # the same hits as the tree built anew (a tie, two shapes at one distance, may differ)
+    rays = rays_at(20000, 9)
+    ha, hb = a.trace(rays), b.trace(rays)
+    ia, ib = ha[:, 1].view(np.uint32), hb[:, 1].view(np.uint32)
+    assert np.array_equal(ia == 0xFFFFFFFF, ib == 0xFFFFFFFF)
+    same = (ia == ib) & (ia != 0xFFFFFFFF)
+    assert np.array_equal(ha[same].view(np.uint32), hb[same].view(np.uint32))
+    diff = (ia != ib)
+    assert diff.mean() < 0.001 and np.all(np.abs(ha[diff, 0] - hb[diff, 0]) <= 1e-5 * np.abs(hb[diff, 0]))
+    # a large move (the instances' places shuffled): the cost grows past 1.1, the tree is built
+    # again with the scene's builder, as a scene made with these poses has it
+    inst2 = inst1.copy()
+    perm = np.r_[0, 1 + rng.permutation(len(inst2) - 1)]
+    x = inst2["xform"].reshape(-1, 3, 4).copy()
+    x[:, :, 3] = x[perm, :, 3]
+    inst2["xform"] = x.reshape(-1, 12)
+    st = a.update(inst2, builder=lr.REFIT)
+    say(f"large move: refitted {st.refitted}")
+    assert st.refitted == 0
+    c = lr.PtScene.instanced(ctx, mtris, first, mats, inst2)
+    n2, x2, c2 = a.top()
+    nc_, xc, cc = c.top()
+    assert np.array_equal(n2, nc_) and np.array_equal(x2, xc) and c2 == cc
