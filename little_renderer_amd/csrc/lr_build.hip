@@ -35,9 +35,10 @@ constexpr int SORT_WAVES = SORT_BLOCK / 64;
 constexpr int SORT_PASSES = 4;                  // 30-bit codes: bits 0-7, 8-15, 16-23, 24-29
 constexpr int HIST_COPIES = 8;                  // one global histogram per XCD (less contention)
 constexpr int PLOC_R = 16;
-constexpr int PLOC_TILE = 256;
-constexpr int PLOC_WIN = PLOC_TILE + 4 * PLOC_R;   // boxes of positions base-2R .. base+256+2R
-constexpr int PLOC_NN = PLOC_TILE + 2 * PLOC_R;    // nearest of positions base-R .. base+256+R
+constexpr int PLOC_PER = 2;                     // positions a thread takes in a PLOC round
+constexpr int PLOC_TILE = 256 * PLOC_PER;        // positions a work-group takes (one claim)
+constexpr int PLOC_WIN = PLOC_TILE + 4 * PLOC_R;   // boxes of positions base-2R .. base+PLOC_TILE+2R
+constexpr int PLOC_NN = PLOC_TILE + 2 * PLOC_R;    // nearest of positions base-R .. base+PLOC_TILE+R
 constexpr int PLOC_MAX_ROUNDS = 128;
 constexpr int BOUNDS_BLOCKS = 512;
 
@@ -780,16 +781,24 @@ __global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t
             snn[q] = (i >= 0 && i < (int)nc) ? nearest_in(i, base, nc, wlo, whi) : LR_NONE;
         }
         __syncthreads();
-        // keep_merge (trace_ploc.wgsl:130-145)
-        const int i = base + tid;
-        const bool valid = i < (int)nc;
-        const uint32_t j = valid ? snn[tid + PLOC_R] : LR_NONE;
-        bool mu = false;
-        if (valid && j < nc) mu = snn[(int)j - base + PLOC_R] == (uint32_t)i;
-        const uint32_t keep = valid && !(mu && (uint32_t)i > j), merge = valid && mu && (uint32_t)i < j;
+        // keep_merge (trace_ploc.wgsl:130-145): PLOC_PER adjacent positions a thread, so that the
+        // scans of the threads' sums keep the positions' order
+        uint32_t j[PLOC_PER], keep[PLOC_PER], merge[PLOC_PER], kc = 0, mc = 0;
+#pragma unroll
+        for (int q = 0; q < PLOC_PER; q++) {
+            const int o = PLOC_PER * tid + q, i = base + o;
+            const bool valid = i < (int)nc;
+            j[q] = valid ? snn[o + PLOC_R] : LR_NONE;
+            bool mu = false;
+            if (valid && j[q] < nc) mu = snn[(int)j[q] - base + PLOC_R] == (uint32_t)i;
+            keep[q] = valid && !(mu && (uint32_t)i > j[q]);
+            merge[q] = valid && mu && (uint32_t)i < j[q];
+            kc += keep[q];
+            mc += merge[q];
+        }
         uint32_t tk, tm;
-        const uint32_t xk = block_excl_scan<BLOCK / 64>(keep, &tk, s_wsum);
-        const uint32_t xm = block_excl_scan<BLOCK / 64>(merge, &tm, s_wsum);
+        const uint32_t xk = block_excl_scan<BLOCK / 64>(kc, &tk, s_wsum);
+        const uint32_t xm = block_excl_scan<BLOCK / 64>(mc, &tm, s_wsum);
         if (tid < 64) {
             // the look-back by the first wave, 64 predecessor tiles at once (lane l: tile t - l):
             // the nearest inclusive prefix among them ends it; a needed tile not yet posted means
@@ -836,18 +845,21 @@ __global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t
             }
         }
         __syncthreads();
-        if (keep) {
-            const uint32_t dest = s_ek + xk;
-            const int li = tid + 2 * PLOC_R;
+        uint32_t dk = s_ek + xk, dm = s_em + xm;
+#pragma unroll
+        for (int q = 0; q < PLOC_PER; q++) {
+            if (!keep[q]) continue;
+            const int li = PLOC_PER * tid + q + 2 * PLOC_R;
             const uint32_t c = wid[li];
             uint32_t out = c;
-            if (merge) {
-                const int lj = (int)j - base + 2 * PLOC_R;
-                const uint32_t k = ((uint32_t)n - 2u) - ((uint32_t)n - nc + s_em + xm);
+            if (merge[q]) {
+                const int lj = (int)j[q] - base + 2 * PLOC_R;
+                const uint32_t k = ((uint32_t)n - 2u) - ((uint32_t)n - nc + dm);
+                dm++;
                 ploc_merge(k, c, wlo[li], whi[li], wid[lj], wlo[lj], whi[lj], (uint32_t)n, node0, ref0, bx, nodes, height);
                 out = k;
             }
-            nxt[dest] = out;
+            nxt[dk++] = out;
         }
         __syncthreads();
     }
