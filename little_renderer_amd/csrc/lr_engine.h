@@ -2,12 +2,15 @@
 // scene, camera, image and sample range) share one pool of path slots that the wavefront
 // kernels keep full.
 //
-// One iteration is four kernels: regen (start new paths in free slots), extend (closest hits
-// of the ray queue), shade (emission, next-event shadow ray, BSDF sample, Russian roulette),
-// shadow (any hit before the lamp). A path that ends gives its slot back to a ring of free
-// slots; regen fills free slots from the global path index space, in which each job owns
-// chunks of [0, w*h*spp) (sample-major: q = sample * w*h + pixel), each chunk an issue record
-// mapping a range of global path indices to (job, first q). The same kernels serve
+// One iteration is four kernels: regen (start new paths at the end of the ray queue), extend
+// (closest hits of the ray queue), shade (emission, next-event shadow ray, BSDF sample, Russian
+// roulette), shadow (any hit before the lamp). A path's state lives beside its ray, at its
+// position in the ray queue (two queues, one per iteration parity): shade reads entry i's state
+// and writes a surviving path's state where its next ray is appended, so every state access is
+// coalesced. A path that ends frees one place of the pool; regen fills free places from the
+// global path index space, in which each job owns chunks of [0, w*h*spp) (sample-major:
+// q = sample * w*h + pixel), each chunk an issue record mapping a range of global path indices
+// to (job, first q). The same kernels serve
 // the reference's one-job-at-a-time shape (a grid per pass, passes in order) and the batched
 // form (persistent work-groups, one hipGraph of several iterations replayed until the work is
 // done), so a job's image is the same to the bit in either, alone or among others: each
@@ -19,7 +22,7 @@
 
 #include "lr_internal.h"
 
-#define LR_SLOT_BITS 22                   // path slots: at most 2^22 (queue ids: job << 22 | slot)
+#define LR_SLOT_BITS 22                   // pool places: at most 2^22 (shadow ids: job << 22 | next queue position)
 #define LR_SLOT_MASK ((1u << LR_SLOT_BITS) - 1u)
 #define LR_JOBS 1024                      // job descriptors held on the device at once
 #define LR_ISSUE 4096                     // issue records (a ring, power of two)
@@ -76,7 +79,7 @@ enum { K_REGEN = 0, K_EXTEND = 1, K_SHADE = 2, K_SHADOW = 3 };
 
 // the engine's counters (device memory, followed by each job slot's progress)
 struct EngCtl {
-    unsigned long long head, tail;          // the free-slot ring holds slots [head, tail)
+    unsigned long long head, tail;          // free places of the pool: tail - head
     unsigned long long next, issue_end;     // the next path index to start; the end of those issued
     unsigned long long iq_lo, iq_hi;        // the issue records in use
     unsigned long long rays_ext, rays_shadow;
@@ -85,7 +88,7 @@ struct EngCtl {
     unsigned int count[4];                  // ray queues [0], [1]; shadow queue [2]
     unsigned int tiles[4];                  // each kernel's tile counter (work-groups claim tiles)
     unsigned int regen_done;                // regen's work-groups that have finished
-    unsigned int pad0;
+    unsigned int spar;                      // the parity of the queue the shade kernel appended to
     unsigned long long pad[3];
 };
 
@@ -103,17 +106,19 @@ struct EngArgs {
     const unsigned long long *iq_start;     // issue record: the first global path index it covers,
     const uint32_t *iq_job;                 // its job slot,
     const uint32_t *iq_q0;                  // and the index in the job of its first path
-    float4 *thr;                            // throughput.rgb, the last bounce's pdf
-    float4 *rad;                            // radiance.rgb, the emission state
-    uint64_t *rng;                          // PCG32 state (stream: the pixel)
-    uint2 *meta;                            // q (index in the job), depth
-    uint32_t *ring;                         // free slots
+    // a path's state, by ray queue (parity) and queue position
+    float4 *thr[2];                         // throughput.rgb, the last bounce's pdf
+    float4 *rad[2];                         // radiance.rgb, the emission state
+    uint64_t *rng[2];                       // PCG32 state (stream: the pixel)
+    uint2 *meta[2];                         // q (index in the job), depth
     float4 *qray[2];                        // o.xyz tmin | d.xyz tmax
-    uint32_t *qid[2];                       // job << 22 | slot
+    uint32_t *qid[2];                       // job << 22
     float4 *hits;                           // t, id, b1, b2 (by queue position)
     float4 *sray;
-    uint32_t *sid;
+    uint32_t *sid;                          // job << 22 | the path's next queue position
     float4 *scon;                           // contribution.rgb, 1 if the path ends after it
+    float4 *srad;                           // the path's radiance and emission state before it
+    uint32_t *sq;                           // the path's q
     uint32_t *spill;                        // walk stacks past the LDS part, by thread
     unsigned long long spill_stride;        // threads of the launch
     int spill_per;
@@ -122,7 +127,6 @@ struct EngArgs {
     const uint16_t *mask;                   // blue-noise ranks
     unsigned long long *counts;             // node records and shapes walked (counting extend only)
     unsigned long long *ostat;              // shade's tiles: look-back status of its ordered appends
-    uint32_t cmask;                         // slots - 1
     int mult;                               // a tile is mult work-groups' widths of entries
 };
 

@@ -21,14 +21,12 @@ namespace {
 
 constexpr int SBLOCK = 256;
 
-__global__ __launch_bounds__(SBLOCK) void k_eng_init(EngProgress *prog, uint32_t *ring, uint32_t C)
+__global__ __launch_bounds__(SBLOCK) void k_eng_init(EngProgress *prog, uint32_t C)
 {
     const size_t nw = sizeof(EngProgress) / 8;
     unsigned long long *w = (unsigned long long *)prog;
-    for (size_t i = (size_t)blockIdx.x * SBLOCK + threadIdx.x; i < (size_t)C || i < nw; i += (size_t)gridDim.x * SBLOCK) {
-        if (i < (size_t)C) ring[i] = (uint32_t)i;
-        if (i < nw) w[i] = (i == offsetof(EngCtl, tail) / 8) ? (unsigned long long)C : 0ull;
-    }
+    for (size_t i = (size_t)blockIdx.x * SBLOCK + threadIdx.x; i < nw; i += (size_t)gridDim.x * SBLOCK)
+        w[i] = (i == offsetof(EngCtl, tail) / 8) ? (unsigned long long)C : 0ull;
 }
 
 __global__ void k_job_put(JobDev *jobs, unsigned long long *jret, unsigned long long *jdone, int slot, JobDev jd)
@@ -500,12 +498,12 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
     const size_t C = e->C;
     const size_t sizes[] = {
         sizeof(EngProgress), LR_JOBS * sizeof(JobDev), LR_ISSUE * 8, LR_ISSUE * 4,   // 0..3
-        C * 16, C * 16, C * 8, C * 8, C * 4,                                         // 4..8 thr rad rng meta ring
+        2 * C * 16, 2 * C * 16, 2 * C * 8, 2 * C * 8, C * 4,                         // 4..8 thr rad rng meta (2 queues), sq
         C * 32, C * 32, C * 4, C * 4, C * 16,                                        // 9..13 qray qray qid qid hits
         C * 32, C * 4, C * 16,                                                       // 14..16 sray sid scon
         e->threads_cap * (size_t)(e->spill_cap > 0 ? e->spill_cap : 1) * 4,          // 17 spill
         2112 * 4, 4096 * 2, 256, 256, 256,                                           // 18..22 tab mask counts clk overflow
-        LR_ISSUE * 4, (C / 64 + 4) * 8};                                             // 23 iq_q0, 24 ostat
+        LR_ISSUE * 4, (C / 64 + 4) * 8, C * 16};                                     // 23 iq_q0, 24 ostat, 25 srad
     constexpr int NS = sizeof(sizes) / sizeof(sizes[0]);
     size_t total = 0;
     for (size_t z : sizes) total += lr_align(z);
@@ -539,11 +537,14 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
     A.iq_start = e->iq_start;
     A.iq_job = e->iq_job;
     A.iq_q0 = e->iq_q0;
-    A.thr = (float4 *)p[4];
-    A.rad = (float4 *)p[5];
-    A.rng = (uint64_t *)p[6];
-    A.meta = (uint2 *)p[7];
-    A.ring = (uint32_t *)p[8];
+    for (int b = 0; b < 2; b++) {
+        A.thr[b] = (float4 *)p[4] + b * C;
+        A.rad[b] = (float4 *)p[5] + b * C;
+        A.rng[b] = (uint64_t *)p[6] + b * C;
+        A.meta[b] = (uint2 *)p[7] + b * C;
+    }
+    A.sq = (uint32_t *)p[8];
+    A.srad = (float4 *)p[25];
     A.qray[0] = (float4 *)p[9];
     A.qray[1] = (float4 *)p[10];
     A.qid[0] = (uint32_t *)p[11];
@@ -558,7 +559,6 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
     A.counts = e->counts;
     A.ostat = (unsigned long long *)p[24];
     A.overflow = e->overflow;
-    A.cmask = e->C - 1;
     A.mult = 1;
     std::vector<float> tab(2112);
     std::vector<uint16_t> mask(4096);
@@ -579,7 +579,7 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
         hipMemcpyAsync(p[19], mask.data(), 4096 * 2, hipMemcpyHostToDevice, e->stream) != hipSuccess)
         rc = -1;
     if (!rc) {
-        hipLaunchKernelGGL(k_eng_init, dim3(1024), dim3(SBLOCK), 0, e->stream, e->prog, A.ring, e->C);
+        hipLaunchKernelGGL(k_eng_init, dim3(8), dim3(SBLOCK), 0, e->stream, e->prog, e->C);
         hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, e->stream, e->clk);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(e->stream) != hipSuccess) rc = -1;
     }

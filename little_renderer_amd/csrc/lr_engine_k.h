@@ -104,9 +104,9 @@ __device__ __forceinline__ void ordered_append2(unsigned long long *stat, uint32
 }
 
 // A path ends: its sample joins its pixel (NaN samples skipped and the clamp applied as
-// trace_image does, tracer.c:347-349), its slot goes back to the ring, its job counts it. Every
+// trace_image does, tracer.c:347-349), its place in the pool is freed, its job counts it. Every
 // lane of the wave calls this (on: the lane's path ends).
-__device__ __forceinline__ void retire(const EngArgs &A, bool on, uint32_t jslot, uint32_t slot, uint32_t q, float4 r)
+__device__ __forceinline__ void retire(const EngArgs &A, bool on, uint32_t jslot, uint32_t q, float4 r)
 {
     if (on) {
         const JobDev &J = A.jobs[jslot];
@@ -124,10 +124,7 @@ __device__ __forceinline__ void retire(const EngArgs &A, bool on, uint32_t jslot
     const uint64_t m = __ballot(on);
     if (m == 0) return;
     const int leader = __ffsll((unsigned long long)m) - 1;
-    unsigned long long base = 0;
-    if ((int)lane_id() == leader) base = atomicAdd(&A.ctl->tail, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader, 64);
-    if (on) A.ring[(uint32_t)(base + (unsigned long long)__popcll(m & lanemask_lt())) & A.cmask] = slot;
+    if ((int)lane_id() == leader) atomicAdd(&A.ctl->tail, (unsigned long long)__popcll(m));
     // the job's count of retired paths: one atomic per job present in the wave
     uint64_t pend = m;
     while (pend) {
@@ -341,8 +338,9 @@ __device__ WalkHit walk4_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax,
     return best;
 }
 
-// regen (wf_gen, trace_wave.wgsl:51-70, with trace_image's seeds, tracer.c:335-346): the free
-// slots [head, head + claimed) start paths next .. next + claimed - 1, each a camera ray
+// regen (wf_gen, trace_wave.wgsl:51-70, with trace_image's seeds, tracer.c:335-346): `claimed`
+// free places of the pool start paths next .. next + claimed - 1, each a camera ray and its state
+// at the end of the ray queue
 template <int BLK>
 __global__ __launch_bounds__(BLK) void k_regen(EngArgs A)
 {
@@ -358,10 +356,7 @@ __global__ __launch_bounds__(BLK) void k_regen(EngArgs A)
         reset_tiles(ctl, K_REGEN);
     }
     for_tiles<BLK>(&ctl->tiles[K_REGEN], n, A.mult, [&](uint32_t i, bool live) {
-        uint32_t id = 0;
-        float4 r0, r1;
         if (live) {
-            const uint32_t slot = A.ring[(uint32_t)(head + i) & A.cmask];
             const unsigned long long g = next + i;
             // the issue record holding g: the last one that starts at or before it
             unsigned long long a = lo, b = hi;   // records [a, b); the answer in [a, b)
@@ -389,20 +384,16 @@ __global__ __launch_bounds__(BLK) void k_regen(EngArgs A)
                      vt = v3(J.cam[9], J.cam[10], J.cam[11]);
             const V3 target = vsub(vadd(ul, vscale(hz, u)), vscale(vt, v));
             const V3 d = vnorm(vsub(target, org));
-            r0 = make_float4(org.x, org.y, org.z, J.eps);
-            r1 = make_float4(d.x, d.y, d.z, 1e30f);
-            A.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-            A.rad[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(EMIT_FULL));
-            A.rng[slot] = st;
-            A.meta[slot] = make_uint2(q, 0u);
-            id = (jslot << LR_SLOT_BITS) | slot;
-        }
-        // every started path has its camera ray: in claim order, after the queue's paths
-        if (live) {
+            // every started path has its camera ray and its state: in claim order, after the
+            // queue's paths
             const uint32_t at = start + i;
-            A.qray[cur][2 * (size_t)at] = r0;
-            A.qray[cur][2 * (size_t)at + 1] = r1;
-            A.qid[cur][at] = id;
+            A.qray[cur][2 * (size_t)at] = make_float4(org.x, org.y, org.z, J.eps);
+            A.qray[cur][2 * (size_t)at + 1] = make_float4(d.x, d.y, d.z, 1e30f);
+            A.qid[cur][at] = jslot << LR_SLOT_BITS;
+            A.thr[cur][at] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            A.rad[cur][at] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(EMIT_FULL));
+            A.rng[cur][at] = st;
+            A.meta[cur][at] = make_uint2(q, 0u);
         }
     });
     // the last work-group to finish sets the queue's length (all have read the old one)
@@ -478,23 +469,26 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
     EngCtl *ctl = A.ctl;
     const int cur = (int)(ctl->iter & 1ull);
     const uint32_t n = ctl->count[cur];
-    if (blockIdx.x == 0 && threadIdx.x == 0) reset_tiles(ctl, K_SHADE);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        reset_tiles(ctl, K_SHADE);
+        ctl->spar = (unsigned int)(1 - cur);
+    }
     for_tiles<BLK>(&ctl->tiles[K_SHADE], n, 1, [&](uint32_t i, bool live) {
         bool next = false, shadow = false;
         float4 nr0, nr1, sr0, sr1, scon;
-        uint32_t id = 0, slot = 0, jslot = 0, q = 0;
-        float4 rad_out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t jslot = 0, q = 0;
+        int depth = 0;
+        uint64_t st = 0;
+        float4 rad_out = make_float4(0.0f, 0.0f, 0.0f, 0.0f), thr_out;
         if (live) {
-            id = A.qid[cur][i];
-            slot = id & LR_SLOT_MASK;
-            jslot = id >> LR_SLOT_BITS;
+            jslot = A.qid[cur][i] >> LR_SLOT_BITS;
             const JobDev &J = A.jobs[jslot];
-            const uint2 mt = A.meta[slot];
+            const uint2 mt = A.meta[cur][i];
             q = mt.x;
-            const int depth = (int)mt.y;
+            depth = (int)mt.y;
             const float4 r0 = A.qray[cur][2 * (size_t)i], r1 = A.qray[cur][2 * (size_t)i + 1];
             const float4 hv = A.hits[i];
-            const float4 thr4 = A.thr[slot], rad4 = A.rad[slot];
+            const float4 thr4 = A.thr[cur][i], rad4 = A.rad[cur][i];
             V3 thr = v3of(thr4), rad = v3of(rad4);
             float bounce_pdf = thr4.w;
             int emit = __float_as_int(rad4.w);
@@ -502,7 +496,7 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
             const V3 d = v3of(r1), o = v3of(r0);
             const uint32_t pixel = q % (uint32_t)J.npix;
             const uint32_t k = J.s0 + q / (uint32_t)J.npix;
-            uint64_t st = A.rng[slot];
+            st = A.rng[cur][i];
             const uint64_t inc = ((uint64_t)pixel << 1u) | 1u;
             const uint64_t seed = ((uint64_t)J.seed_hi << 32) | J.seed_lo;
             const Smp sm{(int)(pixel % (uint32_t)J.W), (int)(pixel / (uint32_t)J.W), k, (uint32_t)seed * 0x9e3779b9u,
@@ -651,29 +645,33 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
                 }
             }
             rad_out = make_float4(rad.x, rad.y, rad.z, __int_as_float(emit));
-            if (next) {
-                A.thr[slot] = make_float4(thr.x, thr.y, thr.z, bounce_pdf);
-                A.rng[slot] = st;
-                A.meta[slot] = make_uint2(q, (uint32_t)depth + 1u);
-            }
-            if (next || shadow) A.rad[slot] = rad_out;
+            thr_out = make_float4(thr.x, thr.y, thr.z, bounce_pdf);
             if (shadow && !next) scon.w = 1.0f;   // the shadow kernel ends the path
         }
         uint32_t at, ss;
         ordered_append2<BLK>(A.ostat, i / BLK, (n + BLK - 1) / BLK, next, shadow, at, ss, &ctl->count[1 - cur],
                              &ctl->count[2]);
+        // the surviving path: its next ray and its state, side by side in the other queue
         if (next) {
             A.qray[1 - cur][2 * (size_t)at] = nr0;
             A.qray[1 - cur][2 * (size_t)at + 1] = nr1;
-            A.qid[1 - cur][at] = id;
+            A.qid[1 - cur][at] = jslot << LR_SLOT_BITS;
+            A.thr[1 - cur][at] = thr_out;
+            A.rad[1 - cur][at] = rad_out;
+            A.rng[1 - cur][at] = st;
+            A.meta[1 - cur][at] = make_uint2(q, (uint32_t)depth + 1u);
         }
+        // the shadow ray carries the path's radiance and q (its contribution is added to one or
+        // the other: the path's next state, or its pixel if the path ends with it)
         if (shadow) {
             A.sray[2 * (size_t)ss] = sr0;
             A.sray[2 * (size_t)ss + 1] = sr1;
-            A.sid[ss] = id;
+            A.sid[ss] = (jslot << LR_SLOT_BITS) | (next ? at : 0u);
             A.scon[ss] = scon;
+            A.srad[ss] = rad_out;
+            A.sq[ss] = q;
         }
-        retire(A, live && !next && !shadow, jslot, slot, q, rad_out);
+        retire(A, live && !next && !shadow, jslot, q, rad_out);
     });
 }
 
@@ -686,6 +684,7 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(E
     __shared__ uint32_t stk[STK][BLK];
     EngCtl *ctl = A.ctl;
     const uint32_t n = ctl->count[2];
+    const int par = (int)ctl->spar;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         reset_tiles(ctl, K_SHADOW);
         ctl->rays_shadow += n;
@@ -695,11 +694,11 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(E
     const WalkStack S{&stk[0][threadIdx.x], BLK, A.spill + gtid, (size_t)A.spill_stride, STK + A.spill_per};
     for_tiles<BLK>(&ctl->tiles[K_SHADOW], n, A.mult, [&](uint32_t i, bool live) {
         bool ends = false;
-        uint32_t slot = 0, jslot = 0, q = 0;
+        uint32_t jslot = 0, q = 0;
         float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (live) {
             const uint32_t id = A.sid[i];
-            slot = id & LR_SLOT_MASK;
+            const uint32_t at = id & LR_SLOT_MASK;
             jslot = id >> LR_SLOT_BITS;
             const JobDev &J = A.jobs[jslot];
             const float4 r0 = A.sray[2 * (size_t)i], r1 = A.sray[2 * (size_t)i + 1];
@@ -717,17 +716,17 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(E
             ends = c.w != 0.0f;
             const bool lit = h.id == LR_NONE;
             if (lit || ends) {
-                r = A.rad[slot];
+                r = A.srad[i];
                 if (lit) {
                     r.x += c.x;
                     r.y += c.y;
                     r.z += c.z;
                 }
-                if (!ends) A.rad[slot] = r;
-                else q = A.meta[slot].x;
+                if (!ends) A.rad[par][at] = r;   // the path's next state (shade wrote it without this)
+                else q = A.sq[i];
             }
         }
-        retire(A, ends, jslot, slot, q, r);
+        retire(A, ends, jslot, q, r);
     });
 }
 
