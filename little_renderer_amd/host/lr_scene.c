@@ -97,74 +97,105 @@ void lrs_camera_make(const double from[3], const double at[3], const double up[3
     }
 }
 
-/* ---- the sampler's blue-noise mask: void-and-cluster (Ulichney 1993) as
- * engine/base/sampler.c:86-132 makes it: a Gaussian energy (sigma 1.9) that wraps around the
- * 64 x 64 tile; a tenth of the points placed at random (PCG32 seeded (0x5eedb10e, 7)) and
- * relaxed (the most crowded moved to the largest gap until nothing moves), ranked by removing
- * the most crowded first; the rest ranked by filling the largest gap ---- */
-#define BNS 64
-#define BNN (BNS * BNS)
-static void bn_add(float *energy, const float *kern, int p, float sign)
+/* ---- the sampler's blue-noise mask (engine/base/sampler.c:86-132): Ulichney's void-and-cluster
+ * method (1993) on a 64 x 64 tile that wraps around. A pattern's "density" at a cell is the sum,
+ * over the pattern's points, of a Gaussian (sigma 1.9) of the cell's toroidal offset from the
+ * point. The tightest cluster is the point of largest density, the largest void the empty cell
+ * of smallest density (ties: the lowest cell index). The initial pattern (a tenth of the cells,
+ * drawn by PCG32 seeded (0x5eedb10e, 7)) is relaxed by moving its tightest cluster to its largest
+ * void until the move is a no-op. Ranks: the initial points by removing tightest clusters from a
+ * copy (the last removed gets 0), the remaining cells by filling largest voids (in order). The
+ * densities are float sums taken in the order points come and go, as the reference's float
+ * build takes them, so the ranks are its ranks. ---- */
+enum { VC_SIDE = 64, VC_CELLS = VC_SIDE * VC_SIDE, VC_MASK = VC_SIDE - 1 };
+
+typedef struct {
+    float density[VC_CELLS];
+    unsigned char set[VC_CELLS];
+} VcPattern;
+
+/* the Gaussian of each toroidal offset (ox, oy), both taken in [-32, 31] */
+static float vc_gauss[VC_CELLS];
+
+static void vc_gauss_init(void)
 {
-    const int px = p % BNS, py = p / BNS;
-    for (int y = 0; y < BNS; y++) {
-        const float *k = kern + ((y - py) & (BNS - 1)) * BNS;
-        float *e = energy + y * BNS;
-        for (int x = 0; x < px; x++) e[x] += sign * k[x - px + BNS];
-        for (int x = px; x < BNS; x++) e[x] += sign * k[x - px];
+    const double sigma = (double)1.9f;   /* the float build's literal (-fsingle-precision-constant) */
+    for (int oy = 0; oy < VC_SIDE; oy++) {
+        const int dy = oy >= VC_SIDE / 2 ? oy - VC_SIDE : oy;
+        for (int ox = 0; ox < VC_SIDE; ox++) {
+            const int dx = ox >= VC_SIDE / 2 ? ox - VC_SIDE : ox;
+            vc_gauss[oy * VC_SIDE + ox] = (float)exp(-(dx * dx + dy * dy) / (2 * sigma * sigma));
+        }
     }
 }
-static int bn_pick(const float *energy, const unsigned char *on, int want_on, int want_max)
+
+/* adds point p to the pattern, or takes it out: its Gaussian splatted onto every cell by offset */
+static void vc_toggle(VcPattern *pat, int p)
 {
-    int best = -1;
-    for (int i = 0; i < BNN; i++) {
-        if (on[i] != want_on) continue;
-        if (best < 0 || (want_max ? energy[i] > energy[best] : energy[i] < energy[best])) best = i;
+    const int px = p & VC_MASK, py = p / VC_SIDE;
+    const int add = !pat->set[p];
+    pat->set[p] = (unsigned char)add;
+    for (int oy = 0; oy < VC_SIDE; oy++) {
+        float *row = pat->density + ((py + oy) & VC_MASK) * VC_SIDE;
+        const float *g = vc_gauss + oy * VC_SIDE;
+        for (int ox = 0; ox < VC_SIDE; ox++) {
+            float *cell = row + ((px + ox) & VC_MASK);
+            if (add) *cell += g[ox];
+            else *cell -= g[ox];
+        }
     }
-    return best;
 }
+
+/* the tightest cluster (of the set cells) or the largest void (of the empty ones) */
+static int vc_cluster(const VcPattern *pat)
+{
+    int at = -1;
+    float top = 0;
+    for (int i = 0; i < VC_CELLS; i++)
+        if (pat->set[i] && (at < 0 || pat->density[i] > top)) { at = i; top = pat->density[i]; }
+    return at;
+}
+static int vc_void(const VcPattern *pat)
+{
+    int at = -1;
+    float low = 0;
+    for (int i = 0; i < VC_CELLS; i++)
+        if (!pat->set[i] && (at < 0 || pat->density[i] < low)) { at = i; low = pat->density[i]; }
+    return at;
+}
+
 void lrs_blue_noise(uint16_t *rank)
 {
-    static float kern[BNN], energy[BNN], saved[BNN];
-    static unsigned char on[BNN], on_saved[BNN];
-    const double sigma = (double)1.9f;   /* the float build's literal (-fsingle-precision-constant) */
-    for (int y = 0; y < BNS; y++)
-        for (int x = 0; x < BNS; x++) {
-            const int dx = x < BNS / 2 ? x : x - BNS, dy = y < BNS / 2 ? y : y - BNS;
-            kern[y * BNS + x] = (float)exp(-(dx * dx + dy * dy) / (2 * sigma * sigma));
-        }
-    for (int i = 0; i < BNN; i++) { on[i] = 0; energy[i] = 0.0f; }
+    static VcPattern start, work;
+    vc_gauss_init();
+    memset(&start, 0, sizeof start);
     Pcg r;
     pcg_seed(&r, 0x5eedb10eULL, 7);
-    const int ones = BNN / 10;
-    for (int n = 0; n < ones;) {
-        const int p = (int)(pcg_u32(&r) % BNN);
-        if (on[p]) continue;
-        on[p] = 1;
-        bn_add(energy, kern, p, 1.0f);
-        n++;
+    const int first = VC_CELLS / 10;
+    int placed = 0;
+    while (placed < first) {
+        const int p = (int)(pcg_u32(&r) % VC_CELLS);
+        if (start.set[p]) continue;
+        vc_toggle(&start, p);
+        placed++;
     }
-    for (int it = 0; it < BNN; it++) {
-        const int c = bn_pick(energy, on, 1, 1);
-        on[c] = 0;
-        bn_add(energy, kern, c, -1.0f);
-        const int v = bn_pick(energy, on, 0, 0);
-        on[v] = 1;
-        bn_add(energy, kern, v, 1.0f);
-        if (v == c) break;
+    for (int moves = 0; moves < VC_CELLS; moves++) {
+        const int from = vc_cluster(&start);
+        vc_toggle(&start, from);
+        const int to = vc_void(&start);
+        vc_toggle(&start, to);
+        if (to == from) break;
     }
-    for (int i = 0; i < BNN; i++) { on_saved[i] = on[i]; saved[i] = energy[i]; }
-    for (int k = ones - 1; k >= 0; k--) {
-        const int c = bn_pick(energy, on, 1, 1);
-        on[c] = 0;
-        bn_add(energy, kern, c, -1.0f);
+    work = start;
+    for (int k = first; k-- > 0;) {
+        const int c = vc_cluster(&work);
+        vc_toggle(&work, c);
         rank[c] = (uint16_t)k;
     }
-    for (int i = 0; i < BNN; i++) { on[i] = on_saved[i]; energy[i] = saved[i]; }
-    for (int k = ones; k < BNN; k++) {
-        const int v = bn_pick(energy, on, 0, 0);
-        on[v] = 1;
-        bn_add(energy, kern, v, 1.0f);
+    work = start;
+    for (int k = first; k < VC_CELLS; k++) {
+        const int v = vc_void(&work);
+        vc_toggle(&work, v);
         rank[v] = (uint16_t)k;
     }
 }
@@ -490,29 +521,22 @@ int lrs_sky_tables(const float *rgb, int w, int h, float *marg, float *cond, flo
         free(lum); free(wm); free(rs); free(rows);
         return -1;
     }
-    /* make_weights (env.c:125-162): the largest luminance of each pixel's 3 x 3 (wrapping
-       across, stopping at the poles), and each row's sin(theta) */
+    /* The sampling weight of a pixel (env.c:125-162, pixel_weight): the largest Rec. 709
+       luminance over its 3 x 3 neighbourhood, the map wrapping around in x and stopping at the
+       poles in y, never below 0; each row's weights are taken times sin(theta) at its centre.
+       Luminances first, then each pixel's nine neighbours in one pass. */
+    for (size_t i = 0; i < (size_t)w * h; i++)
+        lum[i] = 0.2126f * rgb[3 * i] + 0.7152f * rgb[3 * i + 1] + 0.0722f * rgb[3 * i + 2];
     for (int y = 0; y < h; y++) {
-        const float *p = rgb + 3 * (size_t)y * w;
-        float *l = lum + (size_t)y * w, *o = wm + (size_t)y * w;
-        for (int x = 0; x < w; x++) l[x] = 0.2126f * p[3 * x] + 0.7152f * p[3 * x + 1] + 0.0722f * p[3 * x + 2];
+        const int rows3[3] = {y > 0 ? y - 1 : y, y, y + 1 < h ? y + 1 : y};
         for (int x = 0; x < w; x++) {
-            const float a = l[x == 0 ? w - 1 : x - 1], b = l[x], c = l[x + 1 == w ? 0 : x + 1];
-            const float m = a > b ? a : b;
-            o[x] = m > c ? m : c;
+            const int cols3[3] = {(x + w - 1) % w, x, (x + 1) % w};
+            float m = 0;
+            for (int j = 0; j < 3; j++)
+                for (int i = 0; i < 3; i++) m = fmaxf(m, lum[(size_t)rows3[j] * w + cols3[i]]);
+            wm[(size_t)y * w + x] = m;
         }
         rs[y] = sinf((y + 0.5f) * SKY_PI / h);
-    }
-    memcpy(lum, wm, sizeof(float) * (size_t)w * h);
-    for (int y = 0; y < h; y++) {
-        const float *a = lum + (size_t)(y > 0 ? y - 1 : y) * w, *b = lum + (size_t)y * w;
-        const float *c = lum + (size_t)(y + 1 < h ? y + 1 : y) * w;
-        float *o = wm + (size_t)y * w;
-        for (int x = 0; x < w; x++) {
-            float m = a[x] > b[x] ? a[x] : b[x];
-            m = m > c[x] ? m : c[x];
-            o[x] = m > 0 ? m : 0;
-        }
     }
     /* prepare_base's sampling tables (env.c:343-364): each row's cumulative distribution over
        its pixels, the rows' over the map, weights lum_max * sin(theta) summed in double */
