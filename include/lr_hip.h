@@ -213,6 +213,13 @@ int lrh_pt_scene_top(LrhPtScene *s, uint32_t *h_nodes, uint32_t *h_aux, double *
 int lrh_pt_render(LrhContext *c, const LrhPtScene *s, const float *cam, int W, int H, int spp, uint32_t s0, uint64_t seed,
                   int max_depth, int mis, float clamp, float *d_accum);
 
+/* The engine's BSDF on n cases (device memory): in 17 floats each (base rgb, metallic, roughness,
+ * ior, transmission, outside (1/0), v xyz, l xyz, ul, u1, u2; a MAT_PBR material, the normal +z),
+ * out 16 floats each: bsdf_eval's f rgb and pdf, then bsdf_pick(ul) + bsdf_sample_lobe(u1, u2):
+ * ok, lobe, l xyz, weight rgb, pdf, delta, through, 0 (brdf.h:176-340). The kernel-level check of
+ * the shading code, as the reference's gpu_bsdf_run (tests/test_gpu_brdf.c:21-90). Waits. */
+int lrh_bsdf_cases(LrhContext *c, const float *d_in, int n, float *d_out);
+
 /* ---- the engine: many render jobs on one device ----
  * The role of the reference's GpuView trace loop (gpu_view_trace_start/more/passes/read,
  * engine/gpu/view_gpu.h:71-124, and gpu_frame_trace, engine/app/gpu_frame.h:14-26) for any
@@ -309,7 +316,10 @@ int lrh_engine_autotune(LrhEngine *e, const LrhJob *sample, int n, LrhTune *best
 
 /* ---- serving: a request loop over an engine ----
  * Scenes registered with the server stay resident; their bytes and the images of admitted jobs
- * count against a budget of device memory. Requests (thread-safe, from any host thread) wait
+ * count against a budget of device memory: the image arena is made in chunks as admitted images
+ * need it, so resident scenes plus the arena never hold more than the budget (the engine's path
+ * state, fixed by slots_log2 at ~260 bytes a slot, and a finished image's staging buffer come on
+ * top of it). Requests (thread-safe, from any host thread) wait
  * in order until their image fits, then join the running batch at once: their paths take free
  * slots as others end (continuous batching). A worker thread drives the engine. */
 typedef struct LrhServer LrhServer;
@@ -319,7 +329,7 @@ typedef struct {
     uint64_t rays;              /* rays traced (closest-hit + shadow) */
     double busy_ms;             /* time with jobs on the device */
     double p50_ms, p99_ms;      /* latency (submit -> last sample) of the jobs done */
-    size_t budget, in_use;      /* bytes: the budget; scenes + images admitted now */
+    size_t budget, in_use;      /* bytes: the budget; device memory held by the resident scenes and the image arena */
 } LrhServerStats;
 
 int lrh_server_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t budget_bytes, const LrhTune *tune,

@@ -116,6 +116,7 @@ def lib():
         "lrh_pt_scene_info": (I, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_size_t)]),
         "lrh_pt_render": (I, [P, P, P, I, I, I, U, ctypes.c_uint64, I, I, F, P]),
+        "lrh_bsdf_cases": (I, [P, P, I, P]),
         "lrs_batch_scene": (None, [I, ctypes.c_uint64, P, P, P]),
         "lrs_batch_camera": (None, [ctypes.c_uint64, I, I, P]),
         "lrs_batch_plan": (None, [I, ctypes.c_uint64, I, P, P, P, P]),
@@ -401,6 +402,10 @@ class Context:
         """a walk that also adds its node fetches and triangle tests to counts (2 x uint64, device)"""
         _check(lib().lrh_trace_count(self._c, _ptr(nodes), root, _ptr(aux), _ptr(tris), _ptr(rays), max_n,
                                      _ptr(count), 1 if any_hit else 0, _ptr(hits), max_depth, _ptr(counts)))
+
+    def bsdf_cases(self, cases, out):
+        """lrh_bsdf_cases: the device BSDF on (n, 17) cases (device) into (n, 16) (device)"""
+        _check(lib().lrh_bsdf_cases(self._c, _ptr(cases), cases.shape[0], _ptr(out)))
 
     def rays_primary(self, cam, w, h, tmin, tmax, rays):
         _check(lib().lrh_rays_primary(self._c, _ptr(cam), w, h, tmin, tmax, _ptr(rays)))

@@ -181,5 +181,12 @@ struct LrhPtScene {
     size_t bytes = 0;                       // device memory it holds
 };
 
+// the engine with its image arena made at once, or chunk by chunk as images need it up to
+// image_bytes (lazy); a lazy arena's cap, the chunks' bytes, and its chunks holding no image freed
+int lr_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t image_bytes, bool lazy, LrhEngine **out);
+void lr_engine_arena_cap(LrhEngine *e, size_t cap);
+size_t lr_engine_arena_held(const LrhEngine *e);
+void lr_engine_arena_trim(LrhEngine *e);
+
 // a job descriptor's scene part (the shapes, trees, materials, sky) from a resident scene
 void lr_job_scene(JobDev &jd, const LrhPtScene *s);

@@ -106,6 +106,11 @@ struct IssueRec {
     unsigned long long rec, start, end;
 };
 
+struct ArenaChunk {
+    size_t base, bytes;
+    char *ptr;
+};
+
 }  // namespace
 
 struct LrhEngine {
@@ -118,8 +123,12 @@ struct LrhEngine {
     int spill_cap = 0;                         // spill entries per thread (for a 16-entry LDS stack)
     // device memory
     void *dmem = nullptr;
-    char *arena = nullptr;
-    size_t arena_bytes = 0;
+    // job images: blocks of an offset space laid over device chunks (one chunk made at creation,
+    // or, in a lazy arena, chunks made as images need them while the total stays within the cap);
+    // consecutive chunks' offsets are kept 256 apart, so freed blocks never merge across chunks
+    std::vector<ArenaChunk> chunks;
+    size_t arena_bytes = 0, arena_cap = 0, arena_next = 0;
+    bool arena_lazy = false;
     EngProgress *prog = nullptr;
     JobDev *jobs = nullptr;
     unsigned long long *iq_start = nullptr, *counts = nullptr, *clk = nullptr;
@@ -307,7 +316,7 @@ int check_overflow(LrhEngine *e)
     return 0;
 }
 
-size_t arena_take(LrhEngine *e, size_t bytes)
+size_t arena_first_fit(LrhEngine *e, size_t bytes)
 {
     for (auto it = e->free_blocks.begin(); it != e->free_blocks.end(); ++it) {
         if (it->second < bytes) continue;
@@ -317,6 +326,42 @@ size_t arena_take(LrhEngine *e, size_t bytes)
         return off;
     }
     return (size_t)-1;
+}
+
+// a new chunk of device memory in the offset space: false if it cannot be had
+bool arena_grow(LrhEngine *e, size_t bytes)
+{
+    if (e->arena_bytes + bytes > e->arena_cap) return false;
+    char *ptr = nullptr;
+    if (hipMalloc((void **)&ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    e->chunks.push_back({e->arena_next, bytes, ptr});
+    e->free_blocks[e->arena_next] = bytes;
+    e->arena_next += bytes + 256;
+    e->arena_bytes += bytes;
+    return true;
+}
+
+size_t arena_take(LrhEngine *e, size_t bytes)
+{
+    size_t off = arena_first_fit(e, bytes);
+    if (off != (size_t)-1 || !e->arena_lazy) return off;
+    // lazy: a chunk of at least 256 MiB (fewer, larger allocations), within the cap
+    const size_t room = e->arena_cap - e->arena_bytes;
+    size_t want = bytes > ((size_t)256 << 20) ? bytes : ((size_t)256 << 20);
+    if (want > room) want = room & ~(size_t)255;
+    if (want < bytes || !arena_grow(e, want)) return (size_t)-1;
+    return arena_first_fit(e, bytes);
+}
+
+// the device address of an arena offset
+char *arena_ptr(const LrhEngine *e, size_t off)
+{
+    for (const ArenaChunk &c : e->chunks)
+        if (off >= c.base && off < c.base + c.bytes) return c.ptr + (off - c.base);
+    return nullptr;
 }
 
 void arena_give(LrhEngine *e, size_t off, size_t bytes)
@@ -385,7 +430,7 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
     JobDev jd;
     memset(&jd, 0, sizeof(jd));
     lr_job_scene(jd, s);
-    jd.accum = (long long *)(e->arena + off);
+    jd.accum = (long long *)arena_ptr(e, off);
     jd.total = h.total;
     memcpy(jd.cam, j->cam, sizeof(jd.cam));
     jd.W = j->w;
@@ -453,29 +498,9 @@ int job_slot(const LrhEngine *e, int64_t id)
 
 }  // namespace
 
-extern "C" {
-
-double lrh_now_ms(void) { return now_ms(); }
-
-void lrh_engine_destroy(LrhEngine *e)
-{
-    if (!e) return;
-    (void)hipSetDevice(e->device);
-    if (e->stream) (void)hipStreamSynchronize(e->stream);
-    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
-    for (hipEvent_t ev : e->pevs) (void)hipEventDestroy(ev);
-    for (int b = 0; b < 2; b++) {
-        if (e->pin[b]) (void)hipHostFree(e->pin[b]);
-        if (e->pev[b]) (void)hipEventDestroy(e->pev[b]);
-    }
-    if (e->dmem) (void)hipFree(e->dmem);
-    if (e->arena) (void)hipFree(e->arena);
-    if (e->stream) (void)hipStreamDestroy(e->stream);
-    delete e;
-}
-
-int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t image_bytes, LrhEngine **out)
+// lrh_engine_create, with an image arena made at once (lazy = false) or chunk by chunk as images
+// need it, up to image_bytes in all (lazy = true: the server's, whose budget it must keep)
+int lr_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t image_bytes, bool lazy, LrhEngine **out)
 {
     if (!c || !out || slots_log2 < 10 || slots_log2 > LR_SLOT_BITS || max_tree_depth < 1 || max_tree_depth > 1024 ||
         image_bytes < 256)
@@ -507,14 +532,14 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
     constexpr int NS = sizeof(sizes) / sizeof(sizes[0]);
     size_t total = 0;
     for (size_t z : sizes) total += lr_align(z);
+    e->arena_cap = image_bytes & ~(size_t)255;
+    e->arena_lazy = lazy;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess || hipMalloc(&e->dmem, total) != hipSuccess ||
-        hipMalloc((void **)&e->arena, image_bytes) != hipSuccess) {
+        (!lazy && !arena_grow(e, e->arena_cap))) {
         (void)hipGetLastError();
         lrh_engine_destroy(e);
         return lr_fail(LRH_ENOMEM, "lrh_engine_create: %zu + %zu bytes of device memory could not be had", total, image_bytes);
     }
-    e->arena_bytes = image_bytes & ~(size_t)255;
-    e->free_blocks[0] = e->arena_bytes;
     char *p[NS];
     char *at = (char *)e->dmem;
     for (int i = 0; i < NS; i++) {
@@ -595,6 +620,58 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
     e->last.ctl.tail = e->C;
     *out = e;
     return 0;
+}
+
+// a lazy arena's cap (chunks already made stay); the device memory its chunks hold
+void lr_engine_arena_cap(LrhEngine *e, size_t cap) { e->arena_cap = cap & ~(size_t)255; }
+size_t lr_engine_arena_held(const LrhEngine *e) { return e->arena_bytes; }
+
+// frees the chunks of a lazy arena that hold no image
+void lr_engine_arena_trim(LrhEngine *e)
+{
+    if (!e->arena_lazy) return;
+    for (size_t k = 0; k < e->chunks.size();) {
+        const ArenaChunk c = e->chunks[k];
+        auto it = e->free_blocks.find(c.base);
+        if (it == e->free_blocks.end() || it->second != c.bytes) {
+            k++;
+            continue;
+        }
+        (void)hipSetDevice(e->device);
+        (void)hipStreamSynchronize(e->stream);
+        (void)hipFree(c.ptr);
+        e->free_blocks.erase(it);
+        e->arena_bytes -= c.bytes;
+        e->chunks.erase(e->chunks.begin() + (long)k);
+    }
+}
+
+extern "C" {
+
+double lrh_now_ms(void) { return now_ms(); }
+
+void lrh_engine_destroy(LrhEngine *e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (hipEvent_t ev : e->pevs) (void)hipEventDestroy(ev);
+    for (int b = 0; b < 2; b++) {
+        if (e->pin[b]) (void)hipHostFree(e->pin[b]);
+        if (e->pev[b]) (void)hipEventDestroy(e->pev[b]);
+    }
+    if (e->dmem) (void)hipFree(e->dmem);
+    for (const ArenaChunk &ch : e->chunks) (void)hipFree(ch.ptr);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+
+int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t image_bytes, LrhEngine **out)
+{
+    return lr_engine_create(c, slots_log2, max_tree_depth, image_bytes, false, out);
 }
 
 int lrh_engine_tune_set(LrhEngine *e, const LrhTune *t)
@@ -745,7 +822,7 @@ int lrh_engine_image(LrhEngine *e, int64_t id, float *d_out, int add)
     if (h.state != LRH_JOB_DONE) return lr_fail(LRH_EAGAIN, "lrh_engine_image: job %lld is not done", (long long)id);
     const int npix = h.desc.w * h.desc.h;
     hipLaunchKernelGGL(k_resolve, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, e->stream,
-                       (const long long *)(e->arena + h.off), npix, (float)h.desc.spp, (float4 *)d_out, add);
+                       (const long long *)arena_ptr(e, h.off), npix, (float)h.desc.spp, (float4 *)d_out, add);
     ENG_HIP(hipGetLastError());
     ENG_HIP(hipStreamSynchronize(e->stream));
     return 0;

@@ -516,10 +516,10 @@ __global__ __launch_bounds__(256) void k_refit(float4 *nodes, int nn, const uint
     for (;;) {
         const uint32_t p = parent[cur];
         if (p == LR_NONE) return;                   // the root
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);              // this record is out before the arrival
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (__hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+        // release: this record is visible before the arrival; acquire: the sibling's record,
+        // written before its arrival, is visible to the loads below (at most 2 ninst - 1
+        // records, so the guaranteed ordering costs nothing that matters)
+        if (__hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
         const uint32_t pa = __float_as_uint(nodes[2 * (size_t)p].w), pb = __float_as_uint(nodes[2 * (size_t)p + 1].w);
         const uint32_t sib = pa == cur ? pb : pa;
         const uint64_t q0 = ld_dev64(w + 4 * (size_t)sib), q1 = ld_dev64(w + 4 * (size_t)sib + 1);

@@ -89,8 +89,14 @@ void admit(LrhServer *s)
             return;   // until running jobs end and finished ones are released
         }
         int64_t eid = -1;
-        const int rc = lrh_engine_submit(s->eng, &j.job, &eid);
-        if (rc == LRH_EAGAIN) return;
+        int rc = lrh_engine_submit(s->eng, &j.job, &eid);
+        if (rc == LRH_EAGAIN) {
+            // the image may not fit the arena's chunks as they lie: free those holding no image
+            // (what the cap then leaves can be made into one chunk) and try once more
+            lr_engine_arena_trim(s->eng);
+            rc = lrh_engine_submit(s->eng, &j.job, &eid);
+            if (rc == LRH_EAGAIN) return;
+        }
         s->waiting.pop_front();
         if (rc) {
             fail_job(s, j, rc);
@@ -174,9 +180,10 @@ int lrh_server_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
 {
     if (!c || !out || budget_bytes < 4096) return lr_fail(LRH_EINVAL, "lrh_server_create: bad arguments");
     *out = nullptr;
-    // the image arena: what the budget leaves for images at most (scenes are the callers')
+    // the image arena is made chunk by chunk as admitted images need it, within what the budget
+    // leaves beside the resident scenes: scenes plus chunks never pass the budget
     LrhEngine *e = nullptr;
-    int rc = lrh_engine_create(c, slots_log2, max_tree_depth, budget_bytes, &e);
+    int rc = lr_engine_create(c, slots_log2, max_tree_depth, budget_bytes, true, &e);
     if (rc) return rc;
     if (tune && (rc = lrh_engine_tune_set(e, tune)) != 0) {
         lrh_engine_destroy(e);
@@ -194,12 +201,16 @@ int lrh_server_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
 int lrh_server_add_scene(LrhServer *s, const LrhPtScene *scene)
 {
     if (!s || !scene) return lr_fail(LRH_EINVAL, "lrh_server_add_scene: bad arguments");
+    std::lock_guard<std::mutex> eg(s->emu);
     std::lock_guard<std::mutex> g(s->mu);
     if (std::find(s->scenes.begin(), s->scenes.end(), scene) != s->scenes.end()) return 0;
-    if (s->scene_bytes + s->image_bytes + scene->bytes > s->budget)
+    // the arena's chunks are device memory too: those holding no image go first if need be
+    if (s->scene_bytes + scene->bytes + lr_engine_arena_held(s->eng) > s->budget) lr_engine_arena_trim(s->eng);
+    if (s->scene_bytes + scene->bytes + lr_engine_arena_held(s->eng) > s->budget)
         return lr_fail(LRH_EREFUSED, "lrh_server_add_scene: %zu bytes do not fit the budget", scene->bytes);
     s->scenes.push_back(scene);
     s->scene_bytes += scene->bytes;
+    lr_engine_arena_cap(s->eng, s->budget - s->scene_bytes);
     return 0;
 }
 
@@ -300,7 +311,7 @@ int lrh_server_stats(LrhServer *s, LrhServerStats *st)
     st->rays = es.rays_ext + es.rays_shadow;
     st->busy_ms = s->busy_ms;
     st->budget = s->budget;
-    st->in_use = s->scene_bytes + s->image_bytes;
+    st->in_use = s->scene_bytes + lr_engine_arena_held(s->eng);
     if (!s->lat.empty()) {
         std::vector<double> v = s->lat;
         std::sort(v.begin(), v.end());

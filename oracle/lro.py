@@ -11,6 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liblro.so")
+LIB_D = os.path.join(HERE, "liblro_d.so")
 REF_F = os.path.join(HERE, "_ref", "libref_f.so")
 REF_D = os.path.join(HERE, "_ref", "libref_d.so")
 
@@ -23,7 +24,21 @@ class LroCount(ctypes.Structure):
 
 
 _L = None
+_LD = None
 _R = {}
+
+
+def lib_d():
+    """the oracle's path tracer built in double (liblro_d.so): lro_pt_image at the same samples"""
+    global _LD
+    if _LD is None:
+        if not os.path.exists(LIB_D):
+            raise RuntimeError(f"{LIB_D} missing: make -C oracle")
+        L = ctypes.CDLL(LIB_D)
+        L.lro_pt_image.restype = None
+        L.lro_pt_image.argtypes = [P, P, I, I, U, I, U64, F, I, I, P]
+        _LD = L
+    return _LD
 
 
 def lib():
@@ -61,7 +76,9 @@ def lib():
             "lro_blue_noise": (None, [P]),
             "lro_sampler_2d": (None, [P, I, I, U, U, I, P]),
             "lro_bsdf_case": (None, [P, I, P, P, P, P, F, F, F, P]),
+            "lro_bsdf_cases": (None, [P, I, P, P]),
             "lro_pt_image": (None, [P, P, I, I, U, I, U64, F, I, I, P]),
+            "lro_pt_window": (None, [P, P, I, I, I, I, I, I, U, I, U64, F, I, I, P]),
             "lro_env_prepare": (I, [P, P, I, I, F, F]),
             "lro_env_free": (None, [P]),
             "lro_env_eval": (None, [P, P, P]),
@@ -289,7 +306,8 @@ class LroScene(ctypes.Structure):
                 ("mats", ctypes.c_void_p), ("nmats", ctypes.c_int), ("light", ctypes.c_int), ("eps", ctypes.c_float),
                 ("margin", ctypes.c_float), ("mask", ctypes.c_void_p), ("tab", ctypes.c_void_p), ("env", ctypes.c_void_p),
                 ("mtris", ctypes.c_void_p), ("mesh_first", ctypes.c_void_p), ("nmeshes", ctypes.c_int),
-                ("inst", ctypes.c_void_p), ("ninst", ctypes.c_int)]
+                ("inst", ctypes.c_void_p), ("ninst", ctypes.c_int), ("mesh_box", ctypes.c_void_p),
+                ("tnodes", ctypes.c_void_p), ("taux", ctypes.c_void_p)]
 
 
 class Env:
@@ -381,6 +399,14 @@ def blue_noise():
     return m
 
 
+def bsdf_cases(cases, tab):
+    """lro_bsdf_cases: (n, 17) float32 cases -> (n, 16) float32"""
+    c = np.ascontiguousarray(cases, np.float32)
+    out = np.zeros((c.shape[0], 16), np.float32)
+    lib().lro_bsdf_cases(ptr(c), c.shape[0], ptr(np.ascontiguousarray(tab, np.float32)), ptr(out))
+    return out
+
+
 def mats_of(mats):
     """Materials (any ctypes array of structures laid out as LroMat) as an LroMat array"""
     out = (LroMat * len(mats))()
@@ -388,14 +414,26 @@ def mats_of(mats):
     return out
 
 
-def pt_image(scene, W, H, s0, ns, seed=0, clamp=0.0, max_depth=8, mis=True, tab=None, mask=None):
+def pt_image(scene, W, H, s0, ns, seed=0, clamp=0.0, max_depth=8, mis=True, tab=None, mask=None, double=False):
     """lro_pt_image: the rgb sums of samples s0 .. s0+ns-1 of every pixel, (H, W, 3) float32.
     scene: dict of prims (n, 24), tris (m, 12), mats, light, cam, eps, margin; tab, mask: the
-    BSDF tables and blue-noise ranks (as the device gets them)"""
+    BSDF tables and blue-noise ranks (as the device gets them); double: the FP64 build"""
     sc, keep = scene_struct(scene, tab, mask)
     out = np.zeros((H, W, 3), np.float32)
     cam = np.ascontiguousarray(scene["cam"], np.float32)
-    lib().lro_pt_image(ctypes.byref(sc), ptr(cam), W, H, s0, ns, seed, clamp, max_depth, 1 if mis else 0, ptr(out))
+    (lib_d() if double else lib()).lro_pt_image(ctypes.byref(sc), ptr(cam), W, H, s0, ns, seed, clamp, max_depth, 1 if mis else 0, ptr(out))
+    del keep
+    return out
+
+
+def pt_window(scene, W, H, x0, y0, cw, ch, s0, ns, seed=0, clamp=0.0, max_depth=8, mis=True, tab=None, mask=None):
+    """lro_pt_window: pt_image's sums for the pixels [x0, x0+cw) x [y0, y0+ch) of a W x H image,
+    (ch, cw, 3) float32"""
+    sc, keep = scene_struct(scene, tab, mask)
+    out = np.zeros((ch, cw, 3), np.float32)
+    cam = np.ascontiguousarray(scene["cam"], np.float32)
+    lib().lro_pt_window(ctypes.byref(sc), ptr(cam), W, H, x0, y0, cw, ch, s0, ns, seed, clamp, max_depth, 1 if mis else 0,
+                        ptr(out))
     del keep
     return out
 
@@ -408,17 +446,49 @@ def scene_struct(scene, tab=None, mask=None):
     tris = np.ascontiguousarray(scene.get("tris", np.zeros((0, 12), np.float32)), np.float32).reshape(-1, 12)
     mats = mats_of(scene["mats"])
     env = scene.get("sky")
-    mt, mf, inst = None, None, None
+    mt, mf, inst, mbox = None, None, None, None
     if scene.get("instances") is not None:
         mt = np.ascontiguousarray(scene["meshes"][0], np.float32)
         mf = np.ascontiguousarray(scene["meshes"][1], np.int32)
         inst = np.ascontiguousarray(scene["instances"], INST_DTYPE)
-    keep = [prims, tris, mats, tab, mask, env, mt, mf, inst]
+        if scene.get("cull", True):
+            mbox = mesh_boxes(mt, mf)
+    tn, ta = None, None
+    if scene.get("tree", len(tris) > 4096) and len(tris) > 1:
+        # the oracle's own LBVH over the triangles (pinned against brute force), walked instead of it
+        tn, ta = tri_tree(tris)
+    keep = [prims, tris, mats, tab, mask, env, mt, mf, inst, mbox, tn, ta]
     sc = LroScene(ptr(prims), prims.shape[0], ptr(tris), tris.shape[0], ctypes.cast(mats, ctypes.c_void_p), len(mats),
                   scene.get("light", -1), scene["eps"], scene.get("margin", 1e-3), ptr(mask), ptr(tab),
                   ctypes.addressof(env.s) if env is not None else None, ptr(mt), ptr(mf),
-                  0 if mf is None else len(mf) - 1, ptr(inst), 0 if inst is None else len(inst))
+                  0 if mf is None else len(mf) - 1, ptr(inst), 0 if inst is None else len(inst), ptr(mbox),
+                  ptr(tn), ptr(ta))
     return sc, keep
+
+
+_TREES = {}
+
+
+def tri_tree(tris):
+    """the oracle's LBVH over triangles (nodes, aux), kept per triangle array"""
+    key = (tris.shape[0], hash(tris[:: max(1, tris.shape[0] // 64)].tobytes()), hash(tris[-1].tobytes()))
+    if key not in _TREES:
+        nodes, aux, _, _, _ = build_tris(tris)
+        _TREES.clear()
+        _TREES[key] = (np.ascontiguousarray(nodes, np.uint32), np.ascontiguousarray(aux, np.uint32))
+    return _TREES[key]
+
+
+def mesh_boxes(mtris, first):
+    """each mesh's object-space box over its triangles' corners q, q + u, q + v: (nmeshes, 6)"""
+    t = np.asarray(mtris, np.float32).reshape(-1, 12).astype(np.float64)
+    corners = np.stack([t[:, 0:3], t[:, 0:3] + t[:, 4:7], t[:, 0:3] + t[:, 8:11]], 1)
+    out = np.zeros((len(first) - 1, 6), np.float32)
+    for m in range(len(first) - 1):
+        c = corners[first[m]:first[m + 1]].reshape(-1, 3)
+        if len(c):
+            out[m, 0:3], out[m, 3:6] = c.min(0), c.max(0)
+    return out
 
 
 def scene_trace(scene, rays, any_hit=False):

@@ -7,7 +7,11 @@
  * One deliberate difference from trace_image: each (pixel, sample) has its own PCG32 stream
  * (the pixel's seed of tracer.c:339 xor the sample index times a constant), so that samples can
  * be traced in any order, on any device, alone or batched, and give the same numbers. The
- * device path tracer uses the same rule. */
+ * device path tracer uses the same rule.
+ *
+ * Built with -DLRO_DOUBLE (liblro_d.so), `real` is double: the image oracle in FP64 at the
+ * identical samples (the sampler's numbers and the PCG32 streams are integers, the same in
+ * either build). */
 #include "lro_pt.h"
 
 #include <float.h>
@@ -15,7 +19,21 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef LRO_DOUBLE
+/* the same code in double (oracle/liblro_d.so): the same random numbers, every operation of the
+   light transport rounded to FP64 instead of FP32 (float literals are exact in double) */
+typedef double real;
+#define sqrtf sqrt
+#define fabsf fabs
+#define sinf sin
+#define cosf cos
+#define atan2f atan2
+#define acosf acos
+#define floorf floor
+#define copysignf copysign
+#else
 typedef float real;
+#endif
 typedef struct { real x, y, z; } V;
 static inline V v3(real x, real y, real z) { V r = {x, y, z}; return r; }
 static inline V vadd(V a, V b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
@@ -188,7 +206,10 @@ void lro_sampler_2d(const uint16_t *mask, int x, int y, uint32_t index, uint32_t
     LroRng rng;
     lro_rng_seed(&rng, 1, 1);
     Smp s = {&rng, index, seed, x, y, 1, mask};
-    sampler_2d(&s, dim, &out[0], &out[1]);
+    real a, b;
+    sampler_2d(&s, dim, &a, &b);
+    out[0] = (float)a;
+    out[1] = (float)b;
 }
 
 /* ---- the BSDF (brdf.h:43-348) ---- */
@@ -495,6 +516,23 @@ void lro_bsdf_case(const LroMat *m, int outside, const float *tab, const float n
     out[12] = s.pdf; out[13] = (float)s.delta; out[14] = (float)s.through; out[15] = 0;
 }
 
+/* lro_bsdf_case over n cases of 17 floats (base rgb, metallic, roughness, ior, transmission,
+ * outside, v xyz, l xyz, ul, u1, u2; material_pbr, scene.h:70-74; the normal +z): 16 floats each */
+void lro_bsdf_cases(const float *in, int n, const float *tab, float *out)
+{
+    const float nz[3] = {0, 0, 1};
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        const float *c = in + 17 * (size_t)i;
+        LroMat m;
+        memset(&m, 0, sizeof(m));
+        m.type = LRO_MAT_PBR;
+        m.albedo[0] = c[0]; m.albedo[1] = c[1]; m.albedo[2] = c[2];
+        m.metallic = c[3]; m.roughness = c[4]; m.ior = c[5]; m.transmission = c[6];
+        lro_bsdf_case(&m, c[7] > 0.5f, tab, nz, c + 8, c + 11, c[14], c[15], c[16], out + 16 * (size_t)i);
+    }
+}
+
 /* ---- the scene: shapes and their hits (intersect.h:66-112, intersect.c:4-67) ---- */
 typedef struct { real t; V p, n, ng; int front, prim, inst; } Hit;
 
@@ -610,12 +648,35 @@ static V unit_box_normal(V p)
     if (ay >= az) return v3(0, p.y < 0 ? -1.0f : 1.0f, 0);
     return v3(0, 0, p.z < 0 ? -1.0f : 1.0f);
 }
+/* does the ray (object space) meet the padded box b in [tmin, tmax]? in double, so that the
+   test cannot drop a hit the triangle test keeps */
+static int box_meets(const float *b, V o, V d, real tmin, real tmax)
+{
+    double lo = tmin, hi = tmax;
+    const double oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    for (int a = 0; a < 3; a++) {
+        const double pad = 1e-3 * ((double)b[3 + a] - b[a]) + 1e-6;
+        const double bl = b[a] - pad, bh = b[3 + a] + pad;
+        if (fabs(dd[a]) < 1e-30) {
+            if (oo[a] < bl || oo[a] > bh) return 0;
+            continue;
+        }
+        double t0 = (bl - oo[a]) / dd[a], t1 = (bh - oo[a]) / dd[a];
+        if (t0 > t1) { const double x = t0; t0 = t1; t1 = x; }
+        if (t0 > lo) lo = t0;
+        if (t1 < hi) hi = t1;
+        if (lo > hi * (1 + 1e-9) + 1e-9) return 0;
+    }
+    return 1;
+}
+
 /* the closest (any: the first) hit among instance k's shapes, in its space: local id */
 static int inst_test(const LroScene *s, int k, V o, V d, real tmin, real tmax, int any, real *t, int *local)
 {
     const LroInst *in = &s->inst[k];
     V oo, od;
     to_object(in, o, d, &oo, &od);
+    if (in->kind >= 0 && s->mesh_box && !box_meets(s->mesh_box + 6 * (size_t)in->kind, oo, od, tmin, tmax)) return 0;
     if (in->kind == LRO_INST_SPHERE || in->kind == LRO_INST_BOX) {
         real tt;
         int h = in->kind == LRO_INST_SPHERE ? hit_unit_sphere(oo, od, tmin, tmax, &tt) : hit_unit_box(oo, od, tmin, tmax, &tt);
@@ -636,15 +697,64 @@ static int inst_test(const LroScene *s, int k, V o, V d, real tmin, real tmax, i
     return found;
 }
 
+/* the slab test of a node's box against a ray, (tmin, tmax) */
+static int node_meets(const uint32_t *nd, V o, V inv, real tmin, real tmax)
+{
+    const float *lo = (const float *)nd, *hi = (const float *)(nd + 4);
+    real t0 = (lo[0] - o.x) * inv.x, t1 = (hi[0] - o.x) * inv.x;
+    real a = t0 < t1 ? t0 : t1, b = t0 < t1 ? t1 : t0;
+    t0 = (lo[1] - o.y) * inv.y; t1 = (hi[1] - o.y) * inv.y;
+    a = rfmax(a, t0 < t1 ? t0 : t1); b = rfmin(b, t0 < t1 ? t1 : t0);
+    t0 = (lo[2] - o.z) * inv.z; t1 = (hi[2] - o.z) * inv.z;
+    a = rfmax(a, t0 < t1 ? t0 : t1); b = rfmin(b, t0 < t1 ? t1 : t0);
+    return rfmax(a, tmin) <= rfmin(b, tmax);
+}
+/* the triangles through the tree: the closest (any: the first) in (tmin, *tmax), its index in
+   *found; every box meeting the ray is opened (no order), so the kept hit is the brute force's
+   but for equal t */
+static int tree_tris(const LroScene *s, V o, V d, real tmin, real *tmax, int any, int *found)
+{
+    uint32_t stack[512];
+    int sp = 0, hit = 0;
+    const V inv = v3(fabsf(d.x) > 1e-30f ? 1 / d.x : 1e30f, fabsf(d.y) > 1e-30f ? 1 / d.y : 1e30f,
+                     fabsf(d.z) > 1e-30f ? 1 / d.z : 1e30f);
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const uint32_t *nd = s->tnodes + 8 * (size_t)stack[--sp];
+        if (!node_meets(nd, o, inv, tmin, *tmax)) continue;
+        if (nd[7] & 0x80000000u) {
+            for (uint32_t k = 0; k < (nd[7] & 0x7fffffffu); k++) {
+                const int i = (int)s->taux[nd[3] + k];
+                real t;
+                if (hit_tri(s->tris + 12 * (size_t)i, o, d, tmin, *tmax, &t)) {
+                    *found = i;
+                    *tmax = t;
+                    hit = 1;
+                    if (any) return 1;
+                }
+            }
+            continue;
+        }
+        if (sp + 2 > 512) abort();   /* deeper than any tree the tests build */
+        stack[sp++] = nd[7];
+        stack[sp++] = nd[3];
+    }
+    return hit;
+}
+
 /* every shape in id order, then every instance, the closest kept (intersect_world's brute
-   force, accel.c:264-277) */
+   force, accel.c:264-277); the triangles through the tree where the scene has one */
 static int scene_intersect(const LroScene *s, V o, V d, real tmin, real tmax, Hit *h)
 {
     int found = -1, finst = -1;
-    int n = s->nprims + s->ntris;
+    int n = s->tnodes ? s->nprims : s->nprims + s->ntris;
     for (int i = 0; i < n; i++) {
         real t;
         if (shape_test(s, i, o, d, tmin, tmax, &t)) { found = i; tmax = t; }
+    }
+    if (s->tnodes && s->ntris > 0) {
+        int ti;
+        if (tree_tris(s, o, d, tmin, &tmax, 0, &ti)) found = s->nprims + ti;
     }
     for (int k = 0; k < s->ninst; k++) {
         real t;
@@ -685,10 +795,15 @@ static int scene_intersect(const LroScene *s, V o, V d, real tmin, real tmax, Hi
 }
 static int scene_occluded(const LroScene *s, V o, V d, real tmin, real tmax)
 {
-    int n = s->nprims + s->ntris;
+    int n = s->tnodes ? s->nprims : s->nprims + s->ntris;
     for (int i = 0; i < n; i++) {
         real t;
         if (shape_test(s, i, o, d, tmin, tmax, &t)) return 1;
+    }
+    if (s->tnodes && s->ntris > 0) {
+        int ti;
+        real tm = tmax;
+        if (tree_tris(s, o, d, tmin, &tm, 1, &ti)) return 1;
     }
     for (int k = 0; k < s->ninst; k++) {
         real t;
@@ -1062,14 +1177,15 @@ void lro_pt_seed(LroRng *rng, uint64_t pixel, uint32_t k, uint64_t seed)
                           ((uint64_t)k * 0xA0761D6478BD642FULL), pixel);
 }
 
-/* trace_image (tracer.c:329-356): the sum of samples s0 .. s0+ns-1 of every pixel into sum
- * (rgb, float, the samples added in order; NaNs skipped, clamp applied) */
-void lro_pt_image(const LroScene *s, const float cam[12], int W, int H, uint32_t s0, int ns, uint64_t seed, float clamp,
-                  int max_depth, int mis, float *sum)
+/* trace_image (tracer.c:329-356) over the pixels x0 .. x0+cw-1, y0 .. y0+ch-1 of a W x H image:
+ * the sum of samples s0 .. s0+ns-1 of each into sum (ch x cw, rgb, float, the samples added in
+ * order; NaNs skipped, clamp applied) */
+void lro_pt_window(const LroScene *s, const float cam[12], int W, int H, int x0, int y0, int cw, int ch, uint32_t s0,
+                   int ns, uint64_t seed, float clamp, int max_depth, int mis, float *sum)
 {
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int y = 0; y < H; y++)
-        for (int x = 0; x < W; x++) {
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int y = y0; y < y0 + ch; y++)
+        for (int x = x0; x < x0 + cw; x++) {
             uint64_t pixel = (uint64_t)y * W + x;
             V acc = v3(0, 0, 0);
             for (int q = 0; q < ns; q++) {
@@ -1087,7 +1203,14 @@ void lro_pt_image(const LroScene *s, const float cam[12], int W, int H, uint32_t
                 if (clamp > 0 && vmax(c) > clamp) c = vscale(c, clamp / vmax(c));
                 acc = vadd(acc, c);
             }
-            float *o3 = sum + 3 * pixel;
+            float *o3 = sum + 3 * ((size_t)(y - y0) * cw + (x - x0));
             o3[0] += acc.x; o3[1] += acc.y; o3[2] += acc.z;
         }
+}
+
+/* the whole image */
+void lro_pt_image(const LroScene *s, const float cam[12], int W, int H, uint32_t s0, int ns, uint64_t seed, float clamp,
+                  int max_depth, int mis, float *sum)
+{
+    lro_pt_window(s, cam, W, H, 0, 0, W, H, s0, ns, seed, clamp, max_depth, mis, sum);
 }

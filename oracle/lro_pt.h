@@ -71,6 +71,12 @@ typedef struct {
        mesh_first[m] .. mesh_first[m + 1] - 1 of mtris) and the instances, besides the shapes above */
     const float *mtris; const int *mesh_first; int nmeshes;
     const LroInst *inst; int ninst;
+    /* optional: each mesh's box in object space (lo.xyz, hi.xyz), padded: an instance whose box
+       the ray misses is skipped (a cull that drops no hit: the brute force stays exact) */
+    const float *mesh_box;
+    /* optional: a binary tree over tris (lro.h's node layout, aux = triangle index), walked for
+       the triangles instead of the brute force (nodes NULL: brute force) */
+    const uint32_t *tnodes, *taux;
 } LroScene;
 
 /* The closest hit of n rays (8 floats each) in a scene, by brute force over every shape and
@@ -88,11 +94,17 @@ void lro_sampler_2d(const uint16_t *mask, int x, int y, uint32_t index, uint32_t
  * pdf, delta, through */
 void lro_bsdf_case(const LroMat *m, int outside, const float *tab, const float n[3], const float v[3], const float l[3],
                    float ul, float u1, float u2, float out[16]);
+/* lro_bsdf_case on n cases of 17 floats (material_pbr: base rgb, metallic, roughness, ior,
+ * transmission; outside; v; l; ul, u1, u2; the normal +z) into 16 floats each */
+void lro_bsdf_cases(const float *in, int n, const float *tab, float *out);
 /* the per-(pixel, sample) PCG32 stream */
 void lro_pt_seed(LroRng *rng, uint64_t pixel, uint32_t k, uint64_t seed);
 /* adds samples s0 .. s0+ns-1 of every pixel (rgb sums) into sum */
 void lro_pt_image(const LroScene *s, const float cam[12], int W, int H, uint32_t s0, int ns, uint64_t seed, float clamp,
                   int max_depth, int mis, float *sum);
+/* the same for the pixels x0 .. x0+cw-1, y0 .. y0+ch-1 of the W x H image only (sum: ch x cw) */
+void lro_pt_window(const LroScene *s, const float cam[12], int W, int H, int x0, int y0, int cw, int ch, uint32_t s0,
+                   int ns, uint64_t seed, float clamp, int max_depth, int mis, float *sum);
 
 #ifdef __cplusplus
 }

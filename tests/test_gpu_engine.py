@@ -99,6 +99,31 @@ def test_engine_matches_oracle_sample_for_sample(ctx, scenes):
     assert np.median(rel) < 1e-5 and np.mean(rel > 1e-3) < 0.01
 
 
+def test_engine_lamp_lit_ggx_scene_matches_oracle(ctx):
+    # a scene of the batched workload as the benchmark renders it: a 30,000-triangle soup of 14
+    # GGX materials (metals, rough dielectrics, two clear glasses), a diffuse floor, the quad
+    # lamp; two cameras, 64 x 64, 16 spp, against the oracle sample for sample
+    sc = lr.batch_scene(30000, 31)
+    ps = lr.PtScene(ctx, sc["prims"], sc["tris"], sc["mats"], 0, sc["eps"], sc["margin"])
+    W = H = 64
+    eng = lr.Engine(ctx, slots_log2=16, max_tree_depth=max(64, ps.info()[0]), image_bytes=64 << 20)
+    for cam_seed in (301, 302):
+        cam = lr.batch_camera(cam_seed, W, H)
+        say(f"engine: lamp-lit GGX soup 30000 tris, camera {cam_seed}, {W}x{H}x16 vs oracle")
+        g = run_batched(eng, [lr.make_job(ps, cam, W, H, 16, seed=7)])[0]
+        assert np.all(g[..., 3] == 16)
+        o = lro.pt_image(dict(sc, cam=cam), W, H, 0, 16, seed=7, tab=lr.brdf_tables(), mask=lro.blue_noise())
+        rel = np.abs(g[..., :3] - o) / np.maximum(np.abs(o), 1e-3)
+        say(f"  median rel {np.median(rel):.2e}, >1e-3 in {np.mean(rel > 1e-3):.4f}, means {g[..., :3].mean():.5f} "
+            f"{o.mean():.5f}")
+        assert o.mean() > 0
+        assert np.median(rel) < 2e-5
+        assert np.mean(rel > 1e-3) < 0.02
+        assert abs(g[..., :3].mean() - o.mean()) < 2e-3 * o.mean()
+    eng.close()
+    ps.close()
+
+
 def test_batched_equals_plain_bit_for_bit(ctx, scenes):
     jobs = job_set(scenes)
     say(f"engine: {len(jobs)} jobs batched vs plain")

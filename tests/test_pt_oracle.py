@@ -256,3 +256,16 @@ def test_oracle_image_matches_reference_fp64():
     assert np.mean(np.abs(z) > 3) < 0.03 and np.abs(z).max() < 8
     sig = np.sqrt((np.var([b.mean() for b in batches], ddof=1) + np.var([r.mean() for r in refs], ddof=1)) / K)
     assert abs(ours.mean() - theirs.mean()) < 3 * sig + 1e-3 * theirs.mean()
+
+
+def test_tree_walked_oracle_equals_brute_force():
+    # the oracle's path tracer may walk its own LBVH (pinned against brute force) for a scene's
+    # triangles: the images are the brute force's, to the bit but for rays with two hits at
+    # exactly equal t (none expected on a soup)
+    sc = lr.batch_scene(20000, 5)
+    sc["cam"] = lr.batch_camera(9, 24, 24)
+    tab, mask = lr.brdf_tables(), lro.blue_noise()
+    a = lro.pt_image(dict(sc, tree=True), 24, 24, 0, 4, seed=3, tab=tab, mask=mask)
+    b = lro.pt_image(dict(sc, tree=False), 24, 24, 0, 4, seed=3, tab=tab, mask=mask)
+    assert a.sum() > 0
+    assert np.mean(a.view(np.uint32) == b.view(np.uint32)) > 0.999

@@ -231,3 +231,39 @@ def test_unit_sphere_and_box_instances():
     assert np.allclose(h[found, 0], best[found], rtol=2e-5)
     assert np.all(ins[found & (ts < tb)] == 0) and np.all(ins[found & (tb < ts)] == 1)
     assert np.all(ids[found & (ts < tb)] == 0xfffffff0) and np.all(ids[found & (tb < ts)] == 0xfffffff1)
+
+
+def test_instance_cull_drops_no_hit_and_window_equals_image():
+    # the instanced workload's meshes, scaled down to 60 instances: the culled brute force (mesh
+    # boxes, padded) gives the unculled one's hits bit for bit, closest and any
+    wl = lr.instanced_workload(0, 0, n_dynamic=30, n_static=30, n_props=4, n_hulls=2)
+    mt, mf, inst = wl["mtris"], wl["mesh_first"], wl["instances"]
+    oi = np.zeros(len(inst), lro.INST_DTYPE)
+    for k in range(len(inst)):
+        oi[k]["inv"] = lr.affine_inverse(inst[k]["xform"])
+        oi[k]["kind"], oi[k]["material"] = inst[k]["kind"], inst[k]["material"]
+    base = {"mats": wl["mats"], "eps": 1e-4, "light": -1, "meshes": (mt, mf), "instances": oi, "cam": wl["cam"]}
+    rng = np.random.default_rng(3)
+    n = 1500
+    o = rng.uniform(-15, 15, size=(n, 3)).astype(np.float32)
+    o[:, 1] = rng.uniform(1, 10, n)
+    at = np.array([np.asarray(inst[k]["xform"]).reshape(3, 4)[:, 3] for k in range(1, len(inst))])
+    tgt = at[rng.integers(0, len(at), n)] + rng.normal(scale=0.3, size=(n, 3))
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 3], rays[:, 4:7], rays[:, 7] = o, 1e-4, d, 1e30
+    for any_hit in (False, True):
+        culled = lro.scene_trace(base, rays, any_hit)
+        plain = lro.scene_trace(dict(base, cull=False), rays, any_hit)
+        assert np.array_equal(culled.view(np.uint32), plain.view(np.uint32))
+    assert (culled[:, 1].view(np.uint32) != 0xffffffff).mean() > 0.3
+    # a window of the image: the image's pixels there, to the bit
+    sky = lro.Env(lr.sky(64, 35.0, 60.0))
+    sc = dict(base, sky=sky)
+    W, H = 48, 27
+    tab, mask = lr.brdf_tables(), lro.blue_noise()
+    full = lro.pt_image(sc, W, H, 0, 2, seed=4, tab=tab, mask=mask)
+    win = lro.pt_window(sc, W, H, 20, 9, 12, 7, 0, 2, seed=4, tab=tab, mask=mask)
+    assert np.array_equal(win.view(np.uint32), full[9:16, 20:32].view(np.uint32))
+    assert full.sum() > 0
