@@ -78,6 +78,19 @@ int lrh_build(LrhContext *c, const float *d_boxes, int n, const float *h_bounds,
               uint32_t node0, uint32_t ref0, uint32_t ref_add, uint32_t *d_nodes, uint32_t *d_aux,
               uint32_t *d_sorted, LrhBuildStats *stats);
 
+/* lrh_build without allocating: no reference counterpart; for callers that cannot allocate device
+ * memory (code under HIP graph capture, a serving loop's per-frame top-level rebuild, harnesses
+ * that forbid hipMalloc). The same trees, bit for bit, as lrh_build. Its scratch is one fixed
+ * array in the library (8 MiB on each device it runs on, held until the process ends), so n is
+ * at most LRH_NOALLOC_MAX_ITEMS (LRH_EREFUSED above); it calls no allocator (stats, if given,
+ * makes it wait as lrh_build does). Every call on a device shares that device's array: calls
+ * that overlap on one device (two streams, or two host threads) race and must be serialised
+ * by the caller; calls in order on one stream are safe. */
+#define LRH_NOALLOC_MAX_ITEMS 65536
+int lrh_build_noalloc(LrhContext *c, const float *d_boxes, int n, const float *h_bounds, int builder,
+                      uint32_t node0, uint32_t ref0, uint32_t ref_add, uint32_t *d_nodes, uint32_t *d_aux,
+                      uint32_t *d_sorted, LrhBuildStats *stats);
+
 /* Closest hits (any = 0) or first hits (any = 1, shadow rays) of n rays against one binary
  * tree of triangles in the reference GPU layout: the walk of trace.wgsl traverse
  * (:270-392) for a scene of triangles. The shape id of a hit is the aux entry of its leaf.

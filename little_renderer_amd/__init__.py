@@ -93,6 +93,7 @@ def lib():
         "lrh_tri_boxes": (I, [P, P, I, P]),
         "lrh_bounds": (I, [P, P, I, P]),
         "lrh_build": (I, [P, P, I, P, I, U, U, U, P, P, P, ctypes.POINTER(BuildStats)]),
+        "lrh_build_noalloc": (I, [P, P, I, P, I, U, U, U, P, P, P, ctypes.POINTER(BuildStats)]),
         "lrh_trace": (I, [P, P, U, P, P, P, I, I, P, I]),
         "lrh_trace_queue": (I, [P, P, U, P, P, P, I, P, I, P, I]),
         "lrh_trace_count": (I, [P, P, U, P, P, P, I, P, I, P, I, P]),

@@ -73,6 +73,12 @@ int lrh_create(LrhContext **out, int device, void *stream)
         lrh_destroy(c);
         return lr_fail(LRH_EDEVICE, "lrh_create: events or flags could not be made");
     }
+    // the device address of the no-allocation build's scratch, once per device (this one current)
+    const int rc = lr_noalloc_init(device);
+    if (rc) {
+        lrh_destroy(c);
+        return rc;
+    }
     *out = c;
     return 0;
 }
@@ -138,6 +144,20 @@ int lrh_build(LrhContext *c, const float *d_boxes, int n, const float *h_bounds,
     if (builder != LRH_LBVH && builder != LRH_PLOC) return lr_fail(LRH_EINVAL, "lrh_build: builder %d", builder);
     return lr_build_run(c, d_boxes, n, h_bounds, builder, node0, ref0, ref_add, d_nodes, d_aux, d_sorted,
                         d_sorted ? d_sorted + n : nullptr, stats);
+}
+
+int lrh_build_noalloc(LrhContext *c, const float *d_boxes, int n, const float *h_bounds, int builder, uint32_t node0,
+                      uint32_t ref0, uint32_t ref_add, uint32_t *d_nodes, uint32_t *d_aux, uint32_t *d_sorted,
+                      LrhBuildStats *stats)
+{
+    if (!c || !d_boxes || !d_nodes || !d_aux) return lr_fail(LRH_EINVAL, "lrh_build_noalloc: NULL argument");
+    if (n <= 0) return lr_fail(LRH_EINVAL, "lrh_build_noalloc: %d items (nothing to build)", n);
+    if (n > LRH_NOALLOC_MAX_ITEMS)
+        return lr_fail(LRH_EREFUSED, "lrh_build_noalloc: %d items: at most %d (its scratch is fixed)", n,
+                       LRH_NOALLOC_MAX_ITEMS);
+    if (builder != LRH_LBVH && builder != LRH_PLOC) return lr_fail(LRH_EINVAL, "lrh_build_noalloc: builder %d", builder);
+    LR_HIP(hipSetDevice(c->device));
+    return lr_build_noalloc(c, d_boxes, n, h_bounds, builder, node0, ref0, ref_add, d_nodes, d_aux, d_sorted, stats);
 }
 
 int lrh_trace(LrhContext *c, const uint32_t *d_nodes, uint32_t root, const uint32_t *d_aux, const float *d_tris,

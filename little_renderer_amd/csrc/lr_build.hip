@@ -9,6 +9,7 @@
 #include <stdarg.h>
 #include <stddef.h>
 #include <algorithm>
+#include <mutex>
 #include <stdio.h>
 
 #include "lr_device.h"
@@ -1133,14 +1134,15 @@ int lr_launch_bounds(LrhContext *c, const float *boxes, int n, LrBuildParamsDev 
 
 size_t lr_build_scratch_bytes(int n) { return layout(n).total; }
 
-int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bounds, int builder, uint32_t node0,
-                 uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux, uint32_t *sorted_codes_out,
-                 uint32_t *sorted_items_out, LrhBuildStats *stats)
+namespace {
+
+// one build with its scratch at S (layout(n).total bytes, the counters before keys0 at zero)
+int build_at(LrhContext *c, char *S, const float *boxes, int n, const float *host_bounds, int builder, uint32_t node0,
+             uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux, uint32_t *sorted_codes_out,
+             uint32_t *sorted_items_out, LrhBuildStats *stats)
 {
     const Layout L = layout(n);
-    int rc = lr_scratch_reserve(c, LR_SCRATCH_BUILD, L.total);
-    if (rc) return rc;
-    char *S = (char *)c->scratch[LR_SCRATCH_BUILD].base;
+    int rc = 0;
     auto at = [&](size_t off) { return (void *)(S + off); };
     LrBuildParamsDev *prm = (LrBuildParamsDev *)at(L.params);
     uint32_t *k0 = (uint32_t *)at(L.keys0), *v0 = (uint32_t *)at(L.vals0);
@@ -1239,4 +1241,78 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
         c->last_build_ms = ms;
     }
     return 0;
+}
+
+}  // namespace
+
+int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bounds, int builder, uint32_t node0,
+                 uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux, uint32_t *sorted_codes_out,
+                 uint32_t *sorted_items_out, LrhBuildStats *stats)
+{
+    int rc = lr_scratch_reserve(c, LR_SCRATCH_BUILD, layout(n).total);
+    if (rc) return rc;
+    return build_at(c, (char *)c->scratch[LR_SCRATCH_BUILD].base, boxes, n, host_bounds, builder, node0, ref0, ref_add,
+                    nodes, aux, sorted_codes_out, sorted_items_out, stats);
+}
+
+// ---- the build without allocation (lrh_build_noalloc) --------------------------------------------
+// Its scratch is this file-scope array: each device holds its own copy from the library's load
+// until the process ends. In host code `scratch` names a host-side stand-in, not the device
+// copy: the device address is taken once per device (lr_noalloc_init, from lrh_create, with
+// that device current) and only that address is handed to kernels.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+constexpr size_t NOALLOC_BYTES = (size_t)8 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+static char *g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_noalloc_mu;
+
+namespace {
+
+// clears n words (the counters that must start at zero: a kernel, so a caller may capture it)
+__global__ __launch_bounds__(256) void k_clear_words(uint32_t *p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0u;
+}
+
+}  // namespace
+
+int lr_noalloc_init(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return lr_fail(LRH_EINVAL, "device %d: the no-allocation build keeps %d devices", device, NOALLOC_MAX_DEVICES);
+    if (layout(LRH_NOALLOC_MAX_ITEMS).total > NOALLOC_BYTES)
+        return lr_fail(LRH_EDEVICE, "the no-allocation build's scratch (%zu bytes) is below its %d items' %zu",
+                       NOALLOC_BYTES, LRH_NOALLOC_MAX_ITEMS, layout(LRH_NOALLOC_MAX_ITEMS).total);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return 0;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return lr_fail(LRH_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    }
+    g_noalloc_ptr[device] = (char *)ptr;
+    return 0;
+}
+
+int lr_build_noalloc(LrhContext *c, const float *boxes, int n, const float *host_bounds, int builder, uint32_t node0,
+                     uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux, uint32_t *sorted,
+                     LrhBuildStats *stats)
+{
+    char *S = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (c->device >= 0 && c->device < NOALLOC_MAX_DEVICES) S = g_noalloc_ptr[c->device];
+    }
+    if (!S) return lr_fail(LRH_EDEVICE, "lrh_build_noalloc: device %d has no scratch address", c->device);
+    const Layout L = layout(n);
+    if (L.total > NOALLOC_BYTES)
+        return lr_fail(LRH_EREFUSED, "lrh_build_noalloc: %d items need %zu bytes of scratch, above its %zu", n, L.total,
+                       NOALLOC_BYTES);
+    // the counters before the keys start at zero (a call that ends normally leaves them so; this
+    // clear makes the call independent of any earlier one)
+    hipLaunchKernelGGL(k_clear_words, dim3(16), dim3(256), 0, c->stream, (uint32_t *)S, L.keys0 / 4);
+    LR_HIP(hipGetLastError());
+    return build_at(c, S, boxes, n, host_bounds, builder, node0, ref0, ref_add, nodes, aux, sorted,
+                    sorted ? sorted + n : nullptr, stats);
 }

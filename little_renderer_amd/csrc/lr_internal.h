@@ -59,6 +59,13 @@ int lr_build_run(LrhContext *c, const float *boxes, int n, const float *host_bou
                  uint32_t node0, uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux,
                  uint32_t *sorted_codes_out, uint32_t *sorted_items_out, LrhBuildStats *stats);
 
+// the build with its scratch in the library's own device array (lrh_build_noalloc): the array's
+// device address, taken once per device with that device current (from lrh_create)
+int lr_noalloc_init(int device);
+int lr_build_noalloc(LrhContext *c, const float *boxes, int n, const float *host_bounds, int builder, uint32_t node0,
+                     uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux, uint32_t *sorted,
+                     LrhBuildStats *stats);
+
 // ---- lr_trace.hip ----
 int lr_launch_trace(LrhContext *c, const uint32_t *nodes, uint32_t root, const uint32_t *aux, const float *tris,
                     const float *rays, int nrays, const uint32_t *count, int any, float *hits, int max_depth,
