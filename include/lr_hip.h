@@ -60,6 +60,16 @@ int lrh_sync(LrhContext *c);
 /* the context's stream, as a hipStream_t */
 void *lrh_stream(LrhContext *c);
 
+/* Device memory for C callers that link no GPU runtime of their own (the reference's binding:
+ * gpu_buffer / gpu_storage (engine/gpu/gpu_internal.h:77-83) and the readback of
+ * gpu_view_trace_read (engine/gpu/view_gpu.h:117-124) do for it): a buffer of `bytes` on the context's
+ * device, zeroed, ready on return; freed after the context's queued work; host <-> device
+ * copies ordered after that work and waited for. */
+int lrh_buffer_alloc(LrhContext *c, size_t bytes, void **d_out);
+int lrh_buffer_free(LrhContext *c, void *d_buf);
+int lrh_buffer_write(LrhContext *c, void *d_dst, const void *h_src, size_t bytes);
+int lrh_buffer_read(LrhContext *c, void *h_dst, const void *d_src, size_t bytes);
+
 /* triangle_bounds (engine/scene/scene.c:65-74) of n packed triangles on the device: the box
  * of q, q + u, q + v padded by 1e-3 (the boxes upload_instances makes, trace_gpu.c:1497-1502) */
 int lrh_tri_boxes(LrhContext *c, const float *d_tris, int n, float *d_boxes);

@@ -90,6 +90,10 @@ def lib():
         "lrh_destroy": (None, [P]),
         "lrh_sync": (I, [P]),
         "lrh_stream": (P, [P]),
+        "lrh_buffer_alloc": (I, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
+        "lrh_buffer_free": (I, [P, P]),
+        "lrh_buffer_write": (I, [P, P, P, ctypes.c_size_t]),
+        "lrh_buffer_read": (I, [P, P, P, ctypes.c_size_t]),
         "lrh_tri_boxes": (I, [P, P, I, P]),
         "lrh_bounds": (I, [P, P, I, P]),
         "lrh_build": (I, [P, P, I, P, I, U, U, U, P, P, P, ctypes.POINTER(BuildStats)]),

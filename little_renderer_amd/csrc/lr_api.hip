@@ -114,6 +114,56 @@ int lrh_sync(LrhContext *c)
     return 0;
 }
 
+int lrh_buffer_alloc(LrhContext *c, size_t bytes, void **d_out)
+{
+    if (!c || !d_out || bytes == 0) return lr_fail(LRH_EINVAL, "lrh_buffer_alloc: bad arguments");
+    *d_out = nullptr;
+    LR_HIP(hipSetDevice(c->device));
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return lr_fail(LRH_ENOMEM, "lrh_buffer_alloc: %zu bytes of device memory could not be had", bytes);
+    }
+    // zeroed on the context's stream, and waited for: the buffer is ready when the call returns
+    if (hipMemsetAsync(p, 0, bytes, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+        (void)hipFree(p);
+        return lr_fail(LRH_EDEVICE, "lrh_buffer_alloc: the buffer could not be cleared");
+    }
+    *d_out = p;
+    return 0;
+}
+
+int lrh_buffer_free(LrhContext *c, void *d_buf)
+{
+    if (!c) return lr_fail(LRH_EINVAL, "lrh_buffer_free: no context");
+    if (!d_buf) return 0;
+    LR_HIP(hipSetDevice(c->device));
+    LR_HIP(hipStreamSynchronize(c->stream));     // no work of the context still uses it
+    LR_HIP(hipFree(d_buf));
+    return 0;
+}
+
+int lrh_buffer_write(LrhContext *c, void *d_dst, const void *h_src, size_t bytes)
+{
+    if (!c || (bytes && (!d_dst || !h_src))) return lr_fail(LRH_EINVAL, "lrh_buffer_write: bad arguments");
+    if (!bytes) return 0;
+    LR_HIP(hipSetDevice(c->device));
+    LR_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, c->stream));
+    LR_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lrh_buffer_read(LrhContext *c, void *h_dst, const void *d_src, size_t bytes)
+{
+    if (!c || (bytes && (!d_src || !h_dst))) return lr_fail(LRH_EINVAL, "lrh_buffer_read: bad arguments");
+    if (!bytes) return 0;
+    LR_HIP(hipSetDevice(c->device));
+    // after the context's queued work (the kernels that write it), on its stream
+    LR_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+    LR_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int lrh_tri_boxes(LrhContext *c, const float *d_tris, int n, float *d_boxes)
 {
     if (!c || n < 0 || (n > 0 && (!d_tris || !d_boxes))) return lr_fail(LRH_EINVAL, "lrh_tri_boxes: bad arguments");
