@@ -175,8 +175,9 @@ int lrh_pt_scene_info(const LrhPtScene *s, int *depth, int *nshapes, size_t *byt
  * bottom, u = 0.5 looking along +z), times scale, turned by rot radians about y; rays that leave
  * the scene see it (env_eval) and next event estimation samples it by luminance x sin(theta)
  * with MIS (env_sample, env_pdf, sample_direct_env: engine/scene/env.c:54-203,
- * engine/rt/tracer.c:87-101, 415-421). rgb = NULL removes it. A scene with a lamp takes none
- * here (one light per scene). Host memory; the library keeps a copy and the sampling tables. */
+ * engine/rt/tracer.c:87-101, 415-421). rgb = NULL removes it. Beside a lamp, both are sampled at
+ * every rough vertex (a shadow ray each; the sky's numbers then from the plain stream,
+ * env_numbers, tracer.c:232-235). Host memory; the library keeps a copy and the sampling tables. */
 int lrh_pt_scene_sky(LrhPtScene *s, const float *rgb, int w, int h, float scale, float rot);
 
 /* Closest (any = 0) or first (any = 1) hits of n rays (d_rays, lr_hip.h's ray layout) in a
@@ -198,7 +199,13 @@ typedef struct {
 /* A scene of instances: the meshes' triangles in object space (mesh m: triangles
  * mesh_first[m] .. mesh_first[m + 1] - 1 of mtris, lr_hip.h's layout), one BVH per mesh built
  * on the device (LRH_LBVH or LRH_PLOC), and a top-level LBVH over the instances' world boxes
- * (inverse transforms in double on the host). No lamp: light it with lrh_pt_scene_sky. */
+ * (inverse transforms in double on the host). No lamp: light it with lrh_pt_scene_sky, and with
+ * glowing shapes: instances of the unit sphere (unstretched) or box whose material glows (not
+ * LRH_MAT_LIGHT, some emission), at most 16, sampled as lights at every rough vertex with MIS
+ * (Scene.emitters: scene_find_emitters / scene_emitter, engine/scene/scene.c:1000-1031;
+ * emit_sample / emit_pdf, engine/rt/tracer.c:120-216); their shadow rays stop 10 eps before them
+ * (RT_SHADOW_MARGIN is ten RT_EPS in both of the reference's builds, tracer.c:15-23). Any
+ * glowing surface adds its emission where a path finds it, weighted against that NEE (MIS). */
 int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const int *mesh_first, int nmeshes, const LrhMaterial *mats,
                            int nmats, const LrhInstance *inst, int ninst, float eps, int builder, LrhPtScene **out);
 /* New poses for the same instances: the top level is rebuilt on the device (per frame, as the
@@ -341,7 +348,7 @@ int lrh_engine_autotune(LrhEngine *e, const LrhJob *sample, int n, LrhTune *best
  * Scenes registered with the server stay resident; their bytes and the images of admitted jobs
  * count against a budget of device memory: the image arena is made in chunks as admitted images
  * need it, so resident scenes plus the arena never hold more than the budget (the engine's path
- * state, fixed by slots_log2 at ~260 bytes a slot, and a finished image's staging buffer come on
+ * state, fixed by slots_log2 at ~360 bytes a slot, and a finished image's staging buffer come on
  * top of it). Requests (thread-safe, from any host thread) wait
  * in order until their image fits, then join the running batch at once: their paths take free
  * slots as others end (continuous batching). A worker thread drives the engine. */

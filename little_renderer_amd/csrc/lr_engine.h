@@ -60,7 +60,14 @@ struct JobDev {
     const uint32_t *maux;
     const float4 *inst;
     int two_level;
+    // the glowing shapes sampled as lights (Scene.emitters): 5 float4 each, (centre, ball 1/0 |
+    // half axis 0, radius | half axis 1, area | half axis 2, 0 | radiance, 0); each instance's
+    // slot among them (-1: none)
+    const float4 *emit;
+    const int *inst_emit;
+    int nemit;
 };
+#define LR_MAX_EMITTERS 16
 
 // the compressed 4-wide nodes' leaf references hold a first aux entry in 24 bits (all ones is
 // kept free: a two-level walk's leaf entry then never equals its EXIT entry)
@@ -119,6 +126,12 @@ struct EngArgs {
     float4 *scon;                           // contribution.rgb, 1 if the path ends after it
     float4 *srad;                           // the path's radiance and emission state before it
     uint32_t *sq;                           // the path's q
+    // a vertex with more than one light (lamp, sky, glowing shapes) has more shadow rays: the
+    // first in the entry above, the others beside shade's input position, and sx the count of
+    // those (bits 30-31) and that position
+    float4 *xsray[2];
+    float4 *xscon[2];
+    uint32_t *sx;
     uint32_t *spill;                        // walk stacks past the LDS part, by thread
     unsigned long long spill_stride;        // threads of the launch
     int spill_per;
@@ -134,10 +147,12 @@ typedef void (*LrEngFn)(dim3 grid, hipStream_t s, const EngArgs &A);
 
 // the kernels, by work-group size (64, 128, 256: index 0..2), LDS stack (16, 24, 32) and, for the
 // walks, whether they take two-level scenes too (0: one-level scenes only; the two-level walks'
-// registers would lower the one-level walks' occupancy)
+// registers would lower the one-level walks' occupancy); shade and shadow also by whether a
+// vertex may have more than one light (1: a lamp and a sky, glowing shapes; 0: one shadow ray,
+// in fewer registers)
 struct LrEngTable {
-    LrEngFn regen[3], shade[3];
-    LrEngFn extend[3][3][2], extend_count[3][3][2], shadow[3][3][2];
+    LrEngFn regen[3], shade[3][2];
+    LrEngFn extend[3][3][2], extend_count[3][3][2], shadow[3][3][2][2];
 };
 void lr_eng_fill_b64(LrEngTable &t);
 void lr_eng_fill_b128(LrEngTable &t);
@@ -176,10 +191,22 @@ struct LrhPtScene {
     std::vector<uint32_t> mroot_h, mroot4_h;
     float *tab = nullptr;
     uint16_t *mask = nullptr;
+    // the glowing shapes (an instanced scene's), from host copies of the instances and materials
+    float4 *emit = nullptr;
+    int *inst_emit = nullptr;
+    int nemit = 0;
+    std::vector<LrhInstance> hinst;
+    std::vector<LrhMaterial> hmats;
     int nprims = 0, ntris = 0, nmats = 0, light = -1, depth = 1;
     float eps = 1e-4f, margin = 1e-3f;
     size_t bytes = 0;                       // device memory it holds
 };
+
+// a scene whose vertices may need more than one shadow ray
+static inline bool lr_scene_multi_light(const LrhPtScene *s)
+{
+    return (s->light >= 0 && s->env_w > 0) || s->nemit > 0;
+}
 
 // the engine with its image arena made at once, or chunk by chunk as images need it up to
 // image_bytes (lazy); a lazy arena's cap, the chunks' bytes, and its chunks holding no image freed

@@ -160,6 +160,8 @@ struct LrhEngine {
     hipGraphExec_t gexec = nullptr;
     bool graph_ok = false;
     bool two_level = false;     // a two-level scene has been submitted: the walks that take both
+    bool multi = false;         // a scene with more than one light has been submitted: shade and
+                                // shadow that take up to three shadow rays a vertex
     double clk_khz = 100000.0, clk_host0 = 0.0;
     unsigned long long clk_dev0 = 0;
 };
@@ -207,9 +209,10 @@ void launch_iteration(LrhEngine *e, const LrhTune &t, hipEvent_t *ev = nullptr)
     const int tl = e->two_level ? 1 : 0;
     (e->counting ? T.extend_count : T.extend)[bt][sk][tl](gt, e->stream, At);
     if (ev) (void)hipEventRecord(ev[2], e->stream);
-    T.shade[bs](gs, e->stream, As);
+    const int ml = e->multi ? 1 : 0;
+    T.shade[bs][ml](gs, e->stream, As);
     if (ev) (void)hipEventRecord(ev[3], e->stream);
-    T.shadow[bt][sk][tl](gt, e->stream, At);
+    T.shadow[bt][sk][tl][ml](gt, e->stream, At);
     if (ev) (void)hipEventRecord(ev[4], e->stream);
 }
 
@@ -427,6 +430,10 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
         e->two_level = true;
         e->graph_ok = false;
     }
+    if (lr_scene_multi_light(s) && !e->multi) {   // likewise, shade and shadow for several lights
+        e->multi = true;
+        e->graph_ok = false;
+    }
     JobDev jd;
     memset(&jd, 0, sizeof(jd));
     lr_job_scene(jd, s);
@@ -528,7 +535,8 @@ int lr_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t i
         C * 32, C * 4, C * 16,                                                       // 14..16 sray sid scon
         e->threads_cap * (size_t)(e->spill_cap > 0 ? e->spill_cap : 1) * 4,          // 17 spill
         2112 * 4, 4096 * 2, 256, 256, 256,                                           // 18..22 tab mask counts clk overflow
-        LR_ISSUE * 4, (C / 64 + 4) * 8, C * 16};                                     // 23 iq_q0, 24 ostat, 25 srad
+        LR_ISSUE * 4, (C / 64 + 4) * 8, C * 16,                                      // 23 iq_q0, 24 ostat, 25 srad
+        2 * C * 32, 2 * C * 16, C * 4};                                              // 26..28 xsray xscon sx
     constexpr int NS = sizeof(sizes) / sizeof(sizes[0]);
     size_t total = 0;
     for (size_t z : sizes) total += lr_align(z);
@@ -570,6 +578,11 @@ int lr_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t i
     }
     A.sq = (uint32_t *)p[8];
     A.srad = (float4 *)p[25];
+    for (int b = 0; b < 2; b++) {
+        A.xsray[b] = (float4 *)p[26] + 2 * b * C;
+        A.xscon[b] = (float4 *)p[27] + b * C;
+    }
+    A.sx = (uint32_t *)p[28];
     A.qray[0] = (float4 *)p[9];
     A.qray[1] = (float4 *)p[10];
     A.qid[0] = (uint32_t *)p[11];
@@ -774,7 +787,7 @@ int lrh_engine_plain(LrhEngine *e, const LrhJob *job, int64_t *id)
     const dim3 gt((unsigned)((npix + t.block_trace - 1) / t.block_trace)), gs((unsigned)((npix + t.block_shade - 1) / t.block_shade));
     const EngArgs At = args_for(e, t.stack, (size_t)gt.x * t.block_trace, 1);
     const EngArgs As = args_for(e, t.stack, 0, 1);
-    const int tl = e->two_level ? 1 : 0;
+    const int tl = e->two_level ? 1 : 0, ml = e->multi ? 1 : 0;
     for (int s = 0; s < job->spp; s++) {
         // the pass: the job's paths s * npix .. (s + 1) * npix - 1, one per pixel
         rc = issue_chunk(e, slot, npix);
@@ -782,8 +795,8 @@ int lrh_engine_plain(LrhEngine *e, const LrhJob *job, int64_t *id)
         T.regen[bs](gs, e->stream, As);
         for (int d = 0; d <= job->max_depth; d++) {
             (e->counting ? T.extend_count : T.extend)[bt][sk][tl](gt, e->stream, At);
-            T.shade[bs](gs, e->stream, As);
-            T.shadow[bt][sk][tl](gt, e->stream, At);
+            T.shade[bs][ml](gs, e->stream, As);
+            T.shadow[bt][sk][tl][ml](gt, e->stream, At);
         }
     }
     ENG_HIP(hipGetLastError());

@@ -462,8 +462,91 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_extend(E
     }
 }
 
+// ---- glowing shapes as lights (Scene.emitters: tracer.c:120-216) ----
+// 1 - cos of the half angle of the cone a ball of radius r fills from dist2 (squared) away, in
+// the form without cancellation for a small far ball; 0 from inside (ball_cone)
+__device__ __forceinline__ float ball_cone(float r, float dist2)
+{
+    const float sin2 = r * r / dist2;
+    if (!(sin2 < 1.0f)) return 0.0f;
+    return sin2 / (1.0f + sqrtf(1.0f - sin2));
+}
+
+// the density over solid angle (the pick included) NEE would have given emitter k's point at
+// distance t along unit d from p, its outward normal n there (emit_pdf)
+__device__ __forceinline__ float emit_pdf(const JobDev &J, int k, V3 p, V3 d, float t, V3 n)
+{
+    const float4 *e = J.emit + 5 * (size_t)k;
+    const float4 e0 = e[0];
+    if (e0.w != 0.0f) {
+        const V3 to = vsub(v3of(e0), p);
+        const float om = ball_cone(e[1].w, vdot(to, to));
+        return om > 0.0f ? 1.0f / (2.0f * PI_F * om * (float)J.nemit) : 0.0f;
+    }
+    const float cl = -vdot(n, d);
+    return cl > 0.0f ? t * t / (cl * e[2].w * (float)J.nemit) : 0.0f;
+}
+
+// a point of a glowing shape seen from p: direction, distance, density (the pick included) and
+// radiance; false if none (p inside a ball, a box face turned away) (emit_sample)
+__device__ __forceinline__ bool emit_sample(const JobDev &J, V3 p, float u0, float ua, float ub, V3 &wi, float &dist,
+                                            float &pdf, V3 &le)
+{
+    int k = (int)(u0 * (float)J.nemit);
+    if (k >= J.nemit) k = J.nemit - 1;
+    const float4 *e = J.emit + 5 * (size_t)k;
+    const float4 e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
+    le = v3of(e[4]);
+    const V3 c = v3of(e0);
+    if (e0.w != 0.0f) {
+        const float r = e1.w;
+        const V3 to = vsub(c, p);
+        const float d2 = vdot(to, to), om = ball_cone(r, d2);
+        if (!(om > 0.0f)) return false;
+        const float dc = sqrtf(d2), cos_t = 1.0f - ua * om, sin_t = sqrtf(rfmax(0.0f, 1.0f - cos_t * cos_t));
+        const float ph = 2.0f * PI_F * ub;
+        const V3 w = vscale(to, 1.0f / dc);
+        V3 t, b;
+        onb(w, t, b);
+        wi = vadd(vscale(w, cos_t), vadd(vscale(t, sin_t * cosf(ph)), vscale(b, sin_t * sinf(ph))));
+        const float disc = r * r - d2 * sin_t * sin_t;   // the ball's near side along wi
+        dist = dc * cos_t - sqrtf(rfmax(0.0f, disc));
+        pdf = 1.0f / (2.0f * PI_F * om * (float)J.nemit);
+        return dist > 0.0f;
+    }
+    // a box: a face (two per axis, 2 a_j by 2 a_k) by its area, then a point on it (the six
+    // faces walked in order 0+, 0-, 1+, 1-, 2+, 2-; selects, not arrays, keep it in registers)
+    const V3 a0 = v3of(e1), a1 = v3of(e2), a2 = v3of(e3);
+    const float fa0 = 4.0f * vlen(vcross(a1, a2)), fa1 = 4.0f * vlen(vcross(a2, a0)), fa2 = 4.0f * vlen(vcross(a0, a1));
+    float x = ua * e2.w;
+    int f = 0;
+    for (; f < 5; f++) {
+        const float faf = f < 2 ? fa0 : (f < 4 ? fa1 : fa2);
+        if (!(x >= faf)) break;
+        x -= faf;
+    }
+    const int i = f >> 1;
+    const float fai = i == 0 ? fa0 : (i == 1 ? fa1 : fa2);
+    const V3 ai = i == 0 ? a0 : (i == 1 ? a1 : a2), aj = i == 0 ? a1 : (i == 1 ? a2 : a0), ak = i == 0 ? a2 : (i == 1 ? a0 : a1);
+    const float sg = f & 1 ? -1.0f : 1.0f, uu = rfmin(x / fai, 1.0f) * 2.0f - 1.0f, vv = ub * 2.0f - 1.0f;
+    const V3 q = vadd(c, vadd(vscale(ai, sg), vadd(vscale(aj, uu), vscale(ak, vv))));
+    V3 n = vscale(vnorm(vcross(aj, ak)), sg);
+    if (vdot(n, ai) * sg < 0.0f) n = vneg(n);   // outward, whatever the axes' handedness
+    const V3 to = vsub(q, p);
+    const float d2 = vdot(to, to);
+    dist = sqrtf(d2);
+    wi = vscale(to, 1.0f / dist);
+    const float cl = -vdot(n, wi);
+    if (!(cl > 0.0f)) return false;
+    pdf = d2 / (cl * e2.w * (float)J.nemit);
+    return true;
+}
+
 // shade (wf_shade, trace_wave.wgsl:191-310): the bounce of trace_path_sampled (tracer.c:382-507)
-template <int BLK>
+// MULTI: the variant for scenes with more than one light (a lamp and a sky, glowing shapes): up
+// to three shadow rays a vertex. Without it, one (the lamp's, or the sky's in a scene with no
+// lamp), in fewer registers.
+template <int BLK, bool MULTI>
 __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
 {
     EngCtl *ctl = A.ctl;
@@ -477,7 +560,7 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
         bool next = false, shadow = false;
         float4 nr0, nr1, sr0, sr1, scon;
         uint32_t jslot = 0, q = 0;
-        int depth = 0;
+        int depth = 0, nsh = 0;
         uint64_t st = 0;
         float4 rad_out = make_float4(0.0f, 0.0f, 0.0f, 0.0f), thr_out;
         if (live) {
@@ -516,9 +599,10 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
                 const V3 hp = vadd(o, vscale(d, t));
                 V3 outward;
                 int mi;
+                uint32_t ii = LR_NONE;
                 if (J.two_level) {
                     // an instance's hit: the normal by the inverse transpose, its material or the triangle's
-                    const uint32_t ii = __float_as_uint(hv.z);
+                    ii = __float_as_uint(hv.z);
                     const float4 *rows = J.inst + 4 * (size_t)ii;
                     const float4 info = rows[3];
                     const uint32_t kind = __float_as_uint(info.x), imat = __float_as_uint(info.y);
@@ -561,15 +645,49 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
                             }
                         }
                     }
-                } else if (depth < J.max_depth) {
+                } else {
+                    // a glowing surface (material_glows: not a lamp, some emission; tracer.c:450-458):
+                    // found by the path; a glowing shape reached after a rough bounce was sampled by
+                    // NEE too, so weighted against that (MIS) or, without MIS, counted there already
+                    const V3 em = v3of(J.mats[3 * mi + 1]);
+                    if (em.x > 0.0f || em.y > 0.0f || em.z > 0.0f) {
+                        const int k = MULTI && J.nemit && emit != EMIT_FULL && ii != LR_NONE ? J.inst_emit[ii] : -1;
+                        const float pe = k >= 0 && front ? emit_pdf(J, k, o, vnorm(d), t * vlen(d), nrm) : 0.0f;
+                        if (!(pe > 0.0f)) rad = vadd(rad, vmul(thr, em));
+                        else if (emit == EMIT_MIS) rad = vadd(rad, vmul(thr, vscale(em, power_heuristic(bounce_pdf, pe))));
+                    }
+                }
+                if (type != MAT_LIGHT && depth < J.max_depth) {
                     const V3 v = vneg(vnorm(d));
-                    // 1. next event estimation to the lamp (sample_direct_light, tracer.c:56-81),
-                    //    its visibility left to the shadow kernel
+                    // 1. next event estimation of the lamp, the sky and the glowing shapes, in that
+                    //    order (tracer.c:467-478): each a shadow ray whose visibility the shadow
+                    //    kernel decides, adding the contributions in this order. The first ray
+                    //    waits for its place in the shadow queue; the others go beside entry i.
                     const bool rough_part = !b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f;
-                    if (rough_part && J.light < 0 && J.env_w > 0) {
-                        // towards the sky (sample_direct_env, tracer.c:87-101; env_numbers :232-235)
+                    auto add_shadow = [&](V3 wi, float tmax, V3 c) {
+                        const float4 a = make_float4(hp.x, hp.y, hp.z, J.eps), w = make_float4(wi.x, wi.y, wi.z, tmax);
+                        const float4 cc = make_float4(c.x, c.y, c.z, 0.0f);
+                        if (!MULTI || nsh == 0) {
+                            sr0 = a;
+                            sr1 = w;
+                            scon = cc;
+                        } else {
+                            float4 *xr = nsh == 1 ? A.xsray[0] : A.xsray[1];   // (selects: no indexed copy of the arguments)
+                            xr[2 * (size_t)i] = a;
+                            xr[2 * (size_t)i + 1] = w;
+                            (nsh == 1 ? A.xscon[0] : A.xscon[1])[i] = cc;
+                        }
+                        nsh++;
+                    };
+                    // the sky (sample_direct_env, tracer.c:87-101): the lamp's pair when there is no
+                    // lamp, else two plain numbers (env_numbers, :232-235)
+                    auto sky_nee = [&]() {
                         float ua, ub;
-                        s2(sm, 1 + 3 * depth, ua, ub);
+                        if (!MULTI || J.light < 0) s2(sm, 1 + 3 * depth, ua, ub);
+                        else {
+                            ua = pcg_float(st, inc);
+                            ub = pcg_float(st, inc);
+                        }
                         V3 wi;
                         float pdf;
                         const V3 le = env_sample(env, ua, ub, wi, pdf);
@@ -579,15 +697,15 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
                             if (vmax3(f) > 0.0f) {
                                 float g = fabsf(vdot(nrm, wi)) / pdf;
                                 if (J.mis) g *= power_heuristic(pdf, bpdf);
-                                const V3 c = vmul(thr, vscale(vmul(f, le), g));
-                                shadow = true;
-                                sr0 = make_float4(hp.x, hp.y, hp.z, J.eps);
-                                sr1 = make_float4(wi.x, wi.y, wi.z, 1e30f);
-                                scon = make_float4(c.x, c.y, c.z, 0.0f);
+                                add_shadow(wi, 1e30f, vmul(thr, vscale(vmul(f, le), g)));
                             }
                         }
-                    }
+                    };
+                    // (one light: the sky's block first, which the compiler allocates fewer registers for)
+                    if constexpr (!MULTI)
+                        if (rough_part && J.env_w > 0 && J.light < 0) sky_nee();
                     if (rough_part && J.light >= 0) {
+                        // the lamp (sample_direct_light, tracer.c:56-81)
                         float la, lb;
                         s2(sm, 1 + 3 * depth, la, lb);
                         const float ua = lb, ub = la;   // the pair's second number first (tracer.c:50-53)
@@ -606,14 +724,28 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
                             if (vmax3(f) > 0.0f) {
                                 float g = fabsf(cos_surf) * cos_light * l.v.w / dist2;
                                 if (J.mis) g *= power_heuristic(dist2 / (cos_light * l.v.w), bpdf);
-                                const V3 c = vmul(thr, vscale(vmul(f, le), g));
-                                shadow = true;
-                                sr0 = make_float4(hp.x, hp.y, hp.z, J.eps);
-                                sr1 = make_float4(wi.x, wi.y, wi.z, dist - J.margin);
-                                scon = make_float4(c.x, c.y, c.z, 0.0f);
+                                add_shadow(wi, dist - J.margin, vmul(thr, vscale(vmul(f, le), g)));
                             }
                         }
                     }
+                    if constexpr (MULTI)
+                        if (rough_part && J.env_w > 0) sky_nee();
+                    if (MULTI && rough_part && J.nemit > 0) {
+                        // the glowing shapes (sample_direct_emit, tracer.c:202-216): three plain numbers
+                        const float u0 = pcg_float(st, inc), ua = pcg_float(st, inc), ub = pcg_float(st, inc);
+                        V3 wi, le;
+                        float dist, pdf;
+                        if (emit_sample(J, hp, u0, ua, ub, wi, dist, pdf, le)) {
+                            float bpdf;
+                            const V3 f = bsdf_eval(b, A.tab, nrm, v, wi, bpdf);
+                            if (vmax3(f) > 0.0f) {
+                                float g = fabsf(vdot(nrm, wi)) / pdf;
+                                if (J.mis) g *= power_heuristic(pdf, bpdf);
+                                add_shadow(wi, dist - J.margin, vmul(thr, vscale(vmul(f, le), g)));
+                            }
+                        }
+                    }
+                    shadow = nsh > 0;
                     // 2. the next direction from the BSDF, Russian roulette from depth 3
                     bsdf_view(b, A.tab, vdot(v, nrm));
                     float ur = 0.0f, ul = 0.0f, da = 0.0f, db = 0.0f;
@@ -670,6 +802,7 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
             A.scon[ss] = scon;
             A.srad[ss] = rad_out;
             A.sq[ss] = q;
+            if (MULTI) A.sx[ss] = ((uint32_t)(nsh - 1) << 30) | i;
         }
         retire(A, live && !next && !shadow, jslot, q, rad_out);
     });
@@ -678,7 +811,7 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
 // shadow (wf_shadow, trace_wave.wgsl:335-355): any hit before the lamp; if none, the
 // contribution joins the path's radiance. Paths whose last vertex made the shadow ray end here.
 // TL and the register bound as for extend.
-template <int BLK, int STK, bool TL>
+template <int BLK, int STK, bool TL, bool MULTI>
 __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(EngArgs A)
 {
     __shared__ uint32_t stk[STK][BLK];
@@ -696,34 +829,80 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(E
         bool ends = false;
         uint32_t jslot = 0, q = 0;
         float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        int nx = 0;
         if (live) {
             const uint32_t id = A.sid[i];
             const uint32_t at = id & LR_SLOT_MASK;
             jslot = id >> LR_SLOT_BITS;
             const JobDev &J = A.jobs[jslot];
-            const float4 r0 = A.sray[2 * (size_t)i], r1 = A.sray[2 * (size_t)i + 1];
-            uint32_t cn, ct;
-            WalkHit h;
-            if (TL && J.two_level) h = walk4_tl<true, false, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
-            else if (!J.two_level)
-                h = walk4<true, false, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w, r1.w, S,
-                                            A.overflow, cn, ct);
-            else {
-                atomicOr(A.overflow, 2u);
-                h = WalkHit{0.0f, 0u, 0.0f, 0.0f};   // occluded: adds nothing
-            }
-            const float4 c = A.scon[i];
-            ends = c.w != 0.0f;
-            const bool lit = h.id == LR_NONE;
-            if (lit || ends) {
-                r = A.srad[i];
-                if (lit) {
-                    r.x += c.x;
-                    r.y += c.y;
-                    r.z += c.z;
+            if constexpr (!MULTI) {
+                const float4 r0 = A.sray[2 * (size_t)i], r1 = A.sray[2 * (size_t)i + 1];
+                uint32_t cn, ct;
+                WalkHit h;
+                if (TL && J.two_level) h = walk4_tl<true, false, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+                else if (!J.two_level)
+                    h = walk4<true, false, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w, r1.w, S,
+                                                A.overflow, cn, ct);
+                else {
+                    atomicOr(A.overflow, 2u);
+                    h = WalkHit{0.0f, 0u, 0.0f, 0.0f};   // occluded: adds nothing
                 }
-                if (!ends) A.rad[par][at] = r;   // the path's next state (shade wrote it without this)
-                else q = A.sq[i];
+                const float4 c = A.scon[i];
+                ends = c.w != 0.0f;
+                const bool lit = h.id == LR_NONE;
+                if (lit || ends) {
+                    r = A.srad[i];
+                    if (lit) {
+                        r.x += c.x;
+                        r.y += c.y;
+                        r.z += c.z;
+                    }
+                    if (!ends) A.rad[par][at] = r;   // the path's next state (shade wrote it without this)
+                    else q = A.sq[i];
+                }
+            } else {
+                const uint32_t x = A.sx[i];
+                nx = (int)(x >> 30);                       // the vertex's rays past the first
+                const uint32_t src = x & LR_SLOT_MASK;     // where shade left them
+                const float4 c0 = A.scon[i];
+                ends = c0.w != 0.0f;
+                uint32_t lit = 0;                          // bit j: ray j reached its light
+                for (int j = 0; j <= nx; j++) {
+                    const float4 *rr = j == 0 ? A.sray + 2 * (size_t)i : (j == 1 ? A.xsray[0] : A.xsray[1]) + 2 * (size_t)src;
+                    const float4 r0 = rr[0], r1 = rr[1];
+                    uint32_t cn, ct;
+                    WalkHit h;
+                    if (TL && J.two_level) h = walk4_tl<true, false, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+                    else if (!J.two_level)
+                        h = walk4<true, false, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w, r1.w, S,
+                                                    A.overflow, cn, ct);
+                    else {
+                        atomicOr(A.overflow, 2u);
+                        h = WalkHit{0.0f, 0u, 0.0f, 0.0f};   // occluded: adds nothing
+                    }
+                    if (h.id == LR_NONE) lit |= 1u << j;
+                }
+                if (lit || ends) {
+                    // the lights' contributions join the path's radiance in the order shade made them
+                    r = A.srad[i];
+                    for (int j = 0; j <= nx; j++)
+                        if (lit >> j & 1u) {
+                            const float4 c = j == 0 ? c0 : (j == 1 ? A.xscon[0] : A.xscon[1])[src];
+                            r.x += c.x;
+                            r.y += c.y;
+                            r.z += c.z;
+                        }
+                    if (!ends) A.rad[par][at] = r;   // the path's next state (shade wrote it without these)
+                    else q = A.sq[i];
+                }
+            }
+        }
+        // the rays past an entry's first are shadow rays too: counted a wave at a time
+        if constexpr (MULTI) {
+            if (__ballot(nx > 0)) {
+                int cnt = nx;
+                for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+                if ((threadIdx.x & 63) == 0) atomicAdd(&ctl->rays_shadow, (unsigned long long)cnt);
             }
         }
         retire(A, ends, jslot, q, r);
@@ -735,20 +914,20 @@ void L_regen(dim3 g, hipStream_t s, const EngArgs &A)
 {
     hipLaunchKernelGGL(k_regen<BLK>, g, dim3(BLK), 0, s, A);
 }
-template <int BLK>
+template <int BLK, bool MULTI>
 void L_shade(dim3 g, hipStream_t s, const EngArgs &A)
 {
-    hipLaunchKernelGGL(k_shade<BLK>, g, dim3(BLK), 0, s, A);
+    hipLaunchKernelGGL((k_shade<BLK, MULTI>), g, dim3(BLK), 0, s, A);
 }
 template <int BLK, int STK, bool COUNT, bool TL>
 void L_extend(dim3 g, hipStream_t s, const EngArgs &A)
 {
     hipLaunchKernelGGL((k_extend<BLK, STK, COUNT, TL>), g, dim3(BLK), 0, s, A);
 }
-template <int BLK, int STK, bool TL>
+template <int BLK, int STK, bool TL, bool MULTI>
 void L_shadow(dim3 g, hipStream_t s, const EngArgs &A)
 {
-    hipLaunchKernelGGL((k_shadow<BLK, STK, TL>), g, dim3(BLK), 0, s, A);
+    hipLaunchKernelGGL((k_shadow<BLK, STK, TL, MULTI>), g, dim3(BLK), 0, s, A);
 }
 
 template <int BLK, int STK, int K>
@@ -759,8 +938,10 @@ void fill_walks(LrEngTable &t)
     t.extend[b][K][1] = L_extend<BLK, STK, false, true>;
     t.extend_count[b][K][0] = L_extend<BLK, STK, true, false>;
     t.extend_count[b][K][1] = L_extend<BLK, STK, true, true>;
-    t.shadow[b][K][0] = L_shadow<BLK, STK, false>;
-    t.shadow[b][K][1] = L_shadow<BLK, STK, true>;
+    t.shadow[b][K][0][0] = L_shadow<BLK, STK, false, false>;
+    t.shadow[b][K][1][0] = L_shadow<BLK, STK, true, false>;
+    t.shadow[b][K][0][1] = L_shadow<BLK, STK, false, true>;
+    t.shadow[b][K][1][1] = L_shadow<BLK, STK, true, true>;
 }
 
 template <int BLK>
@@ -768,7 +949,8 @@ void fill_table(LrEngTable &t)
 {
     const int b = lr_blk_index(BLK);
     t.regen[b] = L_regen<BLK>;
-    t.shade[b] = L_shade<BLK>;
+    t.shade[b][0] = L_shade<BLK, false>;
+    t.shade[b][1] = L_shade<BLK, true>;
     fill_walks<BLK, 16, 0>(t);
     fill_walks<BLK, 24, 1>(t);
     fill_walks<BLK, 32, 2>(t);

@@ -235,7 +235,7 @@ static void pt_scene_release(LrhPtScene *s)
     if (!s) return;
     void *ps[] = {s->prims, s->tris, s->mats, s->boxes, s->nodes, s->aux, s->tab, s->mask, s->wnodes, s->w4nodes, s->ltris, s->env_rgb,
                   s->env_marg, s->env_cond, s->env_prob, s->mboxes, s->mwnodes, s->inst, s->ifwd, s->mnodes, s->maux,
-                  s->mroot, s->mw4nodes, s->tw4nodes, s->mroot4, s->tparent, s->tarrive, s->tcost_dev};
+                  s->mroot, s->mw4nodes, s->tw4nodes, s->mroot4, s->tparent, s->tarrive, s->tcost_dev, s->emit, s->inst_emit};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete s;
@@ -406,8 +406,6 @@ extern "C" int lrh_pt_scene_sky(LrhPtScene *s, const float *rgb, int w, int h, f
 {
     if (!s || (rgb && (w < 2 || h < 1 || w != 2 * h)) || !(scale >= 0.0f))
         return lr_fail(LRH_EINVAL, "lrh_pt_scene_sky: bad arguments (the map is w x w/2)");
-    if (rgb && s->light >= 0)
-        return lr_fail(LRH_EINVAL, "lrh_pt_scene_sky: a scene with a lamp takes no sky here (one light per scene)");
     (void)hipStreamSynchronize(s->ctx->stream);
     void *old[] = {s->env_rgb, s->env_marg, s->env_cond, s->env_prob};
     for (void *p : old)
@@ -457,6 +455,68 @@ static int pack_instances(const LrhPtScene *s, const LrhInstance *in, int n, con
         memcpy(o + 15, &root4, 4);
         memcpy(&fwd[(size_t)i * 12], x.xform, 48);
     }
+    return 0;
+}
+
+// ---- the glowing shapes (Scene.emitters): scene_find_emitters and scene_emitter
+// (engine/scene/scene.c:1000-1031) over the host copies of the instances and materials ----
+static bool material_glows(const LrhMaterial &m)   // scene.h:87-89
+{
+    return m.type != LRH_MAT_LIGHT && (m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f);
+}
+static float len3(const float *a) { return sqrtf(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+static void cross3(const float *a, const float *b, float *o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// the emitter table (5 float4 each: centre, ball | half axis 0, radius | half axis 1, area |
+// half axis 2 | radiance) and each instance's slot, made again from the current poses
+static int emitters_again(LrhPtScene *s)
+{
+    std::vector<float> rec(LR_MAX_EMITTERS * 20, 0.0f);
+    std::vector<int> slot((size_t)s->ninst, -1);
+    int n = 0;
+    for (int i = 0; i < s->ninst && n < LR_MAX_EMITTERS; i++) {
+        const LrhInstance &x = s->hinst[(size_t)i];
+        if ((x.kind != LRH_INST_SPHERE && x.kind != LRH_INST_BOX) || x.material < 0 || x.material >= s->nmats) continue;
+        const LrhMaterial &m = s->hmats[(size_t)x.material];
+        if (!material_glows(m)) continue;
+        float a[3][3];   // the transform's columns: the shape's half axes in the world
+        for (int k = 0; k < 3; k++)
+            for (int r = 0; r < 3; r++) a[k][r] = x.xform[4 * r + k];
+        const bool ball = x.kind == LRH_INST_SPHERE;
+        if (ball) {      // a stretched ball is no light
+            const float la = len3(a[0]), lb = len3(a[1]), lc = len3(a[2]);
+            if (!(la > 0.0f) || fabsf(la - lb) > 1e-4f * la || fabsf(la - lc) > 1e-4f * la) continue;
+        }
+        float *o = &rec[(size_t)n * 20];
+        o[0] = x.xform[3];
+        o[1] = x.xform[7];
+        o[2] = x.xform[11];
+        o[3] = ball ? 1.0f : 0.0f;
+        for (int k = 0; k < 3; k++)
+            for (int r = 0; r < 3; r++) o[4 + 4 * k + r] = a[k][r];
+        if (ball) {
+            const float r = len3(a[0]);
+            o[7] = r;
+            o[11] = 4.0f * PI_F * r * r;
+        } else {
+            float c12[3], c02[3], c01[3];
+            cross3(a[1], a[2], c12);
+            cross3(a[0], a[2], c02);
+            cross3(a[0], a[1], c01);
+            o[11] = 8.0f * (len3(c12) + len3(c02) + len3(c01));
+        }
+        for (int k = 0; k < 3; k++) o[16 + k] = m.emission[k];
+        slot[(size_t)i] = n++;
+    }
+    LR_HIP(hipStreamSynchronize(s->ctx->stream));
+    LR_HIP(hipMemcpy(s->emit, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
+    LR_HIP(hipMemcpy(s->inst_emit, slot.data(), slot.size() * 4, hipMemcpyHostToDevice));
+    s->nemit = n;
     return 0;
 }
 
@@ -664,6 +724,9 @@ extern "C" int lrh_pt_scene_update(LrhPtScene *s, const LrhInstance *inst, int n
     LR_HIP(hipStreamSynchronize(c->stream));
     LR_HIP(hipMemcpy(s->inst, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
     LR_HIP(hipMemcpy(s->ifwd, fwd.data(), fwd.size() * 4, hipMemcpyHostToDevice));
+    s->hinst.assign(inst, inst + ninst);
+    rc = emitters_again(s);
+    if (rc) return rc;
     return top_again(s, builder, stats);
 }
 
@@ -718,6 +781,10 @@ extern "C" int lrh_pt_scene_move(LrhPtScene *s, const int *idx, const LrhInstanc
         hipLaunchKernelGGL(k_scatter_inst, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (const int *)stage,
                            (const float4 *)(stage + bi), (const float4 *)(stage + bi + br), n, s->inst, s->ifwd);
         if (hipGetLastError() != hipSuccess) rc = lr_fail(LRH_EDEVICE, "lrh_pt_scene_move: a launch failed");
+    }
+    if (!rc) {
+        for (int k = 0; k < n; k++) s->hinst[(size_t)idx[k]] = inst[k];
+        rc = emitters_again(s);
     }
     if (!rc) rc = top_again(s, builder, stats);   // (waits for the stream)
     else (void)hipStreamSynchronize(c->stream);
@@ -813,7 +880,10 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
     s->nmats = nmats;
     s->light = -1;
     s->eps = eps;
-    s->margin = 0.0f;
+    // shadow rays to glowing shapes stop this far before them: ten times the rays' offset, as
+    // RT_SHADOW_MARGIN is RT_EPS's in both of the reference's builds (tracer.c:15-23)
+    s->margin = 10.0f * eps;
+    s->hmats.assign(mats, mats + nmats);
     size_t nnodes = 0;
     std::vector<uint32_t> roots(nmeshes);
     for (int m = 0; m < nmeshes; m++) {
@@ -845,7 +915,9 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
               dev_upload((void **)&s->mw4nodes, nullptr, (size_t)M * 64) && dev_upload((void **)&s->ltris, nullptr, (size_t)M * 48) &&
               dev_upload((void **)&s->tw4nodes, nullptr, (size_t)ninst * 64) &&
               dev_upload((void **)&s->mroot4, nullptr, (size_t)nmeshes * 4) && dev_upload((void **)&s->tparent, nullptr, tn * 4) &&
-              dev_upload((void **)&s->tarrive, nullptr, tn * 4) && dev_upload((void **)&s->tcost_dev, nullptr, 8);
+              dev_upload((void **)&s->tarrive, nullptr, tn * 4) && dev_upload((void **)&s->tcost_dev, nullptr, 8) &&
+              dev_upload((void **)&s->emit, nullptr, LR_MAX_EMITTERS * 80) &&
+              dev_upload((void **)&s->inst_emit, nullptr, (size_t)ninst * 4);
     if (!ok) {
         (void)hipGetLastError();
         pt_scene_release(s);
@@ -897,7 +969,8 @@ extern "C" int lrh_pt_scene_instanced(LrhContext *c, const float *mtris, const i
     s->mroot_h = roots;
     s->mroot4_h = roots4;
     s->bytes = (size_t)M * (48 + 32 + 4 + 64 + 48) + nnodes * (32 + 64) + hm.size() * 4 + tab.size() * 4 + mask.size() * 2 +
-               (size_t)ninst * (64 + 48 + 32 + 4 + 64) + tn * (32 + 64 + 8) + (size_t)nmeshes * 8 + 8;
+               (size_t)ninst * (64 + 48 + 32 + 4 + 64 + 4) + tn * (32 + 64 + 8) + (size_t)nmeshes * 8 + 8 +
+               LR_MAX_EMITTERS * 80;
     rc = lrh_pt_scene_update(s, inst, ninst, LRH_LBVH, nullptr);
     if (rc) {
         pt_scene_release(s);
@@ -936,6 +1009,9 @@ void lr_job_scene(JobDev &jd, const LrhPtScene *s)
     jd.mnodes4 = s->mw4nodes;
     jd.maux = s->maux;
     jd.inst = s->inst;
+    jd.emit = s->emit;
+    jd.inst_emit = s->inst_emit;
+    jd.nemit = s->nemit;
 }
 
 // a batch of rays through a resident scene's own walk (the engine's closest-hit walk: walk4, or

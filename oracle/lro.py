@@ -85,6 +85,7 @@ def lib():
             "lro_env_sample": (None, [P, F, F, P, P, P]),
             "lro_env_pdf": (F, [P, P]),
             "lro_scene_trace": (None, [P, P, I, I, P]),
+            "lro_find_emitters": (I, [P, P, P, I, P, I, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -307,7 +308,8 @@ class LroScene(ctypes.Structure):
                 ("margin", ctypes.c_float), ("mask", ctypes.c_void_p), ("tab", ctypes.c_void_p), ("env", ctypes.c_void_p),
                 ("mtris", ctypes.c_void_p), ("mesh_first", ctypes.c_void_p), ("nmeshes", ctypes.c_int),
                 ("inst", ctypes.c_void_p), ("ninst", ctypes.c_int), ("mesh_box", ctypes.c_void_p),
-                ("tnodes", ctypes.c_void_p), ("taux", ctypes.c_void_p)]
+                ("tnodes", ctypes.c_void_p), ("taux", ctypes.c_void_p), ("emit", ctypes.c_void_p), ("nemit", ctypes.c_int),
+                ("inst_emit", ctypes.c_void_p)]
 
 
 class Env:
@@ -457,13 +459,32 @@ def scene_struct(scene, tab=None, mask=None):
     if scene.get("tree", len(tris) > 4096) and len(tris) > 1:
         # the oracle's own LBVH over the triangles (pinned against brute force), walked instead of it
         tn, ta = tri_tree(tris)
-    keep = [prims, tris, mats, tab, mask, env, mt, mf, inst, mbox, tn, ta]
+    em, slot, nem = None, None, 0
+    if inst is not None and scene.get("fwd") is not None and scene.get("emitters", True):
+        # the glowing shapes (lro_find_emitters) from the instances' object -> world rows
+        em, slot, nem = find_emitters(scene["fwd"], inst["kind"], inst["material"], mats)
+    keep = [prims, tris, mats, tab, mask, env, mt, mf, inst, mbox, tn, ta, em, slot]
     sc = LroScene(ptr(prims), prims.shape[0], ptr(tris), tris.shape[0], ctypes.cast(mats, ctypes.c_void_p), len(mats),
                   scene.get("light", -1), scene["eps"], scene.get("margin", 1e-3), ptr(mask), ptr(tab),
                   ctypes.addressof(env.s) if env is not None else None, ptr(mt), ptr(mf),
                   0 if mf is None else len(mf) - 1, ptr(inst), 0 if inst is None else len(inst), ptr(mbox),
-                  ptr(tn), ptr(ta))
+                  ptr(tn), ptr(ta), ptr(em), nem, ptr(slot))
     return sc, keep
+
+
+EMITTER_FLOATS = 18     # LroEmitter: ball (int), c[3], a[9], r, area, le[3]
+
+
+def find_emitters(fwd, kind, material, mats):
+    """lro_find_emitters: (records (16, 18) float32 as LroEmitter, each instance's slot, count)"""
+    f = np.ascontiguousarray(fwd, np.float32).reshape(-1, 12)
+    k = np.ascontiguousarray(kind, np.int32)
+    m = np.ascontiguousarray(material, np.int32)
+    out = np.zeros((16, EMITTER_FLOATS), np.float32)
+    slot = np.zeros(max(1, len(k)), np.int32)
+    n = lib().lro_find_emitters(ptr(f), ptr(k), ptr(m), len(k), ctypes.cast(mats, ctypes.c_void_p), len(mats), ptr(out),
+                                ptr(slot))
+    return out, slot, n
 
 
 _TREES = {}

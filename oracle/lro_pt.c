@@ -1087,9 +1087,147 @@ static V sample_direct_env(const LroScene *s, const Hit *h, Bsdf *b, V v, int mi
     return vscale(vmul(f, le), g);
 }
 
+/* ---- glowing shapes as lights (Scene.emitters: tracer.c:120-229, scene.c:1000-1031) ----
+ * A glowing instance (a ball or a box) is sampled at rough vertices as the lamp is: one picked
+ * uniformly, then a direction in the cone a ball fills (its density over solid angle), or a
+ * point on a box's faces by area; a path that finds one after a rough bounce is weighted
+ * against that (emit_pdf). The numbers are the plain stream's, three per vertex. */
+static int glows(const LroMat *m)                                 /* material_glows, scene.h:87-89 */
+{
+    return m->type != LRO_MAT_LIGHT && (m->emission[0] > 0 || m->emission[1] > 0 || m->emission[2] > 0);
+}
+
+static V axis_col(const float *fwd, int k) { return v3(fwd[k], fwd[4 + k], fwd[8 + k]); }
+
+int lro_find_emitters(const float *fwd, const int *kind, const int *material, int ninst, const LroMat *mats,
+                      int nmats, LroEmitter *out, int *slot)
+{
+    int n = 0;
+    for (int i = 0; i < ninst; i++) slot[i] = -1;
+    for (int i = 0; i < ninst && n < LRO_MAX_EMITTERS; i++) {
+        const float *x = fwd + 12 * (size_t)i;
+        if ((kind[i] != LRO_INST_SPHERE && kind[i] != LRO_INST_BOX) || material[i] < 0 || material[i] >= nmats) continue;
+        if (!glows(&mats[material[i]])) continue;
+        V a[3] = {axis_col(x, 0), axis_col(x, 1), axis_col(x, 2)};
+        const int ball = kind[i] == LRO_INST_SPHERE;
+        if (ball) {   /* a stretched ball is no light */
+            real la = vlen(a[0]), lb = vlen(a[1]), lc = vlen(a[2]);
+            if (!(la > 0) || fabsf(la - lb) > 1e-4f * la || fabsf(la - lc) > 1e-4f * la) continue;
+        }
+        LroEmitter *e = &out[n];
+        memset(e, 0, sizeof(*e));
+        e->ball = ball;
+        e->c[0] = x[3];
+        e->c[1] = x[7];
+        e->c[2] = x[11];
+        for (int k = 0; k < 3; k++) {
+            e->a[3 * k] = (float)a[k].x;
+            e->a[3 * k + 1] = (float)a[k].y;
+            e->a[3 * k + 2] = (float)a[k].z;
+        }
+        for (int k = 0; k < 3; k++) e->le[k] = mats[material[i]].emission[k];
+        if (ball) {
+            real r = vlen(a[0]);
+            e->r = (float)r;
+            e->area = (float)(4 * (real)PI_ * r * r);
+        } else
+            e->area = (float)(8 * (vlen(vcross(a[1], a[2])) + vlen(vcross(a[0], a[2])) + vlen(vcross(a[0], a[1]))));
+        slot[i] = n++;
+    }
+    return n;
+}
+
+/* 1 - cos of the half angle of the cone a ball of radius r fills from dist2 (squared) away, in
+ * the form without cancellation for a small far ball (ball_cone, tracer.c:140-146); 0 inside */
+static real ball_cone(real r, real dist2)
+{
+    real sin2 = r * r / dist2;
+    if (!(sin2 < 1)) return 0;
+    return sin2 / (1 + sqrtf(1 - sin2));
+}
+
+static V emitter_axis(const LroEmitter *e, int k) { return vload(e->a + 3 * k); }
+
+/* emit_pdf (tracer.c:148-158): the density over solid angle, the pick included, that NEE would
+ * have given the point of emitter k at distance t along unit d from p, its outward normal n */
+static real emit_pdf(const LroScene *s, int k, V p, V d, real t, V n)
+{
+    const LroEmitter *e = &s->emit[k];
+    if (e->ball) {
+        V to = vsub(vload(e->c), p);
+        real om = ball_cone(e->r, vdot(to, to));
+        return om > 0 ? 1 / (2 * (real)PI_ * om * s->nemit) : 0;
+    }
+    real cl = -vdot(n, d);
+    return cl > 0 ? t * t / (cl * e->area * s->nemit) : 0;
+}
+
+/* emit_sample (tracer.c:160-200): a point of a glowing shape seen from p */
+static int emit_sample(const LroScene *s, V p, real u0, real ua, real ub, V *wi, real *dist, real *pdf, V *le)
+{
+    int k = (int)(u0 * s->nemit);
+    if (k >= s->nemit) k = s->nemit - 1;
+    const LroEmitter *e = &s->emit[k];
+    *le = vload(e->le);
+    if (e->ball) {
+        V to = vsub(vload(e->c), p);
+        real d2 = vdot(to, to), om = ball_cone(e->r, d2);
+        if (!(om > 0)) return 0;
+        real dc = sqrtf(d2), cos_t = 1 - ua * om, sin_t = sqrtf(rfmax(0, 1 - cos_t * cos_t));
+        real ph = 2 * (real)PI_ * ub;
+        V w = vscale(to, 1 / dc), t, b;
+        onb(w, &t, &b);
+        *wi = vadd(vscale(w, cos_t), vadd(vscale(t, sin_t * cosf(ph)), vscale(b, sin_t * sinf(ph))));
+        real disc = e->r * e->r - d2 * sin_t * sin_t;   /* the ball's near side along wi */
+        *dist = dc * cos_t - sqrtf(rfmax(0, disc));
+        *pdf = 1 / (2 * (real)PI_ * om * s->nemit);
+        return *dist > 0;
+    }
+    /* a box: one of its six faces (two per axis, 2 a_j by 2 a_k) by area, then a point on it */
+    V a[3] = {emitter_axis(e, 0), emitter_axis(e, 1), emitter_axis(e, 2)};
+    real fa[3];
+    for (int i = 0; i < 3; i++) fa[i] = 4 * vlen(vcross(a[(i + 1) % 3], a[(i + 2) % 3]));
+    real x = ua * e->area;
+    int f = 0;
+    while (f < 5 && x >= fa[f >> 1]) {
+        x -= fa[f >> 1];
+        f++;
+    }
+    int i = f >> 1;
+    real sg = f & 1 ? -1 : 1, uu = rfmin(x / fa[i], 1) * 2 - 1, vv = ub * 2 - 1;
+    V aj = a[(i + 1) % 3], ak = a[(i + 2) % 3];
+    V q = vadd(vload(e->c), vadd(vscale(a[i], sg), vadd(vscale(aj, uu), vscale(ak, vv))));
+    V n = vscale(vnorm(vcross(aj, ak)), sg);
+    if (vdot(n, a[i]) * sg < 0) n = vneg(n);            /* outward, whatever the axes' handedness */
+    V to = vsub(q, p);
+    real d2 = vdot(to, to);
+    *dist = sqrtf(d2);
+    *wi = vscale(to, 1 / *dist);
+    real cl = -vdot(n, *wi);
+    if (!(cl > 0)) return 0;
+    *pdf = d2 / (cl * e->area * s->nemit);
+    return 1;
+}
+
+/* sample_direct_emit (tracer.c:202-216): NEE of the glowing shapes at a surface */
+static V sample_direct_emit(const LroScene *s, const Hit *h, Bsdf *b, V v, int mis, LroRng *rng)
+{
+    real u0 = lro_rng_float(rng), ua = lro_rng_float(rng), ub = lro_rng_float(rng);
+    V wi, le;
+    real dist, pdf;
+    if (!emit_sample(s, h->p, u0, ua, ub, &wi, &dist, &pdf, &le)) return v3(0, 0, 0);
+    real bpdf;
+    V f = bsdf_eval(b, h->n, v, wi, &bpdf);
+    if (!(vmax(f) > 0)) return v3(0, 0, 0);
+    if (scene_occluded(s, h->p, wi, s->eps, dist - s->margin)) return v3(0, 0, 0);
+    real g = fabsf(vdot(h->n, wi)) / pdf;
+    if (mis) g *= power_heuristic(pdf, bpdf);
+    return vscale(vmul(f, le), g);
+}
+
 enum { EMIT_FULL, EMIT_NONE, EMIT_MIS };
 
-/* trace_path_sampled (tracer.c:364-509) without media, glowing shapes or textures */
+/* trace_path_sampled (tracer.c:364-509) without media or textures */
 static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis)
 {
     V radiance = v3(0, 0, 0), throughput = v3(1, 1, 1);
@@ -1121,6 +1259,15 @@ static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis
             }
             break;
         }
+        /* a glowing surface (tracer.c:450-458): found by the path; a glowing shape after a rough
+           bounce was sampled by NEE: weighted against it (MIS) or counted there already */
+        if (glows(m)) {
+            int k = s->nemit && emit != EMIT_FULL && h.inst >= 0 ? s->inst_emit[h.inst] : -1;
+            real pe = k >= 0 && h.front ? emit_pdf(s, k, o, vnorm(d), h.t * vlen(d), h.ng) : 0;
+            if (!(pe > 0)) radiance = vadd(radiance, vmul(throughput, vload(m->emission)));
+            else if (emit == EMIT_MIS)
+                radiance = vadd(radiance, vmul(throughput, vscale(vload(m->emission), power_heuristic(bounce_pdf, pe))));
+        }
         if (depth >= max_depth) break;
         V v = vneg(vnorm(d));
         Bsdf b = bsdf_make(m, h.front, s->tab);
@@ -1138,6 +1285,7 @@ static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis
                 }
                 radiance = vadd(radiance, vmul(throughput, sample_direct_env(s, &h, &b, v, mis, la, lb)));
             }
+            if (s->nemit) radiance = vadd(radiance, vmul(throughput, sample_direct_emit(s, &h, &b, v, mis, sm->rng)));
         }
         bsdf_view(&b, vdot(v, h.n));
         real ur = 0, ul = 0, da = 0, db = 0;

@@ -58,6 +58,15 @@ typedef struct {
     int kind, material;
 } LroInst;
 
+/* a glowing shape sampled as a light (Emitter, engine/scene/scene.h:369; scene_emitter,
+ * scene.c:1015-1031): a ball (centre c, radius r) or a box (centre c, half axes a[0..2] as the
+ * columns of its transform), its area and radiance */
+#define LRO_MAX_EMITTERS 16
+typedef struct {
+    int ball;
+    float c[3], a[9], r, area, le[3];
+} LroEmitter;
+
 typedef struct {
     const LroPrim *prims; int nprims;
     const float *tris; int ntris;        /* 12 floats each, material in the top 6 bits of word 11 */
@@ -77,7 +86,17 @@ typedef struct {
     /* optional: a binary tree over tris (lro.h's node layout, aux = triangle index), walked for
        the triangles instead of the brute force (nodes NULL: brute force) */
     const uint32_t *tnodes, *taux;
+    /* the glowing shapes (Scene.emitters) and each instance's slot among them (-1: none) */
+    const LroEmitter *emit; int nemit;
+    const int *inst_emit;
 } LroScene;
+
+/* scene_find_emitters + scene_emitter (engine/scene/scene.c:1000-1031): the instances that are
+ * glowing shapes (a unit sphere whose three axes have one length, or a unit box, whose material
+ * glows: not a lamp, some emission), at most LRO_MAX_EMITTERS, from each instance's object ->
+ * world rows (fwd, 12 floats), kind and material; slot[i] = its emitter or -1. Returns how many. */
+int lro_find_emitters(const float *fwd, const int *kind, const int *material, int ninst, const LroMat *mats,
+                      int nmats, LroEmitter *out, int *slot);
 
 /* The closest hit of n rays (8 floats each) in a scene, by brute force over every shape and
  * every instance's shapes: hit = t, shape id (a world shape, or the instance's local triangle,

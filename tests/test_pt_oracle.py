@@ -269,3 +269,49 @@ def test_tree_walked_oracle_equals_brute_force():
     b = lro.pt_image(dict(sc, tree=False), 24, 24, 0, 4, seed=3, tab=tab, mask=mask)
     assert a.sum() > 0
     assert np.mean(a.view(np.uint32) == b.view(np.uint32)) > 0.999
+
+
+def _glow_samples(box, mis, emitters=True, n=200, seed=0):
+    """the oracle's per-sample values at the floor point below a glowing shape: one sample per
+    pixel of an n x n image whose view spans a few hundredths of a unit around it"""
+    from glow_scenes import glow_scene
+    sc, _ = glow_scene(box=box, W=n, H=n)
+    sc["emitters"] = emitters
+    img = lro.pt_image(sc, n, n, 0, 1, seed=seed, max_depth=8, mis=mis, tab=lr.brdf_tables(), mask=lro.blue_noise())
+    return img[..., 0].astype(np.float64).ravel()
+
+
+def test_glowing_ball_closed_form():
+    # test_scene.c:118-153: 0.5 * 10 * (10 / 50)^2 = 0.2 with NEE and MIS, with NEE alone, and
+    # with the paths alone (noisier); NEE's noise a fraction of the paths' alone
+    nee_mis, nee, alone = (_glow_samples(False, m, e) for m, e in ((True, True), (False, True), (True, False)))
+    for name, v, tol in (("NEE + MIS", nee_mis, 0.004), ("NEE alone", nee, 0.004), ("paths alone", alone, 0.012)):
+        assert abs(v.mean() - 0.2) < tol, (name, v.mean())
+    assert nee_mis.std() < 0.2 * alone.std() and nee.std() < 0.2 * alone.std(), (nee_mis.std(), nee.std(), alone.std())
+
+
+def test_glowing_box_nee_agrees_with_the_paths_alone():
+    # test_scene.c:155-170: a turned box; NEE's mean is the paths' alone, and its noise under half
+    a, b = _glow_samples(True, True, True), _glow_samples(True, True, False)
+    se = np.sqrt(a.var() / a.size) + np.sqrt(b.var() / b.size)
+    assert abs(a.mean() - b.mean()) < 3 * se, (a.mean(), b.mean(), se)
+    assert a.std() < 0.5 * b.std()
+
+
+def test_emitter_table_rules():
+    # scene_find_emitters (scene.c:1000-1013): glowing unit spheres whose axes have one length and
+    # glowing boxes; a stretched ball, a lamp material, a mesh instance and a dark material are not
+    from glow_scenes import _mats, axes_xform
+    mats = _mats()
+    fwd = np.stack([axes_xform(np.eye(3), (10, 10, 10), (0, 50, 0)), axes_xform(np.eye(3), (10, 20, 10), (0, 0, 0)),
+                    axes_xform(np.eye(3), (3, 4, 5), (1, 2, 3)), axes_xform(np.eye(3), (1, 1, 1), (0, 0, 0)),
+                    axes_xform(np.eye(3), (1, 1, 1), (0, 0, 0))])
+    kind = np.array([lro.INST_SPHERE, lro.INST_SPHERE, lro.INST_BOX, 0, lro.INST_SPHERE])
+    mat = np.array([1, 1, 1, 1, 0])
+    em, slot, n = lro.find_emitters(fwd, kind, mat, mats)
+    assert n == 2 and list(slot[:5]) == [0, -1, 1, -1, -1]
+    e = em[:n]
+    assert e[0, 0].view(np.int32) == 1 and np.allclose(e[0, 1:4], [0, 50, 0]) and abs(e[0, 13] - 10) < 1e-6
+    assert abs(e[0, 14] - 4 * np.pi * 100) < 1e-3                   # 4 pi r^2
+    assert abs(e[1, 14] - 8 * (4 * 5 + 3 * 5 + 3 * 4)) < 1e-3       # 8 (|a1 x a2| + |a0 x a2| + |a0 x a1|)
+    assert np.allclose(e[1, 15:18], 10)
