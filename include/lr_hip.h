@@ -243,6 +243,17 @@ int lrh_pt_scene_top(LrhPtScene *s, uint32_t *h_nodes, uint32_t *h_aux, double *
 int lrh_pt_render(LrhContext *c, const LrhPtScene *s, const float *cam, int W, int H, int spp, uint32_t s0, uint64_t seed,
                   int max_depth, int mis, float clamp, float *d_accum);
 
+/* The denoiser (gpu_denoise_encode, engine/gpu/denoise_gpu.c:73-99, denoise.wgsl): an
+ * edge-avoiding a-trous wavelet filter steered by each pixel's variance (SVGF without its
+ * temporal part) over a w x h image: d_accum float4 per pixel (the sum of its samples' rgb,
+ * their count), d_moments float2 (the sums of its samples' irradiance luminance and of its
+ * square), d_guide 8 floats (lrh_engine_guide's layout). passes 1..5 a-trous passes (holes
+ * 1, 2, 4, 8, 16 apart; 5 is the reference's). d_out: float4 per pixel (the denoised mean, 1);
+ * lamps and the sky as they are, pixels with no sample 0. Asynchronous on the context's
+ * stream; device memory. */
+int lrh_denoise(LrhContext *c, const float *d_accum, const float *d_moments, const float *d_guide, int w, int h,
+                int passes, float *d_out);
+
 /* The engine's BSDF on n cases (device memory): in 17 floats each (base rgb, metallic, roughness,
  * ior, transmission, outside (1/0), v xyz, l xyz, ul, u1, u2; a MAT_PBR material, the normal +z),
  * out 16 floats each: bsdf_eval's f rgb and pdf, then bsdf_pick(ul) + bsdf_sample_lobe(u1, u2):
@@ -271,6 +282,8 @@ typedef struct {
     uint64_t seed;          /* the image's scramble */
     int max_depth, mis;
     float clamp;            /* 0: none */
+    int denoise;            /* 1: also the denoiser's inputs (the guide before the first sample,
+                               the samples' moments), 48 more bytes a pixel; 0: none */
 } LrhJob;
 
 /* The engine's launch shape (what lrh_engine_autotune picks): work-group sizes of the walk
@@ -336,6 +349,16 @@ int lrh_engine_job(LrhEngine *e, int64_t id, LrhJobInfo *info);
 /* the job's image into d_out: float4 per pixel (the sum of its samples' rgb, their count),
  * stored (add = 0) or added (add = 1); returns when d_out holds it */
 int lrh_engine_image(LrhEngine *e, int64_t id, float *d_out, int add);
+/* A finished job submitted with denoise = 1: its guide into d_guide (8 floats a pixel: the
+ * normal facing the camera of the first diffuse surface seen through the pixel's centre, through
+ * mirrors and glass, and its distance along the way | the albedo of the way to it, the kind: 0
+ * a surface, 1 a lamp, a bright glow or the sky; trace.wgsl:1034-1107) and its moments into
+ * d_moments (2 floats a pixel: the sums of its samples' irradiance luminance, the sample over
+ * the guide's albedo on a surface, and of its square; trace.wgsl:1301-1308). Waits. */
+int lrh_engine_guide(LrhEngine *e, int64_t id, float *d_guide, float *d_moments);
+/* A finished job submitted with denoise = 1, denoised (lrh_denoise on its image, moments and
+ * guide) into d_out: float4 per pixel (the denoised mean rgb, 1). Waits. */
+int lrh_engine_denoise(LrhEngine *e, int64_t id, int passes, float *d_out);
 /* forgets a finished job: its image memory and slot go back to the engine */
 int lrh_engine_release(LrhEngine *e, int64_t id);
 int lrh_engine_stats(LrhEngine *e, LrhEngineStats *st);

@@ -50,7 +50,7 @@ class Job(ctypes.Structure):
     """LrhJob: one render job (gpu_frame_trace's arguments)"""
     _fields_ = [("scene", ctypes.c_void_p), ("cam", ctypes.c_float * 12), ("w", ctypes.c_int), ("h", ctypes.c_int),
                 ("spp", ctypes.c_int), ("s0", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("max_depth", ctypes.c_int),
-                ("mis", ctypes.c_int), ("clamp", ctypes.c_float)]
+                ("mis", ctypes.c_int), ("clamp", ctypes.c_float), ("denoise", ctypes.c_int)]
 
 
 class JobInfo(ctypes.Structure):
@@ -122,6 +122,9 @@ def lib():
                                   ctypes.POINTER(ctypes.c_size_t)]),
         "lrh_pt_render": (I, [P, P, P, I, I, I, U, ctypes.c_uint64, I, I, F, P]),
         "lrh_bsdf_cases": (I, [P, P, I, P]),
+        "lrh_denoise": (I, [P, P, P, P, I, I, I, P]),
+        "lrh_engine_guide": (I, [P, ctypes.c_int64, P, P]),
+        "lrh_engine_denoise": (I, [P, ctypes.c_int64, I, P]),
         "lrs_batch_scene": (None, [I, ctypes.c_uint64, P, P, P]),
         "lrs_batch_camera": (None, [ctypes.c_uint64, I, I, P]),
         "lrs_batch_plan": (None, [I, ctypes.c_uint64, I, P, P, P, P]),
@@ -259,6 +262,7 @@ class PtScene:
         _check(lib().lrh_pt_scene_instanced(ctx._c, _ptr(mt), _ptr(mf), len(mf) - 1, ctypes.cast(mats, ctypes.c_void_p),
                                             len(mats), _ptr(inst), len(inst), eps, builder, ctypes.byref(self._h)))
         self.ninst = len(inst)
+        ctx._adopt(self)
         return self
 
     def __init__(self, ctx, prims, tris, mats, light, eps, margin, builder=PLOC):
@@ -269,6 +273,7 @@ class PtScene:
         self._h = ctypes.c_void_p()
         _check(lib().lrh_pt_scene(ctx._c, _ptr(pr), pr.shape[0], _ptr(tr), tr.shape[0], ctypes.cast(mats, ctypes.c_void_p),
                                   len(mats), light, eps, margin, builder, ctypes.byref(self._h)))
+        ctx._adopt(self)
 
     def set_sky(self, rgb, scale=1.0, rot=0.0):
         """lrh_pt_scene_sky: rgb (h, 2h, 3) float32, or None to remove it"""
@@ -356,12 +361,22 @@ class Context:
     """An lrh context on one device (its own stream unless one is given)."""
 
     def __init__(self, device=0, stream=None):
+        import weakref
         L = lib()
         self._c = ctypes.c_void_p()
+        # the scenes, engines and servers made on this context: closed before it (the library's
+        # objects use their context's stream and memory until they are freed)
+        self._children = weakref.WeakSet()
         _check(L.lrh_create(ctypes.byref(self._c), device, stream))
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def close(self):
         if self._c:
+            # servers and engines first, then the scenes their jobs read
+            for child in sorted(self._children, key=lambda c: isinstance(c, PtScene)):
+                child.close()
             lib().lrh_destroy(self._c)
             self._c = ctypes.c_void_p()
 
@@ -380,6 +395,10 @@ class Context:
 
     def tri_boxes(self, tris, boxes):
         _check(lib().lrh_tri_boxes(self._c, _ptr(tris), tris.shape[0], _ptr(boxes)))
+
+    def denoise(self, accum, moments, guide, w, h, out, passes=5):
+        """lrh_denoise on device tensors: accum (h, w, 4), moments (h, w, 2), guide (h, w, 8) -> out (h, w, 4)"""
+        _check(lib().lrh_denoise(self._c, _ptr(accum), _ptr(moments), _ptr(guide), w, h, passes, _ptr(out)))
 
     def bounds(self, boxes, out):
         _check(lib().lrh_bounds(self._c, _ptr(boxes), boxes.shape[0], _ptr(out)))
@@ -451,14 +470,14 @@ def batch_plan(njobs, seed, spp_lo=1):
     return [tuple(int(x[i]) for x in a) for i in range(njobs)]
 
 
-def make_job(scene, cam, w, h, spp, s0=0, seed=0, max_depth=8, mis=True, clamp=0.0):
-    """an LrhJob for a PtScene"""
+def make_job(scene, cam, w, h, spp, s0=0, seed=0, max_depth=8, mis=True, clamp=0.0, denoise=False):
+    """an LrhJob for a PtScene (denoise: also the denoiser's guide and moments)"""
     j = Job()
     j.scene = scene._h.value if isinstance(scene, PtScene) else scene
     for k in range(12):
         j.cam[k] = float(cam[k])
     j.w, j.h, j.spp, j.s0, j.seed = w, h, spp, s0, seed
-    j.max_depth, j.mis, j.clamp = max_depth, 1 if mis else 0, clamp
+    j.max_depth, j.mis, j.clamp, j.denoise = max_depth, 1 if mis else 0, clamp, 1 if denoise else 0
     return j
 
 
@@ -469,6 +488,7 @@ class Engine:
         self.ctx = ctx
         self._e = ctypes.c_void_p()
         _check(lib().lrh_engine_create(ctx._c, slots_log2, max_tree_depth, image_bytes, ctypes.byref(self._e)))
+        ctx._adopt(self)
 
     def close(self):
         if self._e:
@@ -534,6 +554,14 @@ class Engine:
         """the job's image into a device tensor (h, w, 4): sum of the samples' rgb, count"""
         _check(lib().lrh_engine_image(self._e, i, _ptr(out), 1 if add else 0))
 
+    def guide(self, i, guide, moments):
+        """a denoised job's guide (h, w, 8) and moments (h, w, 2) into device tensors"""
+        _check(lib().lrh_engine_guide(self._e, i, _ptr(guide), _ptr(moments)))
+
+    def denoise(self, i, out, passes=5):
+        """a denoised job's filtered image into a device tensor (h, w, 4): rgb, 1"""
+        _check(lib().lrh_engine_denoise(self._e, i, passes, _ptr(out)))
+
     def release(self, i):
         _check(lib().lrh_engine_release(self._e, i))
 
@@ -563,6 +591,7 @@ class Server:
                 setattr(t, k, v)
         _check(lib().lrh_server_create(ctx._c, slots_log2, max_tree_depth, budget,
                                        ctypes.byref(t) if t is not None else None, ctypes.byref(self._s)))
+        ctx._adopt(self)
 
     def close(self):
         if self._s:

@@ -66,7 +66,15 @@ struct JobDev {
     const float4 *emit;
     const int *inst_emit;
     int nemit;
+    // a job with the denoiser's inputs (else both null): per pixel its guide, 2 float4 (the
+    // first diffuse surface's normal, distance | the albedo on the way to it, the kind: 0 a
+    // surface, 1 the sky or a lamp), and the moments of its samples' irradiance luminance (two
+    // int64: the sum in 2^-28 units, the sum of squares in 2^-20 units)
+    const float4 *guide;
+    long long *moments;
 };
+#define LR_MOM2_SHIFT 20                  // the squares' fixed-point units: 2^-20
+#define LR_MOM2_CAP 1099511627776.0f      // a square saturated at 2^40
 #define LR_MAX_EMITTERS 16
 
 // the compressed 4-wide nodes' leaf references hold a first aux entry in 24 bits (all ones is
@@ -144,6 +152,9 @@ struct EngArgs {
 };
 
 typedef void (*LrEngFn)(dim3 grid, hipStream_t s, const EngArgs &A);
+// the denoiser's guide of one job (k_guide, 256 threads, a 16-entry LDS stack)
+typedef void (*LrGuideFn)(dim3 grid, hipStream_t s, const JobDev &J, float4 *guide, uint32_t *spill, size_t stride,
+                          int spill_per, const float *tab, uint32_t *overflow);
 
 // the kernels, by work-group size (64, 128, 256: index 0..2), LDS stack (16, 24, 32) and, for the
 // walks, whether they take two-level scenes too (0: one-level scenes only; the two-level walks'
@@ -153,6 +164,7 @@ typedef void (*LrEngFn)(dim3 grid, hipStream_t s, const EngArgs &A);
 struct LrEngTable {
     LrEngFn regen[3], shade[3][2];
     LrEngFn extend[3][3][2], extend_count[3][3][2], shadow[3][3][2][2];
+    LrGuideFn guide[2];   // by whether it takes two-level scenes
 };
 void lr_eng_fill_b64(LrEngTable &t);
 void lr_eng_fill_b128(LrEngTable &t);
@@ -206,6 +218,14 @@ struct LrhPtScene {
 static inline bool lr_scene_multi_light(const LrhPtScene *s)
 {
     return (s->light >= 0 && s->env_w > 0) || s->nemit > 0;
+}
+
+// a job's block in the engine's image arena: its image (24 bytes a pixel) and, when it is to be
+// denoised, its guide (32) and moments (16)
+static inline size_t lr_engine_job_bytes(const LrhJob &j)
+{
+    const size_t npix = (size_t)j.w * (size_t)j.h;
+    return lr_align(npix * 24) + (j.denoise ? lr_align(npix * 32) + lr_align(npix * 16) : 0);
 }
 
 // the engine with its image arena made at once, or chunk by chunk as images need it up to

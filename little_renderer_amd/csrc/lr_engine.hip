@@ -7,6 +7,7 @@
 // job is done when all its paths have retired, at the device clock its last retire recorded.
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <deque>
 #include <map>
@@ -55,6 +56,15 @@ __global__ void k_issue(EngCtl *ctl, unsigned long long *iq_start, uint32_t *iq_
 }
 
 // wf_resolve's output (trace_wave.wgsl:357-378): float4 (sum, count) per pixel
+// a denoised job's moments to floats: the sum of its samples' irradiance luminance and of its square
+__global__ __launch_bounds__(SBLOCK) void k_moments(const long long *mo, int npix, float2 *out)
+{
+    const int p = blockIdx.x * SBLOCK + threadIdx.x;
+    if (p >= npix) return;
+    out[p] = make_float2((float)((double)mo[2 * (size_t)p] / (double)(1u << LR_FIX_SHIFT)),
+                         (float)((double)mo[2 * (size_t)p + 1] / (double)(1u << LR_MOM2_SHIFT)));
+}
+
 __global__ __launch_bounds__(SBLOCK) void k_resolve(const long long *acc, int npix, float count, float4 *out, int add)
 {
     const int p = blockIdx.x * SBLOCK + threadIdx.x;
@@ -96,7 +106,8 @@ struct HostJob {
     int state = 0;              // 0 free, LRH_JOB_RUNNING, LRH_JOB_DONE
     int64_t id = -1;
     LrhJob desc{};
-    size_t off = 0, bytes = 0;  // its image in the arena
+    size_t off = 0, bytes = 0;  // its block in the arena: a denoised job's guide and moments, then its image
+    size_t acc_off = 0, mom_off = 0, guide_off = 0;
     unsigned long long total = 0, issued = 0;   // its paths, and those issued so far
     double t_submit = 0, t_done = 0;
     unsigned long long min_seq = 0;   // the first progress snapshot taken after its admission
@@ -410,7 +421,7 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
         }
     if (slot < 0) return lr_fail(LRH_EAGAIN, "engine: all %d job slots are in use", LR_JOBS);
     const size_t npix = (size_t)j->w * (size_t)j->h;
-    const size_t bytes = lr_align(npix * 24);
+    const size_t bytes = lr_engine_job_bytes(*j);
     const size_t off = arena_take(e, bytes);
     if (off == (size_t)-1) return lr_fail(LRH_EAGAIN, "engine: the image (%zu bytes) does not fit now", bytes);
     HostJob &h = e->hj[slot];
@@ -420,6 +431,11 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
     h.desc = *j;
     h.off = off;
     h.bytes = bytes;
+    // a denoised job: the guide (32 bytes a pixel), the moments (16), then the image (24)
+    const size_t gbytes = j->denoise ? lr_align(npix * 32) : 0, mbytes = j->denoise ? lr_align(npix * 16) : 0;
+    h.guide_off = off;
+    h.mom_off = off + gbytes;
+    h.acc_off = off + gbytes + mbytes;
     h.total = (unsigned long long)npix * (unsigned long long)j->spp;
     h.t_submit = now_ms();
     h.min_seq = e->seq_next;
@@ -437,7 +453,11 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
     JobDev jd;
     memset(&jd, 0, sizeof(jd));
     lr_job_scene(jd, s);
-    jd.accum = (long long *)arena_ptr(e, off);
+    jd.accum = (long long *)arena_ptr(e, h.acc_off);
+    if (j->denoise) {
+        jd.guide = (const float4 *)arena_ptr(e, h.guide_off);
+        jd.moments = (long long *)arena_ptr(e, h.mom_off);
+    }
     jd.total = h.total;
     memcpy(jd.cam, j->cam, sizeof(jd.cam));
     jd.W = j->w;
@@ -450,10 +470,20 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
     jd.seed_hi = (uint32_t)(j->seed >> 32);
     jd.clamp = j->clamp;
     hipLaunchKernelGGL(k_job_put, dim3(1), dim3(1), 0, e->stream, e->jobs, e->prog->jret, e->prog->jdone, slot, jd);
-    const size_t words = npix * 3;
+    // the image cleared (and a denoised job's moments, just before it)
+    long long *zero = j->denoise ? jd.moments : jd.accum;
+    const size_t words = j->denoise ? (h.acc_off - h.mom_off) / 8 + npix * 3 : npix * 3;
     hipLaunchKernelGGL(k_zero64, dim3((unsigned)((words + SBLOCK - 1) / SBLOCK < 4096 ? (words + SBLOCK - 1) / SBLOCK : 4096)),
-                       dim3(SBLOCK), 0, e->stream, jd.accum, words);
+                       dim3(SBLOCK), 0, e->stream, zero, words);
     ENG_HIP(hipGetLastError());
+    if (j->denoise) {
+        // its guide, before any of its paths (the same stream; the walks' spill area is free
+        // between the graphs' kernels)
+        const size_t threads = std::min((npix + 255) / 256 * 256, (size_t)e->threads_cap / 256 * 256);
+        table().guide[s->two_level ? 1 : 0](dim3((unsigned)(threads / 256)), e->stream, jd, (float4 *)jd.guide, e->base.spill,
+                                             threads, e->spill_cap, e->base.tab, e->overflow);
+        ENG_HIP(hipGetLastError());
+    }
     e->by_id[h.id] = slot;
     e->running++;
     *slot_out = slot;
@@ -835,10 +865,57 @@ int lrh_engine_image(LrhEngine *e, int64_t id, float *d_out, int add)
     if (h.state != LRH_JOB_DONE) return lr_fail(LRH_EAGAIN, "lrh_engine_image: job %lld is not done", (long long)id);
     const int npix = h.desc.w * h.desc.h;
     hipLaunchKernelGGL(k_resolve, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, e->stream,
-                       (const long long *)arena_ptr(e, h.off), npix, (float)h.desc.spp, (float4 *)d_out, add);
+                       (const long long *)arena_ptr(e, h.acc_off), npix, (float)h.desc.spp, (float4 *)d_out, add);
     ENG_HIP(hipGetLastError());
     ENG_HIP(hipStreamSynchronize(e->stream));
     return 0;
+}
+
+int lrh_engine_guide(LrhEngine *e, int64_t id, float *d_guide, float *d_moments)
+{
+    const int slot = job_slot(e, id);
+    if (slot < 0 || !d_guide || !d_moments) return lr_fail(LRH_EINVAL, "lrh_engine_guide: unknown job %lld or no output", (long long)id);
+    const HostJob &h = e->hj[slot];
+    if (!h.desc.denoise) return lr_fail(LRH_EINVAL, "lrh_engine_guide: job %lld was not submitted with denoise = 1", (long long)id);
+    if (h.state != LRH_JOB_DONE) return lr_fail(LRH_EAGAIN, "lrh_engine_guide: job %lld is not done", (long long)id);
+    const int npix = h.desc.w * h.desc.h;
+    ENG_HIP(hipSetDevice(e->device));
+    ENG_HIP(hipMemcpyAsync(d_guide, arena_ptr(e, h.guide_off), (size_t)npix * 32, hipMemcpyDeviceToDevice, e->stream));
+    hipLaunchKernelGGL(k_moments, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, e->stream,
+                       (const long long *)arena_ptr(e, h.mom_off), npix, (float2 *)d_moments);
+    ENG_HIP(hipGetLastError());
+    ENG_HIP(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int lrh_engine_denoise(LrhEngine *e, int64_t id, int passes, float *d_out)
+{
+    const int slot = job_slot(e, id);
+    if (slot < 0 || !d_out || passes < 1 || passes > 5)
+        return lr_fail(LRH_EINVAL, "lrh_engine_denoise: unknown job %lld, no output, or passes not in 1..5", (long long)id);
+    const HostJob &h = e->hj[slot];
+    if (!h.desc.denoise) return lr_fail(LRH_EINVAL, "lrh_engine_denoise: job %lld was not submitted with denoise = 1", (long long)id);
+    if (h.state != LRH_JOB_DONE) return lr_fail(LRH_EAGAIN, "lrh_engine_denoise: job %lld is not done", (long long)id);
+    const int W = h.desc.w, H = h.desc.h, npix = W * H;
+    ENG_HIP(hipSetDevice(e->device));
+    // the image (float4 sums and counts) and the moments (float2) beside the filter's own buffers
+    const size_t fa = lr_align((size_t)npix * 16), fm = lr_align((size_t)npix * 8);
+    char *tmp = nullptr;
+    if (hipMalloc((void **)&tmp, fa + fm + lr_denoise_scratch_bytes(W, H)) != hipSuccess) {
+        (void)hipGetLastError();
+        return lr_fail(LRH_ENOMEM, "lrh_engine_denoise: device memory for the filter could not be had");
+    }
+    hipLaunchKernelGGL(k_resolve, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, e->stream,
+                       (const long long *)arena_ptr(e, h.acc_off), npix, (float)h.desc.spp, (float4 *)tmp, 0);
+    hipLaunchKernelGGL(k_moments, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, e->stream,
+                       (const long long *)arena_ptr(e, h.mom_off), npix, (float2 *)(tmp + fa));
+    int rc = hipGetLastError() == hipSuccess ? 0 : lr_fail(LRH_EDEVICE, "lrh_engine_denoise: a launch failed");
+    if (!rc)
+        rc = lr_denoise_launch(e->stream, (const float *)tmp, (const float *)(tmp + fa), (const float *)arena_ptr(e, h.guide_off),
+                               W, H, passes, tmp + fa + fm, d_out);
+    if (hipStreamSynchronize(e->stream) != hipSuccess && !rc) rc = lr_fail(LRH_EDEVICE, "lrh_engine_denoise: the filter failed");
+    (void)hipFree(tmp);
+    return rc;
 }
 
 int lrh_engine_release(LrhEngine *e, int64_t id)

@@ -106,7 +106,7 @@ __device__ __forceinline__ void ordered_append2(unsigned long long *stat, uint32
 // A path ends: its sample joins its pixel (NaN samples skipped and the clamp applied as
 // trace_image does, tracer.c:347-349), its place in the pool is freed, its job counts it. Every
 // lane of the wave calls this (on: the lane's path ends).
-__device__ __forceinline__ void retire(const EngArgs &A, bool on, uint32_t jslot, uint32_t q, float4 r)
+__device__ __forceinline__ void retire(const EngArgs &A, bool on, uint32_t jslot, uint32_t q, float4 r, bool glow = false)
 {
     if (on) {
         const JobDev &J = A.jobs[jslot];
@@ -114,11 +114,26 @@ __device__ __forceinline__ void retire(const EngArgs &A, bool on, uint32_t jslot
         if (c.x == c.x && c.y == c.y && c.z == c.z) {
             const float m = vmax3(c);
             if (J.clamp > 0.0f && m > J.clamp) c = vscale(c, J.clamp / m);
-            long long *px = J.accum + 3 * (size_t)(q % (uint32_t)J.npix);
+            const uint32_t pix = q % (uint32_t)J.npix;
+            long long *px = J.accum + 3 * (size_t)pix;
             const long long fx = to_fixed(c.x), fy = to_fixed(c.y), fz = to_fixed(c.z);
             if (fx) atomicAdd((unsigned long long *)&px[0], (unsigned long long)fx);
             if (fy) atomicAdd((unsigned long long *)&px[1], (unsigned long long)fy);
             if (fz) atomicAdd((unsigned long long *)&px[2], (unsigned long long)fz);
+            const float4 ga = J.moments ? J.guide[2 * (size_t)pix + 1] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            if (J.moments && !(glow && ga.w == 0.0f)) {
+                // the denoiser's moments: the luminance of the sample over the guide's albedo on a
+                // surface (its irradiance), and its square (trace.wgsl:1301-1309); not a sample whose
+                // camera ray found a bright glow where the guide sees a surface (its pixel's edge
+                // with a lamp: it would make the whole pixel look noisy)
+                const V3 ir = ga.w == 0.0f ? v3(c.x / rfmax(ga.x, 1e-3f), c.y / rfmax(ga.y, 1e-3f), c.z / rfmax(ga.z, 1e-3f)) : c;
+                const float l = lum(ir);
+                const float l2 = rfmin(l * l, LR_MOM2_CAP);
+                long long *mo = J.moments + 2 * (size_t)pix;
+                const long long fl = to_fixed(l), fl2 = __float2ll_rn(l2 * (float)(1u << LR_MOM2_SHIFT));
+                if (fl) atomicAdd((unsigned long long *)&mo[0], (unsigned long long)fl);
+                if (fl2) atomicAdd((unsigned long long *)&mo[1], (unsigned long long)fl2);
+            }
         }
     }
     const uint64_t m = __ballot(on);
@@ -462,6 +477,42 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_extend(E
     }
 }
 
+// The hit of a ray from o along d at t, shape hid (an instanced scene: inst is the instance):
+// its outward normal and material (set_hit, intersect.c:4-10; an instance's normal by the
+// inverse transpose, its material or its triangle's), ii the instance (LR_NONE: none)
+__device__ __forceinline__ void hit_frame(const JobDev &J, V3 o, V3 d, float t, uint32_t hid, uint32_t inst, V3 hp,
+                                          V3 &outward, int &mi, uint32_t &ii)
+{
+    ii = LR_NONE;
+    if (J.two_level) {
+        ii = inst;
+        const float4 *rows = J.inst + 4 * (size_t)ii;
+        const float4 info = rows[3];
+        const uint32_t kind = __float_as_uint(info.x), imat = __float_as_uint(info.y);
+        V3 oo, od, no;
+        to_object(rows, o, d, oo, od);
+        int tmat = 0;
+        if (kind == LR_UNIT_SPHERE) no = vadd(oo, vscale(od, t));
+        else if (kind == LR_UNIT_BOX) no = unit_box_normal(vadd(oo, vscale(od, t)));
+        else {
+            const float4 tu = J.tris[3 * (size_t)hid + 1], tv = J.tris[3 * (size_t)hid + 2];
+            no = vcross(v3of(tu), v3of(tv));
+            tmat = (int)(__float_as_uint(tv.w) >> 26);
+        }
+        outward = normal_to_world(rows, no);
+        mi = imat != LR_NONE ? (int)imat : tmat;
+    } else if (hid < J.nprims) {
+        const Prim pr = load_prim(J.prims, hid);
+        outward = pr.q.w < 0.5f ? vscale(vsub(hp, v3of(pr.c)), 1.0f / pr.c.w) : v3of(pr.n);
+        mi = (int)pr.u.w;
+    } else {
+        const size_t ti = hid - J.nprims;
+        const float4 tu = J.tris[3 * ti + 1], tv = J.tris[3 * ti + 2];
+        outward = vnorm(vcross(v3of(tu), v3of(tv)));
+        mi = (int)(__float_as_uint(tv.w) >> 26);
+    }
+}
+
 // ---- glowing shapes as lights (Scene.emitters: tracer.c:120-216) ----
 // 1 - cos of the half angle of the cone a ball of radius r fills from dist2 (squared) away, in
 // the form without cancellation for a small far ball; 0 from inside (ball_cone)
@@ -561,6 +612,7 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
         float4 nr0, nr1, sr0, sr1, scon;
         uint32_t jslot = 0, q = 0;
         int depth = 0, nsh = 0;
+        bool glow = false;
         uint64_t st = 0;
         float4 rad_out = make_float4(0.0f, 0.0f, 0.0f, 0.0f), thr_out;
         if (live) {
@@ -568,7 +620,8 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
             const JobDev &J = A.jobs[jslot];
             const uint2 mt = A.meta[cur][i];
             q = mt.x;
-            depth = (int)mt.y;
+            depth = (int)(mt.y & 0x7fffffffu);
+            glow = mt.y >> 31;   // the camera ray found a bright glow (glow_bright, trace.wgsl:1185)
             const float4 r0 = A.qray[cur][2 * (size_t)i], r1 = A.qray[cur][2 * (size_t)i + 1];
             const float4 hv = A.hits[i];
             const float4 thr4 = A.thr[cur][i], rad4 = A.rad[cur][i];
@@ -599,39 +652,16 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
                 const V3 hp = vadd(o, vscale(d, t));
                 V3 outward;
                 int mi;
-                uint32_t ii = LR_NONE;
-                if (J.two_level) {
-                    // an instance's hit: the normal by the inverse transpose, its material or the triangle's
-                    ii = __float_as_uint(hv.z);
-                    const float4 *rows = J.inst + 4 * (size_t)ii;
-                    const float4 info = rows[3];
-                    const uint32_t kind = __float_as_uint(info.x), imat = __float_as_uint(info.y);
-                    V3 oo, od, no;
-                    to_object(rows, o, d, oo, od);
-                    int tmat = 0;
-                    if (kind == LR_UNIT_SPHERE) no = vadd(oo, vscale(od, t));
-                    else if (kind == LR_UNIT_BOX) no = unit_box_normal(vadd(oo, vscale(od, t)));
-                    else {
-                        const float4 tu = J.tris[3 * (size_t)hid + 1], tv = J.tris[3 * (size_t)hid + 2];
-                        no = vcross(v3of(tu), v3of(tv));
-                        tmat = (int)(__float_as_uint(tv.w) >> 26);
-                    }
-                    outward = normal_to_world(rows, no);
-                    mi = imat != LR_NONE ? (int)imat : tmat;
-                } else if (hid < J.nprims) {
-                    const Prim pr = load_prim(J.prims, hid);
-                    outward = pr.q.w < 0.5f ? vscale(vsub(hp, v3of(pr.c)), 1.0f / pr.c.w) : v3of(pr.n);
-                    mi = (int)pr.u.w;
-                } else {
-                    const size_t ti = hid - J.nprims;
-                    const float4 tu = J.tris[3 * ti + 1], tv = J.tris[3 * ti + 2];
-                    outward = vnorm(vcross(v3of(tu), v3of(tv)));
-                    mi = (int)(__float_as_uint(tv.w) >> 26);
-                }
+                uint32_t ii;
+                hit_frame(J, o, d, t, hid, __float_as_uint(hv.z), hp, outward, mi, ii);
                 const bool front = vdot(d, outward) < 0.0f;
                 const V3 nrm = front ? outward : vneg(outward);
                 int type;
                 Bsdf b = bsdf_make(J.mats, mi, front, A.tab, type);
+                if (depth == 0 && type != MAT_LIGHT) {   // (a glow, not a lamp: trace.wgsl:1183-1186)
+                    const float4 e1 = J.mats[3 * mi + 1];
+                    if (rfmax(e1.x, rfmax(e1.y, e1.z)) >= 1.0f) glow = true;
+                }
                 if (type == MAT_LIGHT) {
                     if (front) {
                         const V3 em = v3of(J.mats[3 * mi + 1]);
@@ -791,20 +821,20 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
             A.thr[1 - cur][at] = thr_out;
             A.rad[1 - cur][at] = rad_out;
             A.rng[1 - cur][at] = st;
-            A.meta[1 - cur][at] = make_uint2(q, (uint32_t)depth + 1u);
+            A.meta[1 - cur][at] = make_uint2(q, ((uint32_t)depth + 1u) | (glow ? 0x80000000u : 0u));
         }
         // the shadow ray carries the path's radiance and q (its contribution is added to one or
         // the other: the path's next state, or its pixel if the path ends with it)
         if (shadow) {
             A.sray[2 * (size_t)ss] = sr0;
             A.sray[2 * (size_t)ss + 1] = sr1;
-            A.sid[ss] = (jslot << LR_SLOT_BITS) | (next ? at : 0u);
+            A.sid[ss] = (jslot << LR_SLOT_BITS) | (next ? at : (glow ? 1u : 0u));   // (a path that ends: its glow flag)
             A.scon[ss] = scon;
             A.srad[ss] = rad_out;
             A.sq[ss] = q;
             if (MULTI) A.sx[ss] = ((uint32_t)(nsh - 1) << 30) | i;
         }
-        retire(A, live && !next && !shadow, jslot, q, rad_out);
+        retire(A, live && !next && !shadow, jslot, q, rad_out, glow);
     });
 }
 
@@ -830,10 +860,12 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(E
         uint32_t jslot = 0, q = 0;
         float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         int nx = 0;
+        bool glow = false;
         if (live) {
             const uint32_t id = A.sid[i];
             const uint32_t at = id & LR_SLOT_MASK;
             jslot = id >> LR_SLOT_BITS;
+            glow = (at & 1u) != 0u;   // (read only for a path that ends here: then at holds its glow flag)
             const JobDev &J = A.jobs[jslot];
             if constexpr (!MULTI) {
                 const float4 r0 = A.sray[2 * (size_t)i], r1 = A.sray[2 * (size_t)i + 1];
@@ -905,8 +937,80 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(E
                 if ((threadIdx.x & 63) == 0) atomicAdd(&ctl->rays_shadow, (unsigned long long)cnt);
             }
         }
-        retire(A, ends, jslot, q, r);
+        retire(A, ends, jslot, q, r, glow);
     });
+}
+
+// The denoiser's guide of a job (trace.wgsl:1034-1107, guide): each pixel's ray through its
+// centre, through mirrors and glass (refracted unless the glass reflects all) to the first
+// surface light scatters on: its normal facing the ray, its distance along the way, and the
+// albedo of the whole way to it (kind 0); a lamp, a surface glowing at least as bright as white
+// paper under a light of 1 (glow_bright, :1115), or nothing within 6 vertices: kind 1, the
+// albedo 1. A grid-stride loop over the job's pixels; the walks' stacks spill as extend's do.
+template <int BLK, int STK, bool TL>
+__global__ __launch_bounds__(BLK) void k_guide(JobDev J, float4 *guide, uint32_t *spill, size_t spill_stride, int spill_per,
+                                              const float *tab, uint32_t *overflow)
+{
+    __shared__ uint32_t stk[STK][BLK];
+    const size_t gtid = (size_t)blockIdx.x * BLK + threadIdx.x;
+    const WalkStack S{&stk[0][threadIdx.x], BLK, spill + gtid, spill_stride, STK + spill_per};
+    for (size_t p = gtid; p < (size_t)J.npix; p += (size_t)gridDim.x * BLK) {
+        const int x = (int)(p % (uint32_t)J.W), y = (int)(p / (uint32_t)J.W);
+        const float u = ((float)x + 0.5f) / (float)J.W, v = ((float)y + 0.5f) / (float)J.H;
+        const V3 org = v3(J.cam[0], J.cam[1], J.cam[2]);
+        const V3 ul = v3(J.cam[3], J.cam[4], J.cam[5]), hz = v3(J.cam[6], J.cam[7], J.cam[8]),
+                 vt = v3(J.cam[9], J.cam[10], J.cam[11]);
+        V3 o = org, d = vnorm(vsub(vsub(vadd(ul, vscale(hz, u)), vscale(vt, v)), org));
+        V3 gn = v3(0.0f, 0.0f, 0.0f), ga = v3(1.0f, 1.0f, 1.0f), a = v3(1.0f, 1.0f, 1.0f);
+        float gz = 1e30f, kind = 1.0f, dist = 0.0f;
+        for (int i = 0; i < 6; i++) {
+            uint32_t cn, ct;
+            WalkHit h;
+            if (TL && J.two_level) h = walk4_tl<false, false, STK>(J, o, d, J.eps, 1e30f, S, overflow, cn, ct);
+            else h = walk4<false, false, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, o, d, J.eps, 1e30f, S, overflow, cn, ct);
+            if (h.id == LR_NONE) break;
+            const V3 hp = vadd(o, vscale(d, h.t));
+            V3 outward;
+            int mi;
+            uint32_t ii;
+            hit_frame(J, o, d, h.t, h.id, __float_as_uint(h.b1), hp, outward, mi, ii);
+            const bool front = vdot(d, outward) < 0.0f;
+            const V3 nrm = front ? outward : vneg(outward);
+            const float4 m0 = J.mats[3 * mi], m1 = J.mats[3 * mi + 1];
+            int type;
+            const Bsdf b = bsdf_make(J.mats, mi, front, tab, type);
+            if (type == MAT_LIGHT || rfmax(m1.x, rfmax(m1.y, m1.z)) >= 1.0f) {
+                gn = nrm;
+                gz = dist + h.t;
+                break;
+            }
+            if (!b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f) {   // bsdf_rough_part
+                gn = nrm;
+                gz = dist + h.t;
+                ga = vmul(a, v3of(m0));
+                kind = 0.0f;
+                break;
+            }
+            a = vmul(a, v3of(m0));
+            dist += h.t;
+            V3 nxt = vreflect(d, nrm);
+            if (b.transmission > 0.5f && b.metallic < 0.5f) {
+                const float cos_i = rfmin(vdot(vneg(d), nrm), 1.0f);
+                if (b.eta * b.eta * (1.0f - cos_i * cos_i) <= 1.0f) nxt = vrefract(d, nrm, b.eta);
+            }
+            o = hp;
+            d = vnorm(nxt);
+        }
+        guide[2 * p] = make_float4(gn.x, gn.y, gn.z, gz);
+        guide[2 * p + 1] = make_float4(ga.x, ga.y, ga.z, kind);
+    }
+}
+
+template <int BLK, int STK, bool TL>
+void L_guide(dim3 g, hipStream_t s, const JobDev &J, float4 *guide, uint32_t *spill, size_t stride, int spill_per,
+             const float *tab, uint32_t *overflow)
+{
+    hipLaunchKernelGGL((k_guide<BLK, STK, TL>), g, dim3(BLK), 0, s, J, guide, spill, stride, spill_per, tab, overflow);
 }
 
 template <int BLK>
@@ -948,6 +1052,10 @@ template <int BLK>
 void fill_table(LrEngTable &t)
 {
     const int b = lr_blk_index(BLK);
+    if (BLK == 256) {
+        t.guide[0] = L_guide<256, 16, false>;
+        t.guide[1] = L_guide<256, 16, true>;
+    }
     t.regen[b] = L_regen<BLK>;
     t.shade[b][0] = L_shade<BLK, false>;
     t.shade[b][1] = L_shade<BLK, true>;

@@ -66,6 +66,13 @@ int lr_build_noalloc(LrhContext *c, const float *boxes, int n, const float *host
                      uint32_t ref0, uint32_t ref_add, uint32_t *nodes, uint32_t *aux, uint32_t *sorted,
                      LrhBuildStats *stats);
 
+// ---- lr_denoise.hip ----
+size_t lr_denoise_scratch_bytes(int w, int h);
+// the denoiser on stream s: accum (float4 sum, count), moments (float2), guide (2 float4 a
+// pixel) -> out (float4 rgb, 1); scratch: lr_denoise_scratch_bytes
+int lr_denoise_launch(hipStream_t s, const float *accum, const float *moments, const float *guide, int w, int h, int passes,
+                      void *scratch, float *out);
+
 // ---- lr_trace.hip ----
 int lr_launch_trace(LrhContext *c, const uint32_t *nodes, uint32_t root, const uint32_t *aux, const float *tris,
                     const float *rays, int nrays, const uint32_t *count, int any, float *hits, int max_depth,

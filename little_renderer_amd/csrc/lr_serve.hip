@@ -62,7 +62,7 @@ struct LrhServer {
 
 namespace {
 
-size_t image_bytes_of(const LrhJob &j) { return lr_align((size_t)j.w * (size_t)j.h * 24); }
+size_t image_bytes_of(const LrhJob &j) { return lr_engine_job_bytes(j); }
 
 void fail_job(LrhServer *s, SrvJob &j, int rc)
 {

@@ -86,6 +86,13 @@ def lib():
             "lro_env_pdf": (F, [P, P]),
             "lro_scene_trace": (None, [P, P, I, I, P]),
             "lro_find_emitters": (I, [P, P, P, I, P, I, P, P]),
+            "lro_guide": (None, [P, P, I, I, P]),
+            "lro_pt_image_moments": (None, [P, P, I, I, U, I, U64, F, I, I, P, P, P]),
+            "lro_denoise": (None, [P, P, P, I, I, I, P]),
+            "lro_f2h": (ctypes.c_uint16, [F]),
+            "lro_h2f": (F, [ctypes.c_uint16]),
+            "lro_oct_pack": (None, [P, P]),
+            "lro_oct_unpack": (None, [P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -425,6 +432,43 @@ def pt_image(scene, W, H, s0, ns, seed=0, clamp=0.0, max_depth=8, mis=True, tab=
     cam = np.ascontiguousarray(scene["cam"], np.float32)
     (lib_d() if double else lib()).lro_pt_image(ctypes.byref(sc), ptr(cam), W, H, s0, ns, seed, clamp, max_depth, 1 if mis else 0, ptr(out))
     del keep
+    return out
+
+
+def guide(scene, W, H, tab, mask=None):
+    """lro_guide: the denoiser's guide (H, W, 8) float32: n.xyz, z | albedo.rgb, kind (tab: the
+    BSDF tables, which decide a surface's kind)"""
+    if tab is None:
+        raise ValueError("lro.guide needs the BSDF tables")
+    sc, keep = scene_struct(scene, tab, mask)
+    out = np.zeros((H, W, 8), np.float32)
+    cam = np.ascontiguousarray(scene["cam"], np.float32)
+    lib().lro_guide(ctypes.byref(sc), ptr(cam), W, H, ptr(out))
+    del keep
+    return out
+
+
+def pt_image_moments(scene, W, H, s0, ns, guide_, seed=0, clamp=0.0, max_depth=8, mis=True, tab=None, mask=None):
+    """lro_pt_image_moments: (sums (H, W, 3), moments (H, W, 2)) for the guide guide_"""
+    sc, keep = scene_struct(scene, tab, mask)
+    out = np.zeros((H, W, 3), np.float32)
+    mom = np.zeros((H, W, 2), np.float32)
+    g = np.ascontiguousarray(guide_, np.float32)
+    cam = np.ascontiguousarray(scene["cam"], np.float32)
+    lib().lro_pt_image_moments(ctypes.byref(sc), ptr(cam), W, H, s0, ns, seed, clamp, max_depth, 1 if mis else 0, ptr(g),
+                               ptr(out), ptr(mom))
+    del keep
+    return out, mom
+
+
+def denoise(accum, moments, guide_, passes=5):
+    """lro_denoise: accum (H, W, 4), moments (H, W, 2), guide (H, W, 8) -> (H, W, 4)"""
+    a = np.ascontiguousarray(accum, np.float32)
+    m = np.ascontiguousarray(moments, np.float32)
+    g = np.ascontiguousarray(guide_, np.float32)
+    H, W = a.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    lib().lro_denoise(ptr(a), ptr(m), ptr(g), W, H, passes, ptr(out))
     return out
 
 

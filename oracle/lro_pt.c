@@ -1228,8 +1228,9 @@ static V sample_direct_emit(const LroScene *s, const Hit *h, Bsdf *b, V v, int m
 enum { EMIT_FULL, EMIT_NONE, EMIT_MIS };
 
 /* trace_path_sampled (tracer.c:364-509) without media or textures */
-static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis)
+static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis, int *cam_glow)
 {
+    *cam_glow = 0;
     V radiance = v3(0, 0, 0), throughput = v3(1, 1, 1);
     int emit = EMIT_FULL;
     real bounce_pdf = 0;
@@ -1259,6 +1260,9 @@ static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis
             }
             break;
         }
+        /* the camera ray found a surface glowing at least as bright as 1 (not a lamp): the
+           denoiser's moments leave the sample out (trace.wgsl:1183-1186, 1309) */
+        if (depth == 0 && rfmax(m->emission[0], rfmax(m->emission[1], m->emission[2])) >= 1) *cam_glow = 1;
         /* a glowing surface (tracer.c:450-458): found by the path; a glowing shape after a rough
            bounce was sampled by NEE: weighted against it (MIS) or counted there already */
         if (glows(m)) {
@@ -1325,17 +1329,86 @@ void lro_pt_seed(LroRng *rng, uint64_t pixel, uint32_t k, uint64_t seed)
                           ((uint64_t)k * 0xA0761D6478BD642FULL), pixel);
 }
 
+/* The denoiser's guide (trace.wgsl:1034-1107, guide): the ray through each pixel's centre,
+ * through mirrors and glass (refracted unless the glass reflects all) to the first surface light
+ * scatters on (kind 0: its normal facing the ray, its distance along the way, the albedo of the
+ * way); a lamp or a surface glowing at least as bright as 1 (glow_bright, :1115): its normal and
+ * distance, kind 1, albedo 1; nothing within 6 vertices: kind 1, distance 1e30. 8 floats a
+ * pixel: n.xyz, z | albedo.rgb, kind. */
+void lro_guide(const LroScene *s, const float cam[12], int W, int H, float *out)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            V o, d;
+            camera_ray(cam, ((real)x + 0.5f) / W, ((real)y + 0.5f) / H, &o, &d);
+            V gn = v3(0, 0, 0), ga = v3(1, 1, 1), a = v3(1, 1, 1);
+            real gz = 1e30f, kind = 1, dist = 0;
+            for (int i = 0; i < 6; i++) {
+                Hit h;
+                if (!scene_intersect(s, o, d, s->eps, 1e30f, &h)) break;
+                const LroMat *m = &s->mats[hit_material(s, &h)];
+                Bsdf b = bsdf_make(m, h.front, s->tab);
+                if (m->type == LRO_MAT_LIGHT || rfmax(m->emission[0], rfmax(m->emission[1], m->emission[2])) >= 1) {
+                    gn = h.n;
+                    gz = dist + h.t;
+                    break;
+                }
+                if (rough_part(&b)) {
+                    gn = h.n;
+                    gz = dist + h.t;
+                    ga = vmul(a, vload(m->albedo));
+                    kind = 0;
+                    break;
+                }
+                a = vmul(a, vload(m->albedo));
+                dist += h.t;
+                V nxt = vreflect(d, h.n);
+                if (b.transmission > 0.5f && b.metallic < 0.5f) {
+                    real cos_i = rfmin(vdot(vneg(d), h.n), 1);
+                    if (b.eta * b.eta * (1 - cos_i * cos_i) <= 1) nxt = vrefract(d, h.n, b.eta);
+                }
+                o = h.p;
+                d = vnorm(nxt);
+            }
+            float *g = out + 8 * ((size_t)y * W + x);
+            g[0] = (float)gn.x; g[1] = (float)gn.y; g[2] = (float)gn.z; g[3] = (float)gz;
+            g[4] = (float)ga.x; g[5] = (float)ga.y; g[6] = (float)ga.z; g[7] = (float)kind;
+        }
+}
+
+static void pt_window(const LroScene *s, const float cam[12], int W, int H, int x0, int y0, int cw, int ch, uint32_t s0,
+                      int ns, uint64_t seed, float clamp, int max_depth, int mis, float *sum, const float *guide,
+                      float *moments);
+
 /* trace_image (tracer.c:329-356) over the pixels x0 .. x0+cw-1, y0 .. y0+ch-1 of a W x H image:
  * the sum of samples s0 .. s0+ns-1 of each into sum (ch x cw, rgb, float, the samples added in
  * order; NaNs skipped, clamp applied) */
 void lro_pt_window(const LroScene *s, const float cam[12], int W, int H, int x0, int y0, int cw, int ch, uint32_t s0,
                    int ns, uint64_t seed, float clamp, int max_depth, int mis, float *sum)
 {
+    pt_window(s, cam, W, H, x0, y0, cw, ch, s0, ns, seed, clamp, max_depth, mis, sum, NULL, NULL);
+}
+
+/* the whole image, and the denoiser's moments beside (trace.wgsl:1301-1308): per pixel the sums
+ * of its samples' irradiance luminance (the sample over the guide's albedo on a surface) and of
+ * its square (2 floats, added to moments) */
+void lro_pt_image_moments(const LroScene *s, const float cam[12], int W, int H, uint32_t s0, int ns, uint64_t seed,
+                          float clamp, int max_depth, int mis, const float *guide, float *sum, float *moments)
+{
+    pt_window(s, cam, W, H, 0, 0, W, H, s0, ns, seed, clamp, max_depth, mis, sum, guide, moments);
+}
+
+static void pt_window(const LroScene *s, const float cam[12], int W, int H, int x0, int y0, int cw, int ch, uint32_t s0,
+                      int ns, uint64_t seed, float clamp, int max_depth, int mis, float *sum, const float *guide,
+                      float *moments)
+{
 #pragma omp parallel for schedule(dynamic, 1) collapse(2)
     for (int y = y0; y < y0 + ch; y++)
         for (int x = x0; x < x0 + cw; x++) {
             uint64_t pixel = (uint64_t)y * W + x;
             V acc = v3(0, 0, 0);
+            double m1 = 0, m2 = 0;
             for (int q = 0; q < ns; q++) {
                 uint32_t k = s0 + (uint32_t)q;
                 LroRng rng;
@@ -1346,13 +1419,25 @@ void lro_pt_window(const LroScene *s, const float cam[12], int W, int H, int x0,
                 real u = (x + jx) / W, v = (y + jy) / H;
                 V o, d;
                 camera_ray(cam, u, v, &o, &d);
-                V c = trace_path(s, o, d, &sm, max_depth, mis);
+                int cam_glow;
+                V c = trace_path(s, o, d, &sm, max_depth, mis, &cam_glow);
                 if (!(c.x == c.x && c.y == c.y && c.z == c.z)) continue;
                 if (clamp > 0 && vmax(c) > clamp) c = vscale(c, clamp / vmax(c));
                 acc = vadd(acc, c);
+                if (moments && !(cam_glow && guide[8 * pixel + 7] == 0)) {
+                    const float *ga = guide + 8 * pixel + 4;
+                    V ir = ga[3] == 0 ? v3(c.x / rfmax(ga[0], 1e-3f), c.y / rfmax(ga[1], 1e-3f), c.z / rfmax(ga[2], 1e-3f)) : c;
+                    float l = (float)lum(ir);
+                    m1 += l;
+                    m2 += (double)(l * l);
+                }
             }
             float *o3 = sum + 3 * ((size_t)(y - y0) * cw + (x - x0));
             o3[0] += acc.x; o3[1] += acc.y; o3[2] += acc.z;
+            if (moments) {
+                moments[2 * pixel] += (float)m1;
+                moments[2 * pixel + 1] += (float)m2;
+            }
         }
 }
 

@@ -121,6 +121,18 @@ void lro_pt_seed(LroRng *rng, uint64_t pixel, uint32_t k, uint64_t seed);
 /* adds samples s0 .. s0+ns-1 of every pixel (rgb sums) into sum */
 void lro_pt_image(const LroScene *s, const float cam[12], int W, int H, uint32_t s0, int ns, uint64_t seed, float clamp,
                   int max_depth, int mis, float *sum);
+/* the denoiser's guide of a W x H image (8 floats a pixel: n.xyz, z | albedo.rgb, kind) */
+void lro_guide(const LroScene *s, const float cam[12], int W, int H, float *out);
+/* lro_pt_image, and each pixel's moments of its samples' irradiance luminance (2 floats) */
+void lro_pt_image_moments(const LroScene *s, const float cam[12], int W, int H, uint32_t s0, int ns, uint64_t seed,
+                          float clamp, int max_depth, int mis, const float *guide, float *sum, float *moments);
+/* the denoiser (lrh_denoise's filter): accum float4, moments float2, guide 8 floats -> out float4 */
+void lro_denoise(const float *accum, const float *moments, const float *guide, int W, int H, int passes, float *out);
+/* its packing: IEEE-754 halves (round to nearest even), the normal on an octahedron in two snorm16 */
+uint16_t lro_f2h(float f);
+float lro_h2f(uint16_t h);
+void lro_oct_pack(const float n[3], uint16_t out[2]);
+void lro_oct_unpack(const uint16_t in[2], float n[3]);
 /* the same for the pixels x0 .. x0+cw-1, y0 .. y0+ch-1 of the W x H image only (sum: ch x cw) */
 void lro_pt_window(const LroScene *s, const float cam[12], int W, int H, int x0, int y0, int cw, int ch, uint32_t s0,
                    int ns, uint64_t seed, float clamp, int max_depth, int mis, float *sum);

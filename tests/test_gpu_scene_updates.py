@@ -8,6 +8,9 @@ path; the reference's upload_instances, engine/gpu/trace_gpu.c:1385-1512):
 - lrh_pt_scene_mesh: a dynamic mesh's new triangles, its tree rebuilt in place, the top level
   rebuilt. The scene equals one made with the new triangles, by the same checks; a wrong
   triangle count is refused.
+- Either way, the updated scene's image equals the oracle's image of the final scene (brute force
+  over every instance, the final poses and triangles) sample for sample, as a scene made anew
+  does (test_gpu_sky_inst.py).
 """
 import sys
 
@@ -15,6 +18,7 @@ import numpy as np
 import pytest
 
 import little_renderer_amd as lr
+from oracle import lro
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -86,6 +90,36 @@ def image(ctx, ps, cam, W=32, H=32, spp=4):
     return t.cpu().numpy()
 
 
+SKY = None
+
+
+def sky():
+    global SKY
+    if SKY is None:
+        SKY = lr.sky(64, 35.0, 60.0)
+    return SKY
+
+
+def oracle_image(mtris, first, mats, inst, cam, W=32, H=32, spp=4):
+    """the oracle's image of the scene (the sums of samples 0 .. spp-1, seed 3), as image() takes it"""
+    oi = np.zeros(len(inst), lro.INST_DTYPE)
+    for k in range(len(inst)):
+        oi[k]["inv"], oi[k]["kind"], oi[k]["material"] = lr.affine_inverse(inst[k]["xform"]), inst[k]["kind"], inst[k]["material"]
+    sc = {"mats": mats, "light": -1, "eps": 1e-4, "margin": 1e-3, "meshes": (mtris, first), "instances": oi,
+          "fwd": np.ascontiguousarray(inst["xform"]), "sky": lro.Env(sky()), "cam": cam}
+    return lro.pt_image(sc, W, H, 0, spp, seed=3, max_depth=8, tab=lr.brdf_tables(), mask=lro.blue_noise())
+
+
+def against_oracle(ctx, a, cam, parts, what):
+    g = image(ctx, a, cam)
+    o = oracle_image(*parts, cam)
+    rel = np.abs(g[..., :3] - o) / np.maximum(np.abs(o), 1e-3)
+    say(f"  {what} vs the oracle's final scene: median rel {np.median(rel):.2e}, >1e-3 in {np.mean(rel > 1e-3):.4f}")
+    assert np.median(rel) < 2e-5
+    assert np.mean(rel > 1e-3) < 0.02
+    assert abs(g[..., :3].mean() - o.mean()) < 2e-3 * o.mean()
+
+
 def same_scene(ctx, a, b, cam, what):
     rays = rays_at(20000, 8)
     ha, hb = a.trace(rays), b.trace(rays)
@@ -112,12 +146,13 @@ def test_move_equals_a_scene_made_with_the_final_poses(ctx):
     inst1[7]["material"] = 3 if inst0[7]["material"] != 3 else 4      # a new material override too
     say("move 5 of 14 instances")
     a = lr.PtScene.instanced(ctx, mtris, first, mats, inst0)
-    a.set_sky(lr.sky(64, 35.0, 60.0))
+    a.set_sky(sky())
     st = a.move(idx, inst1[idx])
     assert st.depth >= 1
     b = lr.PtScene.instanced(ctx, mtris, first, mats, inst1)
-    b.set_sky(lr.sky(64, 35.0, 60.0))
+    b.set_sky(sky())
     same_scene(ctx, a, b, cam, "moved")
+    against_oracle(ctx, a, cam, (mtris, first, mats, inst1), "moved")
     # nothing moved: nothing sent, the same scene
     st = a.move(np.zeros(0, np.int32), inst1[:0])
     assert st.depth == 0 and st.ms == 0.0
@@ -135,12 +170,13 @@ def test_mesh_update_equals_a_scene_made_with_the_new_triangles(ctx):
     f, l = first[3], first[4]
     say(f"mesh 3 ({l - f} triangles) replaced")
     a = lr.PtScene.instanced(ctx, mtris0, first, mats, inst)
-    a.set_sky(lr.sky(64, 35.0, 60.0))
+    a.set_sky(sky())
     st = a.mesh(3, mtris1[f:l])
     assert st.depth >= 1
     b = lr.PtScene.instanced(ctx, mtris1, first, mats, inst)
-    b.set_sky(lr.sky(64, 35.0, 60.0))
+    b.set_sky(sky())
     same_scene(ctx, a, b, cam, "new mesh")
+    against_oracle(ctx, a, cam, (mtris1, first, mats, inst), "new mesh")
     with pytest.raises(lr.LrhError):
         a.mesh(3, mtris1[f:l - 1])
 
