@@ -458,12 +458,14 @@ def bench_batched(a, torch, lr):
     return out
 
 
-def bench_serving(ctx, lr, pool, depth, tune, rates, nreq):
+def bench_serving(ctx, lr, pool, depth, tune, rates, nreq, target=1e-3):
     """Serving: the pool's scenes resident in a server; for each arrival rate, nreq requests
     (a random scene and camera, a resolution from the batched mix, spp from {4, 16, 64}) arrive
     as a Poisson stream; each is waited for and released by a client thread. Per rate: the rays
     traced per second from the first arrival to the last completion, the completed jobs per
-    second, and p50/p99 latency (submission to the last sample)."""
+    second, and p50/p99 latency (submission to the last sample). Each request carries the workload's
+    target error (1e-3): its first 4 passes decide how many of its spp it takes (the image mean's
+    relative standard error at the target; spp_taken: the mean fraction taken)."""
     import threading
     import queue
     budget = sum(ps.info()[2] for ps in pool) + (24 << 30)
@@ -476,7 +478,7 @@ def bench_serving(ctx, lr, pool, depth, tune, rates, nreq):
         mix = lr.batch_plan(nreq, 200 + ri, spp_lo=4)
         gaps = rng.exponential(1.0 / lam, nreq)
         picks = rng.integers(0, len(pool), nreq)
-        q, lat, done_t = queue.Queue(), [], []
+        q, lat, done_t, frac, errs = queue.Queue(), [], [], [], []
 
         def collect():
             while True:
@@ -486,6 +488,8 @@ def bench_serving(ctx, lr, pool, depth, tune, rates, nreq):
                 info = srv.wait(i, 600000.0)
                 lat.append(info.t_done_ms - info.t_submit_ms)
                 done_t.append(info.t_done_ms)
+                frac.append(info.spp_done / info.spp)
+                errs.append(info.rel_error)
                 srv.release(i)
 
         th = threading.Thread(target=collect)
@@ -500,7 +504,7 @@ def bench_serving(ctx, lr, pool, depth, tune, rates, nreq):
                 time.sleep(d)
             _, w, h, spp = mix[k]
             j = lr.make_job(pool[int(picks[k])], lr.batch_camera(5000 + 1000 * ri + k, w, h), w, h, spp,
-                            seed=10000 + k, max_depth=8)
+                            seed=10000 + k, max_depth=8, target_error=target)
             q.put(srv.submit(j))
         q.put(None)
         th.join()
@@ -514,14 +518,16 @@ def bench_serving(ctx, lr, pool, depth, tune, rates, nreq):
                     "mrays_s": round((r1 - r0) / span / 1e3, 2), "jobs_s": round(jobs_s, 2),
                     "p50_ms": round(percentile(lat, 50), 1),
                     "p99_ms": round(percentile(lat, 99), 1) if len(lat) >= 100 else None,
-                    "span_s": round(span / 1e3, 3)})
+                    "span_s": round(span / 1e3, 3), "spp_taken": round(float(np.mean(frac)), 3),
+                    "max_rel_error": round(float(max(errs)), 6)})
         print(f"serving: {lam:g} jobs/s: {jobs_s:.1f} jobs/s done, p50 {percentile(lat, 50):.1f} ms"
               f"{' (saturated)' if saturated else ''}", file=sys.stderr, flush=True)
     srv.close()
     sat = [r["rate_jobs_s"] for r in res if not r["saturated"]]
     return {"scenes": len(pool), "rates": res,
+            "target_error": target,
             "note": "p99 from >= 100 completions only; saturated: completions < 0.9 x arrivals (latency there grows "
-                    "with the stream's length)",
+                    "with the stream's length); each job stops at the passes its target error needs (spp_taken)",
             "sustained_rate_jobs_s": max(sat) if sat else None,
             "capacity_jobs_s": max(r["jobs_s"] for r in res)}
 

@@ -284,6 +284,11 @@ typedef struct {
     float clamp;            /* 0: none */
     int denoise;            /* 1: also the denoiser's inputs (the guide before the first sample,
                                the samples' moments), 48 more bytes a pixel; 0: none */
+    float target_error;     /* > 0 (and spp > 4): stop at the first pass count K (4 <= K <= spp)
+                               at which the image mean's relative standard error falls to this,
+                               estimated from the first 4 passes (each pass 1 sample a pixel):
+                               K = ceil(4 (e4 / target)^2), e4 that of the 4 passes' means. The
+                               image is then exactly a K-spp job's. 0: all spp samples */
 } LrhJob;
 
 /* The engine's launch shape (what lrh_engine_autotune picks): work-group sizes of the walk
@@ -304,6 +309,10 @@ typedef struct {
     uint64_t paths;         /* w * h * spp */
     double t_submit_ms;     /* when it was admitted (lrh_now_ms's clock) */
     double t_done_ms;       /* when its last sample ended, from the device's clock (0 while running) */
+    int spp_done;           /* the samples per pixel it takes: spp, or K with a target error (0 while
+                               its first 4 passes decide) */
+    float rel_error;        /* with a target error: the image mean's relative standard error at
+                               spp_done passes, as estimated (e4 sqrt(4 / K)); else 0 */
 } LrhJobInfo;
 
 typedef struct {

@@ -50,12 +50,14 @@ class Job(ctypes.Structure):
     """LrhJob: one render job (gpu_frame_trace's arguments)"""
     _fields_ = [("scene", ctypes.c_void_p), ("cam", ctypes.c_float * 12), ("w", ctypes.c_int), ("h", ctypes.c_int),
                 ("spp", ctypes.c_int), ("s0", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("max_depth", ctypes.c_int),
-                ("mis", ctypes.c_int), ("clamp", ctypes.c_float), ("denoise", ctypes.c_int)]
+                ("mis", ctypes.c_int), ("clamp", ctypes.c_float), ("denoise", ctypes.c_int),
+                ("target_error", ctypes.c_float)]
 
 
 class JobInfo(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int), ("w", ctypes.c_int), ("h", ctypes.c_int), ("spp", ctypes.c_int),
-                ("paths", ctypes.c_uint64), ("t_submit_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double)]
+                ("paths", ctypes.c_uint64), ("t_submit_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double),
+                ("spp_done", ctypes.c_int), ("rel_error", ctypes.c_float)]
 
 
 class EngineStats(ctypes.Structure):
@@ -470,14 +472,16 @@ def batch_plan(njobs, seed, spp_lo=1):
     return [tuple(int(x[i]) for x in a) for i in range(njobs)]
 
 
-def make_job(scene, cam, w, h, spp, s0=0, seed=0, max_depth=8, mis=True, clamp=0.0, denoise=False):
-    """an LrhJob for a PtScene (denoise: also the denoiser's guide and moments)"""
+def make_job(scene, cam, w, h, spp, s0=0, seed=0, max_depth=8, mis=True, clamp=0.0, denoise=False, target_error=0.0):
+    """an LrhJob for a PtScene (denoise: also the denoiser's guide and moments; target_error: stop at
+    the pass count whose image mean reaches this relative standard error)"""
     j = Job()
     j.scene = scene._h.value if isinstance(scene, PtScene) else scene
     for k in range(12):
         j.cam[k] = float(cam[k])
     j.w, j.h, j.spp, j.s0, j.seed = w, h, spp, s0, seed
     j.max_depth, j.mis, j.clamp, j.denoise = max_depth, 1 if mis else 0, clamp, 1 if denoise else 0
+    j.target_error = target_error
     return j
 
 

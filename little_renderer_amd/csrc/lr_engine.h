@@ -72,7 +72,11 @@ struct JobDev {
     // int64: the sum in 2^-28 units, the sum of squares in 2^-20 units)
     const float4 *guide;
     long long *moments;
+    // a job with a target error, in its first 4 passes: each pass's luminance summed (fixed point,
+    // 2^-28) into the progress block, for the host to choose the pass count from
+    int pilot;
 };
+#define LR_PILOT_PASSES 4
 #define LR_MOM2_SHIFT 20                  // the squares' fixed-point units: 2^-20
 #define LR_MOM2_CAP 1099511627776.0f      // a square saturated at 2^40
 #define LR_MAX_EMITTERS 16
@@ -111,6 +115,7 @@ struct EngProgress {
     EngCtl ctl;
     unsigned long long jret[LR_JOBS];       // paths retired per job slot
     unsigned long long jdone[LR_JOBS];      // device clock when the slot's last path retired
+    long long jpass[LR_JOBS][LR_PILOT_PASSES];   // a pilot job's first passes: their luminance sums
 };
 
 // everything a kernel needs, passed by value
@@ -118,6 +123,7 @@ struct EngArgs {
     EngCtl *ctl;
     const JobDev *jobs;
     unsigned long long *jret, *jdone;
+    long long (*jpass)[LR_PILOT_PASSES];
     const unsigned long long *iq_start;     // issue record: the first global path index it covers,
     const uint32_t *iq_job;                 // its job slot,
     const uint32_t *iq_q0;                  // and the index in the job of its first path

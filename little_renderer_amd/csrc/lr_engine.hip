@@ -5,6 +5,7 @@
 // Progress: after each graph the engine's counters and every job slot's retired-path count
 // are copied to pinned host memory (two buffers, so the device never waits for the host); a
 // job is done when all its paths have retired, at the device clock its last retire recorded.
+#include <math.h>
 #include <string.h>
 
 #include <algorithm>
@@ -30,11 +31,20 @@ __global__ __launch_bounds__(SBLOCK) void k_eng_init(EngProgress *prog, uint32_t
         w[i] = (i == offsetof(EngCtl, tail) / 8) ? (unsigned long long)C : 0ull;
 }
 
-__global__ void k_job_put(JobDev *jobs, unsigned long long *jret, unsigned long long *jdone, int slot, JobDev jd)
+__global__ void k_job_put(JobDev *jobs, unsigned long long *jret, unsigned long long *jdone,
+                          long long (*jpass)[LR_PILOT_PASSES], int slot, JobDev jd)
 {
     jobs[slot] = jd;
     jret[slot] = 0;
     jdone[slot] = 0;
+    for (int k = 0; k < LR_PILOT_PASSES; k++) jpass[slot][k] = 0;
+}
+
+// a pilot job's chosen size: its paths in all, and no more pilot sums
+__global__ void k_job_total(JobDev *jobs, int slot, unsigned long long total)
+{
+    jobs[slot].total = total;
+    jobs[slot].pilot = 0;
 }
 
 __global__ __launch_bounds__(SBLOCK) void k_zero64(long long *p, size_t n)
@@ -109,6 +119,9 @@ struct HostJob {
     size_t off = 0, bytes = 0;  // its block in the arena: a denoised job's guide and moments, then its image
     size_t acc_off = 0, mom_off = 0, guide_off = 0;
     unsigned long long total = 0, issued = 0;   // its paths, and those issued so far
+    int spp_done = 0;           // the samples per pixel it takes (a target error's: decided after its pilot)
+    bool pilot = false;         // a target error's first 4 passes running (total: theirs)
+    float rel_error = 0.0f;     // the estimated relative standard error at spp_done
     double t_submit = 0, t_done = 0;
     unsigned long long min_seq = 0;   // the first progress snapshot taken after its admission
 };
@@ -274,12 +287,43 @@ double dev_ms(const LrhEngine *e, unsigned long long clk)
 
 // takes in a progress snapshot: jobs whose paths have all retired are done (a snapshot copied
 // before a job's admission may still show the slot's previous job: it is not looked at)
+// A pilot job's pass count from its first 4 passes' luminance sums, their means mu_k over the
+// image: e4 = sqrt(var(mu) / 4) / mean(mu) the image mean's relative standard error after them,
+// K = ceil(4 (e4 / target)^2) the passes that bring it to the target, within [4, spp] (a black
+// image: 4). The rest of its paths are issued after this; the device's total follows first.
+void pilot_decide(LrhEngine *e, int slot, const long long *sums)
+{
+    HostJob &h = e->hj[slot];
+    const double npix = (double)h.desc.w * (double)h.desc.h;
+    double mu[LR_PILOT_PASSES], mean = 0.0, var = 0.0;
+    for (int k = 0; k < LR_PILOT_PASSES; k++) {
+        mu[k] = (double)sums[k] / (double)(1u << LR_FIX_SHIFT) / npix;
+        mean += mu[k] / LR_PILOT_PASSES;
+    }
+    for (int k = 0; k < LR_PILOT_PASSES; k++) var += (mu[k] - mean) * (mu[k] - mean) / (LR_PILOT_PASSES - 1);
+    int K = h.desc.spp;
+    double e4 = 0.0;
+    if (var == 0.0) K = LR_PILOT_PASSES;
+    else if (mean > 0.0) {
+        e4 = sqrt(var / LR_PILOT_PASSES) / mean;
+        const double t = h.desc.target_error;
+        const double k = ceil(LR_PILOT_PASSES * (e4 / t) * (e4 / t));
+        K = k < LR_PILOT_PASSES ? LR_PILOT_PASSES : (k > h.desc.spp ? h.desc.spp : (int)k);
+    }
+    h.pilot = false;
+    h.spp_done = K;
+    h.rel_error = (float)(e4 * sqrt((double)LR_PILOT_PASSES / K));
+    h.total = (unsigned long long)npix * (unsigned long long)K;
+    hipLaunchKernelGGL(k_job_total, dim3(1), dim3(1), 0, e->stream, e->jobs, slot, h.total);
+}
+
 void absorb(LrhEngine *e, const EngProgress &p, unsigned long long seq)
 {
     e->last = p;
     for (int s = 0; s < LR_JOBS; s++) {
         HostJob &j = e->hj[s];
         if (j.state != LRH_JOB_RUNNING || seq < j.min_seq) continue;
+        if (j.pilot && p.jret[s] >= j.total) pilot_decide(e, s, p.jpass[s]);
         if (p.jret[s] >= j.total && j.total > 0) {
             j.state = LRH_JOB_DONE;
             j.t_done = dev_ms(e, p.jdone[s]);
@@ -437,6 +481,13 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
     h.mom_off = off + gbytes;
     h.acc_off = off + gbytes + mbytes;
     h.total = (unsigned long long)npix * (unsigned long long)j->spp;
+    h.spp_done = j->spp;
+    if (j->target_error > 0.0f && j->spp > LR_PILOT_PASSES) {
+        // a target error: the first 4 passes alone until they decide the pass count
+        h.pilot = true;
+        h.spp_done = 0;
+        h.total = (unsigned long long)npix * LR_PILOT_PASSES;
+    }
     h.t_submit = now_ms();
     h.min_seq = e->seq_next;
     const LrhPtScene *s = j->scene;
@@ -469,7 +520,9 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
     jd.seed_lo = (uint32_t)j->seed;
     jd.seed_hi = (uint32_t)(j->seed >> 32);
     jd.clamp = j->clamp;
-    hipLaunchKernelGGL(k_job_put, dim3(1), dim3(1), 0, e->stream, e->jobs, e->prog->jret, e->prog->jdone, slot, jd);
+    jd.pilot = h.pilot ? 1 : 0;
+    hipLaunchKernelGGL(k_job_put, dim3(1), dim3(1), 0, e->stream, e->jobs, e->prog->jret, e->prog->jdone, e->prog->jpass,
+                       slot, jd);
     // the image cleared (and a denoised job's moments, just before it)
     long long *zero = j->denoise ? jd.moments : jd.accum;
     const size_t words = j->denoise ? (h.acc_off - h.mom_off) / 8 + npix * 3 : npix * 3;
@@ -596,6 +649,7 @@ int lr_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t i
     A.ctl = &e->prog->ctl;
     A.jobs = e->jobs;
     A.jret = e->prog->jret;
+    A.jpass = e->prog->jpass;
     A.jdone = e->prog->jdone;
     A.iq_start = e->iq_start;
     A.iq_job = e->iq_job;
@@ -818,7 +872,15 @@ int lrh_engine_plain(LrhEngine *e, const LrhJob *job, int64_t *id)
     const EngArgs At = args_for(e, t.stack, (size_t)gt.x * t.block_trace, 1);
     const EngArgs As = args_for(e, t.stack, 0, 1);
     const int tl = e->two_level ? 1 : 0, ml = e->multi ? 1 : 0;
-    for (int s = 0; s < job->spp; s++) {
+    for (int s = 0; s < h.spp_done || h.pilot; s++) {
+        if (h.pilot && s == LR_PILOT_PASSES) {
+            // a target error: the first passes' sums decide how many passes the job takes
+            ENG_HIP(hipStreamSynchronize(e->stream));
+            long long sums[LR_PILOT_PASSES];
+            ENG_HIP(hipMemcpy(sums, e->prog->jpass[slot], sizeof(sums), hipMemcpyDeviceToHost));
+            pilot_decide(e, slot, sums);
+            if (s >= h.spp_done) break;
+        }
         // the pass: the job's paths s * npix .. (s + 1) * npix - 1, one per pixel
         rc = issue_chunk(e, slot, npix);
         if (rc) return rc;
@@ -851,6 +913,8 @@ int lrh_engine_job(LrhEngine *e, int64_t id, LrhJobInfo *info)
     info->h = h.desc.h;
     info->spp = h.desc.spp;
     info->paths = h.total;
+    info->spp_done = h.spp_done;
+    info->rel_error = h.rel_error;
     info->t_submit_ms = h.t_submit;
     info->t_done_ms = h.state == LRH_JOB_DONE ? h.t_done : 0.0;
     return 0;
@@ -865,7 +929,7 @@ int lrh_engine_image(LrhEngine *e, int64_t id, float *d_out, int add)
     if (h.state != LRH_JOB_DONE) return lr_fail(LRH_EAGAIN, "lrh_engine_image: job %lld is not done", (long long)id);
     const int npix = h.desc.w * h.desc.h;
     hipLaunchKernelGGL(k_resolve, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, e->stream,
-                       (const long long *)arena_ptr(e, h.acc_off), npix, (float)h.desc.spp, (float4 *)d_out, add);
+                       (const long long *)arena_ptr(e, h.acc_off), npix, (float)h.spp_done, (float4 *)d_out, add);
     ENG_HIP(hipGetLastError());
     ENG_HIP(hipStreamSynchronize(e->stream));
     return 0;
@@ -906,7 +970,7 @@ int lrh_engine_denoise(LrhEngine *e, int64_t id, int passes, float *d_out)
         return lr_fail(LRH_ENOMEM, "lrh_engine_denoise: device memory for the filter could not be had");
     }
     hipLaunchKernelGGL(k_resolve, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, e->stream,
-                       (const long long *)arena_ptr(e, h.acc_off), npix, (float)h.desc.spp, (float4 *)tmp, 0);
+                       (const long long *)arena_ptr(e, h.acc_off), npix, (float)h.spp_done, (float4 *)tmp, 0);
     hipLaunchKernelGGL(k_moments, dim3((npix + SBLOCK - 1) / SBLOCK), dim3(SBLOCK), 0, e->stream,
                        (const long long *)arena_ptr(e, h.mom_off), npix, (float2 *)(tmp + fa));
     int rc = hipGetLastError() == hipSuccess ? 0 : lr_fail(LRH_EDEVICE, "lrh_engine_denoise: a launch failed");

@@ -108,10 +108,14 @@ __device__ __forceinline__ void ordered_append2(unsigned long long *stat, uint32
 // lane of the wave calls this (on: the lane's path ends).
 __device__ __forceinline__ void retire(const EngArgs &A, bool on, uint32_t jslot, uint32_t q, float4 r, bool glow = false)
 {
+    int pk = -1;          // a pilot job's pass of this sample (its first 4), else -1
+    long long pl = 0;     // ... and the sample's luminance, fixed point
     if (on) {
         const JobDev &J = A.jobs[jslot];
         V3 c = v3(r.x, r.y, r.z);
-        if (c.x == c.x && c.y == c.y && c.z == c.z) {
+        const bool sane = c.x == c.x && c.y == c.y && c.z == c.z;
+        if (J.pilot && q / (uint32_t)J.npix < LR_PILOT_PASSES) pk = (int)(q / (uint32_t)J.npix);
+        if (sane) {
             const float m = vmax3(c);
             if (J.clamp > 0.0f && m > J.clamp) c = vscale(c, J.clamp / m);
             const uint32_t pix = q % (uint32_t)J.npix;
@@ -120,6 +124,7 @@ __device__ __forceinline__ void retire(const EngArgs &A, bool on, uint32_t jslot
             if (fx) atomicAdd((unsigned long long *)&px[0], (unsigned long long)fx);
             if (fy) atomicAdd((unsigned long long *)&px[1], (unsigned long long)fy);
             if (fz) atomicAdd((unsigned long long *)&px[2], (unsigned long long)fz);
+            if (pk >= 0) pl = to_fixed(lum(c));
             const float4 ga = J.moments ? J.guide[2 * (size_t)pix + 1] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
             if (J.moments && !(glow && ga.w == 0.0f)) {
                 // the denoiser's moments: the luminance of the sample over the guide's albedo on a
@@ -140,6 +145,19 @@ __device__ __forceinline__ void retire(const EngArgs &A, bool on, uint32_t jslot
     if (m == 0) return;
     const int leader = __ffsll((unsigned long long)m) - 1;
     if ((int)lane_id() == leader) atomicAdd(&A.ctl->tail, (unsigned long long)__popcll(m));
+    // a pilot job's first passes: each pass's luminance summed over the wave, one atomic per
+    // (job, pass) present (added before the snapshot after the graph: complete when read)
+    uint64_t pp = __ballot(pk >= 0);
+    while (pp) {
+        const int l = __ffsll((unsigned long long)pp) - 1;
+        const uint32_t j = __shfl(jslot, l, 64);
+        const int k = __shfl(pk, l, 64);
+        const bool mine = pk == k && jslot == j;
+        pp &= ~__ballot(mine);
+        long long v = mine ? pl : 0;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if ((int)lane_id() == l) atomicAdd((unsigned long long *)&A.jpass[j][k], (unsigned long long)v);
+    }
     // the job's count of retired paths: one atomic per job present in the wave
     uint64_t pend = m;
     while (pend) {
