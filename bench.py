@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=200, help="serving: requests per arrival rate (0: skip)")
     ap.add_argument("--serve-rates", default="10,25,50,100,200,400",
                     help="serving: Poisson arrival rates, jobs/s (the workload's 50-400, and two below saturation)")
-    ap.add_argument("--tune", default=None, metavar="BT,STK,BS,BPC,TILE,IPG",
+    ap.add_argument("--tune", default=None, metavar="BT,STK,BS,BPC,TILE,IPG[,REFILL[,REFILL_SHADOW]]",
                     help="batched: this launch shape instead of the autotuner's")
     ap.add_argument("--skip-plain", action="store_true", help="batched: leave out the plain per-job comparison")
     ap.add_argument("--tlas", choices=["lbvh", "ploc", "refit"], default="ploc",
@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="batched: skip the extension kernel's HBM traffic (rocprofv3 --pmc child runs)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline: repeat for about this long")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 3 times (calibrates the PMC byte counters) and exit")
@@ -277,19 +279,68 @@ def build_pool(ctx, lr, plan):
     return pool
 
 
-def measured_traffic(tune):
-    """HBM bytes per launch of the extension kernel from the committed PMC summary
-    (profiles/batched_extend_traffic.txt: two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE,
-    calibrated on a device copy), when this run uses the same kernel variant; else None"""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "batched_extend_traffic.txt")
+def live_traffic(a, tune, limit_s=300):
+    """HBM bytes per launch of the batched extension kernel, measured for this run: this script
+    run again in child processes under rocprofv3 --pmc, once with FETCH_SIZE and once with
+    WRITE_SIZE (the same jobs and launch shape, one batch, the extension kernel's dispatches
+    only), each counter scaled by what it reads for a 512 MiB device copy (two more child runs,
+    --copy-probe). Returns (bytes per launch, detail) or (None, the reason)."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, {"why": "rocprofv3 not found"}
+    me = os.path.abspath(__file__)
+    t = tune.as_dict()
+    shape = ",".join(str(t[k]) for k in ("block_trace", "stack", "block_shade", "blocks_per_cu", "tile",
+                                          "iters_per_graph", "refill", "refill_shadow"))
+    kname = (f"k_extend_rf<{t['block_trace']}, {t['stack']}, false>" if t["refill"] > 0
+             else f"k_extend<{t['block_trace']}, {t['stack']}, false, false>")
+
+    def per_dispatch(counter, args, regex, match):
+        d = tempfile.mkdtemp(prefix="lr_pmc_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "--kernel-include-regex", regex, "--output-format", "csv", "-d", d, "-o", "run",
+               "--", sys.executable, "-u", me] + args
+        p = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
+                             stderr=subprocess.DEVNULL, start_new_session=True)
+        try:
+            p.wait(timeout=limit_s)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            shutil.rmtree(d, ignore_errors=True)
+            raise RuntimeError(f"{counter} run passed {limit_s} s")
+        vals = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for row in csv.DictReader(open(f)):
+                if match in row["Kernel_Name"] and row["Counter_Name"] == counter:
+                    vals.append(float(row["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        if not vals:
+            raise RuntimeError(f"no {counter} dispatches of {match} (rocprofv3 exit {p.returncode})")
+        return sum(vals) / len(vals), len(vals)
+
+    print("bench: HBM traffic of the extension kernel (rocprofv3 --pmc, 4 child runs)", file=sys.stderr, flush=True)
+    run_args = ["--tune", shape, "--jobs", str(a.jobs), "--serve-requests", "0", "--no-cpu", "--steps", "1",
+                "--warmup", "0", "--skip-plain", "--no-traffic"]
     try:
-        rec = json.load(open(path))["batched"]
-    except Exception:
-        return None
-    t = tune.as_dict() if hasattr(tune, "as_dict") else dict(tune)
-    if [t["block_trace"], t["stack"]] != rec["tune"]:
-        return None
-    return rec["bytes_per_launch"]
+        probe = 512 << 20
+        cf, _ = per_dispatch("FETCH_SIZE", ["--copy-probe", "512"], "copyBuffer", "copyBuffer")
+        cw, _ = per_dispatch("WRITE_SIZE", ["--copy-probe", "512"], "copyBuffer", "copyBuffer")
+        ff, wf = probe / (cf * 1024.0), probe / (cw * 1024.0)
+        fk, nf = per_dispatch("FETCH_SIZE", run_args, "k_extend", kname)
+        wk, nw = per_dispatch("WRITE_SIZE", run_args, "k_extend", kname)
+    except Exception as e:   # noqa: BLE001 - reported in the line, the bench goes on
+        return None, {"why": str(e)}
+    fb, wb = fk * 1024.0 * ff, wk * 1024.0 * wf
+    return int(fb + wb), {"kernel": kname, "fetch_bytes": int(fb), "write_bytes": int(wb),
+                          "fetch_factor": round(ff, 4), "write_factor": round(wf, 4), "dispatches": [nf, nw],
+                          "how": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in child runs of this bench (one batch), "
+                                 "each scaled by the counter's reading for a 512 MiB device copy"}
 
 
 def bench_batched(a, torch, lr):
@@ -321,7 +372,8 @@ def bench_batched(a, torch, lr):
     t0 = time.perf_counter()
     if a.tune:
         f = [int(x) for x in a.tune.split(",")]
-        eng.tune = dict(zip(["block_trace", "stack", "block_shade", "blocks_per_cu", "tile", "iters_per_graph"], f))
+        eng.tune = dict(zip(["block_trace", "stack", "block_shade", "blocks_per_cu", "tile", "iters_per_graph", "refill",
+                                 "refill_shadow"], f))
         tune, tune_mr = eng.tune, 0.0
     else:
         tune, tune_mr = eng.autotune(sample)
@@ -408,6 +460,7 @@ def bench_batched(a, torch, lr):
     alg = rf.walk_bytes(rext, nodes, shapes, rf.NODE_BYTES_WIDE)
     dom = max(kms, key=kms.get)
     achieved = alg / (kms["extend"] * 1e-3) / 1e9
+    traffic, traffic_detail = (None, {"why": "--no-traffic"}) if a.no_traffic else live_traffic(a, tune)
 
     out = {
         "metric": METRIC,
@@ -441,10 +494,12 @@ def bench_batched(a, torch, lr):
         "kernel_ms_total": {n: round(kms[n], 2) for n in names},
         "roofline": {"bound": "hbm", "kernel": "extend", "dominant": dom, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(tune),
+                     "traffic": traffic,
+                     "traffic_detail": traffic_detail,
                      "alg_bytes_per_launch": alg // max(kl["extend"], 1), "node_fetches": nodes, "shape_tests": shapes,
                      "rays": rext, "launches": kl["extend"]},
     }
+    out.update(walk_counts(c0, c1, kms, kl))
     if a.serve_requests > 0:
         rates = [float(x) for x in a.serve_rates.split(",") if x]
         # serving answers sooner with shorter graphs (a new job joins at the next graph)
@@ -662,6 +717,7 @@ def bench_instanced(a, torch, lr):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "alg_bytes_per_launch": alg // max(kl["extend"], 1), "node_fetches": nodes, "shape_tests": shapes,
                      "rays": rext},
+        **walk_counts(c0, c1, kms, kl),
         "cpu_baseline": {"value": None, "unit": "Mrays/s", "cores": None, "kind": "reference",
                          "sample": "not run for this workload: the reference's instanced CPU path is not built here"},
     }
@@ -669,6 +725,32 @@ def bench_instanced(a, torch, lr):
     for s_ in scenes:
         s_.close()
     return out
+
+
+def walk_counts(c0, c1, kms, kl):
+    """The counting run's figures for both walks: the shadow walk's roofline (48 B per ray: the
+    ray and its light's contribution; 64 B per wide node, 52 B per shape, like the extension
+    walk's) over its profiled time, and for each walk the share of a wave's lanes at work in its
+    node steps and its shape steps (node records / (64 x the waves' node steps), and so on)."""
+    d = lambda f: getattr(c1, f) - getattr(c0, f)
+    share = lambda a, b: round(a / (64.0 * b), 4) if b else None
+    srays, snodes, sshapes = d("rays_shadow"), d("shadow_node_fetches"), d("shadow_shape_tests")
+    salg = rf.walk_bytes(srays, snodes, sshapes, rf.NODE_BYTES_WIDE)
+    sach = salg / (kms["shadow"] * 1e-3) / 1e9 if kms["shadow"] > 0 else 0.0
+    return {
+        "shadow_roofline": {"bound": "hbm", "kernel": "shadow", "achieved": round(sach, 2), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(sach / HBM_PEAK_GBS, 4),
+                            "alg_bytes_per_launch": salg // max(kl["shadow"], 1), "node_fetches": snodes,
+                            "shape_tests": sshapes, "rays": srays},
+        "lane_shares": {"extend_node_steps": share(d("node_fetches"), d("wave_node_steps")),
+                        "extend_shape_steps": share(d("shape_tests"), d("wave_shape_steps")),
+                        "shadow_node_steps": share(snodes, d("shadow_wave_node_steps")),
+                        "shadow_shape_steps": share(sshapes, d("shadow_wave_shape_steps")),
+                        "extend_uniform_node_steps": round(d("uniform_node_steps") / max(d("wave_node_steps"), 1), 4),
+                        "shadow_uniform_node_steps": round(d("shadow_uniform_node_steps") / max(d("shadow_wave_node_steps"), 1), 4),
+                        "extend_node_steps_per_ray": round(d("wave_node_steps") * 64.0 / max(d("rays_ext"), 1), 2),
+                        "extend_nodes_per_ray": round(d("node_fetches") / max(d("rays_ext"), 1), 2)},
+    }
 
 
 def cpu_baseline_batched(lr, eng, plan, jobs, seconds):

@@ -261,6 +261,63 @@ int lrh_denoise(LrhContext *c, const float *d_accum, const float *d_moments, con
  * the shading code, as the reference's gpu_bsdf_run (tests/test_gpu_brdf.c:21-90). Waits. */
 int lrh_bsdf_cases(LrhContext *c, const float *d_in, int n, float *d_out);
 
+/* ---- rigid bodies: the step that moves an instanced scene's dynamic instances ----
+ * Spheres and boxes in a room of six walls, stepped on the device by the reference's CPU
+ * solver's rules (engine/phys/rigid.h, rigid.c: contacts found once per step with speculative
+ * margins, substeps of gravity, warm-started sequential impulses with Coulomb friction, a bounce
+ * once from the approach speed, the relax pass). Units are the scene's (gravity 981 down: cm).
+ * Sleeping and the swept test of fast bodies (rigid_ccd) are not done. One world holds at most
+ * 4096 bodies. */
+#define LRH_BODY_SPHERE 0
+#define LRH_BODY_BOX 1
+typedef struct LrhRigid LrhRigid;
+/* a body (Body, rigid.h:46-64): a sphere's radius in half[0]; rot: a unit quaternion x, y, z, w
+ * (body frame -> world); inv_mass 0: fixed, nothing moves it */
+typedef struct {
+    int shape;
+    float half[3];
+    float pos[3];
+    float rot[4];
+    float vel[3];
+    float angvel[3];        /* world, radians / s */
+    float inv_mass;
+    float inv_inertia[3];   /* the body's own frame, diagonal */
+    float restitution;      /* 0 no bounce .. 1 elastic */
+    float friction;         /* Coulomb coefficient */
+} LrhBody;
+/* the world's settings (RigidWorld, rigid.h:80-101; rigid_init's defaults at creation: gravity
+ * (0, -981, 0), no room, walls 0.4 / 0.5, 10 iterations, damping 0 / 0.1, warm start on) */
+typedef struct {
+    float gravity[3];
+    int has_room;
+    float room_lo[3], room_hi[3];
+    float wall_restitution, wall_friction;
+    int iterations;
+    float linear_damping, angular_damping;   /* fraction of velocity lost per second */
+    int warm_start;
+} LrhRigidParams;
+typedef struct {
+    uint64_t steps;
+    int contacts, pairs, colours;   /* the last step's */
+    float ms;                       /* device time per step of the last lrh_rigid_steps */
+} LrhRigidStats;
+/* rigid_init (rigid.c:122-132). max_contacts bounds a step's contacts and its listed pairs
+ * (LRH_EREFUSED past them). Own device memory; steps on the context's stream. */
+int lrh_rigid_create(LrhContext *c, int max_bodies, int max_contacts, LrhRigid **out);
+void lrh_rigid_destroy(LrhRigid *r);
+int lrh_rigid_params_get(const LrhRigid *r, LrhRigidParams *p);
+int lrh_rigid_params_set(LrhRigid *r, const LrhRigidParams *p);
+/* rigid_add_sphere / rigid_add_box (rigid.c:177-201): density in g / cm^3 (0: fixed); the
+ * body's index, or < 0 */
+int lrh_rigid_add_sphere(LrhRigid *r, const float *center, float radius, float density);
+int lrh_rigid_add_box(LrhRigid *r, const float *center, const float *half, const float *rot, float density);
+/* bodies [first, first + n) read back / replaced (velocities pushed, bodies fixed ...) */
+int lrh_rigid_get(LrhRigid *r, int first, int n, LrhBody *out);
+int lrh_rigid_set(LrhRigid *r, int first, int n, const LrhBody *in);
+/* nsteps times rigid_steps(w, dt, substeps, NULL) (rigid.c:1371); waits for them */
+int lrh_rigid_steps(LrhRigid *r, float dt, int substeps, int nsteps);
+int lrh_rigid_stats(const LrhRigid *r, LrhRigidStats *st);
+
 /* ---- the engine: many render jobs on one device ----
  * The role of the reference's GpuView trace loop (gpu_view_trace_start/more/passes/read,
  * engine/gpu/view_gpu.h:71-124, and gpu_frame_trace, engine/app/gpu_frame.h:14-26) for any
@@ -295,9 +352,12 @@ typedef struct {
  * kernels (extend, shadow) and of the others (regen, shade), each 64, 128 or 256; the walk's
  * LDS stack (16, 24 or 32 entries; deeper entries spill to device memory); persistent
  * work-groups per compute unit (1..16); the queue tile a work-group claims at a time, in
- * work-group widths (1, 2, 4 or 8); wavefront iterations per captured graph (1..64). */
+ * work-group widths (1, 2, 4 or 8); wavefront iterations per captured graph (1..64); refill:
+ * 0, or 1..64: the batched extension walk (single-level scenes) hands new rays to a wave's
+ * finished lanes once at least that many of its 64 have finished, instead of one ray a lane
+ * per tile; refill_shadow: the same for the shadow walk (single-level scenes, one light). */
 typedef struct {
-    int block_trace, stack, block_shade, blocks_per_cu, tile, iters_per_graph;
+    int block_trace, stack, block_shade, blocks_per_cu, tile, iters_per_graph, refill, refill_shadow;
 } LrhTune;
 
 #define LRH_JOB_RUNNING 1
@@ -324,6 +384,20 @@ typedef struct {
     uint64_t shape_tests;   /* and shapes tested */
     double kernel_ms[4];    /* with profiling on: device time of regen, extend, shade, shadow */
     uint64_t kernel_launches[4];
+    /* with counting on: the extension walks' wave steps (node steps, shape steps; a step is one
+     * wave's pass through that part of the walk, whatever lanes take part), so that
+     * node_fetches / (64 wave_node_steps) is the share of lanes at work in a node step */
+    uint64_t wave_node_steps;
+    uint64_t wave_shape_steps;
+    /* and the shadow walks' node records, shapes and wave steps */
+    uint64_t shadow_node_fetches;
+    uint64_t shadow_shape_tests;
+    uint64_t shadow_wave_node_steps;
+    uint64_t shadow_wave_shape_steps;
+    /* and the node steps in which every lane at work is at one node of one tree (single-level
+     * walks), for the extension and the shadow walks */
+    uint64_t uniform_node_steps;
+    uint64_t shadow_uniform_node_steps;
 } LrhEngineStats;
 
 /* host clock (steady, ms) that job times are given in (clock_ms, engine/base/clock.h:6) */
@@ -335,7 +409,8 @@ int lrh_engine_create(LrhContext *c, int slots_log2, int max_tree_depth, size_t 
 void lrh_engine_destroy(LrhEngine *e);
 int lrh_engine_tune_set(LrhEngine *e, const LrhTune *t);
 int lrh_engine_tune_get(const LrhEngine *e, LrhTune *t);
-/* counting on: the extension walks also count node records and shapes (a slower kernel) */
+/* counting on: the extension and shadow walks also count node records, shapes and their waves'
+ * steps (slower kernels) */
 int lrh_engine_counting(LrhEngine *e, int on);
 /* profiling on: lrh_engine_pump launches the iterations kernel by kernel between hipEvents on
  * the engine's stream (no graph) and sums each kernel's time into the stats */

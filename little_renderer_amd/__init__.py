@@ -40,7 +40,8 @@ class BuildStats(ctypes.Structure):
 class Tune(ctypes.Structure):
     """LrhTune: the engine's launch shape"""
     _fields_ = [("block_trace", ctypes.c_int), ("stack", ctypes.c_int), ("block_shade", ctypes.c_int),
-                ("blocks_per_cu", ctypes.c_int), ("tile", ctypes.c_int), ("iters_per_graph", ctypes.c_int)]
+                ("blocks_per_cu", ctypes.c_int), ("tile", ctypes.c_int), ("iters_per_graph", ctypes.c_int),
+                ("refill", ctypes.c_int), ("refill_shadow", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -63,7 +64,34 @@ class JobInfo(ctypes.Structure):
 class EngineStats(ctypes.Structure):
     _fields_ = [("rays_ext", ctypes.c_uint64), ("rays_shadow", ctypes.c_uint64), ("iterations", ctypes.c_uint64),
                 ("jobs_done", ctypes.c_uint64), ("node_fetches", ctypes.c_uint64), ("shape_tests", ctypes.c_uint64),
-                ("kernel_ms", ctypes.c_double * 4), ("kernel_launches", ctypes.c_uint64 * 4)]
+                ("kernel_ms", ctypes.c_double * 4), ("kernel_launches", ctypes.c_uint64 * 4),
+                ("wave_node_steps", ctypes.c_uint64), ("wave_shape_steps", ctypes.c_uint64),
+                ("shadow_node_fetches", ctypes.c_uint64), ("shadow_shape_tests", ctypes.c_uint64),
+                ("shadow_wave_node_steps", ctypes.c_uint64), ("shadow_wave_shape_steps", ctypes.c_uint64),
+                ("uniform_node_steps", ctypes.c_uint64), ("shadow_uniform_node_steps", ctypes.c_uint64)]
+
+
+class Body(ctypes.Structure):
+    """LrhBody: a rigid body (a sphere's radius in half[0]; rot x, y, z, w)"""
+    _fields_ = [("shape", ctypes.c_int), ("half", ctypes.c_float * 3), ("pos", ctypes.c_float * 3),
+                ("rot", ctypes.c_float * 4), ("vel", ctypes.c_float * 3), ("angvel", ctypes.c_float * 3),
+                ("inv_mass", ctypes.c_float), ("inv_inertia", ctypes.c_float * 3), ("restitution", ctypes.c_float),
+                ("friction", ctypes.c_float)]
+
+
+class RigidParams(ctypes.Structure):
+    _fields_ = [("gravity", ctypes.c_float * 3), ("has_room", ctypes.c_int), ("room_lo", ctypes.c_float * 3),
+                ("room_hi", ctypes.c_float * 3), ("wall_restitution", ctypes.c_float), ("wall_friction", ctypes.c_float),
+                ("iterations", ctypes.c_int), ("linear_damping", ctypes.c_float), ("angular_damping", ctypes.c_float),
+                ("warm_start", ctypes.c_int)]
+
+
+class RigidStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("contacts", ctypes.c_int), ("pairs", ctypes.c_int),
+                ("colours", ctypes.c_int), ("ms", ctypes.c_float)]
+
+
+BODY_SPHERE, BODY_BOX = 0, 1
 
 
 class ServerStats(ctypes.Structure):
@@ -161,6 +189,16 @@ def lib():
         "lrh_server_release": (I, [P, ctypes.c_int64]),
         "lrh_server_stats": (I, [P, ctypes.POINTER(ServerStats)]),
         "lrh_server_destroy": (None, [P]),
+        "lrh_rigid_create": (I, [P, I, I, ctypes.POINTER(P)]),
+        "lrh_rigid_destroy": (None, [P]),
+        "lrh_rigid_params_get": (I, [P, ctypes.POINTER(RigidParams)]),
+        "lrh_rigid_params_set": (I, [P, ctypes.POINTER(RigidParams)]),
+        "lrh_rigid_add_sphere": (I, [P, P, F, F]),
+        "lrh_rigid_add_box": (I, [P, P, P, P, F]),
+        "lrh_rigid_get": (I, [P, I, I, P]),
+        "lrh_rigid_set": (I, [P, I, I, P]),
+        "lrh_rigid_steps": (I, [P, F, I, I]),
+        "lrh_rigid_stats": (I, [P, ctypes.POINTER(RigidStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -359,6 +397,85 @@ class PtScene:
                                    _ptr(accum)))
 
 
+class Rigid:
+    """Rigid bodies stepped on the device (lrh_rigid_*): spheres and boxes in a room."""
+
+    def __init__(self, ctx, max_bodies=4096, max_contacts=65536):
+        self.ctx = ctx
+        self._r = ctypes.c_void_p()
+        _check(lib().lrh_rigid_create(ctx._c, max_bodies, max_contacts, ctypes.byref(self._r)))
+        self._n = 0
+        ctx._adopt(self)
+
+    def close(self):
+        if self._r:
+            lib().lrh_rigid_destroy(self._r)
+            self._r = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        p = RigidParams()
+        _check(lib().lrh_rigid_params_get(self._r, ctypes.byref(p)))
+        return p
+
+    def set_params(self, **kw):
+        """the world's settings changed by name (gravity=(0, -981, 0), room=(lo, hi), ...)"""
+        p = self.params
+        for k, v in kw.items():
+            if k == "room":
+                p.has_room = 1
+                p.room_lo[:], p.room_hi[:] = list(v[0]), list(v[1])
+            elif k == "gravity":
+                p.gravity[:] = list(v)
+            else:
+                setattr(p, k, v)
+        _check(lib().lrh_rigid_params_set(self._r, ctypes.byref(p)))
+
+    def add_sphere(self, c, radius, density=1.0):
+        i = lib().lrh_rigid_add_sphere(self._r, (ctypes.c_float * 3)(*c), radius, density)
+        if i < 0:
+            _check(i)
+        self._n += 1
+        return i
+
+    def add_box(self, c, half, rot=(0.0, 0.0, 0.0, 1.0), density=1.0):
+        i = lib().lrh_rigid_add_box(self._r, (ctypes.c_float * 3)(*c), (ctypes.c_float * 3)(*half),
+                                    (ctypes.c_float * 4)(*rot), density)
+        if i < 0:
+            _check(i)
+        self._n += 1
+        return i
+
+    def bodies(self, first=0, n=None):
+        if n is None:
+            n = self.count - first
+        out = (Body * n)()
+        _check(lib().lrh_rigid_get(self._r, first, n, out))
+        return out
+
+    def set_bodies(self, first, bodies):
+        arr = (Body * len(bodies))(*bodies)
+        _check(lib().lrh_rigid_set(self._r, first, len(bodies), arr))
+
+    def steps(self, dt, substeps=4, n=1):
+        _check(lib().lrh_rigid_steps(self._r, dt, substeps, n))
+
+    def stats(self):
+        st = RigidStats()
+        _check(lib().lrh_rigid_stats(self._r, ctypes.byref(st)))
+        return st
+
+    @property
+    def count(self):
+        return self._n
+
+
 class Context:
     """An lrh context on one device (its own stream unless one is given)."""
 
@@ -514,7 +631,7 @@ class Engine:
     @tune.setter
     def tune(self, t):
         if isinstance(t, dict):
-            x = Tune()
+            x = self.tune   # the keys not given keep their values
             for k, v in t.items():
                 setattr(x, k, v)
             t = x

@@ -138,8 +138,8 @@ __global__ __launch_bounds__(BLOCK) void k_bounds(const float4 *__restrict__ box
     block_min6(v, s);
     if (threadIdx.x == 0) {
         for (int k = 0; k < 6; k++) st_dev((uint32_t *)&partial[6 * blockIdx.x + k], __float_as_uint(v[k]));
-        wait_mem();   // the partials are out (device-scope stores) before the ticket
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+        // the ticket releases this block's partials and, for the last block, acquires the others'
+        last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
     }
     __syncthreads();
     if (!last) return;
@@ -237,9 +237,15 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_morton(const float4 *__restrict_
         const uint32_t v = (&h[0][0])[i];
         if (v) atomicAdd(&mine[i], v);
     }
-    wait_mem();   // this thread's histogram atomics are done before the ticket
+    // The ticket hands the histograms over. They move as device-scope atomics (coherent across
+    // the XCDs once done), so the ticket needs every thread's adds done before it (wait_mem:
+    // the wait, and compiler barriers on both sides; then the work-group barrier), and the last
+    // work-group reads them with device-scope loads after it. An acquire-release ticket instead
+    // writes back and invalidates the L2 once per work-group (about 3900 of them at 16M keys):
+    // measured 1.68 -> 1.89 ms for a 16M build together with the bounds kernel's.
+    wait_mem();
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    if (threadIdx.x == 0) last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
     __syncthreads();
     if (!last) return;
     // the last block: thread t sums digit t % 256 of pass t / 256 over the copies (all the
@@ -600,7 +606,14 @@ __global__ __launch_bounds__(BLOCK) void k_lbvh_top(const uint32_t *__restrict__
         for (;;) {
             const int p = left ? r : l - 1;             // the parent's split
             const int end = left ? l : r;
-            wait_mem();                                 // this subtree's record is out before the arrival
+            // The arrival hands this subtree's record over. The records move as device-scope
+            // accesses (sc1 stores and loads, coherent across the XCDs once done), so the
+            // exchange needs only this thread's stores done before it (wait_mem: the wait, and
+            // compiler barriers on both sides), and the second to arrive reads the first's record
+            // at an address taken from the exchanged value, after it. An acquire-release exchange
+            // instead writes back and invalidates the L2 at every climb: measured 1.68 -> 6.9 ms
+            // for a 16M build.
+            wait_mem();
             const unsigned long long mine = ((unsigned long long)h << 32) | ((uint32_t)end + 1u);
             const unsigned long long o = __hip_atomic_exchange(&other[p], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (o == 0) break;                          // the sibling is not done: it goes on

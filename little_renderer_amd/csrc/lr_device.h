@@ -11,6 +11,19 @@
 #define LR_LEAF 0x80000000u
 #define LR_NONE 0xffffffffu
 
+// loads through the global address space, so that they are global loads, not flat ones (which
+// loads through the walks' pointers, read from job records, would otherwise be)
+typedef float lr_f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ldg4(const float4 *p, size_t i)
+{
+    const lr_f4v v = ((const __attribute__((address_space(1))) lr_f4v *)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t ldg(const uint32_t *p, size_t i)
+{
+    return ((const __attribute__((address_space(1))) uint32_t *)p)[i];
+}
+
 // one BVH node as the reference lays it out (trace.wgsl:2): 32 bytes
 struct alignas(16) LrNode {
     float lox, loy, loz;

@@ -155,6 +155,8 @@ struct EngArgs {
     unsigned long long *counts;             // node records and shapes walked (counting extend only)
     unsigned long long *ostat;              // shade's tiles: look-back status of its ordered appends
     int mult;                               // a tile is mult work-groups' widths of entries
+    int refill;                             // k_extend_rf: idle lanes of a wave that take new rays
+    int refill_shadow;                      // k_shadow_rf: the same
 };
 
 typedef void (*LrEngFn)(dim3 grid, hipStream_t s, const EngArgs &A);
@@ -169,7 +171,10 @@ typedef void (*LrGuideFn)(dim3 grid, hipStream_t s, const JobDev &J, float4 *gui
 // in fewer registers)
 struct LrEngTable {
     LrEngFn regen[3], shade[3][2];
-    LrEngFn extend[3][3][2], extend_count[3][3][2], shadow[3][3][2][2];
+    LrEngFn extend[3][3][2], extend_count[3][3][2], shadow[3][3][2][2], shadow_count[3][3][2][2];
+    // [block][stack][counting]: the walks with lanes refilled (single-level scenes; the shadow
+    // walk's with one shadow ray a vertex)
+    LrEngFn extend_rf[3][3][2], shadow_rf[3][3][2];
     LrGuideFn guide[2];   // by whether it takes two-level scenes
 };
 void lr_eng_fill_b64(LrEngTable &t);

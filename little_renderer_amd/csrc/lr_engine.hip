@@ -173,7 +173,7 @@ struct LrhEngine {
     unsigned long long issue_end = 0, iq_hi = 0;
     int running = 0;
     uint64_t jobs_done = 0;
-    LrhTune tune{256, 16, 256, 8, 1, 8};   // the best measured on the batched workload (MI355X)
+    LrhTune tune{256, 16, 256, 8, 1, 8, 0, 0};   // the best measured on the batched workload (MI355X)
     bool counting = false;
     // profiling: iterations launched one kernel at a time between events (no graph)
     bool profiling = false;
@@ -206,16 +206,19 @@ bool tune_valid(const LrhTune &t)
 {
     return lr_blk_index(t.block_trace) >= 0 && lr_blk_index(t.block_shade) >= 0 && lr_stk_index(t.stack) >= 0 &&
            t.blocks_per_cu >= 1 && t.blocks_per_cu <= 16 && (t.tile == 1 || t.tile == 2 || t.tile == 4 || t.tile == 8) &&
-           t.iters_per_graph >= 1 && t.iters_per_graph <= 64;
+           t.iters_per_graph >= 1 && t.iters_per_graph <= 64 && t.refill >= 0 && t.refill <= 64 && t.refill_shadow >= 0 &&
+           t.refill_shadow <= 64;
 }
 
 // the kernels' arguments for a launch whose walk kernels run `threads` threads
-EngArgs args_for(const LrhEngine *e, int stack, size_t threads, int mult)
+EngArgs args_for(const LrhEngine *e, int stack, size_t threads, int mult, int refill = 0, int refill_shadow = 0)
 {
     EngArgs A = e->base;
     A.spill_per = e->max_tree_depth + 2 - stack > 0 ? e->max_tree_depth + 2 - stack : 0;
     A.spill_stride = threads;
     A.mult = mult;
+    A.refill = refill;
+    A.refill_shadow = refill_shadow;
     return A;
 }
 
@@ -225,18 +228,20 @@ void launch_iteration(LrhEngine *e, const LrhTune &t, hipEvent_t *ev = nullptr)
     const LrEngTable &T = table();
     const int bt = lr_blk_index(t.block_trace), bs = lr_blk_index(t.block_shade), sk = lr_stk_index(t.stack);
     const dim3 gt(e->ncu * t.blocks_per_cu), gs(e->ncu * t.blocks_per_cu);
-    const EngArgs At = args_for(e, t.stack, (size_t)gt.x * t.block_trace, t.tile);
+    const EngArgs At = args_for(e, t.stack, (size_t)gt.x * t.block_trace, t.tile, t.refill, t.refill_shadow);
     const EngArgs As = args_for(e, t.stack, 0, t.tile);
     if (ev) (void)hipEventRecord(ev[0], e->stream);
     T.regen[bs](gs, e->stream, As);
     if (ev) (void)hipEventRecord(ev[1], e->stream);
     const int tl = e->two_level ? 1 : 0;
-    (e->counting ? T.extend_count : T.extend)[bt][sk][tl](gt, e->stream, At);
+    if (!tl && t.refill > 0) T.extend_rf[bt][sk][e->counting ? 1 : 0](gt, e->stream, At);
+    else (e->counting ? T.extend_count : T.extend)[bt][sk][tl](gt, e->stream, At);
     if (ev) (void)hipEventRecord(ev[2], e->stream);
     const int ml = e->multi ? 1 : 0;
     T.shade[bs][ml](gs, e->stream, As);
     if (ev) (void)hipEventRecord(ev[3], e->stream);
-    T.shadow[bt][sk][tl][ml](gt, e->stream, At);
+    if (!tl && !ml && t.refill_shadow > 0) T.shadow_rf[bt][sk][e->counting ? 1 : 0](gt, e->stream, At);
+    else (e->counting ? T.shadow_count : T.shadow)[bt][sk][tl][ml](gt, e->stream, At);
     if (ev) (void)hipEventRecord(ev[4], e->stream);
 }
 
@@ -888,7 +893,7 @@ int lrh_engine_plain(LrhEngine *e, const LrhJob *job, int64_t *id)
         for (int d = 0; d <= job->max_depth; d++) {
             (e->counting ? T.extend_count : T.extend)[bt][sk][tl](gt, e->stream, At);
             T.shade[bs][ml](gs, e->stream, As);
-            T.shadow[bt][sk][tl][ml](gt, e->stream, At);
+            (e->counting ? T.shadow_count : T.shadow)[bt][sk][tl][ml](gt, e->stream, At);
         }
     }
     ENG_HIP(hipGetLastError());
@@ -1000,8 +1005,8 @@ int lrh_engine_stats(LrhEngine *e, LrhEngineStats *st)
     if (!e || !st) return lr_fail(LRH_EINVAL, "lrh_engine_stats: bad arguments");
     int rc = progress_flush(e);
     if (rc) return rc;
-    unsigned long long cnt[2] = {0, 0};
-    ENG_HIP(hipMemcpyAsync(cnt, e->counts, 16, hipMemcpyDeviceToHost, e->stream));
+    unsigned long long cnt[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    ENG_HIP(hipMemcpyAsync(cnt, e->counts, sizeof cnt, hipMemcpyDeviceToHost, e->stream));
     ENG_HIP(hipStreamSynchronize(e->stream));
     st->rays_ext = e->last.ctl.rays_ext;
     st->rays_shadow = e->last.ctl.rays_shadow;
@@ -1009,6 +1014,14 @@ int lrh_engine_stats(LrhEngine *e, LrhEngineStats *st)
     st->jobs_done = e->jobs_done;
     st->node_fetches = cnt[0];
     st->shape_tests = cnt[1];
+    st->wave_node_steps = cnt[2];
+    st->wave_shape_steps = cnt[3];
+    st->shadow_node_fetches = cnt[4];
+    st->shadow_shape_tests = cnt[5];
+    st->shadow_wave_node_steps = cnt[6];
+    st->shadow_wave_shape_steps = cnt[7];
+    st->uniform_node_steps = cnt[8];
+    st->shadow_uniform_node_steps = cnt[9];
     for (int k = 0; k < 4; k++) {
         st->kernel_ms[k] = e->kms[k];
         st->kernel_launches[k] = e->klaunch[k];
@@ -1101,6 +1114,16 @@ int lrh_engine_autotune(LrhEngine *e, const LrhJob *sample, int n, LrhTune *best
         t.block_shade = b;
         consider(t);
     }
+    for (int v : {0, 8, 16, 32}) {
+        LrhTune t = win;
+        t.refill = v;
+        consider(t);
+    }
+    for (int v : {0, 8, 16, 32}) {
+        LrhTune t = win;
+        t.refill_shadow = v;
+        consider(t);
+    }
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     if (rc) return rc;
@@ -1131,7 +1154,7 @@ int lrh_pt_render(LrhContext *c, const LrhPtScene *s, const float *cam, int W, i
     }
     // the caller's stream may still be writing d_accum or the scene
     LR_HIP(hipStreamSynchronize(c->stream));
-    LrhJob j;
+    LrhJob j{};   // (no denoiser inputs, no target error)
     j.scene = s;
     memcpy(j.cam, cam, sizeof(j.cam));
     j.w = W;

@@ -234,7 +234,7 @@ __device__ __forceinline__ V3 unit_box_normal(V3 p)
 // mesh's triangles are read in leaf order (ltris) beside their aux entries.
 template <bool ANY, bool COUNT, int STK>
 __device__ WalkHit walk4_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax, const WalkStack &S, uint32_t *overflow,
-                            uint32_t &cn, uint32_t &ct)
+                            WalkCnt &c)
 {
     // the ray in the space being walked (the world's, or an instance's); the world's inverse
     // direction is computed again on leaving an instance rather than held
@@ -252,7 +252,7 @@ __device__ WalkHit walk4_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax,
     auto pop = [&](uint32_t &x) -> bool {
         if (sp == 0) return false;
         sp--;
-        x = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+        x = stk_get<STK>(S, sp);
         return true;
     };
     // the entry being visited: a top-level node k, INST | i, BLAS | mesh node, MLEAF | (count -
@@ -274,7 +274,7 @@ __device__ WalkHit walk4_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax,
             const uint32_t i = e & 0x3fffffffu;
             const uint32_t kind = __float_as_uint(J.inst[4 * (size_t)i + 3].x);
             if (kind >= LR_UNIT_SPHERE) {   // a unit sphere or box: tested in its space, the ray kept
-                if (COUNT) ct++;
+                if (COUNT) count_shape(c);
                 V3 oo, od;
                 to_object(J.inst + 4 * (size_t)i, o, d, oo, od);
                 const float th = kind == LR_UNIT_SPHERE ? hit_unit_sphere(oo, od, tmin, best.t) : hit_unit_box(oo, od, tmin, best.t);
@@ -300,7 +300,7 @@ __device__ WalkHit walk4_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax,
         if (tag == MLEAF) {   // a mesh's leaf, in the instance's space
             const uint32_t cnt = ((e >> 24) & 0x3fu) + 1u, first = e & 0xffffffu;
             for (uint32_t j = 0; j < cnt; j++) {
-                if (COUNT) ct++;
+                if (COUNT) count_shape(c);
                 const uint32_t x = first + j;
                 const uint32_t sid = J.maux[x];
                 const float4 ta = J.ltris[3 * (size_t)x], tb = J.ltris[3 * (size_t)x + 1], tc = J.ltris[3 * (size_t)x + 2];
@@ -314,7 +314,7 @@ __device__ WalkHit walk4_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax,
             continue;
         }
         const bool blas = tag == LR_WALK_BLAS;
-        if (COUNT) cn++;
+        if (COUNT) count_node(c);
         float t[4];
         uint32_t ref[4];
         wide_test(blas ? J.mnodes4 : J.nodes, e & 0x3fffffffu, oi, inv, tmin, best.t, t, ref);
@@ -437,6 +437,18 @@ __global__ __launch_bounds__(BLK) void k_regen(EngArgs A)
     }
 }
 
+// a counting walk's counters summed over the kernel: a wave's lanes' sums, then one add each
+// (c: node records, shapes, the wave's node steps, its shape steps; u: its uniform node steps)
+__device__ __forceinline__ void add_counts(unsigned long long *c, unsigned long long *u, const WalkCnt &w)
+{
+    unsigned long long v[5] = {w.n, w.t, w.wn, w.wt, w.wu};
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
+        if (lane_id() == 0 && v[k]) atomicAdd(k < 4 ? &c[k] : u, v[k]);
+    }
+}
+
 // extend (wf_ext, trace_wave.wgsl:72-84): the closest hit of every queued ray, each through
 // its own job's tree. Also applies regen's claim to the ring and the issue counter, and clears
 // the queues the shade kernel appends to. TL: two-level scenes may be queued (without it, such
@@ -472,27 +484,268 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_extend(E
     // the shade kernel's look-back status words, one per tile of at least 64 entries
     for (size_t k = gtid; k < (size_t)n / 64 + 2; k += (size_t)gridDim.x * BLK) A.ostat[k] = 0ull;
     const WalkStack S{&stk[0][threadIdx.x], BLK, A.spill + gtid, (size_t)A.spill_stride, STK + A.spill_per};
-    uint32_t cn = 0, ct = 0;
+    WalkCnt wc;
     for_tiles<BLK>(&ctl->tiles[K_EXTEND], n, A.mult, [&](uint32_t i, bool live) {
         if (!live) return;
         const uint32_t id = A.qid[cur][i];
         const JobDev &J = A.jobs[id >> LR_SLOT_BITS];
         const float4 r0 = A.qray[cur][2 * (size_t)i], r1 = A.qray[cur][2 * (size_t)i + 1];
         WalkHit h;
-        if (TL && J.two_level) h = walk4_tl<false, COUNT, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+        if (TL && J.two_level) h = walk4_tl<false, COUNT, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, wc);
         else if (!J.two_level)
             h = walk4<false, COUNT, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w, r1.w, S,
-                                         A.overflow, cn, ct);
+                                         A.overflow, wc);
         else {
             atomicOr(A.overflow, 2u);
             h = WalkHit{r1.w, LR_NONE, 0.0f, 0.0f};
         }
         A.hits[i] = make_float4(h.t, __uint_as_float(h.id), h.b1, h.b2);
     });
-    if (COUNT) {
-        atomicAdd(&A.counts[0], (unsigned long long)cn);
-        atomicAdd(&A.counts[1], (unsigned long long)ct);
+    if (COUNT) add_counts(A.counts, A.counts + 8, wc);
+}
+
+// ---- walks with lanes refilled (dynamic fetch, Aila & Laine 2009) ----------------------------
+// A wave walks its lanes' rays one round at a time (the leaves a lane holds, then one node step:
+// walk4's loop body), and once at least A.refill of its 64 lanes have finished, those take the
+// next entries of the queue: a wave does not run on for its slowest ray with most of its lanes
+// idle. A wave claims the queue 64 entries at a time (the kernel's counter then counts entries,
+// not tiles) and hands them to its idle lanes in order. The walk is walk4's (the same visits in
+// the same order, the same hit); single-level scenes only. Every lane of a wave reaches the top
+// of the round loop together, where the hand-out's ballots and the claim's broadcast are made.
+
+// a lane's walk between rounds
+struct WalkState {
+    V3 o, d, inv;   // (o * inv, the slab test's other term, made again each node step: fewer registers held)
+    float tmin;
+    WalkHit best;
+    int sp;
+    uint32_t cur;
+    const float4 *nodes, *ltris, *prims;
+    const uint32_t *aux;
+    uint32_t nprims;
+};
+
+__device__ __forceinline__ void walk_start(WalkState &w, const JobDev &J, float4 r0, float4 r1)
+{
+    w.o = v3of(r0);
+    w.d = v3of(r1);
+    w.inv = v3(safe_inv(w.d.x), safe_inv(w.d.y), safe_inv(w.d.z));
+    w.tmin = r0.w;
+    w.best = WalkHit{r1.w, LR_NONE, 0.0f, 0.0f};
+    w.sp = 0;
+    w.cur = 0u;
+    w.nodes = J.nodes;
+    w.aux = J.aux;
+    w.ltris = J.ltris;
+    w.prims = J.prims;
+    w.nprims = J.nprims;
+}
+
+// one round of walk4 (ANY: the first hit ends it); true when the walk has ended (w.best its hit)
+template <bool ANY, bool COUNT, int STK>
+__device__ __forceinline__ bool walk_round(WalkState &w, const WalkStack &S, uint32_t *overflow, WalkCnt &wc)
+{
+    while (w.cur & LR_LEAF) {
+        bool done = false;
+        leaf_hits_(w.aux, w.ltris, w.prims, w.nprims, w.cur & 0xffffffu, ((w.cur >> 24) & 0x3fu) + 1u, w.o, w.d, w.tmin, w.best,
+                   ANY, done, wc, COUNT);
+        if ((ANY && done) || w.sp == 0) return true;
+        w.sp--;
+        w.cur = stk_get<STK>(S, w.sp);
     }
+    if (COUNT) {
+        count_node(wc);
+        count_uniform(wc, w.nodes, w.cur);
+    }
+    float t[4];
+    uint32_t ref[4];
+    wide_test(w.nodes, w.cur, v3(w.o.x * w.inv.x, w.o.y * w.inv.y, w.o.z * w.inv.z), w.inv, w.tmin, w.best.t, t, ref);
+    float s0 = t[0] >= 0.0f ? t[0] : 3e38f, s1 = t[1] >= 0.0f ? t[1] : 3e38f;
+    float s2 = t[2] >= 0.0f ? t[2] : 3e38f, s3 = t[3] >= 0.0f ? t[3] : 3e38f;
+    uint32_t c0 = ref[0], c1 = ref[1], c2 = ref[2], c3 = ref[3];
+    cswap_(s0, c0, s1, c1);
+    cswap_(s2, c2, s3, c3);
+    cswap_(s0, c0, s2, c2);
+    cswap_(s1, c1, s3, c3);
+    cswap_(s1, c1, s2, c2);
+    const int m = (s0 < 3e38f) + (s1 < 3e38f) + (s2 < 3e38f) + (s3 < 3e38f);
+    if (m == 0) {
+        if (w.sp == 0) return true;
+        w.sp--;
+        w.cur = stk_get<STK>(S, w.sp);
+        return false;
+    }
+    if (w.sp + m - 1 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped
+        atomicOr(overflow, 1u);
+        return true;
+    }
+    auto push = [&](uint32_t x) {
+        if (w.sp < STK) S.stk[w.sp * S.stride] = x;
+        else S.spill[(size_t)(w.sp - STK) * S.sstride] = x;
+        w.sp++;
+    };
+    if (m > 3) push(c3);
+    if (m > 2) push(c2);
+    if (m > 1) push(c1);
+    w.cur = c0;
+    return false;
+}
+
+// a wave's claimed queue entries not yet handed out, [rb, re), and whether the counter may still
+// give entries
+struct Refill {
+    uint32_t rb = 0, re = 0;
+    bool more = true;
+};
+
+// At the round loop's top (every lane of the wave there): the entry the calling lane takes
+// (LR_NONE: none), idle lanes taking the wave's entries in lane order, a new run of 64 claimed
+// when the wave's is too short.
+__device__ __forceinline__ uint32_t refill_take(Refill &f, unsigned int *ctr, uint32_t n, bool idle_lane, uint64_t idle,
+                                                uint32_t nidle)
+{
+    const uint32_t have = f.re - f.rb;
+    uint32_t nb = 0, ne = 0;
+    if (have < nidle && f.more) {
+        uint32_t b = 0;
+        if (lane_id() == 0) b = atomicAdd(ctr, 64u);
+        b = __shfl(b, 0);
+        if (b < n) {
+            nb = b;
+            ne = b + 64u < n ? b + 64u : n;
+        }
+        f.more = b + 64u < n;
+    }
+    uint32_t e = LR_NONE;
+    if (idle_lane) {
+        const uint32_t k = (uint32_t)__popcll(idle & lanemask_lt());
+        if (k < have) e = f.rb + k;
+        else if (k - have < ne - nb) e = nb + (k - have);
+    }
+    const uint32_t taken = nidle < have + (ne - nb) ? nidle : have + (ne - nb);
+    if (taken <= have) f.rb += taken;
+    else {
+        f.rb = nb + (taken - have);
+        f.re = ne;
+    }
+    return e;
+}
+
+// the extension walk with lanes refilled
+template <int BLK, int STK, bool COUNT>
+__global__ __launch_bounds__(BLK, walk_waves(BLK, STK)) void k_extend_rf(EngArgs A)
+{
+    __shared__ uint32_t stk[STK][BLK];
+    EngCtl *ctl = A.ctl;
+    const int par = (int)(ctl->iter & 1ull);
+    const uint32_t n = ctl->count[par];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned long long c = ctl->claimed;
+        ctl->head += c;
+        ctl->next += c;
+        ctl->claimed = 0;
+        ctl->rays_ext += n;
+        ctl->count[1 - par] = 0;
+        ctl->count[2] = 0;
+        reset_tiles(ctl, K_EXTEND);
+    }
+    const size_t gtid = (size_t)blockIdx.x * BLK + threadIdx.x;
+    for (size_t k = gtid; k < (size_t)n / 64 + 2; k += (size_t)gridDim.x * BLK) A.ostat[k] = 0ull;
+    const WalkStack S{&stk[0][threadIdx.x], BLK, A.spill + gtid, (size_t)A.spill_stride, STK + A.spill_per};
+    WalkCnt wc;
+    Refill f;
+    uint32_t i = LR_NONE;   // the lane's queue entry (LR_NONE: idle)
+    WalkState w;
+    for (;;) {
+        const uint64_t idle = __ballot(i == LR_NONE);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (nidle == 64u && f.rb == f.re && !f.more) break;
+        if (nidle >= (uint32_t)A.refill) {
+            const uint32_t e = refill_take(f, &ctl->tiles[K_EXTEND], n, i == LR_NONE, idle, nidle);
+            if (e != LR_NONE) {
+                const JobDev &J = A.jobs[A.qid[par][e] >> LR_SLOT_BITS];
+                const float4 r0 = A.qray[par][2 * (size_t)e], r1 = A.qray[par][2 * (size_t)e + 1];
+                if (J.two_level) {   // not walked here: flagged, so that the host reports it
+                    atomicOr(A.overflow, 2u);
+                    A.hits[e] = make_float4(r1.w, __uint_as_float(LR_NONE), 0.0f, 0.0f);
+                } else {
+                    i = e;
+                    walk_start(w, J, r0, r1);
+                }
+            }
+        }
+        if (i != LR_NONE && walk_round<false, COUNT, STK>(w, S, A.overflow, wc)) {
+            A.hits[i] = make_float4(w.best.t, __uint_as_float(w.best.id), w.best.b1, w.best.b2);
+            i = LR_NONE;
+        }
+    }
+    if (COUNT) add_counts(A.counts, A.counts + 8, wc);
+}
+
+// the shadow walk with lanes refilled (single-level scenes, one shadow ray a vertex): an entry's
+// light joins its path when the ray reaches it; paths that end are retired at the round's end,
+// every lane of the wave there
+template <int BLK, int STK, bool COUNT>
+__global__ __launch_bounds__(BLK, walk_waves(BLK, STK)) void k_shadow_rf(EngArgs A)
+{
+    __shared__ uint32_t stk[STK][BLK];
+    EngCtl *ctl = A.ctl;
+    const uint32_t n = ctl->count[2];
+    const int par = (int)ctl->spar;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        reset_tiles(ctl, K_SHADOW);
+        ctl->rays_shadow += n;
+        ctl->iter += 1;
+    }
+    const size_t gtid = (size_t)blockIdx.x * BLK + threadIdx.x;
+    const WalkStack S{&stk[0][threadIdx.x], BLK, A.spill + gtid, (size_t)A.spill_stride, STK + A.spill_per};
+    WalkCnt wc;
+    Refill f;
+    uint32_t i = LR_NONE;   // the lane's shadow entry (LR_NONE: idle)
+    WalkState w;
+    for (;;) {
+        const uint64_t idle = __ballot(i == LR_NONE);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (nidle == 64u && f.rb == f.re && !f.more) break;
+        if (nidle >= (uint32_t)A.refill_shadow) {
+            const uint32_t e = refill_take(f, &ctl->tiles[K_SHADOW], n, i == LR_NONE, idle, nidle);
+            if (e != LR_NONE) {
+                i = e;
+                const JobDev &J = A.jobs[A.sid[e] >> LR_SLOT_BITS];
+                walk_start(w, J, A.sray[2 * (size_t)e], A.sray[2 * (size_t)e + 1]);
+                if (J.two_level) {   // not walked here: flagged, so that the host reports it (occluded)
+                    atomicOr(A.overflow, 2u);
+                    w.cur = LR_NONE;   // ends below at once, unlit
+                    w.best.id = 0u;
+                }
+            }
+        }
+        bool ends = false, glow = false;
+        uint32_t jslot = 0, q = 0;
+        float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i != LR_NONE && (w.cur == LR_NONE || walk_round<true, COUNT, STK>(w, S, A.overflow, wc))) {
+            const uint32_t id = A.sid[i];
+            const uint32_t at = id & LR_SLOT_MASK;
+            jslot = id >> LR_SLOT_BITS;
+            glow = (at & 1u) != 0u;
+            const float4 c = A.scon[i];
+            ends = c.w != 0.0f;
+            const bool lit = w.best.id == LR_NONE;
+            if (lit || ends) {
+                r = A.srad[i];
+                if (lit) {
+                    r.x += c.x;
+                    r.y += c.y;
+                    r.z += c.z;
+                }
+                if (!ends) A.rad[par][at] = r;   // the path's next state (shade wrote it without this)
+                else q = A.sq[i];
+            }
+            i = LR_NONE;
+        }
+        if (__ballot(ends)) retire(A, ends, jslot, q, r, glow);
+    }
+    if (COUNT) add_counts(A.counts + 4, A.counts + 9, wc);
 }
 
 // The hit of a ray from o along d at t, shape hid (an instanced scene: inst is the instance):
@@ -859,7 +1112,7 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
 // shadow (wf_shadow, trace_wave.wgsl:335-355): any hit before the lamp; if none, the
 // contribution joins the path's radiance. Paths whose last vertex made the shadow ray end here.
 // TL and the register bound as for extend.
-template <int BLK, int STK, bool TL, bool MULTI>
+template <int BLK, int STK, bool COUNT, bool TL, bool MULTI>
 __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(EngArgs A)
 {
     __shared__ uint32_t stk[STK][BLK];
@@ -873,6 +1126,7 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(E
     }
     const size_t gtid = (size_t)blockIdx.x * BLK + threadIdx.x;
     const WalkStack S{&stk[0][threadIdx.x], BLK, A.spill + gtid, (size_t)A.spill_stride, STK + A.spill_per};
+    WalkCnt wc;
     for_tiles<BLK>(&ctl->tiles[K_SHADOW], n, A.mult, [&](uint32_t i, bool live) {
         bool ends = false;
         uint32_t jslot = 0, q = 0;
@@ -887,12 +1141,11 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(E
             const JobDev &J = A.jobs[jslot];
             if constexpr (!MULTI) {
                 const float4 r0 = A.sray[2 * (size_t)i], r1 = A.sray[2 * (size_t)i + 1];
-                uint32_t cn, ct;
                 WalkHit h;
-                if (TL && J.two_level) h = walk4_tl<true, false, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+                if (TL && J.two_level) h = walk4_tl<true, COUNT, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, wc);
                 else if (!J.two_level)
-                    h = walk4<true, false, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w, r1.w, S,
-                                                A.overflow, cn, ct);
+                    h = walk4<true, COUNT, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w, r1.w, S,
+                                                A.overflow, wc);
                 else {
                     atomicOr(A.overflow, 2u);
                     h = WalkHit{0.0f, 0u, 0.0f, 0.0f};   // occluded: adds nothing
@@ -920,12 +1173,11 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(E
                 for (int j = 0; j <= nx; j++) {
                     const float4 *rr = j == 0 ? A.sray + 2 * (size_t)i : (j == 1 ? A.xsray[0] : A.xsray[1]) + 2 * (size_t)src;
                     const float4 r0 = rr[0], r1 = rr[1];
-                    uint32_t cn, ct;
                     WalkHit h;
-                    if (TL && J.two_level) h = walk4_tl<true, false, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, cn, ct);
+                    if (TL && J.two_level) h = walk4_tl<true, COUNT, STK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, A.overflow, wc);
                     else if (!J.two_level)
-                        h = walk4<true, false, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w, r1.w, S,
-                                                    A.overflow, cn, ct);
+                        h = walk4<true, COUNT, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0), v3of(r1), r0.w, r1.w, S,
+                                                    A.overflow, wc);
                     else {
                         atomicOr(A.overflow, 2u);
                         h = WalkHit{0.0f, 0u, 0.0f, 0.0f};   // occluded: adds nothing
@@ -957,6 +1209,7 @@ __global__ __launch_bounds__(BLK, TL ? 1 : walk_waves(BLK, STK)) void k_shadow(E
         }
         retire(A, ends, jslot, q, r, glow);
     });
+    if (COUNT) add_counts(A.counts + 4, A.counts + 9, wc);
 }
 
 // The denoiser's guide of a job (trace.wgsl:1034-1107, guide): each pixel's ray through its
@@ -982,10 +1235,10 @@ __global__ __launch_bounds__(BLK) void k_guide(JobDev J, float4 *guide, uint32_t
         V3 gn = v3(0.0f, 0.0f, 0.0f), ga = v3(1.0f, 1.0f, 1.0f), a = v3(1.0f, 1.0f, 1.0f);
         float gz = 1e30f, kind = 1.0f, dist = 0.0f;
         for (int i = 0; i < 6; i++) {
-            uint32_t cn, ct;
+            WalkCnt wc;
             WalkHit h;
-            if (TL && J.two_level) h = walk4_tl<false, false, STK>(J, o, d, J.eps, 1e30f, S, overflow, cn, ct);
-            else h = walk4<false, false, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, o, d, J.eps, 1e30f, S, overflow, cn, ct);
+            if (TL && J.two_level) h = walk4_tl<false, false, STK>(J, o, d, J.eps, 1e30f, S, overflow, wc);
+            else h = walk4<false, false, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, o, d, J.eps, 1e30f, S, overflow, wc);
             if (h.id == LR_NONE) break;
             const V3 hp = vadd(o, vscale(d, h.t));
             V3 outward;
@@ -1046,10 +1299,20 @@ void L_extend(dim3 g, hipStream_t s, const EngArgs &A)
 {
     hipLaunchKernelGGL((k_extend<BLK, STK, COUNT, TL>), g, dim3(BLK), 0, s, A);
 }
-template <int BLK, int STK, bool TL, bool MULTI>
+template <int BLK, int STK, bool COUNT>
+void L_extend_rf(dim3 g, hipStream_t s, const EngArgs &A)
+{
+    hipLaunchKernelGGL((k_extend_rf<BLK, STK, COUNT>), g, dim3(BLK), 0, s, A);
+}
+template <int BLK, int STK, bool COUNT>
+void L_shadow_rf(dim3 g, hipStream_t s, const EngArgs &A)
+{
+    hipLaunchKernelGGL((k_shadow_rf<BLK, STK, COUNT>), g, dim3(BLK), 0, s, A);
+}
+template <int BLK, int STK, bool COUNT, bool TL, bool MULTI>
 void L_shadow(dim3 g, hipStream_t s, const EngArgs &A)
 {
-    hipLaunchKernelGGL((k_shadow<BLK, STK, TL, MULTI>), g, dim3(BLK), 0, s, A);
+    hipLaunchKernelGGL((k_shadow<BLK, STK, COUNT, TL, MULTI>), g, dim3(BLK), 0, s, A);
 }
 
 template <int BLK, int STK, int K>
@@ -1060,10 +1323,18 @@ void fill_walks(LrEngTable &t)
     t.extend[b][K][1] = L_extend<BLK, STK, false, true>;
     t.extend_count[b][K][0] = L_extend<BLK, STK, true, false>;
     t.extend_count[b][K][1] = L_extend<BLK, STK, true, true>;
-    t.shadow[b][K][0][0] = L_shadow<BLK, STK, false, false>;
-    t.shadow[b][K][1][0] = L_shadow<BLK, STK, true, false>;
-    t.shadow[b][K][0][1] = L_shadow<BLK, STK, false, true>;
-    t.shadow[b][K][1][1] = L_shadow<BLK, STK, true, true>;
+    t.extend_rf[b][K][0] = L_extend_rf<BLK, STK, false>;
+    t.extend_rf[b][K][1] = L_extend_rf<BLK, STK, true>;
+    t.shadow_rf[b][K][0] = L_shadow_rf<BLK, STK, false>;
+    t.shadow_rf[b][K][1] = L_shadow_rf<BLK, STK, true>;
+    t.shadow[b][K][0][0] = L_shadow<BLK, STK, false, false, false>;
+    t.shadow[b][K][1][0] = L_shadow<BLK, STK, false, true, false>;
+    t.shadow[b][K][0][1] = L_shadow<BLK, STK, false, false, true>;
+    t.shadow[b][K][1][1] = L_shadow<BLK, STK, false, true, true>;
+    t.shadow_count[b][K][0][0] = L_shadow<BLK, STK, true, false, false>;
+    t.shadow_count[b][K][1][0] = L_shadow<BLK, STK, true, true, false>;
+    t.shadow_count[b][K][0][1] = L_shadow<BLK, STK, true, false, true>;
+    t.shadow_count[b][K][1][1] = L_shadow<BLK, STK, true, true, true>;
 }
 
 template <int BLK>

@@ -1025,10 +1025,10 @@ __global__ __launch_bounds__(128) void k_scene_trace(JobDev J, const float4 *__r
     if (i >= n) return;
     const float4 r0 = rays[2 * (size_t)i], r1 = rays[2 * (size_t)i + 1];
     const WalkStack S{&stk[0][threadIdx.x], 128, spill + i, (size_t)n, LR_LDS_STACK + spill_per};
-    uint32_t cn = 0, ct = 0;
-    const WalkHit h = J.two_level ? walk4_tl<ANY, false, LR_LDS_STACK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, overflow, cn, ct)
+    WalkCnt wc;
+    const WalkHit h = J.two_level ? walk4_tl<ANY, false, LR_LDS_STACK>(J, v3of(r0), v3of(r1), r0.w, r1.w, S, overflow, wc)
                                   : walk4<ANY, false, LR_LDS_STACK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, v3of(r0),
-                                                                    v3of(r1), r0.w, r1.w, S, overflow, cn, ct);
+                                                                    v3of(r1), r0.w, r1.w, S, overflow, wc);
     hits[i] = make_float4(h.t, __uint_as_float(h.id), h.b1, h.b2);
 }
 

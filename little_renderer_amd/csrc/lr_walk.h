@@ -117,20 +117,20 @@ __device__ __forceinline__ Prim load_prim(const float4 *prims, uint32_t id)
 __device__ __forceinline__ float hit_prim(const float4 *__restrict__ prims, uint32_t id, V3 o, V3 d, float tmin, float tmax,
                                           float &a_out, float &b_out)
 {
-    const float4 *p = prims + 6 * (size_t)id;
-    const float4 q = p[1];
+    const size_t p = 6 * (size_t)id;
+    const float4 q = ldg4(prims, p + 1);
     if (q.w < 0.5f) {
         Prim s;
-        s.c = p[0];
+        s.c = ldg4(prims, p);
         return hit_sphere(s, o, d, tmin, tmax);
     }
-    const float4 nn = p[4];
+    const float4 nn = ldg4(prims, p + 4);
     const V3 n = v3of(nn);
     const float denom = vdot(n, d);
     if (fabsf(denom) < 1e-12f) return -1.0f;
     const float t = (nn.w - vdot(n, o)) / denom;
     if (t <= tmin || t >= tmax) return -1.0f;
-    const float4 u = p[2], v = p[3], w = p[5];
+    const float4 u = ldg4(prims, p + 2), v = ldg4(prims, p + 3), w = ldg4(prims, p + 5);
     const V3 rel = vsub(vadd(o, vscale(d, t)), v3of(q));
     const float a = vdot(v3of(w), vcross(rel, v3of(v)));
     const float b = vdot(v3of(w), vcross(v3of(u), rel));
@@ -155,6 +155,19 @@ struct WalkStack {
     size_t sstride;
     int cap;
 };
+
+// entry sp of a walk's stack (LDS for the first STK, device memory past them). The LDS entry is
+// read whatever sp is (at a clamped place) and the device one only past STK, as an atomic load
+// (which the compiler does not merge with the other), so that the two stay an LDS read and a
+// rare global one instead of becoming one flat read through a selected address (a vector
+// memory instruction on every pop)
+template <int STK>
+__device__ __forceinline__ uint32_t stk_get(const WalkStack &S, int sp)
+{
+    uint32_t x = S.stk[(sp < STK ? sp : STK - 1) * S.stride];
+    if (sp >= STK) x = __hip_atomic_load(&S.spill[(size_t)(sp - STK) * S.sstride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    return x;
+}
 
 // The closest hit in (tmin, tmax), or (ANY) the first one, over shape ids taken from the
 // leaves' aux entries: ids below nprims are primitives, the rest triangles (id - nprims).
@@ -207,7 +220,7 @@ __device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32
             }
             if (sp == 0) return best;
             sp--;
-            const uint32_t ni = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+            const uint32_t ni = stk_get<STK>(S, sp);
             na = __float_as_uint(words[8 * (size_t)ni + 3]);
             nb = __float_as_uint(words[8 * (size_t)ni + 7]);
             if (COUNT) cn++;
@@ -238,7 +251,7 @@ __device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32
         } else {
             if (sp == 0) break;
             sp--;
-            const uint32_t ni = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+            const uint32_t ni = stk_get<STK>(S, sp);
             na = __float_as_uint(words[8 * (size_t)ni + 3]);
             nb = __float_as_uint(words[8 * (size_t)ni + 7]);
             if (COUNT) cn++;
@@ -249,18 +262,48 @@ __device__ __forceinline__ WalkHit walk(const float4 *__restrict__ nodes, uint32
 
 
 // ---- the engine's walks: leaves -------------------------------------------------------------
+// A walk's own counters (its counting variants): node records fetched and shapes tested by the
+// lane, and the steps its wave took to do them (each counted once, by the wave's first lane at
+// work in it), so that fetched / (64 x node steps) and tested / (64 x shape steps) are the shares
+// of a wave's lanes at work in its node and shape steps.
+struct WalkCnt {
+    uint32_t n = 0, t = 0, wn = 0, wt = 0, wu = 0;   // wu: node steps whose lanes all fetch one node
+};
+__device__ __forceinline__ bool wave_first_lane() { return lane_id() == (uint32_t)__builtin_ctzll(__ballot(1)); }
+__device__ __forceinline__ void count_node(WalkCnt &c)
+{
+    c.n++;
+    if (wave_first_lane()) c.wn++;
+}
+// a node step at node k of the tree at w: counted as uniform when every lane at work in it is at
+// the same node of the same tree
+__device__ __forceinline__ void count_uniform(WalkCnt &c, const void *w, uint32_t k)
+{
+    const uint64_t p = (uint64_t)w;
+    const uint32_t fk = __builtin_amdgcn_readfirstlane(k);
+    const uint32_t flo = __builtin_amdgcn_readfirstlane((uint32_t)p), fhi = __builtin_amdgcn_readfirstlane((uint32_t)(p >> 32));
+    const bool same = k == fk && (uint32_t)p == flo && (uint32_t)(p >> 32) == fhi;
+    const uint64_t differ = __ballot(!same);
+    if (wave_first_lane() && differ == 0) c.wu++;
+}
+__device__ __forceinline__ void count_shape(WalkCnt &c)
+{
+    c.t++;
+    if (wave_first_lane()) c.wt++;
+}
+
 // ltris: the triangles in leaf order (aux entry e's triangle at ltris[3 e..]), so that the
 // triangle's load does not wait for the aux entry's.
 __device__ __forceinline__ void leaf_hits_(const uint32_t *__restrict__ aux, const float4 *__restrict__ ltris,
                                            const float4 *__restrict__ prims, uint32_t nprims, uint32_t first,
                                            uint32_t cnt, V3 o, V3 d, float tmin, WalkHit &best, bool any, bool &done,
-                                           uint32_t &ct, bool count)
+                                           WalkCnt &c, bool count)
 {
     for (uint32_t k = 0; k < cnt; k++) {
         const uint32_t e = first + k;
-        const uint32_t sid = aux[e];
-        const float4 ta = ltris[3 * (size_t)e], tb = ltris[3 * (size_t)e + 1], tc = ltris[3 * (size_t)e + 2];
-        if (count) ct++;
+        const uint32_t sid = ldg(aux, e);
+        const float4 ta = ldg4(ltris, 3 * (size_t)e), tb = ldg4(ltris, 3 * (size_t)e + 1), tc = ldg4(ltris, 3 * (size_t)e + 2);
+        if (count) count_shape(c);
         if (sid < nprims) {
             float a = 0.0f, b = 0.0f;
             const float t = hit_prim(prims, sid, o, d, tmin, best.t, a, b);
@@ -292,14 +335,15 @@ __device__ __forceinline__ void leaf_hits_(const uint32_t *__restrict__ aux, con
 // entry; none LR_NONE), and the children's boxes as 8-bit steps from the corner, rounded
 // outwards.
 
+
 // the entry distances (-1: missed, or no child) and references of a wide node's children: the
 // slab test with each plane's t as one fused multiply-add of its step, fma(q, s * inv, (corner -
 // o) * inv) (a culling test on boxes that hold the children's: its last bit decides no hit)
 __device__ __forceinline__ void wide_test(const float4 *__restrict__ w, uint32_t node, V3 oi, V3 inv, float tmin, float tmax,
                                           float t[4], uint32_t ref[4])
 {
-    const float4 n0 = w[4 * (size_t)node], n1 = w[4 * (size_t)node + 1];
-    const float4 n2 = w[4 * (size_t)node + 2], n3 = w[4 * (size_t)node + 3];
+    const float4 n0 = ldg4(w, 4 * (size_t)node), n1 = ldg4(w, 4 * (size_t)node + 1);
+    const float4 n2 = ldg4(w, 4 * (size_t)node + 2), n3 = ldg4(w, 4 * (size_t)node + 3);
     const uint32_t eb = __float_as_uint(n0.w);
     const float ax = __uint_as_float((eb & 0xffu) << 23) * inv.x;
     const float ay = __uint_as_float(((eb >> 8) & 0xffu) << 23) * inv.y;
@@ -350,10 +394,9 @@ template <bool ANY, bool COUNT, int STK>
 __device__ __forceinline__ WalkHit walk4(const float4 *__restrict__ w4, const uint32_t *__restrict__ aux,
                                          const float4 *__restrict__ ltris, const float4 *__restrict__ prims,
                                          uint32_t nprims, V3 o, V3 d, float tmin, float tmax, const WalkStack &S,
-                                         uint32_t *overflow, uint32_t &cn, uint32_t &ct)
+                                         uint32_t *overflow, WalkCnt &c)
 {
     const V3 inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
-    const V3 oi = v3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
     WalkHit best{tmax, LR_NONE, 0.0f, 0.0f};
     int sp = 0;
     bool done = false;
@@ -361,16 +404,20 @@ __device__ __forceinline__ WalkHit walk4(const float4 *__restrict__ w4, const ui
     for (;;) {
         while (cur & LR_LEAF) {
             leaf_hits_(aux, ltris, prims, nprims, cur & 0xffffffu, ((cur >> 24) & 0x3fu) + 1u, o, d, tmin, best, ANY, done,
-                       ct, COUNT);
+                       c, COUNT);
             if (ANY && done) return best;
             if (sp == 0) return best;
             sp--;
-            cur = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+            cur = stk_get<STK>(S, sp);
         }
-        if (COUNT) cn++;
+        if (COUNT) {
+            count_node(c);
+            count_uniform(c, w4, cur);
+        }
         float t[4];
         uint32_t ref[4];
-        wide_test(w4, cur, oi, inv, tmin, best.t, t, ref);
+        // o * inv, the slab test's other term, made again each node step (fewer registers held)
+        wide_test(w4, cur, v3(o.x * inv.x, o.y * inv.y, o.z * inv.z), inv, tmin, best.t, t, ref);
         // the children hit, nearest first: a sorting network over 4
         float s0 = t[0] >= 0.0f ? t[0] : 3e38f, s1 = t[1] >= 0.0f ? t[1] : 3e38f;
         float s2 = t[2] >= 0.0f ? t[2] : 3e38f, s3 = t[3] >= 0.0f ? t[3] : 3e38f;
@@ -384,7 +431,7 @@ __device__ __forceinline__ WalkHit walk4(const float4 *__restrict__ w4, const ui
         if (m == 0) {
             if (sp == 0) break;
             sp--;
-            cur = sp < STK ? S.stk[sp * S.stride] : S.spill[(size_t)(sp - STK) * S.sstride];
+            cur = stk_get<STK>(S, sp);
             continue;
         }
         if (sp + m - 1 > S.cap) {   // cannot happen with a correct depth: said, never silently dropped

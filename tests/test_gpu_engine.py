@@ -124,6 +124,42 @@ def test_engine_lamp_lit_ggx_scene_matches_oracle(ctx):
     ps.close()
 
 
+def test_server_jobs_match_the_oracle(ctx):
+    # the serving path end to end: a resident scene, jobs submitted from client threads, each
+    # job's image (read back through the server) against the oracle sample for sample
+    sc = lr.batch_scene(6000, 33)
+    ps = lr.PtScene(ctx, sc["prims"], sc["tris"], sc["mats"], 0, sc["eps"], sc["margin"])
+    srv = lr.Server(ctx, slots_log2=16, max_tree_depth=max(64, ps.info()[0]), budget=1 << 30)
+    srv.add_scene(ps)
+    W, H = 48, 40
+    specs = [(401, 8, 3), (402, 4, 9), (403, 12, 1)]
+    got = [None] * len(specs)
+
+    def client(k):
+        cam_seed, spp, seed = specs[k]
+        i = srv.submit(lr.make_job(ps, lr.batch_camera(cam_seed, W, H), W, H, spp, seed=seed))
+        info = srv.wait(i, 120000.0)
+        assert info is not None and info.state == lr.JOB_DONE
+        got[k] = srv.image(i, W, H)
+        srv.release(i)
+
+    th = [threading.Thread(target=client, args=(k,)) for k in range(len(specs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(180)
+    srv.close()
+    tab, mask = lr.brdf_tables(), lro.blue_noise()
+    for (cam_seed, spp, seed), g in zip(specs, got):
+        assert g is not None and np.all(g[..., 3] == spp)
+        o = lro.pt_image(dict(sc, cam=lr.batch_camera(cam_seed, W, H)), W, H, 0, spp, seed=seed, tab=tab, mask=mask)
+        rel = np.abs(g[..., :3] - o) / np.maximum(np.abs(o), 1e-3)
+        say(f"server job (camera {cam_seed}, {spp} spp): median rel {np.median(rel):.2e}, >1e-3 in {np.mean(rel > 1e-3):.4f}")
+        assert o.mean() > 0 and np.median(rel) < 2e-5 and np.mean(rel > 1e-3) < 0.02
+        assert abs(g[..., :3].mean() - o.mean()) < 2e-3 * o.mean()
+    ps.close()
+
+
 def test_batched_equals_plain_bit_for_bit(ctx, scenes):
     jobs = job_set(scenes)
     say(f"engine: {len(jobs)} jobs batched vs plain")
@@ -166,9 +202,19 @@ def test_few_slots_and_every_launch_shape(ctx, scenes):
     assert same(ref, run_batched(small, jobs))
     small.close()
     eng = lr.Engine(ctx, slots_log2=16, max_tree_depth=64, image_bytes=128 << 20)
-    shapes = [dict(block_trace=64, stack=16, block_shade=256, blocks_per_cu=1, tile=1, iters_per_graph=1),
-              dict(block_trace=256, stack=32, block_shade=64, blocks_per_cu=16, tile=8, iters_per_graph=64),
-              dict(block_trace=128, stack=24, block_shade=128, blocks_per_cu=2, tile=4, iters_per_graph=3)]
+    # refill: the walks handing new rays to finished lanes (1: at once, 64: a whole wave's at a
+    # time) or not (0); with few persistent waves (one work-group of 64 a compute unit) each wave
+    # takes many runs of the queue, so that lanes are refilled in the middle of others' walks
+    shapes = [dict(block_trace=64, stack=16, block_shade=256, blocks_per_cu=1, tile=1, iters_per_graph=1, refill=4,
+                   refill_shadow=4),
+              dict(block_trace=256, stack=32, block_shade=64, blocks_per_cu=16, tile=8, iters_per_graph=64, refill=1,
+                   refill_shadow=0),
+              dict(block_trace=128, stack=24, block_shade=128, blocks_per_cu=2, tile=4, iters_per_graph=3, refill=64,
+                   refill_shadow=64),
+              dict(block_trace=256, stack=16, block_shade=256, blocks_per_cu=8, tile=1, iters_per_graph=8, refill=16,
+                   refill_shadow=16),
+              dict(block_trace=64, stack=24, block_shade=64, blocks_per_cu=4, tile=2, iters_per_graph=2, refill=0,
+                   refill_shadow=1)]
     for t in shapes:
         eng.tune = t
         assert same(ref, run_batched(eng, jobs)), t
@@ -192,13 +238,37 @@ def test_autotune_keeps_images(ctx, scenes):
 def test_counting_walks(ctx, scenes):
     jobs = job_set(scenes)[:3]
     eng = lr.Engine(ctx, slots_log2=16, max_tree_depth=64, image_bytes=64 << 20)
+    # few persistent waves (one of 64 lanes a compute unit), so that refilled lanes take many runs
+    eng.tune = dict(block_trace=64, blocks_per_cu=1, refill=0, refill_shadow=0)
     ref = run_batched(eng, jobs)
     eng.counting(True)
     got = run_batched(eng, jobs)
     st = eng.stats()
+    # the walk with lanes refilled visits the same nodes and shapes (only its waves' steps differ)
+    eng.tune = dict(refill=8, refill_shadow=8)
+    got_rf = run_batched(eng, jobs)
+    st_rf = eng.stats()
     eng.close()
-    assert same(ref, got)
+    assert same(ref, got) and same(ref, got_rf)
+    d = lambda f: getattr(st_rf, f) - getattr(st, f)
+    assert d("node_fetches") == st.node_fetches and d("shape_tests") == st.shape_tests
+    assert d("shadow_node_fetches") == st.shadow_node_fetches and d("shadow_shape_tests") == st.shadow_shape_tests
+    say(f"  refilled lanes: extend node steps {st.node_fetches / (64 * d('wave_node_steps')):.3f} (against "
+        f"{st.node_fetches / (64 * st.wave_node_steps):.3f}), shape steps "
+        f"{st.shape_tests / (64 * d('wave_shape_steps')):.3f}")
     assert st.node_fetches > st.rays_ext // 2 and st.shape_tests > 0
+    # every wave step has between 1 and 64 lanes at work in it, in both walks
+    for n, w in ((st.node_fetches, st.wave_node_steps), (st.shape_tests, st.wave_shape_steps),
+                 (st.shadow_node_fetches, st.shadow_wave_node_steps), (st.shadow_shape_tests, st.shadow_wave_shape_steps)):
+        assert 0 < w <= n <= 64 * w
+    say(f"  lanes at work: extend node steps {st.node_fetches / (64 * st.wave_node_steps):.3f}, "
+        f"shape steps {st.shape_tests / (64 * st.wave_shape_steps):.3f}; shadow node steps "
+        f"{st.shadow_node_fetches / (64 * st.shadow_wave_node_steps):.3f}, "
+        f"shape steps {st.shadow_shape_tests / (64 * st.shadow_wave_shape_steps):.3f}")
+    # a shadow ray fetches at least its root
+    assert st.shadow_node_fetches >= st.rays_shadow
+    # uniform node steps (every lane at one node) are node steps, and the roots' are among them
+    assert 0 < st.uniform_node_steps <= st.wave_node_steps and 0 < st.shadow_uniform_node_steps <= st.shadow_wave_node_steps
 
 
 def test_serving_same_results_under_a_tight_budget(ctx, scenes):
