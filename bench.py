@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--tlas", choices=["lbvh", "ploc", "refit"], default="ploc",
                     help="instanced: the per-frame top-level builder (refit: the same tree while its cost stays within 1.1)")
     ap.add_argument("--frames-in-flight", type=int, default=3, help="instanced: frames rendering at once")
+    ap.add_argument("--physics", action="store_true",
+                    help="instanced: the dynamic instances moved by the rigid-body step (1/60 s a frame, 4 substeps) "
+                         "instead of the workload's scripted poses")
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--build-sizes", default="4000000,16000000",
                     help="bvh_stress: also time the builds alone at these sizes (the build kernels' roofline at scale)")
@@ -642,10 +645,28 @@ def bench_instanced(a, torch, lr):
             eng.pump(1)
         eng.release(i)
 
+    rigid, phys = None, {"ms": [], "host_ms": [], "contacts": [], "colours": []}
+    if a.physics:
+        # the dynamic instances as rigid bodies dropped over the ground among the props (fixed):
+        # each frame one rigid-body step, and their poses are the frame's moved records
+        rigid, moving, scales = lr.instanced_rigid(ctx, w0)
+        recs = w0["instances"][moving].copy()
+
     def frame(k):
         si = k % F
         settle(si)
-        st = scenes[si].move(moving, frames[k]["instances"][moving], builder=builder)
+        if rigid is not None:
+            th = time.perf_counter()
+            rigid.steps(1.0 / 60, 4, 1)
+            recs["xform"] = lr.rigid_xforms(rigid.bodies(0, len(moving)), scales)
+            rs = rigid.stats()
+            phys["ms"].append(rs.ms)
+            phys["contacts"].append(rs.contacts)
+            phys["colours"].append(rs.colours)
+            phys["host_ms"].append((time.perf_counter() - th) * 1e3 - rs.ms)
+            st = scenes[si].move(moving, recs, builder=builder)
+        else:
+            st = scenes[si].move(moving, frames[k]["instances"][moving], builder=builder)
         in_flight[si] = eng.submit(lr.make_job(scenes[si], frames[k]["cam"], W, H, 1, seed=k, max_depth=8))
         eng.pump(1)
         return st
@@ -707,6 +728,13 @@ def bench_instanced(a, torch, lr):
                    "scene_bytes": int(nbytes), "walk_depth": depth, "tune": tune.as_dict(),
                    "frames_in_flight": F},
         "tlas_builder": a.tlas,
+        "physics": None if rigid is None else {
+            "bodies": len(w0["instances"]) - 1, "moving": int(len(moving)),
+            "step_ms": round(float(np.mean(phys["ms"][warm:])), 3),
+            "host_ms": round(float(np.mean(phys["host_ms"][warm:])), 3),
+            "contacts": int(np.mean(phys["contacts"][warm:])), "colours": int(np.max(phys["colours"][warm:])),
+            "note": "one rigid-body step (1/60 s, 4 substeps) a frame on the device, then the moving instances' "
+                    "records and the top level (poses read back, as the reference's physics writes Instance.xform)"},
         "records_sent_per_frame": int(len(moving)),
         "tlas_refitted_frames": refits,
         "tlas_build_ms": round(float(np.mean(builds)), 4),
@@ -721,6 +749,9 @@ def bench_instanced(a, torch, lr):
         "cpu_baseline": {"value": None, "unit": "Mrays/s", "cores": None, "kind": "reference",
                          "sample": "not run for this workload: the reference's instanced CPU path is not built here"},
     }
+    if rigid is not None:
+        out["config"]["workload"] += "; dynamic instances moved by the rigid-body step"
+        rigid.close()
     eng.close()
     for s_ in scenes:
         s_.close()

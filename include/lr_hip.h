@@ -267,7 +267,7 @@ int lrh_bsdf_cases(LrhContext *c, const float *d_in, int n, float *d_out);
  * margins, substeps of gravity, warm-started sequential impulses with Coulomb friction, a bounce
  * once from the approach speed, the relax pass). Units are the scene's (gravity 981 down: cm).
  * Sleeping and the swept test of fast bodies (rigid_ccd) are not done. One world holds at most
- * 4096 bodies. */
+ * 8192 bodies. */
 #define LRH_BODY_SPHERE 0
 #define LRH_BODY_BOX 1
 typedef struct LrhRigid LrhRigid;

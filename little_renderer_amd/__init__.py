@@ -829,6 +829,84 @@ def pose(rng, scale, at):
     return np.hstack([r @ np.diag(scale), np.asarray(at, np.float64)[:, None]]).astype(np.float32)
 
 
+CM = 100.0   # the rigid bodies work in centimetres (the reference's physics units), the scenes here in metres
+
+
+def instanced_rigid(ctx, w, n_dynamic=2000, drop=3.0, seed=0):
+    """A rigid world for an instanced_workload scene w: its dynamic instances (1 .. n_dynamic) as
+    bodies (the spheres and the 64-point hulls as balls, the boxes as boxes, at their scale),
+    dropped from up to `drop` above their places; its static props as fixed balls (scale x 0.8,
+    where they stand); the ground as the room's floor with walls at its edges. Units: centimetres
+    (the scene's x 100), gravity 981 down, as the reference's physics. Returns (the world, the
+    dynamic instances' indices, their scales)."""
+    import numpy as np
+    inst = w["instances"]
+    n = len(inst)
+    r = Rigid(ctx, max_bodies=n, max_contacts=max(65536, 16 * n))
+    r.set_params(room=((-20 * CM, 0.0, -20 * CM), (20 * CM, 60 * CM, 20 * CM)))
+    rng = np.random.default_rng(seed)
+    idx, scales, fixed = [], [], []
+    for i in range(1, n):
+        x = inst[i]["xform"].reshape(3, 4).astype(np.float64)
+        s_ = float(np.linalg.norm(x[:, 0]))
+        at = x[:, 3] * CM
+        kind = int(inst[i]["kind"])
+        if i <= n_dynamic:
+            q = quat_of(x[:, :3] / s_)
+            at = at + np.array([0.0, rng.uniform(0.5, drop) * CM, 0.0])
+            if kind == 2:
+                r.add_box(tuple(at), (s_ * CM,) * 3, q, 1.0)
+            else:
+                r.add_sphere(tuple(at), s_ * CM * (1.0 if kind == 1 else 0.95), 1.0)
+            idx.append(i)
+            scales.append(s_)
+        else:
+            fixed.append(r.add_sphere(tuple(at), s_ * 0.8 * CM, 1.0))
+    if fixed:   # the props do not move: fixed bodies
+        b = r.bodies(fixed[0], len(fixed))
+        for x in b:
+            x.inv_mass = 0.0
+            x.inv_inertia[:] = [0.0, 0.0, 0.0]
+        r.set_bodies(fixed[0], list(b))
+    return r, np.asarray(idx, np.int32), np.asarray(scales, np.float32)
+
+
+def quat_of(m):
+    """the unit quaternion (x, y, z, w) of a rotation matrix (Shepperd's method)"""
+    import numpy as np
+    t = m[0, 0] + m[1, 1] + m[2, 2]
+    if t > 0:
+        s_ = np.sqrt(t + 1.0) * 2
+        q = ((m[2, 1] - m[1, 2]) / s_, (m[0, 2] - m[2, 0]) / s_, (m[1, 0] - m[0, 1]) / s_, 0.25 * s_)
+    elif m[0, 0] > m[1, 1] and m[0, 0] > m[2, 2]:
+        s_ = np.sqrt(1.0 + m[0, 0] - m[1, 1] - m[2, 2]) * 2
+        q = (0.25 * s_, (m[0, 1] + m[1, 0]) / s_, (m[0, 2] + m[2, 0]) / s_, (m[2, 1] - m[1, 2]) / s_)
+    elif m[1, 1] > m[2, 2]:
+        s_ = np.sqrt(1.0 + m[1, 1] - m[0, 0] - m[2, 2]) * 2
+        q = ((m[0, 1] + m[1, 0]) / s_, 0.25 * s_, (m[1, 2] + m[2, 1]) / s_, (m[0, 2] - m[2, 0]) / s_)
+    else:
+        s_ = np.sqrt(1.0 + m[2, 2] - m[0, 0] - m[1, 1]) * 2
+        q = ((m[0, 2] + m[2, 0]) / s_, (m[1, 2] + m[2, 1]) / s_, 0.25 * s_, (m[1, 0] - m[0, 1]) / s_)
+    q = np.asarray(q, np.float64)
+    return tuple(q / np.linalg.norm(q))
+
+
+def rigid_xforms(bodies, scales):
+    """the bodies' poses as instance transforms (3 x 4 rows, the scene's units): rotation x scale,
+    position / 100"""
+    import numpy as np
+    n = len(bodies)
+    arr = np.ctypeslib.as_array(bodies).view(np.float32).reshape(n, -1)   # Body: 23 words, the shape first
+    q = arr[:, 7:11].astype(np.float64)
+    p = arr[:, 4:7].astype(np.float64) / CM
+    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    r = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                  2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                  2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1).reshape(n, 3, 3)
+    r *= np.asarray(scales, np.float64)[:, None, None]
+    return np.concatenate([r, p[:, :, None]], 2).reshape(n, 12).astype(np.float32)
+
+
 def instanced_workload(seed=0, frame=0, n_dynamic=2000, n_static=3000, n_props=20, n_hulls=8):
     """The instanced workload: meshes (ground 40 x 40, a 960-triangle sphere, a box, n_hulls
     64-point convex hulls, n_props seeded blobs of 500-20k triangles, log-uniform), 16

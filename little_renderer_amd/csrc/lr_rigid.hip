@@ -30,7 +30,7 @@
 namespace {
 
 constexpr int RB_THREADS = 1024;
-constexpr int RB_MAX_BODIES = 4096;        // the colour masks: two 64-bit words a body in LDS
+constexpr int RB_MAX_BODIES = 8192;        // the colour masks: two 64-bit words a body in LDS (128 KB)
 constexpr int RB_COLOURS = 128;
 constexpr int RB_WALL_SLOTS = 48;          // a box's corners against six walls
 constexpr int RB_PAIR_SLOTS = 8;           // a clipped box face
@@ -344,7 +344,7 @@ __device__ void box_box(const RigidDev &D, int a, int b, float m, Sink &out)
             }
         }
         // the incident face (axis ki, the side facing the reference box)
-        F3 poly[16], tmp[16];
+        F3 poly[8], tmp[8];   // (a quad clipped by four planes: at most 8 corners)
         {
             const float s = most > 0.0f ? -1.0f : 1.0f;
             const int i1 = (ki + 1) % 3, i2 = (ki + 2) % 3;
@@ -694,48 +694,78 @@ __global__ __launch_bounds__(RB_THREADS) void k_rigid_step(RigidDev D, RigidPara
     for (int i = tid; i < nb; i += RB_THREADS) mask[i][0] = mask[i][1] = 0ull;
     for (int k = tid; k < RB_COLOURS; k += RB_THREADS) s_cc[k] = 0;
     __syncthreads();
-    if (tid == 0) {
+    // (by the first wave: its lanes read 64 contacts' bodies at once, lane 0 colours them in
+    // order from the masks, and each lane keeps its contact's colour)
+    if (tid < 64) {
         int ncol = 0;
-        for (int k = 0; k < nc; k++) {
-            const int2 ab = C.ab[k];
-            const bool ma = D.p[ab.x].w > 0.0f, mb = ab.y >= 0 && D.p[ab.y].w > 0.0f;
-            uint64_t m0 = 0, m1 = 0;
-            if (ma) {
-                m0 |= mask[ab.x][0];
-                m1 |= mask[ab.x][1];
+        for (int base = 0; base < nc; base += 64) {
+            const int k = base + tid;
+            uint32_t word = 0xffffffffu;   // the moving bodies: a | b << 16, 0xffff none
+            if (k < nc) {
+                const int2 ab = C.ab[k];
+                const uint32_t a = D.p[ab.x].w > 0.0f ? (uint32_t)ab.x : 0xffffu;
+                const uint32_t b = ab.y >= 0 && D.p[ab.y].w > 0.0f ? (uint32_t)ab.y : 0xffffu;
+                word = a | b << 16;
             }
-            if (mb) {
-                m0 |= mask[ab.y][0];
-                m1 |= mask[ab.y][1];
+            const int cnt = nc - base < 64 ? nc - base : 64;
+            int mine = -1;
+            for (int q = 0; q < cnt; q++) {
+                const uint32_t wq = __shfl(word, q, 64);
+                int col = -1;
+                if (tid == 0) {
+                    const uint32_t a = wq & 0xffffu, b = wq >> 16;
+                    uint64_t m0 = 0, m1 = 0;
+                    if (a != 0xffffu) {
+                        m0 |= mask[a][0];
+                        m1 |= mask[a][1];
+                    }
+                    if (b != 0xffffu) {
+                        m0 |= mask[b][0];
+                        m1 |= mask[b][1];
+                    }
+                    col = ~m0 ? __builtin_ctzll(~m0) : (~m1 ? 64 + __builtin_ctzll(~m1) : -1);
+                    if (col >= 0) {
+                        const uint64_t bit = 1ull << (col & 63);
+                        if (a != 0xffffu) mask[a][col >> 6] |= bit;
+                        if (b != 0xffffu) mask[b][col >> 6] |= bit;
+                        s_cc[col]++;
+                        if (col + 1 > ncol) ncol = col + 1;
+                    } else D.counters[3] |= 8;   // more than RB_COLOURS on one body: not solved this step (said)
+                }
+                col = __shfl(col, 0, 64);
+                if (tid == q) mine = col;
             }
-            int col = ~m0 ? __builtin_ctzll(~m0) : (~m1 ? 64 + __builtin_ctzll(~m1) : -1);
-            if (col < 0) {   // more than RB_COLOURS on one body: not solved this step (said)
-                D.counters[3] |= 8;
-                C.colour[k] = -1;
-                continue;
+            if (k < nc) C.colour[k] = mine;
+        }
+        if (tid == 0) {
+            int run = 0;
+            for (int c = 0; c < RB_COLOURS; c++) {
+                D.cstart[c] = run;
+                const int x = s_cc[c];
+                s_cc[c] = run;   // (now each colour's next place)
+                run += x;
             }
-            const uint64_t bit = 1ull << (col & 63);
-            if (ma) mask[ab.x][col >> 6] |= bit;
-            if (mb) mask[ab.y][col >> 6] |= bit;
-            C.colour[k] = col;
-            s_cc[col]++;
-            if (col + 1 > ncol) ncol = col + 1;
+            D.cstart[RB_COLOURS] = run;
+            s_total = ncol;
+            D.counters[0] = nc;
+            D.counters[1] = np;
+            D.counters[2] = ncol;
         }
-        int run = 0;
-        for (int c = 0; c < RB_COLOURS; c++) {
-            D.cstart[c] = run;
-            run += s_cc[c];
+        // the contacts by colour, each colour's in contact order
+        for (int base = 0; base < nc; base += 64) {
+            const int k = base + tid;
+            const int col = k < nc ? C.colour[k] : -1;
+            const int cnt = nc - base < 64 ? nc - base : 64;
+            int at = -1;
+            for (int q = 0; q < cnt; q++) {
+                const int cq = __shfl(col, q, 64);
+                int pos = -1;
+                if (tid == 0 && cq >= 0) pos = s_cc[cq]++;
+                pos = __shfl(pos, 0, 64);
+                if (tid == q) at = pos;
+            }
+            if (at >= 0) D.order[at] = k;
         }
-        D.cstart[RB_COLOURS] = run;
-        for (int c = 0; c < RB_COLOURS; c++) s_cc[c] = D.cstart[c];
-        for (int k = 0; k < nc; k++) {
-            const int col = C.colour[k];
-            if (col >= 0) D.order[s_cc[col]++] = k;
-        }
-        s_total = ncol;
-        D.counters[0] = nc;
-        D.counters[1] = np;
-        D.counters[2] = ncol;
     }
     __syncthreads();
     const int ncol = s_total;
