@@ -16,6 +16,10 @@ oracle/_ref) on a sample of the same jobs on this host's cores.
 --workload bvh_stress: the 1M-triangle BVH-stress step (LBVH + PLOC builds, 1920x1080 primaries,
 a diffuse bounce and a shadow ray per hit), with its own roofline and CPU baseline.
 
+--workload caustics: one job with light-traced caustics (a glass and a mirror ball in the Cornell
+room, 512x512, 16 spp, one light path a pixel and pass); the light kernel's time, roofline and
+HBM traffic, and the reference's CPU path (trace_image with skip_caustics + light_trace_caustics).
+
 Prints one JSON line.
 """
 import argparse
@@ -36,7 +40,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (batched: 2 batches; bvh_stress: 10)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (batched: 1; bvh_stress: 3)")
-    ap.add_argument("--workload", choices=["batched", "bvh_stress", "instanced"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "bvh_stress", "instanced", "caustics"], default="batched")
     ap.add_argument("--jobs", type=int, default=64, help="batched: jobs (and scenes) in a batch")
     ap.add_argument("--serve-requests", type=int, default=200, help="serving: requests per arrival rate (0: skip)")
     ap.add_argument("--serve-rates", default="10,25,50,100,200,400",
@@ -86,6 +90,8 @@ def main():
         out = bench_bvh_stress(a, torch, lr)
     elif a.workload == "instanced":
         out = bench_instanced(a, torch, lr)
+    elif a.workload == "caustics":
+        out = bench_caustics(a, torch, lr)
     else:
         out = bench_batched(a, torch, lr)
     print(json.dumps(out), flush=True)
@@ -282,12 +288,10 @@ def build_pool(ctx, lr, plan):
     return pool
 
 
-def live_traffic(a, tune, limit_s=300):
-    """HBM bytes per launch of the batched extension kernel, measured for this run: this script
-    run again in child processes under rocprofv3 --pmc, once with FETCH_SIZE and once with
-    WRITE_SIZE (the same jobs and launch shape, one batch, the extension kernel's dispatches
-    only), each counter scaled by what it reads for a 512 MiB device copy (two more child runs,
-    --copy-probe). Returns (bytes per launch, detail) or (None, the reason)."""
+def pmc_per_dispatch(counter, args, regex, match, limit_s=300):
+    """This script run again in a child process under rocprofv3 --pmc counter (args: its command
+    line), the dispatches of kernels matching regex: the mean of the counter over those whose name
+    holds match, and how many. Raises on a time-out or no dispatch."""
     import csv
     import glob
     import shutil
@@ -296,54 +300,63 @@ def live_traffic(a, tune, limit_s=300):
     import tempfile
     prof = shutil.which("rocprofv3")
     if not prof:
-        return None, {"why": "rocprofv3 not found"}
-    me = os.path.abspath(__file__)
-    t = tune.as_dict()
-    shape = ",".join(str(t[k]) for k in ("block_trace", "stack", "block_shade", "blocks_per_cu", "tile",
-                                          "iters_per_graph", "refill", "refill_shadow"))
-    kname = (f"k_extend_rf<{t['block_trace']}, {t['stack']}, false>" if t["refill"] > 0
-             else f"k_extend<{t['block_trace']}, {t['stack']}, false, false>")
-
-    def per_dispatch(counter, args, regex, match):
-        d = tempfile.mkdtemp(prefix="lr_pmc_", dir="/tmp")
-        cmd = [prof, "--pmc", counter, "--kernel-include-regex", regex, "--output-format", "csv", "-d", d, "-o", "run",
-               "--", sys.executable, "-u", me] + args
-        p = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
-                             stderr=subprocess.DEVNULL, start_new_session=True)
-        try:
-            p.wait(timeout=limit_s)
-        except subprocess.TimeoutExpired:
-            os.killpg(p.pid, signal.SIGKILL)
-            p.wait()
-            shutil.rmtree(d, ignore_errors=True)
-            raise RuntimeError(f"{counter} run passed {limit_s} s")
-        vals = []
-        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
-            for row in csv.DictReader(open(f)):
-                if match in row["Kernel_Name"] and row["Counter_Name"] == counter:
-                    vals.append(float(row["Counter_Value"]))
+        raise RuntimeError("rocprofv3 not found")
+    d = tempfile.mkdtemp(prefix="lr_pmc_", dir="/tmp")
+    cmd = [prof, "--pmc", counter, "--kernel-include-regex", regex, "--output-format", "csv", "-d", d, "-o", "run",
+           "--", sys.executable, "-u", os.path.abspath(__file__)] + args
+    p = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
+                         stderr=subprocess.DEVNULL, start_new_session=True)
+    try:
+        p.wait(timeout=limit_s)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.wait()
         shutil.rmtree(d, ignore_errors=True)
-        if not vals:
-            raise RuntimeError(f"no {counter} dispatches of {match} (rocprofv3 exit {p.returncode})")
-        return sum(vals) / len(vals), len(vals)
+        raise RuntimeError(f"{counter} run passed {limit_s} s")
+    vals = []
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for row in csv.DictReader(open(f)):
+            if match in row["Kernel_Name"] and row["Counter_Name"] == counter:
+                vals.append(float(row["Counter_Value"]))
+    shutil.rmtree(d, ignore_errors=True)
+    if not vals:
+        raise RuntimeError(f"no {counter} dispatches of {match} (rocprofv3 exit {p.returncode})")
+    return sum(vals) / len(vals), len(vals)
 
-    print("bench: HBM traffic of the extension kernel (rocprofv3 --pmc, 4 child runs)", file=sys.stderr, flush=True)
-    run_args = ["--tune", shape, "--jobs", str(a.jobs), "--serve-requests", "0", "--no-cpu", "--steps", "1",
-                "--warmup", "0", "--skip-plain", "--no-traffic"]
+
+def kernel_traffic(run_args, regex, kname, what):
+    """HBM bytes per launch of one kernel, measured for this run: this script run again in child
+    processes under rocprofv3 --pmc (run_args: the same work, one step), once with FETCH_SIZE and
+    once with WRITE_SIZE, each counter scaled by what it reads for a 512 MiB device copy (two more
+    child runs, --copy-probe). Returns (bytes per launch, detail) or (None, the reason)."""
+    print(f"bench: HBM traffic of {what} (rocprofv3 --pmc, 4 child runs)", file=sys.stderr, flush=True)
     try:
         probe = 512 << 20
-        cf, _ = per_dispatch("FETCH_SIZE", ["--copy-probe", "512"], "copyBuffer", "copyBuffer")
-        cw, _ = per_dispatch("WRITE_SIZE", ["--copy-probe", "512"], "copyBuffer", "copyBuffer")
+        cf, _ = pmc_per_dispatch("FETCH_SIZE", ["--copy-probe", "512"], "copyBuffer", "copyBuffer")
+        cw, _ = pmc_per_dispatch("WRITE_SIZE", ["--copy-probe", "512"], "copyBuffer", "copyBuffer")
         ff, wf = probe / (cf * 1024.0), probe / (cw * 1024.0)
-        fk, nf = per_dispatch("FETCH_SIZE", run_args, "k_extend", kname)
-        wk, nw = per_dispatch("WRITE_SIZE", run_args, "k_extend", kname)
+        fk, nf = pmc_per_dispatch("FETCH_SIZE", run_args, regex, kname)
+        wk, nw = pmc_per_dispatch("WRITE_SIZE", run_args, regex, kname)
     except Exception as e:   # noqa: BLE001 - reported in the line, the bench goes on
         return None, {"why": str(e)}
     fb, wb = fk * 1024.0 * ff, wk * 1024.0 * wf
     return int(fb + wb), {"kernel": kname, "fetch_bytes": int(fb), "write_bytes": int(wb),
                           "fetch_factor": round(ff, 4), "write_factor": round(wf, 4), "dispatches": [nf, nw],
-                          "how": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in child runs of this bench (one batch), "
+                          "how": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in child runs of this bench (one step), "
                                  "each scaled by the counter's reading for a 512 MiB device copy"}
+
+
+def live_traffic(a, tune):
+    """HBM bytes per launch of the batched extension kernel (kernel_traffic: the same jobs and
+    launch shape, one batch, the extension kernel's dispatches only)."""
+    t = tune.as_dict()
+    shape = ",".join(str(t[k]) for k in ("block_trace", "stack", "block_shade", "blocks_per_cu", "tile",
+                                          "iters_per_graph", "refill", "refill_shadow"))
+    kname = (f"k_extend_rf<{t['block_trace']}, {t['stack']}, false>" if t["refill"] > 0
+             else f"k_extend<{t['block_trace']}, {t['stack']}, false, false>")
+    run_args = ["--tune", shape, "--jobs", str(a.jobs), "--serve-requests", "0", "--no-cpu", "--steps", "1",
+                "--warmup", "0", "--skip-plain", "--no-traffic"]
+    return kernel_traffic(run_args, "k_extend", kname, "the extension kernel")
 
 
 def bench_batched(a, torch, lr):
@@ -514,6 +527,143 @@ def bench_batched(a, torch, lr):
     for ps in pool:
         ps.close()
     return out
+
+
+CAUSTICS_W, CAUSTICS_H, CAUSTICS_SPP, CAUSTICS_PHOTONS = 512, 512, 16, 1.0
+
+
+def bench_caustics(a, torch, lr):
+    """The caustics workload: one job of the caustics scene (a glass and a mirror ball in the
+    Cornell room), 512x512, 16 spp, one light path a pixel and pass (photons 1: 4.2M light paths
+    beside 4.2M camera paths). One step = the job through the engine (submit, drain: its light
+    paths at admission, then its camera paths). value = rays traced per second of wall time over
+    the timed steps: the path tracer's closest-hit and shadow rays and the light tracer's
+    closest-hit rays and rays towards the camera, all counted on the device. The roofline is the
+    light kernel's (k_light: algorithmic bytes from its own counters, 48 B a ray, 64 B a 4-wide
+    node, 52 B a shape, over its time from HIP events on the engine's stream)."""
+    steps = a.steps or 5
+    warm = 1 if a.warmup is None else a.warmup
+    W, H, spp, photons = CAUSTICS_W, CAUSTICS_H, CAUSTICS_SPP, CAUSTICS_PHOTONS
+    ctx = lr.Context(0)
+    sc = lr.caustic_scene(W, H)
+    ps = lr.PtScene(ctx, sc["prims"], sc["tris"], sc["mats"], sc["light"], sc["eps"], sc["margin"])
+    eng = lr.Engine(ctx, slots_log2=22, max_tree_depth=64, image_bytes=_align(W * H * 24) + (64 << 20))
+    if a.tune:
+        f = [int(x) for x in a.tune.split(",")]
+        eng.tune = dict(zip(["block_trace", "stack", "block_shade", "blocks_per_cu", "tile", "iters_per_graph", "refill",
+                             "refill_shadow"], f))
+    tune = eng.tune
+    job = lr.make_job(ps, sc["cam"], W, H, spp, seed=1, max_depth=8, photons=photons)
+    every = lambda st: st.rays_ext + st.rays_shadow + st.light_rays + st.light_shadow_rays
+
+    def step():
+        i = eng.submit(job)
+        assert i is not None
+        eng.drain()
+        eng.release(i)
+
+    for _ in range(warm):
+        step()
+    torch.cuda.synchronize()
+    s0 = eng.stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    s1 = eng.stats()
+    d = lambda f: getattr(s1, f) - getattr(s0, f)
+    rays = every(s1) - every(s0)
+    value = rays / dt / 1e6
+    # kernel times (events around each kernel, the light kernel's at admission)
+    eng.profiling(True)
+    k0 = eng.stats()
+    step()
+    k1 = eng.stats()
+    eng.profiling(False)
+    names = ["regen", "extend", "shade", "shadow"]
+    kms = {n: k1.kernel_ms[q] - k0.kernel_ms[q] for q, n in enumerate(names)}
+    kms["light"] = k1.light_ms - k0.light_ms
+    # the light kernel's node records and shapes (its counting form, in a run of its own)
+    eng.counting(True)
+    c0 = eng.stats()
+    step()
+    c1 = eng.stats()
+    eng.counting(False)
+    lrays = (c1.light_rays - c0.light_rays) + (c1.light_shadow_rays - c0.light_shadow_rays)
+    lnodes, lshapes = c1.light_node_fetches - c0.light_node_fetches, c1.light_shape_tests - c0.light_shape_tests
+    alg = rf.walk_bytes(lrays, lnodes, lshapes, rf.NODE_BYTES_WIDE)
+    achieved = alg / (kms["light"] * 1e-3) / 1e9 if kms["light"] > 0 else 0.0
+    if a.no_traffic:
+        traffic, traffic_detail = None, {"why": "--no-traffic"}
+    else:
+        t = tune.as_dict()
+        shape = ",".join(str(t[k]) for k in ("block_trace", "stack", "block_shade", "blocks_per_cu", "tile",
+                                              "iters_per_graph", "refill", "refill_shadow"))
+        traffic, traffic_detail = kernel_traffic(["--workload", "caustics", "--tune", shape, "--no-cpu", "--steps", "1",
+                                                  "--warmup", "0", "--no-traffic"], "k_light",
+                                                 "k_light<256, 16, false>", "the light kernel")
+    lpaths = d("light_paths") // steps
+    out = {
+        "metric": METRIC + " (caustics workload)",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": round(dt * 1e3 / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (the Cornell room's walls and lamp with a glass and a mirror ball, generated on the host)",
+        "config": {"workload": f"caustics: a glass and a mirror ball in the Cornell room, {W}x{H}, {spp} spp, "
+                               f"{photons:g} light path a pixel and pass (light tracing + the path tracer with "
+                               f"skip_caustics)", "camera_paths_per_step": W * H * spp, "light_paths_per_step": lpaths,
+                   "rays_per_step": rays // steps, "tune": tune.as_dict()},
+        "light": {"ms": round(kms["light"], 4), "mpaths_s": round(lpaths / kms["light"] / 1e3, 2) if kms["light"] else None,
+                  "mrays_s": round(lrays / kms["light"] / 1e3, 2) if kms["light"] else None,
+                  "rays_per_path": round(lrays / max(lpaths, 1), 3),
+                  "camera_rays_per_path": round((c1.light_shadow_rays - c0.light_shadow_rays) / max(lpaths, 1), 3),
+                  "nodes_per_ray": round(lnodes / max(lrays, 1), 2), "shapes_per_ray": round(lshapes / max(lrays, 1), 2)},
+        "kernel_ms_total": {n: round(v, 3) for n, v in kms.items()},
+        "roofline": {"bound": "hbm", "kernel": "light", "dominant": max(kms, key=kms.get), "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "traffic_detail": traffic_detail, "alg_bytes_per_launch": alg,
+                     "node_fetches": lnodes, "shape_tests": lshapes, "rays": lrays, "launches": 1},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_caustics(sc, W, H, (rays // steps) / spp, a.cpu_seconds)
+    eng.close()
+    ps.close()
+    return out
+
+
+def cpu_baseline_caustics(sc, W, H, rays_per_pass, seconds):
+    """The reference's own CPU path for the same job (oracle/_ref, its float build, OpenMP on
+    every host core): trace_image with skip_caustics and light_trace_caustics, one pass (1 spp,
+    W x H light paths) at a time, repeated for about `seconds`; the median pass's time, and the
+    rays a pass the device traced on the same job (the CPU path does not count rays; both follow
+    the same rules)."""
+    from oracle import lro
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        cores = os.cpu_count() or 1
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", cores)), cores)
+    if lro.ref(False) is None:
+        return {"value": None, "unit": "Mrays/s", "cores": threads, "kind": "reference",
+                "sample": "oracle/_ref not built: no CPU baseline", "cpu_model": cpu_model()}
+    reps, t_all = [], time.perf_counter()
+    while not reps or (time.perf_counter() - t_all < seconds and len(reps) < 1000):
+        t0 = time.perf_counter()
+        lro.ref_render_lt(sc, W, H, 1, seed=len(reps), trace=True, skip=True, photons=1.0, double=False)
+        reps.append(time.perf_counter() - t0)
+    med = sorted(reps)[len(reps) // 2]
+    return {"value": round(rays_per_pass / med / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "reference",
+            "sample": f"{len(reps)} passes of the same job at 1 spp ({W * H} camera paths + {W * H} light paths a pass), "
+                      f"{sum(reps):.1f} s; the median pass's time over the device's {rays_per_pass:.0f} rays a pass",
+            "pass_s": round(med, 4), "cpu_model": cpu_model()}
 
 
 def bench_serving(ctx, lr, pool, depth, tune, rates, nreq, target=1e-3):

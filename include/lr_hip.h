@@ -346,6 +346,11 @@ typedef struct {
                                estimated from the first 4 passes (each pass 1 sample a pixel):
                                K = ceil(4 (e4 / target)^2), e4 that of the 4 passes' means. The
                                image is then exactly a K-spp job's. 0: all spp samples */
+    float photons;          /* > 0: light tracing for caustics (light_trace_caustics, tracer.c:
+                               511-665; the GPU's ltrace, trace.wgsl:1341-1511), photons x w x h
+                               light paths (1 .. 16M) a pass, added to the image, the path tracer
+                               then leaving their paths out (skip_caustics); only with a lamp and a
+                               mirror or glass material, and no target error. 0: none */
 } LrhJob;
 
 /* The engine's launch shape (what lrh_engine_autotune picks): work-group sizes of the walk
@@ -398,6 +403,16 @@ typedef struct {
      * walks), for the extension and the shadow walks */
     uint64_t uniform_node_steps;
     uint64_t shadow_uniform_node_steps;
+    /* light tracing for caustics (LrhJob.photons): the light paths traced, their closest-hit rays
+     * and their rays towards the camera, with counting on the node records and shapes their
+     * walks read, its launches (one a job), and with profiling on its device time */
+    uint64_t light_paths;
+    uint64_t light_rays;
+    uint64_t light_shadow_rays;
+    uint64_t light_node_fetches;
+    uint64_t light_shape_tests;
+    uint64_t light_launches;
+    double light_ms;
 } LrhEngineStats;
 
 /* host clock (steady, ms) that job times are given in (clock_ms, engine/base/clock.h:6) */
@@ -450,6 +465,12 @@ int lrh_engine_stats(LrhEngine *e, LrhEngineStats *st);
  * one choice at a time (walk work-group size x stack, then work-groups per CU, then the tile,
  * then the shading work-group size), keeps the fastest in rays per second and sets it. */
 int lrh_engine_autotune(LrhEngine *e, const LrhJob *sample, int n, LrhTune *best, double *mrays);
+/* lrh_pt_render of one job with all its options: light-traced caustics (photons > 0, as
+ * gpu_frame_trace's photons argument, engine/app/gpu_frame.c:49-52, passed on to the GPU's
+ * caustics pass, engine/gpu/trace_gpu.c:1587-1594), a target error. Its sums (and sample
+ * counts) added to d_accum (float4 per pixel); returns when d_accum holds them. Not with
+ * denoise (its outputs come from an engine: lrh_engine_denoise). */
+int lrh_pt_render_job(LrhContext *c, const LrhJob *job, float *d_accum);
 
 /* ---- serving: a request loop over an engine ----
  * Scenes registered with the server stay resident; their bytes and the images of admitted jobs

@@ -52,7 +52,7 @@ class Job(ctypes.Structure):
     _fields_ = [("scene", ctypes.c_void_p), ("cam", ctypes.c_float * 12), ("w", ctypes.c_int), ("h", ctypes.c_int),
                 ("spp", ctypes.c_int), ("s0", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("max_depth", ctypes.c_int),
                 ("mis", ctypes.c_int), ("clamp", ctypes.c_float), ("denoise", ctypes.c_int),
-                ("target_error", ctypes.c_float)]
+                ("target_error", ctypes.c_float), ("photons", ctypes.c_float)]
 
 
 class JobInfo(ctypes.Structure):
@@ -68,7 +68,10 @@ class EngineStats(ctypes.Structure):
                 ("wave_node_steps", ctypes.c_uint64), ("wave_shape_steps", ctypes.c_uint64),
                 ("shadow_node_fetches", ctypes.c_uint64), ("shadow_shape_tests", ctypes.c_uint64),
                 ("shadow_wave_node_steps", ctypes.c_uint64), ("shadow_wave_shape_steps", ctypes.c_uint64),
-                ("uniform_node_steps", ctypes.c_uint64), ("shadow_uniform_node_steps", ctypes.c_uint64)]
+                ("uniform_node_steps", ctypes.c_uint64), ("shadow_uniform_node_steps", ctypes.c_uint64),
+                ("light_paths", ctypes.c_uint64), ("light_rays", ctypes.c_uint64), ("light_shadow_rays", ctypes.c_uint64),
+                ("light_node_fetches", ctypes.c_uint64), ("light_shape_tests", ctypes.c_uint64),
+                ("light_launches", ctypes.c_uint64), ("light_ms", ctypes.c_double)]
 
 
 class Body(ctypes.Structure):
@@ -151,6 +154,7 @@ def lib():
         "lrh_pt_scene_info": (I, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_size_t)]),
         "lrh_pt_render": (I, [P, P, P, I, I, I, U, ctypes.c_uint64, I, I, F, P]),
+        "lrh_pt_render_job": (I, [P, ctypes.POINTER(Job), P]),
         "lrh_bsdf_cases": (I, [P, P, I, P]),
         "lrh_denoise": (I, [P, P, P, P, I, I, I, P]),
         "lrh_engine_guide": (I, [P, ctypes.c_int64, P, P]),
@@ -274,6 +278,36 @@ def cornell(w, h, seed=0):
     return {"tris": tris, "prims": prims, "mats": mats, "light": 0, "cam": cam, "eps": 1e-4, "margin": 1e-3}
 
 
+def sphere_prim(c, r, material):
+    """a sphere in the 96-byte primitive record: centre, radius; q.w 0 (a sphere); u.w material"""
+    import numpy as np
+    p = np.zeros(24, np.float32)
+    p[0:3] = c
+    p[3] = r
+    p[11] = material
+    return p
+
+
+def caustic_scene(w, h, seed=0):
+    """The caustics scene: the Cornell room's five walls and its lamp (lrs_cornell), a clear glass
+    ball (material 18) and a mirror ball (19) on the floor: the lamp's light focused through the
+    glass and thrown by the mirror lands on the floor and the walls (paths lamp -> glass / mirror
+    -> rough -> camera, the ones light tracing adds)"""
+    import numpy as np
+    sc = cornell(w, h, seed)
+    mats = (Material * 20)()
+    ctypes.memmove(mats, sc["mats"], ctypes.sizeof(Material) * 18)
+    g, m = mats[18], mats[19]
+    g.type, g.ior, g.roughness = MAT_GLASS, 1.5, 0.0
+    m.type, m.ior, m.roughness = MAT_MIRROR, 1.5, 0.0
+    for k in range(3):
+        g.albedo[k] = 1.0
+        m.albedo[k] = 0.9
+    prims = np.stack([sc["prims"][0], sphere_prim((0.62, 0.18, 0.42), 0.18, 18), sphere_prim((0.26, 0.12, 0.30), 0.12, 19)])
+    return {"tris": np.ascontiguousarray(sc["tris"][:10]), "prims": prims, "mats": mats, "light": 0, "cam": sc["cam"],
+            "eps": 1e-4, "margin": 1e-3}
+
+
 INST_SPHERE, INST_BOX = -1, -2
 
 
@@ -395,6 +429,11 @@ class PtScene:
         c = np.ascontiguousarray(cam, np.float32)
         _check(lib().lrh_pt_render(self.ctx._c, self._h, _ptr(c), w, h, spp, s0, seed, max_depth, 1 if mis else 0, clamp,
                                    _ptr(accum)))
+
+    def render_job(self, job, accum):
+        """lrh_pt_render_job: a job of this scene (make_job: light-traced caustics, a target error)
+        added into accum (h, w, 4)"""
+        _check(lib().lrh_pt_render_job(self.ctx._c, ctypes.byref(job), _ptr(accum)))
 
 
 class Rigid:
@@ -589,9 +628,11 @@ def batch_plan(njobs, seed, spp_lo=1):
     return [tuple(int(x[i]) for x in a) for i in range(njobs)]
 
 
-def make_job(scene, cam, w, h, spp, s0=0, seed=0, max_depth=8, mis=True, clamp=0.0, denoise=False, target_error=0.0):
+def make_job(scene, cam, w, h, spp, s0=0, seed=0, max_depth=8, mis=True, clamp=0.0, denoise=False, target_error=0.0,
+             photons=0.0):
     """an LrhJob for a PtScene (denoise: also the denoiser's guide and moments; target_error: stop at
-    the pass count whose image mean reaches this relative standard error)"""
+    the pass count whose image mean reaches this relative standard error; photons: light paths per
+    pixel and pass traced from the lamp for the caustics)"""
     j = Job()
     j.scene = scene._h.value if isinstance(scene, PtScene) else scene
     for k in range(12):
@@ -599,6 +640,7 @@ def make_job(scene, cam, w, h, spp, s0=0, seed=0, max_depth=8, mis=True, clamp=0
     j.w, j.h, j.spp, j.s0, j.seed = w, h, spp, s0, seed
     j.max_depth, j.mis, j.clamp, j.denoise = max_depth, 1 if mis else 0, clamp, 1 if denoise else 0
     j.target_error = target_error
+    j.photons = photons
     return j
 
 

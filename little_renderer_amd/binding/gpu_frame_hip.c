@@ -18,8 +18,10 @@
  * moved instances are sent alone, found by their versions as upload_instances finds them
  * (engine/gpu/trace_gpu.c:1391-1452, scene_changed_run, engine/scene/scene.h:359-363), and a
  * dynamic mesh whose version moved on is sent and its tree rebuilt in place (:1486-1512).
- * Refused (1, the CPU renders): textures, media (volumes), vertex normals, light-traced
- * caustics (photons > 0), a scene of instances that also has world primitives or triangles.
+ * Refused (1, the CPU renders): textures, media (volumes), vertex normals, a scene of instances
+ * that also has world primitives or triangles. Light-traced caustics (photons > 0) are traced on
+ * the device: photons x w x h light paths a pass (the GPU's caustics pass, trace_gpu.c:1587-1594),
+ * the path tracer leaving their paths out (skip_caustics).
  */
 #include <math.h>
 #include <stdint.h>
@@ -386,10 +388,6 @@ int gpu_frame_trace(Gpu *gpu, GpuView *view, const Scene *s, const Camera *cam, 
     HipGpu *g = (HipGpu *)gpu;
     HipView *v = (HipView *)view;
     if (!g || !v || !s || !cam || !img || img->w <= 0 || img->h <= 0 || spp < 0) return 1;
-    if (photons > 0) {
-        fprintf(stderr, "gpu (HIP): light-traced caustics are traced on the CPU\n");
-        return 1;
-    }
     LrhPtScene *ps = scene_copy(v, s);
     if (!ps) return 1;
     const int W = img->w, H = img->h, n = W * H;
@@ -400,7 +398,18 @@ int gpu_frame_trace(Gpu *gpu, GpuView *view, const Scene *s, const Camera *cam, 
     int rc = h_acc ? lrh_buffer_alloc(g->ctx, (size_t)n * 4 * sizeof(float), &d_acc) : LRH_ENOMEM;
     /* seed 0: trace_image's for bin/render (apps/render.c:80), so -gpu on and off give the
        same samples; the passes in one call (the engine batches them) */
-    if (!rc) rc = lrh_pt_render(g->ctx, ps, c, W, H, spp, 0, 0, max_depth, mis, (float)clamp, (float *)d_acc);
+    LrhJob job;
+    memset(&job, 0, sizeof(job));
+    job.scene = ps;
+    memcpy(job.cam, c, sizeof(job.cam));
+    job.w = W;
+    job.h = H;
+    job.spp = spp;
+    job.max_depth = max_depth;
+    job.mis = mis;
+    job.clamp = (float)clamp;
+    job.photons = photons > 0 ? (float)photons : 0.0f;
+    if (!rc) rc = lrh_pt_render_job(g->ctx, &job, (float *)d_acc);
     if (!rc) rc = lrh_buffer_read(g->ctx, h_acc, d_acc, (size_t)n * 4 * sizeof(float));
     if (d_acc) lrh_buffer_free(g->ctx, d_acc);
     if (rc) {

@@ -75,6 +75,9 @@ struct JobDev {
     // a job with a target error, in its first 4 passes: each pass's luminance summed (fixed point,
     // 2^-28) into the progress block, for the host to choose the pass count from
     int pilot;
+    // light tracing on: the paths the light tracer covers (the camera saw a rough surface first,
+    // and the lamp is reached right after a mirror or glass) leave the lamp out
+    int skip_caustics;
 };
 #define LR_PILOT_PASSES 4
 #define LR_MOM2_SHIFT 20                  // the squares' fixed-point units: 2^-20
@@ -163,6 +166,11 @@ typedef void (*LrEngFn)(dim3 grid, hipStream_t s, const EngArgs &A);
 // the denoiser's guide of one job (k_guide, 256 threads, a 16-entry LDS stack)
 typedef void (*LrGuideFn)(dim3 grid, hipStream_t s, const JobDev &J, float4 *guide, uint32_t *spill, size_t stride,
                           int spill_per, const float *tab, uint32_t *overflow);
+// a job's light paths for caustics (k_light, 256 threads, a 16-entry LDS stack): npass x nlp;
+// the rays they walk added to cnt[0] (closest hit) and cnt[1] (towards the camera), and in the
+// counting form their node records and shapes to cnt[2] and cnt[3]
+typedef void (*LrLightFn)(dim3 grid, hipStream_t s, const JobDev &J, uint32_t nlp, uint32_t npass, uint32_t *spill,
+                          size_t stride, int spill_per, const float *tab, uint32_t *overflow, unsigned long long *cnt);
 
 // the kernels, by work-group size (64, 128, 256: index 0..2), LDS stack (16, 24, 32) and, for the
 // walks, whether they take two-level scenes too (0: one-level scenes only; the two-level walks'
@@ -176,6 +184,7 @@ struct LrEngTable {
     // walk's with one shadow ray a vertex)
     LrEngFn extend_rf[3][3][2], shadow_rf[3][3][2];
     LrGuideFn guide[2];   // by whether it takes two-level scenes
+    LrLightFn light[2];   // by counting (one-level scenes: only they have a lamp)
 };
 void lr_eng_fill_b64(LrEngTable &t);
 void lr_eng_fill_b128(LrEngTable &t);

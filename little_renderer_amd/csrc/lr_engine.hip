@@ -180,6 +180,10 @@ struct LrhEngine {
     std::vector<hipEvent_t> pevs;
     double kms[4] = {0, 0, 0, 0};
     uint64_t klaunch[4] = {0, 0, 0, 0};
+    // light tracing: its launches and paths, and with profiling on its device time
+    hipEvent_t lev[2] = {nullptr, nullptr};
+    uint64_t light_paths = 0, light_launches = 0;
+    double light_ms = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool graph_ok = false;
@@ -454,7 +458,31 @@ int job_check(const LrhEngine *e, const LrhJob *j)
     if (j->scene->depth > e->max_tree_depth)
         return lr_fail(LRH_EREFUSED, "engine: the scene's tree is %d deep, the engine's walks hold %d", j->scene->depth,
                        e->max_tree_depth);
+    if (!(j->photons >= 0.0f) || (j->photons > 0.0f && j->target_error > 0.0f))
+        return lr_fail(LRH_EINVAL, "engine: photons >= 0, and not with a target error");
     return 0;
+}
+
+// light tracing can add something: a lamp, and a mirror or glass to send its light on
+// (scene_has_specular, tracer.c:546-551: a material whose roughness, as its preset makes it, is
+// at most 0.02; not a lamp)
+bool caustics_possible(const LrhPtScene *s)
+{
+    if (s->light < 0) return false;
+    for (const LrhMaterial &m : s->hmats) {
+        if (m.type == LRH_MAT_LIGHT || m.type == LRH_MAT_DIFFUSE) continue;
+        if (m.type == LRH_MAT_MIRROR || m.type == LRH_MAT_GLASS) return true;
+        const float r = m.roughness < 0.0f ? 0.0f : (m.roughness > 1.0f ? 1.0f : m.roughness);
+        if (r <= 0.02f) return true;
+    }
+    return false;
+}
+
+// light paths a pass: photons per pixel (light_paths, trace_gpu.c:1589-1594)
+uint32_t light_paths(const LrhJob *j)
+{
+    const double n = (double)j->photons * j->w * j->h;
+    return n < 1.0 ? 1u : (n > 1.6e7 ? 16000000u : (uint32_t)n);
 }
 
 // admits a job: its descriptor and its image cleared; its paths are issued later, in chunks
@@ -526,6 +554,8 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
     jd.seed_hi = (uint32_t)(j->seed >> 32);
     jd.clamp = j->clamp;
     jd.pilot = h.pilot ? 1 : 0;
+    const bool caustics = j->photons > 0.0f && caustics_possible(s);
+    jd.skip_caustics = caustics ? 1 : 0;
     hipLaunchKernelGGL(k_job_put, dim3(1), dim3(1), 0, e->stream, e->jobs, e->prog->jret, e->prog->jdone, e->prog->jpass,
                        slot, jd);
     // the image cleared (and a denoised job's moments, just before it)
@@ -541,6 +571,31 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
         table().guide[s->two_level ? 1 : 0](dim3((unsigned)(threads / 256)), e->stream, jd, (float4 *)jd.guide, e->base.spill,
                                              threads, e->spill_cap, e->base.tab, e->overflow);
         ENG_HIP(hipGetLastError());
+    }
+    if (caustics) {
+        // the light paths of all its passes, added to its image now (integer sums: in any order
+        // they come to the same), before any of its paths (the same stream)
+        const uint32_t nlp = light_paths(j);
+        const size_t paths = (size_t)nlp * (size_t)j->spp;
+        const size_t threads = std::min((paths + 255) / 256 * 256, (size_t)e->threads_cap / 256 * 256);
+        if (e->profiling && !e->lev[0]) {
+            ENG_HIP(hipEventCreate(&e->lev[0]));
+            ENG_HIP(hipEventCreate(&e->lev[1]));
+        }
+        if (e->profiling) ENG_HIP(hipEventRecord(e->lev[0], e->stream));
+        table().light[e->counting ? 1 : 0](dim3((unsigned)(threads / 256)), e->stream, jd, nlp, (uint32_t)j->spp, e->base.spill, threads,
+                      e->spill_cap, e->base.tab, e->overflow, e->counts + 10);
+        ENG_HIP(hipGetLastError());
+        e->light_paths += paths;
+        e->light_launches++;
+        if (e->profiling) {
+            // (profiling waits for its events, as the iterations' do)
+            float ms = 0.0f;
+            ENG_HIP(hipEventRecord(e->lev[1], e->stream));
+            ENG_HIP(hipEventSynchronize(e->lev[1]));
+            ENG_HIP(hipEventElapsedTime(&ms, e->lev[0], e->lev[1]));
+            e->light_ms += ms;
+        }
     }
     e->by_id[h.id] = slot;
     e->running++;
@@ -760,6 +815,8 @@ void lrh_engine_destroy(LrhEngine *e)
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     for (hipEvent_t ev : e->pevs) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : e->lev)
+        if (ev) (void)hipEventDestroy(ev);
     for (int b = 0; b < 2; b++) {
         if (e->pin[b]) (void)hipHostFree(e->pin[b]);
         if (e->pev[b]) (void)hipEventDestroy(e->pev[b]);
@@ -1005,7 +1062,7 @@ int lrh_engine_stats(LrhEngine *e, LrhEngineStats *st)
     if (!e || !st) return lr_fail(LRH_EINVAL, "lrh_engine_stats: bad arguments");
     int rc = progress_flush(e);
     if (rc) return rc;
-    unsigned long long cnt[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long cnt[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     ENG_HIP(hipMemcpyAsync(cnt, e->counts, sizeof cnt, hipMemcpyDeviceToHost, e->stream));
     ENG_HIP(hipStreamSynchronize(e->stream));
     st->rays_ext = e->last.ctl.rays_ext;
@@ -1022,6 +1079,13 @@ int lrh_engine_stats(LrhEngine *e, LrhEngineStats *st)
     st->shadow_wave_shape_steps = cnt[7];
     st->uniform_node_steps = cnt[8];
     st->shadow_uniform_node_steps = cnt[9];
+    st->light_paths = e->light_paths;
+    st->light_rays = cnt[10];
+    st->light_shadow_rays = cnt[11];
+    st->light_node_fetches = cnt[12];
+    st->light_shape_tests = cnt[13];
+    st->light_launches = e->light_launches;
+    st->light_ms = e->light_ms;
     for (int k = 0; k < 4; k++) {
         st->kernel_ms[k] = e->kms[k];
         st->kernel_launches[k] = e->klaunch[k];
@@ -1134,13 +1198,15 @@ int lrh_engine_autotune(LrhEngine *e, const LrhJob *sample, int n, LrhTune *best
     return 0;
 }
 
-// lrh_pt_render: a one-job batch on the context's own engine (made, or made larger, on demand)
-int lrh_pt_render(LrhContext *c, const LrhPtScene *s, const float *cam, int W, int H, int spp, uint32_t s0, uint64_t seed,
-                  int max_depth, int mis, float clamp, float *d_accum)
+// lrh_pt_render_job: a one-job batch on the context's own engine (made, or made larger, on demand)
+int lrh_pt_render_job(LrhContext *c, const LrhJob *j, float *d_accum)
 {
-    if (!c || !s || !cam || W <= 0 || H <= 0 || spp <= 0 || max_depth < 0 || !d_accum)
-        return lr_fail(LRH_EINVAL, "lrh_pt_render: bad arguments");
-    const size_t need = lr_align((size_t)W * H * 24);
+    if (!c || !j || !j->scene || !d_accum || j->w <= 0 || j->h <= 0 ||
+        (unsigned long long)j->w * (unsigned long long)j->h >= (1ull << 32))
+        return lr_fail(LRH_EINVAL, "lrh_pt_render_job: bad arguments");
+    if (j->denoise) return lr_fail(LRH_EINVAL, "lrh_pt_render_job: a denoised job needs an engine (lrh_engine_denoise)");
+    const LrhPtScene *s = j->scene;
+    const size_t need = lr_align((size_t)j->w * j->h * 24);
     LrhEngine *e = c->engine;
     if (e && (e->max_tree_depth < s->depth || e->arena_bytes < need)) {
         lrh_engine_destroy(e);
@@ -1154,7 +1220,21 @@ int lrh_pt_render(LrhContext *c, const LrhPtScene *s, const float *cam, int W, i
     }
     // the caller's stream may still be writing d_accum or the scene
     LR_HIP(hipStreamSynchronize(c->stream));
-    LrhJob j{};   // (no denoiser inputs, no target error)
+    int64_t id;
+    int rc = lrh_engine_submit(e, j, &id);
+    if (!rc) rc = lrh_engine_drain(e);
+    if (!rc) rc = lrh_engine_image(e, id, d_accum, 1);
+    if (!rc) rc = lrh_engine_release(e, id);
+    return rc;
+}
+
+// lrh_pt_render: lrh_pt_render_job of a plain job
+int lrh_pt_render(LrhContext *c, const LrhPtScene *s, const float *cam, int W, int H, int spp, uint32_t s0, uint64_t seed,
+                  int max_depth, int mis, float clamp, float *d_accum)
+{
+    if (!c || !s || !cam || W <= 0 || H <= 0 || spp <= 0 || max_depth < 0 || !d_accum)
+        return lr_fail(LRH_EINVAL, "lrh_pt_render: bad arguments");
+    LrhJob j{};   // (no denoiser inputs, no target error, no light tracing)
     j.scene = s;
     memcpy(j.cam, cam, sizeof(j.cam));
     j.w = W;
@@ -1165,12 +1245,7 @@ int lrh_pt_render(LrhContext *c, const LrhPtScene *s, const float *cam, int W, i
     j.max_depth = max_depth;
     j.mis = mis;
     j.clamp = clamp;
-    int64_t id;
-    int rc = lrh_engine_submit(e, &j, &id);
-    if (!rc) rc = lrh_engine_drain(e);
-    if (!rc) rc = lrh_engine_image(e, id, d_accum, 1);
-    if (!rc) rc = lrh_engine_release(e, id);
-    return rc;
+    return lrh_pt_render_job(c, &j, d_accum);
 }
 
 }  // extern "C"

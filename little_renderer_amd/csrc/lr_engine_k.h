@@ -898,6 +898,9 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
             const float4 thr4 = A.thr[cur][i], rad4 = A.rad[cur][i];
             V3 thr = v3of(thr4), rad = v3of(rad4);
             float bounce_pdf = thr4.w;
+            // the emission state (low byte), the caustic chain (bit 8: the camera saw a rough surface
+            // first, light tracing on) and whether the last vertex was a mirror or glass (bit 9)
+            // (tracer.c:371-375)
             int emit = __float_as_int(rad4.w);
             const uint32_t hid = __float_as_uint(hv.y);
             const V3 d = v3of(r1), o = v3of(r0);
@@ -912,9 +915,9 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
                              J.env_rot_c, J.env_rot_s};
             if (hid == LR_NONE) {
                 // escaped: the sky (tracer.c:415-421), black without one
-                if (J.env_w > 0 && emit != EMIT_NONE) {
+                if (J.env_w > 0 && (emit & 0xff) != EMIT_NONE) {
                     V3 le = env_eval(env, d);
-                    if (emit == EMIT_MIS) le = vscale(le, power_heuristic(bounce_pdf, env_pdf(env, d)));
+                    if ((emit & 0xff) == EMIT_MIS) le = vscale(le, power_heuristic(bounce_pdf, env_pdf(env, d)));
                     rad = vadd(rad, vmul(thr, le));
                 }
             } else {
@@ -934,10 +937,10 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
                     if (rfmax(e1.x, rfmax(e1.y, e1.z)) >= 1.0f) glow = true;
                 }
                 if (type == MAT_LIGHT) {
-                    if (front) {
+                    if (front && (emit & 0x300) != 0x300) {   // (camera -> rough ... mirror/glass -> lamp: light tracing's)
                         const V3 em = v3of(J.mats[3 * mi + 1]);
-                        if (emit == EMIT_FULL) rad = vadd(rad, vmul(thr, em));
-                        else if (emit == EMIT_MIS && (int)hid == J.light) {
+                        if ((emit & 0xff) == EMIT_FULL) rad = vadd(rad, vmul(thr, em));
+                        else if ((emit & 0xff) == EMIT_MIS && (int)hid == J.light) {
                             const Prim l = load_prim(J.prims, (uint32_t)J.light);
                             const float cos_light = -vdot(v3of(l.n), d);
                             if (cos_light > 0.0f) {
@@ -952,10 +955,10 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
                     // NEE too, so weighted against that (MIS) or, without MIS, counted there already
                     const V3 em = v3of(J.mats[3 * mi + 1]);
                     if (em.x > 0.0f || em.y > 0.0f || em.z > 0.0f) {
-                        const int k = MULTI && J.nemit && emit != EMIT_FULL && ii != LR_NONE ? J.inst_emit[ii] : -1;
+                        const int k = MULTI && J.nemit && (emit & 0xff) != EMIT_FULL && ii != LR_NONE ? J.inst_emit[ii] : -1;
                         const float pe = k >= 0 && front ? emit_pdf(J, k, o, vnorm(d), t * vlen(d), nrm) : 0.0f;
                         if (!(pe > 0.0f)) rad = vadd(rad, vmul(thr, em));
-                        else if (emit == EMIT_MIS) rad = vadd(rad, vmul(thr, vscale(em, power_heuristic(bounce_pdf, pe))));
+                        else if ((emit & 0xff) == EMIT_MIS) rad = vadd(rad, vmul(thr, vscale(em, power_heuristic(bounce_pdf, pe))));
                     }
                 }
                 if (type != MAT_LIGHT && depth < J.max_depth) {
@@ -1057,10 +1060,10 @@ __global__ __launch_bounds__(BLK) void k_shade(EngArgs A)
                     BSample bs;
                     if (bsdf_sample_lobe(b, A.tab, lobe, nrm, v, da, db, bs)) {
                         thr = vmul(thr, bs.w);
-                        if (bs.delta) emit = EMIT_FULL;
+                        if (bs.delta) emit = EMIT_FULL | (emit & 0x100) | 0x200;
                         else {
                             bounce_pdf = bs.pdf;
-                            emit = J.mis ? EMIT_MIS : EMIT_NONE;
+                            emit = (J.mis ? EMIT_MIS : EMIT_NONE) | (depth == 0 ? (J.skip_caustics ? 0x100 : 0) : (emit & 0x100));
                         }
                         bool go = true;
                         if (depth >= 3) {
@@ -1277,6 +1280,145 @@ __global__ __launch_bounds__(BLK) void k_guide(JobDev J, float4 *guide, uint32_t
     }
 }
 
+// ---- light tracing for caustics (light_trace_caustics, tracer.c:511-665; ltrace, trace.wgsl:1341-1511) ----
+// One thread per light path (a grid-stride loop over npass x nlp of them): from a uniform point
+// of the lamp in a cosine-distributed direction, its power Le area pi; at every rough vertex after
+// the first it is joined to the camera by a shadow ray and added to the pixel it projects to, in
+// the job's image, over nlp (each pass's nlp paths are one estimate, added to that pass's sample
+// of the pixels); it goes on by sampling the BSDF seen from the light's side (through a surface
+// times eta^2, the adjoint of the radiance's factor), Russian roulette from depth 3. The first
+// vertex must be a mirror or glass (lamp -> rough is the path tracer's, by NEE), which leaves
+// exactly the paths the path tracer skips (lamp -> mirror/glass ... -> rough -> camera). The
+// numbers: PCG32 per light path (pass k, path i: stream i, sample index 2^31 + s0 + k). No
+// smooth normals here: the shading and geometric normals are one, the adjoint correction 1.
+template <int BLK, int STK, bool COUNT>
+__global__ __launch_bounds__(BLK) void k_light(JobDev J, uint32_t nlp, uint32_t npass, uint32_t *spill, size_t spill_stride,
+                                              int spill_per, const float *tab, uint32_t *overflow, unsigned long long *cnt)
+{
+    uint32_t n_ext = 0, n_shadow = 0;   // rays walked (cnt[0], cnt[1], summed by wave)
+    WalkCnt wc;                         // COUNT: node records and shapes (cnt[2], cnt[3])
+    __shared__ uint32_t stk[STK][BLK];
+    const size_t gtid = (size_t)blockIdx.x * BLK + threadIdx.x;
+    const WalkStack S{&stk[0][threadIdx.x], BLK, spill + gtid, spill_stride, STK + spill_per};
+    const Prim lamp = load_prim(J.prims, (uint32_t)J.light);
+    const int lmat = (int)lamp.u.w;
+    const V3 le = v3of(J.mats[3 * lmat + 1]);
+    const V3 org = v3(J.cam[0], J.cam[1], J.cam[2]), ul = v3(J.cam[3], J.cam[4], J.cam[5]);
+    const V3 hz = v3(J.cam[6], J.cam[7], J.cam[8]), vt = v3(J.cam[9], J.cam[10], J.cam[11]);
+    const float pix_area = vlen(hz) * vlen(vt) / ((float)J.W * (float)J.H);   // camera_screen_area / pixels
+    const V3 center = vadd(ul, vsub(vscale(hz, 0.5f), vscale(vt, 0.5f)));
+    const V3 fw = vsub(center, org);
+    const float inv_n = 1.0f / (float)nlp;
+    const uint64_t seed = ((uint64_t)J.seed_hi << 32) | J.seed_lo;
+    const uint64_t total = (uint64_t)nlp * npass;
+    for (uint64_t g = gtid; g < total; g += (uint64_t)gridDim.x * BLK) {
+        const uint32_t pass = (uint32_t)(g / nlp), path = (uint32_t)(g % nlp);
+        const uint64_t inc = ((uint64_t)path << 1u) | 1u;
+        uint64_t st = pcg_seed(path, 0x80000000u + J.s0 + pass, seed);
+        // emit: a point of the lamp, a direction about its normal
+        const float e1 = pcg_float(st, inc), e2 = pcg_float(st, inc);
+        V3 o = vadd(v3of(lamp.q), vadd(vscale(v3of(lamp.u), e1), vscale(v3of(lamp.v), e2)));
+        const float c1 = pcg_float(st, inc), c2 = pcg_float(st, inc);
+        V3 d = cosine_hemisphere(v3of(lamp.n), c1, c2);
+        V3 beta = vscale(le, lamp.v.w * PI_F), thr = v3(1.0f, 1.0f, 1.0f);
+        for (int depth = 0; depth < J.max_depth; depth++) {
+            const WalkHit h = walk4<false, COUNT, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, o, d, J.eps, 1e30f, S,
+                                                       overflow, wc);
+            n_ext++;
+            if (h.id == LR_NONE) break;
+            const V3 hp = vadd(o, vscale(d, h.t));
+            V3 outward;
+            int mi;
+            uint32_t ii;
+            hit_frame(J, o, d, h.t, h.id, __float_as_uint(h.b1), hp, outward, mi, ii);
+            const bool front = vdot(d, outward) < 0.0f;
+            const V3 nrm = front ? outward : vneg(outward);
+            int type;
+            Bsdf b = bsdf_make(J.mats, mi, front, tab, type);
+            if (type == MAT_LIGHT) break;
+            const V3 wi = vneg(d);
+            // 1. the rough part joined to the camera (not at the first vertex)
+            if (depth > 0 && (!b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f)) {
+                const V3 to_cam = vsub(org, hp);
+                const float dist2 = vdot(to_cam, to_cam), dist = sqrtf(dist2);
+                const V3 wo = vscale(to_cam, 1.0f / dist);
+                const float cos_o = vdot(nrm, wo);
+                // the BSDF seen from the camera: its side's normal and index
+                Bsdf bc = b;
+                int tc;
+                if (!(cos_o > 0.0f)) bc = bsdf_make(J.mats, mi, !front, tab, tc);
+                const V3 nc = cos_o > 0.0f ? nrm : vneg(nrm);
+                float pdf;
+                V3 f = v3(0.0f, 0.0f, 0.0f);
+                if (cos_o != 0.0f && vdot(nc, wo) > 0.0f) f = bsdf_eval(bc, tab, nc, wo, wi, pdf);
+                // camera_project (camera.h:42-54)
+                const V3 qv = vsub(hp, org);
+                const float z = vdot(qv, fw);
+                if (vmax3(f) > 0.0f && z > 0.0f) {
+                    const V3 e = vsub(vscale(qv, 1.0f / z), fw);
+                    const float sx = 0.5f + vdot(e, hz) / vdot(hz, hz), sy = 0.5f - vdot(e, vt) / vdot(vt, vt);
+                    const float cos_c = z / vlen(qv);
+                    if (sx >= 0.0f && sy >= 0.0f && sx < 1.0f && sy < 1.0f) {
+                        const int px = (int)(sx * (float)J.W), py = (int)(sy * (float)J.H);
+                        if (px < J.W && py < J.H) {
+                            const WalkHit sh = walk4<true, COUNT, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, hp, wo,
+                                                                       J.eps, dist * (1.0f - 1e-4f), S, overflow, wc);
+                            n_shadow++;
+                            if (sh.id == LR_NONE) {
+                                const float wgt = fabsf(cos_o) / (dist2 * pix_area * cos_c * cos_c * cos_c);
+                                const V3 c = vscale(vmul(beta, f), wgt * inv_n);
+                                long long *px3 = J.accum + 3 * ((size_t)py * J.W + px);
+                                const float cc[3] = {c.x, c.y, c.z};
+                                for (int ch = 0; ch < 3; ch++) {
+                                    if (!(cc[ch] > 0.0f)) continue;   // NaN or nothing
+                                    const long long v = to_fixed(cc[ch]);
+                                    if (v) atomicAdd((unsigned long long *)&px3[ch], (unsigned long long)v);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            // 2. on, by the BSDF seen from the light's side (bsdf_sample, brdf.h:343-348)
+            const float ulb = pcg_float(st, inc), u1 = pcg_float(st, inc), u2 = pcg_float(st, inc);
+            const float cv = vdot(wi, nrm);
+            if (!(cv > 0.0f)) break;
+            bsdf_view(b, tab, cv);
+            BSample bs;
+            if (!bsdf_sample_lobe(b, tab, bsdf_pick(b, ulb), nrm, wi, u1, u2, bs)) break;
+            if (depth == 0 && !bs.delta) break;   // lamp -> rough: the path tracer's
+            const float og = vdot(nrm, bs.l);
+            const bool through = og < 0.0f;
+            if (!bs.delta && (through ? og >= 0.0f : og <= 0.0f)) break;
+            V3 f = bs.w;
+            if (through) f = vscale(f, b.eta * b.eta);
+            beta = vmul(beta, f);
+            thr = vmul(thr, f);
+            if (depth >= 3) {
+                const float pr = rfmin(vmax3(thr), 0.95f);
+                if (pcg_float(st, inc) >= pr) break;
+                beta = vscale(beta, 1.0f / pr);
+                thr = vscale(thr, 1.0f / pr);
+            }
+            o = hp;
+            d = vnorm(bs.l);
+        }
+    }
+    unsigned long long v[4] = {n_ext, n_shadow, wc.n, wc.t};
+    for (int k = 0; k < (COUNT ? 4 : 2); k++) {
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
+        if (lane_id() == 0 && v[k]) atomicAdd(&cnt[k], v[k]);
+    }
+}
+
+template <int BLK, int STK, bool COUNT>
+void L_light(dim3 g, hipStream_t s, const JobDev &J, uint32_t nlp, uint32_t npass, uint32_t *spill, size_t stride,
+             int spill_per, const float *tab, uint32_t *overflow, unsigned long long *cnt)
+{
+    hipLaunchKernelGGL((k_light<BLK, STK, COUNT>), g, dim3(BLK), 0, s, J, nlp, npass, spill, stride, spill_per, tab, overflow,
+                       cnt);
+}
+
 template <int BLK, int STK, bool TL>
 void L_guide(dim3 g, hipStream_t s, const JobDev &J, float4 *guide, uint32_t *spill, size_t stride, int spill_per,
              const float *tab, uint32_t *overflow)
@@ -1344,6 +1486,8 @@ void fill_table(LrEngTable &t)
     if (BLK == 256) {
         t.guide[0] = L_guide<256, 16, false>;
         t.guide[1] = L_guide<256, 16, true>;
+        t.light[0] = L_light<256, 16, false>;
+        t.light[1] = L_light<256, 16, true>;
     }
     t.regen[b] = L_regen<BLK>;
     t.shade[b][0] = L_shade<BLK, false>;

@@ -291,6 +291,7 @@ extern "C" int lrh_pt_scene(LrhContext *c, const float *prims, int nprims, const
     s->nprims = nprims;
     s->ntris = ntris;
     s->nmats = nmats;
+    s->hmats.assign(mats, mats + nmats);
     s->light = light;
     s->eps = eps;
     s->margin = margin;

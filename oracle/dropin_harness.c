@@ -93,13 +93,19 @@ static void to_floats(const Image *img, float *out)
     }
 }
 
-static void cpu_image(const Scene *s, const Camera *cam, int W, int H, int spp, uint64_t seed, float *out)
+/* trace_image; photons > 0: with skip_caustics, and light_trace_caustics' W H spp photons light
+ * paths added (frame_render_rt's way, engine/app/frame.c:5-12; the light paths' seed the
+ * harness's own) */
+static void cpu_image(const Scene *s, const Camera *cam, int W, int H, int spp, uint64_t seed, double photons, float *out)
 {
     TraceParams tp = trace_default_params();
     tp.max_depth = 8;
     tp.mis = 1;
+    tp.skip_caustics = photons > 0;
     Vec3 *px = malloc(sizeof(Vec3) * (size_t)W * H);
     trace_image(s, cam, &tp, W, H, spp, seed, 0, px, NULL, NULL);
+    if (photons > 0)
+        light_trace_caustics(s, cam, W, H, (long long)((double)W * H * spp * photons), &tp, seed * 7919 + 17, px);
     for (int i = 0; i < W * H; i++) {
         out[3 * i] = (float)px[i].x;
         out[3 * i + 1] = (float)px[i].y;
@@ -108,14 +114,16 @@ static void cpu_image(const Scene *s, const Camera *cam, int W, int H, int spp, 
     free(px);
 }
 
-/* The scene's image, W x H at spp samples, by the binding (gpu: 3 floats a pixel, traced twice
+/* The scene's image, W x H at spp samples (photons > 0: with light-traced caustics, that many
+ * light paths a pixel and sample), by the binding (gpu: 3 floats a pixel, traced twice
  * on a new GPU and view each time, as two runs of bin/render: gpu2 the second) and by the
  * reference's trace_image, K times with seeds 1 .. K (cpu: K images, independent of the GPU's
  * samples, whose spread is each pixel's Monte Carlo noise). move = 1 (the instanced scene): after
  * the first images, two instances move (scene_move_instance, scene_update_instances) and the same
  * view traces again into gpu2, the CPU the moved scene. 0, 1 if the binding refused or failed,
  * -1 no scene. */
-int dropin_render(int kind, int W, int H, int spp, int move, int K, float *gpu, float *gpu2, float *cpu, int *passes)
+int dropin_render(int kind, int W, int H, int spp, int move, int K, double photons, float *gpu, float *gpu2, float *cpu,
+                  int *passes)
 {
     static Scene s;
     if (make_scene(&s, kind)) return -1;
@@ -129,7 +137,7 @@ int dropin_render(int kind, int W, int H, int spp, int move, int K, float *gpu, 
             rc = 1;
             break;
         }
-        rc = gpu_frame_trace(g, v, &s, &cam, NULL, NULL, spp, 8, 1, 0, 1, 0, &img, 0);
+        rc = gpu_frame_trace(g, v, &s, &cam, NULL, NULL, spp, 8, 1, 0, 1, photons, &img, 0);
         if (!rc) {
             to_floats(&img, run == 0 ? gpu : gpu2);
             passes[run] = gpu_frame_passes(v);
@@ -138,12 +146,13 @@ int dropin_render(int kind, int W, int H, int spp, int move, int K, float *gpu, 
             scene_move_instance(&s, 1, xform_axes(mat3_identity(), vec3(0.5, 0.5, 0.5), vec3(-0.9, 0.5, 0.8)));
             scene_move_instance(&s, 3, xform_axes(mat3_identity(), vec3(0.3, 0.6, 0.3), vec3(1.0, 0.6, 0.5)));
             scene_update_instances(&s, NULL);
-            rc = gpu_frame_trace(g, v, &s, &cam, NULL, NULL, spp, 8, 1, 0, 1, 0, &img, 0);
+            rc = gpu_frame_trace(g, v, &s, &cam, NULL, NULL, spp, 8, 1, 0, 1, photons, &img, 0);
             if (!rc) to_floats(&img, gpu2);
         }
         gpu_frame_close(g, v);
     }
-    for (int k = 0; k < K && !rc; k++) cpu_image(&s, &cam, W, H, spp, 1 + (uint64_t)k, cpu + 3 * (size_t)W * H * k);
+    for (int k = 0; k < K && !rc; k++)
+        cpu_image(&s, &cam, W, H, spp, 1 + (uint64_t)k, photons, cpu + 3 * (size_t)W * H * k);
     image_free(&img);
     scene_free(&s);
     return rc;

@@ -79,6 +79,7 @@ def lib():
             "lro_bsdf_cases": (None, [P, I, P, P]),
             "lro_pt_image": (None, [P, P, I, I, U, I, U64, F, I, I, P]),
             "lro_pt_window": (None, [P, P, I, I, I, I, I, I, U, I, U64, F, I, I, P]),
+            "lro_light_trace": (None, [P, P, I, I, U, U, I, U64, I, P]),
             "lro_env_prepare": (I, [P, P, I, I, F, F]),
             "lro_env_free": (None, [P]),
             "lro_env_eval": (None, [P, P, P]),
@@ -128,6 +129,7 @@ def ref(double=False):
         "refh_bsdf_case": (None, [P, I, P, P, P, F, F, F, P]),
         "refh_render": (None, [P, I, P, I, P, I, P, I, I, I, U64, I, I, P]),
         "refh_render_t": (None, [P, I, P, I, P, I, P, I, I, I, U64, I, I, P, P]),
+        "refh_render_lt": (None, [P, I, P, I, P, I, P, I, P, I, I, I, U64, I, I, I, ctypes.c_double, P]),
         "refh_env_new": (P, [P, I, I, F, F]),
         "refh_env_free": (None, [P]),
         "refh_env_tables": (None, [P, P, P, P, P]),
@@ -316,7 +318,7 @@ class LroScene(ctypes.Structure):
                 ("mtris", ctypes.c_void_p), ("mesh_first", ctypes.c_void_p), ("nmeshes", ctypes.c_int),
                 ("inst", ctypes.c_void_p), ("ninst", ctypes.c_int), ("mesh_box", ctypes.c_void_p),
                 ("tnodes", ctypes.c_void_p), ("taux", ctypes.c_void_p), ("emit", ctypes.c_void_p), ("nemit", ctypes.c_int),
-                ("inst_emit", ctypes.c_void_p)]
+                ("inst_emit", ctypes.c_void_p), ("skip_caustics", ctypes.c_int)]
 
 
 class Env:
@@ -435,6 +437,17 @@ def pt_image(scene, W, H, s0, ns, seed=0, clamp=0.0, max_depth=8, mis=True, tab=
     return out
 
 
+def light_trace(scene, W, H, nlp, s0, npass, seed=0, max_depth=8, tab=None, mask=None):
+    """lro_light_trace: the splats of npass passes of nlp light paths each (passes s0 ..), (H, W, 3)
+    int64 in units of 2^-28 (the engine's fixed point)"""
+    sc, keep = scene_struct(scene, tab, mask)
+    acc = np.zeros((H, W, 3), np.int64)
+    cam = np.ascontiguousarray(scene["cam"], np.float32)
+    lib().lro_light_trace(ctypes.byref(sc), ptr(cam), W, H, nlp, s0, npass, seed, max_depth, ptr(acc))
+    del keep
+    return acc
+
+
 def guide(scene, W, H, tab, mask=None):
     """lro_guide: the denoiser's guide (H, W, 8) float32: n.xyz, z | albedo.rgb, kind (tab: the
     BSDF tables, which decide a surface's kind)"""
@@ -512,7 +525,7 @@ def scene_struct(scene, tab=None, mask=None):
                   scene.get("light", -1), scene["eps"], scene.get("margin", 1e-3), ptr(mask), ptr(tab),
                   ctypes.addressof(env.s) if env is not None else None, ptr(mt), ptr(mf),
                   0 if mf is None else len(mf) - 1, ptr(inst), 0 if inst is None else len(inst), ptr(mbox),
-                  ptr(tn), ptr(ta), ptr(em), nem, ptr(slot))
+                  ptr(tn), ptr(ta), ptr(em), nem, ptr(slot), 1 if scene.get("skip_caustics") else 0)
     return sc, keep
 
 
@@ -564,6 +577,36 @@ def scene_trace(scene, rays, any_hit=False):
     lib().lro_scene_trace(ctypes.byref(sc), ptr(r), r.shape[0], 1 if any_hit else 0, ptr(out))
     del keep
     return out
+
+
+def ref_render_lt(scene, W, H, spp, seed=0, max_depth=8, trace=True, skip=False, photons=0.0, double=True):
+    """The reference's frame_render_rt with light tracing on the same scene (its double build by
+    default): trace_image's mean of spp samples (skip: skip_caustics; trace False: none), plus,
+    photons > 0, light_trace_caustics of W H spp photons light paths; (H, W, 3) float64, None
+    where oracle/_ref was not built. Spheres: the scene's prims whose q.w is 0 (not the lamp)."""
+    R = ref(double)
+    if R is None:
+        return None
+    tris = np.ascontiguousarray(scene["tris"], np.float32)
+    pr = np.ascontiguousarray(scene["prims"], np.float32).reshape(-1, 24)
+    lamp, lamp_mat = None, -1
+    if scene["light"] >= 0:
+        p = pr[scene["light"]]
+        lamp = np.concatenate([p[4:7], p[8:11], p[12:15]]).astype(np.float32)
+        lamp_mat = int(p[11])
+    sph = [np.array([*p[0:4], p[11]], np.float32) for i, p in enumerate(pr) if i != scene["light"] and p[7] < 0.5]
+    sph = np.ascontiguousarray(np.stack(sph) if sph else np.zeros((0, 5), np.float32))
+    m = np.zeros((len(scene["mats"]), 12), np.float32)
+    for i, x in enumerate(scene["mats"]):
+        m[i, 0] = x.type
+        m[i, 1:4] = list(x.albedo)
+        m[i, 4:8] = [x.ior, x.metallic, x.roughness, x.transmission]
+        m[i, 8:11] = list(x.emission)
+    cam = np.ascontiguousarray(scene["cam"], np.float32)
+    px = np.zeros((H, W, 3), np.float64)
+    R.refh_render_lt(ptr(tris), tris.shape[0], ptr(lamp), lamp_mat, ptr(sph), sph.shape[0], ptr(m), m.shape[0], ptr(cam),
+                     W, H, spp, seed, max_depth, 1 if trace else 0, 1 if skip else 0, photons, ptr(px))
+    return px
 
 
 def ref_render(scene, W, H, spp, seed=0, max_depth=8, mis=True, double=True, timed=False):

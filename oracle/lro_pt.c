@@ -1234,6 +1234,9 @@ static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis
     V radiance = v3(0, 0, 0), throughput = v3(1, 1, 1);
     int emit = EMIT_FULL;
     real bounce_pdf = 0;
+    /* skip_caustics (tracer.c:371-375): the camera saw a rough surface first (chain) and the
+       last vertex was a mirror or glass (spec): light tracing has that lamp hit */
+    int caustic_chain = 0, caustic_spec = 0;
     if (mis) d = vnorm(d);
     for (int depth = 0;; depth++) {
         Hit h;
@@ -1247,7 +1250,7 @@ static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis
         }
         const LroMat *m = &s->mats[hit_material(s, &h)];
         if (m->type == LRO_MAT_LIGHT) {
-            if (h.front) {
+            if (h.front && !(caustic_chain && caustic_spec)) {
                 if (emit == EMIT_FULL) radiance = vadd(radiance, vmul(throughput, vload(m->emission)));
                 else if (emit == EMIT_MIS && h.prim == s->light) {
                     const LroPrim *l = &s->prims[s->light];
@@ -1299,10 +1302,14 @@ static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis
         BSample bs;
         if (!bsdf_sample_lobe(&b, lobe, h.n, v, da, db, &bs)) break;
         throughput = vmul(throughput, bs.w);
-        if (bs.delta) emit = EMIT_FULL;
-        else {
+        if (bs.delta) {
+            emit = EMIT_FULL;
+            caustic_spec = 1;
+        } else {
             bounce_pdf = bs.pdf;
             emit = mis ? EMIT_MIS : EMIT_NONE;
+            if (depth == 0) caustic_chain = s->skip_caustics;
+            caustic_spec = 0;
         }
         if (depth >= 3) {                                    /* russian_roulette_u (tracer.c:295-301) */
             real pr = rfmin(vmax(throughput), 0.95f);
@@ -1313,6 +1320,115 @@ static V trace_path(const LroScene *s, V o, V d, Smp *sm, int max_depth, int mis
         d = mis ? vnorm(bs.l) : bs.l;
     }
     return radiance;
+}
+
+/* the engine's fixed point: a sample's radiance in units of 2^-28, saturated at 2^24, rounded to
+   the nearest (even) */
+static long long to_fixed(real x)
+{
+    x = x > 16777216.0f ? 16777216.0f : (x < -16777216.0f ? -16777216.0f : x);
+    return llrint((double)(x * (real)268435456.0f));
+}
+
+/* light_trace_caustics (tracer.c:511-665) as the engine issues it: npass passes of nlp light
+ * paths each (pass k, path i: the PCG32 stream lro_pt_seed(i, 2^31 + s0 + k, seed)). A path
+ * starts at a uniform point of the lamp (two numbers) in a cosine-distributed direction about
+ * its normal (two more), power Le area pi; the first vertex must be a mirror or glass; at every
+ * rough vertex after it the path is joined to the camera (camera_project, camera.h:48-59) by a
+ * shadow ray and c = beta f |cos_o| / (dist^2 pixel_area cos_c^3) / nlp is added to the pixel it
+ * lands in, each channel in the fixed point above (atomic integer sums: any order gives the same
+ * acc, H x W x 3); then on by the BSDF seen from the light's side (three numbers), through a
+ * surface times eta^2, Russian roulette from depth 3 (one more number). One normal per hit: the
+ * adjoint correction of smooth normals is 1. No media. */
+void lro_light_trace(const LroScene *s, const float cam[12], int W, int H, uint32_t nlp, uint32_t s0, int npass,
+                     uint64_t seed, int max_depth, long long *acc)
+{
+    if (s->light < 0 || nlp == 0 || npass <= 0) return;
+    const LroPrim *lamp = &s->prims[s->light];
+    const V le = vload(s->mats[(int)lamp->u[3]].emission);
+    const V org = vload(cam), ul = vload(cam + 3), hz = vload(cam + 6), vt = vload(cam + 9);
+    const real pix_area = vlen(hz) * vlen(vt) / ((real)W * (real)H);
+    const V center = vadd(ul, vsub(vscale(hz, 0.5f), vscale(vt, 0.5f)));
+    const V fw = vsub(center, org);
+    const real inv_n = 1.0f / (real)nlp;
+    const long long total = (long long)nlp * npass;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (long long g = 0; g < total; g++) {
+        const uint32_t pass = (uint32_t)(g / nlp), path = (uint32_t)(g % nlp);
+        LroRng rng;
+        lro_pt_seed(&rng, path, 0x80000000u + s0 + pass, seed);
+        const real e1 = lro_rng_float(&rng), e2 = lro_rng_float(&rng);
+        V o = vadd(vload(lamp->q), vadd(vscale(vload(lamp->u), e1), vscale(vload(lamp->v), e2)));
+        const real c1 = lro_rng_float(&rng), c2 = lro_rng_float(&rng);
+        V d = cosine_hemisphere(vload(lamp->n), c1, c2);
+        V beta = vscale(le, lamp->v[3] * (real)3.14159265358979323846), thr = v3(1, 1, 1);
+        for (int depth = 0; depth < max_depth; depth++) {
+            Hit h;
+            if (!scene_intersect(s, o, d, s->eps, 1e30f, &h)) break;
+            const LroMat *m = &s->mats[hit_material(s, &h)];
+            if (m->type == LRO_MAT_LIGHT) break;
+            Bsdf b = bsdf_make(m, h.front, s->tab);
+            const V wi = vneg(d);
+            /* 1. the rough part joined to the camera (lamp -> rough at the first vertex: NEE's) */
+            if (depth > 0 && rough_part(&b)) {
+                const V to_cam = vsub(org, h.p);
+                const real dist2 = vdot(to_cam, to_cam), dist = sqrtf(dist2);
+                const V wo = vscale(to_cam, 1.0f / dist);
+                const real cos_o = vdot(h.n, wo);
+                /* the BSDF seen from the camera: its side's normal and index */
+                Bsdf bc = cos_o > 0 ? b : bsdf_make(m, !h.front, s->tab);
+                const V nc = cos_o > 0 ? h.n : vneg(h.n);
+                real pdf;
+                V f = v3(0, 0, 0);
+                if (cos_o != 0 && vdot(nc, wo) > 0) f = bsdf_eval(&bc, nc, wo, wi, &pdf);
+                const V qv = vsub(h.p, org);
+                const real z = vdot(qv, fw);
+                if (vmax(f) > 0 && z > 0) {
+                    const V e = vsub(vscale(qv, 1.0f / z), fw);
+                    const real sx = 0.5f + vdot(e, hz) / vdot(hz, hz), sy = 0.5f - vdot(e, vt) / vdot(vt, vt);
+                    const real cos_c = z / vlen(qv);
+                    if (sx >= 0 && sy >= 0 && sx < 1 && sy < 1) {
+                        const int px = (int)(sx * (real)W), py = (int)(sy * (real)H);
+                        if (px < W && py < H && !scene_occluded(s, h.p, wo, s->eps, dist * (1.0f - 1e-4f))) {
+                            const real wgt = fabsf(cos_o) / (dist2 * pix_area * cos_c * cos_c * cos_c);
+                            const V c = vscale(vmul(beta, f), wgt * inv_n);
+                            const real cc[3] = {c.x, c.y, c.z};
+                            long long *a = acc + 3 * ((size_t)py * W + px);
+                            for (int ch = 0; ch < 3; ch++) {
+                                if (!(cc[ch] > 0)) continue;
+                                const long long v = to_fixed(cc[ch]);
+#pragma omp atomic
+                                a[ch] += v;
+                            }
+                        }
+                    }
+                }
+            }
+            /* 2. on, by the BSDF seen from the light's side (bsdf_sample, brdf.h:343-348) */
+            const real ulb = lro_rng_float(&rng), u1 = lro_rng_float(&rng), u2 = lro_rng_float(&rng);
+            const real cv = vdot(wi, h.n);
+            if (!(cv > 0)) break;
+            bsdf_view(&b, cv);
+            BSample bs;
+            if (!bsdf_sample_lobe(&b, bsdf_pick(&b, ulb), h.n, wi, u1, u2, &bs)) break;
+            if (depth == 0 && !bs.delta) break;
+            const real og = vdot(h.n, bs.l);
+            const int through = og < 0;   /* the sample's own flag, with one normal a hit */
+            if (!bs.delta && (through ? og >= 0 : og <= 0)) break;
+            V f = bs.w;
+            if (through) f = vscale(f, b.eta * b.eta);
+            beta = vmul(beta, f);
+            thr = vmul(thr, f);
+            if (depth >= 3) {
+                const real pr = rfmin(vmax(thr), 0.95f);
+                if (lro_rng_float(&rng) >= pr) break;
+                beta = vscale(beta, 1.0f / pr);
+                thr = vscale(thr, 1.0f / pr);
+            }
+            o = h.p;
+            d = vnorm(bs.l);
+        }
+    }
 }
 
 /* camera_ray (camera.h:38-42) */

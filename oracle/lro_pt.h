@@ -89,6 +89,9 @@ typedef struct {
     /* the glowing shapes (Scene.emitters) and each instance's slot among them (-1: none) */
     const LroEmitter *emit; int nemit;
     const int *inst_emit;
+    /* light tracing on (tracer.h:20): the path tracer leaves out the lamp reached right after a
+       mirror or glass on a path whose first vertex was rough (light tracing adds those) */
+    int skip_caustics;
 } LroScene;
 
 /* scene_find_emitters + scene_emitter (engine/scene/scene.c:1000-1031): the instances that are
@@ -121,6 +124,10 @@ void lro_pt_seed(LroRng *rng, uint64_t pixel, uint32_t k, uint64_t seed);
 /* adds samples s0 .. s0+ns-1 of every pixel (rgb sums) into sum */
 void lro_pt_image(const LroScene *s, const float cam[12], int W, int H, uint32_t s0, int ns, uint64_t seed, float clamp,
                   int max_depth, int mis, float *sum);
+/* light tracing for caustics: npass passes of nlp light paths, their splats (fixed point, 2^-28)
+ * added to acc (H x W x 3 int64) */
+void lro_light_trace(const LroScene *s, const float cam[12], int W, int H, uint32_t nlp, uint32_t s0, int npass,
+                     uint64_t seed, int max_depth, long long *acc);
 /* the denoiser's guide of a W x H image (8 floats a pixel: n.xyz, z | albedo.rgb, kind) */
 void lro_guide(const LroScene *s, const float cam[12], int W, int H, float *out);
 /* lro_pt_image, and each pixel's moments of its samples' irradiance luminance (2 floats) */

@@ -254,6 +254,62 @@ void refh_render_t(const float *tris, int ntris, const float *lamp, int lamp_mat
     scene_free(&s);
 }
 
+/* frame_render_rt (frame.c:5-12) with light tracing for caustics: the scene of refh_render plus
+ * spheres (centre xyz, radius, material: 5 floats each); trace_image's mean of spp samples per
+ * pixel with tp.skip_caustics = skip (trace: 0 leaves it out), then, photons > 0,
+ * light_trace_caustics with W H spp photons light paths added (its own seed drawn from seed).
+ * px: (H, W, 3) doubles. */
+void refh_render_lt(const float *tris, int ntris, const float *lamp, int lamp_mat, const float *spheres, int nspheres,
+                    const float *mats, int nmats, const float *cam, int W, int H, int spp, uint64_t seed, int max_depth,
+                    int trace, int skip, double photons, double *px)
+{
+    Scene s;
+    memset(&s, 0, sizeof(s));
+    for (int i = 0; i < nmats; i++) {
+        const float *m = mats + 12 * (size_t)i;
+        Material mat;
+        memset(&mat, 0, sizeof(mat));
+        mat.type = (MatType)(int)m[0];
+        mat.albedo = vec3(m[1], m[2], m[3]);
+        mat.ior = m[4]; mat.metallic = m[5]; mat.roughness = m[6]; mat.transmission = m[7];
+        mat.emission = vec3(m[8], m[9], m[10]);
+        scene_add_material(&s, mat);
+    }
+    s.light = -1;
+    if (lamp)
+        s.light = scene_add_prim(&s, make_quad(vec3(lamp[0], lamp[1], lamp[2]), vec3(lamp[3], lamp[4], lamp[5]),
+                                               vec3(lamp[6], lamp[7], lamp[8]), lamp_mat));
+    for (int i = 0; i < nspheres; i++) {
+        const float *q = spheres + 5 * (size_t)i;
+        scene_add_prim(&s, make_sphere(vec3(q[0], q[1], q[2]), q[3], (int)q[4]));
+    }
+    for (int i = 0; i < ntris; i++) {
+        const float *t = tris + 12 * (size_t)i;
+        uint32_t w;
+        memcpy(&w, t + 11, 4);
+        Vec3 a = vec3(t[0], t[1], t[2]);
+        Vec3 b = v_add(a, vec3(t[4], t[5], t[6])), c = v_add(a, vec3(t[8], t[9], t[10]));
+        scene_add_triangle(&s, a, b, c, NULL, (int)(w >> 26));
+    }
+    AccelParams ap = accel_default_params();
+    scene_set_accel(&s, ACCEL_BVH, &ap);
+    Camera c;
+    c.origin = vec3(cam[0], cam[1], cam[2]);
+    c.upper_left = vec3(cam[3], cam[4], cam[5]);
+    c.horizontal = vec3(cam[6], cam[7], cam[8]);
+    c.vertical = vec3(cam[9], cam[10], cam[11]);
+    TraceParams tp = trace_default_params();
+    tp.max_depth = max_depth;
+    tp.skip_caustics = skip;
+    Vec3 *out = calloc((size_t)W * H, sizeof(Vec3));
+    if (trace) trace_image(&s, &c, &tp, W, H, spp, seed, 0, out, NULL, NULL);
+    if (photons > 0)
+        light_trace_caustics(&s, &c, W, H, (long long)((double)W * H * spp * photons), &tp, seed ^ 0xC0FFEE1234ULL, out);
+    for (int i = 0; i < W * H; i++) { px[3 * i] = out[i].x; px[3 * i + 1] = out[i].y; px[3 * i + 2] = out[i].z; }
+    free(out);
+    scene_free(&s);
+}
+
 /* ---- the reference's sky (engine/scene/env.c), for pinning the oracle's ---- */
 #include "scene/env.h"
 

@@ -13,6 +13,8 @@ engine/app/gpu_frame.c into oracle/_ref/libref_dropin.so with oracle/dropin_harn
   mirrors make pixels' samples heavy-tailed (caustics), so z is not Student's t there: the share
   of |z| > 3 is held to 1.5x the median share that each CPU image shows against the other K - 1
   (what two runs of the same estimator show), and the image means agree within 4 standard errors.
+- With light-traced caustics (gpu_frame_trace's photons): against the reference's CPU
+  trace_image with skip_caustics plus its light_trace_caustics, the same way.
 - Twice on a new GPU and view: the same image to the last bit; exactly spp samples.
 - A scene of instances, two instances moved (scene_move_instance + scene_update_instances): the
   binding's kept copy is brought up to date by the instances' versions and equals the CPU's image
@@ -44,7 +46,7 @@ def L():
         pytest.skip("oracle/_ref/libref_dropin.so was not built (the reference was not mounted at build time)")
     lib = ctypes.CDLL(DROPIN)
     P, I = ctypes.c_void_p, ctypes.c_int
-    lib.dropin_render.argtypes = [I, I, I, I, I, I, P, P, P, P]
+    lib.dropin_render.argtypes = [I, I, I, I, I, I, ctypes.c_double, P, P, P, P]
     lib.dropin_render.restype = I
     lib.dropin_queries.argtypes = [I, I, ctypes.c_uint64, P, P]
     lib.dropin_queries.restype = I
@@ -58,11 +60,11 @@ def ptr(a):
 K = 16
 
 
-def render(L, kind, W, H, spp, move=0):
+def render(L, kind, W, H, spp, move=0, photons=0.0):
     gpu, gpu2 = (np.zeros((H, W, 3), np.float32) for _ in range(2))
     cpu = np.zeros((K, H, W, 3), np.float32)
     passes = np.zeros(2, np.int32)
-    rc = L.dropin_render(kind, W, H, spp, move, K, ptr(gpu), ptr(gpu2), ptr(cpu), ptr(passes))
+    rc = L.dropin_render(kind, W, H, spp, move, K, photons, ptr(gpu), ptr(gpu2), ptr(cpu), ptr(passes))
     assert rc == 0, f"the binding refused or failed (rc {rc})"
     return gpu, gpu2, cpu, passes
 
@@ -99,6 +101,18 @@ def test_gpu_frame_trace_equals_the_reference_cpu_image(L, kind):
     assert np.array_equal(gpu, gpu2), "traced twice on a new GPU and view: not the same image"
     assert gpu.mean() > 0
     compare(gpu, cpu, "gpu_frame_trace vs trace_image")
+
+
+def test_gpu_frame_trace_with_caustics_equals_the_reference_cpu_image(L):
+    # gpu_frame_trace's photons: the device's light tracing against the reference's CPU
+    # trace_image with skip_caustics plus its light_trace_caustics, on its mirror and glass balls
+    W = H = 64
+    spp, photons = 64, 1.0
+    say(f"drop-in: scene 1 {W}x{H} spp {spp}, {photons} light paths a pixel and sample")
+    gpu, gpu2, cpu, passes = render(L, 1, W, H, spp, photons=photons)
+    assert list(passes) == [spp, spp]
+    assert np.array_equal(gpu, gpu2), "traced twice on a new GPU and view: not the same image"
+    compare(gpu, cpu, "gpu_frame_trace with caustics vs trace_image + light_trace_caustics")
 
 
 def test_instanced_scene_under_the_sky_and_moved_instances(L):
