@@ -757,13 +757,13 @@ __device__ __forceinline__ uint32_t nearest_in(int i, int base, uint32_t nc, con
 // into LDS, finds the nearest of positions +- R, marks who stays and who merges, takes its
 // place in the next list from the tiles before it (look-back), writes the merged nodes and
 // its part of the next list. Blocks loop over tiles, so any grid size covers the round.
-__global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t small, uint32_t node0, uint32_t ref0,
+__global__ __launch_bounds__(BLOCK, 7) void k_ploc_round(int round, int n, uint32_t small, uint32_t node0, uint32_t ref0,
                                                         float4 *bx, uint32_t *list0, uint32_t *list1, LrNode *nodes,
                                                         uint32_t *height, PlocState *st, uint64_t *status0, uint64_t *status1)
 {
     __shared__ float4 wlo[PLOC_WIN], whi[PLOC_WIN];
     __shared__ uint32_t wid[PLOC_WIN];
-    __shared__ uint32_t snn[PLOC_NN];
+    __shared__ int8_t snn[PLOC_NN];   // the nearest as an offset from the position (within +-R; 0: none)
     __shared__ uint32_t s_wsum[BLOCK / 64];
     __shared__ uint32_t s_tile, s_ek, s_em;
     const int tid = threadIdx.x;
@@ -792,7 +792,8 @@ __global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t
         __syncthreads();
         for (int q = tid; q < PLOC_NN; q += BLOCK) {
             const int i = base - PLOC_R + q;
-            snn[q] = (i >= 0 && i < (int)nc) ? nearest_in(i, base, nc, wlo, whi) : LR_NONE;
+            const uint32_t bj = (i >= 0 && i < (int)nc) ? nearest_in(i, base, nc, wlo, whi) : LR_NONE;
+            snn[q] = bj == LR_NONE ? (int8_t)0 : (int8_t)((int)bj - i);
         }
         __syncthreads();
         // keep_merge (trace_ploc.wgsl:130-145): PLOC_PER adjacent positions a thread, so that the
@@ -802,9 +803,10 @@ __global__ __launch_bounds__(BLOCK) void k_ploc_round(int round, int n, uint32_t
         for (int q = 0; q < PLOC_PER; q++) {
             const int o = PLOC_PER * tid + q, i = base + o;
             const bool valid = i < (int)nc;
-            j[q] = valid ? snn[o + PLOC_R] : LR_NONE;
+            const int dj = valid ? (int)snn[o + PLOC_R] : 0;
+            j[q] = dj != 0 ? (uint32_t)(i + dj) : LR_NONE;
             bool mu = false;
-            if (valid && j[q] < nc) mu = snn[(int)j[q] - base + PLOC_R] == (uint32_t)i;
+            if (j[q] < nc) mu = (int)j[q] + (int)snn[(int)j[q] - base + PLOC_R] == i;
             keep[q] = valid && !(mu && (uint32_t)i > j[q]);
             merge[q] = valid && mu && (uint32_t)i < j[q];
             kc += keep[q];
