@@ -795,7 +795,7 @@ def bench_instanced(a, torch, lr):
             eng.pump(1)
         eng.release(i)
 
-    rigid, phys = None, {"ms": [], "host_ms": [], "contacts": [], "colours": []}
+    rigid, phys = None, {"ms": [], "host_ms": [], "contacts": [], "colours": [], "phases": []}
     if a.physics:
         # the dynamic instances as rigid bodies dropped over the ground among the props (fixed):
         # each frame one rigid-body step, and their poses are the frame's moved records
@@ -813,6 +813,7 @@ def bench_instanced(a, torch, lr):
             phys["ms"].append(rs.ms)
             phys["contacts"].append(rs.contacts)
             phys["colours"].append(rs.colours)
+            phys["phases"].append(list(rs.phase_ms))
             phys["host_ms"].append((time.perf_counter() - th) * 1e3 - rs.ms)
             st = scenes[si].move(moving, recs, builder=builder)
         else:
@@ -883,6 +884,8 @@ def bench_instanced(a, torch, lr):
             "step_ms": round(float(np.mean(phys["ms"][warm:])), 3),
             "host_ms": round(float(np.mean(phys["host_ms"][warm:])), 3),
             "contacts": int(np.mean(phys["contacts"][warm:])), "colours": int(np.max(phys["colours"][warm:])),
+            "phase_ms": dict(zip(["pairs", "contacts", "colours", "solver"],
+                                 [round(float(x), 3) for x in np.mean(phys["phases"][warm:], 0)])),
             "note": "one rigid-body step (1/60 s, 4 substeps) a frame on the device, then the moving instances' "
                     "records and the top level (poses read back, as the reference's physics writes Instance.xform)"},
         "records_sent_per_frame": int(len(moving)),

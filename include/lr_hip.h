@@ -300,6 +300,7 @@ typedef struct {
     uint64_t steps;
     int contacts, pairs, colours;   /* the last step's */
     float ms;                       /* device time per step of the last lrh_rigid_steps */
+    float phase_ms[4];              /* the last step's phases (device clock): pairs, contacts, colours, solver */
 } LrhRigidStats;
 /* rigid_init (rigid.c:122-132). max_contacts bounds a step's contacts and its listed pairs
  * (LRH_EREFUSED past them). Own device memory; steps on the context's stream. */

@@ -91,7 +91,7 @@ class RigidParams(ctypes.Structure):
 
 class RigidStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_uint64), ("contacts", ctypes.c_int), ("pairs", ctypes.c_int),
-                ("colours", ctypes.c_int), ("ms", ctypes.c_float)]
+                ("colours", ctypes.c_int), ("ms", ctypes.c_float), ("phase_ms", ctypes.c_float * 4)]
 
 
 BODY_SPHERE, BODY_BOX = 0, 1

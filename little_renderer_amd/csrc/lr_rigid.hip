@@ -152,7 +152,9 @@ struct RigidDev {
     int *order;                  // the contacts by colour
     int *cstart;                 // each colour's first place in order (RB_COLOURS + 1)
     int *prev_first;             // last step's first contact of each body (-1: none)
-    int *counters;               // [0] contacts, [1] pairs, [2] colours, [3] overflow flags, [4] last step's contacts
+    int *counters;               // [0] contacts, [1] pairs, [2] colours, [3] overflow flags, [4] last step's contacts;
+                                 // from [8]: the step's phases' device clock (5 x 64 bits: start, pairs, contacts,
+                                 // colours, end)
 };
 
 struct RigidParamsDev {
@@ -510,6 +512,69 @@ __device__ __forceinline__ float eff_mass(float ma, float mb, F3 rad, F3 iad, F3
     return k > 0.0f ? 1.0f / k : 0.0f;
 }
 
+// ---- the broad phase over the whole chip (broad_phase and test_pair, rigid.c:826-900) ----
+// 1. each body's bounding sphere grown by its motion in the step; the step's per-body state reset;
+// the step's first clock
+constexpr int RB_WIDE = 256;
+__global__ __launch_bounds__(RB_WIDE) void k_rigid_prep(RigidDev D, int nb, float dt)
+{
+    const int i = blockIdx.x * RB_WIDE + threadIdx.x;
+    if (i == 0) reinterpret_cast<unsigned long long *>(D.counters + 8)[0] = wall_clock64();
+    if (i >= nb) return;
+    const float4 h = D.h[i];
+    const float r = body_radius(h);
+    D.rad[i] = r;
+    const float grow = 0.5f * SPECULATIVE_MIN + dt * (len(f3of(D.v[i])) + len(f3of(D.w[i])) * r);
+    D.sph[i] = f4(f3of(D.p[i]), r + grow);
+    D.resting[i] = 0;
+    D.prev_first[i] = -1;
+}
+
+// whether bodies i < j make a pair: their grown spheres meet and their margin lets them touch
+// (test_pair); two fixed bodies never do
+__device__ __forceinline__ bool rigid_pair(const RigidDev &D, int i, int j, float4 si, bool fi, float dt)
+{
+    if (fi && D.p[j].w == 0.0f) return false;
+    const float4 sj = D.sph[j];
+    const F3 d = f3(si.x - sj.x, si.y - sj.y, si.z - sj.z);
+    const float rr = (si.w + sj.w) * 1.0001f + 1e-4f;
+    if (dot(d, d) > rr * rr) return false;
+    const float r = D.rad[i] + D.rad[j] + pair_margin(D, i, j, dt);
+    return dot(d, d) <= r * r;
+}
+
+// 2. (EMIT false) each row's pairs counted into rowc[i]; (EMIT true) written from the row's first
+// place, in j order: one wave a row, 64 j at a time, each lane's place from the ballot
+template <bool EMIT>
+__global__ __launch_bounds__(RB_WIDE) void k_rigid_rows(RigidDev D, int nb, float dt, int max_pairs)
+{
+    const int i = blockIdx.x * (RB_WIDE / 64) + (int)(threadIdx.x / 64);
+    if (i >= nb) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const float4 si = D.sph[i];
+    const bool fi = D.p[i].w == 0.0f;
+    int at = EMIT ? D.rowc[i] : 0;
+    for (int j0 = i + 1; j0 < nb; j0 += 64) {
+        const int j = j0 + lane;
+        const bool hit = j < nb && rigid_pair(D, i, j, si, fi, dt);
+        const uint64_t b = __ballot(hit);
+        if (EMIT && hit) {
+            const int q = at + __popcll(b & lanemask_lt());
+            if (q < max_pairs) D.pairs[q] = make_int2(i, j);
+        }
+        at += __popcll(b);
+    }
+    if (!EMIT && lane == 0) D.rowc[i] = at;
+}
+
+// 3. the rows' counts to their first places (in order), the total to counters[5]
+__global__ __launch_bounds__(RB_THREADS) void k_rigid_scan(RigidDev D, int nb)
+{
+    __shared__ int s_w[RB_THREADS / 64];
+    const int np = scan_array(D.rowc, nb, s_w);
+    if (threadIdx.x == 0) D.counters[5] = np;
+}
+
 // The step (rigid_steps, rigid.c:1371-1545), one work-group. cur / prev: this step's contacts
 // and the last one's.
 __global__ __launch_bounds__(RB_THREADS) void k_rigid_step(RigidDev D, RigidParamsDev P, int nb, int cur, float dt, int substeps,
@@ -522,38 +587,13 @@ __global__ __launch_bounds__(RB_THREADS) void k_rigid_step(RigidDev D, RigidPara
     const int tid = threadIdx.x;
     const RigidDev::Contacts &C = D.c[cur], &Q = D.c[1 - cur];
     const int nprev = first_step ? 0 : D.counters[4];
-    if (tid == 0) D.counters[3] = 0;
-
-    // ---- the bodies' bounding spheres, grown by their motion in the step (broad_phase) ----
-    for (int i = tid; i < nb; i += RB_THREADS) {
-        const float4 h = D.h[i];
-        const float r = body_radius(h);
-        D.rad[i] = r;
-        const float grow = 0.5f * SPECULATIVE_MIN + dt * (len(f3of(D.v[i])) + len(f3of(D.w[i])) * r);
-        D.sph[i] = f4(f3of(D.p[i]), r + grow);
-        D.resting[i] = 0;
-        D.prev_first[i] = -1;
+    unsigned long long *clk = reinterpret_cast<unsigned long long *>(D.counters + 8);
+    // the pairs: listed by the broad-phase kernels before it, rowc each row's first place
+    const int np = D.counters[5];
+    if (tid == 0) {
+        D.counters[3] = 0;
+        clk[1] = wall_clock64();
     }
-    __syncthreads();
-    // ---- pairs (i < j) whose grown spheres meet and whose margin lets them touch (test_pair) ----
-    for (int i = tid; i < nb; i += RB_THREADS) {
-        const float4 si = D.sph[i];
-        const bool fi = D.p[i].w == 0.0f;
-        int c = 0;
-        for (int j = i + 1; j < nb; j++) {
-            if (fi && D.p[j].w == 0.0f) continue;
-            const float4 sj = D.sph[j];
-            const F3 d = f3(si.x - sj.x, si.y - sj.y, si.z - sj.z);
-            const float rr = (si.w + sj.w) * 1.0001f + 1e-4f;
-            if (dot(d, d) > rr * rr) continue;
-            const float r = D.rad[i] + D.rad[j] + pair_margin(D, i, j, dt);
-            if (dot(d, d) > r * r) continue;
-            c++;
-        }
-        D.rowc[i] = c;
-    }
-    __syncthreads();
-    const int np = scan_array(D.rowc, nb, s_w);   // rowc: each row's first pair
     if (np > max_pairs) {
         if (tid == 0) {
             D.counters[1] = np;
@@ -561,22 +601,6 @@ __global__ __launch_bounds__(RB_THREADS) void k_rigid_step(RigidDev D, RigidPara
         }
         return;
     }
-    for (int i = tid; i < nb; i += RB_THREADS) {
-        const float4 si = D.sph[i];
-        const bool fi = D.p[i].w == 0.0f;
-        int at = D.rowc[i];
-        for (int j = i + 1; j < nb; j++) {
-            if (fi && D.p[j].w == 0.0f) continue;
-            const float4 sj = D.sph[j];
-            const F3 d = f3(si.x - sj.x, si.y - sj.y, si.z - sj.z);
-            const float rr = (si.w + sj.w) * 1.0001f + 1e-4f;
-            if (dot(d, d) > rr * rr) continue;
-            const float r = D.rad[i] + D.rad[j] + pair_margin(D, i, j, dt);
-            if (dot(d, d) > r * r) continue;
-            D.pairs[at++] = make_int2(i, j);
-        }
-    }
-    __syncthreads();
     // ---- the items' contacts into their slots: item i + rowc[i] is body i's walls, item i + 1 +
     // p pair p (of row i): the reference's order (contacts_of, rigid.c:929-946) ----
     const int nitems = nb + np;
@@ -690,6 +714,7 @@ __global__ __launch_bounds__(RB_THREADS) void k_rigid_step(RigidDev D, RigidPara
         }
     }
     __syncthreads();
+    if (tid == 0) clk[2] = wall_clock64();
     // ---- colours: contact k the first colour neither of its moving bodies has yet, in order ----
     for (int i = tid; i < nb; i += RB_THREADS) mask[i][0] = mask[i][1] = 0ull;
     for (int k = tid; k < RB_COLOURS; k += RB_THREADS) s_cc[k] = 0;
@@ -769,6 +794,7 @@ __global__ __launch_bounds__(RB_THREADS) void k_rigid_step(RigidDev D, RigidPara
     }
     __syncthreads();
     const int ncol = s_total;
+    if (tid == 0) clk[3] = wall_clock64();
     // every pass over the contacts: colour after colour, a colour's contacts in parallel
     auto by_colour = [&](auto &&f) {
         for (int c = 0; c < ncol; c++) {
@@ -895,7 +921,10 @@ __global__ __launch_bounds__(RB_THREADS) void k_rigid_step(RigidDev D, RigidPara
             if (C.p[k].w != 0.0f) solve_contact(D, C, k, C.m[k].w);
         });
     }
-    if (tid == 0) D.counters[4] = nc;
+    if (tid == 0) {
+        D.counters[4] = nc;
+        clk[4] = wall_clock64();
+    }
 }
 
 }  // namespace
@@ -959,7 +988,7 @@ extern "C" int lrh_rigid_create(LrhContext *c, int max_bodies, int max_contacts,
         std::vector<size_t> cs = {K * 8, K * 16, K * 16, K * 16, K * 16, K * 16, K * 16, K * 16, K * 16 * 9, K * 16, K * 4};
         sz.insert(sz.end(), cs.begin(), cs.end());
     }
-    const std::vector<size_t> tail = {K * 4, (RB_COLOURS + 1) * 4, B * 4, 64};
+    const std::vector<size_t> tail = {K * 4, (RB_COLOURS + 1) * 4, B * 4, 128};
     sz.insert(sz.end(), tail.begin(), tail.end());
     size_t total = 0;
     for (size_t z : sz) total += lr_align(z);
@@ -1182,7 +1211,14 @@ extern "C" int lrh_rigid_steps(LrhRigid *r, float dt, int substeps, int nsteps)
     LR_HIP(hipEventRecord(c->ev[0], c->stream));
     for (int s = 0; s < nsteps; s++) {
         r->cur = 1 - r->cur;
-        hipLaunchKernelGGL(k_rigid_step, dim3(1), dim3(RB_THREADS), 0, c->stream, r->D, P, r->n, r->cur, dt, substeps,
+        const int nb = r->n;
+        hipLaunchKernelGGL(k_rigid_prep, dim3((nb + RB_WIDE - 1) / RB_WIDE), dim3(RB_WIDE), 0, c->stream, r->D, nb, dt);
+        hipLaunchKernelGGL(k_rigid_rows<false>, dim3((nb + RB_WIDE / 64 - 1) / (RB_WIDE / 64)), dim3(RB_WIDE), 0, c->stream,
+                           r->D, nb, dt, r->max_pairs);
+        hipLaunchKernelGGL(k_rigid_scan, dim3(1), dim3(RB_THREADS), 0, c->stream, r->D, nb);
+        hipLaunchKernelGGL(k_rigid_rows<true>, dim3((nb + RB_WIDE / 64 - 1) / (RB_WIDE / 64)), dim3(RB_WIDE), 0, c->stream,
+                           r->D, nb, dt, r->max_pairs);
+        hipLaunchKernelGGL(k_rigid_step, dim3(1), dim3(RB_THREADS), 0, c->stream, r->D, P, nb, r->cur, dt, substeps,
                            r->max_pairs, r->max_contacts, r->stats.steps == 0 && s == 0 ? 1 : 0);
         LR_HIP(hipGetLastError());
     }
@@ -1190,8 +1226,16 @@ extern "C" int lrh_rigid_steps(LrhRigid *r, float dt, int substeps, int nsteps)
     LR_HIP(hipEventSynchronize(c->ev[1]));
     float ms = 0.0f;
     LR_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    int cnt[5];
+    int cnt[18];
     LR_HIP(hipMemcpy(cnt, r->D.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    unsigned long long clk[5];
+    memcpy(clk, cnt + 8, sizeof clk);
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0) {
+        (void)hipGetLastError();
+        khz = 100000;
+    }
+    for (int k = 0; k < 4; k++) r->stats.phase_ms[k] = (float)((double)(clk[k + 1] - clk[k]) / (double)khz);
     r->stepped = true;
     r->stats.steps += nsteps;
     r->stats.contacts = cnt[0];
