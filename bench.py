@@ -516,6 +516,15 @@ def bench_batched(a, torch, lr):
                      "rays": rext, "launches": kl["extend"]},
     }
     out.update(walk_counts(c0, c1, kms, kl))
+    # shade's roofline: its algorithmic bytes (roofline.shade_bytes: the entries shaded, the paths
+    # that went on, the shadow rays made, in the profiled batch) over its event-timed launches
+    srext, srsh = k1.rays_ext - k0.rays_ext, k1.rays_shadow - k0.rays_shadow
+    salg = rf.shade_bytes(srext, max(srext - paths, 0), srsh)
+    sach = salg / (kms["shade"] * 1e-3) / 1e9 if kms["shade"] > 0 else 0.0
+    out["shade_roofline"] = {"bound": "hbm", "kernel": "shade", "achieved": round(sach, 2), "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": round(sach / HBM_PEAK_GBS, 4),
+                             "alg_bytes_per_launch": salg // max(kl["shade"], 1), "entries": srext,
+                             "continued": max(srext - paths, 0), "shadow_rays": srsh}
     if a.serve_requests > 0:
         rates = [float(x) for x in a.serve_rates.split(",") if x]
         # serving answers sooner with shorter graphs (a new job joins at the next graph)

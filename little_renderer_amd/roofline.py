@@ -14,6 +14,14 @@ LBVH build (k_bounds, k_morton, k_sort_pass x 4, k_lbvh and k_lbvh_top), per pri
   lbvh      item 4 and box 32 read, leaf node 32 and aux 4 written, and per inner node (n - 1 of
             them, counted per primitive) the sibling's record read and the node written, 32 + 32
 
+Shade (k_shade), per queue entry (one per closest-hit ray): its state read, the queue's slot id 4,
+meta 8, ray 32, hit 16, throughput 16, radiance 16, generator 8 (100); the hit's triangle 48 and
+material 48 (counted for every entry: an escaped ray reads neither, so this part is an upper
+bound); per path that goes on, its next entry written (ray 32, slot id 4, meta 8, throughput 16,
+radiance 16, generator 8: 84); per shadow ray, its entry written (ray 32, id 4, the path's place
+16, radiance 16: 68). The BSDF tables and blue-noise ranks (12 KB, read by every entry) are not
+counted: they stay in the caches.
+
 PLOC build (k_ploc_init, k_ploc_round x rounds, k_ploc_small), the bounds, Morton and sort
 passes before it priced as for the LBVH (bounds 32, morton 40, sort 64 per primitive):
   init      per primitive: item 4 and box 32 read, cluster box 32, aux 4, height 4 and list
@@ -31,6 +39,13 @@ NODE_BYTES_WIDE = 64            # the compressed 4-wide node
 RAY_BYTES, HIT_BYTES, SHAPE_BYTES = 32, 16, 52
 
 LBVH_BYTES = {"bounds": 32, "morton": 40, "sort": 64, "lbvh": 4 + 32 + 32 + 4 + 64}
+
+SHADE_IN, SHADE_HIT, SHADE_NEXT, SHADE_SHADOW = 100, 96, 84, 68
+
+
+def shade_bytes(entries, continued, shadow_rays):
+    """the shade kernel's algorithmic bytes: entries shaded, paths that went on, shadow rays made"""
+    return (SHADE_IN + SHADE_HIT) * entries + SHADE_NEXT * continued + SHADE_SHADOW * shadow_rays
 
 
 def walk_bytes(rays, node_fetches, shape_tests, node_bytes):

@@ -31,3 +31,12 @@ def test_roofline_fraction():
     gbs, frac = rf.roofline(8e9, 1.0)           # 8 GB in a millisecond: 8000 GB/s, the peak
     assert abs(gbs - 8000.0) < 1e-6 and abs(frac - 1.0) < 1e-12
     assert rf.roofline(1, 0.0) == (0.0, 0.0)
+
+
+def test_shade_bytes_by_hand():
+    # an entry's state 4 + 8 + 32 + 16 + 16 + 16 + 8 = 100 read, its triangle 48 and material 48;
+    # a path that goes on writes 32 + 4 + 8 + 16 + 16 + 8 = 84, a shadow ray 32 + 4 + 16 + 16 = 68
+    assert rf.SHADE_IN == 4 + 8 + 32 + 16 + 16 + 16 + 8
+    assert rf.SHADE_NEXT == 32 + 4 + 8 + 16 + 16 + 8 and rf.SHADE_SHADOW == 32 + 4 + 16 + 16
+    assert rf.shade_bytes(10, 6, 7) == 10 * (100 + 96) + 6 * 84 + 7 * 68
+    assert rf.shade_bytes(0, 0, 0) == 0
