@@ -291,7 +291,8 @@ def build_pool(ctx, lr, plan):
 def pmc_per_dispatch(counter, args, regex, match, limit_s=300):
     """This script run again in a child process under rocprofv3 --pmc counter (args: its command
     line), the dispatches of kernels matching regex: the mean of the counter over those whose name
-    holds match, and how many. Raises on a time-out or no dispatch."""
+    holds match (a list of such names: the sum of their means), and how many. Raises on a
+    time-out or no dispatch."""
     import csv
     import glob
     import shutil
@@ -313,15 +314,17 @@ def pmc_per_dispatch(counter, args, regex, match, limit_s=300):
         p.wait()
         shutil.rmtree(d, ignore_errors=True)
         raise RuntimeError(f"{counter} run passed {limit_s} s")
-    vals = []
+    names = match if isinstance(match, (list, tuple)) else [match]
+    vals = {m: [] for m in names}
     for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
         for row in csv.DictReader(open(f)):
-            if match in row["Kernel_Name"] and row["Counter_Name"] == counter:
-                vals.append(float(row["Counter_Value"]))
+            for m in names:
+                if m in row["Kernel_Name"] and row["Counter_Name"] == counter:
+                    vals[m].append(float(row["Counter_Value"]))
     shutil.rmtree(d, ignore_errors=True)
-    if not vals:
-        raise RuntimeError(f"no {counter} dispatches of {match} (rocprofv3 exit {p.returncode})")
-    return sum(vals) / len(vals), len(vals)
+    if not all(vals.values()):
+        raise RuntimeError(f"no {counter} dispatches of {[m for m in names if not vals[m]]} (rocprofv3 exit {p.returncode})")
+    return sum(sum(v) / len(v) for v in vals.values()), min(len(v) for v in vals.values())
 
 
 def kernel_traffic(run_args, regex, kname, what):
@@ -548,8 +551,9 @@ def bench_caustics(a, torch, lr):
     paths at admission, then its camera paths). value = rays traced per second of wall time over
     the timed steps: the path tracer's closest-hit and shadow rays and the light tracer's
     closest-hit rays and rays towards the camera, all counted on the device. The roofline is the
-    light kernel's (k_light: algorithmic bytes from its own counters, 48 B a ray, 64 B a 4-wide
-    node, 52 B a shape, over its time from HIP events on the engine's stream)."""
+    light kernels' (k_light_first + k_light_rest, a pair a chunk of 2^22 light paths: algorithmic
+    bytes from their own counters, 48 B a ray, 64 B a 4-wide node, 52 B a shape, over their time
+    from HIP events on the engine's stream; the traffic a pair's)."""
     steps = a.steps or 5
     warm = 1 if a.warmup is None else a.warmup
     W, H, spp, photons = CAUSTICS_W, CAUSTICS_H, CAUSTICS_SPP, CAUSTICS_PHOTONS
@@ -611,8 +615,10 @@ def bench_caustics(a, torch, lr):
                                               "iters_per_graph", "refill", "refill_shadow"))
         traffic, traffic_detail = kernel_traffic(["--workload", "caustics", "--tune", shape, "--no-cpu", "--steps", "1",
                                                   "--warmup", "0", "--no-traffic"], "k_light",
-                                                 "k_light<256, 16, false>", "the light kernel")
+                                                 ["k_light_first<256, 16, false>", "k_light_rest<256, 16, false>"],
+                                                 "the light kernels")
     lpaths = d("light_paths") // steps
+    chunks = max(1, -(-lpaths // (1 << 22)))
     out = {
         "metric": METRIC + " (caustics workload)",
         "value": round(value, 3),
@@ -638,8 +644,9 @@ def bench_caustics(a, torch, lr):
         "kernel_ms_total": {n: round(v, 3) for n, v in kms.items()},
         "roofline": {"bound": "hbm", "kernel": "light", "dominant": max(kms, key=kms.get), "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "traffic_detail": traffic_detail, "alg_bytes_per_launch": alg,
-                     "node_fetches": lnodes, "shape_tests": lshapes, "rays": lrays, "launches": 1},
+                     "traffic": traffic, "traffic_detail": traffic_detail, "alg_bytes_per_launch": alg // chunks,
+                     "node_fetches": lnodes, "shape_tests": lshapes, "rays": lrays, "launches": chunks,
+                     "launch": "k_light_first + k_light_rest over one chunk of up to 2^22 light paths"},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline_caustics(sc, W, H, (rays // steps) / spp, a.cpu_seconds)

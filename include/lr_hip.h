@@ -351,7 +351,8 @@ typedef struct {
                                511-665; the GPU's ltrace, trace.wgsl:1341-1511), photons x w x h
                                light paths (1 .. 16M) a pass, added to the image, the path tracer
                                then leaving their paths out (skip_caustics); only with a lamp and a
-                               mirror or glass material, and no target error. 0: none */
+                               mirror or glass material, and no target error. The engine makes a
+                               256 MB queue for light paths on its first such job. 0: none */
 } LrhJob;
 
 /* The engine's launch shape (what lrh_engine_autotune picks): work-group sizes of the walk
@@ -477,8 +478,8 @@ int lrh_pt_render_job(LrhContext *c, const LrhJob *job, float *d_accum);
  * Scenes registered with the server stay resident; their bytes and the images of admitted jobs
  * count against a budget of device memory: the image arena is made in chunks as admitted images
  * need it, so resident scenes plus the arena never hold more than the budget (the engine's path
- * state, fixed by slots_log2 at ~360 bytes a slot, and a finished image's staging buffer come on
- * top of it). Requests (thread-safe, from any host thread) wait
+ * state, fixed by slots_log2 at ~360 bytes a slot, a finished image's staging buffer and, once a
+ * job traces light paths, their 256 MB queue come on top of it). Requests (thread-safe, from any host thread) wait
  * in order until their image fits, then join the running batch at once: their paths take free
  * slots as others end (continuous batching). A worker thread drives the engine. */
 typedef struct LrhServer LrhServer;

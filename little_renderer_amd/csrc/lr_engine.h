@@ -80,6 +80,7 @@ struct JobDev {
     int skip_caustics;
 };
 #define LR_PILOT_PASSES 4
+#define LR_LIGHT_CHUNK (1 << 22)          // light paths a launch pair (the queue's entries: 256 MB)
 #define LR_MOM2_SHIFT 20                  // the squares' fixed-point units: 2^-20
 #define LR_MOM2_CAP 1099511627776.0f      // a square saturated at 2^40
 #define LR_MAX_EMITTERS 16
@@ -166,11 +167,13 @@ typedef void (*LrEngFn)(dim3 grid, hipStream_t s, const EngArgs &A);
 // the denoiser's guide of one job (k_guide, 256 threads, a 16-entry LDS stack)
 typedef void (*LrGuideFn)(dim3 grid, hipStream_t s, const JobDev &J, float4 *guide, uint32_t *spill, size_t stride,
                           int spill_per, const float *tab, uint32_t *overflow);
-// a job's light paths for caustics (k_light, 256 threads, a 16-entry LDS stack): npass x nlp;
-// the rays they walk added to cnt[0] (closest hit) and cnt[1] (towards the camera), and in the
-// counting form their node records and shapes to cnt[2] and cnt[3]
-typedef void (*LrLightFn)(dim3 grid, hipStream_t s, const JobDev &J, uint32_t nlp, uint32_t npass, uint32_t *spill,
-                          size_t stride, int spill_per, const float *tab, uint32_t *overflow, unsigned long long *cnt);
+// light paths g0 .. g0+n-1 of a job (path g: pass g / nlp) for caustics (k_light_first, then
+// k_light_rest over the queue of those that go on; 256 threads, a 16-entry LDS stack; n at most
+// the engine's threads); the rays they walk added to cnt[0] (closest hit) and cnt[1] (towards the
+// camera), and in the counting form their node records and shapes to cnt[2] and cnt[3]
+typedef void (*LrLightFn)(hipStream_t s, const JobDev &J, uint32_t nlp, uint64_t g0, uint32_t n, uint32_t *spill,
+                          int spill_per, const float *tab, uint32_t *overflow, float4 *queue, uint32_t *qcount,
+                          unsigned long long *cnt);
 
 // the kernels, by work-group size (64, 128, 256: index 0..2), LDS stack (16, 24, 32) and, for the
 // walks, whether they take two-level scenes too (0: one-level scenes only; the two-level walks'

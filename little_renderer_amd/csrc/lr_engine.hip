@@ -180,7 +180,11 @@ struct LrhEngine {
     std::vector<hipEvent_t> pevs;
     double kms[4] = {0, 0, 0, 0};
     uint64_t klaunch[4] = {0, 0, 0, 0};
-    // light tracing: its launches and paths, and with profiling on its device time
+    // light tracing: the queue of light paths that go on past their first vertex (made on the
+    // first job with light tracing: LR_LIGHT_CHUNK entries of 64 bytes, and its count), its
+    // launches and paths, and with profiling on its device time
+    float4 *lq = nullptr;
+    uint32_t *lqn = nullptr;
     hipEvent_t lev[2] = {nullptr, nullptr};
     uint64_t light_paths = 0, light_launches = 0;
     double light_ms = 0;
@@ -497,6 +501,16 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
             break;
         }
     if (slot < 0) return lr_fail(LRH_EAGAIN, "engine: all %d job slots are in use", LR_JOBS);
+    const bool caustics = j->photons > 0.0f && caustics_possible(j->scene);
+    if (caustics && !e->lq) {
+        // the light paths' queue, made on the first job that traces light paths
+        if (hipMalloc(&e->lq, (size_t)LR_LIGHT_CHUNK * 64 + 256) != hipSuccess) {
+            (void)hipGetLastError();
+            e->lq = nullptr;
+            return lr_fail(LRH_ENOMEM, "engine: the light paths' queue (%d MB) could not be had", LR_LIGHT_CHUNK >> 14);
+        }
+        e->lqn = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(e->lq) + (size_t)LR_LIGHT_CHUNK * 64);
+    }
     const size_t npix = (size_t)j->w * (size_t)j->h;
     const size_t bytes = lr_engine_job_bytes(*j);
     const size_t off = arena_take(e, bytes);
@@ -554,7 +568,6 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
     jd.seed_hi = (uint32_t)(j->seed >> 32);
     jd.clamp = j->clamp;
     jd.pilot = h.pilot ? 1 : 0;
-    const bool caustics = j->photons > 0.0f && caustics_possible(s);
     jd.skip_caustics = caustics ? 1 : 0;
     hipLaunchKernelGGL(k_job_put, dim3(1), dim3(1), 0, e->stream, e->jobs, e->prog->jret, e->prog->jdone, e->prog->jpass,
                        slot, jd);
@@ -577,15 +590,19 @@ int job_admit(LrhEngine *e, const LrhJob *j, int *slot_out)
         // they come to the same), before any of its paths (the same stream)
         const uint32_t nlp = light_paths(j);
         const size_t paths = (size_t)nlp * (size_t)j->spp;
-        const size_t threads = std::min((paths + 255) / 256 * 256, (size_t)e->threads_cap / 256 * 256);
         if (e->profiling && !e->lev[0]) {
             ENG_HIP(hipEventCreate(&e->lev[0]));
             ENG_HIP(hipEventCreate(&e->lev[1]));
         }
         if (e->profiling) ENG_HIP(hipEventRecord(e->lev[0], e->stream));
-        table().light[e->counting ? 1 : 0](dim3((unsigned)(threads / 256)), e->stream, jd, nlp, (uint32_t)j->spp, e->base.spill, threads,
-                      e->spill_cap, e->base.tab, e->overflow, e->counts + 10);
-        ENG_HIP(hipGetLastError());
+        // in chunks of at most LR_LIGHT_CHUNK paths (and the engine's threads, for the spill stride)
+        const uint64_t chunk = std::min<uint64_t>(LR_LIGHT_CHUNK, (uint64_t)e->threads_cap / 256 * 256);
+        for (uint64_t g0 = 0; g0 < (uint64_t)paths; g0 += chunk) {
+            const uint32_t n = (uint32_t)std::min<uint64_t>(chunk, (uint64_t)paths - g0);
+            table().light[e->counting ? 1 : 0](e->stream, jd, nlp, g0, n, e->base.spill, e->spill_cap, e->base.tab, e->overflow,
+                                                e->lq, e->lqn, e->counts + 10);
+            ENG_HIP(hipGetLastError());
+        }
         e->light_paths += paths;
         e->light_launches++;
         if (e->profiling) {
@@ -817,6 +834,7 @@ void lrh_engine_destroy(LrhEngine *e)
     for (hipEvent_t ev : e->pevs) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : e->lev)
         if (ev) (void)hipEventDestroy(ev);
+    if (e->lq) (void)hipFree(e->lq);
     for (int b = 0; b < 2; b++) {
         if (e->pin[b]) (void)hipHostFree(e->pin[b]);
         if (e->pev[b]) (void)hipEventDestroy(e->pev[b]);

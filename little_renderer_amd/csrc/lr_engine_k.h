@@ -1281,129 +1281,145 @@ __global__ __launch_bounds__(BLK) void k_guide(JobDev J, float4 *guide, uint32_t
 }
 
 // ---- light tracing for caustics (light_trace_caustics, tracer.c:511-665; ltrace, trace.wgsl:1341-1511) ----
-// One thread per light path (a grid-stride loop over npass x nlp of them): from a uniform point
-// of the lamp in a cosine-distributed direction, its power Le area pi; at every rough vertex after
-// the first it is joined to the camera by a shadow ray and added to the pixel it projects to, in
-// the job's image, over nlp (each pass's nlp paths are one estimate, added to that pass's sample
-// of the pixels); it goes on by sampling the BSDF seen from the light's side (through a surface
-// times eta^2, the adjoint of the radiance's factor), Russian roulette from depth 3. The first
-// vertex must be a mirror or glass (lamp -> rough is the path tracer's, by NEE), which leaves
-// exactly the paths the path tracer skips (lamp -> mirror/glass ... -> rough -> camera). The
-// numbers: PCG32 per light path (pass k, path i: stream i, sample index 2^31 + s0 + k). No
-// smooth normals here: the shading and geometric normals are one, the adjoint correction 1.
-template <int BLK, int STK, bool COUNT>
-__global__ __launch_bounds__(BLK) void k_light(JobDev J, uint32_t nlp, uint32_t npass, uint32_t *spill, size_t spill_stride,
-                                              int spill_per, const float *tab, uint32_t *overflow, unsigned long long *cnt)
+// A light path (pass k, path i of a pass's nlp; PCG32 stream i, sample index 2^31 + s0 + k): from
+// a uniform point of the lamp in a cosine-distributed direction about its normal, power Le area
+// pi; at every rough vertex after the first it is joined to the camera by a shadow ray and added
+// to the pixel it projects to, in the job's image, over nlp (each pass's nlp paths are one
+// estimate, added to that pass's sample of the pixels); it goes on by sampling the BSDF seen from
+// the light's side (through a surface times eta^2, the adjoint of the radiance's factor), Russian
+// roulette from depth 3. The first vertex must be a mirror or glass (lamp -> rough is the path
+// tracer's, by NEE), which leaves exactly the paths the path tracer skips (lamp -> mirror/glass
+// ... -> rough -> camera). No smooth normals here: the shading and geometric normals are one,
+// the adjoint correction 1.
+// Most light paths end at their first vertex (a rough surface). So the paths are traced in two
+// launches: every path's emission and first vertex, the few that go on appended (their state and
+// generator) to a queue; then those from their second vertex on, one a lane, so that a wave's lanes
+// are not held by the one path in 64 that goes on. The numbers are drawn in the same order either
+// way and the splats are integer sums: the image is the same.
+struct LightState {
+    V3 o, d, beta, thr;
+    uint64_t st;
+    uint32_t path;
+};
+struct LightCtx {
+    Prim lamp;
+    V3 le, org, hz, vt, fw;
+    float pix_area, inv_n;
+    uint64_t seed;
+};
+__device__ __forceinline__ LightCtx light_ctx(const JobDev &J, uint32_t nlp)
 {
-    uint32_t n_ext = 0, n_shadow = 0;   // rays walked (cnt[0], cnt[1], summed by wave)
-    WalkCnt wc;                         // COUNT: node records and shapes (cnt[2], cnt[3])
-    __shared__ uint32_t stk[STK][BLK];
-    const size_t gtid = (size_t)blockIdx.x * BLK + threadIdx.x;
-    const WalkStack S{&stk[0][threadIdx.x], BLK, spill + gtid, spill_stride, STK + spill_per};
-    const Prim lamp = load_prim(J.prims, (uint32_t)J.light);
-    const int lmat = (int)lamp.u.w;
-    const V3 le = v3of(J.mats[3 * lmat + 1]);
-    const V3 org = v3(J.cam[0], J.cam[1], J.cam[2]), ul = v3(J.cam[3], J.cam[4], J.cam[5]);
-    const V3 hz = v3(J.cam[6], J.cam[7], J.cam[8]), vt = v3(J.cam[9], J.cam[10], J.cam[11]);
-    const float pix_area = vlen(hz) * vlen(vt) / ((float)J.W * (float)J.H);   // camera_screen_area / pixels
-    const V3 center = vadd(ul, vsub(vscale(hz, 0.5f), vscale(vt, 0.5f)));
-    const V3 fw = vsub(center, org);
-    const float inv_n = 1.0f / (float)nlp;
-    const uint64_t seed = ((uint64_t)J.seed_hi << 32) | J.seed_lo;
-    const uint64_t total = (uint64_t)nlp * npass;
-    for (uint64_t g = gtid; g < total; g += (uint64_t)gridDim.x * BLK) {
-        const uint32_t pass = (uint32_t)(g / nlp), path = (uint32_t)(g % nlp);
-        const uint64_t inc = ((uint64_t)path << 1u) | 1u;
-        uint64_t st = pcg_seed(path, 0x80000000u + J.s0 + pass, seed);
-        // emit: a point of the lamp, a direction about its normal
-        const float e1 = pcg_float(st, inc), e2 = pcg_float(st, inc);
-        V3 o = vadd(v3of(lamp.q), vadd(vscale(v3of(lamp.u), e1), vscale(v3of(lamp.v), e2)));
-        const float c1 = pcg_float(st, inc), c2 = pcg_float(st, inc);
-        V3 d = cosine_hemisphere(v3of(lamp.n), c1, c2);
-        V3 beta = vscale(le, lamp.v.w * PI_F), thr = v3(1.0f, 1.0f, 1.0f);
-        for (int depth = 0; depth < J.max_depth; depth++) {
-            const WalkHit h = walk4<false, COUNT, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, o, d, J.eps, 1e30f, S,
-                                                       overflow, wc);
-            n_ext++;
-            if (h.id == LR_NONE) break;
-            const V3 hp = vadd(o, vscale(d, h.t));
-            V3 outward;
-            int mi;
-            uint32_t ii;
-            hit_frame(J, o, d, h.t, h.id, __float_as_uint(h.b1), hp, outward, mi, ii);
-            const bool front = vdot(d, outward) < 0.0f;
-            const V3 nrm = front ? outward : vneg(outward);
-            int type;
-            Bsdf b = bsdf_make(J.mats, mi, front, tab, type);
-            if (type == MAT_LIGHT) break;
-            const V3 wi = vneg(d);
-            // 1. the rough part joined to the camera (not at the first vertex)
-            if (depth > 0 && (!b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f)) {
-                const V3 to_cam = vsub(org, hp);
-                const float dist2 = vdot(to_cam, to_cam), dist = sqrtf(dist2);
-                const V3 wo = vscale(to_cam, 1.0f / dist);
-                const float cos_o = vdot(nrm, wo);
-                // the BSDF seen from the camera: its side's normal and index
-                Bsdf bc = b;
-                int tc;
-                if (!(cos_o > 0.0f)) bc = bsdf_make(J.mats, mi, !front, tab, tc);
-                const V3 nc = cos_o > 0.0f ? nrm : vneg(nrm);
-                float pdf;
-                V3 f = v3(0.0f, 0.0f, 0.0f);
-                if (cos_o != 0.0f && vdot(nc, wo) > 0.0f) f = bsdf_eval(bc, tab, nc, wo, wi, pdf);
-                // camera_project (camera.h:42-54)
-                const V3 qv = vsub(hp, org);
-                const float z = vdot(qv, fw);
-                if (vmax3(f) > 0.0f && z > 0.0f) {
-                    const V3 e = vsub(vscale(qv, 1.0f / z), fw);
-                    const float sx = 0.5f + vdot(e, hz) / vdot(hz, hz), sy = 0.5f - vdot(e, vt) / vdot(vt, vt);
-                    const float cos_c = z / vlen(qv);
-                    if (sx >= 0.0f && sy >= 0.0f && sx < 1.0f && sy < 1.0f) {
-                        const int px = (int)(sx * (float)J.W), py = (int)(sy * (float)J.H);
-                        if (px < J.W && py < J.H) {
-                            const WalkHit sh = walk4<true, COUNT, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, hp, wo,
-                                                                       J.eps, dist * (1.0f - 1e-4f), S, overflow, wc);
-                            n_shadow++;
-                            if (sh.id == LR_NONE) {
-                                const float wgt = fabsf(cos_o) / (dist2 * pix_area * cos_c * cos_c * cos_c);
-                                const V3 c = vscale(vmul(beta, f), wgt * inv_n);
-                                long long *px3 = J.accum + 3 * ((size_t)py * J.W + px);
-                                const float cc[3] = {c.x, c.y, c.z};
-                                for (int ch = 0; ch < 3; ch++) {
-                                    if (!(cc[ch] > 0.0f)) continue;   // NaN or nothing
-                                    const long long v = to_fixed(cc[ch]);
-                                    if (v) atomicAdd((unsigned long long *)&px3[ch], (unsigned long long)v);
-                                }
-                            }
+    LightCtx L;
+    L.lamp = load_prim(J.prims, (uint32_t)J.light);
+    L.le = v3of(J.mats[3 * (int)L.lamp.u.w + 1]);
+    L.org = v3(J.cam[0], J.cam[1], J.cam[2]);
+    const V3 ul = v3(J.cam[3], J.cam[4], J.cam[5]);
+    L.hz = v3(J.cam[6], J.cam[7], J.cam[8]);
+    L.vt = v3(J.cam[9], J.cam[10], J.cam[11]);
+    L.pix_area = vlen(L.hz) * vlen(L.vt) / ((float)J.W * (float)J.H);   // camera_screen_area / pixels
+    const V3 center = vadd(ul, vsub(vscale(L.hz, 0.5f), vscale(L.vt, 0.5f)));
+    L.fw = vsub(center, L.org);
+    L.inv_n = 1.0f / (float)nlp;
+    L.seed = ((uint64_t)J.seed_hi << 32) | J.seed_lo;
+    return L;
+}
+
+// one vertex of a light path at `depth`: the walk to it, its rough part joined to the camera
+// (after the first vertex), the next direction drawn; false: the path ends here
+template <int STK, bool COUNT>
+__device__ __forceinline__ bool light_vertex(const JobDev &J, const LightCtx &L, int depth, LightState &P, const WalkStack &S,
+                                             const float *tab, uint32_t *overflow, WalkCnt &wc, uint32_t &n_ext,
+                                             uint32_t &n_shadow)
+{
+    const uint64_t inc = ((uint64_t)P.path << 1u) | 1u;
+    const WalkHit h = walk4<false, COUNT, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, P.o, P.d, J.eps, 1e30f, S,
+                                               overflow, wc);
+    n_ext++;
+    if (h.id == LR_NONE) return false;
+    const V3 hp = vadd(P.o, vscale(P.d, h.t));
+    V3 outward;
+    int mi;
+    uint32_t ii;
+    hit_frame(J, P.o, P.d, h.t, h.id, __float_as_uint(h.b1), hp, outward, mi, ii);
+    const bool front = vdot(P.d, outward) < 0.0f;
+    const V3 nrm = front ? outward : vneg(outward);
+    int type;
+    Bsdf b = bsdf_make(J.mats, mi, front, tab, type);
+    if (type == MAT_LIGHT) return false;
+    const V3 wi = vneg(P.d);
+    // 1. the rough part joined to the camera (not at the first vertex)
+    if (depth > 0 && (!b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f)) {
+        const V3 to_cam = vsub(L.org, hp);
+        const float dist2 = vdot(to_cam, to_cam), dist = sqrtf(dist2);
+        const V3 wo = vscale(to_cam, 1.0f / dist);
+        const float cos_o = vdot(nrm, wo);
+        // the BSDF seen from the camera: its side's normal and index
+        Bsdf bc = b;
+        int tc;
+        if (!(cos_o > 0.0f)) bc = bsdf_make(J.mats, mi, !front, tab, tc);
+        const V3 nc = cos_o > 0.0f ? nrm : vneg(nrm);
+        float pdf;
+        V3 f = v3(0.0f, 0.0f, 0.0f);
+        if (cos_o != 0.0f && vdot(nc, wo) > 0.0f) f = bsdf_eval(bc, tab, nc, wo, wi, pdf);
+        // camera_project (camera.h:48-59)
+        const V3 qv = vsub(hp, L.org);
+        const float z = vdot(qv, L.fw);
+        if (vmax3(f) > 0.0f && z > 0.0f) {
+            const V3 e = vsub(vscale(qv, 1.0f / z), L.fw);
+            const float sx = 0.5f + vdot(e, L.hz) / vdot(L.hz, L.hz), sy = 0.5f - vdot(e, L.vt) / vdot(L.vt, L.vt);
+            const float cos_c = z / vlen(qv);
+            if (sx >= 0.0f && sy >= 0.0f && sx < 1.0f && sy < 1.0f) {
+                const int px = (int)(sx * (float)J.W), py = (int)(sy * (float)J.H);
+                if (px < J.W && py < J.H) {
+                    const WalkHit sh = walk4<true, COUNT, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, hp, wo, J.eps,
+                                                               dist * (1.0f - 1e-4f), S, overflow, wc);
+                    n_shadow++;
+                    if (sh.id == LR_NONE) {
+                        const float wgt = fabsf(cos_o) / (dist2 * L.pix_area * cos_c * cos_c * cos_c);
+                        const V3 c = vscale(vmul(P.beta, f), wgt * L.inv_n);
+                        long long *px3 = J.accum + 3 * ((size_t)py * J.W + px);
+                        const float cc[3] = {c.x, c.y, c.z};
+                        for (int ch = 0; ch < 3; ch++) {
+                            if (!(cc[ch] > 0.0f)) continue;   // NaN or nothing
+                            const long long v = to_fixed(cc[ch]);
+                            if (v) atomicAdd((unsigned long long *)&px3[ch], (unsigned long long)v);
                         }
                     }
                 }
             }
-            // 2. on, by the BSDF seen from the light's side (bsdf_sample, brdf.h:343-348)
-            const float ulb = pcg_float(st, inc), u1 = pcg_float(st, inc), u2 = pcg_float(st, inc);
-            const float cv = vdot(wi, nrm);
-            if (!(cv > 0.0f)) break;
-            bsdf_view(b, tab, cv);
-            BSample bs;
-            if (!bsdf_sample_lobe(b, tab, bsdf_pick(b, ulb), nrm, wi, u1, u2, bs)) break;
-            if (depth == 0 && !bs.delta) break;   // lamp -> rough: the path tracer's
-            const float og = vdot(nrm, bs.l);
-            const bool through = og < 0.0f;
-            if (!bs.delta && (through ? og >= 0.0f : og <= 0.0f)) break;
-            V3 f = bs.w;
-            if (through) f = vscale(f, b.eta * b.eta);
-            beta = vmul(beta, f);
-            thr = vmul(thr, f);
-            if (depth >= 3) {
-                const float pr = rfmin(vmax3(thr), 0.95f);
-                if (pcg_float(st, inc) >= pr) break;
-                beta = vscale(beta, 1.0f / pr);
-                thr = vscale(thr, 1.0f / pr);
-            }
-            o = hp;
-            d = vnorm(bs.l);
         }
     }
+    // 2. on, by the BSDF seen from the light's side (bsdf_sample, brdf.h:343-348)
+    const float ulb = pcg_float(P.st, inc), u1 = pcg_float(P.st, inc), u2 = pcg_float(P.st, inc);
+    const float cv = vdot(wi, nrm);
+    if (!(cv > 0.0f)) return false;
+    bsdf_view(b, tab, cv);
+    BSample bs;
+    if (!bsdf_sample_lobe(b, tab, bsdf_pick(b, ulb), nrm, wi, u1, u2, bs)) return false;
+    if (depth == 0 && !bs.delta) return false;   // lamp -> rough: the path tracer's
+    const float og = vdot(nrm, bs.l);
+    const bool through = og < 0.0f;
+    if (!bs.delta && (through ? og >= 0.0f : og <= 0.0f)) return false;
+    V3 f = bs.w;
+    if (through) f = vscale(f, b.eta * b.eta);
+    P.beta = vmul(P.beta, f);
+    P.thr = vmul(P.thr, f);
+    if (depth >= 3) {
+        const float pr = rfmin(vmax3(P.thr), 0.95f);
+        if (pcg_float(P.st, inc) >= pr) return false;
+        P.beta = vscale(P.beta, 1.0f / pr);
+        P.thr = vscale(P.thr, 1.0f / pr);
+    }
+    P.o = hp;
+    P.d = vnorm(bs.l);
+    return true;
+}
+
+// the launch's rays (cnt[0] closest hit, cnt[1] towards the camera) and, COUNT, its node records
+// and shapes (cnt[2], cnt[3]), summed by wave
+template <bool COUNT>
+__device__ __forceinline__ void light_counts(unsigned long long *cnt, uint32_t n_ext, uint32_t n_shadow, const WalkCnt &wc)
+{
     unsigned long long v[4] = {n_ext, n_shadow, wc.n, wc.t};
     for (int k = 0; k < (COUNT ? 4 : 2); k++) {
         for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
@@ -1411,12 +1427,98 @@ __global__ __launch_bounds__(BLK) void k_light(JobDev J, uint32_t nlp, uint32_t 
     }
 }
 
-template <int BLK, int STK, bool COUNT>
-void L_light(dim3 g, hipStream_t s, const JobDev &J, uint32_t nlp, uint32_t npass, uint32_t *spill, size_t stride,
-             int spill_per, const float *tab, uint32_t *overflow, unsigned long long *cnt)
+// a light path's state in the queue: 64 bytes
+__device__ __forceinline__ void light_put(float4 *q, const LightState &P)
 {
-    hipLaunchKernelGGL((k_light<BLK, STK, COUNT>), g, dim3(BLK), 0, s, J, nlp, npass, spill, stride, spill_per, tab, overflow,
-                       cnt);
+    q[0] = make_float4(P.o.x, P.o.y, P.o.z, __uint_as_float(P.path));
+    q[1] = make_float4(P.d.x, P.d.y, P.d.z, 0.0f);
+    q[2] = make_float4(P.beta.x, P.beta.y, P.beta.z, P.thr.x);
+    q[3] = make_float4(P.thr.y, P.thr.z, __uint_as_float((uint32_t)P.st), __uint_as_float((uint32_t)(P.st >> 32)));
+}
+__device__ __forceinline__ LightState light_get(const float4 *q)
+{
+    const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+    LightState P;
+    P.o = v3(a.x, a.y, a.z);
+    P.path = __float_as_uint(a.w);
+    P.d = v3(b.x, b.y, b.z);
+    P.beta = v3(c.x, c.y, c.z);
+    P.thr = v3(c.w, d.x, d.y);
+    P.st = (uint64_t)__float_as_uint(d.z) | ((uint64_t)__float_as_uint(d.w) << 32);
+    return P;
+}
+
+// 1. light paths g0 .. g0+n-1 (path g: pass g / nlp, path g % nlp): their emission and first
+// vertex; those that go on appended to the queue (a wave's together: one atomic add a wave)
+template <int BLK, int STK, bool COUNT>
+__global__ __launch_bounds__(BLK) void k_light_first(JobDev J, uint32_t nlp, uint64_t g0, uint32_t n, uint32_t *spill,
+                                                    size_t spill_stride, int spill_per, const float *tab, uint32_t *overflow,
+                                                    float4 *queue, uint32_t *qcount, unsigned long long *cnt)
+{
+    __shared__ uint32_t stk[STK][BLK];
+    const uint32_t i = blockIdx.x * BLK + threadIdx.x;
+    const WalkStack S{&stk[0][threadIdx.x], BLK, spill + i, spill_stride, STK + spill_per};
+    const LightCtx L = light_ctx(J, nlp);
+    uint32_t n_ext = 0, n_shadow = 0;
+    WalkCnt wc;
+    bool on = false;
+    LightState P;
+    if (i < n) {
+        const uint64_t g = g0 + i;
+        const uint32_t pass = (uint32_t)(g / nlp);
+        P.path = (uint32_t)(g % nlp);
+        const uint64_t inc = ((uint64_t)P.path << 1u) | 1u;
+        P.st = pcg_seed(P.path, 0x80000000u + J.s0 + pass, L.seed);
+        // emit: a point of the lamp, a direction about its normal
+        const float e1 = pcg_float(P.st, inc), e2 = pcg_float(P.st, inc);
+        P.o = vadd(v3of(L.lamp.q), vadd(vscale(v3of(L.lamp.u), e1), vscale(v3of(L.lamp.v), e2)));
+        const float c1 = pcg_float(P.st, inc), c2 = pcg_float(P.st, inc);
+        P.d = cosine_hemisphere(v3of(L.lamp.n), c1, c2);
+        P.beta = vscale(L.le, L.lamp.v.w * PI_F);
+        P.thr = v3(1.0f, 1.0f, 1.0f);
+        if (J.max_depth > 0) on = light_vertex<STK, COUNT>(J, L, 0, P, S, tab, overflow, wc, n_ext, n_shadow) && J.max_depth > 1;
+    }
+    const uint64_t b = __ballot(on);
+    uint32_t at = 0;
+    if (lane_id() == 0 && b) at = atomicAdd(qcount, (uint32_t)__popcll(b));
+    at = __shfl(at, 0, 64);
+    if (on) light_put(queue + 4 * (size_t)(at + __popcll(b & lanemask_lt())), P);
+    light_counts<COUNT>(cnt, n_ext, n_shadow, wc);
+}
+
+// 2. the queue's light paths from their second vertex on, one a lane
+template <int BLK, int STK, bool COUNT>
+__global__ __launch_bounds__(BLK) void k_light_rest(JobDev J, uint32_t nlp, uint32_t *spill, size_t spill_stride, int spill_per,
+                                                   const float *tab, uint32_t *overflow, const float4 *queue,
+                                                   const uint32_t *qcount, unsigned long long *cnt)
+{
+    __shared__ uint32_t stk[STK][BLK];
+    const size_t gtid = (size_t)blockIdx.x * BLK + threadIdx.x;
+    const WalkStack S{&stk[0][threadIdx.x], BLK, spill + gtid, spill_stride, STK + spill_per};
+    const LightCtx L = light_ctx(J, nlp);
+    const uint32_t nq = *qcount;
+    uint32_t n_ext = 0, n_shadow = 0;
+    WalkCnt wc;
+    for (size_t k = gtid; k < nq; k += (size_t)gridDim.x * BLK) {
+        LightState P = light_get(queue + 4 * k);
+        for (int depth = 1; depth < J.max_depth; depth++)
+            if (!light_vertex<STK, COUNT>(J, L, depth, P, S, tab, overflow, wc, n_ext, n_shadow)) break;
+    }
+    light_counts<COUNT>(cnt, n_ext, n_shadow, wc);
+}
+
+__global__ void k_light_reset(uint32_t *qcount) { *qcount = 0u; }
+
+template <int BLK, int STK, bool COUNT>
+void L_light(hipStream_t s, const JobDev &J, uint32_t nlp, uint64_t g0, uint32_t n, uint32_t *spill, int spill_per,
+             const float *tab, uint32_t *overflow, float4 *queue, uint32_t *qcount, unsigned long long *cnt)
+{
+    const unsigned blocks = (n + BLK - 1) / BLK;
+    hipLaunchKernelGGL(k_light_reset, dim3(1), dim3(1), 0, s, qcount);
+    hipLaunchKernelGGL((k_light_first<BLK, STK, COUNT>), dim3(blocks), dim3(BLK), 0, s, J, nlp, g0, n, spill,
+                       (size_t)blocks * BLK, spill_per, tab, overflow, queue, qcount, cnt);
+    hipLaunchKernelGGL((k_light_rest<BLK, STK, COUNT>), dim3(blocks), dim3(BLK), 0, s, J, nlp, spill, (size_t)blocks * BLK,
+                       spill_per, tab, overflow, queue, qcount, cnt);
 }
 
 template <int BLK, int STK, bool TL>
