@@ -648,6 +648,28 @@ def bench_caustics(a, torch, lr):
                      "node_fetches": lnodes, "shape_tests": lshapes, "rays": lrays, "launches": chunks,
                      "launch": "k_light_first + k_light_rest over one chunk of up to 2^22 light paths"},
     }
+    # the photon cache (lrh_photons_*): one frame's light paths into it, a fade, the lookup at W x H
+    pc = lr.Photons(ctx, 0.01)
+    npc = W * H * spp
+    pc.trace(ps, npc, seed=1)
+    ctx.sync()
+    pt0 = time.perf_counter()
+    pc.trace(ps, npc, seed=2)
+    ctx.sync()
+    pt1 = time.perf_counter()
+    pc.fade(0.5)
+    ctx.sync()
+    pt2 = time.perf_counter()
+    img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    pc.resolve(ps, sc["cam"], W, H, img)
+    pt3 = time.perf_counter()
+    out["photon_cache"] = {"light_paths": npc, "trace_ms": round((pt1 - pt0) * 1e3, 3),
+                           "mpaths_s": round(npc / (pt1 - pt0) / 1e6, 1), "fade_ms": round((pt2 - pt1) * 1e3, 3),
+                           "resolve_ms": round((pt3 - pt2) * 1e3, 3),
+                           "note": "host wall time around each call (the context's stream synchronised): a trace of "
+                                   "W x H x spp light paths into a 2^20-cell table, a fade, the lookup at W x H"}
+    pc.close()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline_caustics(sc, W, H, (rays // steps) / spp, a.cpu_seconds)
     eng.close()

@@ -474,6 +474,36 @@ int lrh_engine_autotune(LrhEngine *e, const LrhJob *sample, int n, LrhTune *best
  * denoise (its outputs come from an engine: lrh_engine_denoise). */
 int lrh_pt_render_job(LrhContext *c, const LrhJob *job, float *d_accum);
 
+/* ---- the photon cache: the viewer's caustics between frames ----
+ * gpu_trace_photons / gpu_trace_photons_clear / gpu_trace_photon_paths (engine/gpu/trace_gpu.c:
+ * 1964-1996) with the light tracer's cache mode (deposit, pc_claim: trace.wgsl:1359-1404), its
+ * fading (trace_decay.wgsl) and its lookup (trace_resolve.wgsl:43-80). A world-space hash grid of
+ * 2^20 cells (32 MB): cells of three sizes (cell, x4, x16), one set per main axis of the
+ * surface's normal and its sign; each holds the power the light paths left there (integer sums:
+ * the same table whatever the order) and its photons. Light paths start at the scene's lamp as
+ * the light tracer's do and leave their power at every rough vertex after a mirror or glass. */
+typedef struct LrhPhotons LrhPhotons;
+/* cell: the finest cells' size (the reference's T.lt.z) */
+int lrh_photons_create(LrhContext *c, float cell, LrhPhotons **out);
+void lrh_photons_destroy(LrhPhotons *pc);
+int lrh_photons_clear(LrhPhotons *pc);
+/* npaths light paths (path i: PCG32 stream i, sample index 2^31 + s0) of a one-level scene,
+ * max_depth vertices each, into the cache (asynchronous, on the context's stream) */
+int lrh_photons_trace(LrhPhotons *pc, const LrhPtScene *s, uint32_t npaths, uint32_t s0, uint64_t seed, int max_depth);
+/* every cell's power and photons times k in [0, 1] (the light paths it holds too); a cell whose
+ * photons faded to nothing gives its slot back when the next slot is free */
+int lrh_photons_fade(LrhPhotons *pc, float k);
+/* the light paths the cache holds (their number, faded with it) */
+double lrh_photons_paths(const LrhPhotons *pc);
+/* each pixel's centre ray (camera cam); where it first meets a rough surface, the caustics'
+ * radiance there from the cache: albedo / pi x the power of the finest cells around the point
+ * (bilinearly) holding at least 16 photons, over their area and the light paths; d_out: float4
+ * per pixel (rgb, 1; 0 where no rough surface is seen). Returns when d_out holds it. */
+int lrh_photons_resolve(LrhPhotons *pc, const LrhPtScene *s, const float *cam, int W, int H, float *d_out);
+/* the table to the host: 2^20 slots of 4 x 64 bits (rgb sums in 2^-8 units; the photons in 1/16
+ * units in the low 32 bits of the fourth word, the cell's tag in its high 32 bits, 0 free) */
+int lrh_photons_read(LrhPhotons *pc, void *h_table);
+
 /* ---- serving: a request loop over an engine ----
  * Scenes registered with the server stay resident; their bytes and the images of admitted jobs
  * count against a budget of device memory: the image arena is made in chunks as admitted images

@@ -155,6 +155,14 @@ def lib():
                                   ctypes.POINTER(ctypes.c_size_t)]),
         "lrh_pt_render": (I, [P, P, P, I, I, I, U, ctypes.c_uint64, I, I, F, P]),
         "lrh_pt_render_job": (I, [P, ctypes.POINTER(Job), P]),
+        "lrh_photons_create": (I, [P, F, ctypes.POINTER(P)]),
+        "lrh_photons_destroy": (None, [P]),
+        "lrh_photons_clear": (I, [P]),
+        "lrh_photons_trace": (I, [P, P, U, U, ctypes.c_uint64, I]),
+        "lrh_photons_fade": (I, [P, F]),
+        "lrh_photons_paths": (ctypes.c_double, [P]),
+        "lrh_photons_resolve": (I, [P, P, P, I, I, P]),
+        "lrh_photons_read": (I, [P, P]),
         "lrh_bsdf_cases": (I, [P, P, I, P]),
         "lrh_denoise": (I, [P, P, P, P, I, I, I, P]),
         "lrh_engine_guide": (I, [P, ctypes.c_int64, P, P]),
@@ -434,6 +442,55 @@ class PtScene:
         """lrh_pt_render_job: a job of this scene (make_job: light-traced caustics, a target error)
         added into accum (h, w, 4)"""
         _check(lib().lrh_pt_render_job(self.ctx._c, ctypes.byref(job), _ptr(accum)))
+
+
+class Photons:
+    """The photon cache (lrh_photons_*): light paths' power left at rough vertices in a world-space
+    hash grid (cells of `cell`, x4, x16), faded between frames, read where the camera sees a rough
+    surface."""
+    SLOTS = 1 << 20
+
+    def __init__(self, ctx, cell):
+        self.ctx = ctx
+        self._p = ctypes.c_void_p()
+        _check(lib().lrh_photons_create(ctx._c, cell, ctypes.byref(self._p)))
+        ctx._adopt(self)
+
+    def trace(self, scene, npaths, s0=0, seed=0, max_depth=8):
+        _check(lib().lrh_photons_trace(self._p, scene._h, npaths, s0, seed, max_depth))
+
+    def fade(self, k):
+        _check(lib().lrh_photons_fade(self._p, k))
+
+    def clear(self):
+        _check(lib().lrh_photons_clear(self._p))
+
+    def paths(self):
+        return lib().lrh_photons_paths(self._p)
+
+    def resolve(self, scene, cam, w, h, out):
+        """into a device tensor (h, w, 4): rgb, 1 where a rough surface is seen"""
+        import numpy as np
+        c = np.ascontiguousarray(cam, np.float32)
+        _check(lib().lrh_photons_resolve(self._p, scene._h, _ptr(c), w, h, _ptr(out)))
+
+    def read(self):
+        """the table: (SLOTS, 4) uint64"""
+        import numpy as np
+        t = np.zeros((self.SLOTS, 4), np.uint64)
+        _check(lib().lrh_photons_read(self._p, _ptr(t)))
+        return t
+
+    def close(self):
+        if self._p:
+            lib().lrh_photons_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Rigid:

@@ -78,7 +78,15 @@ struct JobDev {
     // light tracing on: the paths the light tracer covers (the camera saw a rough surface first,
     // and the lamp is reached right after a mirror or glass) leave the lamp out
     int skip_caustics;
+    // light paths filling a photon cache (lrh_photons_trace): its table and finest cell's size
+    unsigned long long *pc;
+    float pc_cell;
 };
+// the photon cache: LR_PC_SLOTS slots of 4 x 64 bits (rgb power sums in 2^-8 units; the photons
+// in 1/16 units, 32 bits, and the cell's 32-bit tag, 0 = free, in the fourth word)
+#define LR_PC_SLOTS (1u << 20)
+#define LR_PC_SCALE 256.0f
+#define LR_PC_UNIT 16u
 #define LR_PILOT_PASSES 4
 #define LR_LIGHT_CHUNK (1 << 22)          // light paths a launch pair (the queue's entries: 256 MB)
 #define LR_MOM2_SHIFT 20                  // the squares' fixed-point units: 2^-20

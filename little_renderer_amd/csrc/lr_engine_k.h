@@ -1324,9 +1324,79 @@ __device__ __forceinline__ LightCtx light_ctx(const JobDev &J, uint32_t nlp)
     return L;
 }
 
+// ---- the photon cache (gpu_trace_photons, trace_gpu.c:1964-1996; pc_claim / deposit,
+// trace.wgsl:1359-1404): a world-space hash grid of cells in three sizes (the finest, x4, x16),
+// one set per main axis of the surface's normal and its sign. A cell's key (its integer corner,
+// 19 bits an axis, the axis 3 bits, the size 2 bits) mixed by SplitMix64's finaliser (Steele, Lea
+// and Flood 2014) gives its home slot (low bits) and its tag (high 32 bits, never 0); a cell takes
+// the first slot from its home on that is free or its own, at most 8 on (linear probing), by a
+// compare-and-swap of the tag. Sums are integers: any order of deposits gives the same table.
+__device__ __forceinline__ uint64_t pc_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t pc_key(int x, int y, int z, int axis, int level)
+{
+    const uint64_t m = (1ull << 19) - 1ull;
+    return pc_mix(((uint64_t)(uint32_t)x & m) | (((uint64_t)(uint32_t)y & m) << 19) | (((uint64_t)(uint32_t)z & m) << 38) |
+                  ((uint64_t)axis << 57) | ((uint64_t)level << 60));
+}
+// the main axis of n and its sign: 2 axis + (n[axis] < 0)
+__device__ __forceinline__ int pc_axis(V3 n)
+{
+    const float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+    const int a = ax >= ay && ax >= az ? 0 : (ay >= az ? 1 : 2);
+    const float v = a == 0 ? n.x : (a == 1 ? n.y : n.z);
+    return 2 * a + (v < 0.0f ? 1 : 0);
+}
+__device__ __forceinline__ uint32_t pc_tag(uint64_t h) { return (uint32_t)(h >> 32) | 1u; }
+// the cell's slot (claimed if need be), or LR_NONE when the 8 slots from its home are others'
+__device__ __forceinline__ uint32_t pc_claim(unsigned long long *pc, uint64_t h)
+{
+    const uint32_t tag = pc_tag(h);
+    uint32_t idx = (uint32_t)h & (LR_PC_SLOTS - 1u);
+    for (int probe = 0; probe < 8; probe++) {
+        uint32_t *w = reinterpret_cast<uint32_t *>(pc + 4 * (size_t)idx) + 7;
+        const uint32_t old = atomicCAS(w, 0u, tag);
+        if (old == 0u || old == tag) return idx;
+        idx = (idx + 1u) & (LR_PC_SLOTS - 1u);
+    }
+    return LR_NONE;
+}
+// a point's cell of a size: along the normal's main axis offset by 1/pi of a cell, so that an
+// axis-aligned surface at a round coordinate (a floor at y = 0) does not lie on the cells'
+// boundary, where rounding would put some of its points in the layer of cells on either side
+constexpr float PC_OFFSET = 0.31830988f;
+__device__ __forceinline__ int pc_coord(float x, float size, bool normal_axis)
+{
+    return (int)floorf(x / size + (normal_axis ? PC_OFFSET : 0.0f));
+}
+// a photon of power c at p on a surface facing n: into its cell of each size
+__device__ __forceinline__ void pc_deposit(const JobDev &J, V3 p, V3 n, V3 c)
+{
+    const int axis = pc_axis(n), ax = axis / 2;
+    float size = J.pc_cell;
+    for (int level = 0; level < 3; level++, size *= 4.0f) {
+        const uint32_t slot = pc_claim(J.pc, pc_key(pc_coord(p.x, size, ax == 0), pc_coord(p.y, size, ax == 1),
+                                                    pc_coord(p.z, size, ax == 2), axis, level));
+        if (slot == LR_NONE) continue;
+        unsigned long long *w = J.pc + 4 * (size_t)slot;
+        const float cc[3] = {c.x, c.y, c.z};
+        for (int ch = 0; ch < 3; ch++) {
+            const float v = cc[ch] * LR_PC_SCALE;
+            if (!(v > 0.0f) || v > 1.8e18f) continue;
+            atomicAdd(&w[ch], (unsigned long long)__float2ull_rn(v));
+        }
+        atomicAdd(reinterpret_cast<uint32_t *>(w + 3), LR_PC_UNIT);
+    }
+}
+
 // one vertex of a light path at `depth`: the walk to it, its rough part joined to the camera
-// (after the first vertex), the next direction drawn; false: the path ends here
-template <int STK, bool COUNT>
+// (after the first vertex; PC: left in the photon cache instead), the next direction drawn;
+// false: the path ends here
+template <int STK, bool COUNT, bool PC = false>
 __device__ __forceinline__ bool light_vertex(const JobDev &J, const LightCtx &L, int depth, LightState &P, const WalkStack &S,
                                              const float *tab, uint32_t *overflow, WalkCnt &wc, uint32_t &n_ext,
                                              uint32_t &n_shadow)
@@ -1347,8 +1417,10 @@ __device__ __forceinline__ bool light_vertex(const JobDev &J, const LightCtx &L,
     Bsdf b = bsdf_make(J.mats, mi, front, tab, type);
     if (type == MAT_LIGHT) return false;
     const V3 wi = vneg(P.d);
-    // 1. the rough part joined to the camera (not at the first vertex)
-    if (depth > 0 && (!b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f)) {
+    // 1. the rough part joined to the camera, or left in the photon cache (not at the first vertex)
+    if (PC && depth > 0 && (!b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f))
+        pc_deposit(J, hp, nrm, P.beta);
+    if (!PC && depth > 0 && (!b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f)) {
         const V3 to_cam = vsub(L.org, hp);
         const float dist2 = vdot(to_cam, to_cam), dist = sqrtf(dist2);
         const V3 wo = vscale(to_cam, 1.0f / dist);
@@ -1450,7 +1522,7 @@ __device__ __forceinline__ LightState light_get(const float4 *q)
 
 // 1. light paths g0 .. g0+n-1 (path g: pass g / nlp, path g % nlp): their emission and first
 // vertex; those that go on appended to the queue (a wave's together: one atomic add a wave)
-template <int BLK, int STK, bool COUNT>
+template <int BLK, int STK, bool COUNT, bool PC = false>
 __global__ __launch_bounds__(BLK) void k_light_first(JobDev J, uint32_t nlp, uint64_t g0, uint32_t n, uint32_t *spill,
                                                     size_t spill_stride, int spill_per, const float *tab, uint32_t *overflow,
                                                     float4 *queue, uint32_t *qcount, unsigned long long *cnt)
@@ -1476,7 +1548,8 @@ __global__ __launch_bounds__(BLK) void k_light_first(JobDev J, uint32_t nlp, uin
         P.d = cosine_hemisphere(v3of(L.lamp.n), c1, c2);
         P.beta = vscale(L.le, L.lamp.v.w * PI_F);
         P.thr = v3(1.0f, 1.0f, 1.0f);
-        if (J.max_depth > 0) on = light_vertex<STK, COUNT>(J, L, 0, P, S, tab, overflow, wc, n_ext, n_shadow) && J.max_depth > 1;
+        if (J.max_depth > 0)
+            on = light_vertex<STK, COUNT, PC>(J, L, 0, P, S, tab, overflow, wc, n_ext, n_shadow) && J.max_depth > 1;
     }
     const uint64_t b = __ballot(on);
     uint32_t at = 0;
@@ -1487,7 +1560,7 @@ __global__ __launch_bounds__(BLK) void k_light_first(JobDev J, uint32_t nlp, uin
 }
 
 // 2. the queue's light paths from their second vertex on, one a lane
-template <int BLK, int STK, bool COUNT>
+template <int BLK, int STK, bool COUNT, bool PC = false>
 __global__ __launch_bounds__(BLK) void k_light_rest(JobDev J, uint32_t nlp, uint32_t *spill, size_t spill_stride, int spill_per,
                                                    const float *tab, uint32_t *overflow, const float4 *queue,
                                                    const uint32_t *qcount, unsigned long long *cnt)
@@ -1502,7 +1575,7 @@ __global__ __launch_bounds__(BLK) void k_light_rest(JobDev J, uint32_t nlp, uint
     for (size_t k = gtid; k < nq; k += (size_t)gridDim.x * BLK) {
         LightState P = light_get(queue + 4 * k);
         for (int depth = 1; depth < J.max_depth; depth++)
-            if (!light_vertex<STK, COUNT>(J, L, depth, P, S, tab, overflow, wc, n_ext, n_shadow)) break;
+            if (!light_vertex<STK, COUNT, PC>(J, L, depth, P, S, tab, overflow, wc, n_ext, n_shadow)) break;
     }
     light_counts<COUNT>(cnt, n_ext, n_shadow, wc);
 }

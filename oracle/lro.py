@@ -80,6 +80,12 @@ def lib():
             "lro_pt_image": (None, [P, P, I, I, U, I, U64, F, I, I, P]),
             "lro_pt_window": (None, [P, P, I, I, I, I, I, I, U, I, U64, F, I, I, P]),
             "lro_light_trace": (None, [P, P, I, I, U, U, I, U64, I, P]),
+            "lro_pc_key": (U64, [I, I, I, I, I]),
+            "lro_pc_deposit": (None, [P, F, P, P, P]),
+            "lro_pc_deposit_many": (None, [P, F, I, P, P, P]),
+            "lro_pc_fade": (None, [P, ctypes.c_double]),
+            "lro_photons_trace": (None, [P, P, F, U, U, U64, I]),
+            "lro_pc_resolve": (None, [P, P, F, ctypes.c_double, P, I, I, P]),
             "lro_env_prepare": (I, [P, P, I, I, F, F]),
             "lro_env_free": (None, [P]),
             "lro_env_eval": (None, [P, P, P]),
@@ -446,6 +452,61 @@ def light_trace(scene, W, H, nlp, s0, npass, seed=0, max_depth=8, tab=None, mask
     lib().lro_light_trace(ctypes.byref(sc), ptr(cam), W, H, nlp, s0, npass, seed, max_depth, ptr(acc))
     del keep
     return acc
+
+
+PC_SLOTS = 1 << 20
+
+
+def pc_table():
+    """an empty photon-cache table: (2^20, 4) uint64"""
+    return np.zeros((PC_SLOTS, 4), np.uint64)
+
+
+def photons_trace(scene, table, cell, npaths, s0=0, seed=0, max_depth=8, tab=None):
+    """lro_photons_trace: npaths light paths' power left in the table (in place)"""
+    sc, keep = scene_struct(scene, tab, None)
+    lib().lro_photons_trace(ctypes.byref(sc), ptr(table), cell, npaths, s0, seed, max_depth)
+    del keep
+
+
+def pc_deposit(table, cell, p, n, c):
+    """lro_pc_deposit: one photon of power c at p on a surface facing n (in place)"""
+    f = lambda x: np.ascontiguousarray(x, np.float32)
+    lib().lro_pc_deposit(ptr(table), cell, ptr(f(p)), ptr(f(n)), ptr(f(c)))
+
+
+def pc_deposit_many(table, cell, p, n, c):
+    """lro_pc_deposit_many: photons (m, 3) each of positions p, normals n, powers c, in order"""
+    f = lambda x: np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+    p, n, c = f(p), f(n), f(c)
+    lib().lro_pc_deposit_many(ptr(table), cell, p.shape[0], ptr(p), ptr(n), ptr(c))
+
+
+def pc_fade(table, k):
+    lib().lro_pc_fade(ptr(table), k)
+
+
+def pc_key(x, y, z, axis, level):
+    return lib().lro_pc_key(x, y, z, axis, level)
+
+
+def pc_resolve(scene, table, cell, paths, W, H, tab=None):
+    """lro_pc_resolve: (H, W, 4) float32, rgb and 1 where the pixel's centre ray meets a rough surface"""
+    sc, keep = scene_struct(scene, tab, None)
+    out = np.zeros((H, W, 4), np.float32)
+    cam = np.ascontiguousarray(scene["cam"], np.float32)
+    lib().lro_pc_resolve(ctypes.byref(sc), ptr(table), cell, paths, ptr(cam), W, H, ptr(out))
+    del keep
+    return out
+
+
+def pc_cells(table):
+    """the held cells of a table: {tag: (r, g, b, photons x 16)}"""
+    t = np.asarray(table)
+    held = (t[:, 3] >> np.uint64(32)) != 0
+    tags = (t[held, 3] >> np.uint64(32)).astype(np.uint64)
+    rows = t[held]
+    return {int(g): (int(r[0]), int(r[1]), int(r[2]), int(r[3] & np.uint64(0xffffffff))) for g, r in zip(tags, rows)}
 
 
 def guide(scene, W, H, tab, mask=None):

@@ -1330,6 +1330,170 @@ static long long to_fixed(real x)
     return llrint((double)(x * (real)268435456.0f));
 }
 
+/* ---- the photon cache (deposit / pc_claim, trace.wgsl:1359-1404; trace_decay.wgsl; the lookup of
+ * trace_resolve.wgsl:43-80), as the library lays it out: LRO_PC_SLOTS slots of 4 x 64 bits (rgb
+ * sums in 2^-8 units; the photons in 1/16 units in the low 32 bits of the fourth word, the cell's
+ * tag in the high 32). A cell's key (its integer corner, 19 bits an axis, the axis of the
+ * normal and its sign 3 bits, the size 2 bits) mixed by SplitMix64's finaliser (Steele, Lea and
+ * Flood 2014) gives its home slot (low bits) and its 32-bit tag (high bits, never 0); a cell
+ * takes the first slot from its home on that is free or its own, at most 8 on. Here one path
+ * after another: which of two cells meeting in a slot gets it can differ from the device's
+ * (whose order is the hardware's), never a cell's sums. */
+static void camera_ray(const float *cam, real s, real t, V *o, V *d);
+static uint64_t pc_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+uint64_t lro_pc_key(int x, int y, int z, int axis, int level)
+{
+    const uint64_t m = (1ULL << 19) - 1ULL;
+    return pc_mix(((uint64_t)(uint32_t)x & m) | (((uint64_t)(uint32_t)y & m) << 19) | (((uint64_t)(uint32_t)z & m) << 38) |
+                  ((uint64_t)axis << 57) | ((uint64_t)level << 60));
+}
+static int pc_axis(V n)
+{
+    const real ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+    const int a = ax >= ay && ax >= az ? 0 : (ay >= az ? 1 : 2);
+    const real v = a == 0 ? n.x : (a == 1 ? n.y : n.z);
+    return 2 * a + (v < 0 ? 1 : 0);
+}
+static uint32_t pc_tag(uint64_t h) { return (uint32_t)(h >> 32) | 1u; }
+static uint32_t pc_claim(uint64_t *t, uint64_t h)
+{
+    const uint32_t tag = pc_tag(h);
+    uint32_t idx = (uint32_t)h & (LRO_PC_SLOTS - 1u);
+    for (int probe = 0; probe < 8; probe++) {
+        const uint32_t k = (uint32_t)(t[4 * (size_t)idx + 3] >> 32);
+        if (k == 0u) t[4 * (size_t)idx + 3] |= (uint64_t)tag << 32;
+        if (k == 0u || k == tag) return idx;
+        idx = (idx + 1u) & (LRO_PC_SLOTS - 1u);
+    }
+    return 0xffffffffu;
+}
+/* a point's cell: along the normal's main axis offset by 1/pi of a cell (a surface at a round
+ * coordinate then lies inside a layer of cells, not on the boundary of two) */
+static int pc_coord(float x, float size, int normal_axis)
+{
+    return (int)floorf(x / size + (normal_axis ? 0.31830988f : 0.0f));
+}
+/* a photon of power c at p on a surface facing n, into its cell of each size (cell, x4, x16) */
+static void pc_deposit(uint64_t *t, float cell, V p, V n, V c)
+{
+    const int axis = pc_axis(n), ax = axis / 2;
+    float size = cell;
+    for (int level = 0; level < 3; level++, size *= 4.0f) {
+        const uint32_t slot = pc_claim(t, lro_pc_key(pc_coord((float)p.x, size, ax == 0), pc_coord((float)p.y, size, ax == 1),
+                                                     pc_coord((float)p.z, size, ax == 2), axis, level));
+        if (slot == 0xffffffffu) continue;
+        uint64_t *w = t + 4 * (size_t)slot;
+        const float cc[3] = {(float)c.x, (float)c.y, (float)c.z};
+        for (int ch = 0; ch < 3; ch++) {
+            const float v = cc[ch] * 256.0f;
+            if (!(v > 0) || v > 1.8e18f) continue;
+            w[ch] += (uint64_t)llrintf(v);
+        }
+        w[3] += 16u;
+    }
+}
+void lro_pc_deposit(uint64_t *table, float cell, const float p[3], const float n[3], const float c[3])
+{
+    pc_deposit(table, cell, v3(p[0], p[1], p[2]), v3(n[0], n[1], n[2]), v3(c[0], c[1], c[2]));
+}
+void lro_pc_deposit_many(uint64_t *table, float cell, int count, const float *p, const float *n, const float *c)
+{
+    for (int i = 0; i < count; i++) lro_pc_deposit(table, cell, p + 3 * (size_t)i, n + 3 * (size_t)i, c + 3 * (size_t)i);
+}
+/* the fade: sums and photons times k; a cell whose photons faded to nothing gives its slot back
+ * when the next slot is free (as it was before this fade) */
+void lro_pc_fade(uint64_t *t, double k)
+{
+    uint32_t first_tag = (uint32_t)(t[3] >> 32);
+    for (uint32_t s = 0; s < LRO_PC_SLOTS; s++) {
+        uint64_t *w = t + 4 * (size_t)s;
+        const uint64_t w3 = w[3];
+        if ((uint32_t)(w3 >> 32) == 0u) continue;
+        for (int ch = 0; ch < 3; ch++) w[ch] = (uint64_t)llrint((double)w[ch] * k);
+        const uint32_t count = (uint32_t)floor((double)(uint32_t)w3 * k);
+        const uint32_t next = s + 1u < LRO_PC_SLOTS ? (uint32_t)(t[4 * (size_t)(s + 1u) + 3] >> 32) : first_tag;
+        if (count == 0u && next == 0u) w[0] = w[1] = w[2] = w[3] = 0;
+        else w[3] = (w3 & 0xffffffff00000000ULL) | count;
+    }
+}
+static void pc_sum(const uint64_t *t, uint64_t h, double e[4])
+{
+    const uint32_t tag = pc_tag(h);
+    uint32_t idx = (uint32_t)h & (LRO_PC_SLOTS - 1u);
+    e[0] = e[1] = e[2] = e[3] = 0;
+    for (int probe = 0; probe < 8; probe++) {
+        const uint64_t *w = t + 4 * (size_t)idx;
+        const uint32_t k = (uint32_t)(w[3] >> 32);
+        if (k == tag) {
+            e[0] = (double)w[0];
+            e[1] = (double)w[1];
+            e[2] = (double)w[2];
+            e[3] = (double)(uint32_t)w[3] / 16.0;
+            return;
+        }
+        if (k == 0u) return;
+        idx = (idx + 1u) & (LRO_PC_SLOTS - 1u);
+    }
+}
+/* the lookup (trace_resolve.wgsl:43-80) at each pixel's centre ray: where it first meets a rough
+ * surface, albedo / pi x the power of the finest cells around the point (bilinearly, in the
+ * plane of its normal's main axis) holding at least 16 photons (else the largest), over their
+ * area and the light paths (x 256); out: 4 floats a pixel (rgb, 1; 0 where no rough surface) */
+void lro_pc_resolve(const LroScene *s, const uint64_t *t, float cell, double paths, const float cam[12], int W, int H, float *out)
+{
+    const double norm = paths > 0 ? 1.0 / (paths * 256.0) : 0.0;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int p = 0; p < W * H; p++) {
+        const int x = p % W, y = p / W;
+        V o, d;
+        camera_ray(cam, ((real)x + 0.5f) / W, ((real)y + 0.5f) / H, &o, &d);
+        float *r = out + 4 * (size_t)p;
+        r[0] = r[1] = r[2] = r[3] = 0;
+        Hit h;
+        if (!scene_intersect(s, o, d, s->eps, 1e30f, &h)) continue;
+        const LroMat *m = &s->mats[hit_material(s, &h)];
+        if (m->type == LRO_MAT_LIGHT) continue;
+        Bsdf b = bsdf_make(m, h.front, s->tab);
+        if (!rough_part(&b)) continue;
+        const int axis = pc_axis(h.n), ax = axis / 2, t1 = (ax + 1) % 3, t2 = (ax + 2) % 3;
+        const float pc3[3] = {(float)h.p.x, (float)h.p.y, (float)h.p.z};
+        float size = cell;
+        double e[4] = {0, 0, 0, 0};
+        for (int level = 0; level < 3; level++) {
+            float fp[3];
+            int c0[3];
+            for (int k = 0; k < 3; k++) {
+                fp[k] = pc3[k] / size;
+                c0[k] = pc_coord(pc3[k], size, k == ax);
+            }
+            const float uu = fp[t1] - 0.5f, vv = fp[t2] - 0.5f;
+            const float u0 = floorf(uu), v0 = floorf(vv), fu = uu - u0, fv = vv - v0;
+            e[0] = e[1] = e[2] = e[3] = 0;
+            for (int j = 0; j < 4; j++) {
+                int c[3] = {c0[0], c0[1], c0[2]};
+                c[t1] = (int)u0 + (j & 1);
+                c[t2] = (int)v0 + (j >> 1);
+                const double w = ((j & 1) ? fu : 1.0f - fu) * ((j >> 1) ? fv : 1.0f - fv);
+                double cs[4];
+                pc_sum(t, lro_pc_key(c[0], c[1], c[2], axis, level), cs);
+                for (int q = 0; q < 4; q++) e[q] += w * cs[q];
+            }
+            if (e[3] >= 16.0 || level == 2) break;
+            size *= 4.0f;
+        }
+        const double k = norm / ((double)size * size) / 3.14159265358979323846;
+        r[0] = (float)(m->albedo[0] * e[0] * k);
+        r[1] = (float)(m->albedo[1] * e[1] * k);
+        r[2] = (float)(m->albedo[2] * e[2] * k);
+        r[3] = 1;
+    }
+}
+
 /* light_trace_caustics (tracer.c:511-665) as the engine issues it: npass passes of nlp light
  * paths each (pass k, path i: the PCG32 stream lro_pt_seed(i, 2^31 + s0 + k, seed)). A path
  * starts at a uniform point of the lamp (two numbers) in a cosine-distributed direction about
@@ -1340,8 +1504,24 @@ static long long to_fixed(real x)
  * acc, H x W x 3); then on by the BSDF seen from the light's side (three numbers), through a
  * surface times eta^2, Russian roulette from depth 3 (one more number). One normal per hit: the
  * adjoint correction of smooth normals is 1. No media. */
+static void light_paths(const LroScene *s, const float cam[12], int W, int H, uint32_t nlp, uint32_t s0, int npass,
+                        uint64_t seed, int max_depth, long long *acc, uint64_t *pc, float pc_cell);
 void lro_light_trace(const LroScene *s, const float cam[12], int W, int H, uint32_t nlp, uint32_t s0, int npass,
                      uint64_t seed, int max_depth, long long *acc)
+{
+    light_paths(s, cam, W, H, nlp, s0, npass, seed, max_depth, acc, NULL, 0);
+}
+/* the cache mode (lrh_photons_trace): npaths light paths (sample index 2^31 + s0), each leaving
+ * its power at every rough vertex after the first in the table instead of joining the camera;
+ * one path after another */
+void lro_photons_trace(const LroScene *s, uint64_t *table, float cell, uint32_t npaths, uint32_t s0, uint64_t seed,
+                       int max_depth)
+{
+    static const float nocam[12] = {0, 0, 0, 0, 0, -1, 1, 0, 0, 0, 1, 0};
+    light_paths(s, nocam, 1, 1, npaths, s0, 1, seed, max_depth, NULL, table, cell);
+}
+static void light_paths(const LroScene *s, const float cam[12], int W, int H, uint32_t nlp, uint32_t s0, int npass,
+                        uint64_t seed, int max_depth, long long *acc, uint64_t *pc, float pc_cell)
 {
     if (s->light < 0 || nlp == 0 || npass <= 0) return;
     const LroPrim *lamp = &s->prims[s->light];
@@ -1352,7 +1532,7 @@ void lro_light_trace(const LroScene *s, const float cam[12], int W, int H, uint3
     const V fw = vsub(center, org);
     const real inv_n = 1.0f / (real)nlp;
     const long long total = (long long)nlp * npass;
-#pragma omp parallel for schedule(dynamic, 1024)
+#pragma omp parallel for schedule(dynamic, 1024) if (!pc)
     for (long long g = 0; g < total; g++) {
         const uint32_t pass = (uint32_t)(g / nlp), path = (uint32_t)(g % nlp);
         LroRng rng;
@@ -1369,8 +1549,10 @@ void lro_light_trace(const LroScene *s, const float cam[12], int W, int H, uint3
             if (m->type == LRO_MAT_LIGHT) break;
             Bsdf b = bsdf_make(m, h.front, s->tab);
             const V wi = vneg(d);
-            /* 1. the rough part joined to the camera (lamp -> rough at the first vertex: NEE's) */
-            if (depth > 0 && rough_part(&b)) {
+            /* 1. the rough part joined to the camera, or left in the photon cache (lamp -> rough at
+               the first vertex: NEE's) */
+            if (pc && depth > 0 && rough_part(&b)) pc_deposit(pc, pc_cell, h.p, h.n, beta);
+            if (!pc && depth > 0 && rough_part(&b)) {
                 const V to_cam = vsub(org, h.p);
                 const real dist2 = vdot(to_cam, to_cam), dist = sqrtf(dist2);
                 const V wo = vscale(to_cam, 1.0f / dist);

@@ -128,6 +128,16 @@ void lro_pt_image(const LroScene *s, const float cam[12], int W, int H, uint32_t
  * added to acc (H x W x 3 int64) */
 void lro_light_trace(const LroScene *s, const float cam[12], int W, int H, uint32_t nlp, uint32_t s0, int npass,
                      uint64_t seed, int max_depth, long long *acc);
+/* the photon cache (lrh_photons_*): a table of LRO_PC_SLOTS slots of 4 x 64 bits */
+#define LRO_PC_SLOTS (1u << 20)
+uint64_t lro_pc_key(int x, int y, int z, int axis, int level);
+void lro_pc_deposit(uint64_t *table, float cell, const float p[3], const float n[3], const float c[3]);
+void lro_pc_deposit_many(uint64_t *table, float cell, int count, const float *p, const float *n, const float *c);
+void lro_pc_fade(uint64_t *table, double k);
+void lro_photons_trace(const LroScene *s, uint64_t *table, float cell, uint32_t npaths, uint32_t s0, uint64_t seed,
+                       int max_depth);
+void lro_pc_resolve(const LroScene *s, const uint64_t *table, float cell, double paths, const float cam[12], int W, int H,
+                    float *out);
 /* the denoiser's guide of a W x H image (8 floats a pixel: n.xyz, z | albedo.rgb, kind) */
 void lro_guide(const LroScene *s, const float cam[12], int W, int H, float *out);
 /* lro_pt_image, and each pixel's moments of its samples' irradiance luminance (2 floats) */

@@ -166,3 +166,25 @@ def test_photons_refused(ctx):
         eng.submit(lr.make_job(ps, sc["cam"], 16, 16, 4, photons=1.0, target_error=0.01))
     eng.close()
     ps.close()
+
+
+def test_light_chunks_split_invariant(ctx):
+    # 128 x 128, one light path a pixel and pass: 300 passes are 4.9M light paths, two launch
+    # pairs (chunks of 2^22 paths: the second from path 2^22 on); 256 passes are exactly one chunk.
+    # Every number depends only on (pass, path) and (pixel, sample), and the sums are integers:
+    # passes 0..299 equal passes 0..255 plus passes 256..299 (to the float readout's rounding)
+    W = H = 128
+    sc = caustic_scene(W, H)
+    ps = scene_on_device(ctx, sc)
+    eng = lr.Engine(ctx, slots_log2=18, max_tree_depth=64, image_bytes=64 << 20)
+    say("caustics: 300 passes (two light chunks) vs 256 + 44")
+    a, b, c = batched(eng, [lr.make_job(ps, sc["cam"], W, H, 300, seed=8, photons=1.0),
+                            lr.make_job(ps, sc["cam"], W, H, 256, seed=8, photons=1.0),
+                            lr.make_job(ps, sc["cam"], W, H, 44, s0=256, seed=8, photons=1.0)])
+    st = eng.stats()
+    eng.close()
+    ps.close()
+    assert st.light_paths == W * H * (300 + 256 + 44) and st.light_launches == 3
+    bc = b[..., :3].astype(np.float64) + c[..., :3]
+    assert np.all(a[..., 3] == 300)
+    assert np.allclose(a[..., :3], bc, rtol=1e-6, atol=1e-6)
