@@ -662,11 +662,13 @@ def bench_caustics(a, torch, lr):
     pt2 = time.perf_counter()
     img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
-    pc.resolve(ps, sc["cam"], W, H, img)
+    pc.resolve(ps, sc["cam"], W, H, img)   # (its walk stack made on the first call)
     pt3 = time.perf_counter()
+    pc.resolve(ps, sc["cam"], W, H, img)
+    pt4 = time.perf_counter()
     out["photon_cache"] = {"light_paths": npc, "trace_ms": round((pt1 - pt0) * 1e3, 3),
                            "mpaths_s": round(npc / (pt1 - pt0) / 1e6, 1), "fade_ms": round((pt2 - pt1) * 1e3, 3),
-                           "resolve_ms": round((pt3 - pt2) * 1e3, 3),
+                           "resolve_ms": round((pt4 - pt3) * 1e3, 3),
                            "note": "host wall time around each call (the context's stream synchronised): a trace of "
                                    "W x H x spp light paths into a 2^20-cell table, a fade, the lookup at W x H"}
     pc.close()

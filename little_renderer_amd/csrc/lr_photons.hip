@@ -20,7 +20,9 @@ extern "C" void lrs_brdf_tables(float *out);
 namespace {
 
 constexpr int PC_BLK = 256, PC_STK = 16;
-constexpr uint32_t PC_CHUNK = 1u << 18;   // light paths a launch pair (the queue's entries, the spill's threads)
+constexpr uint32_t PC_CHUNK = 1u << 22;   // light paths a launch pair at most
+constexpr size_t PC_SPILL_BYTES = (size_t)256 << 20;   // the walks' spill area at most (a chunk's share)
+constexpr uint32_t PC_RESOLVE = 1u << 18;  // work-items of a lookup at most
 
 __global__ __launch_bounds__(PC_BLK) void k_pc_clear(unsigned long long *t, size_t n)
 {
@@ -142,7 +144,8 @@ struct LrhPhotons {
     float cell = 0.0f;
     double paths = 0.0;                    // the light paths it holds, faded with it
     float *tab = nullptr;                  // the BSDF tables
-    float4 *queue = nullptr;               // PC_CHUNK light paths that go on
+    float4 *queue = nullptr;               // the light paths of a chunk that go on (queue_cap of them)
+    uint32_t queue_cap = 0;
     uint32_t *qn = nullptr, *overflow = nullptr;
     unsigned long long *cnt = nullptr;     // the light kernels' ray counts (unused here)
     uint32_t *spill = nullptr;
@@ -169,6 +172,26 @@ int pc_spill(LrhPhotons *pc, size_t threads, int depth, int *per)
     return 0;
 }
 
+// a chunk's light paths: at most PC_CHUNK, and as many as the spill area holds for the scene's tree;
+// the queue made large enough for them
+int pc_chunk(LrhPhotons *pc, int depth, uint32_t *chunk)
+{
+    const int per = depth + 2 - PC_STK > 0 ? depth + 2 - PC_STK : 1;
+    const size_t by_spill = PC_SPILL_BYTES / ((size_t)per * 4) / PC_BLK * PC_BLK;
+    *chunk = (uint32_t)std::max<size_t>(PC_BLK, std::min<size_t>(PC_CHUNK, by_spill));
+    if (pc->queue_cap >= *chunk) return 0;
+    LR_HIP(hipStreamSynchronize(pc->ctx->stream));
+    if (pc->queue) (void)hipFree(pc->queue);
+    pc->queue = nullptr;
+    pc->queue_cap = 0;
+    if (hipMalloc(&pc->queue, (size_t)*chunk * 64) != hipSuccess) {
+        (void)hipGetLastError();
+        return lr_fail(LRH_ENOMEM, "photons: the light paths' queue (%zu bytes) could not be had", (size_t)*chunk * 64);
+    }
+    pc->queue_cap = *chunk;
+    return 0;
+}
+
 int pc_scene_check(const LrhPtScene *s)
 {
     if (!s) return lr_fail(LRH_EINVAL, "photons: no scene");
@@ -187,7 +210,7 @@ int lrh_photons_create(LrhContext *c, float cell, LrhPhotons **out)
     LrhPhotons *pc = new LrhPhotons;
     pc->ctx = c;
     pc->cell = cell;
-    const size_t tbytes = (size_t)LR_PC_SLOTS * 32, qbytes = (size_t)PC_CHUNK * 64;
+    const size_t tbytes = (size_t)LR_PC_SLOTS * 32, qbytes = 0;
     char *m = nullptr;
     if (hipMalloc(&m, tbytes + qbytes + 2112 * 4 + 256) != hipSuccess) {
         (void)hipGetLastError();
@@ -195,7 +218,6 @@ int lrh_photons_create(LrhContext *c, float cell, LrhPhotons **out)
         return lr_fail(LRH_ENOMEM, "lrh_photons_create: %zu bytes of device memory could not be had", tbytes + qbytes);
     }
     pc->table = reinterpret_cast<unsigned long long *>(m);
-    pc->queue = reinterpret_cast<float4 *>(m + tbytes);
     pc->tab = reinterpret_cast<float *>(m + tbytes + qbytes);
     pc->qn = reinterpret_cast<uint32_t *>(m + tbytes + qbytes + 2112 * 4);
     pc->overflow = pc->qn + 1;
@@ -224,6 +246,7 @@ void lrh_photons_destroy(LrhPhotons *pc)
     if (!pc) return;
     (void)hipStreamSynchronize(pc->ctx->stream);
     if (pc->spill) (void)hipFree(pc->spill);
+    if (pc->queue) (void)hipFree(pc->queue);
     (void)hipFree(pc->table);
     delete pc;
 }
@@ -244,8 +267,11 @@ int lrh_photons_trace(LrhPhotons *pc, const LrhPtScene *s, uint32_t npaths, uint
     if (rc) return rc;
     pc->paths += npaths;
     if (s->light < 0) return 0;   // no lamp: no light paths leave anything (they count all the same)
+    uint32_t chunk = 0;
+    rc = pc_chunk(pc, s->depth, &chunk);
+    if (rc) return rc;
     int per = 0;
-    rc = pc_spill(pc, PC_CHUNK, s->depth, &per);
+    rc = pc_spill(pc, chunk, s->depth, &per);
     if (rc) return rc;
     JobDev J;
     memset(&J, 0, sizeof(J));
@@ -258,8 +284,8 @@ int lrh_photons_trace(LrhPhotons *pc, const LrhPtScene *s, uint32_t npaths, uint
     J.pc = pc->table;
     J.pc_cell = pc->cell;
     hipStream_t st = pc->ctx->stream;
-    for (uint32_t g0 = 0; g0 < npaths; g0 += PC_CHUNK) {
-        const uint32_t n = std::min(PC_CHUNK, npaths - g0);
+    for (uint32_t g0 = 0; g0 < npaths; g0 += chunk) {
+        const uint32_t n = std::min(chunk, npaths - g0);
         const unsigned blocks = (n + PC_BLK - 1) / PC_BLK;
         hipLaunchKernelGGL(k_light_reset, dim3(1), dim3(1), 0, st, pc->qn);
         hipLaunchKernelGGL((k_light_first<PC_BLK, PC_STK, false, true>), dim3(blocks), dim3(PC_BLK), 0, st, J, npaths,
@@ -289,7 +315,7 @@ int lrh_photons_resolve(LrhPhotons *pc, const LrhPtScene *s, const float *cam, i
     int rc = pc_scene_check(s);
     if (rc) return rc;
     const size_t npix = (size_t)W * (size_t)H;
-    const size_t threads = std::min<size_t>((npix + PC_BLK - 1) / PC_BLK * PC_BLK, PC_CHUNK);
+    const size_t threads = std::min<size_t>((npix + PC_BLK - 1) / PC_BLK * PC_BLK, PC_RESOLVE);
     int per = 0;
     rc = pc_spill(pc, threads, s->depth, &per);
     if (rc) return rc;
