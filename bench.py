@@ -615,7 +615,8 @@ def bench_caustics(a, torch, lr):
                                               "iters_per_graph", "refill", "refill_shadow"))
         traffic, traffic_detail = kernel_traffic(["--workload", "caustics", "--tune", shape, "--no-cpu", "--steps", "1",
                                                   "--warmup", "0", "--no-traffic"], "k_light",
-                                                 ["k_light_first<256, 16, false>", "k_light_rest<256, 16, false>"],
+                                                 # (COUNT false; the names may go on with the photon form's flag)
+                                                 ["k_light_first<256, 16, false", "k_light_rest<256, 16, false"],
                                                  "the light kernels")
     lpaths = d("light_paths") // steps
     chunks = max(1, -(-lpaths // (1 << 22)))

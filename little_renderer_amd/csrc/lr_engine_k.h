@@ -1352,14 +1352,17 @@ __device__ __forceinline__ int pc_axis(V3 n)
     return 2 * a + (v < 0.0f ? 1 : 0);
 }
 __device__ __forceinline__ uint32_t pc_tag(uint64_t h) { return (uint32_t)(h >> 32) | 1u; }
-// the cell's slot (claimed if need be), or LR_NONE when the 8 slots from its home are others'
+// the cell's slot (claimed if need be), or LR_NONE when the 8 slots from its home are others'. A
+// slot's tag only goes from 0 to a cell's during a trace, so a plain read that shows a tag is the
+// answer; only a slot that reads free takes the compare-and-swap
 __device__ __forceinline__ uint32_t pc_claim(unsigned long long *pc, uint64_t h)
 {
     const uint32_t tag = pc_tag(h);
     uint32_t idx = (uint32_t)h & (LR_PC_SLOTS - 1u);
     for (int probe = 0; probe < 8; probe++) {
         uint32_t *w = reinterpret_cast<uint32_t *>(pc + 4 * (size_t)idx) + 7;
-        const uint32_t old = atomicCAS(w, 0u, tag);
+        uint32_t old = __atomic_load_n(w, __ATOMIC_RELAXED);
+        if (old == 0u) old = atomicCAS(w, 0u, tag);
         if (old == 0u || old == tag) return idx;
         idx = (idx + 1u) & (LR_PC_SLOTS - 1u);
     }
@@ -1373,8 +1376,36 @@ __device__ __forceinline__ int pc_coord(float x, float size, bool normal_axis)
 {
     return (int)floorf(x / size + (normal_axis ? PC_OFFSET : 0.0f));
 }
-// a photon of power c at p on a surface facing n: into its cell of each size
-__device__ __forceinline__ void pc_deposit(const JobDev &J, V3 p, V3 n, V3 c)
+// a work-group's sums before they go to the table: PC_LDS cells keyed by their slot (open
+// addressing, 8 probes; a photon whose cell finds no room goes to the table at once). A focused
+// caustic sends many photons to a few cells, whose adds would otherwise queue at their slots.
+constexpr int PC_LDS = 256;
+struct PcLds {
+    unsigned long long sum[PC_LDS][3];
+    uint32_t cnt[PC_LDS];
+    uint32_t slot[PC_LDS];
+};
+__device__ __forceinline__ void pc_lds_init(PcLds &L)
+{
+    for (int i = threadIdx.x; i < PC_LDS; i += blockDim.x) {
+        L.sum[i][0] = L.sum[i][1] = L.sum[i][2] = 0ull;
+        L.cnt[i] = 0u;
+        L.slot[i] = LR_NONE;
+    }
+}
+__device__ __forceinline__ void pc_lds_flush(const JobDev &J, const PcLds &L)
+{
+    for (int i = threadIdx.x; i < PC_LDS; i += blockDim.x) {
+        if (L.slot[i] == LR_NONE) continue;
+        unsigned long long *w = J.pc + 4 * (size_t)L.slot[i];
+        for (int ch = 0; ch < 3; ch++)
+            if (L.sum[i][ch]) atomicAdd(&w[ch], L.sum[i][ch]);
+        atomicAdd(reinterpret_cast<uint32_t *>(w + 3), L.cnt[i]);
+    }
+}
+// a photon of power c at p on a surface facing n: into its cell of each size (through the
+// work-group's sums L when given)
+__device__ __forceinline__ void pc_deposit(const JobDev &J, V3 p, V3 n, V3 c, PcLds *L)
 {
     const int axis = pc_axis(n), ax = axis / 2;
     float size = J.pc_cell;
@@ -1382,14 +1413,33 @@ __device__ __forceinline__ void pc_deposit(const JobDev &J, V3 p, V3 n, V3 c)
         const uint32_t slot = pc_claim(J.pc, pc_key(pc_coord(p.x, size, ax == 0), pc_coord(p.y, size, ax == 1),
                                                     pc_coord(p.z, size, ax == 2), axis, level));
         if (slot == LR_NONE) continue;
-        unsigned long long *w = J.pc + 4 * (size_t)slot;
         const float cc[3] = {c.x, c.y, c.z};
+        unsigned long long v[3];
         for (int ch = 0; ch < 3; ch++) {
-            const float v = cc[ch] * LR_PC_SCALE;
-            if (!(v > 0.0f) || v > 1.8e18f) continue;
-            atomicAdd(&w[ch], (unsigned long long)__float2ull_rn(v));
+            const float x = cc[ch] * LR_PC_SCALE;
+            v[ch] = (x > 0.0f && x <= 1.8e18f) ? (unsigned long long)__float2ull_rn(x) : 0ull;
         }
-        atomicAdd(reinterpret_cast<uint32_t *>(w + 3), LR_PC_UNIT);
+        int at = -1;
+        if (L) {
+            uint32_t i = slot & (PC_LDS - 1);
+            for (int probe = 0; probe < 8; probe++, i = (i + 1u) & (PC_LDS - 1)) {
+                const uint32_t old = atomicCAS(&L->slot[i], LR_NONE, slot);
+                if (old == LR_NONE || old == slot) {
+                    at = (int)i;
+                    break;
+                }
+            }
+        }
+        if (at >= 0) {
+            for (int ch = 0; ch < 3; ch++)
+                if (v[ch]) atomicAdd(&L->sum[at][ch], v[ch]);
+            atomicAdd(&L->cnt[at], LR_PC_UNIT);
+        } else {
+            unsigned long long *w = J.pc + 4 * (size_t)slot;
+            for (int ch = 0; ch < 3; ch++)
+                if (v[ch]) atomicAdd(&w[ch], v[ch]);
+            atomicAdd(reinterpret_cast<uint32_t *>(w + 3), LR_PC_UNIT);
+        }
     }
 }
 
@@ -1399,7 +1449,7 @@ __device__ __forceinline__ void pc_deposit(const JobDev &J, V3 p, V3 n, V3 c)
 template <int STK, bool COUNT, bool PC = false>
 __device__ __forceinline__ bool light_vertex(const JobDev &J, const LightCtx &L, int depth, LightState &P, const WalkStack &S,
                                              const float *tab, uint32_t *overflow, WalkCnt &wc, uint32_t &n_ext,
-                                             uint32_t &n_shadow)
+                                             uint32_t &n_shadow, PcLds *pl)
 {
     const uint64_t inc = ((uint64_t)P.path << 1u) | 1u;
     const WalkHit h = walk4<false, COUNT, STK>(J.nodes, J.aux, J.ltris, J.prims, J.nprims, P.o, P.d, J.eps, 1e30f, S,
@@ -1419,7 +1469,7 @@ __device__ __forceinline__ bool light_vertex(const JobDev &J, const LightCtx &L,
     const V3 wi = vneg(P.d);
     // 1. the rough part joined to the camera, or left in the photon cache (not at the first vertex)
     if (PC && depth > 0 && (!b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f))
-        pc_deposit(J, hp, nrm, P.beta);
+        pc_deposit(J, hp, nrm, P.beta, pl);
     if (!PC && depth > 0 && (!b.smooth || (1.0f - b.metallic) * (1.0f - b.transmission) > 0.0f)) {
         const V3 to_cam = vsub(L.org, hp);
         const float dist2 = vdot(to_cam, to_cam), dist = sqrtf(dist2);
@@ -1549,7 +1599,8 @@ __global__ __launch_bounds__(BLK) void k_light_first(JobDev J, uint32_t nlp, uin
         P.beta = vscale(L.le, L.lamp.v.w * PI_F);
         P.thr = v3(1.0f, 1.0f, 1.0f);
         if (J.max_depth > 0)
-            on = light_vertex<STK, COUNT, PC>(J, L, 0, P, S, tab, overflow, wc, n_ext, n_shadow) && J.max_depth > 1;
+            on = light_vertex<STK, COUNT, PC>(J, L, 0, P, S, tab, overflow, wc, n_ext, n_shadow, nullptr) &&
+                 J.max_depth > 1;
     }
     const uint64_t b = __ballot(on);
     uint32_t at = 0;
@@ -1559,7 +1610,8 @@ __global__ __launch_bounds__(BLK) void k_light_first(JobDev J, uint32_t nlp, uin
     light_counts<COUNT>(cnt, n_ext, n_shadow, wc);
 }
 
-// 2. the queue's light paths from their second vertex on, one a lane
+// 2. the queue's light paths from their second vertex on, one a lane (PC: their deposits summed
+// in the work-group first; work-groups past the queue's end leave at once)
 template <int BLK, int STK, bool COUNT, bool PC = false>
 __global__ __launch_bounds__(BLK) void k_light_rest(JobDev J, uint32_t nlp, uint32_t *spill, size_t spill_stride, int spill_per,
                                                    const float *tab, uint32_t *overflow, const float4 *queue,
@@ -1570,12 +1622,25 @@ __global__ __launch_bounds__(BLK) void k_light_rest(JobDev J, uint32_t nlp, uint
     const WalkStack S{&stk[0][threadIdx.x], BLK, spill + gtid, spill_stride, STK + spill_per};
     const LightCtx L = light_ctx(J, nlp);
     const uint32_t nq = *qcount;
+    if ((size_t)blockIdx.x * BLK >= nq) return;   // the whole work-group: none of its lanes has a path
     uint32_t n_ext = 0, n_shadow = 0;
     WalkCnt wc;
+    PcLds *pl = nullptr;
+    if constexpr (PC) {
+        __shared__ PcLds sums;
+        pl = &sums;
+        pc_lds_init(sums);
+        __syncthreads();
+    }
     for (size_t k = gtid; k < nq; k += (size_t)gridDim.x * BLK) {
         LightState P = light_get(queue + 4 * k);
         for (int depth = 1; depth < J.max_depth; depth++)
-            if (!light_vertex<STK, COUNT, PC>(J, L, depth, P, S, tab, overflow, wc, n_ext, n_shadow)) break;
+            if (!light_vertex<STK, COUNT, PC>(J, L, depth, P, S, tab, overflow, wc, n_ext, n_shadow, pl))
+                break;
+    }
+    if constexpr (PC) {
+        __syncthreads();
+        pc_lds_flush(J, *pl);
     }
     light_counts<COUNT>(cnt, n_ext, n_shadow, wc);
 }
