@@ -322,7 +322,7 @@ __device__ WalkHit walk4_tl(const JobDev &J, V3 o, V3 d, float tmin, float tmax,
         uint32_t c[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            if (!(t[k] >= 0.0f)) continue;
+            if (!(t[k] < 3e38f)) continue;
             if (!(ref[k] & LR_LEAF)) {
                 c[k] = (blas ? LR_WALK_BLAS : 0u) | ref[k];
                 s[k] = t[k];
@@ -560,15 +560,14 @@ __device__ __forceinline__ bool walk_round(WalkState &w, const WalkStack &S, uin
     float t[4];
     uint32_t ref[4];
     wide_test(w.nodes, w.cur, v3(w.o.x * w.inv.x, w.o.y * w.inv.y, w.o.z * w.inv.z), w.inv, w.tmin, w.best.t, t, ref);
-    float s0 = t[0] >= 0.0f ? t[0] : 3e38f, s1 = t[1] >= 0.0f ? t[1] : 3e38f;
-    float s2 = t[2] >= 0.0f ? t[2] : 3e38f, s3 = t[3] >= 0.0f ? t[3] : 3e38f;
+    const int m = (t[0] < 3e38f) + (t[1] < 3e38f) + (t[2] < 3e38f) + (t[3] < 3e38f);
+    float s0 = t[0], s1 = t[1], s2 = t[2], s3 = t[3];
     uint32_t c0 = ref[0], c1 = ref[1], c2 = ref[2], c3 = ref[3];
     cswap_(s0, c0, s1, c1);
     cswap_(s2, c2, s3, c3);
     cswap_(s0, c0, s2, c2);
     cswap_(s1, c1, s3, c3);
     cswap_(s1, c1, s2, c2);
-    const int m = (s0 < 3e38f) + (s1 < 3e38f) + (s2 < 3e38f) + (s3 < 3e38f);
     if (m == 0) {
         if (w.sp == 0) return true;
         w.sp--;

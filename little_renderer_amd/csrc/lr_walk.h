@@ -336,9 +336,13 @@ __device__ __forceinline__ void leaf_hits_(const uint32_t *__restrict__ aux, con
 // outwards.
 
 
-// the entry distances (-1: missed, or no child) and references of a wide node's children: the
+// the entry distances (3e38: missed, or no child) and references of a wide node's children: the
 // slab test with each plane's t as one fused multiply-add of its step, fma(q, s * inv, (corner -
-// o) * inv) (a culling test on boxes that hold the children's: its last bit decides no hit)
+// o) * inv) (a culling test on boxes that hold the children's: its last bit decides no hit).
+// The step s * inv has the sign of inv (s > 0, or 0), and a fused multiply-add is monotone in q,
+// so along an axis whose inv is negative the high plane is the near one: each axis's near and far
+// planes are picked once a node, not ordered by a min and a max a child. A hit's entry is at most
+// tmax (at most 1e30), so 3e38 sorts every miss last.
 __device__ __forceinline__ void wide_test(const float4 *__restrict__ w, uint32_t node, V3 oi, V3 inv, float tmin, float tmax,
                                           float t[4], uint32_t ref[4])
 {
@@ -352,6 +356,9 @@ __device__ __forceinline__ void wide_test(const float4 *__restrict__ w, uint32_t
     const float bz = __builtin_fmaf(n0.z, inv.z, -oi.z);
     const uint32_t lx = __float_as_uint(n2.x), ly = __float_as_uint(n2.y), lz = __float_as_uint(n2.z);
     const uint32_t hx = __float_as_uint(n2.w), hy = __float_as_uint(n3.x), hz = __float_as_uint(n3.y);
+    const bool fx = inv.x < 0.0f, fy = inv.y < 0.0f, fz = inv.z < 0.0f;   // (flipped: high plane near)
+    const uint32_t nx = fx ? hx : lx, ny = fy ? hy : ly, nz = fz ? hz : lz;
+    const uint32_t gx = fx ? lx : hx, gy = fy ? ly : hy, gz = fz ? lz : hz;
     ref[0] = __float_as_uint(n1.x);
     ref[1] = __float_as_uint(n1.y);
     ref[2] = __float_as_uint(n1.z);
@@ -359,15 +366,15 @@ __device__ __forceinline__ void wide_test(const float4 *__restrict__ w, uint32_t
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int sh = 8 * k;
-        const float t0x = __builtin_fmaf((float)((lx >> sh) & 0xffu), ax, bx);
-        const float t0y = __builtin_fmaf((float)((ly >> sh) & 0xffu), ay, by);
-        const float t0z = __builtin_fmaf((float)((lz >> sh) & 0xffu), az, bz);
-        const float t1x = __builtin_fmaf((float)((hx >> sh) & 0xffu), ax, bx);
-        const float t1y = __builtin_fmaf((float)((hy >> sh) & 0xffu), ay, by);
-        const float t1z = __builtin_fmaf((float)((hz >> sh) & 0xffu), az, bz);
-        const float enter = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-        const float leave = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
-        t[k] = ref[k] == LR_NONE || leave < enter ? -1.0f : enter;
+        const float t0x = __builtin_fmaf((float)((nx >> sh) & 0xffu), ax, bx);
+        const float t0y = __builtin_fmaf((float)((ny >> sh) & 0xffu), ay, by);
+        const float t0z = __builtin_fmaf((float)((nz >> sh) & 0xffu), az, bz);
+        const float t1x = __builtin_fmaf((float)((gx >> sh) & 0xffu), ax, bx);
+        const float t1y = __builtin_fmaf((float)((gy >> sh) & 0xffu), ay, by);
+        const float t1z = __builtin_fmaf((float)((gz >> sh) & 0xffu), az, bz);
+        const float enter = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tmin));
+        const float leave = fminf(fminf(t1x, t1y), fminf(t1z, tmax));
+        t[k] = ref[k] == LR_NONE || leave < enter ? 3e38f : enter;
         // one child after the other: interleaving the four holds all 24 planes at once
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -419,15 +426,14 @@ __device__ __forceinline__ WalkHit walk4(const float4 *__restrict__ w4, const ui
         // o * inv, the slab test's other term, made again each node step (fewer registers held)
         wide_test(w4, cur, v3(o.x * inv.x, o.y * inv.y, o.z * inv.z), inv, tmin, best.t, t, ref);
         // the children hit, nearest first: a sorting network over 4
-        float s0 = t[0] >= 0.0f ? t[0] : 3e38f, s1 = t[1] >= 0.0f ? t[1] : 3e38f;
-        float s2 = t[2] >= 0.0f ? t[2] : 3e38f, s3 = t[3] >= 0.0f ? t[3] : 3e38f;
+        const int m = (t[0] < 3e38f) + (t[1] < 3e38f) + (t[2] < 3e38f) + (t[3] < 3e38f);
+        float s0 = t[0], s1 = t[1], s2 = t[2], s3 = t[3];
         uint32_t c0 = ref[0], c1 = ref[1], c2 = ref[2], c3 = ref[3];
         cswap_(s0, c0, s1, c1);
         cswap_(s2, c2, s3, c3);
         cswap_(s0, c0, s2, c2);
         cswap_(s1, c1, s3, c3);
         cswap_(s1, c1, s2, c2);
-        const int m = (s0 < 3e38f) + (s1 < 3e38f) + (s2 < 3e38f) + (s3 < 3e38f);
         if (m == 0) {
             if (sp == 0) break;
             sp--;
